@@ -1,0 +1,153 @@
+/* swarm_hip.h - C ABI of the MI355X (gfx950) per-step crowd pipeline.
+ *
+ * The drop-in boundary for the reference's GL dispatch sequence: the reference's C++ simulator
+ * class CrowdSim (include/swarm/crowd_sim.hpp:53-194) drives calc_grid.comp, bitonic_sort*.comp,
+ * reorder.comp and physics.comp through OpenGL; here the same calls drive hand-written HIP kernels.
+ * Plain pointers and sizes only. Every host array is caller-owned and copied on the call.
+ * Return codes: SWARM_OK (0) or a negative SWARM_E*; swarm_last_error() holds the message.
+ * One thread per context; independent contexts may run on different threads. Readbacks
+ * synchronise the context's own stream only.
+ *
+ * Units: the reference's model units (1 length unit = 31.25 m, 1 time unit = 17.490671 s).
+ * Agent state is SoA FP32: x, y, vx, vy, speed (velocity.w), w (1 active, 0 evacuated).
+ */
+#ifndef SWARM_HIP_H
+#define SWARM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWARM_OK 0
+#define SWARM_EINVAL (-1)   /* bad argument (agent count, iterations, layout, null pointer) */
+#define SWARM_ESIZE (-2)    /* size mismatch */
+#define SWARM_ENOMEM (-3)   /* device allocation failed */
+#define SWARM_ESCENE (-4)   /* invalid scene */
+#define SWARM_EHIP (-5)     /* HIP runtime error or no device */
+
+typedef struct swarm_scene swarm_scene;
+typedef struct swarm_ctx swarm_ctx;
+typedef struct swarm_batch swarm_batch;
+
+/* ---------------- scene preparation (input tables; not part of the step) ---------------- */
+
+/* Benchmark floors */
+#define SWARM_LAYOUT_ROOM 0          /* 40 x 30 m room, two 2 m exits */
+#define SWARM_LAYOUT_BOWL 1          /* 200 x 150 m seated bowl, 240 rows, 16 gangways, 8 exits */
+#define SWARM_LAYOUT_CORRIDOR 2      /* 60 x 20 m corridor, one 1.5 m exit */
+#define SWARM_LAYOUT_BOWL_EIGHTH 3   /* the bowl cropped to 1/8 */
+#define SWARM_LAYOUT_PILLAR 4        /* 30 x 30 m room with a 4 x 4 m pillar */
+#define SWARM_LAYOUT_BOWL_QUARTER 5  /* the bowl cropped to 1/4 */
+
+/* A rectangle world (replaces RectGeometry, include/swarm/nav_field.hpp:135-160, plus the SDF bake
+ * of src/sdf.cpp and the stair bands). Rects are (lox, loy, hix, hiy), model units. */
+typedef struct {
+    float xmin, ymin, xmax, ymax;
+    float hash_cell, sdf_spacing, sdf_clamp;
+    int nav_res_x, nav_res_y;
+    float nav_origin_x, nav_origin_y, nav_cell, nav_inv_cell;
+    float nav_body;
+    double metres_per_unit;
+    int n_walls, n_exits, n_stairs;
+    const float* walls;          /* n_walls x 4 */
+    const float* exits;          /* n_exits x 4 */
+    const float* exit_width_m;   /* n_exits (0: the rect's longer side) */
+    const float* stairs;         /* n_stairs x 4 */
+    const float* stair_factor;   /* n_stairs */
+} swarm_scene_desc;
+
+/* Sizes and scalars of a built scene */
+typedef struct {
+    float xmin, ymin, xmax, ymax, hash_span_x, hash_span_y;
+    int hash_nx, hash_ny, perc_ring, proj_ring;
+    int sdf_nx, sdf_ny;
+    float sdf_ox, sdf_oy, sdf_h, sdf_inv_h;
+    int sdf_min_win;
+    int nav_res_x, nav_res_y;
+    float nav_origin_x, nav_origin_y, nav_cell, nav_inv_cell;
+    int K, E;
+    float seconds_per_unit, dist_quantum;
+    double build_seconds;
+} swarm_scene_info;
+
+/* nav::build + buildChoiceTable (src/nav_field.cpp:344-366, src/exit_choice.cpp:143-317) and the SDF bake
+ * (src/sdf.cpp) for a rectangle world. max_radius / max_v0 size the neighbour rings. threads <= 0: all. */
+int swarm_scene_build(const swarm_scene_desc* desc, float max_radius, float max_v0, int threads, swarm_scene** out);
+/* The same for a benchmark floor (SWARM_LAYOUT_*) */
+int swarm_scene_build_layout(int layout, float max_radius, float max_v0, int threads, swarm_scene** out);
+int swarm_scene_info_get(const swarm_scene* s, swarm_scene_info* info);
+/* Copies of the tables; any pointer may be NULL. Sizes: sdf, sdf_min sdf_nx*sdf_ny; nav_info, stair
+ * nav_res_x*nav_res_y; nav_slot, nav_dist cells*K; inv_cap_s E; exit_rects 4E. */
+int swarm_scene_read(const swarm_scene* s, float* sdf, float* sdf_min, uint32_t* nav_info, uint32_t* nav_slot,
+                     uint16_t* nav_dist, float* inv_cap_s, float* stair, float* exit_rects);
+void swarm_scene_destroy(swarm_scene* s);
+
+/* Seeded agents on a benchmark floor (replaces CrowdSim's spawners, src/crowd_sim.cpp:485-808): 'active' of its
+ * slots, padded to 'total' with parked agents; r ~ U[0.20, 0.25] m, v0 ~ N(mean, sd) m/s clipped to [0.5, 2.0]. */
+int swarm_place_agents(int layout, size_t active, size_t total, uint64_t seed, double speed_mean, double speed_sd,
+                       float* x, float* y, float* r, float* v0, float* w);
+/* Slots of a benchmark floor (capacity query; x, y may be NULL) */
+int swarm_layout_slots(int layout, float* x, float* y, size_t cap, size_t* count);
+
+/* tau (model time) and exp(-DT / tau) for a relaxation time in seconds, folded from decimals (sim_params.hpp:128-129,510) */
+int swarm_model_params(float tau_s, float* tau, float* relax_step);
+
+/* ---------------- the plain step (CrowdSim) ---------------- */
+
+typedef struct {
+    float tau_s;           /* relaxation time, seconds (0.5) */
+    float obstacle_gain;   /* wall-avoidance gain (20) */
+    int proj_iters;        /* Jacobi iterations (32) */
+    int use_graph;         /* capture steps in a hipGraph (1) */
+} swarm_config;
+
+/* CrowdSim(shaderDir, numBoids, venue, options) (crowd_sim.hpp:62-73); n: any positive agent count */
+int swarm_create(const swarm_scene* scene, int n, const swarm_config* cfg, swarm_ctx** out);
+/* CrowdSim::upload / CpuCrowdSim::setState (crowd_sim.hpp:90, cpu_crowd_sim.hpp:91): both halves; resets the exit
+ * choice, the evacuation record and the step counter */
+int swarm_upload(swarm_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w,
+                 const float* r, const float* v0);
+/* CpuCrowdSim::restoreAgentState (cpu_crowd_sim.hpp:94): the agent state only */
+int swarm_restore_state(swarm_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w);
+/* CpuCrowdSim::setDriveTable / physics.comp TEST_DRIVE_TABLE (cpu_crowd_sim.hpp:121): forced e_goal; NULL clears */
+int swarm_set_drive(swarm_ctx* c, const float* dx, const float* dy);
+/* CrowdSim::step (crowd_sim.hpp:91, crowd_sim.cpp:316-412), nsteps times; asynchronous */
+int swarm_step(swarm_ctx* c, int nsteps);
+/* CrowdSim::stepExitChoice / CpuCrowdSim::stepExitChoice (cpu_crowd_sim.hpp:104): the exit choice pass alone */
+int swarm_step_exit_choice(swarm_ctx* c);
+/* The choice state, as the reference's choice harness sets it (tools/choice_check.hpp:88-100: setAssign,
+ * setCounters, setAhead): step counter, assigned exits (n), queue counters (128), queue-ahead histograms
+ * (2 x 64 x 1024); NULL arrays are left as they are */
+int swarm_set_choice_state(swarm_ctx* c, uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead);
+int swarm_sync(swarm_ctx* c);
+/* CrowdSim::setProjectionIterations (crowd_sim.hpp:125) */
+int swarm_set_proj_iters(swarm_ctx* c, int iterations);
+/* currentStateBuffer readback (crowd_sim.hpp:109; CpuCrowdSim::state) */
+int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w);
+/* gridBuffer readback (crowd_sim.hpp:115; CpuCrowdSim::grid): sorted (cell, agent) pairs of the last step's input */
+int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id);
+/* cellBuffer readback (crowd_sim.hpp:117; CpuCrowdSim::cells): start and count per cell */
+int swarm_read_cells(swarm_ctx* c, uint32_t* start, uint32_t* count);
+/* assignBuffer readback (crowd_sim.hpp:121; CpuCrowdSim::assignedExits) */
+int swarm_read_assign(swarm_ctx* c, uint32_t* assign);
+/* readChoiceCounters (crowd_sim.hpp:133): queue[128], switches, forced, evaluations, eligible */
+int swarm_read_choice_counters(swarm_ctx* c, uint32_t* out132);
+/* Evacuation record (metrics_deposit.comp:96-104's events): per agent its step and exit (-1: none; E: domain), per exit counts (E + 1) */
+int swarm_read_evac(swarm_ctx* c, int32_t* step, int32_t* exit_id, uint32_t* per_exit);
+/* step counter, active agents at the last step's input, running sum of active agents over steps */
+int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
+/* the context's HIP stream (hipStream_t), for events around swarm_step */
+int swarm_get_stream(swarm_ctx* c, void** stream);
+int swarm_num_cells(swarm_ctx* c);
+void swarm_destroy(swarm_ctx* c);
+
+/* The message of the last failed call on this thread */
+const char* swarm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

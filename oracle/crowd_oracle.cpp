@@ -1,0 +1,909 @@
+// Oracle: CPU restatement of the reference's per-step pipeline (CpuCrowdSim::step,
+// src/cpu_crowd_sim.cpp:967-1088) on flat 2D floors.
+//
+// TEST INFRASTRUCTURE ONLY: the checker of the GPU path (tests, smoke check, CPU baseline).
+//
+// Operation order follows the reference expression by expression (float, no FMA contraction:
+// build with -ffp-contract=off). The 2D restriction of the reference's 3D model: every agent stands on
+// one flat floor, so the floor rest, gravity, pitch fence, fence CCD, sweep height test and step-up
+// limit are identities, and wall normals have no vertical component. Generalisations: a rectangular
+// domain and hash grid, per-agent radius r_i and free speed v0_i, a 2D wall-distance grid sampled
+// bilinearly with exact weights, exits as rectangles, a stair speed-factor field, and a configurable
+// projection ring (the reference searches 3x3: its cell is large enough, sim_params.hpp:533).
+#include "crowd_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+// ---- model constants (include/swarm/sim_params.hpp), folded values bit-identical ----
+constexpr float DT = 0.01f;
+constexpr float K_SEP = 100.0f;
+constexpr float R_SEP = 0.016f, BODY_DIAM = 0.0128f;
+constexpr float SEP_MARGIN = R_SEP - BODY_DIAM;   // exact (Sterbenz)
+constexpr float HEADWAY_RATE = 0x1.d26afcp+5f;     // 1.0 / T_HEADWAY (sim_params.hpp:511)
+constexpr float R_PERC2 = 0x1.096bbap-9f;          // R_PERC * R_PERC (sim_params.hpp:503)
+constexpr float PROJ_RELAX = 1.0f;
+constexpr float PROJ_MAX_SHIFT_RADII = 0.05f;
+constexpr float W_OBS = 10.0f;
+constexpr float GOAL_DEADZONE = 1e-4f;
+constexpr float OBSTACLE_RANGE = 0.025f;
+constexpr float OBSTACLE_MIN_NORMAL = 0.01f;
+constexpr float OBSTACLE_GRAD_H = 0.0015f;
+constexpr float OBSTACLE_GRAD_SCALE = 0x1.4d5556p+7f;   // 0.25 / OBSTACLE_GRAD_H (sim_params.hpp:505)
+constexpr float SAFE_NORMALIZE_EPS = 1e-8f;
+constexpr float SEP_STACKED_XY2 = 1e-12f;
+constexpr float FEELER_REACH_RADII = 1.25f;
+constexpr int WALL_ROOT_STEPS = 3, WALL_RECAST_STEPS = 2, WALL_PASSES = 3, STATIC_SWEEP_BISECTIONS = 8;
+constexpr float WALL_RECAST_TOL_RADII = 0.02f;
+constexpr float CENTRE_GRAD_MIN = 1e-3f;
+constexpr float PLANE_PROBE_RADII = 2.5f;
+constexpr float PARK_COORD = 1000.0f;
+constexpr uint32_t INACTIVE_CELL = 0xFFFFFFFFu;
+constexpr float HASH_TO_RADIANS = 0x1.921fb6p-30f;
+constexpr int CHOICE_STEPS = 17;
+constexpr float CHOICE_REL_MARGIN = 0.10f, CHOICE_ABS_MARGIN_S = 5.0f, CHOICE_SWITCH_SHARE = 0.10f;
+constexpr uint32_t CHOICE_MIN_SWITCHES = 1u;
+constexpr int CHOICE_GAIN_BINS = 256;
+constexpr int CHOICE_EXIT_SHIFT = 25;
+constexpr uint32_t CHOICE_ANCHOR_MASK = (1u << CHOICE_EXIT_SHIFT) - 1u;
+constexpr uint32_t CHOICE_EMPTY = 63u, CHOICE_DIST_EMPTY = 0xFFFFu, CHOICE_NONE = 0xFFFFFFFFu;
+constexpr int CHOICE_MAX_EXITS = 63, CHOICE_QUEUE_STRIDE = 64, CHOICE_AHEAD_BINS = 1024, CHOICE_AHEAD_SHIFT = 6;
+constexpr uint32_t NAV_FIT = 0x80000000u, NAV_HAS = 0x40000000u;
+constexpr float PLANE_WALL = 1.0f, PLANE_GROUNDED = 2.0f;
+
+const float FEELER_DIRS[16][2] = {
+    { 1.0f, 0.0f }, { 0.92387953f, 0.38268343f }, { 0.70710678f, 0.70710678f }, { 0.38268343f, 0.92387953f },
+    { 0.0f, 1.0f }, { -0.38268343f, 0.92387953f }, { -0.70710678f, 0.70710678f }, { -0.92387953f, 0.38268343f },
+    { -1.0f, 0.0f }, { -0.92387953f, -0.38268343f }, { -0.70710678f, -0.70710678f }, { -0.38268343f, -0.92387953f },
+    { 0.0f, -1.0f }, { 0.38268343f, -0.92387953f }, { 0.70710678f, -0.70710678f }, { 0.92387953f, -0.38268343f } };
+
+// ---- glm semantics written out (external/glm: dot = x*x' + y*y', length = sqrt(dot), mix = x(1-a) + y a) ----
+struct V2 { float x, y; };
+inline V2 v2(float x, float y) { return V2{ x, y }; }
+inline V2 operator+(V2 a, V2 b) { return { a.x + b.x, a.y + b.y }; }
+inline V2 operator-(V2 a, V2 b) { return { a.x - b.x, a.y - b.y }; }
+inline V2 operator-(V2 a) { return { -a.x, -a.y }; }
+inline V2 operator*(V2 a, float s) { return { a.x * s, a.y * s }; }
+inline V2 operator*(float s, V2 a) { return { s * a.x, s * a.y }; }
+inline V2 operator/(V2 a, float s) { return { a.x / s, a.y / s }; }
+inline float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+inline float length(V2 a) { return std::sqrt(dot(a, a)); }
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+inline V2 mix2(V2 a, V2 b, float t) { return { mixf(a.x, b.x, t), mixf(a.y, b.y, t) }; }
+inline bool nonzero(V2 a) { return a.x != 0.0f || a.y != 0.0f; }
+V2 safeNormalize(V2 v) { float l = length(v); return l > SAFE_NORMALIZE_EPS ? v / l : V2{ 0.0f, 0.0f }; }   // physics.comp / cpu_crowd_sim.cpp:89
+struct G3 { float gx, gy, w; };   // mapExactGrad's vec4 (gradient xy, value)
+
+uint32_t hash32(uint32_t x) {   // lowbias32 (cpu_crowd_sim.cpp:116-121)
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// The stacked-pair direction's sine and cosine: Cody-Waite reduction by pi/2 and the Cephes single
+// precision polynomials (S. L. Moshier, sinf.c / cosf.c), the same float operations as the GPU's.
+void sincosCephes(float x, float* s, float* c) {
+    const float FOPI = 1.27323954473516f;
+    const float DP1 = 0.78515625f, DP2 = 2.4187564849853515625e-4f, DP3 = 3.77489497744594108e-8f;
+    int j = (int)(x * FOPI);
+    float y = (float)j;
+    if (j & 1) { j += 1; y += 1.0f; }
+    j &= 7;
+    const float z = ((x - y * DP1) - y * DP2) - y * DP3;
+    const float zz = z * z;
+    const float ps = ((-1.9515295891e-4f * zz + 8.3321608736e-3f) * zz - 1.6666654611e-1f) * zz * z + z;
+    const float pc = ((2.443315711809948e-5f * zz - 1.388731625493765e-3f) * zz + 4.166664568298827e-2f) * zz * zz - 0.5f * zz + 1.0f;
+    float sn, cs;
+    switch (j) {
+    case 0: sn = ps; cs = pc; break;
+    case 2: sn = pc; cs = -ps; break;
+    case 4: sn = -ps; cs = -pc; break;
+    default: sn = -pc; cs = ps; break;
+    }
+    *s = sn; *c = cs;
+}
+
+V2 pairDirection(uint32_t self, uint32_t other) {   // cpu_crowd_sim.cpp:123-129
+    uint32_t lo = std::min(self, other), hi = std::max(self, other);
+    float angle = float(hash32(hash32(lo) ^ hi)) * HASH_TO_RADIANS;
+    float s, c;
+    sincosCephes(angle, &s, &c);
+    V2 dir{ c, s };
+    return self == lo ? dir : -dir;
+}
+
+// Squared support and contact distances: the reference's folded constants R_SEP * R_SEP and BODY_DIAM * BODY_DIAM
+// (sim_params.hpp:502,512; the squares of the decimals, one float step off the float squares) where the pair has the
+// reference's distances, else the float square
+constexpr float R_SEP2 = 0x1.0c6f7ap-12f, BODY_DIAM2 = 0x1.5798eep-13f;
+inline float sepSupport2(float S) { return S == R_SEP ? R_SEP2 : S * S; }
+inline float contact2(float D) { return D == BODY_DIAM ? BODY_DIAM2 : D * D; }
+
+// separationForce (cpu_crowd_sim.cpp:131-139): support S = r_i + r_j + SEP_MARGIN
+V2 separationForce(V2 offset, uint32_t self, uint32_t other, float S) {
+    float d2 = dot(offset, offset);
+    if (d2 >= sepSupport2(S)) return V2{ 0.0f, 0.0f };
+    float w = 1.0f - std::sqrt(d2) / S;
+    float magnitude = K_SEP * w * w;
+    float dxy2 = dot(offset, offset);
+    if (dxy2 <= SEP_STACKED_XY2) return magnitude * pairDirection(self, other);
+    return magnitude * offset * (1.0f / std::sqrt(dxy2));
+}
+
+// contactCorrection (cpu_crowd_sim.cpp:141-148): contact distance D = r_i + r_j
+V2 contactCorrection(V2 offset, uint32_t self, uint32_t other, float D) {
+    float dxy2 = dot(offset, offset);
+    if (dxy2 >= contact2(D)) return V2{ 0.0f, 0.0f };
+    float dxy = std::sqrt(dxy2);
+    float shift = 0.5f * (D - dxy);
+    if (dxy2 <= SEP_STACKED_XY2) return shift * pairDirection(self, other);
+    return shift * (offset / dxy);
+}
+
+float rootStep(float br[4], float t, G3 g, V2 u) {   // cpu_crowd_sim.cpp:173-178; br = (a, f(a), b, f(b))
+    if (g.w > 0.0f) { br[0] = t; br[1] = g.w; } else { br[2] = t; br[3] = g.w; }
+    float du = g.gx * u.x + g.gy * u.y;
+    float tn = du < 0.0f ? t - g.w / du : -1.0f;
+    return tn > br[0] && tn < br[2] ? tn : (br[0] * br[3] - br[2] * br[1]) / (br[3] - br[1]);
+}
+
+float newtonStep(float t, G3 g, V2 u) {   // cpu_crowd_sim.cpp:180-183
+    float du = g.gx * u.x + g.gy * u.y;
+    return du < 0.0f ? t - g.w / du : t;
+}
+
+struct SlotState { float x, y, w, vx, vy, acc; };   // position xy, active flag, velocity xy, projection displacement (velocity.w)
+
+} // namespace
+
+struct oracle_ctx {
+    // scene (copied)
+    oracle_scene sc;
+    std::vector<float> sdf, invCapS, stair, exitRects;
+    std::vector<uint32_t> navInfo, navSlot;
+    std::vector<uint16_t> navDist;
+    oracle_params par;
+    int n = 0, threads = 1;
+    // agent state, ping-pong in agent order
+    std::vector<float> x[2], y[2], vx[2], vy[2], spd[2], w[2];
+    std::vector<float> r, v0;
+    int readIdx = 0, writeIdx = 1;
+    std::vector<float> driveX, driveY;
+    bool drive = false;
+    // grid
+    std::vector<uint32_t> key, id, keyTmp, idTmp, bucketStart, cellStart, cellCount;
+    // slot order
+    std::vector<SlotState> slotA, slotB;
+    std::vector<float> planes;   // 4 per slot
+    std::vector<float> egX, egY;   // e_goal per agent (reference: goalDirection, evaluated on demand)
+    // exit choice (nav::ChoiceState)
+    std::vector<uint32_t> assign;
+    uint32_t queue[128] = {}, switches = 0, forced = 0, evaluations = 0, eligible = 0;
+    uint32_t choiceStep = 0;
+    std::vector<uint8_t> pendEligible;
+    std::vector<uint32_t> pendBest, pendBin, hist, threshold, ahead;
+    // evacuation
+    std::vector<int32_t> evacStep, evacExit;
+    int steps = 0;
+    double times[8] = {};
+
+    // ---------------- sampling ----------------
+    // mapExactGrad (cpu_crowd_sim.cpp:239-262) on the 2D grid; map() shares its arithmetic (exact weights)
+    G3 mapExactGrad(V2 p) const {
+        const float fx = (p.x - sc.sdfOX) * sc.sdfInvH, fy = (p.y - sc.sdfOY) * sc.sdfInvH;
+        const bool outside = fx < -0.5f || fy < -0.5f || fx > float(sc.sdfNX) - 0.5f || fy > float(sc.sdfNY) - 0.5f;
+        const int ix = std::clamp(int(std::floor(fx)), 0, sc.sdfNX - 2), iy = std::clamp(int(std::floor(fy)), 0, sc.sdfNY - 2);
+        const float wx = fx - float(ix), wy = fy - float(iy);
+        const float* row0 = &sdf[size_t(iy) * size_t(sc.sdfNX)];
+        const float* row1 = row0 + sc.sdfNX;
+        const float c00 = row0[ix], c10 = row0[ix + 1], c01 = row1[ix], c11 = row1[ix + 1];
+        const float x0 = mixf(c00, c10, wx), x1 = mixf(c01, c11, wx);
+        if (outside) return G3{ 0.0f, 0.0f, 1.0f };
+        return G3{ mixf(c10 - c00, c11 - c01, wy) * sc.sdfInvH, (x1 - x0) * sc.sdfInvH, mixf(x0, x1, wy) };
+    }
+    float map(V2 p) const { return mapExactGrad(p).w; }
+
+    // calcGradient / calcNormal (cpu_crowd_sim.cpp:297-305): the tetrahedral stencil's xy rows
+    V2 calcGradient(V2 p) const {
+        const float H = OBSTACLE_GRAD_H;
+        const float m1 = map(v2(p.x + H, p.y + -H)), m2 = map(v2(p.x + -H, p.y + -H));
+        const float m3 = map(v2(p.x + -H, p.y + H)), m4 = map(v2(p.x + H, p.y + H));
+        const V2 g{ ((m1 + -m2) + -m3) + m4, ((-m1 + -m2) + m3) + m4 };
+        return g * OBSTACLE_GRAD_SCALE;
+    }
+
+    // ---------------- navigation (exit_choice.cpp:512-578) ----------------
+    bool fit(size_t i) const { return (navInfo[i] & NAV_FIT) != 0u; }
+    bool has(size_t i) const { return (navInfo[i] & NAV_HAS) != 0u; }
+    int choiceCell(V2 p) const {   // choiceCell (exit_choice.cpp:512-548), no surface rule
+        const float gx = (p.x - sc.navOX) * sc.navInvCell, gy = (p.y - sc.navOY) * sc.navInvCell;
+        const int cx = std::clamp(int(std::floor(gx)), 0, sc.navRX - 1), cy = std::clamp(int(std::floor(gy)), 0, sc.navRY - 1);
+        int i = cx + cy * sc.navRX;
+        if (!fit(size_t(i))) {
+            float best = 1e9f;
+            for (int r = 1; r <= 2 && best == 1e9f; r++)
+                for (int dy = -r; dy <= r; dy++)
+                    for (int dx = -r; dx <= r; dx++) {
+                        if (std::max(std::abs(dx), std::abs(dy)) != r) continue;
+                        const int mx = cx + dx, my = cy + dy;
+                        if (mx < 0 || my < 0 || mx >= sc.navRX || my >= sc.navRY) continue;
+                        const int j = mx + my * sc.navRX;
+                        if (!fit(size_t(j)) || !has(size_t(j))) continue;
+                        const float qx = gx - (float(mx) + 0.5f), qy = gy - (float(my) + 0.5f);
+                        const float d2 = qx * qx + qy * qy;
+                        if (d2 < best) { best = d2; i = j; }
+                    }
+        }
+        return i;
+    }
+    uint32_t slotExit(size_t cell, int k) const { return (navSlot[cell * size_t(sc.K) + size_t(k)] >> CHOICE_EXIT_SHIFT) & 63u; }
+    uint32_t slotAnchor(size_t cell, int k) const { return navSlot[cell * size_t(sc.K) + size_t(k)] & CHOICE_ANCHOR_MASK; }
+    uint32_t slotDistQ(size_t cell, int k) const { return navDist[cell * size_t(sc.K) + size_t(k)]; }
+    float slotDist(size_t cell, int k) const {
+        const uint32_t q = slotDistQ(cell, k);
+        return q == CHOICE_DIST_EMPTY ? std::numeric_limits<float>::infinity() : float(q) * sc.distQuantum;
+    }
+    V2 anchorCentre(uint32_t a) const {   // exit_choice.cpp:550-555
+        const uint32_t res = uint32_t(sc.navRX);
+        const float cx = float(a % res) + 0.5f, cy = float(a / res) + 0.5f;
+        const float px = cx * sc.navCell, py = cy * sc.navCell;
+        return V2{ sc.navOX + px, sc.navOY + py };
+    }
+    V2 choiceDirection(V2 p, uint32_t exit) const {   // exit_choice.cpp:557-566
+        const size_t c = size_t(choiceCell(p));
+        for (int k = 0; k < sc.K; ++k) {
+            if (slotExit(c, k) != exit) continue;
+            const V2 d = anchorCentre(slotAnchor(c, k)) - p;
+            const float l = length(d);
+            return l > GOAL_DEADZONE ? d / l : V2{ 0.0f, 0.0f };
+        }
+        return V2{ 0.0f, 0.0f };
+    }
+    V2 goalDirection(V2 p, uint32_t agent) const {   // cpu_crowd_sim.cpp:213-218
+        if (drive) return V2{ driveX[agent], driveY[agent] };
+        return choiceDirection(p, assign[agent]);
+    }
+    float estimatedTime(float dist, uint32_t exit, uint32_t queued) const {   // exit_choice.cpp:580-584
+        const float walk = dist * sc.secondsPerUnit;
+        const float wait = float(queued) * invCapS[exit];
+        return walk + wait;
+    }
+    int slotOf(size_t cell, uint32_t exit) const {
+        int k = -1;
+        for (int i = 0; i < sc.K; ++i)
+            if (slotExit(cell, i) == exit) k = i;
+        return k;
+    }
+    uint32_t aheadOf(const uint32_t* aRead, uint32_t exit, uint32_t q) const {
+        return aRead[size_t(exit) * CHOICE_AHEAD_BINS + std::min(q >> CHOICE_AHEAD_SHIFT, uint32_t(CHOICE_AHEAD_BINS - 1))];
+    }
+    int bestCandidate(size_t cell, const uint32_t* aRead, float& eBest) const {   // exit_choice.cpp:593-603
+        int best = -1;
+        eBest = 0.0f;
+        for (int k = 0; k < sc.K; ++k) {
+            const uint32_t ex = slotExit(cell, k);
+            if (ex == CHOICE_EMPTY) break;
+            const float e = estimatedTime(slotDist(cell, k), ex, aheadOf(aRead, ex, slotDistQ(cell, k)));
+            if (best < 0 || e < eBest) { best = k; eBest = e; }
+        }
+        return best;
+    }
+
+    // exit_choice.cpp:683-791, sequential in agent order
+    void chooseExits() {
+        const uint32_t qRead = (choiceStep & 1u) * uint32_t(CHOICE_QUEUE_STRIDE), qWrite = uint32_t(CHOICE_QUEUE_STRIDE) - qRead;
+        const bool evalStep = choiceStep > 0u && choiceStep % uint32_t(CHOICE_STEPS) == 0u;
+        const size_t perParity = size_t(CHOICE_QUEUE_STRIDE) * CHOICE_AHEAD_BINS;
+        uint32_t* aRead = ahead.data() + size_t(choiceStep & 1u) * perParity;
+        uint32_t* aWrite = ahead.data() + size_t(1u - (choiceStep & 1u)) * perParity;
+        for (int e = 0; e < sc.E; ++e) {
+            uint32_t run = 0;
+            for (int b = 0; b < CHOICE_AHEAD_BINS; ++b) {
+                const size_t i = size_t(e) * CHOICE_AHEAD_BINS + size_t(b);
+                const uint32_t h = aRead[i];
+                aRead[i] = run;
+                run += h;
+                aWrite[i] = 0u;
+            }
+        }
+        const float* px = x[readIdx].data();
+        const float* py = y[readIdx].data();
+        const float* pw = w[readIdx].data();
+        if (evalStep) {
+            for (int i = 0; i < n; ++i) {   // evaluate (exit_choice.cpp:605-623)
+                pendEligible[size_t(i)] = 0;
+                if (pw[i] == 0.0f) continue;
+                const size_t c = size_t(choiceCell(V2{ px[i], py[i] }));
+                if (slotExit(c, 0) == CHOICE_EMPTY) continue;
+                const uint32_t current = assign[size_t(i)];
+                const int curK = slotOf(c, current);
+                if (curK < 0) continue;
+                ++evaluations;
+                float eBest;
+                const int best = bestCandidate(c, aRead, eBest);
+                const float eCur = estimatedTime(slotDist(c, curK), current, aheadOf(aRead, current, slotDistQ(c, curK)));
+                const float gain = eCur - eBest;
+                if (best != curK && gain > std::max(CHOICE_REL_MARGIN * eCur, CHOICE_ABS_MARGIN_S)) {
+                    pendEligible[size_t(i)] = 1;
+                    pendBest[size_t(i)] = slotExit(c, best);
+                    pendBin[size_t(i)] = std::min(uint32_t(gain), uint32_t(CHOICE_GAIN_BINS - 1));
+                    ++hist[size_t(current) * CHOICE_GAIN_BINS + pendBin[size_t(i)]];
+                    ++eligible;
+                }
+            }
+            for (int e = 0; e < sc.E; ++e) {   // switchThreshold (exit_choice.cpp:625-639)
+                uint32_t* h = hist.data() + size_t(e) * CHOICE_GAIN_BINS;
+                const uint32_t queued = queue[qRead + uint32_t(e)];
+                const uint32_t cap = std::max(CHOICE_MIN_SWITCHES, uint32_t(float(queued) * CHOICE_SWITCH_SHARE));
+                uint32_t cum = 0, thr = uint32_t(CHOICE_GAIN_BINS);
+                for (int b = CHOICE_GAIN_BINS - 1; b >= 0; --b) {
+                    const uint32_t cnt = h[b];
+                    if (cnt == 0) continue;
+                    if (cum + cnt > cap) {
+                        if (cum == 0) thr = uint32_t(b);
+                        break;
+                    }
+                    cum += cnt;
+                    thr = uint32_t(b);
+                }
+                threshold[size_t(e)] = thr;
+                std::fill(h, h + CHOICE_GAIN_BINS, 0u);
+            }
+        }
+        std::fill(queue + qWrite, queue + qWrite + CHOICE_QUEUE_STRIDE, 0u);
+        for (int i = 0; i < n; ++i) {   // apply (exit_choice.cpp:641-656) and the counts of the next step
+            if (pw[i] == 0.0f) continue;
+            const size_t c = size_t(choiceCell(V2{ px[i], py[i] }));
+            const uint32_t current = assign[size_t(i)];
+            uint32_t ex = current;
+            const bool hasPath = slotExit(c, 0) != CHOICE_EMPTY;
+            const int curK = hasPath ? slotOf(c, current) : -1;
+            if (hasPath && curK < 0) {
+                float eBest;
+                ex = slotExit(c, bestCandidate(c, aRead, eBest));
+                forced += current != CHOICE_NONE ? 1u : 0u;
+            } else if (evalStep && curK >= 0 && pendEligible[size_t(i)] && pendBin[size_t(i)] >= threshold[current]) {
+                ex = pendBest[size_t(i)];
+                ++switches;
+            }
+            assign[size_t(i)] = ex;
+            if (ex != CHOICE_NONE) ++queue[qWrite + ex];
+            const int k = slotOf(c, ex);
+            if (k >= 0) ++aWrite[size_t(ex) * CHOICE_AHEAD_BINS + std::min(slotDistQ(c, k) >> CHOICE_AHEAD_SHIFT, uint32_t(CHOICE_AHEAD_BINS - 1))];
+        }
+        ++choiceStep;
+    }
+
+    // ---------------- static contacts (cpu_crowd_sim.cpp:338-500, 2D) ----------------
+    float wallContact(V2 np, float fc, V2 u, float fr, float reach, float recastTol, V2& away) const {   // :338-363
+        float br[4] = { 0.0f, fc, reach, fr };
+        float t = reach * fc / (fc - fr);
+        G3 g{ 0.0f, 0.0f, 0.0f };
+        for (int i = 0; i < WALL_ROOT_STEPS; ++i) {
+            g = mapExactGrad(np + u * t);
+            t = rootStep(br, t, g, u);
+        }
+        float e = t;   // a 2D gradient has n_z = 0: always a wall
+        away = -u;
+        V2 nn = safeNormalize(V2{ g.gx, g.gy });
+        V2 u2 = -nn;
+        float t2 = 0.0f, t3 = t * dot(u, u2);
+        G3 g2{ 0.0f, 0.0f, 0.0f };
+        for (int i = 0; i < WALL_RECAST_STEPS; ++i) {
+            t2 = t3;
+            g2 = mapExactGrad(np + u2 * t2);
+            t3 = newtonStep(t2, g2, u2);
+        }
+        bool ok = (g2.gx * u2.x + g2.gy * u2.y) < 0.0f && std::abs(t3 - t2) < recastTol && t3 > 0.0f;
+        if (ok && t3 < e) { e = t3; away = nn; }
+        return e;
+    }
+
+    // clearWalls (:365-421); wall = (normal, clearance - r)
+    bool clearWalls(V2& np, V2& nv, float rad, float wall[3]) const {
+        bool moved = false;
+        wall[0] = wall[1] = wall[2] = 0.0f;
+        const float reach = FEELER_REACH_RADII * rad;
+        const float recastTol = WALL_RECAST_TOL_RADII * rad;
+        for (int pass = 0; pass < WALL_PASSES; ++pass) {
+            float fc = map(np);
+            float fr1 = 0.0f, fr2 = 0.0f;
+            int i1 = -1, i2 = -1;
+            for (int i = 0; i < 16; ++i) {
+                const V2 f = np + V2{ FEELER_DIRS[i][0] * reach, FEELER_DIRS[i][1] * reach };
+                float v = map(f);
+                if (v < fr1) { fr2 = fr1; i2 = i1; fr1 = v; i1 = i; }
+                else if (v < fr2) { fr2 = v; i2 = i; }
+            }
+            if (i1 < 0) break;
+            V2 hNorm;
+            float clearance;
+            if (fc > 0.0f) {
+                clearance = wallContact(np, fc, V2{ FEELER_DIRS[i1][0], FEELER_DIRS[i1][1] }, fr1, reach, recastTol, hNorm);
+                if (i2 >= 0) {
+                    V2 h2;
+                    float c2 = wallContact(np, fc, V2{ FEELER_DIRS[i2][0], FEELER_DIRS[i2][1] }, fr2, reach, recastTol, h2);
+                    if (c2 < clearance) { clearance = c2; hNorm = h2; }
+                }
+            } else {
+                G3 g = mapExactGrad(np);
+                hNorm = safeNormalize(V2{ g.gx, g.gy });
+                clearance = g.w / std::max(length(V2{ g.gx, g.gy }), CENTRE_GRAD_MIN);
+            }
+            if (clearance >= rad) {
+                if (clearance < 1e9f) { wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = clearance - rad; }
+                break;
+            }
+            np = np + hNorm * (rad - clearance);
+            moved = true;
+            wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = 0.0f;
+            float vDotN = dot(nv, hNorm);
+            if (vDotN < 0.0f) nv = nv - hNorm * vDotN;
+        }
+        return moved;
+    }
+
+    void wallPlaneProbe(V2 np, float rad, float wall[3]) const {   // :423-442
+        wall[0] = wall[1] = wall[2] = 0.0f;
+        const float reach = PLANE_PROBE_RADII * rad;
+        float fc = map(np);
+        float fr = 0.0f;
+        int i1 = -1;
+        for (int i = 0; i < 16; ++i) {
+            const V2 f = np + V2{ FEELER_DIRS[i][0] * reach, FEELER_DIRS[i][1] * reach };
+            float v = map(f);
+            if (v < fr) { fr = v; i1 = i; }
+        }
+        if (i1 < 0 || fc <= 0.0f) return;
+        V2 away;
+        float c = wallContact(np, fc, V2{ FEELER_DIRS[i1][0], FEELER_DIRS[i1][1] }, fr, reach, WALL_RECAST_TOL_RADII * rad, away);
+        if (c < 1e9f) { wall[0] = away.x; wall[1] = away.y; wall[2] = std::max(c - rad, 0.0f); }
+    }
+
+    bool solid(V2 p) const { return mapExactGrad(p).w <= 0.0f; }   // bakedSolid (:444) without the fence
+
+    bool sweepBack(V2 pOld, V2& np, V2& nv) const {   // :446-461 at one floor height
+        if (!solid(np)) return false;
+        float lo = 0.0f, hi = 1.0f;
+        if (!solid(np)) lo = 1.0f;
+        for (int i = 0; i < STATIC_SWEEP_BISECTIONS; ++i) {
+            float t = 0.5f * (lo + hi);
+            if (solid(mix2(pOld, np, t))) hi = t;
+            else lo = t;
+        }
+        V2 d = safeNormalize(np - pOld);
+        np = mix2(pOld, np, lo);
+        float vd = dot(nv, d);
+        if (vd > 0.0f) nv = nv - d * vd;
+        return true;
+    }
+
+    // resolveStatic (:465-500): on a flat floor the centre is 'grounded' unless it lies in a wall
+    bool resolveStatic(V2 pOld, V2& np, V2& nv, float rad, float wall[3]) const {
+        bool grounded = !solid(np);
+        if (!grounded && sweepBack(pOld, np, nv)) grounded = !solid(np);
+        const bool walls = clearWalls(np, nv, rad, wall);
+        if (walls && grounded) {
+            float w2[3];
+            clearWalls(np, nv, rad, w2);
+            if (w2[0] != 0.0f || w2[1] != 0.0f) { wall[0] = w2[0]; wall[1] = w2[1]; wall[2] = w2[2]; }
+        }
+        return grounded;
+    }
+
+    // ---------------- the passes ----------------
+    int exitOf(V2 p) const {
+        for (int e = 0; e < sc.E; ++e) {
+            const float* r = &exitRects[size_t(e) * 4];
+            if (p.x >= r[0] && p.x <= r[2] && p.y >= r[1] && p.y <= r[3]) return e;
+        }
+        return -1;
+    }
+    float stairFactor(V2 p) const {
+        const float gx = (p.x - sc.navOX) * sc.navInvCell, gy = (p.y - sc.navOY) * sc.navInvCell;
+        const int cx = std::clamp(int(std::floor(gx)), 0, sc.navRX - 1), cy = std::clamp(int(std::floor(gy)), 0, sc.navRY - 1);
+        return stair[size_t(cx) + size_t(cy) * size_t(sc.navRX)];
+    }
+
+    // PHYSICS_PASS 1 (cpu_crowd_sim.cpp:655-777)
+    void predict(uint32_t slot) {
+        const uint32_t self = id[slot];
+        const uint32_t cid = key[slot];
+        if (cid == INACTIVE_CELL) return;
+        const float* px = x[readIdx].data();
+        const float* py = y[readIdx].data();
+        const V2 p{ px[self], py[self] };
+        const V2 v{ vx[readIdx][self], vy[readIdx][self] };
+        const float ri = r[self];
+        if (exitOf(p) >= 0 || p.x < sc.xmin || p.x > sc.xmax || p.y < sc.ymin || p.y > sc.ymax) {
+            slotA[slot] = SlotState{ PARK_COORD, PARK_COORD, 0.0f, 0.0f, 0.0f, 0.0f };
+            return;
+        }
+        const V2 eGoal{ egX[self], egY[self] };
+        const int cx = int(cid % uint32_t(sc.hashNX));
+        const int cy = int(cid / uint32_t(sc.hashNX));
+        V2 sep{ 0.0f, 0.0f };
+        float sAhead = 1e9f;   // the smallest gap (centre distance - contact distance) to an agent ahead
+        const int R = sc.percRing;
+        for (int yy = -R; yy <= R; yy++) {
+            for (int xx = -R; xx <= R; xx++) {
+                const int nx = cx + xx, ny = cy + yy;
+                if (nx < 0 || nx >= sc.hashNX || ny < 0 || ny >= sc.hashNY) continue;
+                const size_t c = size_t(nx) + size_t(ny) * size_t(sc.hashNX);
+                const uint32_t st = cellStart[c], cnt = cellCount[c];
+                for (uint32_t k = st; k < st + cnt; k++) {
+                    if (k == slot) continue;
+                    const uint32_t other = id[k];
+                    const V2 op{ px[other], py[other] };
+                    const V2 offset = p - op;
+                    const float dxy2 = dot(offset, offset);
+                    if (dxy2 < R_PERC2) {
+                        const float D = ri + r[other];
+                        sep = sep + separationForce(offset, self, other, D + SEP_MARGIN);
+                        const float ahead = -dot(offset, eGoal);
+                        const float lateral = std::abs(offset.x * eGoal.y - offset.y * eGoal.x);
+                        const float dxy = std::sqrt(dxy2);
+                        const float gap = dxy - D;
+                        if (ahead > 0.0f && lateral < D && gap < sAhead) {
+                            const V2 eOther{ egX[other], egY[other] };
+                            const float aheadOther = dot(offset, eOther);
+                            const bool mutual = aheadOther > 0.0f && std::abs(offset.x * eOther.y - offset.y * eOther.x) < D;
+                            if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = gap;
+                        }
+                    }
+                }
+            }
+        }
+        const float vFree = stairFactor(p) * v0[self];
+        const float vDes = std::clamp(sAhead * HEADWAY_RATE, 0.0f, vFree) * length(eGoal);
+        const V2 eDir = safeNormalize(eGoal + sep);
+
+        V2 aObs{ 0.0f, 0.0f };   // obstacle avoidance (:739-748)
+        const float dTorso = map(p);
+        if (dTorso < OBSTACLE_RANGE) {
+            const V2 norm = safeNormalize(calcGradient(p));
+            if (length(norm) > OBSTACLE_MIN_NORMAL) aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * par.obstacleGain;
+        }
+        const V2 vEq = eDir * vDes + aObs * par.tau;   // kinematics (:752-766)
+        V2 nv = vEq + (v - vEq) * par.relaxStep;
+        const float along = dot(nv, eGoal);
+        if (along > vDes) nv = nv - eGoal * (along - vDes);
+        if (length(nv) > v0[self]) nv = safeNormalize(nv) * v0[self];
+        V2 np = p + nv * DT;
+
+        float wall[3];
+        const bool grounded = resolveStatic(p, np, nv, ri, wall);
+        if (wall[0] == 0.0f && wall[1] == 0.0f) wallPlaneProbe(np, ri, wall);
+        slotA[slot] = SlotState{ np.x, np.y, 1.0f, nv.x, nv.y, 0.0f };
+        const V2 wn{ wall[0], wall[1] };
+        float* pl = &planes[size_t(slot) * 4];
+        pl[0] = wn.x; pl[1] = wn.y; pl[2] = dot(np, wn) - wall[2];
+        pl[3] = (nonzero(wn) ? PLANE_WALL : 0.0f) + (grounded ? PLANE_GROUNDED : 0.0f);
+    }
+
+    // projectSlot (cpu_crowd_sim.cpp:781-822): one Jacobi iteration, reads 'in'
+    SlotState projectSlot(const std::vector<SlotState>& in, uint32_t slot, uint32_t self, uint32_t cid) const {
+        SlotState b = in[slot];
+        if (b.w == 0.0f) return b;
+        const int cx = int(cid % uint32_t(sc.hashNX));
+        const int cy = int(cid / uint32_t(sc.hashNX));
+        const float ri = r[self];
+        V2 corr{ 0.0f, 0.0f };
+        int nContacts = 0;
+        const int R = sc.projRing;
+        for (int yy = -R; yy <= R; yy++) {
+            for (int xx = -R; xx <= R; xx++) {
+                const int nx = cx + xx, ny = cy + yy;
+                if (nx < 0 || nx >= sc.hashNX || ny < 0 || ny >= sc.hashNY) continue;
+                const size_t c = size_t(nx) + size_t(ny) * size_t(sc.hashNX);
+                const uint32_t st = cellStart[c], cnt = cellCount[c];
+                for (uint32_t k = st; k < st + cnt; k++) {
+                    if (k == slot) continue;
+                    const SlotState& o = in[k];
+                    const V2 offset = V2{ b.x, b.y } - V2{ o.x, o.y };
+                    const uint32_t other = id[k];
+                    const float D = ri + r[other];
+                    if (o.w != 0.0f && dot(offset, offset) < contact2(D)) {
+                        corr = corr + contactCorrection(offset, self, other, D);
+                        ++nContacts;
+                    }
+                }
+            }
+        }
+        corr = corr * (PROJ_RELAX / float(std::max(nContacts, 1)));
+        const float maxShift = PROJ_MAX_SHIFT_RADII * ri;
+        float len = length(corr);
+        if (len > maxShift) corr = corr * (maxShift / len);
+        V2 q = V2{ b.x, b.y } + corr;
+        const float* pl = &planes[size_t(slot) * 4];
+        if (std::fmod(pl[3], 2.0f) == PLANE_WALL) {
+            float s = pl[2] - dot(q, V2{ pl[0], pl[1] });
+            if (s > 0.0f) q = q + V2{ pl[0], pl[1] } * s;
+        }
+        const V2 d = q - V2{ b.x, b.y };
+        b.x = q.x; b.y = q.y;
+        b.acc += length(d);
+        return b;
+    }
+
+    // PHYSICS_PASS 3 (cpu_crowd_sim.cpp:835-866)
+    void finalPass(const std::vector<SlotState>& in, uint32_t slot, bool projectHere) {
+        const uint32_t self = id[slot];
+        const uint32_t cid = key[slot];
+        const int wi = writeIdx, ri0 = readIdx;
+        if (cid == INACTIVE_CELL) {
+            x[wi][self] = x[ri0][self]; y[wi][self] = y[ri0][self]; w[wi][self] = w[ri0][self];
+            vx[wi][self] = vx[ri0][self]; vy[wi][self] = vy[ri0][self]; spd[wi][self] = spd[ri0][self];
+            return;
+        }
+        const SlotState b = projectHere ? projectSlot(in, slot, self, cid) : in[slot];
+        if (b.w == 0.0f) {
+            x[wi][self] = b.x; y[wi][self] = b.y; w[wi][self] = 0.0f;
+            vx[wi][self] = b.vx; vy[wi][self] = b.vy; spd[wi][self] = b.acc;
+            return;
+        }
+        V2 np{ b.x, b.y }, nv{ b.vx, b.vy };
+        const V2 pOld{ x[ri0][self], y[ri0][self] };
+        if (b.acc > 0.0f) {
+            float wall[3];
+            resolveStatic(pOld, np, nv, r[self], wall);
+        }
+        x[wi][self] = np.x; y[wi][self] = np.y; w[wi][self] = 1.0f;
+        vx[wi][self] = nv.x; vy[wi][self] = nv.y; spd[wi][self] = length(nv);
+    }
+};
+
+namespace {
+template <class Body>
+void parallelFor(int threads, int n, bool dynamic, const Body& body) {
+    if (dynamic) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+        for (int i = 0; i < n; ++i) body(i);
+    } else {
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; ++i) body(i);
+    }
+}
+double msBetween(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+} // namespace
+
+extern "C" {
+
+oracle_ctx* oracle_create(const oracle_scene* s, const oracle_params* p, int n) {
+    if (!s || !p || n <= 0) return nullptr;
+    oracle_ctx* c = new oracle_ctx();
+    c->sc = *s;
+    c->par = *p;
+    c->n = n;
+    c->threads = p->threads > 0 ? p->threads : omp_get_max_threads();
+    c->sdf.assign(s->sdf, s->sdf + size_t(s->sdfNX) * s->sdfNY);
+    const size_t nc = size_t(s->navRX) * s->navRY;
+    c->navInfo.assign(s->navInfo, s->navInfo + nc);
+    c->navSlot.assign(s->navSlot, s->navSlot + nc * size_t(s->K));
+    c->navDist.assign(s->navDist, s->navDist + nc * size_t(s->K));
+    c->stair.assign(s->stair, s->stair + nc);
+    c->invCapS.assign(s->invCapS, s->invCapS + s->E);
+    c->exitRects.assign(s->exitRects, s->exitRects + size_t(s->E) * 4);
+    c->sc.sdf = nullptr; c->sc.navInfo = nullptr; c->sc.navSlot = nullptr; c->sc.navDist = nullptr;
+    c->sc.stair = nullptr; c->sc.invCapS = nullptr; c->sc.exitRects = nullptr;
+    for (int h = 0; h < 2; ++h) {
+        c->x[h].assign(size_t(n), 0.0f); c->y[h].assign(size_t(n), 0.0f); c->vx[h].assign(size_t(n), 0.0f);
+        c->vy[h].assign(size_t(n), 0.0f); c->spd[h].assign(size_t(n), 0.0f); c->w[h].assign(size_t(n), 0.0f);
+    }
+    c->r.assign(size_t(n), 0.0f); c->v0.assign(size_t(n), 0.0f);
+    const size_t cells = size_t(s->hashNX) * s->hashNY;
+    c->key.assign(size_t(n), 0u); c->id.assign(size_t(n), 0u); c->keyTmp.assign(size_t(n), 0u); c->idTmp.assign(size_t(n), 0u);
+    c->bucketStart.assign(cells + 2, 0u);
+    c->cellStart.assign(cells, 0u); c->cellCount.assign(cells, 0u);
+    c->slotA.assign(size_t(n), SlotState{}); c->slotB.assign(size_t(n), SlotState{});
+    c->planes.assign(size_t(n) * 4, 0.0f);
+    c->egX.assign(size_t(n), 0.0f); c->egY.assign(size_t(n), 0.0f);
+    return c;
+}
+
+void oracle_destroy(oracle_ctx* c) { delete c; }
+
+void oracle_set_state(oracle_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w,
+                      const float* r, const float* v0) {
+    const size_t n = size_t(c->n);
+    for (int h = 0; h < 2; ++h) {
+        c->x[h].assign(x, x + n); c->y[h].assign(y, y + n); c->vx[h].assign(vx, vx + n); c->vy[h].assign(vy, vy + n);
+        c->w[h].assign(w, w + n);
+        for (size_t i = 0; i < n; ++i) c->spd[h][i] = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
+    }
+    c->r.assign(r, r + n); c->v0.assign(v0, v0 + n);
+    c->readIdx = 0; c->writeIdx = 1;
+    // a fresh exit choice (nav::ChoiceState::reset, exit_choice.cpp:671-679)
+    c->assign.assign(n, CHOICE_NONE);
+    std::memset(c->queue, 0, sizeof(c->queue));
+    c->switches = c->forced = c->evaluations = c->eligible = 0;
+    c->choiceStep = 0;
+    c->pendEligible.assign(n, 0); c->pendBest.assign(n, 0u); c->pendBin.assign(n, 0u);
+    c->hist.assign(size_t(CHOICE_MAX_EXITS) * CHOICE_GAIN_BINS, 0u);
+    c->threshold.assign(size_t(CHOICE_MAX_EXITS), uint32_t(CHOICE_GAIN_BINS));
+    c->ahead.assign(size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS, 0u);
+    c->evacStep.assign(n, -1); c->evacExit.assign(n, -1);
+    c->steps = 0;
+}
+
+void oracle_restore_state(oracle_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w) {
+    const size_t n = size_t(c->n);
+    for (int h = 0; h < 2; ++h) {
+        c->x[h].assign(x, x + n); c->y[h].assign(y, y + n); c->vx[h].assign(vx, vx + n); c->vy[h].assign(vy, vy + n);
+        c->w[h].assign(w, w + n);
+        for (size_t i = 0; i < n; ++i) c->spd[h][i] = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
+    }
+    c->readIdx = 0; c->writeIdx = 1;
+}
+
+void oracle_choice_step(oracle_ctx* c) {
+    if (c->sc.E > 0) c->chooseExits();
+}
+
+void oracle_set_choice_state(oracle_ctx* c, uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead) {
+    c->choiceStep = step;
+    if (assign) c->assign.assign(assign, assign + c->n);
+    if (queue128) std::memcpy(c->queue, queue128, sizeof(c->queue));
+    if (ahead) c->ahead.assign(ahead, ahead + size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS);
+}
+
+void oracle_set_drive(oracle_ctx* c, const float* dx, const float* dy) {
+    if (!dx || !dy) { c->drive = false; return; }
+    c->driveX.assign(dx, dx + c->n);
+    c->driveY.assign(dy, dy + c->n);
+    c->drive = true;
+}
+
+void oracle_step(oracle_ctx* c, int nsteps) {
+    using Clock = std::chrono::steady_clock;
+    const int n = c->n;
+    const int T = c->threads;
+    for (int s = 0; s < nsteps; ++s) {
+        const auto t0 = Clock::now();
+        // hash (calc_grid.comp, cpu_crowd_sim.cpp:983-993): a NaN position is reset in the input
+        {
+            float* px = c->x[c->readIdx].data();
+            float* py = c->y[c->readIdx].data();
+            float* pw = c->w[c->readIdx].data();
+            const oracle_scene& sc = c->sc;
+            parallelFor(T, n, false, [&](int i) {
+                if (std::isnan(px[i])) { px[i] = 0.0f; py[i] = 0.0f; pw[i] = 1.0f; }
+                uint32_t k = INACTIVE_CELL;
+                if (pw[i] != 0.0f) {
+                    const int gx = std::clamp(int((px[i] - sc.xmin) / sc.hashSpanX * float(sc.hashNX)), 0, sc.hashNX - 1);
+                    const int gy = std::clamp(int((py[i] - sc.ymin) / sc.hashSpanY * float(sc.hashNY)), 0, sc.hashNY - 1);
+                    k = uint32_t(gx + gy * sc.hashNX);
+                }
+                c->keyTmp[size_t(i)] = k;
+                c->idTmp[size_t(i)] = uint32_t(i);
+            });
+        }
+        const auto t1 = Clock::now();
+        // stable counting sort (cpu_crowd_sim.cpp:997-1006): ascending (cell, agent), inactive at the tail
+        {
+            const uint32_t cells = uint32_t(c->sc.hashNX) * uint32_t(c->sc.hashNY);
+            auto bucketOf = [cells](uint32_t g) { return g == INACTIVE_CELL ? cells : g; };
+            std::fill(c->bucketStart.begin(), c->bucketStart.end(), 0u);
+            for (int i = 0; i < n; ++i) ++c->bucketStart[bucketOf(c->keyTmp[size_t(i)]) + 1];
+            for (size_t b = 1; b < c->bucketStart.size(); ++b) c->bucketStart[b] += c->bucketStart[b - 1];
+            for (int i = 0; i < n; ++i) {
+                const uint32_t pos = c->bucketStart[bucketOf(c->keyTmp[size_t(i)])]++;
+                c->key[pos] = c->keyTmp[size_t(i)];
+                c->id[pos] = c->idTmp[size_t(i)];
+            }
+        }
+        const auto t2 = Clock::now();
+        // cell ranges (reorder.comp, cpu_crowd_sim.cpp:1034-1043)
+        std::fill(c->cellStart.begin(), c->cellStart.end(), 0u);
+        std::fill(c->cellCount.begin(), c->cellCount.end(), 0u);
+        for (int i = 0; i < n; ++i) {
+            const uint32_t g = c->key[size_t(i)];
+            if (g == INACTIVE_CELL) continue;
+            ++c->cellCount[g];
+            if (i == 0 || g != c->key[size_t(i) - 1]) c->cellStart[g] = uint32_t(i);
+        }
+        const auto t3 = Clock::now();
+        // exit choice (PHYSICS_PASS 0), then every agent's e_goal from the start-of-step state
+        if (!c->drive && c->sc.E > 0) c->chooseExits();
+        {
+            const float* px = c->x[c->readIdx].data();
+            const float* py = c->y[c->readIdx].data();
+            const float* pw = c->w[c->readIdx].data();
+            parallelFor(T, n, true, [&](int i) {
+                if (pw[i] == 0.0f) return;
+                const V2 e = c->goalDirection(V2{ px[i], py[i] }, uint32_t(i));
+                c->egX[size_t(i)] = e.x;
+                c->egY[size_t(i)] = e.y;
+            });
+        }
+        const auto t4 = Clock::now();
+        parallelFor(T, n, true, [&](int slot) { c->predict(uint32_t(slot)); });
+        const auto t5 = Clock::now();
+        std::vector<SlotState>* in = &c->slotA;
+        std::vector<SlotState>* out = &c->slotB;
+        for (int it = 1; it < c->par.projIters; ++it) {
+            parallelFor(T, n, true, [&](int slot) {
+                const uint32_t cid = c->key[size_t(slot)];
+                if (cid != INACTIVE_CELL) (*out)[size_t(slot)] = c->projectSlot(*in, uint32_t(slot), c->id[size_t(slot)], cid);
+            });
+            std::swap(in, out);
+        }
+        const auto t6 = Clock::now();
+        const bool projectInFinal = c->par.projIters > 0;
+        parallelFor(T, n, true, [&](int slot) { c->finalPass(*in, uint32_t(slot), projectInFinal); });
+        // evacuation events: active at the step's input, parked at its output
+        {
+            const float* wIn = c->w[c->readIdx].data();
+            const float* wOut = c->w[c->writeIdx].data();
+            const float* px = c->x[c->readIdx].data();
+            const float* py = c->y[c->readIdx].data();
+            for (int i = 0; i < n; ++i)
+                if (wIn[i] != 0.0f && wOut[i] == 0.0f) {
+                    c->evacStep[size_t(i)] = c->steps;
+                    const int e = c->exitOf(V2{ px[i], py[i] });
+                    c->evacExit[size_t(i)] = e >= 0 ? e : c->sc.E;
+                }
+        }
+        std::swap(c->readIdx, c->writeIdx);
+        ++c->steps;
+        const auto t7 = Clock::now();
+        c->times[0] = msBetween(t0, t1); c->times[1] = msBetween(t1, t2); c->times[2] = msBetween(t2, t3);
+        c->times[3] = msBetween(t3, t4); c->times[4] = msBetween(t4, t5); c->times[5] = msBetween(t5, t6);
+        c->times[6] = msBetween(t6, t7); c->times[7] = msBetween(t0, t7);
+    }
+}
+
+void oracle_get_state(const oracle_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w) {
+    const int h = c->readIdx;
+    const size_t n = size_t(c->n);
+    if (x) std::copy(c->x[h].begin(), c->x[h].end(), x);
+    if (y) std::copy(c->y[h].begin(), c->y[h].end(), y);
+    if (vx) std::copy(c->vx[h].begin(), c->vx[h].end(), vx);
+    if (vy) std::copy(c->vy[h].begin(), c->vy[h].end(), vy);
+    if (speed) std::copy(c->spd[h].begin(), c->spd[h].end(), speed);
+    if (w) std::copy(c->w[h].begin(), c->w[h].end(), w);
+    (void)n;
+}
+void oracle_get_grid(const oracle_ctx* c, uint32_t* key, uint32_t* id) {
+    std::copy(c->key.begin(), c->key.end(), key);
+    std::copy(c->id.begin(), c->id.end(), id);
+}
+void oracle_get_cells(const oracle_ctx* c, uint32_t* start, uint32_t* count) {
+    std::copy(c->cellStart.begin(), c->cellStart.end(), start);
+    std::copy(c->cellCount.begin(), c->cellCount.end(), count);
+}
+void oracle_get_assign(const oracle_ctx* c, uint32_t* a) { std::copy(c->assign.begin(), c->assign.end(), a); }
+void oracle_get_choice_counters(const oracle_ctx* c, uint32_t* o) {
+    std::memcpy(o, c->queue, sizeof(c->queue));
+    o[128] = c->switches; o[129] = c->forced; o[130] = c->evaluations; o[131] = c->eligible;
+}
+void oracle_get_evac(const oracle_ctx* c, int32_t* step, int32_t* exitId) {
+    std::copy(c->evacStep.begin(), c->evacStep.end(), step);
+    std::copy(c->evacExit.begin(), c->evacExit.end(), exitId);
+}
+int oracle_num_steps(const oracle_ctx* c) { return c->steps; }
+void oracle_get_times(const oracle_ctx* c, double* t8) { std::memcpy(t8, c->times, sizeof(c->times)); }
+
+int oracle_constants(float* out, int cap) {
+    const float v[] = { DT, K_SEP, R_SEP, BODY_DIAM, SEP_MARGIN, HEADWAY_RATE, R_PERC2, PROJ_MAX_SHIFT_RADII, W_OBS,
+                        GOAL_DEADZONE, OBSTACLE_RANGE, OBSTACLE_MIN_NORMAL, OBSTACLE_GRAD_H, OBSTACLE_GRAD_SCALE,
+                        SAFE_NORMALIZE_EPS, SEP_STACKED_XY2, FEELER_REACH_RADII, WALL_RECAST_TOL_RADII, CENTRE_GRAD_MIN,
+                        PLANE_PROBE_RADII, PARK_COORD, HASH_TO_RADIANS, CHOICE_REL_MARGIN, CHOICE_ABS_MARGIN_S,
+                        CHOICE_SWITCH_SHARE, float(CHOICE_STEPS) };
+    const int m = int(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < m && i < cap; ++i) out[i] = v[i];
+    return m;
+}
+
+} // extern "C"

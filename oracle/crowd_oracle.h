@@ -1,0 +1,79 @@
+/* Oracle of the per-step crowd pipeline: a CPU restatement of the reference's CpuCrowdSim
+ * (src/cpu_crowd_sim.cpp, include/swarm/cpu_crowd_sim.hpp) restricted to flat 2D floors.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU path is compared against; only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it. The product path never
+ * links, loads or calls it.
+ *
+ * Plain C ABI (ctypes): every array is caller-owned and copied on the call. */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_ctx oracle_ctx;
+
+/* The scene tables (the same arrays the GPU path uploads). */
+typedef struct {
+    float xmin, ymin, xmax, ymax;          /* domain: agents outside are parked */
+    float hashSpanX, hashSpanY;            /* spatial hash: cell = clamp(int((p - min) / span * n)) */
+    int hashNX, hashNY;
+    int percRing, projRing;                /* neighbour rings of the predict pass and the projection */
+    int sdfNX, sdfNY;                      /* wall distance grid [sdfNY][sdfNX], texel (i, j) at (ox + i h, oy + j h) */
+    float sdfOX, sdfOY, sdfInvH;
+    const float* sdf;
+    int navRX, navRY;                      /* navigation grid and candidate table (dense per cell) */
+    float navOX, navOY, navCell, navInvCell;
+    int K, E;                              /* candidate slots per cell, exits */
+    const uint32_t* navInfo;               /* per cell: NAV_FIT 0x80000000 | NAV_HAS 0x40000000 */
+    const uint32_t* navSlot;               /* [cell * K + k]: anchor | exit << 25 */
+    const uint16_t* navDist;               /* [cell * K + k]: distance quanta */
+    const float* invCapS;                  /* per exit: seconds per queued agent */
+    float secondsPerUnit, distQuantum;
+    const float* stair;                    /* per nav cell: free-speed factor */
+    const float* exitRects;                /* E x (lox, loy, hix, hiy) */
+} oracle_scene;
+
+typedef struct {
+    float relaxStep;      /* exp(-DT / tau), folded on the host */
+    float tau;            /* relaxation time (model time) */
+    float obstacleGain;   /* wall-avoidance gain (OBSTACLE_GAIN) */
+    int projIters;        /* Jacobi iterations (PROJ_ITERATIONS) */
+    int threads;          /* OpenMP threads, <= 0: all */
+} oracle_params;
+
+oracle_ctx* oracle_create(const oracle_scene* scene, const oracle_params* params, int n);
+void oracle_destroy(oracle_ctx* c);
+/* A full state (both halves), resets the exit choice and the evacuation record (CpuCrowdSim::setState) */
+void oracle_set_state(oracle_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w,
+                      const float* r, const float* v0);
+/* The agent state only (both halves), the exit choice runs on (CpuCrowdSim::restoreAgentState) */
+void oracle_restore_state(oracle_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w);
+/* The exit choice pass alone on the current state (CpuCrowdSim::stepExitChoice) */
+void oracle_choice_step(oracle_ctx* c);
+/* Sets the choice state: step counter, assigned exits (n), queue counters (128), queue-ahead histograms
+ * (2 x 64 x 1024); NULL arrays are left as they are */
+void oracle_set_choice_state(oracle_ctx* c, uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead);
+/* Forced desired direction per agent (CpuCrowdSim::setDriveTable); NULL: the navigation table */
+void oracle_set_drive(oracle_ctx* c, const float* dx, const float* dy);
+void oracle_step(oracle_ctx* c, int nsteps);
+void oracle_get_state(const oracle_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w);
+/* sorted (cell, agent) pairs and the cell table of the last step's input */
+void oracle_get_grid(const oracle_ctx* c, uint32_t* key, uint32_t* id);
+void oracle_get_cells(const oracle_ctx* c, uint32_t* start, uint32_t* count);
+void oracle_get_assign(const oracle_ctx* c, uint32_t* assign);
+/* queue[128], switches, forced, evaluations, eligible */
+void oracle_get_choice_counters(const oracle_ctx* c, uint32_t* out132);
+/* evacuation: per agent the step it left at (-1: not yet) and its exit (E: out of the domain) */
+void oracle_get_evac(const oracle_ctx* c, int32_t* step, int32_t* exitId);
+int oracle_num_steps(const oracle_ctx* c);
+/* wall clock of the last step per stage, ms: hash, sort, cells, choice, predict, project, final, total */
+void oracle_get_times(const oracle_ctx* c, double* t8);
+/* exported model constants, for the constant-parity test */
+int oracle_constants(float* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif

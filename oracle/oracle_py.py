@@ -1,0 +1,176 @@
+"""ctypes binding of the oracle (liboracle.so): test infrastructure only.
+
+The oracle is the CPU restatement of the reference's CpuCrowdSim that the GPU path is checked against.
+Only the tests, the smoke check and the benchmark's CPU-baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OracleScene(C.Structure):
+    _fields_ = [("xmin", C.c_float), ("ymin", C.c_float), ("xmax", C.c_float), ("ymax", C.c_float),
+                ("hashSpanX", C.c_float), ("hashSpanY", C.c_float), ("hashNX", C.c_int), ("hashNY", C.c_int),
+                ("percRing", C.c_int), ("projRing", C.c_int),
+                ("sdfNX", C.c_int), ("sdfNY", C.c_int), ("sdfOX", C.c_float), ("sdfOY", C.c_float), ("sdfInvH", C.c_float),
+                ("sdf", C.c_void_p),
+                ("navRX", C.c_int), ("navRY", C.c_int), ("navOX", C.c_float), ("navOY", C.c_float),
+                ("navCell", C.c_float), ("navInvCell", C.c_float), ("K", C.c_int), ("E", C.c_int),
+                ("navInfo", C.c_void_p), ("navSlot", C.c_void_p), ("navDist", C.c_void_p), ("invCapS", C.c_void_p),
+                ("secondsPerUnit", C.c_float), ("distQuantum", C.c_float), ("stair", C.c_void_p), ("exitRects", C.c_void_p)]
+
+
+class OracleParams(C.Structure):
+    _fields_ = [("relaxStep", C.c_float), ("tau", C.c_float), ("obstacleGain", C.c_float), ("projIters", C.c_int),
+                ("threads", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.oracle_create.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleParams), C.c_int]
+        L.oracle_create.restype = vp
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_set_state.argtypes = [vp] + [_f32p] * 7
+        L.oracle_set_drive.argtypes = [vp, vp, vp]
+        L.oracle_restore_state.argtypes = [vp] + [_f32p] * 5
+        L.oracle_choice_step.argtypes = [vp]
+        L.oracle_set_choice_state.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.oracle_step.argtypes = [vp, C.c_int]
+        L.oracle_get_state.argtypes = [vp] + [_f32p] * 6
+        L.oracle_get_grid.argtypes = [vp, _u32p, _u32p]
+        L.oracle_get_cells.argtypes = [vp, _u32p, _u32p]
+        L.oracle_get_assign.argtypes = [vp, _u32p]
+        L.oracle_get_choice_counters.argtypes = [vp, _u32p]
+        L.oracle_get_evac.argtypes = [vp, _i32p, _i32p]
+        L.oracle_num_steps.argtypes = [vp]
+        L.oracle_get_times.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
+        L.oracle_constants.argtypes = [_f32p, C.c_int]
+        _lib = L
+    return _lib
+
+
+def scene_struct(info, tables):
+    """An OracleScene over the scene's numpy tables (keep 'tables' alive while it is used)."""
+    s = OracleScene()
+    i = info
+    s.xmin, s.ymin, s.xmax, s.ymax = i.xmin, i.ymin, i.xmax, i.ymax
+    s.hashSpanX, s.hashSpanY, s.hashNX, s.hashNY = i.hash_span_x, i.hash_span_y, i.hash_nx, i.hash_ny
+    s.percRing, s.projRing = i.perc_ring, i.proj_ring
+    s.sdfNX, s.sdfNY, s.sdfOX, s.sdfOY, s.sdfInvH = i.sdf_nx, i.sdf_ny, i.sdf_ox, i.sdf_oy, i.sdf_inv_h
+    s.navRX, s.navRY, s.navOX, s.navOY = i.nav_res_x, i.nav_res_y, i.nav_origin_x, i.nav_origin_y
+    s.navCell, s.navInvCell, s.K, s.E = i.nav_cell, i.nav_inv_cell, i.K, i.E
+    s.secondsPerUnit, s.distQuantum = i.seconds_per_unit, i.dist_quantum
+    s.sdf = tables["sdf"].ctypes.data
+    s.navInfo = tables["nav_info"].ctypes.data
+    s.navSlot = tables["nav_slot"].ctypes.data
+    s.navDist = tables["nav_dist"].ctypes.data
+    s.invCapS = tables["inv_cap_s"].ctypes.data
+    s.stair = tables["stair"].ctypes.data
+    s.exitRects = tables["exit_rects"].ctypes.data
+    return s
+
+
+class Oracle:
+    def __init__(self, info, tables, n, relax_step, tau, obstacle_gain=20.0, proj_iters=32, threads=0):
+        self._tables = tables
+        sc = scene_struct(info, tables)
+        p = OracleParams(relax_step, tau, obstacle_gain, proj_iters, threads)
+        self.h = lib().oracle_create(C.byref(sc), C.byref(p), n)
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+        self.n = n
+        self.cells = info.hash_nx * info.hash_ny
+
+    def set_state(self, a):
+        f = lambda k: np.ascontiguousarray(a[k], np.float32)
+        lib().oracle_set_state(self.h, f("x"), f("y"), f("vx"), f("vy"), f("w"), f("r"), f("v0"))
+
+    def restore(self, a):
+        f = lambda k: np.ascontiguousarray(a[k], np.float32)
+        lib().oracle_restore_state(self.h, f("x"), f("y"), f("vx"), f("vy"), f("w"))
+
+    def choice_step(self):
+        lib().oracle_choice_step(self.h)
+
+    def set_choice_state(self, step, assign=None, queue=None, ahead=None):
+        keep = [None if a is None else np.ascontiguousarray(a, np.uint32) for a in (assign, queue, ahead)]
+        ptr = [None if a is None else a.ctypes.data for a in keep]
+        lib().oracle_set_choice_state(self.h, step, *ptr)
+
+    def set_drive(self, dx=None, dy=None):
+        if dx is None:
+            lib().oracle_set_drive(self.h, None, None)
+        else:
+            self._dx, self._dy = np.ascontiguousarray(dx, np.float32), np.ascontiguousarray(dy, np.float32)
+            lib().oracle_set_drive(self.h, self._dx.ctypes.data, self._dy.ctypes.data)
+
+    def step(self, k=1):
+        lib().oracle_step(self.h, k)
+
+    def state(self):
+        s = {k: np.zeros(self.n, np.float32) for k in ("x", "y", "vx", "vy", "speed", "w")}
+        lib().oracle_get_state(self.h, s["x"], s["y"], s["vx"], s["vy"], s["speed"], s["w"])
+        return s
+
+    def grid(self):
+        key, idx = np.zeros(self.n, np.uint32), np.zeros(self.n, np.uint32)
+        lib().oracle_get_grid(self.h, key, idx)
+        return key, idx
+
+    def cell_table(self):
+        st, cn = np.zeros(self.cells, np.uint32), np.zeros(self.cells, np.uint32)
+        lib().oracle_get_cells(self.h, st, cn)
+        return st, cn
+
+    def assign(self):
+        a = np.zeros(self.n, np.uint32)
+        lib().oracle_get_assign(self.h, a)
+        return a
+
+    def choice_counters(self):
+        o = np.zeros(132, np.uint32)
+        lib().oracle_get_choice_counters(self.h, o)
+        return o
+
+    def evac(self):
+        st, ex = np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+        lib().oracle_get_evac(self.h, st, ex)
+        return st, ex
+
+    def times(self):
+        t = np.zeros(8, np.float64)
+        lib().oracle_get_times(self.h, t)
+        return t
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def constants():
+    out = np.zeros(64, np.float32)
+    n = lib().oracle_constants(out, 64)
+    return out[:n]

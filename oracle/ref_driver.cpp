@@ -1,0 +1,319 @@
+// Golden-vector driver over the reference's own CPU code (src/cpu_crowd_sim.cpp, src/nav_field.cpp,
+// src/exit_choice.cpp, built from /root/reference by oracle/ref_build.sh into oracle/_ref/).
+// It runs the reference's CpuCrowdSim, nav::build / buildChoiceTable and nav::choiceStep on small known
+// cases and writes their inputs and outputs to a binary record file; tests/golden/make_golden.py packs
+// them into the committed fixtures. TEST INFRASTRUCTURE ONLY.
+//
+// Record format: u32 name length, name, u8 dtype ('f' float32, 'u' uint32, 'i' int32, 'b' uint8, 'd' float64),
+// u32 ndim, u64 dims[ndim], raw little-endian data.
+#include <swarm/cpu_crowd_sim.hpp>
+#include <swarm/exit_choice.hpp>
+#include <swarm/nav_field.hpp>
+#include <swarm/sdf.hpp>
+#include <swarm/venue.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace swarm;
+
+namespace {
+FILE* g_out = nullptr;
+
+template <class T> void put(const std::string& name, char dtype, const std::vector<T>& v, std::vector<uint64_t> dims = {}) {
+    if (dims.empty()) dims = { uint64_t(v.size()) };
+    const uint32_t len = uint32_t(name.size());
+    std::fwrite(&len, 4, 1, g_out);
+    std::fwrite(name.data(), 1, len, g_out);
+    std::fwrite(&dtype, 1, 1, g_out);
+    const uint32_t nd = uint32_t(dims.size());
+    std::fwrite(&nd, 4, 1, g_out);
+    std::fwrite(dims.data(), 8, nd, g_out);
+    std::fwrite(v.data(), sizeof(T), v.size(), g_out);
+}
+
+struct Rng {   // splitmix64
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uniform() { return float(double(next() >> 11) * (1.0 / 9007199254740992.0)); }
+};
+
+// A flat floor at z = 0 in the terrace's baked volume: the value is the height above it
+std::vector<float> flatVolume() {
+    std::vector<float> vol(SDF_VOXELS);
+    for (int x = 0; x < SDF_RES; x++)
+        for (int y = 0; y < SDF_RES; y++)
+            for (int z = 0; z < SDF_RES; z++) vol[(size_t(x) * SDF_RES + y) * SDF_RES + z] = SDF_ORIGIN.z + float(z - 1) * SDF_SPACING;
+    return vol;
+}
+
+// Terrace pitch scenario: n agents in a box on the fenced pitch, forced drive directions (counterflow), K steps
+void terraceCase(const std::string& tag, const std::vector<float>& vol, int n, float half, float minDist, int steps, uint64_t seed,
+                 int inactive, bool nanAgent, int outside) {
+    Rng rng{ seed };
+    std::vector<GpuBoid> b(static_cast<size_t>(n));
+    std::vector<glm::vec2> drive(static_cast<size_t>(n));
+    std::vector<float> px, py;
+    int placed = 0;
+    for (int i = 0; i < n; ++i) {
+        GpuBoid& a = b[size_t(i)];
+        a.velocity = glm::vec4(0.0f);
+        if (i < inactive) {   // evacuated agents
+            a.position = glm::vec4(PARK_COORD, PARK_COORD, PARK_COORD, 0.0f);
+            drive[size_t(i)] = glm::vec2(1.0f, 0.0f);
+            continue;
+        }
+        if (i < inactive + outside) {   // beyond the domain: parked by their first step
+            a.position = glm::vec4(1.2f + 0.01f * float(i), -1.3f, BODY_RADIUS, 1.0f);
+            drive[size_t(i)] = glm::vec2(0.0f, 1.0f);
+            continue;
+        }
+        if (nanAgent && i == n - 1) {   // a NaN position: the hash resets it to the origin
+            a.position = glm::vec4(std::nanf(""), 0.0f, 0.0f, 1.0f);
+            drive[size_t(i)] = glm::vec2(0.0f, 1.0f);
+            continue;
+        }
+        float x = 0, y = 0;
+        for (int tries = 0; tries < 10000; ++tries) {   // a box [0.05, 0.05 + 2 half] x [-half, half], away from the origin
+            x = 0.05f + 2.0f * half * rng.uniform();
+            y = -half + 2.0f * half * rng.uniform();
+            bool ok = true;
+            for (size_t k = 0; k < px.size() && ok; ++k) ok = (px[k] - x) * (px[k] - x) + (py[k] - y) * (py[k] - y) >= minDist * minDist;
+            if (ok) break;
+        }
+        px.push_back(x);
+        py.push_back(y);
+        ++placed;
+        a.position = glm::vec4(x, y, BODY_RADIUS, 1.0f);
+        const float ang = 6.2831853f * rng.uniform();
+        const float sp = 0.5f * rng.uniform();
+        a.velocity = glm::vec4(sp * std::cos(ang), sp * std::sin(ang), 0.0f, 0.0f);
+        // counterflow lanes with a slight random tilt
+        const float dir = (i % 2) ? 1.0f : -1.0f;
+        const float tilt = 0.3f * (rng.uniform() - 0.5f);
+        const glm::vec2 d = glm::normalize(glm::vec2(dir, tilt));
+        drive[size_t(i)] = d;
+    }
+    CpuCrowdSim sim(vol, 4, CpuCrowdSim::SortMode::Counting);
+    sim.setState(b);
+    sim.setDriveTable(&drive);
+    auto dumpState = [&](const std::vector<GpuBoid>& s, const std::string& p) {
+        std::vector<float> x(static_cast<size_t>(n)), y(static_cast<size_t>(n)), z(static_cast<size_t>(n)), w(static_cast<size_t>(n)), vx(static_cast<size_t>(n)), vy(static_cast<size_t>(n)), vz(static_cast<size_t>(n)), vw(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            x[size_t(i)] = s[size_t(i)].position.x; y[size_t(i)] = s[size_t(i)].position.y; z[size_t(i)] = s[size_t(i)].position.z;
+            w[size_t(i)] = s[size_t(i)].position.w; vx[size_t(i)] = s[size_t(i)].velocity.x; vy[size_t(i)] = s[size_t(i)].velocity.y;
+            vz[size_t(i)] = s[size_t(i)].velocity.z; vw[size_t(i)] = s[size_t(i)].velocity.w;
+        }
+        put(p + "x", 'f', x); put(p + "y", 'f', y); put(p + "z", 'f', z); put(p + "w", 'f', w);
+        put(p + "vx", 'f', vx); put(p + "vy", 'f', vy); put(p + "vz", 'f', vz); put(p + "speed", 'f', vw);
+    };
+    dumpState(b, tag + "/in/");
+    std::vector<float> dx(static_cast<size_t>(n)), dy(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) { dx[size_t(i)] = drive[size_t(i)].x; dy[size_t(i)] = drive[size_t(i)].y; }
+    put(tag + "/in/dx", 'f', dx);
+    put(tag + "/in/dy", 'f', dy);
+    for (int k = 1; k <= steps; ++k) {
+        sim.step();
+        const std::string p = tag + "/step" + std::to_string(k) + "/";
+        dumpState(sim.state(), p);
+        std::vector<uint32_t> key(static_cast<size_t>(n)), id(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) { key[size_t(i)] = sim.grid()[size_t(i)].gridID; id[size_t(i)] = sim.grid()[size_t(i)].boidID; }
+        put(p + "key", 'u', key);
+        put(p + "id", 'u', id);
+        std::vector<uint32_t> st(sim.cells().size()); std::vector<uint32_t> cn(sim.cells().size());
+        for (size_t c = 0; c < st.size(); ++c) { st[c] = sim.cells()[c].startIndex; cn[c] = sim.cells()[c].agentCount; }
+        put(p + "cell_start", 'u', st);
+        put(p + "cell_count", 'u', cn);
+    }
+}
+
+// nav_check::roomWalls (tools/nav_check.hpp:137-153), restated for the driver's worlds
+using nav::Rect;
+struct Gap { int side; float from, to; };
+std::vector<Rect> roomWalls(float x0, float y0, float x1, float y1, float t, const std::vector<Gap>& gaps) {
+    std::vector<Rect> out;
+    auto addSide = [&](int side, float a0, float a1, auto make) {
+        std::vector<std::pair<float, float>> cuts;
+        for (const Gap& g : gaps)
+            if (g.side == side) cuts.push_back({ g.from, g.to });
+        std::sort(cuts.begin(), cuts.end());
+        float a = a0;
+        for (const auto& c : cuts) { if (c.first > a) out.push_back(make(a, c.first)); a = c.second; }
+        if (a1 > a) out.push_back(make(a, a1));
+    };
+    addSide(0, x0 - t, x1 + t, [&](float a, float b) { return Rect{ { a, y0 - t }, { b, y0 } }; });
+    addSide(2, x0 - t, x1 + t, [&](float a, float b) { return Rect{ { a, y1 }, { b, y1 + t } }; });
+    addSide(3, y0, y1, [&](float a, float b) { return Rect{ { x0 - t, a }, { x0, b } }; });
+    addSide(1, y0, y1, [&](float a, float b) { return Rect{ { x1, a }, { x1 + t, b } }; });
+    return out;
+}
+nav::Grid testGrid(float x0, float span, double cell) {
+    nav::Grid g;
+    g.res = int(std::lround(span / cell));
+    g.origin = x0;
+    g.cell = float(cell);
+    g.invCell = float(1.0 / cell);
+    return g;
+}
+
+void putRects(const std::string& p, const std::vector<Rect>& rs) {
+    std::vector<float> v;
+    for (const Rect& r : rs) { v.push_back(r.lo.x); v.push_back(r.lo.y); v.push_back(r.hi.x); v.push_back(r.hi.y); }
+    put(p, 'f', v, { uint64_t(rs.size()), 4 });
+}
+
+// A rectangle world's grid (cls, out, exit region), per-exit fields and candidate table
+void navCase(const std::string& tag, const std::vector<Rect>& walls, const std::vector<Rect>& exits, const std::vector<float>& widths,
+             const nav::Grid& grid, float body) {
+    nav::RectGeometry geo(walls, exits, body);
+    if (!widths.empty()) geo.setExitWidths(widths);
+    const nav::Field f = nav::buildGrid(geo, grid);
+    putRects(tag + "/walls", walls);
+    putRects(tag + "/exits", exits);
+    put(tag + "/widths", 'f', widths);
+    put(tag + "/grid", 'd', std::vector<double>{ double(grid.res), double(grid.origin), double(grid.cell), double(grid.invCell), double(body) });
+    put(tag + "/cls", 'b', f.cls);
+    put(tag + "/out", 'b', f.out);
+    std::vector<int32_t> er(f.exitRegion.begin(), f.exitRegion.end());
+    put(tag + "/exit_region", 'i', er);
+    for (int e = 0; e < f.exitCount; ++e) {
+        const nav::ExitField ef = nav::buildExitField(f, e);
+        put(tag + "/field" + std::to_string(e) + "/dist", 'f', ef.dist);
+        put(tag + "/field" + std::to_string(e) + "/anchor", 'i', ef.anchor);
+    }
+    const nav::ChoiceTable t = nav::buildChoiceTable(geo, f);
+    put(tag + "/table/index", 'u', t.index);
+    std::vector<uint32_t> slots, dists;
+    for (const GpuNavChoiceCell& c : t.entries) {
+        for (int k = 0; k < nav::CHOICE_K; ++k) slots.push_back(c.slot[k]);
+        for (int k = 0; k < nav::CHOICE_DIST_WORDS; ++k) dists.push_back(c.dist[k]);
+    }
+    put(tag + "/table/slots", 'u', slots, { uint64_t(t.entries.size()), uint64_t(nav::CHOICE_K) });
+    put(tag + "/table/dists", 'u', dists, { uint64_t(t.entries.size()), uint64_t(nav::CHOICE_DIST_WORDS) });
+    put(tag + "/table/scalars", 'f', std::vector<float>{ t.secondsPerUnit, t.distQuantum });
+    put(tag + "/table/inv_cap_s", 'f', t.invCapS);
+}
+
+GpuBoid agentAt(float x, float y, bool active = true) {
+    GpuBoid b{};
+    b.position = glm::vec4(x, y, 0.0f, active ? 1.0f : 0.0f);
+    return b;
+}
+
+void putState(const std::string& p, const std::vector<GpuBoid>& st) {
+    std::vector<float> x, y, w;
+    for (const GpuBoid& b : st) { x.push_back(b.position.x); y.push_back(b.position.y); w.push_back(b.position.w); }
+    put(p + "x", 'f', x); put(p + "y", 'f', y); put(p + "w", 'f', w);
+}
+void putChoice(const std::string& p, const nav::ChoiceState& c) {
+    put(p + "assign", 'u', c.assign);
+    std::vector<uint32_t> cnt(c.counters.queue, c.counters.queue + 128);
+    cnt.push_back(c.counters.switches); cnt.push_back(c.counters.forced); cnt.push_back(c.counters.evaluations); cnt.push_back(c.counters.eligible);
+    put(p + "counters", 'u', cnt);
+}
+
+// The exit-choice known answers of test_gpu nav.exit_choice (tools/test_gpu.cpp:3968-4100) on the two-door room
+// (tools/choice_check.hpp:21-36), run through the reference's CPU pass nav::choiceStep
+void choiceCases() {
+    const std::vector<Rect> walls = roomWalls(0.0f, 0.0f, 20.0f, 10.0f, 0.3f, { { 3, 4.4f, 5.6f }, { 1, 4.4f, 5.6f } });
+    const std::vector<Rect> exits = { { { -1.0f, 4.4f }, { -0.3f, 5.6f } }, { { 20.3f, 4.4f }, { 21.0f, 5.6f } } };
+    const nav::Grid grid = testGrid(-1.0f, 22.0f, 0.1);
+    navCase("room2", walls, exits, { 1.2f, 1.2f }, grid, 0.0f);
+    nav::RectGeometry geo(walls, exits);
+    geo.setExitWidths({ 1.2f, 1.2f });
+    const nav::Field f = nav::buildGrid(geo, grid);
+    const nav::ChoiceTable rt = nav::buildChoiceTable(geo, f);
+    const int cs = choiceSteps(DT);
+    // (a) one door queued, mirrored, no queue: steps 0 and 1
+    for (int variant = 0; variant < 3; ++variant) {
+        const bool queueLeft = variant != 1, withQueue = variant != 2;
+        std::vector<GpuBoid> st;
+        for (int k = 0; k < 50 && withQueue; ++k) {
+            const float x = 0.55f + 0.1f * float(k % 25);
+            st.push_back(agentAt(queueLeft ? x : 20.0f - x, 4.65f + 0.2f * float(k / 25)));
+        }
+        const size_t first = st.size();
+        for (float y : { 4.75f, 4.95f, 5.05f, 5.25f })
+            for (float x : { 9.95f, 10.05f }) st.push_back(agentAt(x, y, false));
+        std::vector<GpuBoid> active = st;
+        for (size_t i = first; i < st.size(); ++i) active[i].position.w = 1.0f;
+        nav::ChoiceState c;
+        c.reset(st.size());
+        const std::string p = "choice_a" + std::to_string(variant) + "/";
+        putState(p + "s0/", st);
+        nav::choiceStep(rt, st, c, cs);
+        putChoice(p + "s0/", c);
+        putState(p + "s1/", active);
+        nav::choiceStep(rt, active, c, cs);
+        putChoice(p + "s1/", c);
+    }
+    // (b) hysteresis margins: one evaluation from a set state
+    for (int variant = 0; variant < 2; ++variant) {
+        const uint32_t qLeft = variant ? 120u : 100u, qRight = 100u;
+        const std::vector<GpuBoid> st = { agentAt(11.05f, 4.95f), agentAt(11.05f, 5.05f) };
+        nav::ChoiceState c;
+        c.reset(st.size());
+        c.step = uint32_t(cs);
+        c.assign = { 0u, 0u };
+        c.counters.queue[(c.step & 1u) * nav::CHOICE_QUEUE_STRIDE + 0] = qLeft;
+        c.counters.queue[(c.step & 1u) * nav::CHOICE_QUEUE_STRIDE + 1] = qRight;
+        const size_t par = size_t(c.step & 1u) * nav::CHOICE_QUEUE_STRIDE * nav::CHOICE_AHEAD_BINS;
+        c.ahead[par + 0 * nav::CHOICE_AHEAD_BINS] = qLeft;
+        c.ahead[par + 1 * nav::CHOICE_AHEAD_BINS] = qRight;
+        const std::string p = "choice_b" + std::to_string(variant) + "/";
+        putState(p, st);
+        put(p + "set", 'u', std::vector<uint32_t>{ c.step, qLeft, qRight });
+        nav::choiceStep(rt, st, c, cs);
+        putChoice(p, c);
+    }
+    // (ii) no oscillation: 190 agents on the centre line, all assigned left, 40 evaluations of the closed loop
+    {
+        std::vector<GpuBoid> st;
+        for (int k = 0; k < 190; ++k) st.push_back(agentAt(0.55f + 0.1f * float(k), 5.05f));
+        nav::ChoiceState c;
+        c.reset(st.size());
+        c.assign.assign(st.size(), 0u);
+        c.counters.queue[0] = uint32_t(st.size());
+        putState("choice_c/", st);
+        std::vector<uint32_t> hist;
+        for (int s = 0; s < 40 * cs + 1; ++s) {
+            nav::choiceStep(rt, st, c, cs);
+            if (s % cs == 0) hist.insert(hist.end(), c.assign.begin(), c.assign.end());
+        }
+        put("choice_c/assign_at_evals", 'u', hist, { uint64_t(hist.size() / st.size()), uint64_t(st.size()) });
+        putChoice("choice_c/final/", c);
+    }
+}
+} // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_driver <out.bin>\n"); return 2; }
+    g_out = std::fopen(argv[1], "wb");
+    if (!g_out) return 1;
+    const std::vector<float> vol = flatVolume();
+    // boxes x in [0.05, 0.05 + 2 half]: every agent stays farther than OBSTACLE_RANGE from the pitch fence
+    // (|x|, |y| < 0.30) and, in the first step, farther than R_PERC from the origin (the NaN agent's reset)
+    terraceCase("t_sparse", vol, 512, 0.10f, 0.011f, 6, 1337, 0, false, 0);
+    terraceCase("t_dense", vol, 1024, 0.085f, 0.0085f, 8, 4242, 37, true, 5);
+    choiceCases();
+    {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents
+        const std::vector<Rect> walls = { { { -0.3f, -0.3f }, { 20.3f, 0.0f } }, { { -0.3f, 0.0f }, { 0.0f, 2.3f } }, { { 0.0f, 2.0f }, { 18.0f, 2.3f } },
+                                          { { 17.7f, 2.3f }, { 18.0f, 15.0f } }, { { 20.0f, 0.0f }, { 20.3f, 15.0f } } };
+        navCase("lcorr", walls, { { { 17.0f, 15.025f }, { 21.0f, 16.0f } } }, {}, testGrid(-1.0f, 22.5f, 0.1), 0.0f);
+    }
+    {   // a room with a body radius and three exits (the table's candidate order by distance)
+        const std::vector<Rect> walls = roomWalls(0.0f, 0.0f, 12.0f, 9.0f, 0.3f, { { 2, 5.4f, 6.6f }, { 0, 1.0f, 2.0f }, { 1, 6.0f, 7.5f } });
+        const std::vector<Rect> exits = { { { 5.4f, 9.325f }, { 6.6f, 10.0f } }, { { 1.0f, -1.0f }, { 2.0f, -0.325f } }, { { 12.325f, 6.0f }, { 13.0f, 7.5f } } };
+        navCase("room3", walls, exits, { 1.2f, 1.0f, 1.5f }, testGrid(-1.0f, 14.0f, 0.1), 0.2f);
+    }
+    std::fclose(g_out);
+    return 0;
+}

@@ -1,0 +1,345 @@
+// Device-side model functions of the per-step pipeline (gfx950). Each follows the reference's
+// physics.comp / cpu_crowd_sim.cpp expression by expression on a flat 2D floor, in FP32 with no FMA
+// contraction (the library is built with -ffp-contract=off and correctly rounded divide and sqrt), so the
+// results are bit-identical to the reference's CPU arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "swarm_params.h"
+
+namespace swarm {
+
+// The read-only tables of one scene as the kernels see them
+struct SceneView {
+    float xmin, ymin, xmax, ymax;
+    float hashSpanX, hashSpanY;
+    int hashNX, hashNY;
+    int percRing, projRing;
+    int sdfNX, sdfNY;
+    float sdfOX, sdfOY, sdfInvH;
+    const float* sdf;
+    const float* sdfMin;
+    int sdfMinWin;
+    int navRX, navRY;
+    float navOX, navOY, navCell, navInvCell;
+    int K, E;
+    const uint32_t* navInfo;
+    const uint32_t* navSlot;
+    const uint16_t* navDist;
+    const float* invCapS;
+    float secondsPerUnit, distQuantum;
+    const float* stair;
+    const float* exitRects;   // E x 4
+};
+
+struct ModelParams {
+    float relaxStep;      // exp(-DT / tau)
+    float tau;            // relaxation time, model time
+    float obstacleGain;   // OBSTACLE_GAIN
+    int projIters;        // PROJ_ITERATIONS
+};
+
+struct V2 { float x, y; };
+__device__ inline V2 operator+(V2 a, V2 b) { return { a.x + b.x, a.y + b.y }; }
+__device__ inline V2 operator-(V2 a, V2 b) { return { a.x - b.x, a.y - b.y }; }
+__device__ inline V2 operator-(V2 a) { return { -a.x, -a.y }; }
+__device__ inline V2 operator*(V2 a, float s) { return { a.x * s, a.y * s }; }
+__device__ inline V2 operator*(float s, V2 a) { return { s * a.x, s * a.y }; }
+__device__ inline V2 operator/(V2 a, float s) { return { a.x / s, a.y / s }; }
+__device__ inline float dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+__device__ inline float len2(V2 a) { return sqrtf(dot2(a, a)); }
+__device__ inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ inline V2 mix2(V2 a, V2 b, float t) { return { mixf(a.x, b.x, t), mixf(a.y, b.y, t) }; }
+__device__ inline bool nonzero(V2 a) { return a.x != 0.0f || a.y != 0.0f; }
+__device__ inline V2 safeNormalize(V2 v) { const float l = len2(v); return l > SAFE_NORMALIZE_EPS ? v / l : V2{ 0.0f, 0.0f }; }
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+struct G3 { float gx, gy, w; };
+
+// calc_grid.comp cellID on a rectangular grid
+__device__ inline uint32_t hashCell(const SceneView& s, float px, float py) {
+    const int gx = clampi(int((px - s.xmin) / s.hashSpanX * float(s.hashNX)), 0, s.hashNX - 1);
+    const int gy = clampi(int((py - s.ymin) / s.hashSpanY * float(s.hashNY)), 0, s.hashNY - 1);
+    return uint32_t(gx + gy * s.hashNX);
+}
+
+// mapExactGrad (cpu_crowd_sim.cpp:239-262) on the 2D grid; map() is its value (exact weights, no filtering)
+__device__ inline G3 mapExactGrad(const SceneView& s, V2 p) {
+    const float fx = (p.x - s.sdfOX) * s.sdfInvH, fy = (p.y - s.sdfOY) * s.sdfInvH;
+    const bool outside = fx < -0.5f || fy < -0.5f || fx > float(s.sdfNX) - 0.5f || fy > float(s.sdfNY) - 0.5f;
+    const int ix = clampi(int(floorf(fx)), 0, s.sdfNX - 2), iy = clampi(int(floorf(fy)), 0, s.sdfNY - 2);
+    const float wx = fx - float(ix), wy = fy - float(iy);
+    const float* row0 = s.sdf + size_t(iy) * size_t(s.sdfNX);
+    const float* row1 = row0 + s.sdfNX;
+    const float c00 = row0[ix], c10 = row0[ix + 1], c01 = row1[ix], c11 = row1[ix + 1];
+    const float x0 = mixf(c00, c10, wx), x1 = mixf(c01, c11, wx);
+    if (outside) return G3{ 0.0f, 0.0f, 1.0f };
+    return G3{ mixf(c10 - c00, c11 - c01, wy) * s.sdfInvH, (x1 - x0) * s.sdfInvH, mixf(x0, x1, wy) };
+}
+__device__ inline float mapSdf(const SceneView& s, V2 p) { return mapExactGrad(s, p).w; }
+
+// True when no wall sample within PLANE_PROBE_RADII r_max of p can be <= 0 (the min-filtered grid is > 0 over
+// the window of p's bilinear cell), so the wall passes would find nothing: the exact results, without the samples
+__device__ inline bool wallFree(const SceneView& s, V2 p) {
+    const float fx = (p.x - s.sdfOX) * s.sdfInvH, fy = (p.y - s.sdfOY) * s.sdfInvH;
+    const float lo = float(s.sdfMinWin);
+    if (!(fx >= lo && fy >= lo && fx <= float(s.sdfNX - 2 - s.sdfMinWin) && fy <= float(s.sdfNY - 2 - s.sdfMinWin))) return false;
+    const int ix = int(floorf(fx)), iy = int(floorf(fy));
+    return s.sdfMin[size_t(iy) * size_t(s.sdfNX) + size_t(ix)] > 0.0f;
+}
+
+// calcGradient's xy rows (cpu_crowd_sim.cpp:297-303)
+__device__ inline V2 calcGradient(const SceneView& s, V2 p) {
+    const float H = OBSTACLE_GRAD_H;
+    const float m1 = mapSdf(s, V2{ p.x + H, p.y + -H }), m2 = mapSdf(s, V2{ p.x + -H, p.y + -H });
+    const float m3 = mapSdf(s, V2{ p.x + -H, p.y + H }), m4 = mapSdf(s, V2{ p.x + H, p.y + H });
+    const V2 g{ ((m1 + -m2) + -m3) + m4, ((-m1 + -m2) + m3) + m4 };
+    return g * OBSTACLE_GRAD_SCALE;
+}
+
+__device__ inline V2 pairDirection(uint32_t self, uint32_t other) {   // cpu_crowd_sim.cpp:123-129
+    const uint32_t lo = self < other ? self : other, hi = self < other ? other : self;
+    const float angle = float(hash32(hash32(lo) ^ hi)) * HASH_TO_RADIANS;
+    float sn, cs;
+    sincos_cephes(angle, &sn, &cs);
+    const V2 dir{ cs, sn };
+    return self == lo ? dir : -dir;
+}
+
+// squared separation support / contact distance: the reference's folded constants at its distances
+__device__ inline float sepSupport2(float S) { return S == R_SEP ? R_SEP2 : S * S; }
+__device__ inline float contact2(float D) { return D == BODY_DIAM ? BODY_DIAM2 : D * D; }
+
+__device__ inline V2 separationForce(V2 offset, uint32_t self, uint32_t other, float S) {   // :131-139
+    const float d2 = dot2(offset, offset);
+    if (d2 >= sepSupport2(S)) return V2{ 0.0f, 0.0f };
+    const float w = 1.0f - sqrtf(d2) / S;
+    const float magnitude = K_SEP * w * w;
+    const float dxy2 = dot2(offset, offset);
+    if (dxy2 <= SEP_STACKED_XY2) return magnitude * pairDirection(self, other);
+    return magnitude * offset * (1.0f / sqrtf(dxy2));
+}
+
+__device__ inline V2 contactCorrection(V2 offset, uint32_t self, uint32_t other, float D) {   // :141-148
+    const float dxy2 = dot2(offset, offset);
+    if (dxy2 >= contact2(D)) return V2{ 0.0f, 0.0f };
+    const float dxy = sqrtf(dxy2);
+    const float shift = 0.5f * (D - dxy);
+    if (dxy2 <= SEP_STACKED_XY2) return shift * pairDirection(self, other);
+    return shift * (offset / dxy);
+}
+
+// ---- navigation / exit choice (exit_choice.cpp:512-603) ----
+__device__ inline int choiceCell(const SceneView& s, V2 p) {
+    const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;
+    const int cx = clampi(int(floorf(gx)), 0, s.navRX - 1), cy = clampi(int(floorf(gy)), 0, s.navRY - 1);
+    int i = cx + cy * s.navRX;
+    if (!(s.navInfo[i] & NAV_FIT)) {
+        float best = 1e9f;
+        for (int r = 1; r <= 2 && best == 1e9f; r++)
+            for (int dy = -r; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++) {
+                    if ((dx < 0 ? -dx : dx) != r && (dy < 0 ? -dy : dy) != r) continue;
+                    const int mx = cx + dx, my = cy + dy;
+                    if (mx < 0 || my < 0 || mx >= s.navRX || my >= s.navRY) continue;
+                    const int j = mx + my * s.navRX;
+                    const uint32_t info = s.navInfo[j];
+                    if (!(info & NAV_FIT) || !(info & NAV_HAS)) continue;
+                    const float qx = gx - (float(mx) + 0.5f), qy = gy - (float(my) + 0.5f);
+                    const float d2 = qx * qx + qy * qy;
+                    if (d2 < best) { best = d2; i = j; }
+                }
+    }
+    return i;
+}
+__device__ inline uint32_t slotExitOf(uint32_t w) { return (w >> CHOICE_EXIT_SHIFT) & 63u; }
+__device__ inline float slotDistOf(const SceneView& s, uint32_t q) {
+    return q == CHOICE_DIST_EMPTY ? __builtin_inff() : float(q) * s.distQuantum;
+}
+__device__ inline V2 anchorCentre(const SceneView& s, uint32_t a) {
+    const uint32_t res = uint32_t(s.navRX);
+    const float cx = float(a % res) + 0.5f, cy = float(a / res) + 0.5f;
+    const float px = cx * s.navCell, py = cy * s.navCell;
+    return V2{ s.navOX + px, s.navOY + py };
+}
+__device__ inline V2 choiceDirection(const SceneView& s, int cell, V2 p, uint32_t exit) {
+    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+    for (int k = 0; k < s.K; ++k) {
+        const uint32_t w = sl[k];
+        if (slotExitOf(w) != exit) continue;
+        const V2 d = anchorCentre(s, w & CHOICE_ANCHOR_MASK) - p;
+        const float l = len2(d);
+        return l > GOAL_DEADZONE ? d / l : V2{ 0.0f, 0.0f };
+    }
+    return V2{ 0.0f, 0.0f };
+}
+__device__ inline float estimatedTime(const SceneView& s, float dist, uint32_t exit, uint32_t queued) {
+    const float walk = dist * s.secondsPerUnit;
+    const float wait = float(queued) * s.invCapS[exit];
+    return walk + wait;
+}
+__device__ inline uint32_t aheadBin(uint32_t q) {
+    const uint32_t b = q >> CHOICE_AHEAD_SHIFT;
+    return b < uint32_t(CHOICE_AHEAD_BINS - 1) ? b : uint32_t(CHOICE_AHEAD_BINS - 1);
+}
+__device__ inline int slotOfExit(const SceneView& s, int cell, uint32_t exit) {
+    int k = -1;
+    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+    for (int i = 0; i < s.K; ++i)
+        if (slotExitOf(sl[i]) == exit) k = i;
+    return k;
+}
+__device__ inline int bestCandidate(const SceneView& s, int cell, const uint32_t* aRead, float& eBest) {
+    int best = -1;
+    eBest = 0.0f;
+    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+    const uint16_t* dq = s.navDist + size_t(cell) * size_t(s.K);
+    for (int k = 0; k < s.K; ++k) {
+        const uint32_t ex = slotExitOf(sl[k]);
+        if (ex == CHOICE_EMPTY) break;
+        const uint32_t q = dq[k];
+        const float e = estimatedTime(s, slotDistOf(s, q), ex, aRead[size_t(ex) * CHOICE_AHEAD_BINS + aheadBin(q)]);
+        if (best < 0 || e < eBest) { best = k; eBest = e; }
+    }
+    return best;
+}
+
+__device__ inline int exitOf(const SceneView& s, V2 p) {
+    for (int e = 0; e < s.E; ++e) {
+        const float* r = s.exitRects + 4 * e;
+        if (p.x >= r[0] && p.x <= r[2] && p.y >= r[1] && p.y <= r[3]) return e;
+    }
+    return -1;
+}
+__device__ inline bool parkTest(const SceneView& s, V2 p) {
+    return exitOf(s, p) >= 0 || p.x < s.xmin || p.x > s.xmax || p.y < s.ymin || p.y > s.ymax;
+}
+__device__ inline float stairFactor(const SceneView& s, V2 p) {
+    const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;
+    const int cx = clampi(int(floorf(gx)), 0, s.navRX - 1), cy = clampi(int(floorf(gy)), 0, s.navRY - 1);
+    return s.stair[size_t(cx) + size_t(cy) * size_t(s.navRX)];
+}
+
+// ---- static wall contacts (cpu_crowd_sim.cpp:173-500, 2D) ----
+__device__ inline float rootStep(float br[4], float t, G3 g, V2 u) {
+    if (g.w > 0.0f) { br[0] = t; br[1] = g.w; } else { br[2] = t; br[3] = g.w; }
+    const float du = g.gx * u.x + g.gy * u.y;
+    const float tn = du < 0.0f ? t - g.w / du : -1.0f;
+    return tn > br[0] && tn < br[2] ? tn : (br[0] * br[3] - br[2] * br[1]) / (br[3] - br[1]);
+}
+__device__ inline float newtonStep(float t, G3 g, V2 u) {
+    const float du = g.gx * u.x + g.gy * u.y;
+    return du < 0.0f ? t - g.w / du : t;
+}
+__device__ inline float wallContact(const SceneView& s, V2 np, float fc, V2 u, float fr, float reach, float recastTol, V2& away) {
+    float br[4] = { 0.0f, fc, reach, fr };
+    float t = reach * fc / (fc - fr);
+    G3 g{ 0.0f, 0.0f, 0.0f };
+    for (int i = 0; i < WALL_ROOT_STEPS; ++i) {
+        g = mapExactGrad(s, np + u * t);
+        t = rootStep(br, t, g, u);
+    }
+    float e = t;
+    away = -u;
+    const V2 nn = safeNormalize(V2{ g.gx, g.gy });
+    const V2 u2 = -nn;
+    float t2 = 0.0f, t3 = t * dot2(u, u2);
+    G3 g2{ 0.0f, 0.0f, 0.0f };
+    for (int i = 0; i < WALL_RECAST_STEPS; ++i) {
+        t2 = t3;
+        g2 = mapExactGrad(s, np + u2 * t2);
+        t3 = newtonStep(t2, g2, u2);
+    }
+    const bool ok = (g2.gx * u2.x + g2.gy * u2.y) < 0.0f && fabsf(t3 - t2) < recastTol && t3 > 0.0f;
+    if (ok && t3 < e) { e = t3; away = nn; }
+    return e;
+}
+__device__ inline bool clearWalls(const SceneView& s, V2& np, V2& nv, float rad, float wall[3]) {
+    bool moved = false;
+    wall[0] = wall[1] = wall[2] = 0.0f;
+    const float reach = FEELER_REACH_RADII * rad;
+    const float recastTol = WALL_RECAST_TOL_RADII * rad;
+    for (int pass = 0; pass < WALL_PASSES; ++pass) {
+        const float fc = mapSdf(s, np);
+        float fr1 = 0.0f, fr2 = 0.0f;
+        int i1 = -1, i2 = -1;
+        for (int i = 0; i < FEELER_COUNT; ++i) {
+            const V2 f = np + V2{ FEELER_DIRS[i].x * reach, FEELER_DIRS[i].y * reach };
+            const float v = mapSdf(s, f);
+            if (v < fr1) { fr2 = fr1; i2 = i1; fr1 = v; i1 = i; }
+            else if (v < fr2) { fr2 = v; i2 = i; }
+        }
+        if (i1 < 0) break;
+        V2 hNorm;
+        float clearance;
+        if (fc > 0.0f) {
+            clearance = wallContact(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr1, reach, recastTol, hNorm);
+            if (i2 >= 0) {
+                V2 h2;
+                const float c2 = wallContact(s, np, fc, V2{ FEELER_DIRS[i2].x, FEELER_DIRS[i2].y }, fr2, reach, recastTol, h2);
+                if (c2 < clearance) { clearance = c2; hNorm = h2; }
+            }
+        } else {
+            const G3 g = mapExactGrad(s, np);
+            hNorm = safeNormalize(V2{ g.gx, g.gy });
+            const float gl = len2(V2{ g.gx, g.gy });
+            clearance = g.w / (gl > CENTRE_GRAD_MIN ? gl : CENTRE_GRAD_MIN);
+        }
+        if (clearance >= rad) {
+            if (clearance < NO_CONTACT) { wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = clearance - rad; }
+            break;
+        }
+        np = np + hNorm * (rad - clearance);
+        moved = true;
+        wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = 0.0f;
+        const float vDotN = dot2(nv, hNorm);
+        if (vDotN < 0.0f) nv = nv - hNorm * vDotN;
+    }
+    return moved;
+}
+__device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, float wall[3]) {
+    wall[0] = wall[1] = wall[2] = 0.0f;
+    const float reach = PLANE_PROBE_RADII * rad;
+    const float fc = mapSdf(s, np);
+    float fr = 0.0f;
+    int i1 = -1;
+    for (int i = 0; i < FEELER_COUNT; ++i) {
+        const V2 f = np + V2{ FEELER_DIRS[i].x * reach, FEELER_DIRS[i].y * reach };
+        const float v = mapSdf(s, f);
+        if (v < fr) { fr = v; i1 = i; }
+    }
+    if (i1 < 0 || fc <= 0.0f) return;
+    V2 away;
+    const float c = wallContact(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr, reach, WALL_RECAST_TOL_RADII * rad, away);
+    if (c < NO_CONTACT) { wall[0] = away.x; wall[1] = away.y; const float m = c - rad; wall[2] = m > 0.0f ? m : 0.0f; }
+}
+__device__ inline bool solidAt(const SceneView& s, V2 p) { return mapExactGrad(s, p).w <= 0.0f; }
+__device__ inline bool sweepBack(const SceneView& s, V2 pOld, V2& np, V2& nv) {
+    if (!solidAt(s, np)) return false;
+    float lo = 0.0f, hi = 1.0f;
+    for (int i = 0; i < STATIC_SWEEP_BISECTIONS; ++i) {
+        const float t = 0.5f * (lo + hi);
+        if (solidAt(s, mix2(pOld, np, t))) hi = t;
+        else lo = t;
+    }
+    const V2 d = safeNormalize(np - pOld);
+    np = mix2(pOld, np, lo);
+    const float vd = dot2(nv, d);
+    if (vd > 0.0f) nv = nv - d * vd;
+    return true;
+}
+// resolveStatic on a flat floor: grounded unless the centre lies in a wall
+__device__ inline bool resolveStatic(const SceneView& s, V2 pOld, V2& np, V2& nv, float rad, float wall[3]) {
+    if (wallFree(s, np)) { wall[0] = wall[1] = wall[2] = 0.0f; return true; }
+    bool grounded = !solidAt(s, np);
+    if (!grounded && sweepBack(s, pOld, np, nv)) grounded = !solidAt(s, np);
+    const bool walls = clearWalls(s, np, nv, rad, wall);
+    if (walls && grounded) {
+        float w2[3];
+        clearWalls(s, np, nv, rad, w2);
+        if (w2[0] != 0.0f || w2[1] != 0.0f) { wall[0] = w2[0]; wall[1] = w2[1]; wall[2] = w2[2]; }
+    }
+    return grounded;
+}
+
+} // namespace swarm

@@ -1,0 +1,299 @@
+// HipCrowdSim: device buffers, upload/readback and the step's launch sequence (CrowdSim::step,
+// src/crowd_sim.cpp:316-412: hash, sort, cells + gather, exit choice, predict, PROJ_ITERATIONS - 1 projection
+// iterations, final; then the ping-pong halves swap).
+#include "hip_crowd_sim.hpp"
+
+#include <charconv>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace swarm {
+
+namespace {
+// a float's shortest round-trip decimal as a double (how the reference folds constants, sim_params.hpp:36-50)
+double decimalOf(float f) {
+    char buf[64];
+    auto res = std::to_chars(buf, buf + sizeof buf, f);
+    *res.ptr = '\0';
+    return std::strtod(buf, nullptr);
+}
+} // namespace
+
+void modelParamsFor(float tauS, float& tau, float& relaxStep) {
+    tau = float(decimalOf(tauS) / decimalOf(TIME_SCALE_S_PER_UNIT));
+    relaxStep = float(std::exp(-decimalOf(DT) / decimalOf(tau)));
+}
+
+void DeviceScene::upload(const Scene& s) {
+    auto put = [&](const void* src, size_t bytes) -> void* {
+        void* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 4));
+        owned.push_back(p);
+        if (bytes) SWARM_HIP_CHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        return p;
+    };
+    SceneView& v = view;
+    v.xmin = s.xmin; v.ymin = s.ymin; v.xmax = s.xmax; v.ymax = s.ymax;
+    v.hashSpanX = s.hashSpanX; v.hashSpanY = s.hashSpanY; v.hashNX = s.hashNX; v.hashNY = s.hashNY;
+    v.percRing = s.percRing; v.projRing = s.projRing;
+    v.sdfNX = s.sdfNX; v.sdfNY = s.sdfNY; v.sdfOX = s.sdfOX; v.sdfOY = s.sdfOY; v.sdfInvH = s.sdfInvH;
+    v.sdf = static_cast<const float*>(put(s.sdf.data(), s.sdf.size() * 4));
+    v.sdfMin = static_cast<const float*>(put(s.sdfMin.data(), s.sdfMin.size() * 4));
+    v.sdfMinWin = s.sdfMinWin;
+    v.navRX = s.nav.resX; v.navRY = s.nav.resY; v.navOX = s.nav.originX; v.navOY = s.nav.originY;
+    v.navCell = s.nav.cell; v.navInvCell = s.nav.invCell; v.K = s.K; v.E = s.E;
+    v.navInfo = static_cast<const uint32_t*>(put(s.navInfo.data(), s.navInfo.size() * 4));
+    v.navSlot = static_cast<const uint32_t*>(put(s.navSlot.data(), s.navSlot.size() * 4));
+    v.navDist = static_cast<const uint16_t*>(put(s.navDist.data(), s.navDist.size() * 2));
+    v.invCapS = static_cast<const float*>(put(s.invCapS.data(), s.invCapS.size() * 4));
+    v.secondsPerUnit = s.secondsPerUnit; v.distQuantum = s.distQuantum;
+    v.stair = static_cast<const float*>(put(s.stair.data(), s.stair.size() * 4));
+    std::vector<float> ex;
+    for (const Rect& r : s.exits) { ex.push_back(r.lox); ex.push_back(r.loy); ex.push_back(r.hix); ex.push_back(r.hiy); }
+    v.exitRects = static_cast<const float*>(put(ex.data(), ex.size() * 4));
+}
+
+void DeviceScene::release() {
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+}
+
+template <class T> T* HipCrowdSim::alloc(size_t count) {
+    void* p = nullptr;
+    SWARM_HIP_CHECK(hipMalloc(&p, (count > 0 ? count : 1) * sizeof(T)));
+    bufs_.push_back(p);
+    SWARM_HIP_CHECK(hipMemsetAsync(p, 0, (count > 0 ? count : 1) * sizeof(T), stream_));
+    return static_cast<T*>(p);
+}
+
+HipCrowdSim::HipCrowdSim(const Scene& scene, int numAgents, const SimOptions& opt) : opt_(opt) {
+    if (numAgents <= 0) throw std::invalid_argument("HipCrowdSim: agent count must be positive");
+    if (opt.projIters < 0) throw std::invalid_argument("HipCrowdSim: projection iterations must be >= 0");
+    if (scene.hashNX <= 0 || scene.hashNY <= 0 || scene.sdfNX < 2 || scene.sdfNY < 2 || scene.nav.resX <= 0 || scene.nav.resY <= 0)
+        throw std::invalid_argument("HipCrowdSim: invalid scene");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("HipCrowdSim: no HIP device");
+    n_ = numAgents;
+    C_ = scene.hashNX * scene.hashNY;
+    nb_ = C_ + (n_ + HASH_BLOCK - 1) / HASH_BLOCK;
+    modelParamsFor(opt.tauS, mp_.tau, mp_.relaxStep);
+    mp_.obstacleGain = opt.obstacleGain;
+    mp_.projIters = opt.projIters;
+    SWARM_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    scene_.upload(scene);
+    const size_t n = size_t(n_);
+    for (int h = 0; h < 2; ++h) {
+        half_[h].x = alloc<float>(n); half_[h].y = alloc<float>(n); half_[h].vx = alloc<float>(n);
+        half_[h].vy = alloc<float>(n); half_[h].spd = alloc<float>(n); half_[h].w = alloc<float>(n);
+    }
+    rad_ = alloc<float>(n); v0_ = alloc<float>(n);
+    driveX_ = alloc<float>(n); driveY_ = alloc<float>(n);
+    grid_.key = alloc<uint32_t>(n); grid_.loc = alloc<uint32_t>(n);
+    grid_.bcnt = alloc<uint32_t>(size_t(nb_)); grid_.bstart = alloc<uint32_t>(size_t(nb_) + 1);
+    grid_.tileSum = alloc<uint32_t>(size_t((nb_ + SCAN_TILE - 1) / SCAN_TILE)); grid_.tmp = alloc<uint32_t>(n);
+    sb_.sid = alloc<uint32_t>(n); sb_.skey = alloc<uint32_t>(n);
+    sb_.cellStart = alloc<uint32_t>(size_t(C_)); sb_.cellCount = alloc<uint32_t>(size_t(C_));
+    sb_.sx = alloc<float>(n); sb_.sy = alloc<float>(n); sb_.sr = alloc<float>(n);
+    sb_.egx = alloc<float>(n); sb_.egy = alloc<float>(n);
+    sb_.pw = alloc<float>(n); sb_.pvx = alloc<float>(n); sb_.pvy = alloc<float>(n);
+    sb_.planes = alloc<float4>(n);
+    sb_.v0 = v0_;
+    pa_.x = alloc<float>(n); pa_.y = alloc<float>(n); pa_.acc = alloc<float>(n);
+    pb_.x = alloc<float>(n); pb_.y = alloc<float>(n); pb_.acc = alloc<float>(n);
+    ch_.assign = alloc<uint32_t>(n); ch_.pending = alloc<uint32_t>(n);
+    ch_.queue = alloc<uint32_t>(128); ch_.counters = alloc<uint32_t>(4);
+    ch_.hist = alloc<uint32_t>(size_t(CHOICE_MAX_EXITS + 1) * CHOICE_GAIN_BINS);
+    ch_.threshold = alloc<uint32_t>(64);
+    ch_.ahead = alloc<uint32_t>(size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS);
+    ev_.step = alloc<int32_t>(n); ev_.exit = alloc<int32_t>(n); ev_.perExit = alloc<uint32_t>(64);
+    ctrl_ = alloc<uint32_t>(CTRL_WORDS);
+    const uint32_t sentinel = uint32_t(n_);
+    SWARM_HIP_CHECK(hipMemcpyAsync(grid_.bstart + nb_, &sentinel, 4, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+HipCrowdSim::~HipCrowdSim() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    dropGraph();
+    for (void* p : bufs_) (void)hipFree(p);
+    scene_.release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void HipCrowdSim::dropGraph() {
+    if (graphExec_) { (void)hipGraphExecDestroy(graphExec_); graphExec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+}
+
+void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r,
+                         const float* v0) {
+    const size_t b = size_t(n_) * 4;
+    restoreAgentState(x, y, vx, vy, w);
+    SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r, b, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(v0_, v0, b, hipMemcpyHostToDevice, stream_));
+    // a fresh exit choice (nav::ChoiceState::reset) and evacuation record
+    std::vector<uint32_t> none(static_cast<size_t>(n_), CHOICE_NONE);
+    std::vector<int32_t> minus(static_cast<size_t>(n_), -1);
+    std::vector<uint32_t> thr(64, uint32_t(CHOICE_GAIN_BINS));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ch_.assign, none.data(), b, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.pending, 0, b, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.queue, 0, 128 * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.counters, 0, 4 * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.hist, 0, size_t(CHOICE_MAX_EXITS + 1) * CHOICE_GAIN_BINS * 4, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ch_.threshold, thr.data(), 64 * 4, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.ahead, 0, size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS * 4, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ev_.step, minus.data(), b, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ev_.exit, minus.data(), b, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ev_.perExit, 0, 64 * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ctrl_, 0, CTRL_WORDS * 4, stream_));
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void HipCrowdSim::restoreAgentState(const float* x, const float* y, const float* vx, const float* vy, const float* w) {
+    const size_t b = size_t(n_) * 4;
+    std::vector<float> spd(static_cast<size_t>(n_));
+    for (size_t i = 0; i < size_t(n_); ++i) spd[i] = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i]);
+    for (int h = 0; h < 2; ++h) {
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].x, x, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].y, y, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].vx, vx, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].vy, vy, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].w, w, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].spd, spd.data(), b, hipMemcpyHostToDevice, stream_));
+    }
+    parity_ = 0;
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void HipCrowdSim::setDrive(const float* dx, const float* dy) {
+    const bool on = dx && dy;
+    if (on) {
+        SWARM_HIP_CHECK(hipMemcpyAsync(driveX_, dx, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(driveY_, dy, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    if (on != drive_) dropGraph();
+    drive_ = on;
+}
+
+void HipCrowdSim::setProjectionIterations(int iterations) {
+    if (iterations < 0) throw std::invalid_argument("setProjectionIterations: iterations must be >= 0");
+    if (iterations != mp_.projIters) dropGraph();
+    mp_.projIters = iterations;
+}
+
+void HipCrowdSim::enqueueStep(int parity) {
+    const AgentHalf rd = half_[parity], wr = half_[1 - parity];
+    StepBuffers b = sb_;
+    b.avx = rd.vx;
+    b.avy = rd.vy;
+    launchGridPasses(stream_, n_, scene_.view, rd, grid_, b, rad_, ctrl_);
+    launchChoice(stream_, n_, scene_.view, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    launchPhysics(stream_, n_, scene_.view, mp_, b, pa_, pb_, rd, wr, ev_, ctrl_);
+}
+
+void HipCrowdSim::buildGraph() {
+    SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueueStep(0);
+    enqueueStep(1);
+    SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+    SWARM_HIP_CHECK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+}
+
+void HipCrowdSim::step(int nsteps) {
+    if (nsteps < 0) throw std::invalid_argument("step: negative step count");
+    int left = nsteps;
+    if (opt_.useGraph && left >= 2) {
+        if (parity_ == 1) { enqueueStep(1); parity_ = 0; --left; }
+        if (!graphExec_) buildGraph();
+        while (left >= 2) {
+            SWARM_HIP_CHECK(hipGraphLaunch(graphExec_, stream_));
+            left -= 2;
+        }
+    }
+    while (left > 0) {
+        enqueueStep(parity_);
+        parity_ = 1 - parity_;
+        --left;
+    }
+    SWARM_HIP_CHECK(hipGetLastError());
+}
+
+void HipCrowdSim::stepExitChoice() {
+    const AgentHalf rd = half_[parity_];
+    StepBuffers b = sb_;
+    b.avx = rd.vx;
+    b.avy = rd.vy;
+    launchGridPasses(stream_, n_, scene_.view, rd, grid_, b, rad_, ctrl_);
+    launchChoice(stream_, n_, scene_.view, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    launchStepIncrement(stream_, ctrl_);
+    SWARM_HIP_CHECK(hipGetLastError());
+}
+
+void HipCrowdSim::setChoiceState(uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead) {
+    SWARM_HIP_CHECK(hipMemcpyAsync(ctrl_ + CTRL_STEP, &step, 4, hipMemcpyHostToDevice, stream_));
+    if (assign) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.assign, assign, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
+    if (queue128) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.queue, queue128, 128 * 4, hipMemcpyHostToDevice, stream_));
+    if (ahead)
+        SWARM_HIP_CHECK(hipMemcpyAsync(ch_.ahead, ahead, size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS * 4, hipMemcpyHostToDevice, stream_));
+    sync();
+}
+
+void HipCrowdSim::sync() { SWARM_HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+void HipCrowdSim::readState(float* x, float* y, float* vx, float* vy, float* speed, float* w) {
+    const AgentHalf h = half_[parity_];
+    const size_t b = size_t(n_) * 4;
+    if (x) SWARM_HIP_CHECK(hipMemcpyAsync(x, h.x, b, hipMemcpyDeviceToHost, stream_));
+    if (y) SWARM_HIP_CHECK(hipMemcpyAsync(y, h.y, b, hipMemcpyDeviceToHost, stream_));
+    if (vx) SWARM_HIP_CHECK(hipMemcpyAsync(vx, h.vx, b, hipMemcpyDeviceToHost, stream_));
+    if (vy) SWARM_HIP_CHECK(hipMemcpyAsync(vy, h.vy, b, hipMemcpyDeviceToHost, stream_));
+    if (speed) SWARM_HIP_CHECK(hipMemcpyAsync(speed, h.spd, b, hipMemcpyDeviceToHost, stream_));
+    if (w) SWARM_HIP_CHECK(hipMemcpyAsync(w, h.w, b, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readGrid(uint32_t* key, uint32_t* id) {
+    const size_t b = size_t(n_) * 4;
+    if (key) SWARM_HIP_CHECK(hipMemcpyAsync(key, sb_.skey, b, hipMemcpyDeviceToHost, stream_));
+    if (id) SWARM_HIP_CHECK(hipMemcpyAsync(id, sb_.sid, b, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readCells(uint32_t* start, uint32_t* count) {
+    const size_t b = size_t(C_) * 4;
+    if (start) SWARM_HIP_CHECK(hipMemcpyAsync(start, sb_.cellStart, b, hipMemcpyDeviceToHost, stream_));
+    if (count) SWARM_HIP_CHECK(hipMemcpyAsync(count, sb_.cellCount, b, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readAssign(uint32_t* assign) {
+    SWARM_HIP_CHECK(hipMemcpyAsync(assign, ch_.assign, size_t(n_) * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readChoiceCounters(uint32_t* out) {
+    SWARM_HIP_CHECK(hipMemcpyAsync(out, ch_.queue, 128 * 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(out + 128, ch_.counters, 4 * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit) {
+    const size_t b = size_t(n_) * 4;
+    if (step) SWARM_HIP_CHECK(hipMemcpyAsync(step, ev_.step, b, hipMemcpyDeviceToHost, stream_));
+    if (exitId) SWARM_HIP_CHECK(hipMemcpyAsync(exitId, ev_.exit, b, hipMemcpyDeviceToHost, stream_));
+    if (perExit) SWARM_HIP_CHECK(hipMemcpyAsync(perExit, ev_.perExit, size_t(scene_.view.E + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum) {
+    uint32_t c[CTRL_WORDS];
+    SWARM_HIP_CHECK(hipMemcpyAsync(c, ctrl_, sizeof c, hipMemcpyDeviceToHost, stream_));
+    sync();
+    if (step) *step = c[CTRL_STEP];
+    if (nActive) *nActive = c[CTRL_NACTIVE];
+    if (activeSum) *activeSum = uint64_t(c[CTRL_ACTIVE_SUM_LO]) | (uint64_t(c[CTRL_ACTIVE_SUM_LO + 1]) << 32);
+}
+
+} // namespace swarm

@@ -1,0 +1,117 @@
+// Scene preparation for the per-step pipeline: the input tables the step reads, built once on the
+// host. Rectangle worlds (flat floor, axis-aligned wall boxes, exit rectangles, stair bands) are
+// turned into
+//   - a 2D FP32 signed-distance grid of the walls (the step's wall contacts and obstacle avoidance),
+//   - the navigation grid and one any-angle distance field per exit (Lazy Theta* over legal moves,
+//     following src/nav_field.cpp:184-366 of the reference), merged into a per-cell candidate table
+//     (following src/exit_choice.cpp:143-317),
+//   - a stair speed-factor field on the navigation grid,
+// plus seeded agent placements for the benchmark floors.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace swarm {
+
+// Closed axis-aligned rectangle (RectGeometry's Rect, include/swarm/nav_field.hpp:135-138)
+struct Rect {
+    float lox, loy, hix, hiy;
+    bool contains(float x, float y) const { return x >= lox && x <= hix && y >= loy && y <= hiy; }
+};
+
+// A navigation grid: res cells per axis from origin, cell edge 'cell', invCell = 1 / cell rounded once
+// (nav::Grid, nav_field.hpp:74-84, generalised to a rectangle)
+struct NavGrid {
+    int resX = 0, resY = 0;
+    float originX = 0.0f, originY = 0.0f;
+    float cell = 0.0f, invCell = 0.0f;
+    size_t cells() const { return size_t(resX) * size_t(resY); }
+    int index(int ix, int iy) const { return ix + iy * resX; }
+    void centre(int ix, int iy, float& x, float& y) const;   // origin + (i + 0.5) cell, in double, rounded once
+};
+
+struct SceneDesc {
+    // domain: agents outside it are parked
+    float xmin = 0, ymin = 0, xmax = 0, ymax = 0;
+    float hashCell = 0.016f;     // spatial-hash cell (model units)
+    float sdfSpacing = 0.0032f;  // SDF texel pitch; texel centres at xmin + (i + 0.5) pitch
+    float sdfClamp = 0.05f;      // SDF values are clamped to this (well beyond every reach the step uses)
+    NavGrid nav;                 // navigation grid (cell, origin, res)
+    float navBody = 0.008f;      // a FIT cell's centre clears every wall by this
+    double metresPerUnit = 31.25;
+    std::vector<Rect> walls, exits, stairs;
+    std::vector<float> stairFactor;   // per stair rect
+    std::vector<float> exitWidthM;    // per exit, metres (0: the exit rect's longer side)
+};
+
+// The tables the step reads (uploaded to the GPU as they are)
+struct Scene {
+    // domain and spatial hash (calc_grid.comp: cell = clamp(int((p - min) / span * res)))
+    float xmin, ymin, xmax, ymax;
+    float hashSpanX, hashSpanY;
+    int hashNX, hashNY;
+    int percRing, projRing;
+    // signed-distance grid, row-major [sdfNY][sdfNX]; texel (i, j) at (sdfOX + i h, sdfOY + j h)
+    int sdfNX, sdfNY;
+    float sdfOX, sdfOY, sdfH, sdfInvH;
+    std::vector<float> sdf;
+    std::vector<float> sdfMin;   // per texel: min over [i - win, i + win + 1]^2 (a conservative 'no wall near' test)
+    int sdfMinWin;
+    // navigation / exit-choice table, dense per nav cell
+    NavGrid nav;
+    int K = 0, E = 0;                      // candidate slots per cell, exits
+    std::vector<uint32_t> navInfo;         // NAV_FIT | NAV_HAS
+    std::vector<uint32_t> navSlot;         // [cell * K + k]: anchor | exit << 25 (exit 63: empty)
+    std::vector<uint16_t> navDist;         // [cell * K + k]: distance quanta (0xFFFF: empty)
+    std::vector<float> invCapS;            // per exit: 60 / (w_e J)
+    float secondsPerUnit = 0.0f, distQuantum = 0.0f;
+    std::vector<float> stair;              // per nav cell: free-speed factor
+    std::vector<Rect> exits;
+    float maxRadius = 0.0f, maxV0 = 0.0f;  // used for the rings and sdfMin window
+    double buildSeconds = 0.0;
+};
+
+// Build every table of a scene. maxRadius / maxV0: the largest agent radius and free speed the scene
+// will carry (they size the projection ring and the SDF-min window).
+Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads = 0);
+
+// --- the navigation builder, exposed for known-answer tests ---
+enum NavClass : uint8_t { NAV_CLASS_FIT = 0, NAV_CLASS_NOFIT = 1 };
+struct NavField {
+    NavGrid grid;
+    std::vector<uint8_t> cls, out;
+    std::vector<int8_t> exitRegion;
+    int exitCount = 0;
+};
+NavField buildNavGrid(const SceneDesc& d, int threads = 0);
+// One exit's field: dist (grid units; +inf: no path) and anchor (-1: none); exitFilter -1: every exit
+void lazyThetaStar(const NavField& f, int exitFilter, std::vector<double>& g, std::vector<int32_t>& par,
+                   std::vector<uint8_t>& closed);
+
+// --- agent placements of the benchmark floors (model units) ---
+struct Agents {
+    std::vector<float> x, y, r, v0, w;   // w: 1 active, 0 parked (x, y = PARK_COORD)
+    size_t size() const { return x.size(); }
+};
+
+// splitmix64: a portable, seeded generator (doubles in [0, 1) from the top 53 bits)
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next();
+    double uniform();                 // [0, 1)
+    double normal();                  // Box-Muller, cached pair not kept (one draw each call)
+};
+
+// Layout ids of the benchmark floors
+enum Layout { LAYOUT_ROOM = 0, LAYOUT_BOWL = 1, LAYOUT_CORRIDOR = 2, LAYOUT_BOWL_EIGHTH = 3, LAYOUT_PILLAR = 4, LAYOUT_BOWL_QUARTER = 5 };
+SceneDesc layoutDesc(int layout);
+// Seat / lattice slots of a layout (agent centres in model units) in their fixed order
+void layoutSlots(int layout, std::vector<float>& sx, std::vector<float>& sy);
+// Agents on a layout: 'active' of its slots chosen (room-like layouts: seeded random choice of lattice
+// sites with jitter; bowls: the first 'active' seats in slot order, jittered), padded with parked agents to
+// 'total'. Radii U[0.20, 0.25] m, free speed N(1.34, 0.26) m/s clipped to [0.5, 2.0].
+Agents placeAgents(int layout, size_t active, size_t total, uint64_t seed, double speedMean = 1.34, double speedSd = 0.26);
+
+} // namespace swarm

@@ -1,0 +1,199 @@
+// extern "C" layer over HipCrowdSim and the scene builder (include/swarm_hip.h). No exception crosses it:
+// std::invalid_argument -> SWARM_EINVAL, a device allocation failure -> SWARM_ENOMEM, other errors -> SWARM_EHIP.
+#include "../../include/swarm_hip.h"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "hip_crowd_sim.hpp"
+#include "scene.hpp"
+
+struct swarm_scene { swarm::Scene s; };
+struct swarm_ctx { swarm::HipCrowdSim* sim; };
+
+namespace {
+thread_local std::string g_err;
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return SWARM_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return SWARM_EINVAL;
+    } catch (const std::bad_alloc& e) {
+        g_err = e.what();
+        return SWARM_ENOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return g_err.find("out of memory") != std::string::npos ? SWARM_ENOMEM : SWARM_EHIP;
+    }
+}
+
+std::vector<swarm::Rect> rects(const float* p, int n) {
+    std::vector<swarm::Rect> r;
+    for (int i = 0; i < n; ++i) r.push_back(swarm::Rect{ p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3] });
+    return r;
+}
+} // namespace
+
+extern "C" {
+
+const char* swarm_last_error(void) { return g_err.c_str(); }
+
+int swarm_scene_build(const swarm_scene_desc* d, float max_radius, float max_v0, int threads, swarm_scene** out) {
+    if (!d || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        swarm::SceneDesc sd;
+        sd.xmin = d->xmin; sd.ymin = d->ymin; sd.xmax = d->xmax; sd.ymax = d->ymax;
+        sd.hashCell = d->hash_cell; sd.sdfSpacing = d->sdf_spacing; sd.sdfClamp = d->sdf_clamp;
+        sd.nav.resX = d->nav_res_x; sd.nav.resY = d->nav_res_y; sd.nav.originX = d->nav_origin_x; sd.nav.originY = d->nav_origin_y;
+        sd.nav.cell = d->nav_cell; sd.nav.invCell = d->nav_inv_cell; sd.navBody = d->nav_body;
+        sd.metresPerUnit = d->metres_per_unit;
+        sd.walls = rects(d->walls, d->n_walls);
+        sd.exits = rects(d->exits, d->n_exits);
+        sd.stairs = rects(d->stairs, d->n_stairs);
+        for (int i = 0; i < d->n_exits; ++i) sd.exitWidthM.push_back(d->exit_width_m ? d->exit_width_m[i] : 0.0f);
+        for (int i = 0; i < d->n_stairs; ++i) sd.stairFactor.push_back(d->stair_factor[i]);
+        if (sd.nav.resX <= 0 || sd.nav.resY <= 0 || !(sd.nav.cell > 0.0f) || !(sd.sdfSpacing > 0.0f))
+            throw std::invalid_argument("swarm_scene_build: empty grid");
+        auto* s = new swarm_scene{ swarm::buildScene(sd, max_radius, max_v0, threads) };
+        *out = s;
+    });
+}
+
+int swarm_scene_build_layout(int layout, float max_radius, float max_v0, int threads, swarm_scene** out) {
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { *out = new swarm_scene{ swarm::buildScene(swarm::layoutDesc(layout), max_radius, max_v0, threads) }; });
+}
+
+int swarm_scene_info_get(const swarm_scene* sc, swarm_scene_info* i) {
+    if (!sc || !i) { g_err = "null argument"; return SWARM_EINVAL; }
+    const swarm::Scene& s = sc->s;
+    i->xmin = s.xmin; i->ymin = s.ymin; i->xmax = s.xmax; i->ymax = s.ymax;
+    i->hash_span_x = s.hashSpanX; i->hash_span_y = s.hashSpanY; i->hash_nx = s.hashNX; i->hash_ny = s.hashNY;
+    i->perc_ring = s.percRing; i->proj_ring = s.projRing;
+    i->sdf_nx = s.sdfNX; i->sdf_ny = s.sdfNY; i->sdf_ox = s.sdfOX; i->sdf_oy = s.sdfOY; i->sdf_h = s.sdfH; i->sdf_inv_h = s.sdfInvH;
+    i->sdf_min_win = s.sdfMinWin;
+    i->nav_res_x = s.nav.resX; i->nav_res_y = s.nav.resY; i->nav_origin_x = s.nav.originX; i->nav_origin_y = s.nav.originY;
+    i->nav_cell = s.nav.cell; i->nav_inv_cell = s.nav.invCell;
+    i->K = s.K; i->E = s.E; i->seconds_per_unit = s.secondsPerUnit; i->dist_quantum = s.distQuantum;
+    i->build_seconds = s.buildSeconds;
+    return SWARM_OK;
+}
+
+int swarm_scene_read(const swarm_scene* sc, float* sdf, float* sdf_min, uint32_t* nav_info, uint32_t* nav_slot, uint16_t* nav_dist,
+                     float* inv_cap_s, float* stair, float* exit_rects) {
+    if (!sc) { g_err = "null argument"; return SWARM_EINVAL; }
+    const swarm::Scene& s = sc->s;
+    if (sdf) std::memcpy(sdf, s.sdf.data(), s.sdf.size() * 4);
+    if (sdf_min) std::memcpy(sdf_min, s.sdfMin.data(), s.sdfMin.size() * 4);
+    if (nav_info) std::memcpy(nav_info, s.navInfo.data(), s.navInfo.size() * 4);
+    if (nav_slot) std::memcpy(nav_slot, s.navSlot.data(), s.navSlot.size() * 4);
+    if (nav_dist) std::memcpy(nav_dist, s.navDist.data(), s.navDist.size() * 2);
+    if (inv_cap_s) std::memcpy(inv_cap_s, s.invCapS.data(), s.invCapS.size() * 4);
+    if (stair) std::memcpy(stair, s.stair.data(), s.stair.size() * 4);
+    if (exit_rects)
+        for (size_t e = 0; e < s.exits.size(); ++e) {
+            exit_rects[4 * e] = s.exits[e].lox; exit_rects[4 * e + 1] = s.exits[e].loy;
+            exit_rects[4 * e + 2] = s.exits[e].hix; exit_rects[4 * e + 3] = s.exits[e].hiy;
+        }
+    return SWARM_OK;
+}
+
+void swarm_scene_destroy(swarm_scene* s) { delete s; }
+
+int swarm_place_agents(int layout, size_t active, size_t total, uint64_t seed, double mean, double sd, float* x, float* y,
+                       float* r, float* v0, float* w) {
+    if (!x || !y || !r || !v0 || !w) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::Agents a = swarm::placeAgents(layout, active, total, seed, mean, sd);
+        std::memcpy(x, a.x.data(), total * 4); std::memcpy(y, a.y.data(), total * 4);
+        std::memcpy(r, a.r.data(), total * 4); std::memcpy(v0, a.v0.data(), total * 4);
+        std::memcpy(w, a.w.data(), total * 4);
+    });
+}
+
+int swarm_layout_slots(int layout, float* x, float* y, size_t cap, size_t* count) {
+    return guarded([&] {
+        std::vector<float> sx, sy;
+        swarm::layoutSlots(layout, sx, sy);
+        if (count) *count = sx.size();
+        for (size_t i = 0; i < sx.size() && i < cap; ++i) {
+            if (x) x[i] = float(double(sx[i]) / swarm::M_PER_UNIT);
+            if (y) y[i] = float(double(sy[i]) / swarm::M_PER_UNIT);
+        }
+    });
+}
+
+int swarm_model_params(float tau_s, float* tau, float* relax) {
+    if (!tau || !relax || !(tau_s > 0.0f)) { g_err = "invalid relaxation time"; return SWARM_EINVAL; }
+    swarm::modelParamsFor(tau_s, *tau, *relax);
+    return SWARM_OK;
+}
+
+int swarm_create(const swarm_scene* scene, int n, const swarm_config* cfg, swarm_ctx** out) {
+    if (!scene || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        swarm::SimOptions o;
+        if (cfg) {
+            o.tauS = cfg->tau_s; o.obstacleGain = cfg->obstacle_gain; o.projIters = cfg->proj_iters; o.useGraph = cfg->use_graph != 0;
+            if (!(o.tauS > 0.0f)) throw std::invalid_argument("swarm_create: relaxation time must be positive");
+        }
+        *out = new swarm_ctx{ new swarm::HipCrowdSim(scene->s, n, o) };
+    });
+}
+
+#define CTX_CHECK(c) if (!(c) || !(c)->sim) { g_err = "null context"; return SWARM_EINVAL; }
+
+int swarm_upload(swarm_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r,
+                 const float* v0) {
+    CTX_CHECK(c);
+    if (!x || !y || !vx || !vy || !w || !r || !v0) { g_err = "null state array"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->upload(x, y, vx, vy, w, r, v0); });
+}
+int swarm_restore_state(swarm_ctx* c, const float* x, const float* y, const float* vx, const float* vy, const float* w) {
+    CTX_CHECK(c);
+    if (!x || !y || !vx || !vy || !w) { g_err = "null state array"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->restoreAgentState(x, y, vx, vy, w); });
+}
+int swarm_set_drive(swarm_ctx* c, const float* dx, const float* dy) { CTX_CHECK(c); return guarded([&] { c->sim->setDrive(dx, dy); }); }
+int swarm_step(swarm_ctx* c, int nsteps) { CTX_CHECK(c); return guarded([&] { c->sim->step(nsteps); }); }
+int swarm_step_exit_choice(swarm_ctx* c) { CTX_CHECK(c); return guarded([&] { c->sim->stepExitChoice(); }); }
+int swarm_set_choice_state(swarm_ctx* c, uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setChoiceState(step, assign, queue128, ahead); });
+}
+int swarm_sync(swarm_ctx* c) { CTX_CHECK(c); return guarded([&] { c->sim->sync(); }); }
+int swarm_set_proj_iters(swarm_ctx* c, int it) { CTX_CHECK(c); return guarded([&] { c->sim->setProjectionIterations(it); }); }
+int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readState(x, y, vx, vy, speed, w); });
+}
+int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id) { CTX_CHECK(c); return guarded([&] { c->sim->readGrid(key, id); }); }
+int swarm_read_cells(swarm_ctx* c, uint32_t* start, uint32_t* count) { CTX_CHECK(c); return guarded([&] { c->sim->readCells(start, count); }); }
+int swarm_read_assign(swarm_ctx* c, uint32_t* a) { CTX_CHECK(c); return guarded([&] { c->sim->readAssign(a); }); }
+int swarm_read_choice_counters(swarm_ctx* c, uint32_t* o) { CTX_CHECK(c); return guarded([&] { c->sim->readChoiceCounters(o); }); }
+int swarm_read_evac(swarm_ctx* c, int32_t* step, int32_t* exit_id, uint32_t* per_exit) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readEvac(step, exit_id, per_exit); });
+}
+int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readControl(step, n_active, active_sum); });
+}
+int swarm_get_stream(swarm_ctx* c, void** stream) {
+    CTX_CHECK(c);
+    if (!stream) { g_err = "null argument"; return SWARM_EINVAL; }
+    *stream = c->sim->stream();
+    return SWARM_OK;
+}
+int swarm_num_cells(swarm_ctx* c) { CTX_CHECK(c); return c->sim->numCells(); }
+void swarm_destroy(swarm_ctx* c) {
+    if (!c) return;
+    delete c->sim;
+    delete c;
+}
+
+} // extern "C"

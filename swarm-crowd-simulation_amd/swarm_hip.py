@@ -1,0 +1,280 @@
+"""Python binding (ctypes) of libswarm_hip.so, the MI355X per-step crowd pipeline.
+
+The C ABI is include/swarm_hip.h; this module only marshals numpy arrays through it. It fails loudly
+when the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libswarm_hip.so")
+
+LAYOUT_ROOM, LAYOUT_BOWL, LAYOUT_CORRIDOR, LAYOUT_BOWL_EIGHTH, LAYOUT_PILLAR, LAYOUT_BOWL_QUARTER = range(6)
+M_PER_UNIT = 31.25
+S_PER_UNIT = 17.490671
+INACTIVE_CELL = 0xFFFFFFFF
+CHOICE_NONE = 0xFFFFFFFF
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("xmin", C.c_float), ("ymin", C.c_float), ("xmax", C.c_float), ("ymax", C.c_float),
+                ("hash_cell", C.c_float), ("sdf_spacing", C.c_float), ("sdf_clamp", C.c_float),
+                ("nav_res_x", C.c_int), ("nav_res_y", C.c_int),
+                ("nav_origin_x", C.c_float), ("nav_origin_y", C.c_float), ("nav_cell", C.c_float), ("nav_inv_cell", C.c_float),
+                ("nav_body", C.c_float), ("metres_per_unit", C.c_double),
+                ("n_walls", C.c_int), ("n_exits", C.c_int), ("n_stairs", C.c_int),
+                ("walls", C.POINTER(C.c_float)), ("exits", C.POINTER(C.c_float)), ("exit_width_m", C.POINTER(C.c_float)),
+                ("stairs", C.POINTER(C.c_float)), ("stair_factor", C.POINTER(C.c_float))]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("xmin", C.c_float), ("ymin", C.c_float), ("xmax", C.c_float), ("ymax", C.c_float),
+                ("hash_span_x", C.c_float), ("hash_span_y", C.c_float),
+                ("hash_nx", C.c_int), ("hash_ny", C.c_int), ("perc_ring", C.c_int), ("proj_ring", C.c_int),
+                ("sdf_nx", C.c_int), ("sdf_ny", C.c_int),
+                ("sdf_ox", C.c_float), ("sdf_oy", C.c_float), ("sdf_h", C.c_float), ("sdf_inv_h", C.c_float),
+                ("sdf_min_win", C.c_int), ("nav_res_x", C.c_int), ("nav_res_y", C.c_int),
+                ("nav_origin_x", C.c_float), ("nav_origin_y", C.c_float), ("nav_cell", C.c_float), ("nav_inv_cell", C.c_float),
+                ("K", C.c_int), ("E", C.c_int), ("seconds_per_unit", C.c_float), ("dist_quantum", C.c_float),
+                ("build_seconds", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("tau_s", C.c_float), ("obstacle_gain", C.c_float), ("proj_iters", C.c_int), ("use_graph", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.swarm_last_error.restype = C.c_char_p
+    L.swarm_scene_build.argtypes = [C.POINTER(SceneDesc), C.c_float, C.c_float, C.c_int, C.POINTER(vp)]
+    L.swarm_scene_build_layout.argtypes = [C.c_int, C.c_float, C.c_float, C.c_int, C.POINTER(vp)]
+    L.swarm_scene_info_get.argtypes = [vp, C.POINTER(SceneInfo)]
+    L.swarm_scene_read.argtypes = [vp, _f32p, _f32p, _u32p, _u32p, _u16p, _f32p, _f32p, _f32p]
+    L.swarm_scene_destroy.argtypes = [vp]
+    L.swarm_scene_destroy.restype = None
+    L.swarm_place_agents.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_uint64, C.c_double, C.c_double,
+                                     _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.swarm_layout_slots.argtypes = [C.c_int, vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.swarm_model_params.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.swarm_create.argtypes = [vp, C.c_int, C.POINTER(Config), C.POINTER(vp)]
+    L.swarm_upload.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.swarm_restore_state.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.swarm_set_drive.argtypes = [vp, vp, vp]
+    L.swarm_step.argtypes = [vp, C.c_int]
+    L.swarm_step_exit_choice.argtypes = [vp]
+    L.swarm_set_choice_state.argtypes = [vp, C.c_uint32, vp, vp, vp]
+    L.swarm_sync.argtypes = [vp]
+    L.swarm_set_proj_iters.argtypes = [vp, C.c_int]
+    L.swarm_read_state.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.swarm_read_grid.argtypes = [vp, _u32p, _u32p]
+    L.swarm_read_cells.argtypes = [vp, _u32p, _u32p]
+    L.swarm_read_assign.argtypes = [vp, _u32p]
+    L.swarm_read_choice_counters.argtypes = [vp, _u32p]
+    L.swarm_read_evac.argtypes = [vp, _i32p, _i32p, _u32p]
+    L.swarm_read_control.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+    L.swarm_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.swarm_num_cells.argtypes = [vp]
+    L.swarm_destroy.argtypes = [vp]
+    L.swarm_destroy.restype = None
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"swarm error {rc}: {lib().swarm_last_error().decode()}")
+
+
+def model_params(tau_s=0.5):
+    tau, relax = C.c_float(), C.c_float()
+    check(lib().swarm_model_params(tau_s, C.byref(tau), C.byref(relax)))
+    return tau.value, relax.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Scene:
+    """Built scene tables (SDF grid, navigation/candidate table, stair field, exits)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        inf = SceneInfo()
+        check(lib().swarm_scene_info_get(self.h, C.byref(inf)))
+        self.info = inf
+
+    @classmethod
+    def layout(cls, layout, max_radius=0.25 / M_PER_UNIT, max_v0=2.0 * S_PER_UNIT / M_PER_UNIT, threads=0):
+        h = C.c_void_p()
+        check(lib().swarm_scene_build_layout(layout, max_radius, max_v0, threads, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rects(cls, domain, hash_cell, sdf_spacing, sdf_clamp, nav, nav_body, walls, exits, exit_width_m=None,
+              stairs=None, stair_factor=None, metres_per_unit=M_PER_UNIT, max_radius=0.0064, max_v0=0.75, threads=0):
+        """A rectangle world. nav = (res_x, res_y, origin_x, origin_y, cell, inv_cell)."""
+        walls = np.ascontiguousarray(np.asarray(walls, np.float32).reshape(-1, 4))
+        exits = np.ascontiguousarray(np.asarray(exits, np.float32).reshape(-1, 4))
+        ew = np.ascontiguousarray(np.zeros(len(exits), np.float32) if exit_width_m is None else np.asarray(exit_width_m, np.float32))
+        st = np.ascontiguousarray(np.zeros((0, 4), np.float32) if stairs is None else np.asarray(stairs, np.float32).reshape(-1, 4))
+        sf = np.ascontiguousarray(np.zeros(len(st), np.float32) if stair_factor is None else np.asarray(stair_factor, np.float32))
+        d = SceneDesc()
+        d.xmin, d.ymin, d.xmax, d.ymax = domain
+        d.hash_cell, d.sdf_spacing, d.sdf_clamp = hash_cell, sdf_spacing, sdf_clamp
+        d.nav_res_x, d.nav_res_y, d.nav_origin_x, d.nav_origin_y, d.nav_cell, d.nav_inv_cell = nav
+        d.nav_body, d.metres_per_unit = nav_body, metres_per_unit
+        d.n_walls, d.n_exits, d.n_stairs = len(walls), len(exits), len(st)
+        d.walls, d.exits, d.exit_width_m, d.stairs, d.stair_factor = _fp(walls), _fp(exits), _fp(ew), _fp(st), _fp(sf)
+        h = C.c_void_p()
+        check(lib().swarm_scene_build(C.byref(d), max_radius, max_v0, threads, C.byref(h)))
+        return cls(h)
+
+    def tables(self):
+        """Copies of every table, as numpy arrays (the oracle's inputs)."""
+        i = self.info
+        cells = i.nav_res_x * i.nav_res_y
+        t = dict(sdf=np.zeros(i.sdf_nx * i.sdf_ny, np.float32), sdf_min=np.zeros(i.sdf_nx * i.sdf_ny, np.float32),
+                 nav_info=np.zeros(cells, np.uint32), nav_slot=np.zeros(cells * i.K, np.uint32),
+                 nav_dist=np.zeros(cells * i.K, np.uint16), inv_cap_s=np.zeros(max(i.E, 1), np.float32),
+                 stair=np.zeros(cells, np.float32), exit_rects=np.zeros(max(4 * i.E, 4), np.float32))
+        check(lib().swarm_scene_read(self.h, t["sdf"], t["sdf_min"], t["nav_info"], t["nav_slot"], t["nav_dist"],
+                                     t["inv_cap_s"], t["stair"], t["exit_rects"]))
+        return t
+
+    def close(self):
+        if self.h:
+            lib().swarm_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def place_agents(layout, active, total, seed, speed_mean=1.34, speed_sd=0.26):
+    a = {k: np.zeros(total, np.float32) for k in ("x", "y", "r", "v0", "w")}
+    check(lib().swarm_place_agents(layout, active, total, seed, speed_mean, speed_sd, a["x"], a["y"], a["r"], a["v0"], a["w"]))
+    a["vx"] = np.zeros(total, np.float32)
+    a["vy"] = np.zeros(total, np.float32)
+    return a
+
+
+def layout_capacity(layout):
+    n = C.c_size_t()
+    check(lib().swarm_layout_slots(layout, None, None, 0, C.byref(n)))
+    return n.value
+
+
+class Sim:
+    """The GPU step of one scene (the reference's CrowdSim surface)."""
+
+    def __init__(self, scene, n, tau_s=0.5, obstacle_gain=20.0, proj_iters=32, use_graph=True):
+        cfg = Config(tau_s, obstacle_gain, proj_iters, 1 if use_graph else 0)
+        h = C.c_void_p()
+        check(lib().swarm_create(scene.h, n, C.byref(cfg), C.byref(h)))
+        self.h, self.n, self.scene = h, n, scene
+        self.cells = lib().swarm_num_cells(self.h)
+
+    def upload(self, a):
+        f = lambda k: np.ascontiguousarray(a[k], np.float32)
+        check(lib().swarm_upload(self.h, f("x"), f("y"), f("vx"), f("vy"), f("w"), f("r"), f("v0")))
+
+    def restore(self, a):
+        f = lambda k: np.ascontiguousarray(a[k], np.float32)
+        check(lib().swarm_restore_state(self.h, f("x"), f("y"), f("vx"), f("vy"), f("w")))
+
+    def set_drive(self, dx=None, dy=None):
+        if dx is None:
+            check(lib().swarm_set_drive(self.h, None, None))
+        else:
+            self._dx, self._dy = np.ascontiguousarray(dx, np.float32), np.ascontiguousarray(dy, np.float32)
+            check(lib().swarm_set_drive(self.h, self._dx.ctypes.data, self._dy.ctypes.data))
+
+    def step(self, k=1):
+        check(lib().swarm_step(self.h, k))
+
+    def sync(self):
+        check(lib().swarm_sync(self.h))
+
+    def choice_step(self):
+        check(lib().swarm_step_exit_choice(self.h))
+
+    def set_choice_state(self, step, assign=None, queue=None, ahead=None):
+        keep = [None if a is None else np.ascontiguousarray(a, np.uint32) for a in (assign, queue, ahead)]
+        ptr = [None if a is None else a.ctypes.data for a in keep]
+        check(lib().swarm_set_choice_state(self.h, step, *ptr))
+
+    def set_proj_iters(self, it):
+        check(lib().swarm_set_proj_iters(self.h, it))
+
+    def state(self):
+        s = {k: np.zeros(self.n, np.float32) for k in ("x", "y", "vx", "vy", "speed", "w")}
+        check(lib().swarm_read_state(self.h, s["x"], s["y"], s["vx"], s["vy"], s["speed"], s["w"]))
+        return s
+
+    def grid(self):
+        key, idx = np.zeros(self.n, np.uint32), np.zeros(self.n, np.uint32)
+        check(lib().swarm_read_grid(self.h, key, idx))
+        return key, idx
+
+    def cell_table(self):
+        st, cn = np.zeros(self.cells, np.uint32), np.zeros(self.cells, np.uint32)
+        check(lib().swarm_read_cells(self.h, st, cn))
+        return st, cn
+
+    def assign(self):
+        a = np.zeros(self.n, np.uint32)
+        check(lib().swarm_read_assign(self.h, a))
+        return a
+
+    def choice_counters(self):
+        o = np.zeros(132, np.uint32)
+        check(lib().swarm_read_choice_counters(self.h, o))
+        return o
+
+    def evac(self):
+        st, ex = np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+        per = np.zeros(64, np.uint32)
+        check(lib().swarm_read_evac(self.h, st, ex, per))
+        return st, ex, per[: self.scene.info.E + 1]
+
+    def control(self):
+        s, a, t = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().swarm_read_control(self.h, C.byref(s), C.byref(a), C.byref(t)))
+        return s.value, a.value, t.value
+
+    def stream(self):
+        p = C.c_void_p()
+        check(lib().swarm_get_stream(self.h, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            lib().swarm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

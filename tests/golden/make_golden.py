@@ -1,0 +1,71 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own CPU code.
+
+Runs oracle/_ref/ref_driver (built from /root/reference by oracle/ref_build.sh; this container only) and
+packs its records into compressed .npz files:
+  terrace_kat.npz  CpuCrowdSim steps on the terrace pitch with forced drive directions (state, sorted grid,
+                   cell table after every step)
+  nav_kat.npz      nav::buildGrid / buildExitField / buildChoiceTable on rectangle worlds
+  choice_kat.npz   nav::choiceStep on the two-door room (test_gpu nav.exit_choice's known answers)
+
+Usage: python tests/golden/make_golden.py
+"""
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+DTYPES = {b"f": np.float32, b"u": np.uint32, b"i": np.int32, b"b": np.uint8, b"d": np.float64}
+
+
+def read_records(path):
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        (n,) = struct.unpack_from("<I", data, pos); pos += 4
+        name = data[pos:pos + n].decode(); pos += n
+        dt = DTYPES[data[pos:pos + 1]]; pos += 1
+        (nd,) = struct.unpack_from("<I", data, pos); pos += 4
+        dims = struct.unpack_from("<%dQ" % nd, data, pos); pos += 8 * nd
+        cnt = int(np.prod(dims)) if nd else 0
+        arr = np.frombuffer(data, dtype=dt, count=cnt, offset=pos).reshape(dims).copy()
+        pos += cnt * np.dtype(dt).itemsize
+        out[name] = arr
+    return out
+
+
+def main():
+    drv = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+    subprocess.run([os.path.join(REPO, "oracle", "ref_build.sh")], check=True)
+    with tempfile.TemporaryDirectory() as td:
+        rec = os.path.join(td, "golden.bin")
+        subprocess.run([drv, rec], check=True)
+        r = read_records(rec)
+    terrace = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("t_")}
+    choice = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("choice_")}
+    nav = {}
+    for k, v in r.items():
+        if not (k.startswith("room2/") or k.startswith("lcorr/") or k.startswith("room3/")):
+            continue
+        if k.endswith("/table/slots"):   # keep the non-empty candidate slots only (at most the exit count)
+            ne = int(r[k.replace("slots", "inv_cap_s")].size)
+            v = v[:, :ne]
+        if k.endswith("/table/dists"):
+            ne = int(r[k.replace("dists", "inv_cap_s")].size)
+            v = v[:, : (ne + 1) // 2]
+        nav[k.replace("/", "__")] = v
+    np.savez_compressed(os.path.join(HERE, "terrace_kat.npz"), **terrace)
+    np.savez_compressed(os.path.join(HERE, "nav_kat.npz"), **nav)
+    np.savez_compressed(os.path.join(HERE, "choice_kat.npz"), **choice)
+    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,48 @@
+"""The C ABI library: it loads, exports every function include/*.h declares, and rejects bad arguments
+with the documented codes (no GPU compute here: the calls fail before touching a device)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+
+import swarm_hip as sh
+from common import HERE
+
+REPO = os.path.dirname(HERE)
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(swarm_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 25
+    lib = C.CDLL(sh.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bad_arguments_are_rejected():
+    L = sh.lib()
+    h = C.c_void_p()
+    assert L.swarm_scene_build_layout(99, 0.008, 1.1, 1, C.byref(h)) == -1   # SWARM_EINVAL: unknown layout
+    assert b"unknown layout" in L.swarm_last_error()
+    assert L.swarm_step(None, 1) == -1
+    assert L.swarm_model_params(-1.0, C.byref(C.c_float()), C.byref(C.c_float())) == -1
+    x = np.zeros(4, np.float32)
+    assert L.swarm_place_agents(sh.LAYOUT_ROOM, 10**7, 10**7, 1, 1.34, 0.26, x, x, x, x, x) == -1
+
+
+def test_model_params_fold():
+    tau, relax = sh.model_params(0.5)
+    assert np.float32(tau) == np.float32(float.fromhex("0x1.d45d2cp-6"))     # sim_params.hpp TAU_RELAX
+    assert np.float32(relax) == np.float32(float.fromhex("0x1.68de1cp-1"))   # RELAX_STEP

@@ -1,0 +1,72 @@
+"""Constant parity (the reference's parity.constants, tools/test_gpu.cpp:2187): every folded model constant is
+recomputed here from its decimal factors the way the reference folds them (glsl_fold, sim_params.hpp:28-54:
+each float's shortest round-trip decimal, the expression in double, one rounding) and compared with the
+oracle's exported bits; the product's header holds the same literals (checked textually)."""
+import math
+import os
+import re
+
+import numpy as np
+
+import oracle_py as op
+from common import HERE
+
+f32 = np.float32
+
+
+def dec(x):
+    return float(np.format_float_positional(f32(x), unique=True))
+
+
+def fold_mul(a, b):
+    return f32(dec(a) * dec(b))
+
+
+def fold_div(a, b):
+    return f32(dec(a) / dec(b))
+
+
+TS = 17.490671
+
+
+def expected():
+    t_headway = fold_div(0.3, TS)
+    tau = fold_div(0.5, TS)
+    return {
+        "HEADWAY_RATE": f32(1.0 / dec(t_headway)),
+        "R_PERC2": fold_mul(0.045, 0.045),
+        "OBSTACLE_GRAD_SCALE": fold_div(0.25, 0.0015),
+        "HASH_TO_RADIANS": f32(6.28318530718 / 4294967296.0),
+        "RELAX_STEP": f32(math.exp(-dec(0.01) / dec(tau))),
+        "TAU_RELAX": tau,
+        "T_HEADWAY": t_headway,
+        "CHOICE_S_PER_UNIT": fold_div(31.25, 1.34),
+        "R_SEP2": fold_mul(0.016, 0.016),
+        "BODY_DIAM2": fold_mul(0.0128, 0.0128),
+    }
+
+
+def test_oracle_constants_are_the_folds():
+    c = op.constants()
+    e = expected()
+    assert c[5] == e["HEADWAY_RATE"] and c[6] == e["R_PERC2"] and c[13] == e["OBSTACLE_GRAD_SCALE"]
+    assert c[21] == e["HASH_TO_RADIANS"]
+    # SEP_MARGIN: BODY_DIAM + SEP_MARGIN is R_SEP exactly (the uniform-radius case reduces to the reference)
+    assert f32(c[3] + c[4]) == f32(0.016)
+    assert int(c[25]) == int(dec(fold_div(3.0, TS)) / dec(0.01) + 0.5) == 17   # CHOICE_STEPS
+
+
+def test_product_header_literals():
+    text = open(os.path.join(os.path.dirname(HERE), "swarm-crowd-simulation_amd", "csrc", "swarm_params.h")).read()
+    e = expected()
+    for name in ("TAU_RELAX", "T_HEADWAY", "HEADWAY_RATE", "RELAX_STEP", "R_PERC2", "OBSTACLE_GRAD_SCALE", "HASH_TO_RADIANS",
+                 "CHOICE_S_PER_UNIT", "R_SEP2", "BODY_DIAM2"):
+        m = re.search(r"constexpr float " + name + r" = (0x[0-9a-fp.+-]+)f;", text)
+        assert m, name
+        assert f32(float.fromhex(m.group(1))) == e[name], name
+
+
+def test_folding_differs_from_float_products():
+    """Why the folds matter: two of them are one float step from the float products."""
+    assert fold_mul(0.016, 0.016) != f32(0.016) * f32(0.016)
+    assert fold_mul(0.0128, 0.0128) != f32(0.0128) * f32(0.0128)

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP path through the C ABI against the reference's golden vectors and the oracle.
+
+Bars: bit-exact for everything (hash keys, sorted permutation, cell table, exit choices and counters,
+positions, velocities, speeds, flags): the kernels evaluate the oracle's float operations in the same
+order, so no tolerance is needed. The per-step bar of the task (1e-4 m, 1e-4 m/s) is strictly weaker.
+"""
+import numpy as np
+import pytest
+
+import swarm_hip as sh
+from common import agents_from_golden, assert_bits, golden, kat_rect_scene, oracle_for, terrace_scene
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "vx", "vy", "speed", "w")
+
+
+def compare_step(sim, o, what, cells=True):
+    g, c = sim.state(), o.state()
+    for f in FIELDS:
+        assert_bits(g[f], c[f], f"{what} {f}")
+    gk, gi = sim.grid()
+    ck, ci = o.grid()
+    assert np.array_equal(gk, ck), f"{what} keys"
+    assert np.array_equal(gi, ci), f"{what} permutation"
+    if cells:
+        gs, gc = sim.cell_table()
+        cs, cc = o.cell_table()
+        assert np.array_equal(gs, cs) and np.array_equal(gc, cc), f"{what} cell table"
+
+
+# ---------------- against the reference's own golden vectors ----------------
+@pytest.mark.parametrize("tag,steps", [("t_sparse", 6), ("t_dense", 8)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_gpu_matches_reference_terrace(tag, steps, graph):
+    kat = golden("terrace_kat.npz")
+    a = agents_from_golden(kat, tag)
+    n = a["x"].size
+    sim = sh.Sim(terrace_scene(), n, use_graph=graph)
+    sim.upload(a)
+    sim.set_drive(kat[tag + "__in__dx"], kat[tag + "__in__dy"])
+    for k in range(1, steps + 1):
+        sim.step(1)
+        p = f"{tag}__step{k}__"
+        s = sim.state()
+        for f in FIELDS:
+            assert_bits(s[f], kat[p + f], f"{tag} step {k} {f}")
+        key, idx = sim.grid()
+        assert np.array_equal(key, kat[p + "key"]) and np.array_equal(idx, kat[p + "id"])
+        st, cn = sim.cell_table()
+        assert np.array_equal(st, kat[p + "cell_start"]) and np.array_equal(cn, kat[p + "cell_count"])
+
+
+def test_gpu_choice_matches_reference():
+    import test_choice_golden as tcg
+
+    ck = golden("choice_kat.npz")
+    room = kat_rect_scene(golden("nav_kat.npz"), "room2")
+    tcg.check_all(lambda n: sh.Sim(room, n), ck)
+
+
+# ---------------- against the oracle on the benchmark floors ----------------
+def run_pair(layout, active, total, seed, steps, check_every=1, proj_iters=32, cells=True, **place):
+    scene = sh.Scene.layout(layout)
+    a = sh.place_agents(layout, active, total, seed, **place)
+    sim = sh.Sim(scene, total, proj_iters=proj_iters)
+    sim.upload(a)
+    o = oracle_for(scene, total, proj_iters=proj_iters)
+    o.set_state(a)
+    done = 0
+    while done < steps:
+        k = min(check_every, steps - done)
+        sim.step(k)
+        o.step(k)
+        done += k
+        compare_step(sim, o, f"layout {layout} step {done}", cells=cells)
+        assert np.array_equal(sim.assign(), o.assign()), f"layout {layout} step {done} exit choices"
+        assert np.array_equal(sim.choice_counters(), o.choice_counters()), f"layout {layout} step {done} choice counters"
+    return sim, o
+
+
+def test_room_4096_steps_bit_exact():
+    """The small plain config: 4,096 agents, 40 x 30 m room; 60 steps, every quantity every step (crosses
+    three exit-choice evaluations)."""
+    run_pair(sh.LAYOUT_ROOM, 4096, 4096, 1337, 60)
+
+
+def test_room_evacuation_curve():
+    """Evacuation events (agent, step, exit) identical over 600 steps."""
+    sim, o = run_pair(sh.LAYOUT_ROOM, 4096, 4096, 99, 600, check_every=100, cells=False)
+    gs, ge, gper = sim.evac()
+    cs, ce = o.evac()
+    assert np.array_equal(gs, cs) and np.array_equal(ge, ce)
+    assert (gs >= 0).sum() > 100   # people did leave
+    assert gper.sum() == (gs >= 0).sum()
+
+
+def test_bowl_63424_agents_bit_exact():
+    """The large plain config: 63,424 agents on the 200 x 150 m bowl, sort size 65,536 (2,112 parked)."""
+    run_pair(sh.LAYOUT_BOWL, 63424, 65536, 1337, 3)
+
+
+@pytest.mark.parametrize("layout", [sh.LAYOUT_CORRIDOR, sh.LAYOUT_PILLAR, sh.LAYOUT_BOWL_EIGHTH])
+def test_other_floors_bit_exact(layout):
+    cap = sh.layout_capacity(layout)
+    run_pair(layout, min(3000, cap), 3001, 5, 20, check_every=5)
+
+
+def test_edge_cases_sizes_and_flags():
+    """Agent counts that are not multiples of the block size, a single agent, all-inactive, no projection."""
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    for active, total, it in [(1, 1, 32), (200, 777, 32), (0, 300, 32), (500, 513, 0), (500, 513, 1)]:
+        a = sh.place_agents(sh.LAYOUT_ROOM, active, total, 3)
+        sim = sh.Sim(scene, total, proj_iters=it)
+        sim.upload(a)
+        o = oracle_for(scene, total, proj_iters=it)
+        o.set_state(a)
+        for k in range(3):
+            sim.step(1)
+            o.step(1)
+            compare_step(sim, o, f"edge {active}/{total}/{it} step {k}")
+
+
+def test_nan_position_reset():
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, 300, 300, 11)
+    a["x"][17] = np.nan
+    sim = sh.Sim(scene, 300)
+    sim.upload(a)
+    o = oracle_for(scene, 300)
+    o.set_state(a)
+    sim.step(1)
+    o.step(1)
+    compare_step(sim, o, "nan step 1")
+    assert not np.isnan(sim.state()["x"]).any()
+
+
+def test_graph_and_eager_identical():
+    """Determinism: a hipGraph replay and eager launches give the same bits (two independent runs)."""
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, 4096, 4096, 21)
+    s1 = sh.Sim(scene, 4096, use_graph=True)
+    s2 = sh.Sim(scene, 4096, use_graph=False)
+    s1.upload(a)
+    s2.upload(a)
+    s1.step(41)
+    s2.step(41)
+    x1, x2 = s1.state(), s2.state()
+    for f in FIELDS:
+        assert_bits(x1[f], x2[f], f"graph vs eager {f}")
+    assert np.array_equal(s1.assign(), s2.assign())
