@@ -1,0 +1,143 @@
+"""Benchmark of the per-step crowd pipeline on one MI355X.
+
+Workload: the large plain configuration, 63,424 active agents (sort size 65,536: 2,112 parked) seated on the
+200 x 150 m bowl (240 rows of 0.1 m boxes, 16 gangways, 8 exits of 2.4 m, stair bands at 0.6x), per-agent
+radius U[0.20, 0.25] m and free speed N(1.34, 0.26) m/s clipped to [0.5, 2.0], seed 1337. A step is the whole
+pipeline: hash, counting sort, cell table + gather, the queue-aware exit choice, predict, 31 Jacobi projection
+iterations, final pass. Inputs are resident in HBM before the timed region; the timed region is K consecutive
+steps (hipGraph replays) bracketed by HIP events on the pipeline's stream.
+
+Prints one JSON line: agent-steps/s (the sum over the timed steps of the agents active at each step's input,
+per second), the roofline of the dominant kernel (the projection iteration, HBM-bound), and the CPU baseline
+(the oracle, a CPU restatement of the reference's CpuCrowdSim, on the same workload, host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "swarm-crowd-simulation_amd"))
+
+import numpy as np  # noqa: E402
+
+import swarm_hip as sh  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+ACTIVE, TOTAL, SEED = 63424, 65536, 1337
+
+
+def step_bytes(n_total, cells, active, iters=32):
+    """Algorithmic HBM bytes of one plain step (compulsory traffic, neighbour reads cache-resident; the per-kernel
+    terms of the per-step accounting): hash 20N, sort 16N, ranges 4N + 8C, gather 64A, predict 100A,
+    projection 36A per iteration, final 64A, evacuation 8A."""
+    return 40 * n_total + 8 * cells + (236 + 36 * (iters - 1)) * active
+
+
+PROJ_BYTES_PER_SLOT = 52   # one projection launch per active slot: key 4, x/y/acc 12, flag 4, radius 4, plane 16 read; x/y/acc 12 written
+
+
+def cpu_baseline(scene, a, steps):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py as op
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    tau, relax = sh.model_params(0.5)
+    o = op.Oracle(scene.info, scene.tables(), TOTAL, relax, tau, threads=threads)
+    o.set_state(a)
+    o.step(1)   # warm-up
+    active = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        active += int(o.state()["w"].sum())
+        o.step(1)
+    dt = time.perf_counter() - t0
+    return dict(value=active / dt, unit="agent-steps/s", cores=threads, kind="port",
+                sample=f"{steps} consecutive steps of the 63,424-agent bowl after one warm-up step, oracle (OpenMP dynamic,64)",
+                ms_per_step=1000.0 * dt / steps)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    scene = sh.Scene.layout(sh.LAYOUT_BOWL)
+    a = sh.place_agents(sh.LAYOUT_BOWL, ACTIVE, TOTAL, SEED)
+    build_s = time.perf_counter() - t0
+    sim = sh.Sim(scene, TOTAL)
+    sim.upload(a)
+    print(f"# scene built in {build_s:.1f} s; hash grid {scene.info.hash_nx}x{scene.info.hash_ny}, "
+          f"SDF {scene.info.sdf_nx}x{scene.info.sdf_ny}, nav {scene.info.nav_res_x}x{scene.info.nav_res_y} x {scene.info.E} exits",
+          file=sys.stderr, flush=True)
+
+    sim.step(args.warmup)
+    sim.sync()
+    _, _, sum0 = sim.control()
+    ms = sim.time_steps(args.steps)
+    _, n_active, sum1 = sim.control()
+    agent_steps = sum1 - sum0
+    value = agent_steps / (ms / 1000.0)
+    ms_per_step = ms / args.steps
+    print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e6:.1f} M agent-steps/s", file=sys.stderr, flush=True)
+
+    # per-pass timing (eager launches with events between passes) for the dominant kernel's launch duration
+    sim.set_timing(True)
+    sim.step(args.profile_steps)
+    t = sim.timings()
+    sim.set_timing(False)
+    passes = dict(grid=t[0], choice=t[1], predict=t[2], projection=t[3], final=t[4], total_eager=t[5])
+    proj_launch_ms = t[6]
+    mean_active = agent_steps / args.steps
+    proj_bytes = PROJ_BYTES_PER_SLOT * mean_active
+    achieved = proj_bytes / (proj_launch_ms * 1e-3) / 1e9
+    cells = scene.info.hash_nx * scene.info.hash_ny
+    b_step = step_bytes(TOTAL, cells, mean_active)
+    step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
+    print(f"# passes ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in passes.items()) +
+          f"; one projection launch {proj_launch_ms * 1000:.2f} us", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, a, args.cpu_steps)
+        print(f"# cpu baseline {cpu['ms_per_step']:.1f} ms/step on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "agent-steps/s @63,424 agents + %HBM roofline; batched & adjoint vs plain",
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded seat placement on the generated bowl floor",
+        "config": {"workload": "plain step, 63,424 agents on the 200 x 150 m seated bowl, sort size 65,536",
+                   "agents_active": ACTIVE, "agents_total": TOTAL, "hash_cells": cells, "proj_iterations": 32,
+                   "exits": scene.info.E, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,
+                     "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+        "passes_ms": passes,
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -139,6 +139,13 @@ int swarm_read_choice_counters(swarm_ctx* c, uint32_t* out132);
 int swarm_read_evac(swarm_ctx* c, int32_t* step, int32_t* exit_id, uint32_t* per_exit);
 /* step counter, active agents at the last step's input, running sum of active agents over steps */
 int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
+/* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:141-149): per-pass HIP events (steps then run
+ * without graph capture). ms8: mean ms per step of grid passes (hash, sort, cells, gather), exit choice,
+ * predict, projection iterations, final, total; the mean ms of one projection launch; the steps timed */
+int swarm_set_timing(swarm_ctx* c, int on);
+int swarm_read_timings(swarm_ctx* c, double* ms8);
+/* nsteps steps between two HIP events on the context's stream (synchronises); *ms: their elapsed time */
+int swarm_time_steps(swarm_ctx* c, int nsteps, double* ms);
 /* the context's HIP stream (hipStream_t), for events around swarm_step */
 int swarm_get_stream(swarm_ctx* c, void** stream);
 int swarm_num_cells(swarm_ctx* c);

@@ -71,8 +71,9 @@ void launchGridPasses(hipStream_t st, int n, const SceneView& s, AgentHalf rd, c
                       const float* rad, uint32_t* ctrl);
 void launchChoice(hipStream_t st, int n, const SceneView& s, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b,
                   const float* driveX, const float* driveY, uint32_t* ctrl);
+// stamps (optional, 2 events): recorded after predict and after the projection iterations
 void launchPhysics(hipStream_t st, int n, const SceneView& s, const ModelParams& mp, StepBuffers& b, ProjBuffers pa,
-                   ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl);
+                   ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps = nullptr);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
 
 } // namespace swarm

@@ -508,14 +508,16 @@ void launchChoice(hipStream_t st, int n, const SceneView& s, AgentHalf rd, const
 }
 
 void launchPhysics(hipStream_t st, int n, const SceneView& s, const ModelParams& mp, StepBuffers& b, ProjBuffers pa,
-                   ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl) {
+                   ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
     const int g = blocks(n, PHYS_BLOCK);
     k_predict<<<g, PHYS_BLOCK, 0, st>>>(n, s, mp, b, pa);
+    if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < mp.projIters; ++it) {
         k_project<<<g, PHYS_BLOCK, 0, st>>>(n, s, b, in, out);
         const ProjBuffers t = in; in = out; out = t;
     }
+    if (stamps) (void)hipEventRecord(stamps[1], st);
     k_final<<<g, PHYS_BLOCK, 0, st>>>(n, s, mp, b, in, rd, wr, ev, ctrl);
 }
 

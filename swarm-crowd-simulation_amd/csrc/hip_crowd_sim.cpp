@@ -188,9 +188,59 @@ void HipCrowdSim::enqueueStep(int parity) {
     StepBuffers b = sb_;
     b.avx = rd.vx;
     b.avy = rd.vy;
+    std::vector<hipEvent_t> ev;
+    if (timing_) {   // stamps: start, sorted, choice, predict, projection, final
+        ev.resize(6);
+        for (hipEvent_t& e : ev) SWARM_HIP_CHECK(hipEventCreate(&e));
+        SWARM_HIP_CHECK(hipEventRecord(ev[0], stream_));
+    }
     launchGridPasses(stream_, n_, scene_.view, rd, grid_, b, rad_, ctrl_);
+    if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[1], stream_));
     launchChoice(stream_, n_, scene_.view, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
-    launchPhysics(stream_, n_, scene_.view, mp_, b, pa_, pb_, rd, wr, ev_, ctrl_);
+    if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
+    launchPhysics(stream_, n_, scene_.view, mp_, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
+    if (timing_) {
+        SWARM_HIP_CHECK(hipEventRecord(ev[5], stream_));
+        pendingStamps_.push_back(ev);
+        if (pendingStamps_.size() >= 64) collectTimings();
+    }
+}
+
+void HipCrowdSim::collectTimings() {
+    for (std::vector<hipEvent_t>& ev : pendingStamps_) {
+        SWARM_HIP_CHECK(hipEventSynchronize(ev.back()));
+        float ms[5];
+        for (int k = 0; k < 5; ++k) SWARM_HIP_CHECK(hipEventElapsedTime(&ms[k], ev[size_t(k)], ev[size_t(k) + 1]));
+        float total = 0.0f;
+        SWARM_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[5]));
+        timeSum_[0] += ms[0];   // hash + sort + cells + gather
+        timeSum_[1] += ms[1];   // exit choice
+        timeSum_[2] += ms[2];   // predict
+        timeSum_[3] += ms[3];   // projection iterations
+        timeSum_[4] += ms[4];   // final
+        timeSum_[5] += total;
+        ++timedSteps_;
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    pendingStamps_.clear();
+}
+
+void HipCrowdSim::setTimingEnabled(bool on) {
+    if (!on && timing_) collectTimings();
+    timing_ = on;
+    if (on) {
+        for (double& t : timeSum_) t = 0.0;
+        timedSteps_ = 0;
+    }
+}
+
+int HipCrowdSim::readTimings(double* ms8) {
+    collectTimings();
+    const double n = timedSteps_ > 0 ? double(timedSteps_) : 1.0;
+    for (int k = 0; k < 6; ++k) ms8[k] = timeSum_[k] / n;
+    ms8[6] = mp_.projIters > 1 ? ms8[3] / double(mp_.projIters - 1) : 0.0;
+    ms8[7] = double(timedSteps_);
+    return timedSteps_;
 }
 
 void HipCrowdSim::buildGraph() {
@@ -204,7 +254,7 @@ void HipCrowdSim::buildGraph() {
 void HipCrowdSim::step(int nsteps) {
     if (nsteps < 0) throw std::invalid_argument("step: negative step count");
     int left = nsteps;
-    if (opt_.useGraph && left >= 2) {
+    if (opt_.useGraph && !timing_ && left >= 2) {
         if (parity_ == 1) { enqueueStep(1); parity_ = 0; --left; }
         if (!graphExec_) buildGraph();
         while (left >= 2) {
@@ -218,6 +268,21 @@ void HipCrowdSim::step(int nsteps) {
         --left;
     }
     SWARM_HIP_CHECK(hipGetLastError());
+}
+
+double HipCrowdSim::timeSteps(int nsteps) {
+    hipEvent_t a, b;
+    SWARM_HIP_CHECK(hipEventCreate(&a));
+    SWARM_HIP_CHECK(hipEventCreate(&b));
+    SWARM_HIP_CHECK(hipEventRecord(a, stream_));
+    step(nsteps);
+    SWARM_HIP_CHECK(hipEventRecord(b, stream_));
+    SWARM_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    SWARM_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms);
 }
 
 void HipCrowdSim::stepExitChoice() {

@@ -45,6 +45,8 @@ public:
     void restoreAgentState(const float* x, const float* y, const float* vx, const float* vy, const float* w);
     void setDrive(const float* dx, const float* dy);   // nullptr: the navigation table (CpuCrowdSim::setDriveTable)
     void step(int nsteps);
+    // step(nsteps) between two HIP events on the context's stream; returns their elapsed ms (synchronises)
+    double timeSteps(int nsteps);
     // The exit choice pass alone on the current state (CrowdSim::stepExitChoice); advances the step counter
     void stepExitChoice();
     // Sets the exit choice's state (tools and tests: the reference's choice harness setters,
@@ -61,6 +63,13 @@ public:
     void readChoiceCounters(uint32_t* out132);
     void readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit);
     void readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum);
+
+    // Per-pass GPU timing (CrowdSim::setTimingEnabled / readTimings, crowd_sim.hpp:141-149): while enabled every
+    // step runs eagerly with HIP events between its passes on the context's stream. readTimings: the mean ms per
+    // step of hash, sort (scan + scatter + in-bucket order + gather), exit choice, predict, projection (all
+    // PROJ_ITERATIONS - 1 launches), final, total, and the mean ms of one projection launch; returns the steps timed
+    void setTimingEnabled(bool on);
+    int readTimings(double* ms8);
 
     int numAgents() const { return n_; }
     int numCells() const { return C_; }
@@ -93,6 +102,13 @@ private:
     int parity_ = 0;   // the half holding the current state
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
+    // timing: events per stamp (hash, sort, choice, predict, project, final, end) per recorded step
+    bool timing_ = false;
+    std::vector<hipEvent_t> events_;
+    std::vector<std::vector<hipEvent_t>> pendingStamps_;
+    double timeSum_[8] = {};
+    int timedSteps_ = 0;
+    void collectTimings();
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

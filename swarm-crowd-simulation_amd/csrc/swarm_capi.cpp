@@ -183,6 +183,17 @@ int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_
     CTX_CHECK(c);
     return guarded([&] { c->sim->readControl(step, n_active, active_sum); });
 }
+int swarm_set_timing(swarm_ctx* c, int on) { CTX_CHECK(c); return guarded([&] { c->sim->setTimingEnabled(on != 0); }); }
+int swarm_read_timings(swarm_ctx* c, double* ms8) {
+    CTX_CHECK(c);
+    if (!ms8) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readTimings(ms8); });
+}
+int swarm_time_steps(swarm_ctx* c, int nsteps, double* ms) {
+    CTX_CHECK(c);
+    if (!ms) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { *ms = c->sim->timeSteps(nsteps); });
+}
 int swarm_get_stream(swarm_ctx* c, void** stream) {
     CTX_CHECK(c);
     if (!stream) { g_err = "null argument"; return SWARM_EINVAL; }

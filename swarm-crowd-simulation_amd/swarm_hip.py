@@ -90,6 +90,9 @@ def lib():
     L.swarm_read_evac.argtypes = [vp, _i32p, _i32p, _u32p]
     L.swarm_read_control.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
     L.swarm_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.swarm_set_timing.argtypes = [vp, C.c_int]
+    L.swarm_read_timings.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
+    L.swarm_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.swarm_num_cells.argtypes = [vp]
     L.swarm_destroy.argtypes = [vp]
     L.swarm_destroy.restype = None
@@ -262,6 +265,20 @@ class Sim:
         s, a, t = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().swarm_read_control(self.h, C.byref(s), C.byref(a), C.byref(t)))
         return s.value, a.value, t.value
+
+    def set_timing(self, on):
+        check(lib().swarm_set_timing(self.h, 1 if on else 0))
+
+    def timings(self):
+        """Mean ms per step: grid passes, exit choice, predict, projection, final, total; one projection launch; steps."""
+        t = np.zeros(8, np.float64)
+        check(lib().swarm_read_timings(self.h, t))
+        return t
+
+    def time_steps(self, k):
+        ms = C.c_double()
+        check(lib().swarm_time_steps(self.h, k, C.byref(ms)))
+        return ms.value
 
     def stream(self):
         p = C.c_void_p()
