@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--copy-probe", action="store_true", default=True)
+    ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
@@ -94,7 +96,7 @@ def main():
 
     # per-pass timing (eager launches with events between passes) for the dominant kernel's launch duration
     sim.set_timing(True)
-    sim.step(args.profile_steps)
+    sim.step(max(1, args.profile_steps))
     t = sim.timings()
     sim.set_timing(False)
     passes = dict(grid=t[0], choice=t[1], predict=t[2], projection=t[3], final=t[4], total_eager=t[5])
@@ -107,6 +109,13 @@ def main():
     step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
     print(f"# passes ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in passes.items()) +
           f"; one projection launch {proj_launch_ms * 1000:.2f} us", file=sys.stderr, flush=True)
+
+    copy_gbs = None
+    if args.copy_probe:   # a plain copy of 1 GiB: the measured HBM copy bandwidth (read + write bytes / time)
+        nb = 1 << 30
+        cms = sh.copy_probe(nb, 10)
+        copy_gbs = 2.0 * nb / (cms * 1e-3) / 1e9
+        print(f"# copy probe: 1 GiB in {cms:.3f} ms = {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -132,7 +141,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,
-                     "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+                     "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
+                     "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -146,6 +146,9 @@ int swarm_set_timing(swarm_ctx* c, int on);
 int swarm_read_timings(swarm_ctx* c, double* ms8);
 /* nsteps steps between two HIP events on the context's stream (synchronises); *ms: their elapsed time */
 int swarm_time_steps(swarm_ctx* c, int nsteps, double* ms);
+/* A plain device-to-device copy kernel of 'bytes' (reps launches, mean ms per launch): the measured copy
+ * bandwidth and the calibration of HBM traffic counters. No counterpart in the reference (measurement aid). */
+int swarm_copy_probe(size_t bytes, int reps, double* ms);
 /* the context's HIP stream (hipStream_t), for events around swarm_step */
 int swarm_get_stream(swarm_ctx* c, void** stream);
 int swarm_num_cells(swarm_ctx* c);

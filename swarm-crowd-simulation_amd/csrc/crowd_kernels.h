@@ -44,7 +44,13 @@ struct StepBuffers {
     float4* planes;                         // wall plane (normal, offset, flags)
     const float* avx; const float* avy;     // agent order, start of step
     const float* v0;                        // agent order
+    // the projection's contact candidates per slot, slot-interleaved: nbr[j * n + slot] (NBR_CAP entries);
+    // nbrCount[slot] (NBR_OVERFLOW: more than NBR_CAP, the slot scans the ring instead)
+    uint32_t* nbr; uint32_t* nbrCount;
 };
+
+constexpr int NBR_CAP = 48;
+constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
 
 // the projection's slot-order ping-pong
 struct ProjBuffers {
@@ -75,5 +81,6 @@ void launchChoice(hipStream_t st, int n, const SceneView& s, AgentHalf rd, const
 void launchPhysics(hipStream_t st, int n, const SceneView& s, const ModelParams& mp, StepBuffers& b, ProjBuffers pa,
                    ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps = nullptr);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
+void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes);
 
 } // namespace swarm

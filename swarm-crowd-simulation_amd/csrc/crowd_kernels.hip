@@ -20,6 +20,21 @@
 
 namespace swarm {
 
+// base[key] += the number of lanes of this wave that pass 'valid' with this key: one atomic per distinct key of
+// the wave instead of one per lane (the counts are integer sums, so the result does not depend on the order).
+// Every lane of the wave must reach the call.
+__device__ inline void waveCountAdd(uint32_t* base, uint32_t key, bool valid) {
+    uint64_t pending = __ballot(valid);
+    const int lane = __lane_id();
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(valid && key == k) & pending;
+        if (lane == leader) atomicAdd(&base[k], uint32_t(__popcll(same)));
+        pending &= ~same;
+    }
+}
+
 __device__ inline uint32_t bucketOf(uint32_t key, uint32_t agent, uint32_t C) {
     return key == INACTIVE_CELL ? C + agent / uint32_t(HASH_BLOCK) : key;
 }
@@ -184,32 +199,39 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_eval(int n, SceneView s, 
     const uint32_t step = ctrl[CTRL_STEP];
     if (!evalStep(step)) return;
     const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    pending[i] = 0u;
-    if (aw[i] == 0.0f) return;
-    const uint32_t* aRead = ahead + size_t(step & 1u) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS;
-    const V2 p{ ax[i], ay[i] };
-    const int c = choiceCell(s, p);
-    const uint32_t* sl = s.navSlot + size_t(c) * size_t(s.K);
-    const uint16_t* dq = s.navDist + size_t(c) * size_t(s.K);
-    if (slotExitOf(sl[0]) == CHOICE_EMPTY) return;
-    const uint32_t current = assign[i];
-    const int curK = slotOfExit(s, c, current);
-    if (curK < 0) return;
-    atomicAdd(&counters[CNT_EVALUATIONS], 1u);
-    float eBest;
-    const int best = bestCandidate(s, c, aRead, eBest);
-    const uint32_t qc = dq[curK];
-    const float eCur = estimatedTime(s, slotDistOf(s, qc), current, aRead[size_t(current) * CHOICE_AHEAD_BINS + aheadBin(qc)]);
-    const float gain = eCur - eBest;
-    const float m = CHOICE_REL_MARGIN * eCur;
-    if (best != curK && gain > (m < CHOICE_ABS_MARGIN_S ? CHOICE_ABS_MARGIN_S : m)) {
-        const uint32_t g = uint32_t(gain);
-        const uint32_t bin = g < uint32_t(CHOICE_GAIN_BINS - 1) ? g : uint32_t(CHOICE_GAIN_BINS - 1);
-        pending[i] = 0x80000000u | (slotExitOf(sl[best]) << 16) | bin;
-        atomicAdd(&hist[size_t(current) * CHOICE_GAIN_BINS + bin], 1u);
-        atomicAdd(&counters[CNT_ELIGIBLE], 1u);
+    bool evaluated = false, eligible = false;
+    uint32_t hkey = 0;
+    if (i < n) {
+        pending[i] = 0u;
+        if (aw[i] != 0.0f) {
+            const uint32_t* aRead = ahead + size_t(step & 1u) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS;
+            const V2 p{ ax[i], ay[i] };
+            const int c = choiceCell(s, p);
+            const uint32_t* sl = s.navSlot + size_t(c) * size_t(s.K);
+            const uint16_t* dq = s.navDist + size_t(c) * size_t(s.K);
+            const uint32_t current = assign[i];
+            const int curK = slotExitOf(sl[0]) == CHOICE_EMPTY ? -1 : slotOfExit(s, c, current);
+            if (curK >= 0) {
+                evaluated = true;
+                float eBest;
+                const int best = bestCandidate(s, c, aRead, eBest);
+                const uint32_t qc = dq[curK];
+                const float eCur = estimatedTime(s, slotDistOf(s, qc), current, aRead[size_t(current) * CHOICE_AHEAD_BINS + aheadBin(qc)]);
+                const float gain = eCur - eBest;
+                const float m = CHOICE_REL_MARGIN * eCur;
+                if (best != curK && gain > (m < CHOICE_ABS_MARGIN_S ? CHOICE_ABS_MARGIN_S : m)) {
+                    const uint32_t g = uint32_t(gain);
+                    const uint32_t bin = g < uint32_t(CHOICE_GAIN_BINS - 1) ? g : uint32_t(CHOICE_GAIN_BINS - 1);
+                    pending[i] = 0x80000000u | (slotExitOf(sl[best]) << 16) | bin;
+                    eligible = true;
+                    hkey = current * uint32_t(CHOICE_GAIN_BINS) + bin;
+                }
+            }
+        }
     }
+    waveCountAdd(hist, hkey, eligible);
+    waveCountAdd(counters, uint32_t(CNT_EVALUATIONS), evaluated);
+    waveCountAdd(counters, uint32_t(CNT_ELIGIBLE), eligible);
 }
 
 // thresholds stage (evaluation steps only), one thread per exit
@@ -248,39 +270,51 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s,
                                                              uint32_t* __restrict__ counters, float* __restrict__ egx,
                                                              float* __restrict__ egy, const uint32_t* __restrict__ ctrl) {
     const int slot = blockIdx.x * HASH_BLOCK + threadIdx.x;
-    if (slot >= n || skey[slot] == INACTIVE_CELL) return;
+    const bool live = slot < n && skey[slot] != INACTIVE_CELL;
     const uint32_t step = ctrl[CTRL_STEP];
     const bool ev = evalStep(step);
     const uint32_t par = step & 1u;
     const uint32_t* aRead = ahead + size_t(par) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS;
     uint32_t* aWrite = ahead + size_t(1u - par) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS;
     const uint32_t qWrite = uint32_t(CHOICE_QUEUE_STRIDE) - par * uint32_t(CHOICE_QUEUE_STRIDE);
-    const uint32_t id = sid[slot];
-    const V2 p{ sx[slot], sy[slot] };
-    const int c = choiceCell(s, p);
-    const uint32_t* sl = s.navSlot + size_t(c) * size_t(s.K);
-    const uint32_t current = assign[id];
-    uint32_t ex = current;
-    const bool hasPath = slotExitOf(sl[0]) != CHOICE_EMPTY;
-    const int curK = hasPath ? slotOfExit(s, c, current) : -1;
-    if (hasPath && curK < 0) {
-        float eBest;
-        ex = slotExitOf(sl[bestCandidate(s, c, aRead, eBest)]);
-        if (current != CHOICE_NONE) atomicAdd(&counters[CNT_FORCED], 1u);
-    } else if (ev && curK >= 0) {
-        const uint32_t pd = pending[id];
-        if ((pd & 0x80000000u) && (pd & 0xFFFFu) >= threshold[current]) {
-            ex = (pd >> 16) & 63u;
-            atomicAdd(&counters[CNT_SWITCHES], 1u);
+    bool forced = false, switched = false, counted = false, binned = false;
+    uint32_t ex = CHOICE_NONE, bin = 0;
+    if (live) {
+        const uint32_t id = sid[slot];
+        const V2 p{ sx[slot], sy[slot] };
+        const int c = choiceCell(s, p);
+        const uint32_t* sl = s.navSlot + size_t(c) * size_t(s.K);
+        const uint32_t current = assign[id];
+        ex = current;
+        const bool hasPath = slotExitOf(sl[0]) != CHOICE_EMPTY;
+        const int curK = hasPath ? slotOfExit(s, c, current) : -1;
+        if (hasPath && curK < 0) {
+            float eBest;
+            ex = slotExitOf(sl[bestCandidate(s, c, aRead, eBest)]);
+            forced = current != CHOICE_NONE;
+        } else if (ev && curK >= 0) {
+            const uint32_t pd = pending[id];
+            if ((pd & 0x80000000u) && (pd & 0xFFFFu) >= threshold[current]) {
+                ex = (pd >> 16) & 63u;
+                switched = true;
+            }
         }
+        assign[id] = ex;
+        counted = ex != CHOICE_NONE;
+        const int k = slotOfExit(s, c, ex);
+        if (k >= 0) {
+            binned = true;
+            bin = ex * uint32_t(CHOICE_AHEAD_BINS) + aheadBin(s.navDist[size_t(c) * size_t(s.K) + size_t(k)]);
+        }
+        const V2 e = choiceDirection(s, c, p, ex);
+        egx[slot] = e.x;
+        egy[slot] = e.y;
     }
-    assign[id] = ex;
-    if (ex != CHOICE_NONE) atomicAdd(&queue[qWrite + ex], 1u);
-    const int k = slotOfExit(s, c, ex);
-    if (k >= 0) atomicAdd(&aWrite[size_t(ex) * CHOICE_AHEAD_BINS + aheadBin(s.navDist[size_t(c) * size_t(s.K) + size_t(k)])], 1u);
-    const V2 e = choiceDirection(s, c, p, ex);
-    egx[slot] = e.x;
-    egy[slot] = e.y;
+    // the counts of the next step: one atomic per distinct key in the wave (integer sums: order-free)
+    waveCountAdd(queue + qWrite, ex, counted);
+    waveCountAdd(aWrite, bin, binned);
+    waveCountAdd(counters, uint32_t(CNT_FORCED), forced);
+    waveCountAdd(counters, uint32_t(CNT_SWITCHES), switched);
 }
 
 // e_goal from a forced direction table (TEST_DRIVE_TABLE), or zero without a table and without exits
@@ -373,19 +407,25 @@ __global__ void __launch_bounds__(PHYS_BLOCK) k_predict(int n, SceneView s, Mode
     b.planes[slot] = make_float4(wn.x, wn.y, dot2(np, wn) - wall[2], (nonzero(wn) ? 1.0f : 0.0f) + (grounded ? 2.0f : 0.0f));
 }
 
-// ---------------- PHYSICS_PASS 2: one Jacobi iteration ----------------
-// projectSlot (cpu_crowd_sim.cpp:781-822); returns the slot's new position and displacement accumulator
-__device__ inline void projectOne(const SceneView& s, const StepBuffers& b, const ProjBuffers& in, int slot, uint32_t cid,
-                                  float& ox, float& oy, float& oacc) {
-    const float bx = in.x[slot], by = in.y[slot];
-    ox = bx; oy = by; oacc = in.acc[slot];
-    if (b.pw[slot] == 0.0f) return;
-    const uint32_t self = b.sid[slot];
+// ---------------- the projection's contact candidates ----------------
+// Every projection iteration moves an agent by at most PROJ_MAX_SHIFT_RADII r (the capped correction; the wall
+// plane is a projection onto a half-space holding the predicted position, which moves it no farther), so over the
+// step's P iterations a pair can touch only if its predicted distance is below D + P (shift_i + shift_j). The
+// candidates are the ring's slots that pass this bound (with 0.1% of float slack), in ring order: summing the
+// contacts over them gives the ring scan's sums bit for bit, at a fraction of its loads. Parked slots never
+// count (the ring scan skips them too).
+__global__ void __launch_bounds__(PHYS_BLOCK) k_contact_list(int n, SceneView s, ModelParams mp, StepBuffers b, ProjBuffers pred) {
+    const int slot = blockIdx.x * PHYS_BLOCK + threadIdx.x;
+    if (slot >= n) return;
+    const uint32_t cid = b.skey[slot];
+    if (cid == INACTIVE_CELL || b.pw[slot] == 0.0f) return;
+    const V2 p{ pred.x[slot], pred.y[slot] };
     const float ri = b.sr[slot];
+    const float P = float(mp.projIters);
     const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
-    V2 corr{ 0.0f, 0.0f };
-    int nContacts = 0;
     const int R = s.projRing;
+    const size_t stride = size_t(n);
+    uint32_t cnt = 0;
     for (int yy = -R; yy <= R; yy++) {
         const int ny = cy + yy;
         if (ny < 0 || ny >= s.hashNY) continue;
@@ -395,12 +435,66 @@ __device__ inline void projectOne(const SceneView& s, const StepBuffers& b, cons
             const int c = nx + ny * s.hashNX;
             const uint32_t st = b.cellStart[c], en = st + b.cellCount[c];
             for (uint32_t k = st; k < en; k++) {
-                if (k == uint32_t(slot)) continue;
-                const V2 offset = V2{ bx, by } - V2{ in.x[k], in.y[k] };
-                const float D = ri + b.sr[k];
-                if (b.pw[k] != 0.0f && dot2(offset, offset) < contact2(D)) {
-                    corr = corr + contactCorrection(offset, self, b.sid[k], D);
-                    ++nContacts;
+                if (k == uint32_t(slot) || b.pw[k] == 0.0f) continue;
+                const float rk = b.sr[k];
+                const float reach = ((ri + rk) + P * (PROJ_MAX_SHIFT_RADII * (ri + rk))) * 1.001f + 1e-6f;
+                const V2 d = p - V2{ pred.x[k], pred.y[k] };
+                if (dot2(d, d) < reach * reach) {
+                    if (cnt < uint32_t(NBR_CAP)) b.nbr[size_t(cnt) * stride + size_t(slot)] = k;
+                    ++cnt;
+                }
+            }
+        }
+    }
+    b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
+}
+
+// ---------------- PHYSICS_PASS 2: one Jacobi iteration ----------------
+// projectSlot (cpu_crowd_sim.cpp:781-822); returns the slot's new position and displacement accumulator
+__device__ inline void projectOne(const SceneView& s, const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cid,
+                                  float& ox, float& oy, float& oacc) {
+    const float bx = in.x[slot], by = in.y[slot];
+    ox = bx; oy = by; oacc = in.acc[slot];
+    if (b.pw[slot] == 0.0f) return;
+    const uint32_t self = b.sid[slot];
+    const float ri = b.sr[slot];
+    V2 corr{ 0.0f, 0.0f };
+    int nContacts = 0;
+    const uint32_t cnt = b.nbrCount[slot];
+    if (cnt != NBR_OVERFLOW) {
+        // the candidates in ring order: the same contacts summed in the same order as the ring scan below
+        const size_t stride = size_t(n);
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const uint32_t k = b.nbr[size_t(j) * stride + size_t(slot)];
+            const V2 offset = V2{ bx, by } - V2{ in.x[k], in.y[k] };
+            const float D = ri + b.sr[k];
+            const float dxy2 = dot2(offset, offset);
+            if (dxy2 < contact2(D)) {   // contactCorrection, inlined: the pair's id only for a stacked pair
+                const float dxy = sqrtf(dxy2);
+                const float shift = 0.5f * (D - dxy);
+                corr = corr + (dxy2 <= SEP_STACKED_XY2 ? shift * pairDirection(self, b.sid[k]) : shift * (offset / dxy));
+                ++nContacts;
+            }
+        }
+    } else {
+        const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
+        const int R = s.projRing;
+        for (int yy = -R; yy <= R; yy++) {
+            const int ny = cy + yy;
+            if (ny < 0 || ny >= s.hashNY) continue;
+            for (int xx = -R; xx <= R; xx++) {
+                const int nx = cx + xx;
+                if (nx < 0 || nx >= s.hashNX) continue;
+                const int c = nx + ny * s.hashNX;
+                const uint32_t st = b.cellStart[c], en = st + b.cellCount[c];
+                for (uint32_t k = st; k < en; k++) {
+                    if (k == uint32_t(slot)) continue;
+                    const V2 offset = V2{ bx, by } - V2{ in.x[k], in.y[k] };
+                    const float D = ri + b.sr[k];
+                    if (b.pw[k] != 0.0f && dot2(offset, offset) < contact2(D)) {
+                        corr = corr + contactCorrection(offset, self, b.sid[k], D);
+                        ++nContacts;
+                    }
                 }
             }
         }
@@ -426,7 +520,7 @@ __global__ void __launch_bounds__(PHYS_BLOCK) k_project(int n, SceneView s, Step
     const uint32_t cid = b.skey[slot];
     if (cid == INACTIVE_CELL) return;
     float x, y, acc;
-    projectOne(s, b, in, slot, cid, x, y, acc);
+    projectOne(s, b, in, n, slot, cid, x, y, acc);
     out.x[slot] = x; out.y[slot] = y; out.acc[slot] = acc;
 }
 
@@ -443,7 +537,7 @@ __global__ void __launch_bounds__(PHYS_BLOCK) k_final(int n, SceneView s, ModelP
             wr.vx[self] = rd.vx[self]; wr.vy[self] = rd.vy[self]; wr.spd[self] = rd.spd[self];
         } else {
             float x, y, acc;
-            if (mp.projIters > 0) projectOne(s, b, in, slot, cid, x, y, acc);
+            if (mp.projIters > 0) projectOne(s, b, in, n, slot, cid, x, y, acc);
             else { x = in.x[slot]; y = in.y[slot]; acc = in.acc[slot]; }
             if (b.pw[slot] == 0.0f) {   // parked this step: an evacuation event
                 wr.x[self] = x; wr.y[self] = y; wr.w[self] = 0.0f;
@@ -468,6 +562,16 @@ __global__ void __launch_bounds__(PHYS_BLOCK) k_final(int n, SceneView s, ModelP
         ctrl[CTRL_STEP] = step + 1u;
         *reinterpret_cast<unsigned long long*>(ctrl + CTRL_ACTIVE_SUM_LO) += ctrl[CTRL_NACTIVE];
     }
+}
+
+// a plain streaming copy of known size: the HBM traffic counters' calibration and the measured copy bandwidth
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n4; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+
+void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    const size_t n4 = bytes / 16;
+    k_copy_probe<<<4096, 256, 0, st>>>(static_cast<const float4*>(src), static_cast<float4*>(dst), n4);
 }
 
 // the step counter of an exit-choice pass run on its own (CrowdSim::stepExitChoice)
@@ -511,6 +615,7 @@ void launchPhysics(hipStream_t st, int n, const SceneView& s, const ModelParams&
                    ProjBuffers pb, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
     const int g = blocks(n, PHYS_BLOCK);
     k_predict<<<g, PHYS_BLOCK, 0, st>>>(n, s, mp, b, pa);
+    if (mp.projIters > 0) k_contact_list<<<g, PHYS_BLOCK, 0, st>>>(n, s, mp, b, pa);
     if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < mp.projIters; ++it) {

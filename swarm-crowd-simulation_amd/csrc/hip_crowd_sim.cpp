@@ -99,6 +99,8 @@ HipCrowdSim::HipCrowdSim(const Scene& scene, int numAgents, const SimOptions& op
     sb_.pw = alloc<float>(n); sb_.pvx = alloc<float>(n); sb_.pvy = alloc<float>(n);
     sb_.planes = alloc<float4>(n);
     sb_.v0 = v0_;
+    sb_.nbr = alloc<uint32_t>(size_t(NBR_CAP) * n);
+    sb_.nbrCount = alloc<uint32_t>(n);
     pa_.x = alloc<float>(n); pa_.y = alloc<float>(n); pa_.acc = alloc<float>(n);
     pb_.x = alloc<float>(n); pb_.y = alloc<float>(n); pb_.acc = alloc<float>(n);
     ch_.assign = alloc<uint32_t>(n); ch_.pending = alloc<uint32_t>(n);

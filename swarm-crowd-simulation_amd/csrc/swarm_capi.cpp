@@ -194,6 +194,32 @@ int swarm_time_steps(swarm_ctx* c, int nsteps, double* ms) {
     if (!ms) { g_err = "null argument"; return SWARM_EINVAL; }
     return guarded([&] { *ms = c->sim->timeSteps(nsteps); });
 }
+int swarm_copy_probe(size_t bytes, int reps, double* ms) {
+    if (!ms || bytes < 16 || reps <= 0) { g_err = "invalid copy probe"; return SWARM_EINVAL; }
+    return guarded([&] {
+        void *a = nullptr, *b = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0, e1;
+        SWARM_HIP_CHECK(hipMalloc(&a, bytes));
+        SWARM_HIP_CHECK(hipMalloc(&b, bytes));
+        SWARM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        SWARM_HIP_CHECK(hipMemsetAsync(a, 0, bytes, st));
+        SWARM_HIP_CHECK(hipEventCreate(&e0));
+        SWARM_HIP_CHECK(hipEventCreate(&e1));
+        swarm::launchCopyProbe(st, a, b, bytes);   // warm-up
+        SWARM_HIP_CHECK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) swarm::launchCopyProbe(st, a, b, bytes);
+        SWARM_HIP_CHECK(hipEventRecord(e1, st));
+        SWARM_HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        SWARM_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        *ms = double(t) / reps;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+        (void)hipFree(a); (void)hipFree(b);
+    });
+}
+
 int swarm_get_stream(swarm_ctx* c, void** stream) {
     CTX_CHECK(c);
     if (!stream) { g_err = "null argument"; return SWARM_EINVAL; }

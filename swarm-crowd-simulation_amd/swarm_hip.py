@@ -105,6 +105,14 @@ def check(rc):
         raise RuntimeError(f"swarm error {rc}: {lib().swarm_last_error().decode()}")
 
 
+def copy_probe(nbytes, reps=10):
+    """Mean ms of one launch of a plain copy kernel of nbytes (read nbytes, write nbytes)."""
+    ms = C.c_double()
+    lib().swarm_copy_probe.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+    check(lib().swarm_copy_probe(nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
 def model_params(tau_s=0.5):
     tau, relax = C.c_float(), C.c_float()
     check(lib().swarm_model_params(tau_s, C.byref(tau), C.byref(relax)))
