@@ -30,7 +30,6 @@ extern "C" {
 
 typedef struct swarm_scene swarm_scene;
 typedef struct swarm_ctx swarm_ctx;
-typedef struct swarm_batch swarm_batch;
 
 /* ---------------- scene preparation (input tables; not part of the step) ---------------- */
 
@@ -149,6 +148,29 @@ int swarm_time_steps(swarm_ctx* c, int nsteps, double* ms);
 /* A plain device-to-device copy kernel of 'bytes' (reps launches, mean ms per launch): the measured copy
  * bandwidth and the calibration of HBM traffic counters. No counterpart in the reference (measurement aid). */
 int swarm_copy_probe(size_t bytes, int reps, double* ms);
+/* ---------------- the batched step ----------------
+ * No counterpart in the reference (its CrowdSim steps one venue; the batch is the ensemble form of
+ * CrowdSim::step, crowd_sim.cpp:316-412). Scenario s runs scene layouts[scen_layout[s]] with scen_agents[s]
+ * agents; every pass is one launch for all scenarios. The state arrays of swarm_upload / swarm_read_state /
+ * swarm_read_assign / swarm_read_evac are the scenarios' arrays concatenated in scenario order; grid keys and
+ * cell tables are over global cells (scenario s's cells follow scenario s-1's) and grid ids are global agent
+ * indices. Each scenario's results equal its own plain context's, bit for bit. swarm_set_drive is refused. */
+int swarm_batch_create(const swarm_scene* const* layouts, int n_layouts, const int* scen_layout, const int* scen_agents, int scenarios,
+                       const swarm_config* cfg, swarm_ctx** out);
+int swarm_num_scenarios(swarm_ctx* c);
+/* readChoiceCounters per scenario: 132 words each (queue[128], switches, forced, evaluations, eligible) */
+int swarm_read_choice_counters_all(swarm_ctx* c, uint32_t* out);
+/* the evacuation counters per scenario: 64 words each (exit e's count at e, out-of-domain at E) */
+int swarm_read_evac_counters(swarm_ctx* c, uint32_t* per_exit64);
+/* Launch shape of the physics passes (predict, projection, final): block 0 (default 128), 64, 128, 192 or 256
+ * threads; grid 0 (one slot per thread) or a persistent grid of that many blocks. Results do not depend on it.
+ * The reference fixes its workgroup size in the shaders (physics.comp:22, local_size_x = 128). */
+int swarm_set_physics_shape(swarm_ctx* c, int block, int grid);
+int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid);
+/* Times each candidate physics launch shape over 'steps' steps from the current state, keeps the fastest and
+ * restores the state; *best_ms: its mean ms per step. No counterpart in the reference. */
+int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);
+
 /* the context's HIP stream (hipStream_t), for events around swarm_step */
 int swarm_get_stream(swarm_ctx* c, void** stream);
 int swarm_num_cells(swarm_ctx* c);

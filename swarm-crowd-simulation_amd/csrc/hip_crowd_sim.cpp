@@ -69,20 +69,55 @@ template <class T> T* HipCrowdSim::alloc(size_t count) {
 
 HipCrowdSim::HipCrowdSim(const Scene& scene, int numAgents, const SimOptions& opt) : opt_(opt) {
     if (numAgents <= 0) throw std::invalid_argument("HipCrowdSim: agent count must be positive");
-    if (opt.projIters < 0) throw std::invalid_argument("HipCrowdSim: projection iterations must be >= 0");
-    if (scene.hashNX <= 0 || scene.hashNY <= 0 || scene.sdfNX < 2 || scene.sdfNY < 2 || scene.nav.resX <= 0 || scene.nav.resY <= 0)
-        throw std::invalid_argument("HipCrowdSim: invalid scene");
+    init({ &scene }, { 0 }, { numAgents }, false);
+}
+
+HipCrowdSim::HipCrowdSim(const std::vector<const Scene*>& layouts, const std::vector<int>& scenLayout, const std::vector<int>& scenAgents,
+                         const SimOptions& opt)
+    : opt_(opt) {
+    init(layouts, scenLayout, scenAgents, true);
+}
+
+void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vector<int>& scenLayout, const std::vector<int>& scenAgents,
+                       bool batched) {
+    if (opt_.projIters < 0) throw std::invalid_argument("HipCrowdSim: projection iterations must be >= 0");
+    if (layouts.empty() || scenLayout.empty() || scenLayout.size() != scenAgents.size())
+        throw std::invalid_argument("HipCrowdSim: a batch needs scenes and one layout and agent count per scenario");
+    for (const Scene* s : layouts)
+        if (!s || s->hashNX <= 0 || s->hashNY <= 0 || s->sdfNX < 2 || s->sdfNY < 2 || s->nav.resX <= 0 || s->nav.resY <= 0)
+            throw std::invalid_argument("HipCrowdSim: invalid scene");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("HipCrowdSim: no HIP device");
-    n_ = numAgents;
-    C_ = scene.hashNX * scene.hashNY;
+    batched_ = batched;
+    S_ = int(scenLayout.size());
+    const size_t nScen = static_cast<size_t>(S_);
+    std::vector<uint32_t> cellBase(nScen), agentBase(nScen), lay(nScen);
+    size_t nTotal = 0, cTotal = 0;
+    maxExits_ = 0;
+    for (int s = 0; s < S_; ++s) {
+        const int l = scenLayout[size_t(s)];
+        if (l < 0 || l >= int(layouts.size())) throw std::invalid_argument("HipCrowdSim: scenario layout out of range");
+        if (scenAgents[size_t(s)] <= 0) throw std::invalid_argument("HipCrowdSim: a scenario needs at least one agent");
+        lay[size_t(s)] = uint32_t(l);
+        cellBase[size_t(s)] = uint32_t(cTotal);
+        agentBase[size_t(s)] = uint32_t(nTotal);
+        cTotal += size_t(layouts[size_t(l)]->hashNX) * size_t(layouts[size_t(l)]->hashNY);
+        nTotal += size_t(scenAgents[size_t(s)]);
+        maxExits_ = std::max(maxExits_, layouts[size_t(l)]->E);
+    }
+    if (nTotal > size_t(INT32_MAX) / 2 || cTotal > size_t(UINT32_MAX) / 2) throw std::invalid_argument("HipCrowdSim: batch too large");
+    n_ = int(nTotal);
+    C_ = int(cTotal);
     nb_ = C_ + (n_ + HASH_BLOCK - 1) / HASH_BLOCK;
-    modelParamsFor(opt.tauS, mp_.tau, mp_.relaxStep);
-    mp_.obstacleGain = opt.obstacleGain;
-    mp_.projIters = opt.projIters;
+    exitStride_ = batched_ ? std::max(1, maxExits_) : CHOICE_QUEUE_STRIDE;
+    modelParamsFor(opt_.tauS, mp_.tau, mp_.relaxStep);
+    mp_.obstacleGain = opt_.obstacleGain;
+    mp_.projIters = opt_.projIters;
     SWARM_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    scene_.upload(scene);
+    scenes_.resize(layouts.size());
+    for (size_t l = 0; l < layouts.size(); ++l) scenes_[l].upload(*layouts[l]);
     const size_t n = size_t(n_);
+    const size_t S = size_t(S_);
     for (int h = 0; h < 2; ++h) {
         half_[h].x = alloc<float>(n); half_[h].y = alloc<float>(n); half_[h].vx = alloc<float>(n);
         half_[h].vy = alloc<float>(n); half_[h].spd = alloc<float>(n); half_[h].w = alloc<float>(n);
@@ -93,33 +128,69 @@ HipCrowdSim::HipCrowdSim(const Scene& scene, int numAgents, const SimOptions& op
     grid_.bcnt = alloc<uint32_t>(size_t(nb_)); grid_.bstart = alloc<uint32_t>(size_t(nb_) + 1);
     grid_.tileSum = alloc<uint32_t>(size_t((nb_ + SCAN_TILE - 1) / SCAN_TILE)); grid_.tmp = alloc<uint32_t>(n);
     sb_.sid = alloc<uint32_t>(n); sb_.skey = alloc<uint32_t>(n);
-    sb_.cellStart = alloc<uint32_t>(size_t(C_)); sb_.cellCount = alloc<uint32_t>(size_t(C_));
+    sb_.bstart = grid_.bstart;
     sb_.sx = alloc<float>(n); sb_.sy = alloc<float>(n); sb_.sr = alloc<float>(n);
     sb_.egx = alloc<float>(n); sb_.egy = alloc<float>(n);
     sb_.pw = alloc<float>(n); sb_.pvx = alloc<float>(n); sb_.pvy = alloc<float>(n);
     sb_.planes = alloc<float4>(n);
     sb_.v0 = v0_;
     sb_.nbr = alloc<uint32_t>(size_t(NBR_CAP) * n);
+    sb_.nbrD = alloc<float>(size_t(NBR_CAP) * n);
     sb_.nbrCount = alloc<uint32_t>(n);
-    pa_.x = alloc<float>(n); pa_.y = alloc<float>(n); pa_.acc = alloc<float>(n);
-    pb_.x = alloc<float>(n); pb_.y = alloc<float>(n); pb_.acc = alloc<float>(n);
+    pa_.p = alloc<float2>(n); pa_.acc = alloc<float>(n);
+    pb_.p = alloc<float2>(n); pb_.acc = alloc<float>(n);
     ch_.assign = alloc<uint32_t>(n); ch_.pending = alloc<uint32_t>(n);
-    ch_.queue = alloc<uint32_t>(128); ch_.counters = alloc<uint32_t>(4);
-    ch_.hist = alloc<uint32_t>(size_t(CHOICE_MAX_EXITS + 1) * CHOICE_GAIN_BINS);
-    ch_.threshold = alloc<uint32_t>(64);
-    ch_.ahead = alloc<uint32_t>(size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS);
-    ev_.step = alloc<int32_t>(n); ev_.exit = alloc<int32_t>(n); ev_.perExit = alloc<uint32_t>(64);
+    ch_.queue = alloc<uint32_t>(S * QUEUE_STRIDE); ch_.counters = alloc<uint32_t>(S * COUNTER_STRIDE);
+    ch_.hist = alloc<uint32_t>(S * size_t(exitStride_) * CHOICE_GAIN_BINS);
+    ch_.threshold = alloc<uint32_t>(S * THRESHOLD_STRIDE);
+    ch_.exitStride = exitStride_;
+    ch_.ahead = alloc<uint32_t>(S * 2u * size_t(exitStride_) * CHOICE_AHEAD_BINS);
+    ev_.step = alloc<int32_t>(n); ev_.exit = alloc<int32_t>(n); ev_.perExit = alloc<uint32_t>(S * PER_EXIT_STRIDE);
     ctrl_ = alloc<uint32_t>(CTRL_WORDS);
     const uint32_t sentinel = uint32_t(n_);
     SWARM_HIP_CHECK(hipMemcpyAsync(grid_.bstart + nb_, &sentinel, 4, hipMemcpyHostToDevice, stream_));
+    if (batched_) {   // the per-scenario tables
+        std::vector<SceneView> views;
+        for (const DeviceScene& d : scenes_) views.push_back(d.view);
+        std::vector<uint32_t> agentScen(n);
+        for (int s = 0; s < S_; ++s)
+            for (int i = 0; i < scenAgents[size_t(s)]; ++i) agentScen[size_t(agentBase[size_t(s)]) + size_t(i)] = uint32_t(s);
+        SceneView* dv = alloc<SceneView>(views.size());
+        uint32_t* dl = alloc<uint32_t>(S);
+        uint32_t* dc = alloc<uint32_t>(S);
+        uint32_t* da = alloc<uint32_t>(S);
+        uint32_t* das = alloc<uint32_t>(n);
+        SWARM_HIP_CHECK(hipMemcpyAsync(dv, views.data(), views.size() * sizeof(SceneView), hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(dl, lay.data(), S * 4, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(dc, cellBase.data(), S * 4, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(da, agentBase.data(), S * 4, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(das, agentScen.data(), n * 4, hipMemcpyHostToDevice, stream_));
+        bt_.views = dv; bt_.scenLayout = dl; bt_.cellBase = dc; bt_.agentBase = da; bt_.agentScen = das;
+        bt_.slotScen = alloc<uint32_t>(n);
+    }
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+StepLaunch HipCrowdSim::launchDesc() const {
+    StepLaunch L;
+    L.n = n_;
+    L.scenarios = S_;
+    L.batched = batched_;
+    L.s = scenes_[0].view;
+    L.bt = bt_;
+    L.mp = mp_;
+    L.maxExits = maxExits_;
+    L.cells = uint32_t(C_);
+    L.physBlock = physBlock_;
+    L.physGrid = physGrid_;
+    return L;
 }
 
 HipCrowdSim::~HipCrowdSim() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropGraph();
     for (void* p : bufs_) (void)hipFree(p);
-    scene_.release();
+    for (DeviceScene& d : scenes_) d.release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -131,23 +202,24 @@ void HipCrowdSim::dropGraph() {
 void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r,
                          const float* v0) {
     const size_t b = size_t(n_) * 4;
+    const size_t S = size_t(S_);
     restoreAgentState(x, y, vx, vy, w);
     SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r, b, hipMemcpyHostToDevice, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(v0_, v0, b, hipMemcpyHostToDevice, stream_));
     // a fresh exit choice (nav::ChoiceState::reset) and evacuation record
     std::vector<uint32_t> none(static_cast<size_t>(n_), CHOICE_NONE);
     std::vector<int32_t> minus(static_cast<size_t>(n_), -1);
-    std::vector<uint32_t> thr(64, uint32_t(CHOICE_GAIN_BINS));
+    std::vector<uint32_t> thr(S * THRESHOLD_STRIDE, uint32_t(CHOICE_GAIN_BINS));
     SWARM_HIP_CHECK(hipMemcpyAsync(ch_.assign, none.data(), b, hipMemcpyHostToDevice, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(ch_.pending, 0, b, stream_));
-    SWARM_HIP_CHECK(hipMemsetAsync(ch_.queue, 0, 128 * 4, stream_));
-    SWARM_HIP_CHECK(hipMemsetAsync(ch_.counters, 0, 4 * 4, stream_));
-    SWARM_HIP_CHECK(hipMemsetAsync(ch_.hist, 0, size_t(CHOICE_MAX_EXITS + 1) * CHOICE_GAIN_BINS * 4, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(ch_.threshold, thr.data(), 64 * 4, hipMemcpyHostToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemsetAsync(ch_.ahead, 0, size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.queue, 0, S * QUEUE_STRIDE * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.counters, 0, S * COUNTER_STRIDE * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.hist, 0, S * size_t(exitStride_) * CHOICE_GAIN_BINS * 4, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ch_.threshold, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ch_.ahead, 0, S * 2u * size_t(exitStride_) * CHOICE_AHEAD_BINS * 4, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(ev_.step, minus.data(), b, hipMemcpyHostToDevice, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(ev_.exit, minus.data(), b, hipMemcpyHostToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemsetAsync(ev_.perExit, 0, 64 * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(ev_.perExit, 0, S * PER_EXIT_STRIDE * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(ctrl_, 0, CTRL_WORDS * 4, stream_));
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
 }
@@ -170,6 +242,7 @@ void HipCrowdSim::restoreAgentState(const float* x, const float* y, const float*
 
 void HipCrowdSim::setDrive(const float* dx, const float* dy) {
     const bool on = dx && dy;
+    if (on && batched_) throw std::invalid_argument("setDrive: forced directions are a plain-step tool");
     if (on) {
         SWARM_HIP_CHECK(hipMemcpyAsync(driveX_, dx, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
         SWARM_HIP_CHECK(hipMemcpyAsync(driveY_, dy, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
@@ -185,6 +258,83 @@ void HipCrowdSim::setProjectionIterations(int iterations) {
     mp_.projIters = iterations;
 }
 
+void HipCrowdSim::setPhysicsShape(int block, int grid) {
+    if (!(block == 0 || block == 64 || block == 128 || block == 192 || block == 256) || grid < 0)
+        throw std::invalid_argument("setPhysicsShape: block 0, 64, 128, 192 or 256; grid >= 0");
+    if (block != physBlock_ || grid != physGrid_) dropGraph();
+    physBlock_ = block;
+    physGrid_ = grid;
+}
+
+std::vector<std::pair<void*, size_t>> HipCrowdSim::persistentState() const {
+    const size_t b = size_t(n_) * 4, S = size_t(S_);
+    std::vector<std::pair<void*, size_t>> v;
+    for (const AgentHalf& h : half_)
+        for (float* p : { h.x, h.y, h.vx, h.vy, h.spd, h.w }) v.emplace_back(p, b);
+    v.emplace_back(ch_.assign, b);
+    v.emplace_back(ch_.pending, b);
+    v.emplace_back(ch_.queue, S * QUEUE_STRIDE * 4);
+    v.emplace_back(ch_.counters, S * COUNTER_STRIDE * 4);
+    v.emplace_back(ch_.hist, S * size_t(exitStride_) * CHOICE_GAIN_BINS * 4);
+    v.emplace_back(ch_.threshold, S * THRESHOLD_STRIDE * 4);
+    v.emplace_back(ch_.ahead, S * 2u * size_t(exitStride_) * CHOICE_AHEAD_BINS * 4);
+    v.emplace_back(ev_.step, b);
+    v.emplace_back(ev_.exit, b);
+    v.emplace_back(ev_.perExit, S * PER_EXIT_STRIDE * 4);
+    v.emplace_back(ctrl_, CTRL_WORDS * 4);
+    return v;
+}
+
+double HipCrowdSim::autotune(int steps) {
+    if (steps <= 0) throw std::invalid_argument("autotune: steps must be positive");
+    int dev = 0, cus = 0;
+    SWARM_HIP_CHECK(hipGetDevice(&dev));
+    SWARM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) cus = 256;
+    // one slot per thread at three block sizes, and persistent grids of 4 and 8 blocks per CU
+    const std::pair<int, int> shapes[] = { { 64, 0 }, { 128, 0 }, { 256, 0 }, { 128, 8 * cus }, { 256, 4 * cus }, { 64, 8 * cus } };
+    const auto state = persistentState();
+    size_t total = 0;
+    for (const auto& s : state) total += s.second;
+    char* snap = nullptr;
+    SWARM_HIP_CHECK(hipMalloc(&snap, total));
+    auto copyAll = [&](bool save) {
+        size_t off = 0;
+        for (const auto& s : state) {
+            char* dst = save ? snap + off : static_cast<char*>(s.first);
+            const char* src = save ? static_cast<const char*>(s.first) : snap + off;
+            SWARM_HIP_CHECK(hipMemcpyAsync(dst, src, s.second, hipMemcpyDeviceToDevice, stream_));
+            off += s.second;
+        }
+    };
+    const int parity0 = parity_;
+    const bool timing0 = timing_;
+    timing_ = false;
+    double best = 0.0;
+    std::pair<int, int> bestShape = { physBlock_, physGrid_ };
+    try {
+        copyAll(true);
+        for (const auto& sh : shapes) {
+            setPhysicsShape(sh.first, sh.second);
+            step(2);   // graph build and warm-up
+            const double ms = timeSteps(steps) / double(steps);
+            if (best == 0.0 || ms < best) { best = ms; bestShape = sh; }
+            copyAll(false);
+            parity_ = parity0;
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipFree(snap);
+        timing_ = timing0;
+        throw;
+    }
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    SWARM_HIP_CHECK(hipFree(snap));
+    timing_ = timing0;
+    setPhysicsShape(bestShape.first, bestShape.second);
+    return best;
+}
+
 void HipCrowdSim::enqueueStep(int parity) {
     const AgentHalf rd = half_[parity], wr = half_[1 - parity];
     StepBuffers b = sb_;
@@ -196,11 +346,12 @@ void HipCrowdSim::enqueueStep(int parity) {
         for (hipEvent_t& e : ev) SWARM_HIP_CHECK(hipEventCreate(&e));
         SWARM_HIP_CHECK(hipEventRecord(ev[0], stream_));
     }
-    launchGridPasses(stream_, n_, scene_.view, rd, grid_, b, rad_, ctrl_);
+    const StepLaunch L = launchDesc();
+    launchGridPasses(stream_, L, rd, grid_, b, rad_, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[1], stream_));
-    launchChoice(stream_, n_, scene_.view, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
-    launchPhysics(stream_, n_, scene_.view, mp_, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
+    launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
     if (timing_) {
         SWARM_HIP_CHECK(hipEventRecord(ev[5], stream_));
         pendingStamps_.push_back(ev);
@@ -215,11 +366,7 @@ void HipCrowdSim::collectTimings() {
         for (int k = 0; k < 5; ++k) SWARM_HIP_CHECK(hipEventElapsedTime(&ms[k], ev[size_t(k)], ev[size_t(k) + 1]));
         float total = 0.0f;
         SWARM_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[5]));
-        timeSum_[0] += ms[0];   // hash + sort + cells + gather
-        timeSum_[1] += ms[1];   // exit choice
-        timeSum_[2] += ms[2];   // predict
-        timeSum_[3] += ms[3];   // projection iterations
-        timeSum_[4] += ms[4];   // final
+        for (int k = 0; k < 5; ++k) timeSum_[k] += ms[k];   // grid passes, choice, predict, projection, final
         timeSum_[5] += total;
         ++timedSteps_;
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -292,8 +439,9 @@ void HipCrowdSim::stepExitChoice() {
     StepBuffers b = sb_;
     b.avx = rd.vx;
     b.avy = rd.vy;
-    launchGridPasses(stream_, n_, scene_.view, rd, grid_, b, rad_, ctrl_);
-    launchChoice(stream_, n_, scene_.view, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    const StepLaunch L = launchDesc();
+    launchGridPasses(stream_, L, rd, grid_, b, rad_, ctrl_);
+    launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     launchStepIncrement(stream_, ctrl_);
     SWARM_HIP_CHECK(hipGetLastError());
 }
@@ -302,8 +450,13 @@ void HipCrowdSim::setChoiceState(uint32_t step, const uint32_t* assign, const ui
     SWARM_HIP_CHECK(hipMemcpyAsync(ctrl_ + CTRL_STEP, &step, 4, hipMemcpyHostToDevice, stream_));
     if (assign) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.assign, assign, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
     if (queue128) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.queue, queue128, 128 * 4, hipMemcpyHostToDevice, stream_));
-    if (ahead)
-        SWARM_HIP_CHECK(hipMemcpyAsync(ch_.ahead, ahead, size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS * 4, hipMemcpyHostToDevice, stream_));
+    if (ahead) {   // the harness's layout: parity x 64 exits x 1024 bins, into scenario 0's exit stride
+        for (int par = 0; par < 2; ++par)
+            SWARM_HIP_CHECK(hipMemcpyAsync(ch_.ahead + size_t(par) * size_t(exitStride_) * CHOICE_AHEAD_BINS,
+                                           ahead + size_t(par) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS,
+                                           size_t(std::min(exitStride_, CHOICE_QUEUE_STRIDE)) * CHOICE_AHEAD_BINS * 4, hipMemcpyHostToDevice,
+                                           stream_));
+    }
     sync();
 }
 
@@ -329,10 +482,15 @@ void HipCrowdSim::readGrid(uint32_t* key, uint32_t* id) {
 }
 
 void HipCrowdSim::readCells(uint32_t* start, uint32_t* count) {
-    const size_t b = size_t(C_) * 4;
-    if (start) SWARM_HIP_CHECK(hipMemcpyAsync(start, sb_.cellStart, b, hipMemcpyDeviceToHost, stream_));
-    if (count) SWARM_HIP_CHECK(hipMemcpyAsync(count, sb_.cellCount, b, hipMemcpyDeviceToHost, stream_));
+    // the reference's cell table (start 0 for an empty cell) from the bucket starts: cell c holds [bstart[c], bstart[c + 1])
+    std::vector<uint32_t> bs(static_cast<size_t>(C_) + 1);
+    SWARM_HIP_CHECK(hipMemcpyAsync(bs.data(), grid_.bstart, bs.size() * 4, hipMemcpyDeviceToHost, stream_));
     sync();
+    for (size_t c = 0; c < size_t(C_); ++c) {
+        const uint32_t cnt = bs[c + 1] - bs[c];
+        if (start) start[c] = cnt ? bs[c] : 0u;
+        if (count) count[c] = cnt;
+    }
 }
 
 void HipCrowdSim::readAssign(uint32_t* assign) {
@@ -346,11 +504,27 @@ void HipCrowdSim::readChoiceCounters(uint32_t* out) {
     sync();
 }
 
+void HipCrowdSim::readChoiceCountersAll(uint32_t* out) {
+    std::vector<uint32_t> q(size_t(S_) * QUEUE_STRIDE), c(size_t(S_) * COUNTER_STRIDE);
+    SWARM_HIP_CHECK(hipMemcpyAsync(q.data(), ch_.queue, q.size() * 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(c.data(), ch_.counters, c.size() * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    for (size_t s = 0; s < size_t(S_); ++s) {
+        std::memcpy(out + s * 132, q.data() + s * QUEUE_STRIDE, 128 * 4);
+        std::memcpy(out + s * 132 + 128, c.data() + s * COUNTER_STRIDE, 4 * 4);
+    }
+}
+
 void HipCrowdSim::readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit) {
     const size_t b = size_t(n_) * 4;
     if (step) SWARM_HIP_CHECK(hipMemcpyAsync(step, ev_.step, b, hipMemcpyDeviceToHost, stream_));
     if (exitId) SWARM_HIP_CHECK(hipMemcpyAsync(exitId, ev_.exit, b, hipMemcpyDeviceToHost, stream_));
-    if (perExit) SWARM_HIP_CHECK(hipMemcpyAsync(perExit, ev_.perExit, size_t(scene_.view.E + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    if (perExit) SWARM_HIP_CHECK(hipMemcpyAsync(perExit, ev_.perExit, size_t(scenes_[0].view.E + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readEvacCounters(uint32_t* perExit64) {
+    SWARM_HIP_CHECK(hipMemcpyAsync(perExit64, ev_.perExit, size_t(S_) * PER_EXIT_STRIDE * 4, hipMemcpyDeviceToHost, stream_));
     sync();
 }
 

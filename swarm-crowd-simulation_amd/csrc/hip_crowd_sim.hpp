@@ -2,12 +2,18 @@
 // (include/swarm/crowd_sim.hpp:53-194) and CpuCrowdSim (include/swarm/cpu_crowd_sim.hpp:80-134):
 // construct from a scene, upload a state, step, read the state, grid and cell table back. It owns every
 // device buffer and one HIP stream; readbacks synchronise that stream only.
+//
+// A batch: many independent scenarios (each one of a few scene layouts, with its own agents) stepped together,
+// one launch per pass for all of them. Agents are concatenated scenario by scenario; cell keys are global cell
+// indices (scenario-major), so the counting sort and the cell ranges are segmented by scenario; every scenario
+// keeps its own exit-choice state and evacuation counters; results are each scenario's plain results bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "crowd_kernels.h"
@@ -34,7 +40,11 @@ struct DeviceScene {
 
 class HipCrowdSim {
 public:
+    // the plain step of one scene
     HipCrowdSim(const Scene& scene, int numAgents, const SimOptions& opt);
+    // a batch: scenario s runs scene layouts[scenLayout[s]] with scenAgents[s] agents
+    HipCrowdSim(const std::vector<const Scene*>& layouts, const std::vector<int>& scenLayout, const std::vector<int>& scenAgents,
+                const SimOptions& opt);
     ~HipCrowdSim();
     HipCrowdSim(const HipCrowdSim&) = delete;
     HipCrowdSim& operator=(const HipCrowdSim&) = delete;
@@ -43,50 +53,68 @@ public:
     void upload(const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r, const float* v0);
     // The agent state only (both halves), as the reference's benchmark snapshot restore: the exit choice runs on
     void restoreAgentState(const float* x, const float* y, const float* vx, const float* vy, const float* w);
-    void setDrive(const float* dx, const float* dy);   // nullptr: the navigation table (CpuCrowdSim::setDriveTable)
+    void setDrive(const float* dx, const float* dy);   // nullptr: the navigation table (CpuCrowdSim::setDriveTable); plain only
     void step(int nsteps);
     // step(nsteps) between two HIP events on the context's stream; returns their elapsed ms (synchronises)
     double timeSteps(int nsteps);
     // The exit choice pass alone on the current state (CrowdSim::stepExitChoice); advances the step counter
     void stepExitChoice();
-    // Sets the exit choice's state (tools and tests: the reference's choice harness setters,
+    // Sets the exit choice's state of scenario 0 (tools and tests: the reference's choice harness setters,
     // tools/choice_check.hpp:88-100): step counter, assigned exits, queue counters (128), queue-ahead histograms
     // (2 x 64 x 1024); nullptr arrays are left as they are
     void setChoiceState(uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead);
     void sync();
     void setProjectionIterations(int iterations);
-
-    void readState(float* x, float* y, float* vx, float* vy, float* speed, float* w);
-    void readGrid(uint32_t* key, uint32_t* id);
-    void readCells(uint32_t* start, uint32_t* count);
-    void readAssign(uint32_t* assign);
-    void readChoiceCounters(uint32_t* out132);
-    void readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit);
-    void readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum);
+    // Launch shape of the physics passes: threads per block, blocks (0: one slot per thread; else a persistent grid)
+    void setPhysicsShape(int block, int grid);
+    int physicsBlock() const { return physBlock_; }
+    int physicsGrid() const { return physGrid_; }
+    // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, keeps
+    // the fastest and puts the state back as it was (a slot's result does not depend on the shape). Returns the
+    // fastest shape's mean ms per step.
+    double autotune(int steps);
 
     // Per-pass GPU timing (CrowdSim::setTimingEnabled / readTimings, crowd_sim.hpp:141-149): while enabled every
     // step runs eagerly with HIP events between its passes on the context's stream. readTimings: the mean ms per
-    // step of hash, sort (scan + scatter + in-bucket order + gather), exit choice, predict, projection (all
+    // step of the grid passes (hash, sort, cell ranges, gather), exit choice, predict, projection (all
     // PROJ_ITERATIONS - 1 launches), final, total, and the mean ms of one projection launch; returns the steps timed
     void setTimingEnabled(bool on);
     int readTimings(double* ms8);
 
+    void readState(float* x, float* y, float* vx, float* vy, float* speed, float* w);
+    void readGrid(uint32_t* key, uint32_t* id);
+    void readCells(uint32_t* start, uint32_t* count);       // the reference's cell table over the global cells
+    void readAssign(uint32_t* assign);
+    void readChoiceCounters(uint32_t* out132);               // scenario 0: queue[128], switches, forced, evaluations, eligible
+    void readChoiceCountersAll(uint32_t* out);               // every scenario: 132 words each
+    void readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit);   // perExit: scenario 0's E + 1 counters
+    void readEvacCounters(uint32_t* perExit64);              // every scenario: PER_EXIT_STRIDE words each
+    void readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum);
+
     int numAgents() const { return n_; }
     int numCells() const { return C_; }
-    int numExits() const { return scene_.view.E; }
+    int numScenarios() const { return S_; }
+    int numExits() const { return scenes_[0].view.E; }
     hipStream_t stream() const { return stream_; }
-    const SceneView& view() const { return scene_.view; }
+    const SceneView& view() const { return scenes_[0].view; }
 
 private:
+    void init(const std::vector<const Scene*>& layouts, const std::vector<int>& scenLayout, const std::vector<int>& scenAgents, bool batched);
     void enqueueStep(int parity);
     void buildGraph();
     void dropGraph();
+    StepLaunch launchDesc() const;
     template <class T> T* alloc(size_t count);
+    void collectTimings();
+    // every buffer a step writes that outlives the step (agent halves, exit choice, evacuation, control)
+    std::vector<std::pair<void*, size_t>> persistentState() const;
 
-    DeviceScene scene_;
+    std::vector<DeviceScene> scenes_;
     SimOptions opt_;
     ModelParams mp_{};
-    int n_ = 0, C_ = 0, nb_ = 0;
+    bool batched_ = false;
+    int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
+    int physBlock_ = 0, physGrid_ = 0;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};
@@ -98,17 +126,16 @@ private:
     ProjBuffers pa_{}, pb_{};
     ChoiceBuffers ch_{};
     EvacBuffers ev_{};
+    BatchView bt_{};
     uint32_t* ctrl_ = nullptr;
     int parity_ = 0;   // the half holding the current state
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;
-    // timing: events per stamp (hash, sort, choice, predict, project, final, end) per recorded step
+    // timing: events per stamp (start, sorted, choice, predict, projection, final) per recorded step
     bool timing_ = false;
-    std::vector<hipEvent_t> events_;
     std::vector<std::vector<hipEvent_t>> pendingStamps_;
     double timeSum_[8] = {};
     int timedSteps_ = 0;
-    void collectTimings();
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

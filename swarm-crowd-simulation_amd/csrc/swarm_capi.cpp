@@ -220,6 +220,56 @@ int swarm_copy_probe(size_t bytes, int reps, double* ms) {
     });
 }
 
+int swarm_batch_create(const swarm_scene* const* layouts, int n_layouts, const int* scen_layout, const int* scen_agents, int scenarios,
+                       const swarm_config* cfg, swarm_ctx** out) {
+    if (!layouts || n_layouts <= 0 || !scen_layout || !scen_agents || scenarios <= 0 || !out) {
+        g_err = "swarm_batch_create: null or empty argument";
+        return SWARM_EINVAL;
+    }
+    return guarded([&] {
+        swarm::SimOptions o;
+        if (cfg) {
+            o.tauS = cfg->tau_s; o.obstacleGain = cfg->obstacle_gain; o.projIters = cfg->proj_iters; o.useGraph = cfg->use_graph != 0;
+            if (!(o.tauS > 0.0f)) throw std::invalid_argument("swarm_batch_create: relaxation time must be positive");
+        }
+        std::vector<const swarm::Scene*> ls;
+        for (int l = 0; l < n_layouts; ++l) {
+            if (!layouts[l]) throw std::invalid_argument("swarm_batch_create: null scene");
+            ls.push_back(&layouts[l]->s);
+        }
+        const std::vector<int> sl(scen_layout, scen_layout + scenarios), sa(scen_agents, scen_agents + scenarios);
+        *out = new swarm_ctx{ new swarm::HipCrowdSim(ls, sl, sa, o) };
+    });
+}
+int swarm_num_scenarios(swarm_ctx* c) { CTX_CHECK(c); return c->sim->numScenarios(); }
+int swarm_read_choice_counters_all(swarm_ctx* c, uint32_t* out) {
+    CTX_CHECK(c);
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readChoiceCountersAll(out); });
+}
+int swarm_read_evac_counters(swarm_ctx* c, uint32_t* per_exit64) {
+    CTX_CHECK(c);
+    if (!per_exit64) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readEvacCounters(per_exit64); });
+}
+int swarm_set_physics_shape(swarm_ctx* c, int block, int grid) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setPhysicsShape(block, grid); });
+}
+int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid) {
+    CTX_CHECK(c);
+    if (block) *block = c->sim->physicsBlock();
+    if (grid) *grid = c->sim->physicsGrid();
+    return SWARM_OK;
+}
+int swarm_autotune(swarm_ctx* c, int steps, double* best_ms) {
+    CTX_CHECK(c);
+    return guarded([&] {
+        const double ms = c->sim->autotune(steps);
+        if (best_ms) *best_ms = ms;
+    });
+}
+
 int swarm_get_stream(swarm_ctx* c, void** stream) {
     CTX_CHECK(c);
     if (!stream) { g_err = "null argument"; return SWARM_EINVAL; }

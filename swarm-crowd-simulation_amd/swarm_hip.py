@@ -94,6 +94,13 @@ def lib():
     L.swarm_read_timings.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
     L.swarm_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.swarm_num_cells.argtypes = [vp]
+    L.swarm_batch_create.argtypes = [C.POINTER(vp), C.c_int, _i32p, _i32p, C.c_int, C.POINTER(Config), C.POINTER(vp)]
+    L.swarm_num_scenarios.argtypes = [vp]
+    L.swarm_read_choice_counters_all.argtypes = [vp, _u32p]
+    L.swarm_read_evac_counters.argtypes = [vp, _u32p]
+    L.swarm_set_physics_shape.argtypes = [vp, C.c_int, C.c_int]
+    L.swarm_get_physics_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.swarm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.swarm_destroy.argtypes = [vp]
     L.swarm_destroy.restype = None
     _lib = L
@@ -288,6 +295,20 @@ class Sim:
         check(lib().swarm_time_steps(self.h, k, C.byref(ms)))
         return ms.value
 
+    def set_physics_shape(self, block=0, grid=0):
+        check(lib().swarm_set_physics_shape(self.h, block, grid))
+
+    def physics_shape(self):
+        b, g = C.c_int(), C.c_int()
+        check(lib().swarm_get_physics_shape(self.h, C.byref(b), C.byref(g)))
+        return b.value, g.value
+
+    def autotune(self, steps=10):
+        """Keeps the fastest physics launch shape over `steps` steps from the current state (the state is restored)."""
+        ms = C.c_double()
+        check(lib().swarm_autotune(self.h, steps, C.byref(ms)))
+        return ms.value
+
     def stream(self):
         p = C.c_void_p()
         check(lib().swarm_get_stream(self.h, C.byref(p)))
@@ -303,3 +324,55 @@ class Sim:
             self.close()
         except Exception:
             pass
+
+
+class BatchSim(Sim):
+    """Many independent scenarios stepped together, one launch per pass: scenario s runs
+    layouts[scen_layout[s]] with scen_agents[s] agents. State arrays are the scenarios' arrays concatenated in
+    scenario order; grid keys are global cells and grid ids global agents."""
+
+    def __init__(self, layouts, scen_layout, scen_agents, tau_s=0.5, obstacle_gain=20.0, proj_iters=32, use_graph=True):
+        cfg = Config(tau_s, obstacle_gain, proj_iters, 1 if use_graph else 0)
+        hs = (C.c_void_p * len(layouts))(*[sc.h.value if isinstance(sc.h, C.c_void_p) else sc.h for sc in layouts])
+        sl = np.ascontiguousarray(scen_layout, np.int32)
+        sa = np.ascontiguousarray(scen_agents, np.int32)
+        h = C.c_void_p()
+        check(lib().swarm_batch_create(hs, len(layouts), sl, sa, len(sl), C.byref(cfg), C.byref(h)))
+        self.h, self.n = h, int(sa.sum())
+        self.layouts, self.scen_layout, self.scen_agents = list(layouts), sl, sa
+        self.scene = layouts[int(sl[0])]
+        self.cells = lib().swarm_num_cells(self.h)
+        self.scenarios = lib().swarm_num_scenarios(self.h)
+        self.agent_base = np.concatenate([[0], np.cumsum(sa)[:-1]]).astype(np.int64)
+        nc = [layouts[int(l)].info.hash_nx * layouts[int(l)].info.hash_ny for l in sl]
+        self.scen_cells = np.asarray(nc, np.int64)
+        self.cell_base = np.concatenate([[0], np.cumsum(nc)[:-1]]).astype(np.int64)
+
+    def choice_counters_all(self):
+        o = np.zeros(132 * self.scenarios, np.uint32)
+        check(lib().swarm_read_choice_counters_all(self.h, o))
+        return o.reshape(self.scenarios, 132)
+
+    def evac_counters(self):
+        o = np.zeros(64 * self.scenarios, np.uint32)
+        check(lib().swarm_read_evac_counters(self.h, o))
+        return o.reshape(self.scenarios, 64)
+
+
+BATCH_LAYOUTS = (LAYOUT_ROOM, LAYOUT_CORRIDOR, LAYOUT_BOWL_EIGHTH, LAYOUT_PILLAR)
+
+
+def batch_workload(scenarios=256, seed=1337, lo=1024, hi=16384, load_lo=0.3, load_hi=1.0, layouts=BATCH_LAYOUTS):
+    """The ensemble workload: scenario s runs layouts[s % len(layouts)] with N_s agents (log-uniform in [lo, hi],
+    the scenario's padded size), of which A_s = min(N_s, floor(load_s * capacity)) are placed (load_s ~ U[load_lo,
+    load_hi], placement seed seed + s) and the rest parked. Returns (scen_layout as indices into `layouts`, N_s,
+    A_s, the concatenated agent arrays)."""
+    rng = np.random.default_rng(seed)
+    n = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), scenarios))).astype(np.int64)
+    load = rng.uniform(load_lo, load_hi, scenarios)
+    scen_layout = np.arange(scenarios, dtype=np.int32) % len(layouts)
+    caps = [layout_capacity(l) for l in layouts]
+    active = np.array([min(int(n[s]), int(np.floor(load[s] * caps[scen_layout[s]]))) for s in range(scenarios)], np.int64)
+    parts = [place_agents(layouts[scen_layout[s]], int(active[s]), int(n[s]), seed + s) for s in range(scenarios)]
+    agents = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    return scen_layout, n.astype(np.int32), active, agents

@@ -46,3 +46,18 @@ def test_model_params_fold():
     tau, relax = sh.model_params(0.5)
     assert np.float32(tau) == np.float32(float.fromhex("0x1.d45d2cp-6"))     # sim_params.hpp TAU_RELAX
     assert np.float32(relax) == np.float32(float.fromhex("0x1.68de1cp-1"))   # RELAX_STEP
+
+
+def test_batch_create_rejects_bad_arguments():
+    L = sh.lib()
+    h = C.c_void_p()
+    empty = (C.c_void_p * 1)()
+    sl = np.zeros(1, np.int32)
+    sa = np.ones(1, np.int32)
+    assert L.swarm_batch_create(None, 1, sl, sa, 1, None, C.byref(h)) == -1
+    assert L.swarm_batch_create(empty, 0, sl, sa, 1, None, C.byref(h)) == -1
+    assert L.swarm_batch_create(empty, 1, sl, sa, 0, None, C.byref(h)) == -1
+    assert L.swarm_batch_create(empty, 1, sl, sa, 1, None, C.byref(h)) == -1   # a null scene
+    assert b"null scene" in L.swarm_last_error()
+    assert L.swarm_autotune(None, 3, None) == -1
+    assert L.swarm_set_physics_shape(None, 128, 0) == -1
