@@ -36,6 +36,19 @@ def step_bytes(n_total, cells, active, iters=32):
 
 
 PROJ_BYTES_PER_SLOT = 52   # one projection launch per active slot: key 4, x/y/acc 12, flag 4, radius 4, plane 16 read; x/y/acc 12 written
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "traffic_bowl_63424.md")
+
+
+def measured_traffic():
+    """HBM bytes per projection launch from the FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/
+    (each counter scaled by what it reads for a copy kernel of known size); None when absent."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch k_project:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene, a, steps):
@@ -58,6 +71,53 @@ def cpu_baseline(scene, a, steps):
                 ms_per_step=1000.0 * dt / steps)
 
 
+def batch_bench(steps, warmup, loop_steps):
+    """The ensemble (256 scenarios of the room, corridor, bowl-eighth and pillar floors, about 1.4 M agents) as one
+    batched step, against the same scenarios stepped by a loop over plain contexts (each scenario's steps, then a
+    synchronisation, one scenario after another)."""
+    scen_layout, totals, actives, agents = sh.batch_workload()
+    scenes = [sh.Scene.layout(l) for l in sh.BATCH_LAYOUTS]
+    b = sh.BatchSim(scenes, scen_layout, totals)
+    b.upload(agents)
+    tune_ms = b.autotune(5)
+    shape = b.physics_shape()
+    b.step(warmup)
+    b.sync()
+    _, _, s0 = b.control()
+    ms = b.time_steps(steps)
+    _, _, s1 = b.control()
+    batched = (s1 - s0) / (ms * 1e-3)
+    cells = int(b.cells)
+    b_step = step_bytes(int(totals.sum()), cells, (s1 - s0) / steps)
+    b.close()
+    plains = []
+    for s in range(len(totals)):
+        lo = int(np.sum(totals[:s]))
+        p = sh.Sim(scenes[scen_layout[s]], int(totals[s]))
+        p.upload({k: v[lo:lo + int(totals[s])] for k, v in agents.items()})
+        p.step(2)
+        plains.append(p)
+    for p in plains:
+        p.sync()
+    sum0 = sum(p.control()[2] for p in plains)
+    t0 = time.perf_counter()
+    for p in plains:
+        p.step(loop_steps)
+        p.sync()
+    dt = time.perf_counter() - t0
+    sum1 = sum(p.control()[2] for p in plains)
+    looped = (sum1 - sum0) / dt
+    for p in plains:
+        p.close()
+    return {"value": batched, "unit": "agent-steps/s", "ms_per_step": ms / steps, "steps": steps,
+            "scenarios": int(len(totals)), "agents_total": int(totals.sum()), "agents_active": int(actives.sum()),
+            "physics_shape": list(shape), "autotune_ms_per_step": tune_ms,
+            "step_algorithmic_bytes": b_step, "step_frac": b_step / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "looped_plain": {"value": looped, "unit": "agent-steps/s", "steps": loop_steps,
+                             "ms_per_round": 1000.0 * dt / loop_steps},
+            "speedup_vs_looped_plain": batched / looped}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +128,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true", default=True)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false")
+    ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--loop-steps", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
@@ -117,6 +180,14 @@ def main():
         copy_gbs = 2.0 * nb / (cms * 1e-3) / 1e9
         print(f"# copy probe: 1 GiB in {cms:.3f} ms = {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch:
+        sim.close()
+        batch = batch_bench(args.batch_steps, 5, args.loop_steps)
+        print(f"# batched ensemble: {batch['ms_per_step']:.3f} ms/step, {batch['value'] / 1e6:.1f} M agent-steps/s; "
+              f"looped plain {batch['looped_plain']['value'] / 1e6:.1f} M; x{batch['speedup_vs_looped_plain']:.2f}",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, a, args.cpu_steps)
@@ -139,11 +210,12 @@ def main():
                    "agents_active": ACTIVE, "agents_total": TOTAL, "hash_cells": cells, "proj_iterations": 32,
                    "exits": scene.info.E, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration)", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,
                      "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
+        "batched": batch,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line))

@@ -115,7 +115,7 @@ int swarm_restore_state(swarm_ctx* c, const float* x, const float* y, const floa
 int swarm_set_drive(swarm_ctx* c, const float* dx, const float* dy);
 /* CrowdSim::step (crowd_sim.hpp:91, crowd_sim.cpp:316-412), nsteps times; asynchronous */
 int swarm_step(swarm_ctx* c, int nsteps);
-/* CrowdSim::stepExitChoice / CpuCrowdSim::stepExitChoice (cpu_crowd_sim.hpp:104): the exit choice pass alone */
+/* CrowdSim::stepExitChoice / CpuCrowdSim::stepExitChoice (crowd_sim.hpp:116, cpu_crowd_sim.hpp:104): the exit choice pass alone */
 int swarm_step_exit_choice(swarm_ctx* c);
 /* The choice state, as the reference's choice harness sets it (tools/choice_check.hpp:88-100: setAssign,
  * setCounters, setAhead): step counter, assigned exits (n), queue counters (128), queue-ahead histograms
@@ -124,21 +124,27 @@ int swarm_set_choice_state(swarm_ctx* c, uint32_t step, const uint32_t* assign, 
 int swarm_sync(swarm_ctx* c);
 /* CrowdSim::setProjectionIterations (crowd_sim.hpp:125) */
 int swarm_set_proj_iters(swarm_ctx* c, int iterations);
-/* currentStateBuffer readback (crowd_sim.hpp:109; CpuCrowdSim::state) */
+/* currentStateBuffer readback (crowd_sim.hpp:132; CpuCrowdSim::state, cpu_crowd_sim.hpp:129) */
 int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w);
-/* gridBuffer readback (crowd_sim.hpp:115; CpuCrowdSim::grid): sorted (cell, agent) pairs of the last step's input */
+/* gridBuffer readback (crowd_sim.hpp:134; CpuCrowdSim::grid, cpu_crowd_sim.hpp:131): sorted (cell, agent) pairs of the last step's input */
 int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id);
-/* cellBuffer readback (crowd_sim.hpp:117; CpuCrowdSim::cells): start and count per cell */
+/* cellBuffer readback (crowd_sim.hpp:135; CpuCrowdSim::cells, cpu_crowd_sim.hpp:132): start and count per cell */
 int swarm_read_cells(swarm_ctx* c, uint32_t* start, uint32_t* count);
-/* assignBuffer readback (crowd_sim.hpp:121; CpuCrowdSim::assignedExits) */
+/* assignBuffer readback (crowd_sim.hpp:109; CpuCrowdSim::assignedExits, cpu_crowd_sim.hpp:100) */
 int swarm_read_assign(swarm_ctx* c, uint32_t* assign);
-/* readChoiceCounters (crowd_sim.hpp:133): queue[128], switches, forced, evaluations, eligible */
+/* readChoiceCounters (crowd_sim.hpp:115): queue[128], switches, forced, evaluations, eligible */
 int swarm_read_choice_counters(swarm_ctx* c, uint32_t* out132);
 /* Evacuation record (metrics_deposit.comp:96-104's events): per agent its step and exit (-1: none; E: domain), per exit counts (E + 1) */
 int swarm_read_evac(swarm_ctx* c, int32_t* step, int32_t* exit_id, uint32_t* per_exit);
+/* previousStateBuffer readback (crowd_sim.hpp:133): the state one step earlier (before the first step: the upload) */
+int swarm_read_prev_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w);
+/* CrowdSim::sortGrid (crowd_sim.hpp:99): the sort stage alone on keys of the caller's own. keys: n cell indices
+ * (< swarm_num_cells) or 0xFFFFFFFF (inactive); out: the sorted keys and the permutation (ascending (key, index),
+ * the stable counting sort's order). Leaves the context's own grid readback holding this sort. */
+int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, uint32_t* ids);
 /* step counter, active agents at the last step's input, running sum of active agents over steps */
 int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
-/* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:141-149): per-pass HIP events (steps then run
+/* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:142-143): per-pass HIP events (steps then run
  * without graph capture). ms8: mean ms per step of grid passes (hash, sort, cells, gather), exit choice,
  * predict, projection iterations, final, total; the mean ms of one projection launch; the steps timed */
 int swarm_set_timing(swarm_ctx* c, int on);

@@ -14,7 +14,10 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 constexpr int PHYS_BLOCK = 128;    // slots per block of the physics passes (default)
 constexpr int PHYS_MAX_BLOCK = 256;
+constexpr int XCDS = 8;            // accelerator complex dies of an MI355X (one L2 each)
 constexpr int NBR_CAP = 48;        // contact candidates per slot
+constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projection thread
+constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
 
 // control words on the device
@@ -105,6 +108,9 @@ struct StepLaunch {
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,
                       uint32_t* ctrl);
+// the sort stage alone on the keys already in g.key (CrowdSim::sortGrid)
+void launchSortKeys(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,
+                    uint32_t* ctrl);
 void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, const float* driveX,
                   const float* driveY, uint32_t* ctrl);
 // stamps (optional, 2 events): recorded after predict and after the projection iterations before the final pass

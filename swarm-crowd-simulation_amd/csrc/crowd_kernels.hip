@@ -1,10 +1,10 @@
 // The per-step pipeline kernels for gfx950.
 //
-//   hash      calc_grid.comp: agent -> cell key, a per-bucket atomic count (cells, then one inactive bucket
-//             per 256 agents)
+//   hash      calc_grid.comp: agent -> cell key and its offset in its bucket: an atomic count per cell; one
+//             inactive bucket per block of 256 agents, counted and ranked by the block's ballots
 //   scan      two passes over the bucket counts (tile sums, then tile-local exclusive scans): bucket starts
 //             (the cell ranges) and the counts cleared for the next step
-//   scatter   agent -> its bucket at start + atomic offset
+//   scatter   agent -> its bucket at start + offset
 //   fixgather each bucket's run sorted by agent id (runs are a few agents), so the permutation is the
 //             stable counting sort's: ascending (cell, agent), inactive agents at the tail in agent order
 //             (the reference's SortMode::Counting, cpu_crowd_sim.cpp:997-1006); the state is gathered into
@@ -41,22 +41,16 @@ __device__ inline void waveCountAdd(uint32_t* base, uint32_t key, bool valid) {
     }
 }
 
-__device__ inline uint32_t bucketOf(uint32_t key, uint32_t agent, uint32_t C) {
-    return key == INACTIVE_CELL ? C + agent / uint32_t(HASH_BLOCK) : key;
+// XCD-aware block numbering for the slot-order kernels. The GPU hands consecutive workgroups to its XCDs in
+// turn and each XCD has its own L2; renumbered, XCD x takes a contiguous 1/XCDS of the slots, so neighbours
+// (slots are sorted by cell) are mostly read through one L2. A bijection on [0, gridDim.x).
+__device__ inline int xcdBlock() {
+    const int g = int(gridDim.x), b = int(blockIdx.x), q = g / XCDS;
+    return b < q * XCDS ? (b % XCDS) * q + b / XCDS : b;
 }
 
-// f(k) for every slot k of the (2R + 1)^2 cells around local cell (cx, cy), in the ring scan's order
-template <class F>
-__device__ inline void forRing(const SceneView& s, const uint32_t* __restrict__ bstart, uint32_t cellBase, int cx, int cy, int R, F&& f) {
-    const int x0 = cx - R < 0 ? 0 : cx - R;
-    const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
-    for (int yy = -R; yy <= R; yy++) {
-        const int ny = cy + yy;
-        if (ny < 0 || ny >= s.hashNY) continue;
-        const uint32_t row = cellBase + uint32_t(ny * s.hashNX);
-        const uint32_t st = bstart[row + uint32_t(x0)], en = bstart[row + uint32_t(x1) + 1u];
-        for (uint32_t k = st; k < en; k++) f(k);
-    }
+__device__ inline uint32_t bucketOf(uint32_t key, uint32_t agent, uint32_t C) {
+    return key == INACTIVE_CELL ? C + agent / uint32_t(HASH_BLOCK) : key;
 }
 
 // the scene of an agent / slot: plain instances use the kernel's scene and no offsets
@@ -77,24 +71,52 @@ template <bool B> __device__ inline ScnRef scnOfSlot(const BatchView& bt, uint32
 }
 
 // ---------------- grid passes ----------------
+// Agent i's offset in its bucket. The block's inactive agents are its own bucket (C + block): their offsets are
+// their rank by agent id, from the block's ballots, and the bucket's count is one store. Cell buckets take an
+// atomic offset. Every thread of the block must call it.
+__device__ inline void bucketOffsets(int i, int n, uint32_t k, uint32_t* __restrict__ loc, uint32_t* __restrict__ bcnt, uint32_t C) {
+    __shared__ uint32_t waveInactive[HASH_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool inactive = i < n && k == INACTIVE_CELL;
+    const uint64_t m = __ballot(inactive);
+    if (lane == 0) waveInactive[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < HASH_BLOCK / 64; ++w) {
+        before += w < wave ? waveInactive[w] : 0u;
+        total += waveInactive[w];
+    }
+    if (inactive) loc[i] = before + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    else if (i < n) loc[i] = atomicAdd(&bcnt[k], 1u);
+    if (threadIdx.x == 0) bcnt[C + blockIdx.x] = total;
+}
+
 template <bool B>
 __global__ void __launch_bounds__(HASH_BLOCK) k_hash(int n, SceneView s0, BatchView bt, float* __restrict__ ax, float* __restrict__ ay,
                                                      float* __restrict__ aw, uint32_t* __restrict__ key, uint32_t* __restrict__ loc,
                                                      uint32_t* __restrict__ bcnt, uint32_t C) {
     const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    float x = ax[i], y = ay[i], w = aw[i];
-    if (isnan(x)) {   // calc_grid.comp resets a NaN position in the input state
-        x = 0.0f; y = 0.0f; w = 1.0f;
-        ax[i] = x; ay[i] = y; aw[i] = w;
-    }
     uint32_t k = INACTIVE_CELL;
-    if (w != 0.0f) {
-        const ScnRef r = scnOfAgent<B>(bt, uint32_t(i));
-        k = r.cellBase + hashCell(sceneOf<B>(s0, bt, r.sc), x, y);
+    if (i < n) {
+        float x = ax[i], y = ay[i], w = aw[i];
+        if (isnan(x)) {   // calc_grid.comp resets a NaN position in the input state
+            x = 0.0f; y = 0.0f; w = 1.0f;
+            ax[i] = x; ay[i] = y; aw[i] = w;
+        }
+        if (w != 0.0f) {
+            const ScnRef r = scnOfAgent<B>(bt, uint32_t(i));
+            k = r.cellBase + hashCell(sceneOf<B>(s0, bt, r.sc), x, y);
+        }
+        key[i] = k;
     }
-    key[i] = k;
-    loc[i] = atomicAdd(&bcnt[bucketOf(k, uint32_t(i), C)], 1u);
+    bucketOffsets(i, n, k, loc, bcnt, C);
+}
+
+// the sort stage alone on given keys (CrowdSim::sortGrid): the bucket offsets of key[i]
+__global__ void __launch_bounds__(HASH_BLOCK) k_key_count(int n, const uint32_t* __restrict__ key, uint32_t* __restrict__ loc,
+                                                          uint32_t* __restrict__ bcnt, uint32_t C) {
+    const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    bucketOffsets(i, n, i < n ? key[i] : INACTIVE_CELL, loc, bcnt, C);
 }
 
 // tile sums of the bucket counts
@@ -177,11 +199,13 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_fix_gather(int n, BatchView bt, 
     if (s >= n) return;
     const uint32_t id = tmp[s];
     const uint32_t k = key[id];
-    const uint32_t b = bucketOf(k, id, C);
-    const uint32_t st = bstart[b], en = bstart[b + 1];
-    uint32_t rank = 0;
-    for (uint32_t t = st; t < en; ++t) rank += tmp[t] < id ? 1u : 0u;
-    const uint32_t f = st + rank;
+    uint32_t f = uint32_t(s);   // an inactive bucket is in agent order already
+    if (k != INACTIVE_CELL) {
+        const uint32_t st = bstart[k], en = bstart[k + 1];
+        uint32_t rank = 0;
+        for (uint32_t t = st; t < en; ++t) rank += tmp[t] < id ? 1u : 0u;
+        f = st + rank;
+    }
     sid[f] = id;
     skey[f] = k;
     if constexpr (B) bt.slotScen[f] = bt.agentScen[id];
@@ -322,7 +346,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
                                                              const uint32_t* __restrict__ sid, const float* __restrict__ sx,
                                                              const float* __restrict__ sy, ChoiceBuffers c, float* __restrict__ egx,
                                                              float* __restrict__ egy, const uint32_t* __restrict__ ctrl) {
-    const int slot = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    const int slot = xcdBlock() * HASH_BLOCK + threadIdx.x;
     const bool live = slot < n && skey[slot] != INACTIVE_CELL;
     const uint32_t step = ctrl[CTRL_STEP];
     const bool ev = evalStep(step);
@@ -409,26 +433,47 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
     const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
     V2 sep{ 0.0f, 0.0f };
     float sAhead = 1e9f;   // the smallest gap (centre distance - contact distance) to an agent ahead
-    forRing(s, b.bstart, sr.cellBase, cx, cy, s.percRing, [&](uint32_t k) {
-        if (k == uint32_t(slot)) return;
-        const V2 offset = p - V2{ b.sx[k], b.sy[k] };
-        const float dxy2 = dot2(offset, offset);
-        if (dxy2 < R_PERC2) {
-            const uint32_t other = b.sid[k] - sr.agentBase;
-            const float D = ri + b.sr[k];
-            sep = sep + separationForce(offset, self, other, D + SEP_MARGIN);
-            const float ahead = -dot2(offset, eGoal);
-            const float lateral = fabsf(offset.x * eGoal.y - offset.y * eGoal.x);
-            const float dxy = sqrtf(dxy2);
-            const float gap = dxy - D;
-            if (ahead > 0.0f && lateral < D && gap < sAhead) {
-                const V2 eOther{ b.egx[k], b.egy[k] };
-                const float aheadOther = dot2(offset, eOther);
-                const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < D;
-                if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = gap;
+    const int R = s.percRing;
+    const int x0 = cx - R < 0 ? 0 : cx - R;
+    const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
+    for (int yy = -R; yy <= R; yy++) {
+        const int ny = cy + yy;
+        if (ny < 0 || ny >= s.hashNY) continue;
+        const uint32_t row = sr.cellBase + uint32_t(ny * s.hashNX);
+        const uint32_t st = b.bstart[row + uint32_t(x0)], en = b.bstart[row + uint32_t(x1) + 1u];
+        // the row's slots in chunks: a chunk's loads are in flight together; the sums keep the ring order
+        for (uint32_t k0 = st; k0 < en; k0 += PRED_CHUNK) {
+            float qx[PRED_CHUNK], qy[PRED_CHUNK], qr[PRED_CHUNK];
+#pragma unroll
+            for (int u = 0; u < PRED_CHUNK; ++u) {
+                const uint32_t k = k0 + uint32_t(u) < en ? k0 + uint32_t(u) : en - 1u;
+                qx[u] = b.sx[k]; qy[u] = b.sy[k]; qr[u] = b.sr[k];
+            }
+#pragma unroll
+            for (int u = 0; u < PRED_CHUNK; ++u) {
+                const uint32_t k = k0 + uint32_t(u);
+                if (k >= en) break;
+                if (k == uint32_t(slot)) continue;
+                const V2 offset = p - V2{ qx[u], qy[u] };
+                const float dxy2 = dot2(offset, offset);
+                if (dxy2 < R_PERC2) {
+                    const uint32_t other = b.sid[k] - sr.agentBase;
+                    const float D = ri + qr[u];
+                    sep = sep + separationForce(offset, self, other, D + SEP_MARGIN);
+                    const float ahead = -dot2(offset, eGoal);
+                    const float lateral = fabsf(offset.x * eGoal.y - offset.y * eGoal.x);
+                    const float dxy = sqrtf(dxy2);
+                    const float gap = dxy - D;
+                    if (ahead > 0.0f && lateral < D && gap < sAhead) {
+                        const V2 eOther{ b.egx[k], b.egy[k] };
+                        const float aheadOther = dot2(offset, eOther);
+                        const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < D;
+                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = gap;
+                    }
+                }
             }
         }
-    });
+    }
     const float vFree = stairFactor(s, p) * b.v0[agent];
     float vc = sAhead * HEADWAY_RATE;
     vc = vc < 0.0f ? 0.0f : (vFree < vc ? vFree : vc);
@@ -462,7 +507,7 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
 // the physics kernels loop over slots with the grid's stride: one slot per thread, or a persistent grid
 template <bool B>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers out) {
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) predictSlot<B>(slot, s0, bt, mp, b, out);
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) predictSlot<B>(slot, s0, bt, mp, b, out);
 }
 
 // ---------------- PHYSICS_PASS 2: the Jacobi projection ----------------
@@ -512,27 +557,47 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
     uint32_t cnt = 0;
     const size_t stride = size_t(n);
     const float Pf = float(P);
-    forRing(s, b.bstart, sr.cellBase, cx, cy, s.projRing, [&](uint32_t k) {
-        if (k == uint32_t(slot) || b.pw[k] == 0.0f) return;
-        const float2 op = in.p[k];
-        const V2 offset = V2{ bp.x, bp.y } - V2{ op.x, op.y };
-        const float D = ri + b.sr[k];
-        const float dxy2 = dot2(offset, offset);
-        if (dxy2 < contact2(D)) {
-            corr = corr + overlapCorrection(offset, dxy2, D, self, b.sid, k, sr.agentBase);
-            ++nContacts;
-        }
-        if (collect) {
-            const float reach = (D + Pf * (PROJ_MAX_SHIFT_RADII * D)) * 1.001f + 1e-6f;
-            if (dxy2 < reach * reach) {
-                if (cnt < uint32_t(NBR_CAP)) {
-                    b.nbr[size_t(cnt) * stride + size_t(slot)] = k;
-                    b.nbrD[size_t(cnt) * stride + size_t(slot)] = D;
+    const int R = s.projRing;
+    const int x0 = cx - R < 0 ? 0 : cx - R;
+    const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
+    for (int yy = -R; yy <= R; yy++) {
+        const int ny = cy + yy;
+        if (ny < 0 || ny >= s.hashNY) continue;
+        const uint32_t row = sr.cellBase + uint32_t(ny * s.hashNX);
+        const uint32_t st = b.bstart[row + uint32_t(x0)], en = b.bstart[row + uint32_t(x1) + 1u];
+        for (uint32_t k0 = st; k0 < en; k0 += PRED_CHUNK) {   // chunks of loads in flight together, ring order kept
+            float2 qp[PRED_CHUNK];
+            float qw[PRED_CHUNK], qr[PRED_CHUNK];
+#pragma unroll
+            for (int u = 0; u < PRED_CHUNK; ++u) {
+                const uint32_t k = k0 + uint32_t(u) < en ? k0 + uint32_t(u) : en - 1u;
+                qp[u] = in.p[k]; qw[u] = b.pw[k]; qr[u] = b.sr[k];
+            }
+#pragma unroll
+            for (int u = 0; u < PRED_CHUNK; ++u) {
+                const uint32_t k = k0 + uint32_t(u);
+                if (k >= en) break;
+                if (k == uint32_t(slot) || qw[u] == 0.0f) continue;
+                const V2 offset = V2{ bp.x, bp.y } - V2{ qp[u].x, qp[u].y };
+                const float D = ri + qr[u];
+                const float dxy2 = dot2(offset, offset);
+                if (dxy2 < contact2(D)) {
+                    corr = corr + overlapCorrection(offset, dxy2, D, self, b.sid, k, sr.agentBase);
+                    ++nContacts;
                 }
-                ++cnt;
+                if (collect) {
+                    const float reach = (D + Pf * (PROJ_MAX_SHIFT_RADII * D)) * 1.001f + 1e-6f;
+                    if (dxy2 < reach * reach) {
+                        if (cnt < uint32_t(NBR_CAP)) {
+                            b.nbr[size_t(cnt) * stride + size_t(slot)] = k;
+                            b.nbrD[size_t(cnt) * stride + size_t(slot)] = D;
+                        }
+                        ++cnt;
+                    }
+                }
             }
         }
-    });
+    }
     if (collect) b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
     finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
 }
@@ -547,15 +612,30 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     const size_t stride = size_t(n);
-    for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t k = b.nbr[size_t(j) * stride + size_t(slot)];
-        const float D = b.nbrD[size_t(j) * stride + size_t(slot)];
-        const float2 op = in.p[k];
-        const V2 offset = V2{ bp.x, bp.y } - V2{ op.x, op.y };
-        const float dxy2 = dot2(offset, offset);
-        if (dxy2 < contact2(D)) {
-            corr = corr + overlapCorrection(offset, dxy2, D, self, b.sid, k, sr.agentBase);
-            ++nContacts;
+    // candidates in chunks: the chunk's index and position loads are all in flight before the first use; the
+    // sum still runs in candidate order
+    for (uint32_t j0 = 0; j0 < cnt; j0 += PROJ_CHUNK) {
+        uint32_t kk[PROJ_CHUNK];
+        float dd[PROJ_CHUNK];
+        float2 pp[PROJ_CHUNK];
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) {
+            const bool in_ = j0 + uint32_t(u) < cnt;
+            const size_t at = size_t(j0 + uint32_t(u)) * stride + size_t(slot);
+            kk[u] = in_ ? b.nbr[at] : uint32_t(slot);
+            dd[u] = in_ ? b.nbrD[at] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) pp[u] = in.p[kk[u]];
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) {
+            if (j0 + uint32_t(u) >= cnt) break;
+            const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
+            const float dxy2 = dot2(offset, offset);
+            if (dxy2 < contact2(dd[u])) {
+                corr = corr + overlapCorrection(offset, dxy2, dd[u], self, b.sid, kk[u], sr.agentBase);
+                ++nContacts;
+            }
         }
     }
     finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
@@ -565,7 +645,7 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
 template <bool B>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
                                                             ProjBuffers out, int P, int first) {
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
         const uint32_t key = b.skey[slot];
         if (key == INACTIVE_CELL) continue;
         float x, y, acc;
@@ -588,7 +668,7 @@ template <bool B>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers in,
                                                           int useList, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* __restrict__ ctrl) {
     const uint32_t step = ctrl[CTRL_STEP];
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
         const uint32_t key = b.skey[slot];
         const uint32_t agent = b.sid[slot];
         if (key == INACTIVE_CELL) {
@@ -669,6 +749,24 @@ static void gridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const 
     k_scatter<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bstart, g.tmp, C);
     k_fix_gather<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.key, g.tmp, g.bstart, C, b.sid, b.skey, rd.x, rd.y, rad,
                                                                    b.sx, b.sy, b.sr);
+}
+
+void launchSortKeys(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,
+                    uint32_t* ctrl) {
+    const int n = L.n;
+    const uint32_t C = L.cells;
+    const uint32_t nb = C + uint32_t(blocks(n, HASH_BLOCK));
+    const int tiles = blocks(int(nb), SCAN_TILE);
+    k_key_count<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bcnt, C);
+    k_scan_reduce<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, g.tileSum);
+    k_scan_apply<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, C, g.tileSum, g.bstart, ctrl);
+    k_scatter<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bstart, g.tmp, C);
+    if (L.batched)
+        k_fix_gather<true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.key, g.tmp, g.bstart, C, b.sid, b.skey, rd.x, rd.y, rad,
+                                                                          b.sx, b.sy, b.sr);
+    else
+        k_fix_gather<false><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.key, g.tmp, g.bstart, C, b.sid, b.skey, rd.x, rd.y,
+                                                                           rad, b.sx, b.sy, b.sr);
 }
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,

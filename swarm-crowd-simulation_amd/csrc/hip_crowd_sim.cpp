@@ -84,8 +84,9 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     if (layouts.empty() || scenLayout.empty() || scenLayout.size() != scenAgents.size())
         throw std::invalid_argument("HipCrowdSim: a batch needs scenes and one layout and agent count per scenario");
     for (const Scene* s : layouts)
-        if (!s || s->hashNX <= 0 || s->hashNY <= 0 || s->sdfNX < 2 || s->sdfNY < 2 || s->nav.resX <= 0 || s->nav.resY <= 0)
-            throw std::invalid_argument("HipCrowdSim: invalid scene");
+        if (!s || s->hashNX <= 0 || s->hashNY <= 0 || s->sdfNX < 2 || s->sdfNY < 2 || s->nav.resX <= 0 || s->nav.resY <= 0 || s->E < 0 ||
+            s->E >= PER_EXIT_STRIDE)
+            throw std::invalid_argument("HipCrowdSim: invalid scene (grids, or more than 63 exits)");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("HipCrowdSim: no HIP device");
     batched_ = batched;
@@ -472,6 +473,29 @@ void HipCrowdSim::readState(float* x, float* y, float* vx, float* vy, float* spe
     if (speed) SWARM_HIP_CHECK(hipMemcpyAsync(speed, h.spd, b, hipMemcpyDeviceToHost, stream_));
     if (w) SWARM_HIP_CHECK(hipMemcpyAsync(w, h.w, b, hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+void HipCrowdSim::readPrevState(float* x, float* y, float* vx, float* vy, float* speed, float* w) {
+    const AgentHalf h = half_[1 - parity_];
+    const size_t b = size_t(n_) * 4;
+    if (x) SWARM_HIP_CHECK(hipMemcpyAsync(x, h.x, b, hipMemcpyDeviceToHost, stream_));
+    if (y) SWARM_HIP_CHECK(hipMemcpyAsync(y, h.y, b, hipMemcpyDeviceToHost, stream_));
+    if (vx) SWARM_HIP_CHECK(hipMemcpyAsync(vx, h.vx, b, hipMemcpyDeviceToHost, stream_));
+    if (vy) SWARM_HIP_CHECK(hipMemcpyAsync(vy, h.vy, b, hipMemcpyDeviceToHost, stream_));
+    if (speed) SWARM_HIP_CHECK(hipMemcpyAsync(speed, h.spd, b, hipMemcpyDeviceToHost, stream_));
+    if (w) SWARM_HIP_CHECK(hipMemcpyAsync(w, h.w, b, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t* ids) {
+    if (!keys) throw std::invalid_argument("sortGrid: null keys");
+    for (size_t i = 0; i < size_t(n_); ++i)
+        if (keys[i] != INACTIVE_CELL && keys[i] >= uint32_t(C_)) throw std::invalid_argument("sortGrid: a key is neither a cell nor inactive");
+    SWARM_HIP_CHECK(hipMemcpyAsync(grid_.key, keys, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
+    StepBuffers b = sb_;
+    launchSortKeys(stream_, launchDesc(), half_[parity_], grid_, b, rad_, ctrl_);
+    SWARM_HIP_CHECK(hipGetLastError());
+    readGrid(sortedKeys, ids);
 }
 
 void HipCrowdSim::readGrid(uint32_t* key, uint32_t* id) {

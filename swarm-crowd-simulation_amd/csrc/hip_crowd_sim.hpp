@@ -82,6 +82,10 @@ public:
     int readTimings(double* ms8);
 
     void readState(float* x, float* y, float* vx, float* vy, float* speed, float* w);
+    void readPrevState(float* x, float* y, float* vx, float* vy, float* speed, float* w);   // previousStateBuffer
+    // The sort stage alone on the caller's keys (CrowdSim::sortGrid): n keys, each a cell (< numCells) or
+    // INACTIVE_CELL; returns the sorted keys and the permutation (the grid readback then holds this sort)
+    void sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t* ids);
     void readGrid(uint32_t* key, uint32_t* id);
     void readCells(uint32_t* start, uint32_t* count);       // the reference's cell table over the global cells
     void readAssign(uint32_t* assign);

@@ -171,6 +171,15 @@ int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, flo
     CTX_CHECK(c);
     return guarded([&] { c->sim->readState(x, y, vx, vy, speed, w); });
 }
+int swarm_read_prev_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readPrevState(x, y, vx, vy, speed, w); });
+}
+int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, uint32_t* ids) {
+    CTX_CHECK(c);
+    if (!keys) { g_err = "null keys"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->sortGrid(keys, sorted_keys, ids); });
+}
 int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id) { CTX_CHECK(c); return guarded([&] { c->sim->readGrid(key, id); }); }
 int swarm_read_cells(swarm_ctx* c, uint32_t* start, uint32_t* count) { CTX_CHECK(c); return guarded([&] { c->sim->readCells(start, count); }); }
 int swarm_read_assign(swarm_ctx* c, uint32_t* a) { CTX_CHECK(c); return guarded([&] { c->sim->readAssign(a); }); }

@@ -84,6 +84,8 @@ def lib():
     L.swarm_set_proj_iters.argtypes = [vp, C.c_int]
     L.swarm_read_state.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.swarm_read_grid.argtypes = [vp, _u32p, _u32p]
+    L.swarm_read_prev_state.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.swarm_sort_grid.argtypes = [vp, _u32p, _u32p, _u32p]
     L.swarm_read_cells.argtypes = [vp, _u32p, _u32p]
     L.swarm_read_assign.argtypes = [vp, _u32p]
     L.swarm_read_choice_counters.argtypes = [vp, _u32p]
@@ -249,6 +251,18 @@ class Sim:
         s = {k: np.zeros(self.n, np.float32) for k in ("x", "y", "vx", "vy", "speed", "w")}
         check(lib().swarm_read_state(self.h, s["x"], s["y"], s["vx"], s["vy"], s["speed"], s["w"]))
         return s
+
+    def prev_state(self):
+        s = {k: np.zeros(self.n, np.float32) for k in ("x", "y", "vx", "vy", "speed", "w")}
+        check(lib().swarm_read_prev_state(self.h, s["x"], s["y"], s["vx"], s["vy"], s["speed"], s["w"]))
+        return s
+
+    def sort_grid(self, keys):
+        """The sort stage alone on the given keys (cells or INACTIVE_CELL): (sorted keys, permutation)."""
+        k = np.ascontiguousarray(keys, np.uint32)
+        sk, ids = np.zeros(self.n, np.uint32), np.zeros(self.n, np.uint32)
+        check(lib().swarm_sort_grid(self.h, k, sk, ids))
+        return sk, ids
 
     def grid(self):
         key, idx = np.zeros(self.n, np.uint32), np.zeros(self.n, np.uint32)

@@ -149,3 +149,40 @@ def test_graph_and_eager_identical():
     for f in FIELDS:
         assert_bits(x1[f], x2[f], f"graph vs eager {f}")
     assert np.array_equal(s1.assign(), s2.assign())
+
+
+def test_sort_grid_on_given_keys():
+    """CrowdSim::sortGrid: the sort stage alone on keys of the caller's own, against a stable sort by (key, index)."""
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    n = 5000
+    sim = sh.Sim(scene, n)
+    sim.upload(sh.place_agents(sh.LAYOUT_ROOM, 100, n, 1))
+    rng = np.random.default_rng(7)
+    for dense in (3, 200, sim.cells):
+        keys = rng.integers(0, dense, n).astype(np.uint32)
+        keys[rng.random(n) < 0.1] = sh.INACTIVE_CELL
+        sk, ids = sim.sort_grid(keys)
+        order = np.argsort(keys.astype(np.uint64), kind="stable")
+        assert np.array_equal(ids, order.astype(np.uint32)), f"permutation ({dense} distinct keys)"
+        assert np.array_equal(sk, keys[order]), "sorted keys"
+        st, cn = sim.cell_table()
+        act = keys[keys != sh.INACTIVE_CELL]
+        assert np.array_equal(cn, np.bincount(act, minlength=sim.cells).astype(np.uint32)), "cell counts"
+    bad = np.full(n, sim.cells, np.uint32)
+    with pytest.raises(RuntimeError):
+        sim.sort_grid(bad)
+
+
+def test_previous_state_is_last_input():
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, 1000, 1024, 4)
+    sim = sh.Sim(scene, 1024)
+    sim.upload(a)
+    p0 = sim.prev_state()
+    assert_bits(p0["x"], a["x"], "before the first step: the upload")
+    for k in range(3):
+        before = sim.state()
+        sim.step(1)
+        prev = sim.prev_state()
+        for f in FIELDS:
+            assert_bits(prev[f], before[f], f"previous state after step {k + 1} {f}")
