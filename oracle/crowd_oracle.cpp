@@ -907,3 +907,10 @@ int oracle_constants(float* out, int cap) {
 }
 
 } // extern "C"
+
+// the stacked-pair direction of two agents (pairDirection), for the adjoint's FP64 reference
+extern "C" void oracle_pair_direction(uint32_t self, uint32_t other, float* dx, float* dy) {
+    const V2 d = pairDirection(self, other);
+    *dx = d.x;
+    *dy = d.y;
+}

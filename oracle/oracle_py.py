@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
@@ -174,3 +175,64 @@ def constants():
     out = np.zeros(64, np.float32)
     n = lib().oracle_constants(out, 64)
     return out[:n]
+
+
+class AdjRef:
+    """FP64 reference of the adjoint's frozen-decision model (forward-mode derivatives and finite differences)."""
+
+    def __init__(self, info, tables, n, P, K, overlap_weight, a, tau_s, radius_scale, gain):
+        L = lib()
+        if not getattr(L, "_adjref_sigs", False):
+            vp = C.c_void_p
+            L.adjref_create.restype = vp
+            L.adjref_create.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_int, C.c_int, C.c_float] + [_f32p] * 7 + [C.c_float] * 3
+            L.adjref_destroy.argtypes = [vp]
+            L.adjref_add_step.argtypes = [vp, C.c_int, vp]
+            L.adjref_set_exit.argtypes = [vp, vp]
+            L.adjref_loss.restype = C.c_double
+            L.adjref_loss.argtypes = [vp, vp, C.c_double, C.POINTER(C.c_double)]
+            L.adjref_directional.argtypes = [vp, C.c_int, _f64p, _f64p, C.POINTER(C.c_double), C.c_int]
+            L._adjref_sigs = True
+        self._tables = tables
+        self._scene = scene_struct(info, tables)
+        f = lambda k: np.ascontiguousarray(a[k], np.float32)
+        self.n = n
+        self.h = L.adjref_create(C.byref(self._scene), n, P, K, overlap_weight, f("x"), f("y"), f("vx"), f("vy"), f("w"), f("r"),
+                                 f("v0"), tau_s, radius_scale, gain)
+
+    def add_step(self, step, tape_view):
+        assert lib().adjref_add_step(self.h, step, C.addressof(tape_view)) == 0
+
+    def set_exit(self, exit_recs):
+        lib().adjref_set_exit(self.h, C.addressof(exit_recs))
+
+    def loss(self, direction=None, eps=0.0):
+        err = C.c_double()
+        d = None if direction is None else np.ascontiguousarray(direction, np.float64)
+        v = lib().adjref_loss(self.h, None if d is None else d.ctypes.data, eps, C.byref(err))
+        return v, err.value
+
+    def step_errors(self, K):
+        e, a = np.zeros(K, np.float64), np.zeros(K, np.int32)
+        lib().adjref_step_errors.argtypes = [C.c_void_p, _f64p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        lib().adjref_step_errors(self.h, e, a)
+        return e, a
+
+    def directional(self, dirs, anchored=False):
+        """Directional derivatives (forward mode); anchored: along the tape's FP32 trajectory."""
+        dirs = np.ascontiguousarray(dirs, np.float64)
+        out = np.zeros(dirs.shape[0], np.float64)
+        loss = C.c_double()
+        lib().adjref_directional(self.h, dirs.shape[0], dirs.reshape(-1), out, C.byref(loss), 1 if anchored else 0)
+        return out, loss.value
+
+    def close(self):
+        if self.h:
+            lib().adjref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -231,7 +231,29 @@ __device__ inline float newtonStep(float t, G3 g, V2 u) {
     const float du = g.gx * u.x + g.gy * u.y;
     return du < 0.0f ? t - g.w / du : t;
 }
-__device__ inline float wallContact(const SceneView& s, V2 np, float fc, V2 u, float fr, float reach, float recastTol, V2& away) {
+// ---- wall contacts, with an optional recorder ----
+// A recorder sees every push of resolveStatic / clearWalls and the wall plane's source, so that a taped forward
+// (the adjoint's) is the production step bit for bit: NoRec compiles to nothing. A clearance measurement is kept
+// as its local linear model: the clearance t0 measured from x0 along the ray u, and the distance field's gradient
+// g at the hit point, so that t(x) = t0 - g.(x - x0) / (g.u) (exact for a flat wall).
+struct ContactRec {
+    V2 u, g, x0;
+    float t0;
+};
+struct PlaneRec {
+    int kind;          // 0 none, 1 a push (offset n.np), 2 a contact beyond the radius (offset n.np - (t - r)), 3 a probe
+    V2 n;
+    ContactRec c;
+};
+struct NoRec {
+    static constexpr bool on = false;
+    __device__ void push(V2, const ContactRec&, bool) {}
+    __device__ void sweep(float, V2, bool) {}
+};
+
+template <bool REC>
+__device__ inline float wallContact(const SceneView& s, V2 np, float fc, V2 u, float fr, float reach, float recastTol, V2& away,
+                                    ContactRec* info = nullptr) {
     float br[4] = { 0.0f, fc, reach, fr };
     float t = reach * fc / (fc - fr);
     G3 g{ 0.0f, 0.0f, 0.0f };
@@ -251,12 +273,21 @@ __device__ inline float wallContact(const SceneView& s, V2 np, float fc, V2 u, f
         t3 = newtonStep(t2, g2, u2);
     }
     const bool ok = (g2.gx * u2.x + g2.gy * u2.y) < 0.0f && fabsf(t3 - t2) < recastTol && t3 > 0.0f;
-    if (ok && t3 < e) { e = t3; away = nn; }
+    const bool recast = ok && t3 < e;
+    if (recast) { e = t3; away = nn; }
+    if constexpr (REC) {
+        const V2 ray = recast ? u2 : u;
+        const G3 gh = mapExactGrad(s, np + ray * e);
+        *info = ContactRec{ ray, V2{ gh.gx, gh.gy }, np, e };
+    }
     return e;
 }
-__device__ inline bool clearWalls(const SceneView& s, V2& np, V2& nv, float rad, float wall[3]) {
+
+template <class R>
+__device__ inline bool clearWalls(const SceneView& s, V2& np, V2& nv, float rad, float wall[3], R& rec, PlaneRec* plane = nullptr) {
     bool moved = false;
     wall[0] = wall[1] = wall[2] = 0.0f;
+    if constexpr (R::on) plane->kind = 0;
     const float reach = FEELER_REACH_RADII * rad;
     const float recastTol = WALL_RECAST_TOL_RADII * rad;
     for (int pass = 0; pass < WALL_PASSES; ++pass) {
@@ -272,21 +303,32 @@ __device__ inline bool clearWalls(const SceneView& s, V2& np, V2& nv, float rad,
         if (i1 < 0) break;
         V2 hNorm;
         float clearance;
+        ContactRec ci{};
         if (fc > 0.0f) {
-            clearance = wallContact(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr1, reach, recastTol, hNorm);
+            clearance = wallContact<R::on>(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr1, reach, recastTol, hNorm, &ci);
             if (i2 >= 0) {
                 V2 h2;
-                const float c2 = wallContact(s, np, fc, V2{ FEELER_DIRS[i2].x, FEELER_DIRS[i2].y }, fr2, reach, recastTol, h2);
-                if (c2 < clearance) { clearance = c2; hNorm = h2; }
+                ContactRec ci2{};
+                const float c2 = wallContact<R::on>(s, np, fc, V2{ FEELER_DIRS[i2].x, FEELER_DIRS[i2].y }, fr2, reach, recastTol, h2, &ci2);
+                if (c2 < clearance) { clearance = c2; hNorm = h2; ci = ci2; }
             }
         } else {
             const G3 g = mapExactGrad(s, np);
             hNorm = safeNormalize(V2{ g.gx, g.gy });
             const float gl = len2(V2{ g.gx, g.gy });
-            clearance = g.w / (gl > CENTRE_GRAD_MIN ? gl : CENTRE_GRAD_MIN);
+            const float L = gl > CENTRE_GRAD_MIN ? gl : CENTRE_GRAD_MIN;
+            clearance = g.w / L;
+            if constexpr (R::on) {   // the centre's clearance d / L: the ray u with g.u = -L
+                const float g2n = g.gx * g.gx + g.gy * g.gy;
+                const V2 u = g2n > 0.0f ? V2{ g.gx, g.gy } * (-L / g2n) : V2{ 0.0f, 0.0f };
+                ci = ContactRec{ u, V2{ g.gx, g.gy }, np, clearance };
+            }
         }
         if (clearance >= rad) {
-            if (clearance < NO_CONTACT) { wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = clearance - rad; }
+            if (clearance < NO_CONTACT) {
+                wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = clearance - rad;
+                if constexpr (R::on) *plane = PlaneRec{ 2, hNorm, ci };
+            }
             break;
         }
         np = np + hNorm * (rad - clearance);
@@ -294,10 +336,16 @@ __device__ inline bool clearWalls(const SceneView& s, V2& np, V2& nv, float rad,
         wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = 0.0f;
         const float vDotN = dot2(nv, hNorm);
         if (vDotN < 0.0f) nv = nv - hNorm * vDotN;
+        if constexpr (R::on) {   // the clearance's model is at the pre-push position
+            rec.push(hNorm, ci, vDotN < 0.0f);
+            *plane = PlaneRec{ 1, hNorm, ci };
+        }
     }
     return moved;
 }
-__device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, float wall[3]) {
+
+template <class R>
+__device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, float wall[3], R& rec, PlaneRec* plane = nullptr) {
     wall[0] = wall[1] = wall[2] = 0.0f;
     const float reach = PLANE_PROBE_RADII * rad;
     const float fc = mapSdf(s, np);
@@ -310,11 +358,21 @@ __device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, floa
     }
     if (i1 < 0 || fc <= 0.0f) return;
     V2 away;
-    const float c = wallContact(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr, reach, WALL_RECAST_TOL_RADII * rad, away);
-    if (c < NO_CONTACT) { wall[0] = away.x; wall[1] = away.y; const float m = c - rad; wall[2] = m > 0.0f ? m : 0.0f; }
+    ContactRec ci{};
+    const float c = wallContact<R::on>(s, np, fc, V2{ FEELER_DIRS[i1].x, FEELER_DIRS[i1].y }, fr, reach, WALL_RECAST_TOL_RADII * rad, away, &ci);
+    if (c < NO_CONTACT) {
+        wall[0] = away.x; wall[1] = away.y;
+        const float m = c - rad;
+        wall[2] = m > 0.0f ? m : 0.0f;
+        if constexpr (R::on) *plane = PlaneRec{ m > 0.0f ? 3 : 1, away, ci };
+    }
+    (void)rec;
 }
+
 __device__ inline bool solidAt(const SceneView& s, V2 p) { return mapExactGrad(s, p).w <= 0.0f; }
-__device__ inline bool sweepBack(const SceneView& s, V2 pOld, V2& np, V2& nv) {
+
+template <class R>
+__device__ inline bool sweepBack(const SceneView& s, V2 pOld, V2& np, V2& nv, R& rec) {
     if (!solidAt(s, np)) return false;
     float lo = 0.0f, hi = 1.0f;
     for (int i = 0; i < STATIC_SWEEP_BISECTIONS; ++i) {
@@ -326,20 +384,40 @@ __device__ inline bool sweepBack(const SceneView& s, V2 pOld, V2& np, V2& nv) {
     np = mix2(pOld, np, lo);
     const float vd = dot2(nv, d);
     if (vd > 0.0f) nv = nv - d * vd;
+    if constexpr (R::on) rec.sweep(lo, d, vd > 0.0f);
     return true;
 }
+
 // resolveStatic on a flat floor: grounded unless the centre lies in a wall
-__device__ inline bool resolveStatic(const SceneView& s, V2 pOld, V2& np, V2& nv, float rad, float wall[3]) {
-    if (wallFree(s, np)) { wall[0] = wall[1] = wall[2] = 0.0f; return true; }
+template <class R>
+__device__ inline bool resolveStatic(const SceneView& s, V2 pOld, V2& np, V2& nv, float rad, float wall[3], R& rec,
+                                     PlaneRec* plane = nullptr) {
+    if (wallFree(s, np)) {
+        wall[0] = wall[1] = wall[2] = 0.0f;
+        if constexpr (R::on) plane->kind = 0;
+        return true;
+    }
     bool grounded = !solidAt(s, np);
-    if (!grounded && sweepBack(s, pOld, np, nv)) grounded = !solidAt(s, np);
-    const bool walls = clearWalls(s, np, nv, rad, wall);
+    if (!grounded && sweepBack(s, pOld, np, nv, rec)) grounded = !solidAt(s, np);
+    const bool walls = clearWalls(s, np, nv, rad, wall, rec, plane);
     if (walls && grounded) {
         float w2[3];
-        clearWalls(s, np, nv, rad, w2);
-        if (w2[0] != 0.0f || w2[1] != 0.0f) { wall[0] = w2[0]; wall[1] = w2[1]; wall[2] = w2[2]; }
+        PlaneRec p2{};
+        clearWalls(s, np, nv, rad, w2, rec, &p2);
+        if (w2[0] != 0.0f || w2[1] != 0.0f) {
+            wall[0] = w2[0]; wall[1] = w2[1]; wall[2] = w2[2];
+            if constexpr (R::on) *plane = p2;
+        }
     }
     return grounded;
+}
+__device__ inline bool resolveStatic(const SceneView& s, V2 pOld, V2& np, V2& nv, float rad, float wall[3]) {
+    NoRec r;
+    return resolveStatic(s, pOld, np, nv, rad, wall, r);
+}
+__device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, float wall[3]) {
+    NoRec r;
+    wallPlaneProbe(s, np, rad, wall, r);
 }
 
 } // namespace swarm

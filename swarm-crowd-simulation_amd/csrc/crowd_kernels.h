@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/swarm_adjoint.h"
 #include "crowd_device.h"
 
 namespace swarm {
@@ -90,7 +91,67 @@ struct EvacBuffers {
     int32_t* step;        // agent order: the step the agent left at (-1)
     int32_t* exit;        // agent order: its exit (E: out of the domain)
     uint32_t* perExit;    // per scenario PER_EXIT_STRIDE counters
+    float2* inside;       // agent order: its position at that step's input (inside the exit)
+    float2* before;       // agent order: its position one step earlier
 };
+
+// One step's tape of the adjoint's forward (swarm_adjoint.h's records), slot order
+struct TapeStep {
+    swarm_predict_rec* pred;
+    swarm_final_rec* fin;
+    uint64_t* mask;            // P x n: the contacts of iteration it among the slot's candidates, mask[(it - 1) * n + slot]
+    uint64_t* capBits;         // bit it - 1: iteration it's correction was capped
+    uint64_t* planeBits;       // bit it - 1: iteration it's wall plane applied
+    uint64_t* overlap;         // the candidates overlapping at the step's end
+    const uint32_t* assign;    // agent order: this step's exits
+    int efixed;                // e_goal is a drive direction (or zero): no goal anchor
+    uint32_t* err;             // bit 0: candidates overflowed; bit 1: more wall pushes than recorded
+};
+
+// records the wall pushes and the sweep of one resolveStatic into a predict or final record
+struct WallTape {
+    static constexpr bool on = true;
+    uint32_t* flags;
+    uint32_t* nev;
+    swarm_wall_event* ev;
+    float* sw;                 // sweep lo, dx, dy
+    uint32_t* err;
+    __device__ void push(V2 a, const ContactRec& c, bool vel) {
+        const uint32_t k = *nev;
+        if (k >= uint32_t(SWARM_TAPE_MAX_WALL)) { atomicOr(err, 2u); return; }
+        ev[k] = swarm_wall_event{ a.x, a.y, c.u.x, c.u.y, c.g.x, c.g.y, c.x0.x, c.x0.y, c.t0, vel ? 1u : 0u };
+        *nev = k + 1u;
+    }
+    __device__ void sweep(float lo, V2 d, bool vel) {
+        sw[0] = lo; sw[1] = d.x; sw[2] = d.y;
+        *flags |= SWARM_REC_SWEEP | (vel ? SWARM_REC_SWEEP_VEL : 0u);
+    }
+};
+
+// XCD-aware block numbering for the slot-order kernels. The GPU hands consecutive workgroups to its XCDs in
+// turn and each XCD has its own L2; renumbered, XCD x takes a contiguous 1/XCDS of the slots, so neighbours
+// (slots are sorted by cell) are mostly read through one L2. A bijection on [0, gridDim.x).
+__device__ inline int xcdBlock() {
+    const int g = int(gridDim.x), b = int(blockIdx.x), q = g / XCDS;
+    return b < q * XCDS ? (b % XCDS) * q + b / XCDS : b;
+}
+
+// the scene of an agent / slot: plain instances use the kernel's scene and no offsets
+struct ScnRef {
+    uint32_t sc, cellBase, agentBase;
+};
+template <bool B> __device__ inline const SceneView& sceneOf(const SceneView& s0, const BatchView& bt, uint32_t sc) {
+    if constexpr (B) return bt.views[bt.scenLayout[sc]];
+    else return s0;
+}
+template <bool B> __device__ inline ScnRef scnOfAgent(const BatchView& bt, uint32_t agent) {
+    if constexpr (B) { const uint32_t sc = bt.agentScen[agent]; return ScnRef{ sc, bt.cellBase[sc], bt.agentBase[sc] }; }
+    else return ScnRef{ 0u, 0u, 0u };
+}
+template <bool B> __device__ inline ScnRef scnOfSlot(const BatchView& bt, uint32_t slot) {
+    if constexpr (B) { const uint32_t sc = bt.slotScen[slot]; return ScnRef{ sc, bt.cellBase[sc], bt.agentBase[sc] }; }
+    else return ScnRef{ 0u, 0u, 0u };
+}
 
 // the plain step: 'bt' unused; the batched step: s unused, scenarios = the number of scenarios
 struct StepLaunch {
@@ -116,6 +177,14 @@ void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const Choic
 // stamps (optional, 2 events): recorded after predict and after the projection iterations before the final pass
 void launchPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuffers pa, ProjBuffers pb, AgentHalf rd, AgentHalf wr,
                    EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps = nullptr);
+// The physics passes of the adjoint's taped forward: positions of every iteration into q ((P + 1) x n: q[0]
+// predict's output, q[it] iteration it's), the accumulator through acc0 / acc1, the records into tp
+void launchPhysicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, float2* q, float* acc0, float* acc1, AgentHalf rd,
+                         AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, const TapeStep& tp);
+// The adjoint's loss terms: the overlap term of the step just run (weight * (D - d)^2 per overlapping candidate
+// pair, tp: also the overlap masks) and the exit term of the state s; both added to *loss
+void launchOverlap(hipStream_t st, const StepLaunch& L, StepBuffers& b, AgentHalf wr, float weight, double* loss, const TapeStep* tp);
+void launchExitLoss(hipStream_t st, const StepLaunch& L, AgentHalf s, const uint32_t* assign, swarm_exit_rec* out, double* loss);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes);
 

@@ -41,33 +41,8 @@ __device__ inline void waveCountAdd(uint32_t* base, uint32_t key, bool valid) {
     }
 }
 
-// XCD-aware block numbering for the slot-order kernels. The GPU hands consecutive workgroups to its XCDs in
-// turn and each XCD has its own L2; renumbered, XCD x takes a contiguous 1/XCDS of the slots, so neighbours
-// (slots are sorted by cell) are mostly read through one L2. A bijection on [0, gridDim.x).
-__device__ inline int xcdBlock() {
-    const int g = int(gridDim.x), b = int(blockIdx.x), q = g / XCDS;
-    return b < q * XCDS ? (b % XCDS) * q + b / XCDS : b;
-}
-
 __device__ inline uint32_t bucketOf(uint32_t key, uint32_t agent, uint32_t C) {
     return key == INACTIVE_CELL ? C + agent / uint32_t(HASH_BLOCK) : key;
-}
-
-// the scene of an agent / slot: plain instances use the kernel's scene and no offsets
-struct ScnRef {
-    uint32_t sc, cellBase, agentBase;
-};
-template <bool B> __device__ inline const SceneView& sceneOf(const SceneView& s0, const BatchView& bt, uint32_t sc) {
-    if constexpr (B) return bt.views[bt.scenLayout[sc]];
-    else return s0;
-}
-template <bool B> __device__ inline ScnRef scnOfAgent(const BatchView& bt, uint32_t agent) {
-    if constexpr (B) { const uint32_t sc = bt.agentScen[agent]; return ScnRef{ sc, bt.cellBase[sc], bt.agentBase[sc] }; }
-    else return ScnRef{ 0u, 0u, 0u };
-}
-template <bool B> __device__ inline ScnRef scnOfSlot(const BatchView& bt, uint32_t slot) {
-    if constexpr (B) { const uint32_t sc = bt.slotScen[slot]; return ScnRef{ sc, bt.cellBase[sc], bt.agentBase[sc] }; }
-    else return ScnRef{ 0u, 0u, 0u };
 }
 
 // ---------------- grid passes ----------------
@@ -410,20 +385,31 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_goal_drive(int n, const uint32_t
 }
 
 // ---------------- PHYSICS_PASS 1: predict ----------------
-template <bool B>
+template <bool B, bool REC = false>
 __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchView& bt, const ModelParams& mp, const StepBuffers& b,
-                                   const ProjBuffers& out) {
+                                   const ProjBuffers& out, const TapeStep& tp = TapeStep{}) {
     const uint32_t key = b.skey[slot];
+    swarm_predict_rec* rec = nullptr;
+    if constexpr (REC) {
+        rec = tp.pred + slot;
+        rec->flags = 0u;
+        rec->n_events = 0u;
+        rec->plane_kind = 0u;
+        tp.capBits[slot] = 0ull;
+        tp.planeBits[slot] = 0ull;
+    }
     if (key == INACTIVE_CELL) return;
     const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
     const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
     const uint32_t self = b.sid[slot] - sr.agentBase;   // the scene's own agent ids (pair hash, tie-break)
     const V2 p{ b.sx[slot], b.sy[slot] };
     const float ri = b.sr[slot];
+    if constexpr (REC) rec->flags = SWARM_REC_ACTIVE;
     if (parkTest(s, p)) {
         out.p[slot] = make_float2(PARK_COORD, PARK_COORD);
         out.acc[slot] = 0.0f;
         b.pw[slot] = 0.0f; b.pvx[slot] = 0.0f; b.pvy[slot] = 0.0f;
+        if constexpr (REC) rec->flags |= SWARM_REC_PARKED;
         return;
     }
     const uint32_t agent = b.sid[slot];
@@ -433,6 +419,7 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
     const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
     V2 sep{ 0.0f, 0.0f };
     float sAhead = 1e9f;   // the smallest gap (centre distance - contact distance) to an agent ahead
+    uint32_t leader = NBR_OVERFLOW;
     const int R = s.percRing;
     const int x0 = cx - R < 0 ? 0 : cx - R;
     const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
@@ -468,35 +455,83 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
                         const V2 eOther{ b.egx[k], b.egy[k] };
                         const float aheadOther = dot2(offset, eOther);
                         const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < D;
-                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = gap;
+                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) {
+                            sAhead = gap;
+                            if constexpr (REC) leader = k;
+                        }
                     }
                 }
             }
         }
     }
-    const float vFree = stairFactor(s, p) * b.v0[agent];
+    const float stair = stairFactor(s, p);
+    const float vFree = stair * b.v0[agent];
     float vc = sAhead * HEADWAY_RATE;
-    vc = vc < 0.0f ? 0.0f : (vFree < vc ? vFree : vc);
+    const bool vcZero = vc < 0.0f, vcFree = !vcZero && vFree < vc;
+    vc = vcZero ? 0.0f : (vcFree ? vFree : vc);
     const float vDes = vc * len2(eGoal);
     const V2 eDir = safeNormalize(eGoal + sep);
 
     V2 aObs{ 0.0f, 0.0f };
+    bool obst = false;
     const float dTorso = mapSdf(s, p);
     if (dTorso < OBSTACLE_RANGE) {
         const V2 norm = safeNormalize(calcGradient(s, p));
-        if (len2(norm) > OBSTACLE_MIN_NORMAL) aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * mp.obstacleGain;
+        if (len2(norm) > OBSTACLE_MIN_NORMAL) {
+            aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * mp.obstacleGain;
+            obst = true;
+        }
     }
     const V2 vEq = eDir * vDes + aObs * mp.tau;
     V2 nv = vEq + (v - vEq) * mp.relaxStep;
     const float along = dot2(nv, eGoal);
-    if (along > vDes) nv = nv - eGoal * (along - vDes);
+    const bool alongCap = along > vDes;
+    if (alongCap) nv = nv - eGoal * (along - vDes);
     const float vmax = b.v0[agent];
-    if (len2(nv) > vmax) nv = safeNormalize(nv) * vmax;
+    const bool clamp = len2(nv) > vmax;
+    if (clamp) nv = safeNormalize(nv) * vmax;
     V2 np = p + nv * DT;
 
     float wall[3];
-    const bool grounded = resolveStatic(s, p, np, nv, ri, wall);
-    if (wall[0] == 0.0f && wall[1] == 0.0f && !wallFree(s, np)) wallPlaneProbe(s, np, ri, wall);
+    bool grounded;
+    if constexpr (REC) {
+        uint32_t f = SWARM_REC_ACTIVE;
+        if (leader != NBR_OVERFLOW) { f |= SWARM_REC_LEADER; rec->leader = leader; }
+        f |= vcZero ? SWARM_REC_VC_ZERO : (vcFree ? SWARM_REC_VC_FREE : SWARM_REC_VC_RATE);
+        if (!nonzero(eDir)) f |= SWARM_REC_EDIR_ZERO;
+        if (obst) f |= SWARM_REC_OBST;
+        if (alongCap) f |= SWARM_REC_ALONG;
+        if (clamp) f |= SWARM_REC_CLAMP;
+        // the goal anchor behind e_goal (choiceDirection), or a constant e_goal
+        bool efixed = tp.efixed != 0;
+        if (!efixed) {
+            const int cell = choiceCell(s, p);
+            const uint32_t ex = tp.assign[agent];
+            const int kk = slotOfExit(s, cell, ex);
+            if (kk < 0) efixed = true;
+            else {
+                const V2 a = anchorCentre(s, s.navSlot[size_t(cell) * size_t(s.K) + size_t(kk)] & CHOICE_ANCHOR_MASK);
+                rec->anchor_x = a.x; rec->anchor_y = a.y;
+                if (!(len2(a - p) > GOAL_DEADZONE)) efixed = true;
+            }
+        }
+        if (efixed) f |= SWARM_REC_EFIXED;
+        rec->flags = f;
+        rec->egoal_x = eGoal.x; rec->egoal_y = eGoal.y;
+        rec->stair = stair;
+        rec->sep_x = sep.x; rec->sep_y = sep.y;
+        WallTape wt{ &rec->flags, &rec->n_events, rec->ev, &rec->sweep_lo, tp.err };
+        PlaneRec pl{};
+        grounded = resolveStatic(s, p, np, nv, ri, wall, wt, &pl);
+        if (wall[0] == 0.0f && wall[1] == 0.0f && !wallFree(s, np)) wallPlaneProbe(s, np, ri, wall, wt, &pl);
+        rec->plane_kind = uint32_t(pl.kind == 3 ? 2 : pl.kind);
+        rec->plane_nx = pl.n.x; rec->plane_ny = pl.n.y;
+        rec->pc_ux = pl.c.u.x; rec->pc_uy = pl.c.u.y; rec->pc_gx = pl.c.g.x; rec->pc_gy = pl.c.g.y;
+        rec->pc_x0 = pl.c.x0.x; rec->pc_y0 = pl.c.x0.y; rec->pc_t0 = pl.c.t0;
+    } else {
+        grounded = resolveStatic(s, p, np, nv, ri, wall);
+        if (wall[0] == 0.0f && wall[1] == 0.0f && !wallFree(s, np)) wallPlaneProbe(s, np, ri, wall);
+    }
     out.p[slot] = make_float2(np.x, np.y);
     out.acc[slot] = 0.0f;
     b.pw[slot] = 1.0f; b.pvx[slot] = nv.x; b.pvy[slot] = nv.y;
@@ -505,28 +540,35 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
 }
 
 // the physics kernels loop over slots with the grid's stride: one slot per thread, or a persistent grid
-template <bool B>
-__global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers out) {
-    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) predictSlot<B>(slot, s0, bt, mp, b, out);
+template <bool B, bool REC = false>
+__global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers out,
+                                                            TapeStep tp = TapeStep{}) {
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x)
+        predictSlot<B, REC>(slot, s0, bt, mp, b, out, tp);
 }
 
 // ---------------- PHYSICS_PASS 2: the Jacobi projection ----------------
 // The end of projectSlot (cpu_crowd_sim.cpp:805-821): the mean correction, its cap, the wall plane, the accumulator
+// capped / planed: whether the cap and the wall plane applied (the adjoint's tape)
 __device__ inline void finishProjection(const StepBuffers& b, int slot, float bx, float by, float acc, V2 corr, int nContacts,
-                                        float ri, float& ox, float& oy, float& oacc) {
+                                        float ri, float& ox, float& oy, float& oacc, bool* capped = nullptr, bool* planed = nullptr) {
     corr = corr * (PROJ_RELAX / float(nContacts > 1 ? nContacts : 1));
     const float maxShift = PROJ_MAX_SHIFT_RADII * ri;
     const float len = len2(corr);
-    if (len > maxShift) corr = corr * (maxShift / len);
+    const bool cap = len > maxShift;
+    if (cap) corr = corr * (maxShift / len);
     V2 q = V2{ bx, by } + corr;
     const float4 pl = b.planes[slot];
+    bool pld = false;
     if (int(pl.w) & 1) {
         const float sd = pl.z - dot2(q, V2{ pl.x, pl.y });
-        if (sd > 0.0f) q = q + V2{ pl.x, pl.y } * sd;
+        pld = sd > 0.0f;
+        if (pld) q = q + V2{ pl.x, pl.y } * sd;
     }
     const V2 d = q - V2{ bx, by };
     ox = q.x; oy = q.y;
     oacc = acc + len2(d);
+    if (capped) { *capped = cap; *planed = pld; }
 }
 
 // contactCorrection (cpu_crowd_sim.cpp:141-148) for a pair already known to overlap (dxy2 < D^2); the other
@@ -545,7 +587,8 @@ __device__ inline V2 overlapCorrection(V2 offset, float dxy2, float D, uint32_t 
 // The candidates are the ring's active slots within that bound (with 0.1% of float slack), in ring order, so the
 // later iterations' sums over them are the ring scan's sums bit for bit.
 __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t key,
-                                   const ScnRef& sr, int P, bool collect, float& ox, float& oy, float& oacc) {
+                                   const ScnRef& sr, int P, bool collect, float& ox, float& oy, float& oacc, uint64_t* mask = nullptr,
+                                   bool* capped = nullptr, bool* planed = nullptr) {
     const float2 bp = in.p[slot];
     const float acc = in.acc[slot];
     const uint32_t self = b.sid[slot] - sr.agentBase;
@@ -584,6 +627,7 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
                 if (dxy2 < contact2(D)) {
                     corr = corr + overlapCorrection(offset, dxy2, D, self, b.sid, k, sr.agentBase);
                     ++nContacts;
+                    if (mask && cnt < uint32_t(NBR_CAP)) *mask |= 1ull << cnt;   // the contact's index among the candidates
                 }
                 if (collect) {
                     const float reach = (D + Pf * (PROJ_MAX_SHIFT_RADII * D)) * 1.001f + 1e-6f;
@@ -599,12 +643,13 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
         }
     }
     if (collect) b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
-    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
+    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
 // One iteration for one slot over its contact candidates
 __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
-                                   float& ox, float& oy, float& oacc) {
+                                   float& ox, float& oy, float& oacc, uint64_t* mask = nullptr, bool* capped = nullptr,
+                                   bool* planed = nullptr) {
     const float2 bp = in.p[slot];
     const float acc = in.acc[slot];
     const uint32_t self = b.sid[slot] - sr.agentBase;
@@ -635,42 +680,63 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
             if (dxy2 < contact2(dd[u])) {
                 corr = corr + overlapCorrection(offset, dxy2, dd[u], self, b.sid, kk[u], sr.agentBase);
                 ++nContacts;
+                if (mask) *mask |= 1ull << (j0 + uint32_t(u));
             }
         }
     }
-    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
+    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
-// iteration 'first' (1: ring scan + candidates) or a later one (the candidates), slot order in -> out
-template <bool B>
+// iteration 'first' (1: ring scan + candidates) or a later one (the candidates), slot order in -> out. REC: iteration
+// 'it' of the adjoint's taped forward (contact masks, cap and plane bits)
+template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
-                                                            ProjBuffers out, int P, int first) {
+                                                            ProjBuffers out, int P, int first, TapeStep tp = TapeStep{}, int it = 0) {
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
         const uint32_t key = b.skey[slot];
         if (key == INACTIVE_CELL) continue;
         float x, y, acc;
+        uint64_t mask = 0ull;
+        bool capped = false, planed = false;
         if (b.pw[slot] == 0.0f) {   // parked this step: carried
             const float2 bp = in.p[slot];
             x = bp.x; y = bp.y; acc = in.acc[slot];
         } else {
             const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
             const uint32_t cnt = first ? NBR_OVERFLOW : b.nbrCount[slot];
-            if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc);
-            else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+            if constexpr (REC) {
+                if (cnt == NBR_OVERFLOW) {
+                    projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc, &mask, &capped, &planed);
+                    if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
+                } else {
+                    projectList(b, in, n, slot, cnt, sr, x, y, acc, &mask, &capped, &planed);
+                }
+            } else {
+                if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc);
+                else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+            }
         }
         out.p[slot] = make_float2(x, y);
         out.acc[slot] = acc;
+        if constexpr (REC) {
+            tp.mask[size_t(it - 1) * size_t(n) + size_t(slot)] = mask;
+            if (capped) tp.capBits[slot] |= 1ull << (it - 1);
+            if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
+        }
     }
 }
 
 // ---------------- PHYSICS_PASS 3: final ----------------
-template <bool B>
+// REC: the adjoint's taped forward (the last iteration's contacts, the final pass's wall pushes)
+template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers in,
-                                                          int useList, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* __restrict__ ctrl) {
+                                                          int useList, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* __restrict__ ctrl,
+                                                          TapeStep tp = TapeStep{}, ProjBuffers last = ProjBuffers{}) {
     const uint32_t step = ctrl[CTRL_STEP];
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
         const uint32_t key = b.skey[slot];
         const uint32_t agent = b.sid[slot];
+        if constexpr (REC) { tp.fin[slot].flags = 0u; tp.fin[slot].n_events = 0u; }
         if (key == INACTIVE_CELL) {
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.w[agent] = rd.w[agent];
             wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.spd[agent] = rd.spd[agent];
@@ -681,12 +747,33 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
             if (b.pw[slot] == 0.0f || mp.projIters == 0) {
                 const float2 bp = in.p[slot];
                 x = bp.x; y = bp.y; acc = in.acc[slot];
+                if constexpr (REC) tp.mask[size_t(mp.projIters - 1) * size_t(n) + size_t(slot)] = 0ull;
             } else {
                 const uint32_t cnt = useList ? b.nbrCount[slot] : NBR_OVERFLOW;
-                if (cnt == NBR_OVERFLOW) projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc);
-                else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+                if constexpr (REC) {
+                    uint64_t mask = 0ull;
+                    bool capped = false, planed = false;
+                    if (cnt == NBR_OVERFLOW) {
+                        projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc, &mask, &capped, &planed);
+                        atomicOr(tp.err, 1u);
+                    } else {
+                        projectList(b, in, n, slot, cnt, sr, x, y, acc, &mask, &capped, &planed);
+                    }
+                    const int it = mp.projIters;
+                    tp.mask[size_t(it - 1) * size_t(n) + size_t(slot)] = mask;
+                    if (capped) tp.capBits[slot] |= 1ull << (it - 1);
+                    if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
+                } else {
+                    if (cnt == NBR_OVERFLOW) projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc);
+                    else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+                }
             }
+            if constexpr (REC) { last.p[slot] = make_float2(x, y); last.acc[slot] = acc; }
             if (b.pw[slot] == 0.0f) {   // parked this step: an evacuation event
+                // the crossing: the input position (inside) and the previous step's input, which the write half
+                // still holds
+                ev.inside[agent] = make_float2(b.sx[slot], b.sy[slot]);
+                ev.before[agent] = make_float2(wr.x[agent], wr.y[agent]);
                 wr.x[agent] = x; wr.y[agent] = y; wr.w[agent] = 0.0f;
                 wr.vx[agent] = 0.0f; wr.vy[agent] = 0.0f; wr.spd[agent] = acc;
                 const int e = exitOf(s, V2{ b.sx[slot], b.sy[slot] });
@@ -698,7 +785,15 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                 V2 np{ x, y }, nv{ b.pvx[slot], b.pvy[slot] };
                 if (acc > 0.0f) {
                     float wall[3];
-                    resolveStatic(s, V2{ b.sx[slot], b.sy[slot] }, np, nv, b.sr[slot], wall);
+                    if constexpr (REC) {
+                        swarm_final_rec* fr = tp.fin + slot;
+                        fr->flags = SWARM_REC_MOVED;
+                        WallTape wt{ &fr->flags, &fr->n_events, fr->ev, &fr->sweep_lo, tp.err };
+                        PlaneRec pl{};
+                        resolveStatic(s, V2{ b.sx[slot], b.sy[slot] }, np, nv, b.sr[slot], wall, wt, &pl);
+                    } else {
+                        resolveStatic(s, V2{ b.sx[slot], b.sy[slot] }, np, nv, b.sr[slot], wall);
+                    }
                 }
                 wr.x[agent] = np.x; wr.y[agent] = np.y; wr.w[agent] = 1.0f;
                 wr.vx[agent] = nv.x; wr.vy[agent] = nv.y; wr.spd[agent] = len2(nv);
@@ -715,6 +810,76 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
             *reinterpret_cast<unsigned long long*>(ctrl + CTRL_ACTIVE_SUM_LO) += ctrl[CTRL_NACTIVE];
         }
     }
+}
+
+// ---------------- the adjoint's loss terms ----------------
+// wave sum of v, one double atomic per wave
+__device__ inline void waveAddDouble(double* acc, float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.0f) atomicAdd(acc, double(v));
+}
+
+// The overlap term at a step's end: each slot's contact candidates that overlap at the final positions (both
+// active), each pair added once (from its lower slot) as (D - d)^2 * weight; MASK: the overlapping candidates' bits
+template <bool MASK>
+__global__ void __launch_bounds__(256) k_overlap(int n, StepBuffers b, AgentHalf wr, float weight, double* loss, TapeStep tp) {
+    const int slot = xcdBlock() * int(blockDim.x) + int(threadIdx.x);
+    float part = 0.0f;
+    if (slot < n) {
+        uint64_t bits = 0ull;
+        if (b.skey[slot] != INACTIVE_CELL && b.pw[slot] != 0.0f) {
+            const uint32_t cnt = b.nbrCount[slot];
+            const uint32_t ai = b.sid[slot];
+            const V2 pi{ wr.x[ai], wr.y[ai] };
+            const float ri = b.sr[slot];
+            for (uint32_t k = 0; cnt != NBR_OVERFLOW && k < cnt; ++k) {
+                const uint32_t j = b.nbr[size_t(k) * size_t(n) + size_t(slot)];
+                const uint32_t aj = b.sid[j];
+                const V2 d = pi - V2{ wr.x[aj], wr.y[aj] };
+                const float D = ri + b.sr[j];
+                const float d2 = dot2(d, d);
+                if (d2 < contact2(D)) {
+                    bits |= 1ull << k;
+                    if (uint32_t(slot) < j) {
+                        const float ov = D - sqrtf(d2);
+                        part += weight * ov * ov;
+                    }
+                }
+            }
+        }
+        if constexpr (MASK) tp.overlap[slot] = bits;
+    }
+    waveAddDouble(loss, part);
+}
+
+// The exit term after the last step: an active agent's D = g(anchor) + |x - anchor| for its exit (the goal anchor
+// of its cell; g(anchor) = the cell's distance less the centre's distance to the anchor), D^2 into the loss
+template <bool B>
+__global__ void __launch_bounds__(256) k_exit_loss(int n, SceneView s0, BatchView bt, AgentHalf st, const uint32_t* __restrict__ assign,
+                                                   swarm_exit_rec* __restrict__ out, double* loss) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    float part = 0.0f;
+    if (i < n) {
+        swarm_exit_rec r{ 0.0f, 0.0f, 0.0f, 0u };
+        const uint32_t ex = assign[i];
+        if (st.w[i] != 0.0f && ex != CHOICE_NONE) {
+            const ScnRef sr = scnOfAgent<B>(bt, uint32_t(i));
+            const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
+            const V2 p{ st.x[i], st.y[i] };
+            const int cell = choiceCell(s, p);
+            const int k = slotOfExit(s, cell, ex);
+            if (k >= 0) {
+                const V2 a = anchorCentre(s, s.navSlot[size_t(cell) * size_t(s.K) + size_t(k)] & CHOICE_ANCHOR_MASK);
+                const V2 c = anchorCentre(s, uint32_t(cell));
+                const float base = slotDistOf(s, s.navDist[size_t(cell) * size_t(s.K) + size_t(k)]) - len2(c - a);
+                const float D = base + len2(p - a);
+                r = swarm_exit_rec{ a.x, a.y, base, 1u };
+                part = D * D;
+            }
+        }
+        out[i] = r;
+    }
+    waveAddDouble(loss, part);
 }
 
 // a plain streaming copy of known size: the HBM traffic counters' calibration and the measured copy bandwidth
@@ -817,6 +982,38 @@ void launchPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuff
                    EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
     if (L.batched) physicsPasses<true>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps);
     else physicsPasses<false>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps);
+}
+
+template <bool B>
+static void physicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, float2* q, float* acc0, float* acc1, AgentHalf rd,
+                          AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, const TapeStep& tp) {
+    const int n = L.n;
+    const int bs = L.physBlock > 0 ? L.physBlock : PHYS_BLOCK;
+    const int g = blocks(n, bs);
+    const int P = L.mp.projIters;
+    float* acc[2] = { acc0, acc1 };
+    auto at = [&](int it) { return ProjBuffers{ q + size_t(it) * size_t(n), acc[it & 1] }; };
+    k_predict<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, at(0), tp);
+    for (int it = 1; it < P; ++it) k_project<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, it == 1 ? 1 : 0, tp, it);
+    k_final<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, at(P - 1), 1, rd, wr, ev, ctrl, tp, at(P));
+}
+
+void launchPhysicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, float2* q, float* acc0, float* acc1, AgentHalf rd,
+                         AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, const TapeStep& tp) {
+    if (L.batched) physicsRecord<true>(st, L, b, q, acc0, acc1, rd, wr, ev, ctrl, tp);
+    else physicsRecord<false>(st, L, b, q, acc0, acc1, rd, wr, ev, ctrl, tp);
+}
+
+void launchOverlap(hipStream_t st, const StepLaunch& L, StepBuffers& b, AgentHalf wr, float weight, double* loss, const TapeStep* tp) {
+    const int g = blocks(L.n, 256);
+    if (tp) k_overlap<true><<<g, 256, 0, st>>>(L.n, b, wr, weight, loss, *tp);
+    else k_overlap<false><<<g, 256, 0, st>>>(L.n, b, wr, weight, loss, TapeStep{});
+}
+
+void launchExitLoss(hipStream_t st, const StepLaunch& L, AgentHalf s, const uint32_t* assign, swarm_exit_rec* out, double* loss) {
+    const int g = blocks(L.n, 256);
+    if (L.batched) k_exit_loss<true><<<g, 256, 0, st>>>(L.n, L.s, L.bt, s, assign, out, loss);
+    else k_exit_loss<false><<<g, 256, 0, st>>>(L.n, L.s, L.bt, s, assign, out, loss);
 }
 
 } // namespace swarm

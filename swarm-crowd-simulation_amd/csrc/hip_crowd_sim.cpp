@@ -147,6 +147,7 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     ch_.exitStride = exitStride_;
     ch_.ahead = alloc<uint32_t>(S * 2u * size_t(exitStride_) * CHOICE_AHEAD_BINS);
     ev_.step = alloc<int32_t>(n); ev_.exit = alloc<int32_t>(n); ev_.perExit = alloc<uint32_t>(S * PER_EXIT_STRIDE);
+    ev_.inside = alloc<float2>(n); ev_.before = alloc<float2>(n);
     ctrl_ = alloc<uint32_t>(CTRL_WORDS);
     const uint32_t sentinel = uint32_t(n_);
     SWARM_HIP_CHECK(hipMemcpyAsync(grid_.bstart + nb_, &sentinel, 4, hipMemcpyHostToDevice, stream_));
@@ -191,6 +192,8 @@ HipCrowdSim::~HipCrowdSim() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     dropGraph();
     for (void* p : bufs_) (void)hipFree(p);
+    for (void* p : adjBufs_) (void)hipFree(p);
+    for (char* p : ckpt_) (void)hipFree(p);
     for (DeviceScene& d : scenes_) d.release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -206,6 +209,7 @@ void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const 
     const size_t S = size_t(S_);
     restoreAgentState(x, y, vx, vy, w);
     SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r, b, hipMemcpyHostToDevice, stream_));
+    r0_.assign(r, r + n_);
     SWARM_HIP_CHECK(hipMemcpyAsync(v0_, v0, b, hipMemcpyHostToDevice, stream_));
     // a fresh exit choice (nav::ChoiceState::reset) and evacuation record
     std::vector<uint32_t> none(static_cast<size_t>(n_), CHOICE_NONE);
@@ -353,6 +357,7 @@ void HipCrowdSim::enqueueStep(int parity) {
     launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
     launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
+    if (lossOn_) launchOverlap(stream_, L, b, wr, overlapWeight_, lossAcc_, nullptr);
     if (timing_) {
         SWARM_HIP_CHECK(hipEventRecord(ev[5], stream_));
         pendingStamps_.push_back(ev);
@@ -559,6 +564,310 @@ void HipCrowdSim::readControl(uint32_t* step, uint32_t* nActive, uint64_t* activ
     if (step) *step = c[CTRL_STEP];
     if (nActive) *nActive = c[CTRL_NACTIVE];
     if (activeSum) *activeSum = uint64_t(c[CTRL_ACTIVE_SUM_LO]) | (uint64_t(c[CTRL_ACTIVE_SUM_LO + 1]) << 32);
+}
+
+} // namespace swarm
+
+// ---------------------------------------------------------------------------------------------------------------
+// The adjoint of K steps. The forward runs the plain step (graph replays) with the loss terms and keeps the
+// persistent state every 'every' steps; the backward replays each segment from its checkpoint with the taped
+// kernels (the same arithmetic: the same bits) into a segment's tape and runs the adjoint kernels over it backwards.
+namespace swarm {
+
+void HipCrowdSim::setModel(float tauS, float obstacleGain) {
+    if (!(tauS > 0.0f)) throw std::invalid_argument("setModel: the relaxation time must be positive");
+    opt_.tauS = tauS;
+    opt_.obstacleGain = obstacleGain;
+    modelParamsFor(tauS, mp_.tau, mp_.relaxStep);
+    mp_.obstacleGain = obstacleGain;
+    dropGraph();
+}
+
+void HipCrowdSim::snapshotTo(char* dst) {
+    size_t off = 0;
+    for (const auto& s : persistentState()) {
+        SWARM_HIP_CHECK(hipMemcpyAsync(dst + off, s.first, s.second, hipMemcpyDeviceToDevice, stream_));
+        off += s.second;
+    }
+}
+
+void HipCrowdSim::restoreFrom(const char* src) {
+    size_t off = 0;
+    for (const auto& s : persistentState()) {
+        SWARM_HIP_CHECK(hipMemcpyAsync(s.first, src + off, s.second, hipMemcpyDeviceToDevice, stream_));
+        off += s.second;
+    }
+}
+
+void HipCrowdSim::allocAdjoint(int every) {
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
+        adjBufs_.push_back(p);
+        return p;
+    };
+    const size_t n = size_t(n_);
+    if (!lossAcc_) {
+        lossAcc_ = static_cast<double*>(dalloc(8 * size_t(S_)));
+        exitRec_ = static_cast<swarm_exit_rec*>(dalloc(n * sizeof(swarm_exit_rec)));
+        finX_ = static_cast<float*>(dalloc(n * 4));
+        finY_ = static_cast<float*>(dalloc(n * 4));
+        float** agent[] = { &adj_.gx, &adj_.gy, &adj_.gvx, &adj_.gvy, &adj_.hx, &adj_.hy, &adj_.hvx, &adj_.hvy, &adj_.gv0, &adj_.gR };
+        for (float** f : agent) *f = static_cast<float*>(dalloc(n * 4));
+        float2** slot2[] = { &adj_.gq, &adj_.gq2, &adj_.w[0], &adj_.w[1], &adj_.gself[0], &adj_.gself[1], &adj_.gpv, &adj_.gsep, &adj_.gp, &adj_.gpo };
+        for (float2** f : slot2) *f = static_cast<float2*>(dalloc(n * 8));
+        float** slot1[] = { &adj_.gpc, &adj_.gr, &adj_.gs };
+        for (float** f : slot1) *f = static_cast<float*>(dalloc(n * 4));
+        adj_.gtau = static_cast<float*>(dalloc(4 * size_t(S_)));
+        adj_.ggain = static_cast<float*>(dalloc(4 * size_t(S_)));
+        tape_.acc0 = static_cast<float*>(dalloc(n * 4));
+        tape_.acc1 = static_cast<float*>(dalloc(n * 4));
+        tape_.err = static_cast<uint32_t*>(dalloc(16));
+    }
+    if (every <= adjCap_) return;
+    // a segment's tape: 'every' steps
+    const size_t C = size_t(every), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    float** f1[] = { &tape_.inx, &tape_.iny, &tape_.invx, &tape_.invy, &tape_.inw, &tape_.outx, &tape_.outy, &tape_.sx, &tape_.sy,
+                     &tape_.sr, &tape_.egx, &tape_.egy, &tape_.pw, &tape_.pvx, &tape_.pvy };
+    for (float** f : f1) *f = static_cast<float*>(dalloc(C * n * 4));
+    tape_.sid = static_cast<uint32_t*>(dalloc(C * n * 4));
+    tape_.skey = static_cast<uint32_t*>(dalloc(C * n * 4));
+    tape_.bstart = static_cast<uint32_t*>(dalloc(C * nbs * 4));
+    tape_.planes = static_cast<float4*>(dalloc(C * n * 16));
+    tape_.nbr = static_cast<uint32_t*>(dalloc(C * size_t(NBR_CAP) * n * 4));
+    tape_.nbrD = static_cast<float*>(dalloc(C * size_t(NBR_CAP) * n * 4));
+    tape_.nbrCount = static_cast<uint32_t*>(dalloc(C * n * 4));
+    tape_.pred = static_cast<swarm_predict_rec*>(dalloc(C * n * sizeof(swarm_predict_rec)));
+    tape_.fin = static_cast<swarm_final_rec*>(dalloc(C * n * sizeof(swarm_final_rec)));
+    tape_.mask = static_cast<uint64_t*>(dalloc(C * P * n * 8));
+    tape_.capBits = static_cast<uint64_t*>(dalloc(C * n * 8));
+    tape_.planeBits = static_cast<uint64_t*>(dalloc(C * n * 8));
+    tape_.overlap = static_cast<uint64_t*>(dalloc(C * n * 8));
+    tape_.q = static_cast<float2*>(dalloc(C * (P + 1) * n * 8));
+    const uint32_t sentinel = uint32_t(n_);
+    for (size_t k = 0; k < C; ++k)
+        SWARM_HIP_CHECK(hipMemcpyAsync(tape_.bstart + k * nbs + size_t(nb_), &sentinel, 4, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    adjCap_ = every;
+}
+
+double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float overlapWeight) {
+    if (K <= 0 || every <= 0) throw std::invalid_argument("adjointForward: K and the checkpoint interval must be positive");
+    if (mp_.projIters < 2 || mp_.projIters > SWARM_TAPE_MAX_ITERS)
+        throw std::invalid_argument("adjointForward: the adjoint needs 2 to 32 projection iterations");
+    if (r0_.size() != size_t(n_)) throw std::invalid_argument("adjointForward: upload a state first");
+    if (!(radiusScale > 0.0f)) throw std::invalid_argument("adjointForward: the radius scale must be positive");
+    allocAdjoint(std::min(every, K));
+    // this run's radii
+    std::vector<float> r(r0_.size());
+    for (size_t i = 0; i < r.size(); ++i) r[i] = radiusScale * r0_[i];
+    SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r.data(), r.size() * 4, hipMemcpyHostToDevice, stream_));
+    adjK_ = K;
+    adjEvery_ = std::min(every, K);
+    adjSegs_ = (K + adjEvery_ - 1) / adjEvery_;
+    adjScale_ = radiusScale;
+    overlapWeight_ = overlapWeight;
+    tapedSeg_ = -1;
+    snapBytes_ = 0;
+    for (const auto& s : persistentState()) snapBytes_ += s.second;
+    while (int(ckpt_.size()) < adjSegs_) {
+        char* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, snapBytes_));
+        ckpt_.push_back(p);
+    }
+    ckptParity_.assign(size_t(adjSegs_), 0);
+    SWARM_HIP_CHECK(hipMemsetAsync(lossAcc_, 0, 8 * size_t(S_), stream_));
+    if (!lossOn_) { lossOn_ = true; dropGraph(); }
+    for (int seg = 0; seg < adjSegs_; ++seg) {
+        snapshotTo(ckpt_[size_t(seg)]);
+        ckptParity_[size_t(seg)] = parity_;
+        step(std::min(adjEvery_, K - seg * adjEvery_));
+    }
+    lossOn_ = false;
+    dropGraph();
+    launchExitLoss(stream_, launchDesc(), half_[parity_], ch_.assign, exitRec_, lossAcc_);
+    SWARM_HIP_CHECK(hipGetLastError());
+    SWARM_HIP_CHECK(hipMemcpyAsync(finX_, half_[parity_].x, size_t(n_) * 4, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(finY_, half_[parity_].y, size_t(n_) * 4, hipMemcpyDeviceToDevice, stream_));
+    double loss = 0.0;
+    SWARM_HIP_CHECK(hipMemcpyAsync(&loss, lossAcc_, 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    return loss;
+}
+
+void HipCrowdSim::tapedStep(int k) {
+    const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    const AgentHalf rd = half_[parity_], wr = half_[1 - parity_];
+    const size_t b = n * 4;
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.inx + K * n, rd.x, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.iny + K * n, rd.y, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.invx + K * n, rd.vx, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.invy + K * n, rd.vy, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.inw + K * n, rd.w, b, hipMemcpyDeviceToDevice, stream_));
+    StepBuffers sb = sb_;
+    sb.sid = tape_.sid + K * n; sb.skey = tape_.skey + K * n; sb.bstart = tape_.bstart + K * nbs;
+    sb.sx = tape_.sx + K * n; sb.sy = tape_.sy + K * n; sb.sr = tape_.sr + K * n;
+    sb.egx = tape_.egx + K * n; sb.egy = tape_.egy + K * n;
+    sb.pw = tape_.pw + K * n; sb.pvx = tape_.pvx + K * n; sb.pvy = tape_.pvy + K * n;
+    sb.planes = tape_.planes + K * n;
+    sb.nbr = tape_.nbr + K * size_t(NBR_CAP) * n; sb.nbrD = tape_.nbrD + K * size_t(NBR_CAP) * n; sb.nbrCount = tape_.nbrCount + K * n;
+    sb.avx = rd.vx; sb.avy = rd.vy;
+    GridBuffers gb = grid_;
+    gb.bstart = tape_.bstart + K * nbs;
+    const StepLaunch L = launchDesc();
+    launchGridPasses(stream_, L, rd, gb, sb, rad_, ctrl_);
+    launchChoice(stream_, L, rd, ch_, sb, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    TapeStep tp{ tape_.pred + K * n, tape_.fin + K * n, tape_.mask + K * P * n, tape_.capBits + K * n, tape_.planeBits + K * n,
+                 tape_.overlap + K * n, ch_.assign, (drive_ || (!batched_ && L.s.E == 0)) ? 1 : 0, tape_.err };
+    launchPhysicsRecord(stream_, L, sb, tape_.q + K * (P + 1) * n, tape_.acc0, tape_.acc1, rd, wr, ev_, ctrl_, tp);
+    launchOverlap(stream_, L, sb, wr, overlapWeight_, lossAcc_ + 0, &tp);
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.outx + K * n, wr.x, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.outy + K * n, wr.y, b, hipMemcpyDeviceToDevice, stream_));
+    SWARM_HIP_CHECK(hipGetLastError());
+    parity_ = 1 - parity_;
+}
+
+void HipCrowdSim::tapeSegment(int seg) {
+    if (seg < 0 || seg >= adjSegs_) throw std::invalid_argument("adjoint: no such segment");
+    restoreFrom(ckpt_[size_t(seg)]);
+    parity_ = ckptParity_[size_t(seg)];
+    SWARM_HIP_CHECK(hipMemsetAsync(tape_.err, 0, 4, stream_));
+    const int s0 = seg * adjEvery_, s1 = std::min(adjK_, s0 + adjEvery_);
+    for (int k = 0; k < s1 - s0; ++k) tapedStep(k);
+    uint32_t err = 0;
+    SWARM_HIP_CHECK(hipMemcpyAsync(&err, tape_.err, 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    tapedSeg_ = seg;
+    if (err & 1u) throw std::runtime_error("adjoint: a slot has more than 48 contact candidates (unsupported by the adjoint)");
+    if (err & 2u) throw std::runtime_error("adjoint: more wall pushes in one resolveStatic than the tape holds");
+}
+
+AdjStep HipCrowdSim::adjStepView(int k) const {
+    const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    AdjStep t{};
+    t.n = n_; t.P = mp_.projIters;
+    t.sid = tape_.sid + K * n; t.skey = tape_.skey + K * n; t.bstart = tape_.bstart + K * nbs;
+    t.sx = tape_.sx + K * n; t.sy = tape_.sy + K * n; t.sr = tape_.sr + K * n;
+    t.invx = tape_.invx + K * n; t.invy = tape_.invy + K * n;
+    t.outx = tape_.outx + K * n; t.outy = tape_.outy + K * n;
+    t.pred = tape_.pred + K * n; t.fin = tape_.fin + K * n;
+    t.nbr = tape_.nbr + K * size_t(NBR_CAP) * n; t.cnt = tape_.nbrCount + K * n;
+    t.mask = tape_.mask + K * P * n; t.capBits = tape_.capBits + K * n; t.planeBits = tape_.planeBits + K * n;
+    t.overlap = tape_.overlap + K * n;
+    t.q = tape_.q + K * (P + 1) * n;
+    t.planes = tape_.planes + K * n;
+    t.v0 = v0_;
+    return t;
+}
+
+void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
+    if (adjK_ <= 0) throw std::invalid_argument("adjointBackward: run adjointForward first");
+    const size_t n = size_t(n_), S = size_t(S_);
+    // the exit term's adjoint from the forward's final positions
+    if (exitTermOn_) {
+        AgentHalf fin = half_[parity_];
+        fin.x = finX_;
+        fin.y = finY_;
+        launchAdjointSeed(stream_, n_, fin, exitRec_, adj_);
+    } else {
+        launchAdjointZero(stream_, n_, adj_);
+    }
+    SWARM_HIP_CHECK(hipMemsetAsync(adj_.gv0, 0, n * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(adj_.gR, 0, n * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(adj_.gtau, 0, S * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(adj_.ggain, 0, S * 4, stream_));
+    const StepLaunch L = launchDesc();
+    for (int seg = adjSegs_ - 1; seg >= 0; --seg) {
+        tapeSegment(seg);
+        const int len = std::min(adjK_, (seg + 1) * adjEvery_) - seg * adjEvery_;
+        for (int k = len - 1; k >= 0; --k) {
+            const int s = seg * adjEvery_ + k;   // the step; its output is state s + 1
+            if (seedsOn_) launchAdjointEventSeed(stream_, n_, s + 1, seedStep_, seedIn_, seedOut_, adj_);
+            launchAdjointStep(stream_, L, adjStepView(k), adj_, overlapWeight_);
+            std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
+            std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
+            if (window_ > 0 && s % window_ == 0) launchAdjointZero(stream_, n_, adj_);
+        }
+        SWARM_HIP_CHECK(hipGetLastError());
+    }
+    std::vector<float> gR(n), gtau(S), ggain(S);
+    SWARM_HIP_CHECK(hipMemcpyAsync(gradV0, adj_.gv0, n * 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(gR.data(), adj_.gR, n * 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(gtau.data(), adj_.gtau, S * 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(ggain.data(), adj_.ggain, S * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    // per scenario: dL/dtau_s (tau = tau_s / T), dL/dscale = sum r0 dL/dr, dL/dgain
+    std::vector<double> gs(S, 0.0);
+    std::vector<uint32_t> base(S + 1, 0);
+    if (batched_) {
+        std::vector<uint32_t> ab(S);
+        SWARM_HIP_CHECK(hipMemcpy(ab.data(), bt_.agentBase, S * 4, hipMemcpyDeviceToHost));
+        for (size_t s = 0; s < S; ++s) base[s] = ab[s];
+    }
+    base[S] = uint32_t(n);
+    for (size_t s = 0; s < S; ++s)
+        for (uint32_t i = base[s]; i < base[s + 1]; ++i) gs[s] += double(r0_[i]) * double(gR[i]);
+    for (size_t s = 0; s < S; ++s) {
+        gradGlobals[3 * s + 0] = double(gtau[s]) / double(TIME_SCALE_S_PER_UNIT);
+        gradGlobals[3 * s + 1] = gs[s];
+        gradGlobals[3 * s + 2] = double(ggain[s]);
+    }
+}
+
+void HipCrowdSim::adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit) {
+    if (adjK_ <= 0 || step < 0 || step >= adjK_) throw std::invalid_argument("adjointTape: no such step of the last forward");
+    const int seg = step / adjEvery_;
+    if (tapedSeg_ != seg) tapeSegment(seg);
+    const size_t n = size_t(n_), K = size_t(step - seg * adjEvery_), P = size_t(mp_.projIters);
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+        if (dst) SWARM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+    };
+    get(out.x, tape_.inx + K * n, n * 4); get(out.y, tape_.iny + K * n, n * 4);
+    get(out.vx, tape_.invx + K * n, n * 4); get(out.vy, tape_.invy + K * n, n * 4); get(out.w, tape_.inw + K * n, n * 4);
+    get(out.sid, tape_.sid + K * n, n * 4);
+    get(out.pred, tape_.pred + K * n, n * sizeof(swarm_predict_rec));
+    get(out.fin, tape_.fin + K * n, n * sizeof(swarm_final_rec));
+    get(out.sx, tape_.sx + K * n, n * 4); get(out.sy, tape_.sy + K * n, n * 4); get(out.sr, tape_.sr + K * n, n * 4);
+    get(out.nbr, tape_.nbr + K * size_t(NBR_CAP) * n, size_t(NBR_CAP) * n * 4);
+    get(out.nbr_count, tape_.nbrCount + K * n, n * 4);
+    get(out.mask, tape_.mask + K * P * n, P * n * 8);
+    get(out.cap_bits, tape_.capBits + K * n, n * 8);
+    get(out.plane_bits, tape_.planeBits + K * n, n * 8);
+    get(out.overlap, tape_.overlap + K * n, n * 8);
+    if (exit && step == adjK_ - 1) get(exit, exitRec_, n * sizeof(swarm_exit_rec));
+    sync();
+}
+
+} // namespace swarm
+
+namespace swarm {
+
+void HipCrowdSim::setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window) {
+    if (window < 0) throw std::invalid_argument("setAdjointTerms: the window must be >= 0");
+    exitTermOn_ = exitTerm;
+    window_ = window;
+    seedsOn_ = step && gin && gout;
+    if (!seedsOn_) return;
+    const size_t n = size_t(n_);
+    if (!seedStep_) {
+        void* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, n * 4)); adjBufs_.push_back(p); seedStep_ = static_cast<int32_t*>(p);
+        SWARM_HIP_CHECK(hipMalloc(&p, n * 8)); adjBufs_.push_back(p); seedIn_ = static_cast<float2*>(p);
+        SWARM_HIP_CHECK(hipMalloc(&p, n * 8)); adjBufs_.push_back(p); seedOut_ = static_cast<float2*>(p);
+    }
+    SWARM_HIP_CHECK(hipMemcpyAsync(seedStep_, step, n * 4, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(seedIn_, gin, n * 8, hipMemcpyHostToDevice, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(seedOut_, gout, n * 8, hipMemcpyHostToDevice, stream_));
+    sync();
+}
+
+void HipCrowdSim::readEvacPoints(int32_t* step, int32_t* exitId, float* inside, float* before) {
+    const size_t n = size_t(n_);
+    if (step) SWARM_HIP_CHECK(hipMemcpyAsync(step, ev_.step, n * 4, hipMemcpyDeviceToHost, stream_));
+    if (exitId) SWARM_HIP_CHECK(hipMemcpyAsync(exitId, ev_.exit, n * 4, hipMemcpyDeviceToHost, stream_));
+    if (inside) SWARM_HIP_CHECK(hipMemcpyAsync(inside, ev_.inside, n * 8, hipMemcpyDeviceToHost, stream_));
+    if (before) SWARM_HIP_CHECK(hipMemcpyAsync(before, ev_.before, n * 8, hipMemcpyDeviceToHost, stream_));
+    sync();
 }
 
 } // namespace swarm

@@ -16,6 +16,7 @@
 #include <utility>
 #include <vector>
 
+#include "adjoint.h"
 #include "crowd_kernels.h"
 #include "scene.hpp"
 
@@ -95,6 +96,25 @@ public:
     void readEvacCounters(uint32_t* perExit64);              // every scenario: PER_EXIT_STRIDE words each
     void readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum);
 
+    // ---- the adjoint of K steps (swarm_adjoint.h) ----
+    // Runs K steps from the current state with the loss terms, keeping the state every 'every' steps; returns the
+    // loss. The radii of the run are radiusScale x the uploaded radii.
+    double adjointForward(int K, int every, float radiusScale, float overlapWeight);
+    // dL/dv0 (n, agent order) and dL/d(tau_s, radius scale, wall gain) of the last adjointForward; per scenario in
+    // a batch (globals: 3 per scenario). Leaves the state unspecified.
+    void adjointBackward(float* gradV0, double* gradGlobals);
+    // one step's tape of the last adjointForward (recomputed from its checkpoint)
+    void adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit);
+    // the relaxation time (seconds) and wall gain of the next steps
+    void setModel(float tauS, float obstacleGain);
+    // Loss terms of the next adjointBackward beyond the built-in ones: position adjoints at evacuation crossings
+    // (agent a: gin at the input of step step[a], where it is inside its exit; gout at the input of step
+    // step[a] - 1; step < 1: none), whether the exit term counts, and a truncation window (> 0: the state adjoint
+    // is cleared every 'window' steps going backwards, so every path is at most that long; 0: the exact adjoint)
+    void setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window);
+    // per agent: the step it left at, its exit, its position at that step's input and one step earlier (2 floats each)
+    void readEvacPoints(int32_t* step, int32_t* exitId, float* inside, float* before);
+
     int numAgents() const { return n_; }
     int numCells() const { return C_; }
     int numScenarios() const { return S_; }
@@ -112,6 +132,13 @@ private:
     void collectTimings();
     // every buffer a step writes that outlives the step (agent halves, exit choice, evacuation, control)
     std::vector<std::pair<void*, size_t>> persistentState() const;
+    // adjoint helpers
+    void snapshotTo(char* dst);
+    void restoreFrom(const char* src);
+    void allocAdjoint(int every);
+    void tapedStep(int k);
+    void tapeSegment(int seg);
+    AdjStep adjStepView(int k) const;
 
     std::vector<DeviceScene> scenes_;
     SimOptions opt_;
@@ -140,6 +167,42 @@ private:
     std::vector<std::vector<hipEvent_t>> pendingStamps_;
     double timeSum_[8] = {};
     int timedSteps_ = 0;
+
+    // the adjoint: loss terms in the step, checkpoints, a segment's tape, the backward's buffers
+    std::vector<float> r0_;              // the uploaded radii
+    bool lossOn_ = false;
+    float overlapWeight_ = 0.0f;
+    double* lossAcc_ = nullptr;          // device
+    swarm_exit_rec* exitRec_ = nullptr;  // device, n
+    float* finX_ = nullptr;              // device, n: the forward's final positions
+    float* finY_ = nullptr;
+    int32_t* seedStep_ = nullptr;        // device, n: evacuation-crossing seeds (setAdjointTerms)
+    float2* seedIn_ = nullptr;
+    float2* seedOut_ = nullptr;
+    bool seedsOn_ = false, exitTermOn_ = true;
+    int window_ = 0;
+    int adjK_ = 0, adjEvery_ = 0, adjSegs_ = 0, adjCap_ = 0, tapedSeg_ = -1;
+    float adjScale_ = 1.0f;
+    size_t snapBytes_ = 0;
+    std::vector<char*> ckpt_;            // device snapshots, one per segment
+    std::vector<int> ckptParity_;
+    struct Tape {
+        float *inx, *iny, *invx, *invy, *inw, *outx, *outy;
+        uint32_t *sid, *skey, *bstart;
+        float *sx, *sy, *sr, *egx, *egy, *pw, *pvx, *pvy;
+        float4* planes;
+        uint32_t* nbr;
+        float* nbrD;
+        uint32_t* nbrCount;
+        swarm_predict_rec* pred;
+        swarm_final_rec* fin;
+        uint64_t *mask, *capBits, *planeBits, *overlap;
+        float2* q;
+        float *acc0, *acc1;
+        uint32_t* err;
+    } tape_{};
+    AdjBufs adj_{};
+    std::vector<void*> adjBufs_;         // the adjoint's own allocations
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

@@ -1,5 +1,6 @@
 // extern "C" layer over HipCrowdSim and the scene builder (include/swarm_hip.h). No exception crosses it:
 // std::invalid_argument -> SWARM_EINVAL, a device allocation failure -> SWARM_ENOMEM, other errors -> SWARM_EHIP.
+#include "../../include/swarm_adjoint.h"
 #include "../../include/swarm_hip.h"
 
 #include <cstring>
@@ -277,6 +278,34 @@ int swarm_autotune(swarm_ctx* c, int steps, double* best_ms) {
         const double ms = c->sim->autotune(steps);
         if (best_ms) *best_ms = ms;
     });
+}
+
+int swarm_adjoint_forward(swarm_ctx* c, int K, int checkpoint_every, float radius_scale, float overlap_weight, double* loss) {
+    CTX_CHECK(c);
+    if (!loss) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { *loss = c->sim->adjointForward(K, checkpoint_every, radius_scale, overlap_weight); });
+}
+int swarm_adjoint_backward(swarm_ctx* c, float* grad_v0, double* grad_globals) {
+    CTX_CHECK(c);
+    if (!grad_v0 || !grad_globals) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->adjointBackward(grad_v0, grad_globals); });
+}
+int swarm_adjoint_tape(swarm_ctx* c, int step, const swarm_tape_view* out, swarm_exit_rec* exit) {
+    CTX_CHECK(c);
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->adjointTape(step, *out, exit); });
+}
+int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setAdjointTerms(step, gin, gout, exit_term != 0, window); });
+}
+int swarm_read_evac_points(swarm_ctx* c, int32_t* step, int32_t* exit_id, float* inside, float* before) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readEvacPoints(step, exit_id, inside, before); });
+}
+int swarm_set_model(swarm_ctx* c, float tau_s, float obstacle_gain) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setModel(tau_s, obstacle_gain); });
 }
 
 int swarm_get_stream(swarm_ctx* c, void** stream) {

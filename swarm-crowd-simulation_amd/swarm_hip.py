@@ -390,3 +390,119 @@ def batch_workload(scenarios=256, seed=1337, lo=1024, hi=16384, load_lo=0.3, loa
     parts = [place_agents(layouts[scen_layout[s]], int(active[s]), int(n[s]), seed + s) for s in range(scenarios)]
     agents = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
     return scen_layout, n.astype(np.int32), active, agents
+
+
+# ---------------- the adjoint of K steps (include/swarm_adjoint.h) ----------------
+TAPE_NBR = 48
+
+
+class WallEvent(C.Structure):
+    _fields_ = [(f, C.c_float) for f in ("ax", "ay", "ux", "uy", "gx", "gy", "x0", "y0", "t0")] + [("vel", C.c_uint32)]
+
+
+class PredictRec(C.Structure):
+    _fields_ = ([("flags", C.c_uint32), ("leader", C.c_uint32)] +
+                [(f, C.c_float) for f in ("anchor_x", "anchor_y", "egoal_x", "egoal_y", "stair", "sep_x", "sep_y",
+                                          "sweep_lo", "sweep_dx", "sweep_dy")] +
+                [("n_events", C.c_uint32), ("plane_kind", C.c_uint32)] +
+                [(f, C.c_float) for f in ("plane_nx", "plane_ny", "pc_ux", "pc_uy", "pc_gx", "pc_gy", "pc_x0", "pc_y0", "pc_t0")] +
+                [("ev", WallEvent * 6)])
+
+
+class FinalRec(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("n_events", C.c_uint32), ("sweep_lo", C.c_float), ("sweep_dx", C.c_float),
+                ("sweep_dy", C.c_float), ("ev", WallEvent * 6)]
+
+
+class ExitRec(C.Structure):
+    _fields_ = [("anchor_x", C.c_float), ("anchor_y", C.c_float), ("base", C.c_float), ("flags", C.c_uint32)]
+
+
+class TapeView(C.Structure):
+    _fields_ = ([(f, C.c_void_p) for f in ("x", "y", "vx", "vy", "w", "sid", "pred", "fin", "sx", "sy", "sr", "nbr", "nbr_count",
+                                            "mask", "cap_bits", "plane_bits", "overlap")])
+
+
+def _adjoint_lib():
+    L = lib()
+    if not getattr(L, "_adjoint_sigs", False):
+        vp = C.c_void_p
+        L.swarm_adjoint_forward.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_double)]
+        L.swarm_adjoint_backward.argtypes = [vp, _f32p, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
+        L.swarm_adjoint_tape.argtypes = [vp, C.c_int, C.POINTER(TapeView), vp]
+        L.swarm_set_model.argtypes = [vp, C.c_float, C.c_float]
+        L._adjoint_sigs = True
+    return L
+
+
+def new_tape(n, P):
+    """Host arrays for one step's tape (swarm_adjoint_tape) of a context with n agents and P iterations."""
+    t = dict(x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), vx=np.zeros(n, np.float32), vy=np.zeros(n, np.float32),
+             w=np.zeros(n, np.float32), sid=np.zeros(n, np.uint32), pred=(PredictRec * n)(), fin=(FinalRec * n)(),
+             sx=np.zeros(n, np.float32), sy=np.zeros(n, np.float32), sr=np.zeros(n, np.float32),
+             nbr=np.zeros(TAPE_NBR * n, np.uint32), nbr_count=np.zeros(n, np.uint32), mask=np.zeros(P * n, np.uint64),
+             cap_bits=np.zeros(n, np.uint64), plane_bits=np.zeros(n, np.uint64), overlap=np.zeros(n, np.uint64))
+    view = TapeView()
+    for k, a in t.items():
+        setattr(view, k, C.addressof(a) if isinstance(a, C.Array) else a.ctypes.data)
+    t["view"] = view
+    return t
+
+
+def _sim_adjoint_forward(self, K, every=20, radius_scale=1.0, overlap_weight=0.1):
+    """Runs K steps from the current state keeping checkpoints every `every` steps; returns the loss."""
+    loss = C.c_double()
+    check(_adjoint_lib().swarm_adjoint_forward(self.h, K, every, radius_scale, overlap_weight, C.byref(loss)))
+    return loss.value
+
+
+def _sim_adjoint_backward(self):
+    """(dL/dv0 per agent, [dL/dtau_s, dL/dradius_scale, dL/dwall_gain] per scenario) of the last adjoint_forward."""
+    gv0 = np.zeros(self.n, np.float32)
+    g = np.zeros(3 * getattr(self, "scenarios", 1), np.float64)
+    check(_adjoint_lib().swarm_adjoint_backward(self.h, gv0, g))
+    return gv0, g
+
+
+def _sim_adjoint_tape(self, step, tape, exit_recs=None):
+    check(_adjoint_lib().swarm_adjoint_tape(self.h, step, C.byref(tape["view"]), None if exit_recs is None else C.addressof(exit_recs)))
+    return tape
+
+
+def _sim_set_model(self, tau_s, obstacle_gain=20.0):
+    check(_adjoint_lib().swarm_set_model(self.h, tau_s, obstacle_gain))
+
+
+Sim.adjoint_forward = _sim_adjoint_forward
+Sim.adjoint_backward = _sim_adjoint_backward
+Sim.adjoint_tape = _sim_adjoint_tape
+Sim.set_model = _sim_set_model
+
+
+def _sim_set_adjoint_terms(self, step=None, gin=None, gout=None, exit_term=True, window=0):
+    """Evacuation-crossing terms of the next adjoint_backward (None: none), the exit term, the truncation window."""
+    L = _adjoint_lib()
+    if not getattr(L, "_terms_sigs", False):
+        L.swarm_adjoint_set_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.swarm_read_evac_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L._terms_sigs = True
+    if step is None:
+        check(L.swarm_adjoint_set_terms(self.h, None, None, None, 1 if exit_term else 0, window))
+        return
+    self._terms = (np.ascontiguousarray(step, np.int32), np.ascontiguousarray(gin, np.float32), np.ascontiguousarray(gout, np.float32))
+    s, i, o = self._terms
+    check(L.swarm_adjoint_set_terms(self.h, s.ctypes.data, i.ctypes.data, o.ctypes.data, 1 if exit_term else 0, window))
+
+
+def _sim_evac_points(self):
+    """Per agent: the step it left at, its exit, its position at that step's input and one step earlier (n x 2)."""
+    L = _adjoint_lib()
+    L.swarm_read_evac_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    st, ex = np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+    ins, bef = np.zeros((self.n, 2), np.float32), np.zeros((self.n, 2), np.float32)
+    check(L.swarm_read_evac_points(self.h, st.ctypes.data, ex.ctypes.data, ins.ctypes.data, bef.ctypes.data))
+    return st, ex, ins, bef
+
+
+Sim.set_adjoint_terms = _sim_set_adjoint_terms
+Sim.evac_points = _sim_evac_points

@@ -1,0 +1,201 @@
+"""The GPU adjoint of K steps against its FP64 reference and FP64 central finite differences.
+
+The adjoint differentiates the step with every discrete decision frozen at the forward's values (include/
+swarm_adjoint.h). Its checker is oracle/adjoint_ref.cpp: the same frozen-decision model evaluated in double
+precision from the forward's definitions, giving directional derivatives by forward-mode dual numbers and the loss
+itself for central differences (eps 1e-6 along unit directions in parameter units). On 64 random directions:
+- the GPU gradient (FP32, reverse mode, checkpoints every 20 steps) equals the FP64 product of the step Jacobians
+  along the same recorded trajectory (the implementation's exactness);
+- the FP64 model's forward mode equals its own central differences (the model's differentiability);
+- the GPU gradient equals FP64 central differences to 1e-3 over a horizon where the FP32 and FP64 trajectories
+  stay within rounding of each other.
+Beyond tens of steps this crowd is chaotic (the last test measures it on the plain forward), so the pathwise
+derivative of a long-horizon loss grows exponentially and no finite-difference step resolves it.
+
+Workload: the 40 x 30 m room with two 2 m exits (the small plain configuration's floor), seeded placement; loss
+= sum of squared nav-field distances to the exits at step K + 0.1 x the summed squared overlaps of every step.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as op
+import swarm_hip as sh
+from common import assert_bits
+
+pytestmark = pytest.mark.gpu
+
+P = 32
+# parameter units of a direction: v0 (model speed), tau_s (s), radius scale, wall gain
+UNITS = (0.05, 0.05, 0.02, 1.0)
+
+
+def make(n, K, every=20, seed=1337, scale=1.0, ovw=0.1):
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, seed)
+    sim = sh.Sim(scene, n)
+    sim.upload(a)
+    loss = sim.adjoint_forward(K, every, scale, ovw)
+    return scene, a, sim, loss
+
+
+def reference(scene, a, sim, K, ovw=0.1, scale=1.0, tau_s=0.5, gain=20.0):
+    ref = op.AdjRef(scene.info, scene.tables(), sim.n, P, K, ovw, a, tau_s, scale, gain)
+    tape = sh.new_tape(sim.n, P)
+    ex = (sh.ExitRec * sim.n)()
+    for s in range(K):
+        sim.adjoint_tape(s, tape, ex if s == K - 1 else None)
+        ref.add_step(s, tape["view"])
+    ref.set_exit(ex)
+    return ref
+
+
+def directions(n, m, seed=7):
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((m, n + 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, :n] *= UNITS[0]
+    d[:, n] *= UNITS[1]
+    d[:, n + 1] *= UNITS[2]
+    d[:, n + 2] *= UNITS[3]
+    return d
+
+
+def rel_err(a, b):
+    floor = 1e-2 * np.max(np.abs(b))
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def test_adjoint_forward_is_the_plain_forward():
+    n, K = 1024, 45
+    scene, a, sim, loss = make(n, K)
+    plain = sh.Sim(scene, n)
+    plain.upload(a)
+    tape = sh.new_tape(n, P)
+    for s in range(K):
+        st = plain.state()
+        sim.adjoint_tape(s, tape)
+        for f in ("x", "y", "vx", "vy", "w"):
+            assert_bits(tape[f], st[f], f"taped step {s} input {f}")
+        plain.step(1)
+    assert np.isfinite(loss) and loss > 0.0
+
+
+def check_directions(n, K, m=64, seed=1337, free=True):
+    scene, a, sim, loss = make(n, K, seed=seed)
+    gv0, g = sim.adjoint_backward()
+    ref = reference(scene, a, sim, K)
+    d = directions(n, m)
+    gpu = d @ np.concatenate([gv0.astype(np.float64), g[:3]])
+    along, _ = ref.directional(d, anchored=True)
+    out = dict(loss=loss, gv0=gv0, g=g, gpu=gpu, along=along, ref=ref, sim=sim, a=a, K=K)
+    if free:
+        out["free"], _ = ref.directional(d, anchored=False)
+        eps = 1e-6
+        out["fd"] = np.array([(ref.loss(u / np.linalg.norm(u), eps)[0] - ref.loss(-u / np.linalg.norm(u), eps)[0]) / (2 * eps)
+                              * np.linalg.norm(u) for u in d])
+    return out
+
+
+@pytest.fixture(scope="module")
+def room_run():
+    # the differentiable configuration's floor and agents (4,096 in the 40 x 30 m room), K = 20 steps (within the
+    # crowd's predictability horizon, see test_the_crowd_is_chaotic_beyond_tens_of_steps), checkpoints every 20
+    return check_directions(4096, 20)
+
+
+def test_fp64_model_follows_the_forward(room_run):
+    loss64, traj = room_run["ref"].loss()
+    assert traj < 1e-4, f"FP64 model drifts from the FP32 forward by {traj}"
+    assert loss64 == pytest.approx(room_run["loss"], rel=1e-5)
+
+
+def test_gradient_is_the_tangent_along_the_trajectory(room_run):
+    # the GPU adjoint (FP32, reverse mode, 2 checkpoint segments) against the FP64 forward-mode product of the step
+    # Jacobians along the same recorded trajectory: the implementation's exactness
+    e = rel_err(room_run["gpu"], room_run["along"])
+    print(f"gpu vs FP64 tangent along the trajectory: max rel err {e.max():.2e}")
+    assert e.max() < 1e-4
+
+
+def test_fp64_forward_mode_matches_central_differences(room_run):
+    # the FP64 model's forward mode against its own central differences (eps 1e-6, unit directions in parameter units)
+    e = rel_err(room_run["free"], room_run["fd"])
+    print(f"FP64 forward mode vs central differences: max rel err {e.max():.2e}")
+    assert e.max() < 1e-4
+
+
+def test_gradient_matches_central_differences():
+    # the GPU gradient against FP64 central differences of the model, over a horizon where the FP32 and FP64
+    # trajectories stay within rounding of each other (10 steps: they differ by ~1e-6 of the room)
+    r = check_directions(4096, 10)
+    e = rel_err(r["gpu"], r["fd"])
+    print(f"gpu vs FP64 central differences (K = 10): max rel err {e.max():.2e}, median {np.median(e):.2e}")
+    assert e.max() < 1e-3
+
+
+def test_gradient_near_central_differences_at_20_steps(room_run):
+    # at 20 steps the FP32 and FP64 trajectories have drifted apart by the crowd's amplification of rounding (a few
+    # 1e-5 of the room), and single directions differ from the FP64 differences by a few 1e-3
+    e = rel_err(room_run["gpu"], room_run["fd"])
+    print(f"gpu vs FP64 central differences (K = 20): max rel err {e.max():.2e}, median {np.median(e):.2e}")
+    assert np.median(e) < 1e-3 and e.max() < 1e-2
+
+
+def test_gradient_is_the_tangent_over_longer_horizons():
+    # 60 steps, 3 checkpoint segments: along the recorded trajectory the FP32 adjoint stays the exact tangent up to
+    # FP32 rounding of tangents that have grown by orders of magnitude
+    r = check_directions(4096, 60, m=16, free=False)
+    e = rel_err(r["gpu"], r["along"])
+    print(f"gpu vs FP64 tangent along the trajectory (K = 60): max rel err {e.max():.2e}")
+    assert e.max() < 5e-3
+
+
+def test_the_crowd_is_chaotic_beyond_tens_of_steps():
+    # Why the checks above stop at tens of steps: the plain forward itself (no adjoint involved) amplifies a
+    # 1e-6 perturbation of the initial positions by orders of magnitude within 40 steps in this crowd (4,096
+    # agents in the room: hard-disc contacts), so the pathwise derivative of a K-step loss grows like exp(lambda K)
+    # and central differences at eps 1e-6 leave the linear regime.
+    n = 4096
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, 1337)
+    b = {k: v.copy() for k, v in a.items()}
+    b["x"] = (b["x"] + 1e-6 * np.random.default_rng(3).standard_normal(n)).astype(np.float32)
+    s1, s2 = sh.Sim(scene, n), sh.Sim(scene, n)
+    s1.upload(a)
+    s2.upload(b)
+    s1.step(40)
+    s2.step(40)
+    x1, x2 = s1.state(), s2.state()
+    act = (x1["w"] > 0) & (x2["w"] > 0)
+    growth = np.max(np.hypot(x1["x"][act] - x2["x"][act], x1["y"][act] - x2["y"][act])) / 1e-6
+    print(f"largest displacement after 40 steps / initial perturbation: {growth:.3e}")
+    assert growth > 1e2
+
+
+def test_torch_autograd_wrapper(room_run):
+    torch = pytest.importorskip("torch")
+    sim, a, K = room_run["sim"], room_run["a"], room_run["K"]
+
+    class CrowdLoss(torch.autograd.Function):
+        """The K-step loss as an autograd function of (v0, [tau_s, radius scale, wall gain]) over the C ABI pair."""
+
+        @staticmethod
+        def forward(ctx, v0, glob):
+            st = dict(a)
+            st["v0"] = v0.detach().cpu().numpy().astype(np.float32)
+            sim.set_model(float(glob[0]), float(glob[2]))
+            sim.upload(st)
+            return torch.tensor(sim.adjoint_forward(K, 20, float(glob[1]), 0.1), dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gout):
+            gv0, g = sim.adjoint_backward()
+            return gout * torch.from_numpy(gv0.astype(np.float64)), gout * torch.from_numpy(g[:3])
+
+    v0 = torch.tensor(a["v0"].astype(np.float64), requires_grad=True)
+    glob = torch.tensor([0.5, 1.0, 20.0], dtype=torch.float64, requires_grad=True)
+    loss = CrowdLoss.apply(v0, glob)
+    loss.backward()
+    assert float(loss) == pytest.approx(room_run["loss"], rel=1e-6)
+    np.testing.assert_allclose(v0.grad.numpy(), room_run["gv0"], rtol=1e-5, atol=1e-6 * np.abs(room_run["gv0"]).max())
+    np.testing.assert_allclose(glob.grad.numpy(), room_run["g"][:3], rtol=1e-5)
