@@ -118,6 +118,38 @@ def batch_bench(steps, warmup, loop_steps):
             "speedup_vs_looped_plain": batched / looped}
 
 
+def adjoint_bench(K=200, every=20, n=4096):
+    """The differentiable configuration: 4,096 agents in the 40 x 30 m room, K steps. The forward is K plain steps
+    (graph replays); the adjoint is the adjoint's forward (the same steps with the loss terms and checkpoints) plus
+    its backward (each segment replayed with the tape, then the adjoint kernels)."""
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, SEED)
+    sim = sh.Sim(scene, n)
+    sim.upload(a)
+    sim.step(2)
+    sim.upload(a)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(K)
+    sim.sync()
+    fwd = time.perf_counter() - t0
+    _, _, steps_sum = sim.control()
+    sim.upload(a)
+    sim.adjoint_forward(K, every, 1.0, 0.1)   # warm-up (allocations)
+    sim.adjoint_backward()
+    sim.upload(a)
+    t0 = time.perf_counter()
+    loss = sim.adjoint_forward(K, every, 1.0, 0.1)
+    t1 = time.perf_counter()
+    gv0, g = sim.adjoint_backward()
+    t2 = time.perf_counter()
+    total = t2 - t0
+    return {"K": K, "checkpoint_every": every, "agents": n, "forward_ms": 1000.0 * fwd,
+            "adjoint_forward_ms": 1000.0 * (t1 - t0), "adjoint_backward_ms": 1000.0 * (t2 - t1),
+            "cost_vs_forward": total / fwd, "agent_steps_per_s": steps_sum / total, "loss": loss,
+            "gradient_finite": bool(np.isfinite(gv0).all() and np.isfinite(g).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +163,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--loop-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
@@ -188,6 +221,12 @@ def main():
               f"looped plain {batch['looped_plain']['value'] / 1e6:.1f} M; x{batch['speedup_vs_looped_plain']:.2f}",
               file=sys.stderr, flush=True)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench()
+        print(f"# adjoint K={adjoint['K']}: forward {adjoint['forward_ms']:.1f} ms, adjoint {adjoint['adjoint_forward_ms']:.1f} + "
+              f"{adjoint['adjoint_backward_ms']:.1f} ms ({adjoint['cost_vs_forward']:.2f}x)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, a, args.cpu_steps)
@@ -216,6 +255,7 @@ def main():
                      "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
         "batched": batch,
+        "adjoint": adjoint,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line))

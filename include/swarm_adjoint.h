@@ -108,8 +108,11 @@ int swarm_adjoint_tape(swarm_ctx* c, int step, const swarm_tape_view* out, swarm
  * curves): per agent, the position adjoint gin (2 floats) at the input of step step[a] (the agent is inside its exit
  * there) and gout (2 floats) at the input of step step[a] - 1 (step < 1: none; NULL arrays: no such terms).
  * exit_term: whether the built-in exit-distance term counts. window > 0 truncates the adjoint: its state is cleared
- * every 'window' steps going backwards, so each path of the gradient spans at most that many steps (0: exact). */
-int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window);
+ * every 'window' steps going backwards, so each path of the gradient spans at most that many steps (0: exact).
+ * local != 0: each agent's own chain only (no transfer across agents through contacts, separation or headway, no
+ * contact-projection derivative), a stable biased estimator for horizons where the exact gradient explodes. */
+int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window,
+                            int local);
 /* Per agent: the step it left at (-1: none), its exit, its position at that step's input (inside the exit) and one
  * step earlier (x, y pairs): the crossing of its exit's boundary lies between the two. */
 int swarm_read_evac_points(swarm_ctx* c, int32_t* step, int32_t* exit_id, float* inside, float* before);

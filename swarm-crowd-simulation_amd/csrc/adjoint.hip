@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) k_adj_project(AdjStep t, BatchView bt, Ad
     const float ri = t.sr[i];
     const V2 wi{ wv[i].x, wv[i].y };
     float gr = g.gr[i];
-    for (uint64_t m = t.mask[size_t(it - 1) * n + size_t(i)]; m; m &= m - 1ull) {
+    for (uint64_t m = t.local ? 0ull : t.mask[size_t(it - 1) * n + size_t(i)]; m; m &= m - 1ull) {
         const int k = __ffsll((unsigned long long)m) - 1;
         const uint32_t j = t.nbr[size_t(k) * n + size_t(i)];
         const float2 qj = t.q[size_t(it - 1) * n + size_t(j)];
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) k_adj_predict_a(AdjStep t, SceneView s0, 
         }
         const float2 po = g.gpo[i];
         g.gp[i] = make_float2(gp.x + po.x, gp.y + po.y);
-        g.gsep[i] = make_float2(gsep.x, gsep.y);
+        g.gsep[i] = t.local ? make_float2(0.0f, 0.0f) : make_float2(gsep.x, gsep.y);
         g.gs[i] = gs;
         g.gr[i] = gr;
         g.hvx[a] = gv.x; g.hvy[a] = gv.y;
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(256) k_adj_predict_b(AdjStep t, SceneView s0, 
     const V2 gsi{ gsi2.x, gsi2.y };
     const uint32_t cid = t.skey[i] - sr.cellBase;
     const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
-    const int R = s.percRing;
+    const int R = t.local ? -1 : s.percRing;   // local: no neighbour terms
     const int x0 = cx - R < 0 ? 0 : cx - R;
     const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
     for (int yy = -R; yy <= R; yy++) {

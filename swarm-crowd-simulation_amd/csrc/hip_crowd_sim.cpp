@@ -757,6 +757,7 @@ AdjStep HipCrowdSim::adjStepView(int k) const {
     t.q = tape_.q + K * (P + 1) * n;
     t.planes = tape_.planes + K * n;
     t.v0 = v0_;
+    t.local = local_ ? 1 : 0;
     return t;
 }
 
@@ -842,10 +843,11 @@ void HipCrowdSim::adjointTape(int step, const swarm_tape_view& out, swarm_exit_r
 
 namespace swarm {
 
-void HipCrowdSim::setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window) {
+void HipCrowdSim::setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window, bool local) {
     if (window < 0) throw std::invalid_argument("setAdjointTerms: the window must be >= 0");
     exitTermOn_ = exitTerm;
     window_ = window;
+    local_ = local;
     seedsOn_ = step && gin && gout;
     if (!seedsOn_) return;
     const size_t n = size_t(n_);

@@ -109,9 +109,11 @@ public:
     void setModel(float tauS, float obstacleGain);
     // Loss terms of the next adjointBackward beyond the built-in ones: position adjoints at evacuation crossings
     // (agent a: gin at the input of step step[a], where it is inside its exit; gout at the input of step
-    // step[a] - 1; step < 1: none), whether the exit term counts, and a truncation window (> 0: the state adjoint
-    // is cleared every 'window' steps going backwards, so every path is at most that long; 0: the exact adjoint)
-    void setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window);
+    // step[a] - 1; step < 1: none), whether the exit term counts, a truncation window (> 0: the state adjoint
+    // is cleared every 'window' steps going backwards, so every path is at most that long; 0: the exact adjoint),
+    // and 'local': each agent's own chain only (no transfer across agents through contacts, separation or headway,
+    // no contact-projection derivative): a stable, biased estimator for horizons where the exact one explodes
+    void setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window, bool local = false);
     // per agent: the step it left at, its exit, its position at that step's input and one step earlier (2 floats each)
     void readEvacPoints(int32_t* step, int32_t* exitId, float* inside, float* before);
 
@@ -181,6 +183,7 @@ private:
     float2* seedOut_ = nullptr;
     bool seedsOn_ = false, exitTermOn_ = true;
     int window_ = 0;
+    bool local_ = false;
     int adjK_ = 0, adjEvery_ = 0, adjSegs_ = 0, adjCap_ = 0, tapedSeg_ = -1;
     float adjScale_ = 1.0f;
     size_t snapBytes_ = 0;

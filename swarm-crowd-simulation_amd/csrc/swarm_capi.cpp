@@ -295,9 +295,9 @@ int swarm_adjoint_tape(swarm_ctx* c, int step, const swarm_tape_view* out, swarm
     if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
     return guarded([&] { c->sim->adjointTape(step, *out, exit); });
 }
-int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window) {
+int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window, int local) {
     CTX_CHECK(c);
-    return guarded([&] { c->sim->setAdjointTerms(step, gin, gout, exit_term != 0, window); });
+    return guarded([&] { c->sim->setAdjointTerms(step, gin, gout, exit_term != 0, window, local != 0); });
 }
 int swarm_read_evac_points(swarm_ctx* c, int32_t* step, int32_t* exit_id, float* inside, float* before) {
     CTX_CHECK(c);

@@ -479,19 +479,21 @@ Sim.adjoint_tape = _sim_adjoint_tape
 Sim.set_model = _sim_set_model
 
 
-def _sim_set_adjoint_terms(self, step=None, gin=None, gout=None, exit_term=True, window=0):
-    """Evacuation-crossing terms of the next adjoint_backward (None: none), the exit term, the truncation window."""
+def _sim_set_adjoint_terms(self, step=None, gin=None, gout=None, exit_term=True, window=0, local=False):
+    """Evacuation-crossing terms of the next adjoint_backward (None: none), the exit term, the truncation window,
+    the local (per-agent) estimator."""
     L = _adjoint_lib()
     if not getattr(L, "_terms_sigs", False):
-        L.swarm_adjoint_set_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.swarm_adjoint_set_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.swarm_read_evac_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L._terms_sigs = True
     if step is None:
-        check(L.swarm_adjoint_set_terms(self.h, None, None, None, 1 if exit_term else 0, window))
+        check(L.swarm_adjoint_set_terms(self.h, None, None, None, 1 if exit_term else 0, window, 1 if local else 0))
         return
     self._terms = (np.ascontiguousarray(step, np.int32), np.ascontiguousarray(gin, np.float32), np.ascontiguousarray(gout, np.float32))
     s, i, o = self._terms
-    check(L.swarm_adjoint_set_terms(self.h, s.ctypes.data, i.ctypes.data, o.ctypes.data, 1 if exit_term else 0, window))
+    check(L.swarm_adjoint_set_terms(self.h, s.ctypes.data, i.ctypes.data, o.ctypes.data, 1 if exit_term else 0, window,
+                                    1 if local else 0))
 
 
 def _sim_evac_points(self):
