@@ -109,8 +109,9 @@ int swarm_adjoint_tape(swarm_ctx* c, int step, const swarm_tape_view* out, swarm
  * there) and gout (2 floats) at the input of step step[a] - 1 (step < 1: none; NULL arrays: no such terms).
  * exit_term: whether the built-in exit-distance term counts. window > 0 truncates the adjoint: its state is cleared
  * every 'window' steps going backwards, so each path of the gradient spans at most that many steps (0: exact).
- * local != 0: each agent's own chain only (no transfer across agents through contacts, separation or headway, no
- * contact-projection derivative), a stable biased estimator for horizons where the exact gradient explodes. */
+ * local != 0: each agent's own chain only (no transfer across agents through contacts, separation, headway or
+ * overlaps), cut at every step where the agent touched another agent in the projection (its position is then held
+ * by the crowd): a stable, biased estimator for horizons where the exact gradient explodes. */
 int swarm_adjoint_set_terms(swarm_ctx* c, const int32_t* step, const float* gin, const float* gout, int exit_term, int window,
                             int local);
 /* Per agent: the step it left at (-1: none), its exit, its position at that step's input (inside the exit) and one

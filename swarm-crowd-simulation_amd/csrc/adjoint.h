@@ -30,8 +30,8 @@ struct AdjStep {
     const float2* q;                    // (P + 1) x n
     const float4* planes;
     const float* v0;                    // agent order
-    int local;                          // 1: each agent's own chain only (no contact, separation or headway
-                                        // terms across agents, no contact-projection derivative)
+    int local;                          // 1: each agent's own chain only (no contact, separation, headway or
+                                        // overlap terms across agents), ended at steps where it touched another
 };
 
 struct AdjBufs {

@@ -109,9 +109,14 @@ __global__ void __launch_bounds__(256) k_adj_final(AdjStep t, BatchView bt, AdjB
         return;
     }
     V2 gp{ g.gx[a], g.gy[a] }, gv{ g.gvx[a], g.gvy[a] };
+    if (t.local) {   // local: an agent that touched another this step is held by the crowd; its own chain ends here
+        uint64_t touched = 0ull;
+        for (int it = 1; it <= P; ++it) touched |= t.mask[size_t(it - 1) * size_t(t.n) + size_t(i)];
+        if (touched) { gp = z; gv = z; }
+    }
     float gr = 0.0f;
     const V2 pi{ t.outx[a], t.outy[a] };
-    for (uint64_t m = t.overlap[i]; m; m &= m - 1ull) {
+    for (uint64_t m = t.local ? 0ull : t.overlap[i]; m; m &= m - 1ull) {
         const int k = __ffsll((unsigned long long)m) - 1;
         const uint32_t j = t.nbr[size_t(k) * size_t(t.n) + size_t(i)];
         const uint32_t aj = t.sid[j];

@@ -111,8 +111,9 @@ public:
     // (agent a: gin at the input of step step[a], where it is inside its exit; gout at the input of step
     // step[a] - 1; step < 1: none), whether the exit term counts, a truncation window (> 0: the state adjoint
     // is cleared every 'window' steps going backwards, so every path is at most that long; 0: the exact adjoint),
-    // and 'local': each agent's own chain only (no transfer across agents through contacts, separation or headway,
-    // no contact-projection derivative): a stable, biased estimator for horizons where the exact one explodes
+    // and 'local': each agent's own chain only (no transfer across agents through contacts, separation, headway or
+    // overlaps), cut at every step where the agent touched another (its position is then held by the crowd): a
+    // stable, biased estimator for horizons where the exact one explodes
     void setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window, bool local = false);
     // per agent: the step it left at, its exit, its position at that step's input and one step earlier (2 floats each)
     void readEvacPoints(int32_t* step, int32_t* exitId, float* inside, float* before);
