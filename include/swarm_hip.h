@@ -173,6 +173,10 @@ int swarm_read_evac_counters(swarm_ctx* c, uint32_t* per_exit64);
  * The reference fixes its workgroup size in the shaders (physics.comp:22, local_size_x = 128). */
 int swarm_set_physics_shape(swarm_ctx* c, int block, int grid);
 int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid);
+/* Candidate pruning in the projection: every 'every' iterations (default 8; 0: never) each slot drops the contact
+ * candidates that cannot touch in the iterations left. Results are the same bit for bit either way. No counterpart
+ * in the reference (its projection scans the 3 x 3 cell ring every iteration, physics.comp:1367-1418). */
+int swarm_set_projection_prune(swarm_ctx* c, int every);
 /* Times each candidate physics launch shape over 'steps' steps from the current state, keeps the fastest and
  * restores the state; *best_ms: its mean ms per step. No counterpart in the reference. */
 int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);

@@ -130,6 +130,11 @@ class Calibration:
     def set_target(self, theta):
         self.target = self.forward(theta)["F"]
 
+    def loss(self, theta):
+        """The loss alone (one forward)."""
+        R = self.forward(theta)["F"].mean(axis=0) - self.target.mean(axis=0)
+        return float(np.sum(R ** 2))
+
     def headway_seeds(self, dtau, f):
         """Per-crossing seeds of the headway estimator: +c_j at crossing j and -c_j at j - 1 for consecutive
         crossings of one exit whose crossing points share a window block, c_j the suffix sum of dL/dtau from j on;
@@ -180,9 +185,12 @@ class Calibration:
         return loss, self.per_scenario.sum(axis=0)
 
 
-def calibrate(hidden=(1.20, 0.30, 0.5), start=(1.34, 0.26, 0.7), iterations=30, lr=0.03, cal=None, log=None, **kw):
+def calibrate(hidden=(1.20, 0.30, 0.5), start=(1.34, 0.26, 0.7), iterations=80, lr=0.02, average=0.25, cal=None, log=None,
+              **kw):
     """Fits (mean, sd, tau) to the hidden parameters' exit-flow curves by Adam on parameters scaled by their
-    magnitudes, the step size decaying as a cosine over the iterations. Returns the history and timings."""
+    magnitudes, the step size decaying as a cosine over the iterations; the estimate is the mean of the last
+    'average' fraction of the iterates (Polyak averaging: the loss and its gradient carry the crowd's chaotic
+    scatter, and mean speed and relaxation time trade off along a narrow valley). Returns the history and timings."""
     cal = cal if cal is not None else Calibration(**kw)
     cal.set_target(hidden)
     scale = np.array([1.0, 0.3, 0.5])
@@ -206,7 +214,8 @@ def calibrate(hidden=(1.20, 0.30, 0.5), start=(1.34, 0.26, 0.7), iterations=30, 
         if log is not None:
             log(hist[-1])
     elapsed = time.perf_counter() - t0
-    theta = x * scale
+    tail = max(1, int(round(average * iterations)))
+    theta = np.mean([h["theta"] for h in hist[-tail:]], axis=0)
     final_loss, _ = cal.run(theta)
     return dict(theta=theta.tolist(), hidden=list(hidden), error=(theta - np.array(hidden)).tolist(),
                 start=list(start), final_loss=final_loss, history=hist, seconds=elapsed, iterations=iterations,
@@ -264,8 +273,8 @@ if __name__ == "__main__":
     ap.add_argument("--agents", type=int, default=15832)
     ap.add_argument("--K", type=int, default=400)
     ap.add_argument("--every", type=int, default=25)
-    ap.add_argument("--iterations", type=int, default=30)
-    ap.add_argument("--lr", type=float, default=0.03)
+    ap.add_argument("--iterations", type=int, default=80)
+    ap.add_argument("--lr", type=float, default=0.02)
     ap.add_argument("--window", type=int, default=25)
     ap.add_argument("--estimator", default="headway", choices=("headway", "tau", "local"))
     ap.add_argument("--looped", type=int, default=8)
@@ -276,6 +285,8 @@ if __name__ == "__main__":
                     log=lambda h: print(json.dumps({k: (round(v, 5) if isinstance(v, float) else
                                                         [round(x, 5) for x in v] if isinstance(v, list) else v)
                                                     for k, v in h.items()}), flush=True))
+    # the loss's scatter floor: the hidden parameters moved by 1e-4 (the crowd's chaos alone)
+    res["noise_floor"] = [cal.loss(np.array(res["hidden"]) + d) for d in np.eye(3) * 1e-4]
     res["throughput"] = throughput(cal, res["hidden"], looped=args.looped)
     res["estimator"] = args.estimator if args.estimator == "local" else "%s, window %d" % (args.estimator, args.window)
     print(json.dumps({k: v for k, v in res.items() if k != "history"}), flush=True)

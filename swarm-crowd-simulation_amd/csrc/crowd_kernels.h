@@ -18,6 +18,7 @@ constexpr int PHYS_MAX_BLOCK = 256;
 constexpr int XCDS = 8;            // accelerator complex dies of an MI355X (one L2 each)
 constexpr int NBR_CAP = 48;        // contact candidates per slot
 constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projection thread
+constexpr int PROJ_PRUNE_EVERY = 8; // projection iterations between candidate prunings (0: never)
 constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
 
@@ -165,6 +166,7 @@ struct StepLaunch {
     uint32_t cells = 0;   // global cells
     int physBlock = 0;    // threads per block of the physics passes (0: PHYS_BLOCK)
     int physGrid = 0;     // blocks of the physics passes (0: one slot per thread; else a persistent grid)
+    int pruneEvery = 0;   // > 0: the projection drops candidates out of reach every that many iterations
 };
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,

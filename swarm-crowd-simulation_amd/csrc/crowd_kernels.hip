@@ -687,11 +687,64 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
     finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
+// One iteration over the candidates that also drops those out of reach for the rest of the step: with 'rem'
+// iterations left (this one included) each agent moves by at most rem PROJ_MAX_SHIFT_RADII r, so a pair farther apart
+// than D + rem PROJ_MAX_SHIFT_RADII D (with the collection's slack) cannot touch again. The kept candidates keep
+// their order, so every later sum is the unpruned sum bit for bit (the dropped terms are never contacts).
+__device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
+                                        int rem, float& ox, float& oy, float& oacc) {
+    const float2 bp = in.p[slot];
+    const float acc = in.acc[slot];
+    const uint32_t self = b.sid[slot] - sr.agentBase;
+    const float ri = b.sr[slot];
+    V2 corr{ 0.0f, 0.0f };
+    int nContacts = 0;
+    const size_t stride = size_t(n);
+    const float remf = float(rem);
+    uint32_t kept = 0;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += PROJ_CHUNK) {
+        uint32_t kk[PROJ_CHUNK];
+        float dd[PROJ_CHUNK];
+        float2 pp[PROJ_CHUNK];
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) {
+            const bool in_ = j0 + uint32_t(u) < cnt;
+            const size_t at = size_t(j0 + uint32_t(u)) * stride + size_t(slot);
+            kk[u] = in_ ? b.nbr[at] : uint32_t(slot);
+            dd[u] = in_ ? b.nbrD[at] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) pp[u] = in.p[kk[u]];
+#pragma unroll
+        for (int u = 0; u < PROJ_CHUNK; ++u) {
+            if (j0 + uint32_t(u) >= cnt) break;
+            const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
+            const float dxy2 = dot2(offset, offset);
+            if (dxy2 < contact2(dd[u])) {
+                corr = corr + overlapCorrection(offset, dxy2, dd[u], self, b.sid, kk[u], sr.agentBase);
+                ++nContacts;
+            }
+            const float reach = (dd[u] + remf * (PROJ_MAX_SHIFT_RADII * dd[u])) * 1.001f + 1e-6f;
+            if (dxy2 < reach * reach) {
+                if (kept != j0 + uint32_t(u)) {   // reads of this chunk are done; later chunks sit above 'kept'
+                    b.nbr[size_t(kept) * stride + size_t(slot)] = kk[u];
+                    b.nbrD[size_t(kept) * stride + size_t(slot)] = dd[u];
+                }
+                ++kept;
+            }
+        }
+    }
+    if (kept != cnt) b.nbrCount[slot] = kept;
+    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
+}
+
 // iteration 'first' (1: ring scan + candidates) or a later one (the candidates), slot order in -> out. REC: iteration
 // 'it' of the adjoint's taped forward (contact masks, cap and plane bits)
+// 'prune' > 0 (not REC): the candidates are pruned for the 'prune' iterations left, this one included
 template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
-                                                            ProjBuffers out, int P, int first, TapeStep tp = TapeStep{}, int it = 0) {
+                                                            ProjBuffers out, int P, int first, TapeStep tp = TapeStep{}, int it = 0,
+                                                            int prune = 0) {
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
         const uint32_t key = b.skey[slot];
         if (key == INACTIVE_CELL) continue;
@@ -713,6 +766,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
                 }
             } else {
                 if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc);
+                else if (prune > 0) projectListPrune(b, in, n, slot, cnt, sr, prune, x, y, acc);
                 else projectList(b, in, n, slot, cnt, sr, x, y, acc);
             }
         }
@@ -971,7 +1025,9 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
     if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < P; ++it) {
-        k_project<B><<<g, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P, it == 1 ? 1 : 0);
+        // pruning at iterations 1 + k pruneEvery (k >= 1); the final pass is iteration P
+        const int prune = L.pruneEvery > 0 && it > 1 && (it - 1) % L.pruneEvery == 0 ? P - it + 1 : 0;
+        k_project<B><<<g, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P, it == 1 ? 1 : 0, TapeStep{}, 0, prune);
         const ProjBuffers t = in; in = out; out = t;
     }
     if (stamps) (void)hipEventRecord(stamps[1], st);

@@ -185,6 +185,8 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.cells = uint32_t(C_);
     L.physBlock = physBlock_;
     L.physGrid = physGrid_;
+    // the overlap loss term reads the candidate lists after the step: they stay whole while it is on
+    L.pruneEvery = lossOn_ ? 0 : pruneEvery_;
     return L;
 }
 
@@ -269,6 +271,12 @@ void HipCrowdSim::setPhysicsShape(int block, int grid) {
     if (block != physBlock_ || grid != physGrid_) dropGraph();
     physBlock_ = block;
     physGrid_ = grid;
+}
+
+void HipCrowdSim::setPruneEvery(int every) {
+    if (every < 0) throw std::invalid_argument("setPruneEvery: every must be >= 0");
+    if (every != pruneEvery_) dropGraph();
+    pruneEvery_ = every;
 }
 
 std::vector<std::pair<void*, size_t>> HipCrowdSim::persistentState() const {

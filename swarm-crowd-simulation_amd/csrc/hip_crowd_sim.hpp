@@ -70,6 +70,8 @@ public:
     void setPhysicsShape(int block, int grid);
     int physicsBlock() const { return physBlock_; }
     int physicsGrid() const { return physGrid_; }
+    // Candidate pruning in the projection every 'every' iterations (0: off); results are the same bit for bit
+    void setPruneEvery(int every);
     // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, keeps
     // the fastest and puts the state back as it was (a slot's result does not depend on the shape). Returns the
     // fastest shape's mean ms per step.
@@ -149,6 +151,7 @@ private:
     bool batched_ = false;
     int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
     int physBlock_ = 0, physGrid_ = 0;
+    int pruneEvery_ = PROJ_PRUNE_EVERY;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};

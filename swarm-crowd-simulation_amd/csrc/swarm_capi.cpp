@@ -266,6 +266,10 @@ int swarm_set_physics_shape(swarm_ctx* c, int block, int grid) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setPhysicsShape(block, grid); });
 }
+int swarm_set_projection_prune(swarm_ctx* c, int every) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setPruneEvery(every); });
+}
 int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid) {
     CTX_CHECK(c);
     if (block) *block = c->sim->physicsBlock();

@@ -101,6 +101,7 @@ def lib():
     L.swarm_read_choice_counters_all.argtypes = [vp, _u32p]
     L.swarm_read_evac_counters.argtypes = [vp, _u32p]
     L.swarm_set_physics_shape.argtypes = [vp, C.c_int, C.c_int]
+    L.swarm_set_projection_prune.argtypes = [vp, C.c_int]
     L.swarm_get_physics_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.swarm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.swarm_destroy.argtypes = [vp]
@@ -311,6 +312,10 @@ class Sim:
 
     def set_physics_shape(self, block=0, grid=0):
         check(lib().swarm_set_physics_shape(self.h, block, grid))
+
+    def set_projection_prune(self, every=8):
+        """Candidate pruning every `every` projection iterations (0: never); results are bit-identical."""
+        check(lib().swarm_set_projection_prune(self.h, every))
 
     def physics_shape(self):
         b, g = C.c_int(), C.c_int()

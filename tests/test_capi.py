@@ -61,3 +61,4 @@ def test_batch_create_rejects_bad_arguments():
     assert b"null scene" in L.swarm_last_error()
     assert L.swarm_autotune(None, 3, None) == -1
     assert L.swarm_set_physics_shape(None, 128, 0) == -1
+    assert L.swarm_set_projection_prune(None, 8) == -1

@@ -151,6 +151,24 @@ def test_graph_and_eager_identical():
     assert np.array_equal(s1.assign(), s2.assign())
 
 
+@pytest.mark.parametrize("layout,active,total,steps", [(sh.LAYOUT_ROOM, 4096, 4096, 120), (sh.LAYOUT_BOWL, 63424, 65536, 12)])
+def test_candidate_pruning_is_exact(layout, active, total, steps):
+    """Pruning the projection's contact candidates (every 1 or 8 iterations) gives the unpruned run's bits."""
+    scene = sh.Scene.layout(layout)
+    a = sh.place_agents(layout, active, total, 8)
+    runs = []
+    for every in (0, 1, 8):
+        s = sh.Sim(scene, total)
+        s.set_projection_prune(every)
+        s.upload(a)
+        s.step(steps)
+        runs.append((every, s.state(), s.assign()))
+    for every, st, asg in runs[1:]:
+        for f in FIELDS:
+            assert_bits(st[f], runs[0][1][f], f"prune every {every}: {f}")
+        assert np.array_equal(asg, runs[0][2])
+
+
 def test_sort_grid_on_given_keys():
     """CrowdSim::sortGrid: the sort stage alone on keys of the caller's own, against a stable sort by (key, index)."""
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
