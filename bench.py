@@ -126,6 +126,7 @@ def adjoint_bench(K=200, every=20, n=4096):
     a = sh.place_agents(sh.LAYOUT_ROOM, n, n, SEED)
     sim = sh.Sim(scene, n)
     sim.upload(a)
+    tune_ms = sim.autotune(5)   # launch shape and projection lanes for this crowd (the state is restored)
     sim.step(2)
     sim.upload(a)
     sim.sync()
@@ -144,7 +145,8 @@ def adjoint_bench(K=200, every=20, n=4096):
     gv0, g = sim.adjoint_backward()
     t2 = time.perf_counter()
     total = t2 - t0
-    return {"K": K, "checkpoint_every": every, "agents": n, "forward_ms": 1000.0 * fwd,
+    return {"K": K, "checkpoint_every": every, "agents": n, "physics_shape": list(sim.physics_shape()),
+            "projection_lanes": sim.projection_lanes(), "autotune_ms_per_step": tune_ms, "forward_ms": 1000.0 * fwd,
             "adjoint_forward_ms": 1000.0 * (t1 - t0), "adjoint_backward_ms": 1000.0 * (t2 - t1),
             "cost_vs_forward": total / fwd, "agent_steps_per_s": steps_sum / total, "loss": loss,
             "gradient_finite": bool(np.isfinite(gv0).all() and np.isfinite(g).all())}
@@ -180,6 +182,8 @@ def main():
           f"SDF {scene.info.sdf_nx}x{scene.info.sdf_ny}, nav {scene.info.nav_res_x}x{scene.info.nav_res_y} x {scene.info.E} exits",
           file=sys.stderr, flush=True)
 
+    tune_ms = sim.autotune(5)   # launch shape and projection lanes (the state is restored)
+    shape, lanes = list(sim.physics_shape()), sim.projection_lanes()
     sim.step(args.warmup)
     sim.sync()
     _, _, sum0 = sim.control()
@@ -247,7 +251,8 @@ def main():
         "data": "synthetic: seeded seat placement on the generated bowl floor",
         "config": {"workload": "plain step, 63,424 agents on the 200 x 150 m seated bowl, sort size 65,536",
                    "agents_active": ACTIVE, "agents_total": TOTAL, "hash_cells": cells, "proj_iterations": 32,
-                   "exits": scene.info.E, "parallelism": "single GPU"},
+                   "exits": scene.info.E, "parallelism": "single GPU", "physics_shape": shape, "projection_lanes": lanes,
+                   "autotune_ms_per_step": tune_ms},
         "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,

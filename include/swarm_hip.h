@@ -177,8 +177,16 @@ int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid);
  * candidates that cannot touch in the iterations left. Results are the same bit for bit either way. No counterpart
  * in the reference (its projection scans the 3 x 3 cell ring every iteration, physics.comp:1367-1418). */
 int swarm_set_projection_prune(swarm_ctx* c, int every);
-/* Times each candidate physics launch shape over 'steps' steps from the current state, keeps the fastest and
- * restores the state; *best_ms: its mean ms per step. No counterpart in the reference. */
+/* Lanes per slot in the projection's iterations over the candidate lists (1 default, 2, 4 or 8): the lanes load
+ * and test a slot's candidates together and add the contact terms in candidate order, so results are the same bit
+ * for bit. More lanes pay where slots are few and candidates many (4 lanes: 4,096 agents in the 40 x 30 m room,
+ * 308 -> 289 us per step; 1 lane at the 63,424-agent bowl); swarm_autotune picks it. No counterpart in the
+ * reference. */
+int swarm_set_projection_lanes(swarm_ctx* c, int lanes);
+int swarm_get_projection_lanes(swarm_ctx* c, int* lanes);
+/* Times each candidate physics launch shape over 'steps' steps from the current state, then the projection's lanes
+ * per slot at the fastest shape; keeps the fastest and restores the state; *best_ms: its mean ms per step. No
+ * counterpart in the reference. */
 int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);
 
 /* the context's HIP stream (hipStream_t), for events around swarm_step */

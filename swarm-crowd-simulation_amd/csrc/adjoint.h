@@ -43,6 +43,8 @@ struct AdjBufs {
     float *gpc, *gr, *gs;               // slot
     float *gv0, *gR;                    // agent order, summed over steps
     float *gtau, *ggain;                // per scenario
+    uint32_t *flags, *lead;             // slot: the step's predict flags and headway leader (NBR_OVERFLOW: none),
+                                        // copied out of the predict records by the step's first backward kernel
 };
 
 // the adjoint of the exit term of the final state s
@@ -50,6 +52,11 @@ void launchAdjointSeed(hipStream_t st, int n, AgentHalf s, const swarm_exit_rec*
 // evacuation-crossing seeds at state 'state': gin for agents that left at that step, gout for those that left at
 // the next one (added to g.gx, g.gy)
 void launchAdjointEventSeed(hipStream_t st, int n, int state, const int32_t* seedStep, const float2* gin, const float2* gout, AdjBufs g);
+// the same with state = *base + rel (a captured segment reads its segment's first state from the device)
+void launchAdjointEventSeedRel(hipStream_t st, int n, const int* base, int rel, const int32_t* seedStep, const float2* gin,
+                               const float2* gout, AdjBufs g);
+// *base = value, in stream order
+void launchAdjointSetBase(hipStream_t st, int* base, int value);
 // zeroes the state adjoint g.gx .. g.gvy
 void launchAdjointZero(hipStream_t st, int n, AdjBufs g);
 // one step backwards (g.gx.. in, g.hx.. out)

@@ -92,6 +92,8 @@ __global__ void __launch_bounds__(256) k_adj_final(AdjStep t, BatchView bt, AdjB
     const uint32_t a = t.sid[i];
     const swarm_predict_rec& pr = t.pred[i];
     const V2 z{ 0.0f, 0.0f };
+    g.flags[i] = pr.flags;
+    g.lead[i] = (pr.flags & SWARM_REC_LEADER) ? pr.leader : NBR_OVERFLOW;
     g.gpc[i] = 0.0f; g.gr[i] = 0.0f; g.gs[i] = 0.0f;
     g.gsep[i] = make_float2(0.0f, 0.0f); g.gp[i] = make_float2(0.0f, 0.0f); g.gpo[i] = make_float2(0.0f, 0.0f);
     g.gpv[i] = make_float2(0.0f, 0.0f);
@@ -156,15 +158,15 @@ template <bool B>
 __global__ void __launch_bounds__(256) k_adj_project(AdjStep t, BatchView bt, AdjBufs g, int it) {
     const int i = xcdBlock() * int(blockDim.x) + int(threadIdx.x);
     if (i >= t.n) return;
-    const swarm_predict_rec& pr = t.pred[i];
-    if (!(pr.flags & SWARM_REC_ACTIVE)) return;
+    const uint32_t pf = g.flags[i];
+    if (!(pf & SWARM_REC_ACTIVE)) return;
     const size_t n = size_t(t.n);
     const float2* wv = g.w[it & 1];
     const float2 s = g.gself[it & 1][i];
     V2 gq{ s.x, s.y };
     float2* wNext = g.w[(it - 1) & 1];
     float2* sNext = g.gself[(it - 1) & 1];
-    if (pr.flags & SWARM_REC_PARKED) {   // carried: no contacts, nobody's contact
+    if (pf & SWARM_REC_PARKED) {   // carried: no contacts, nobody's contact
         if (it > 1) { sNext[i] = make_float2(gq.x, gq.y); wNext[i] = make_float2(0.0f, 0.0f); }
         else g.gq[i] = make_float2(gq.x, gq.y);
         return;
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(256) k_adj_project(AdjStep t, BatchView bt, Ad
 template <bool B>
 __global__ void __launch_bounds__(256) k_adj_predict_a(AdjStep t, SceneView s0, BatchView bt, ModelParams mp, AdjBufs g) {
     const int i = xcdBlock() * int(blockDim.x) + int(threadIdx.x);
-    const bool live = i < t.n && (t.pred[i].flags & SWARM_REC_ACTIVE) && !(t.pred[i].flags & SWARM_REC_PARKED);
+    const bool live = i < t.n && (g.flags[i] & SWARM_REC_ACTIVE) && !(g.flags[i] & SWARM_REC_PARKED);
     float gtau = 0.0f, ggain = 0.0f;
     uint32_t scn = 0u;
     if (live) {
@@ -362,8 +364,7 @@ template <bool B>
 __global__ void __launch_bounds__(256) k_adj_predict_b(AdjStep t, SceneView s0, BatchView bt, AdjBufs g) {
     const int i = xcdBlock() * int(blockDim.x) + int(threadIdx.x);
     if (i >= t.n) return;
-    const swarm_predict_rec& pr = t.pred[i];
-    if (!(pr.flags & SWARM_REC_ACTIVE)) return;
+    if (!(g.flags[i] & SWARM_REC_ACTIVE)) return;
     const uint32_t a = t.sid[i];
     const ScnRef sr = scnOfSlot<B>(bt, uint32_t(i));
     const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
@@ -411,8 +412,7 @@ __global__ void __launch_bounds__(256) k_adj_predict_b(AdjStep t, SceneView s0, 
                     gr += dmS * nd;
                 }
             }
-            const swarm_predict_rec& pk = t.pred[k];
-            if ((pk.flags & SWARM_REC_LEADER) && pk.leader == uint32_t(i)) {   // slot k's headway leader is this slot
+            if (g.lead[k] == uint32_t(i)) {   // slot k's headway leader is this slot
                 const float gsk = g.gs[k];
                 if (gsk != 0.0f) {
                     gp = gp + off * (gsk / sqrtf(d2));
@@ -451,6 +451,20 @@ __global__ void __launch_bounds__(256) k_adj_event_seed(int n, int state, const 
     else if (s == state + 1) { g.gx[i] += gout[i].x; g.gy[i] += gout[i].y; }
 }
 
+// the same with the state taken from a device word (base + rel): one captured graph serves every segment
+__global__ void __launch_bounds__(256) k_adj_event_seed_rel(int n, const int* __restrict__ base, int rel,
+                                                            const int32_t* __restrict__ seedStep, const float2* __restrict__ gin,
+                                                            const float2* __restrict__ gout, AdjBufs g) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int state = *base + rel;
+    const int32_t s = seedStep[i];
+    if (s == state) { g.gx[i] += gin[i].x; g.gy[i] += gin[i].y; }
+    else if (s == state + 1) { g.gx[i] += gout[i].x; g.gy[i] += gout[i].y; }
+}
+
+__global__ void k_adj_set_int(int* __restrict__ p, int v) { *p = v; }
+
 __global__ void __launch_bounds__(256) k_adj_zero(int n, AdjBufs g) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
@@ -477,6 +491,13 @@ void launchAdjointSeed(hipStream_t st, int n, AgentHalf s, const swarm_exit_rec*
 void launchAdjointEventSeed(hipStream_t st, int n, int state, const int32_t* seedStep, const float2* gin, const float2* gout, AdjBufs g) {
     k_adj_event_seed<<<blocks256(n), 256, 0, st>>>(n, state, seedStep, gin, gout, g);
 }
+
+void launchAdjointEventSeedRel(hipStream_t st, int n, const int* base, int rel, const int32_t* seedStep, const float2* gin,
+                               const float2* gout, AdjBufs g) {
+    k_adj_event_seed_rel<<<blocks256(n), 256, 0, st>>>(n, base, rel, seedStep, gin, gout, g);
+}
+
+void launchAdjointSetBase(hipStream_t st, int* base, int value) { k_adj_set_int<<<1, 1, 0, st>>>(base, value); }
 
 void launchAdjointZero(hipStream_t st, int n, AdjBufs g) { k_adj_zero<<<blocks256(n), 256, 0, st>>>(n, g); }
 

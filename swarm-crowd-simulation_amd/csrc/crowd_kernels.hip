@@ -738,6 +738,113 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
     finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
 }
 
+// A later iteration with Q lanes per slot (the production step's iterations that do not prune). Lane l of a slot
+// loads and tests candidates j = l, l + Q, ..., PROJ_CHUNK of them per round, so Q x PROJ_CHUNK candidates are in
+// flight together; every lane then adds the contact terms in candidate order (j ascending), reading the other lanes'
+// terms by shuffles within the slot's Q lanes, so the sum is projectList's bit for bit. Lane 0 finishes and writes.
+// A slot's Q lanes are adjacent lanes of one wave and follow the same path.
+// REC: iteration 'it' of the adjoint's taped forward (contact masks, cap and plane bits, as k_project<B, true>)
+template <bool B, int Q, bool REC = false>
+__global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
+                                                                  ProjBuffers out, int P, TapeStep tp = TapeStep{}, int it = 0) {
+    static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
+    const int total = n * Q;
+    const size_t stride = size_t(n);
+    // blockDim and the grid stride are multiples of Q, so a slot's lanes take every branch together
+    for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int slot = t / Q;
+        const int lane = t % Q;
+        const uint32_t key = b.skey[slot];
+        if (key == INACTIVE_CELL) continue;
+        const float2 bp = in.p[slot];
+        const float acc = in.acc[slot];
+        if (b.pw[slot] == 0.0f) {   // parked this step: carried
+            if (lane == 0) {
+                out.p[slot] = bp; out.acc[slot] = acc;
+                if constexpr (REC) tp.mask[size_t(it - 1) * stride + size_t(slot)] = 0ull;
+            }
+            continue;
+        }
+        const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
+        const uint32_t cnt = b.nbrCount[slot];
+        if (cnt == NBR_OVERFLOW) {   // the ring scan, on lane 0
+            if (lane == 0) {
+                float x, y, oacc;
+                if constexpr (REC) {
+                    uint64_t mask = 0ull;
+                    bool capped = false, planed = false;
+                    projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, false, x, y, oacc, &mask, &capped, &planed);
+                    atomicOr(tp.err, 1u);
+                    tp.mask[size_t(it - 1) * stride + size_t(slot)] = mask;
+                } else {
+                    projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, false, x, y, oacc);
+                }
+                out.p[slot] = make_float2(x, y);
+                out.acc[slot] = oacc;
+            }
+            continue;
+        }
+        const uint32_t self = b.sid[slot] - sr.agentBase;
+        V2 corr{ 0.0f, 0.0f };
+        int nContacts = 0;
+        uint64_t mask = 0ull;   // REC: the contacts' indices among the candidates
+        for (uint32_t j0 = 0; j0 < cnt; j0 += uint32_t(Q * PROJ_CHUNK)) {
+            uint32_t kk[PROJ_CHUNK];
+            float dd[PROJ_CHUNK];
+            float2 pp[PROJ_CHUNK];
+#pragma unroll
+            for (int c = 0; c < PROJ_CHUNK; ++c) {
+                const uint32_t j = j0 + uint32_t(c * Q + lane);
+                const bool in_ = j < cnt;
+                kk[c] = in_ ? b.nbr[size_t(j) * stride + size_t(slot)] : uint32_t(slot);
+                dd[c] = in_ ? b.nbrD[size_t(j) * stride + size_t(slot)] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < PROJ_CHUNK; ++c) pp[c] = in.p[kk[c]];
+            float tx[PROJ_CHUNK], ty[PROJ_CHUNK];
+            int hit[PROJ_CHUNK];
+#pragma unroll
+            for (int c = 0; c < PROJ_CHUNK; ++c) {
+                const uint32_t j = j0 + uint32_t(c * Q + lane);
+                const V2 offset = V2{ bp.x, bp.y } - V2{ pp[c].x, pp[c].y };
+                const float dxy2 = dot2(offset, offset);
+                hit[c] = j < cnt && dxy2 < contact2(dd[c]);
+                V2 term{ 0.0f, 0.0f };
+                if (hit[c]) term = overlapCorrection(offset, dxy2, dd[c], self, b.sid, kk[c], sr.agentBase);
+                tx[c] = term.x;
+                ty[c] = term.y;
+            }
+            // candidate order: c outer, lane inner
+#pragma unroll
+            for (int c = 0; c < PROJ_CHUNK; ++c) {
+#pragma unroll
+                for (int l = 0; l < Q; ++l) {
+                    const int h = __shfl(hit[c], l, Q);
+                    const float x = __shfl(tx[c], l, Q);
+                    const float y = __shfl(ty[c], l, Q);
+                    if (h) {
+                        corr = corr + V2{ x, y };
+                        ++nContacts;
+                        if constexpr (REC) mask |= 1ull << (j0 + uint32_t(c * Q + l));
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            float x, y, oacc;
+            bool capped = false, planed = false;
+            finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, b.sr[slot], x, y, oacc, &capped, &planed);
+            out.p[slot] = make_float2(x, y);
+            out.acc[slot] = oacc;
+            if constexpr (REC) {
+                tp.mask[size_t(it - 1) * stride + size_t(slot)] = mask;
+                if (capped) tp.capBits[slot] |= 1ull << (it - 1);
+                if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
+            }
+        }
+    }
+}
+
 // iteration 'first' (1: ring scan + candidates) or a later one (the candidates), slot order in -> out. REC: iteration
 // 'it' of the adjoint's taped forward (contact masks, cap and plane bits)
 // 'prune' > 0 (not REC): the candidates are pruned for the 'prune' iterations left, this one included
@@ -1027,7 +1134,16 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
     for (int it = 1; it < P; ++it) {
         // pruning at iterations 1 + k pruneEvery (k >= 1); the final pass is iteration P
         const int prune = L.pruneEvery > 0 && it > 1 && (it - 1) % L.pruneEvery == 0 ? P - it + 1 : 0;
-        k_project<B><<<g, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P, it == 1 ? 1 : 0, TapeStep{}, 0, prune);
+        if (it == 1 || prune > 0 || L.projLanes <= 1) {
+            k_project<B><<<g, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P, it == 1 ? 1 : 0, TapeStep{}, 0, prune);
+        } else {
+            const int q = L.projLanes;
+            const int full = blocks(n * q, bs);
+            const int gq = L.physGrid > 0 && L.physGrid < full ? L.physGrid : full;
+            if (q == 2) k_project_lanes<B, 2><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+            else if (q == 4) k_project_lanes<B, 4><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+            else k_project_lanes<B, 8><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+        }
         const ProjBuffers t = in; in = out; out = t;
     }
     if (stamps) (void)hipEventRecord(stamps[1], st);
@@ -1050,7 +1166,14 @@ static void physicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, f
     float* acc[2] = { acc0, acc1 };
     auto at = [&](int it) { return ProjBuffers{ q + size_t(it) * size_t(n), acc[it & 1] }; };
     k_predict<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, at(0), tp);
-    for (int it = 1; it < P; ++it) k_project<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, it == 1 ? 1 : 0, tp, it);
+    const int lanes = L.projLanes;
+    const int gq = blocks(n * lanes, bs);
+    for (int it = 1; it < P; ++it) {
+        if (it == 1 || lanes <= 1) k_project<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, it == 1 ? 1 : 0, tp, it);
+        else if (lanes == 2) k_project_lanes<B, 2, true><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, tp, it);
+        else if (lanes == 4) k_project_lanes<B, 4, true><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, tp, it);
+        else k_project_lanes<B, 8, true><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, at(it - 1), at(it), P, tp, it);
+    }
     k_final<B, true><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, at(P - 1), 1, rd, wr, ev, ctrl, tp, at(P));
 }
 

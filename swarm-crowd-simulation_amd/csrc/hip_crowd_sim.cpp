@@ -187,6 +187,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.physGrid = physGrid_;
     // the overlap loss term reads the candidate lists after the step: they stay whole while it is on
     L.pruneEvery = lossOn_ ? 0 : pruneEvery_;
+    L.projLanes = projLanes_;
     return L;
 }
 
@@ -201,8 +202,20 @@ HipCrowdSim::~HipCrowdSim() {
 }
 
 void HipCrowdSim::dropGraph() {
-    if (graphExec_) { (void)hipGraphExecDestroy(graphExec_); graphExec_ = nullptr; }
-    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    for (int i = 0; i < 2; ++i) {
+        if (graphExec_[i]) { (void)hipGraphExecDestroy(graphExec_[i]); graphExec_[i] = nullptr; }
+        if (graph_[i]) { (void)hipGraphDestroy(graph_[i]); graph_[i] = nullptr; }
+    }
+    dropAdjointGraphs();
+}
+
+void HipCrowdSim::dropAdjointGraphs() {
+    for (int i = 0; i < 2; ++i) {
+        if (replayExec_[i]) { (void)hipGraphExecDestroy(replayExec_[i]); replayExec_[i] = nullptr; }
+        if (replayGraph_[i]) { (void)hipGraphDestroy(replayGraph_[i]); replayGraph_[i] = nullptr; }
+        if (backExec_[i]) { (void)hipGraphExecDestroy(backExec_[i]); backExec_[i] = nullptr; }
+        if (backGraph_[i]) { (void)hipGraphDestroy(backGraph_[i]); backGraph_[i] = nullptr; }
+    }
 }
 
 void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r,
@@ -273,6 +286,12 @@ void HipCrowdSim::setPhysicsShape(int block, int grid) {
     physGrid_ = grid;
 }
 
+void HipCrowdSim::setProjectionLanes(int lanes) {
+    if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setProjectionLanes: 1, 2, 4 or 8");
+    if (lanes != projLanes_) dropGraph();
+    projLanes_ = lanes;
+}
+
 void HipCrowdSim::setPruneEvery(int every) {
     if (every < 0) throw std::invalid_argument("setPruneEvery: every must be >= 0");
     if (every != pruneEvery_) dropGraph();
@@ -325,16 +344,30 @@ double HipCrowdSim::autotune(int steps) {
     timing_ = false;
     double best = 0.0;
     std::pair<int, int> bestShape = { physBlock_, physGrid_ };
+    int bestLanes = projLanes_;
+    auto trial = [&]() {
+        step(2);   // graph build and warm-up
+        const double ms = timeSteps(steps) / double(steps);
+        copyAll(false);
+        parity_ = parity0;
+        return ms;
+    };
     try {
         copyAll(true);
         for (const auto& sh : shapes) {
             setPhysicsShape(sh.first, sh.second);
-            step(2);   // graph build and warm-up
-            const double ms = timeSteps(steps) / double(steps);
+            const double ms = trial();
             if (best == 0.0 || ms < best) { best = ms; bestShape = sh; }
-            copyAll(false);
-            parity_ = parity0;
         }
+        // then, at the fastest shape, the projection's lanes per slot
+        setPhysicsShape(bestShape.first, bestShape.second);
+        for (int lanes : { 1, 2, 4, 8 }) {
+            if (lanes == bestLanes) continue;
+            setProjectionLanes(lanes);
+            const double ms = trial();
+            if (ms < best) { best = ms; bestLanes = lanes; }
+        }
+        setProjectionLanes(bestLanes);
     } catch (...) {
         (void)hipStreamSynchronize(stream_);
         (void)hipFree(snap);
@@ -407,11 +440,13 @@ int HipCrowdSim::readTimings(double* ms8) {
 }
 
 void HipCrowdSim::buildGraph() {
+    const int g = lossOn_ ? 1 : 0;
     SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     enqueueStep(0);
     enqueueStep(1);
-    SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
-    SWARM_HIP_CHECK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+    SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &graph_[g]));
+    SWARM_HIP_CHECK(hipGraphInstantiate(&graphExec_[g], graph_[g], nullptr, nullptr, 0));
+    if (lossOn_) lossGraphOvw_ = overlapWeight_;
 }
 
 void HipCrowdSim::step(int nsteps) {
@@ -419,9 +454,10 @@ void HipCrowdSim::step(int nsteps) {
     int left = nsteps;
     if (opt_.useGraph && !timing_ && left >= 2) {
         if (parity_ == 1) { enqueueStep(1); parity_ = 0; --left; }
-        if (!graphExec_) buildGraph();
+        const int g = lossOn_ ? 1 : 0;
+        if (!graphExec_[g]) buildGraph();
         while (left >= 2) {
-            SWARM_HIP_CHECK(hipGraphLaunch(graphExec_, stream_));
+            SWARM_HIP_CHECK(hipGraphLaunch(graphExec_[g], stream_));
             left -= 2;
         }
     }
@@ -626,13 +662,18 @@ void HipCrowdSim::allocAdjoint(int every) {
         for (float2** f : slot2) *f = static_cast<float2*>(dalloc(n * 8));
         float** slot1[] = { &adj_.gpc, &adj_.gr, &adj_.gs };
         for (float** f : slot1) *f = static_cast<float*>(dalloc(n * 4));
+        adj_.flags = static_cast<uint32_t*>(dalloc(n * 4));
+        adj_.lead = static_cast<uint32_t*>(dalloc(n * 4));
         adj_.gtau = static_cast<float*>(dalloc(4 * size_t(S_)));
         adj_.ggain = static_cast<float*>(dalloc(4 * size_t(S_)));
         tape_.acc0 = static_cast<float*>(dalloc(n * 4));
         tape_.acc1 = static_cast<float*>(dalloc(n * 4));
         tape_.err = static_cast<uint32_t*>(dalloc(16));
+        stateBase_ = static_cast<int*>(dalloc(16));
+        adjG0_ = adj_.gx;
     }
     if (every <= adjCap_) return;
+    dropAdjointGraphs();   // they hold the old tape's addresses
     // a segment's tape: 'every' steps
     const size_t C = size_t(every), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
     float** f1[] = { &tape_.inx, &tape_.iny, &tape_.invx, &tape_.invy, &tape_.inw, &tape_.outx, &tape_.outy, &tape_.sx, &tape_.sy,
@@ -670,6 +711,7 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     std::vector<float> r(r0_.size());
     for (size_t i = 0; i < r.size(); ++i) r[i] = radiusScale * r0_[i];
     SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r.data(), r.size() * 4, hipMemcpyHostToDevice, stream_));
+    if (std::min(every, K) != adjEvery_ || overlapWeight != overlapWeight_) dropAdjointGraphs();
     adjK_ = K;
     adjEvery_ = std::min(every, K);
     adjSegs_ = (K + adjEvery_ - 1) / adjEvery_;
@@ -685,7 +727,11 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     }
     ckptParity_.assign(size_t(adjSegs_), 0);
     SWARM_HIP_CHECK(hipMemsetAsync(lossAcc_, 0, 8 * size_t(S_), stream_));
-    if (!lossOn_) { lossOn_ = true; dropGraph(); }
+    lossOn_ = true;   // graph_[1]: the step with the loss terms, for this overlap weight
+    if (graphExec_[1] && lossGraphOvw_ != overlapWeight_) {
+        (void)hipGraphExecDestroy(graphExec_[1]); graphExec_[1] = nullptr;
+        (void)hipGraphDestroy(graph_[1]); graph_[1] = nullptr;
+    }
     for (int seg = 0; seg < adjSegs_; ++seg) {
         snapshotTo(ckpt_[size_t(seg)]);
         ckptParity_[size_t(seg)] = parity_;
@@ -741,7 +787,20 @@ void HipCrowdSim::tapeSegment(int seg) {
     parity_ = ckptParity_[size_t(seg)];
     SWARM_HIP_CHECK(hipMemsetAsync(tape_.err, 0, 4, stream_));
     const int s0 = seg * adjEvery_, s1 = std::min(adjK_, s0 + adjEvery_);
-    for (int k = 0; k < s1 - s0; ++k) tapedStep(k);
+    const int len = s1 - s0;
+    if (opt_.useGraph && len == adjEvery_) {   // a full segment: one captured replay per starting parity
+        const int p = parity_;
+        if (!replayExec_[p]) {
+            SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < len; ++k) tapedStep(k);
+            SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &replayGraph_[p]));
+            SWARM_HIP_CHECK(hipGraphInstantiate(&replayExec_[p], replayGraph_[p], nullptr, nullptr, 0));
+        }
+        SWARM_HIP_CHECK(hipGraphLaunch(replayExec_[p], stream_));
+        parity_ = (len & 1) ? 1 - p : p;
+    } else {
+        for (int k = 0; k < len; ++k) tapedStep(k);
+    }
     uint32_t err = 0;
     SWARM_HIP_CHECK(hipMemcpyAsync(&err, tape_.err, 4, hipMemcpyDeviceToHost, stream_));
     sync();
@@ -785,17 +844,29 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.gR, 0, n * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.gtau, 0, S * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.ggain, 0, S * 4, stream_));
-    const StepLaunch L = launchDesc();
     for (int seg = adjSegs_ - 1; seg >= 0; --seg) {
         tapeSegment(seg);
         const int len = std::min(adjK_, (seg + 1) * adjEvery_) - seg * adjEvery_;
-        for (int k = len - 1; k >= 0; --k) {
-            const int s = seg * adjEvery_ + k;   // the step; its output is state s + 1
-            if (seedsOn_) launchAdjointEventSeed(stream_, n_, s + 1, seedStep_, seedIn_, seedOut_, adj_);
-            launchAdjointStep(stream_, L, adjStepView(k), adj_, overlapWeight_);
-            std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
-            std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
-            if (window_ > 0 && s % window_ == 0) launchAdjointZero(stream_, n_, adj_);
+        // a full segment whose clearing pattern repeats from segment to segment: one captured backward per
+        // ping-pong half; the segment's first state comes from the device word stateBase_
+        if (opt_.useGraph && len == adjEvery_ && (window_ == 0 || adjEvery_ % window_ == 0)) {
+            launchAdjointSetBase(stream_, stateBase_, seg * adjEvery_);
+            const int h = adj_.gx == adjG0_ ? 0 : 1;
+            const AdjBufs before = adj_;
+            if (!backExec_[h]) {
+                SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                backwardSegment(-1, len);
+                SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &backGraph_[h]));
+                SWARM_HIP_CHECK(hipGraphInstantiate(&backExec_[h], backGraph_[h], nullptr, nullptr, 0));
+                adj_ = before;
+            }
+            SWARM_HIP_CHECK(hipGraphLaunch(backExec_[h], stream_));
+            if (len & 1) {
+                std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
+                std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
+            }
+        } else {
+            backwardSegment(seg, len);
         }
         SWARM_HIP_CHECK(hipGetLastError());
     }
@@ -820,6 +891,23 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
         gradGlobals[3 * s + 0] = double(gtau[s]) / double(TIME_SCALE_S_PER_UNIT);
         gradGlobals[3 * s + 1] = gs[s];
         gradGlobals[3 * s + 2] = double(ggain[s]);
+    }
+}
+
+// the backward of one segment's 'len' steps from its tape; seg < 0: the segment's first state is read from stateBase_
+// (the captured form), and the clearing pattern is that of a segment starting at a multiple of the window
+void HipCrowdSim::backwardSegment(int seg, int len) {
+    const StepLaunch L = launchDesc();
+    for (int k = len - 1; k >= 0; --k) {
+        const int rel = seg < 0 ? k : seg * adjEvery_ + k;   // the step (relative to the segment when captured)
+        if (seedsOn_) {
+            if (seg < 0) launchAdjointEventSeedRel(stream_, n_, stateBase_, k + 1, seedStep_, seedIn_, seedOut_, adj_);
+            else launchAdjointEventSeed(stream_, n_, rel + 1, seedStep_, seedIn_, seedOut_, adj_);
+        }
+        launchAdjointStep(stream_, L, adjStepView(k), adj_, overlapWeight_);
+        std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
+        std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
+        if (window_ > 0 && rel % window_ == 0) launchAdjointZero(stream_, n_, adj_);
     }
 }
 
@@ -853,6 +941,7 @@ namespace swarm {
 
 void HipCrowdSim::setAdjointTerms(const int32_t* step, const float* gin, const float* gout, bool exitTerm, int window, bool local) {
     if (window < 0) throw std::invalid_argument("setAdjointTerms: the window must be >= 0");
+    if (window != window_ || local != local_ || (step && gin && gout) != seedsOn_) dropAdjointGraphs();
     exitTermOn_ = exitTerm;
     window_ = window;
     local_ = local;

@@ -72,9 +72,12 @@ public:
     int physicsGrid() const { return physGrid_; }
     // Candidate pruning in the projection every 'every' iterations (0: off); results are the same bit for bit
     void setPruneEvery(int every);
-    // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, keeps
-    // the fastest and puts the state back as it was (a slot's result does not depend on the shape). Returns the
-    // fastest shape's mean ms per step.
+    // Lanes per slot in the projection's non-pruning iterations (1, 2, 4, 8); results are the same bit for bit
+    void setProjectionLanes(int lanes);
+    int projectionLanes() const { return projLanes_; }
+    // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, then the
+    // projection's lanes per slot at the fastest shape; keeps the fastest and puts the state back as it was (a
+    // slot's result depends on neither). Returns the fastest setting's mean ms per step.
     double autotune(int steps);
 
     // Per-pass GPU timing (CrowdSim::setTimingEnabled / readTimings, crowd_sim.hpp:141-149): while enabled every
@@ -132,6 +135,8 @@ private:
     void enqueueStep(int parity);
     void buildGraph();
     void dropGraph();
+    void dropAdjointGraphs();
+    void backwardSegment(int seg, int len);
     StepLaunch launchDesc() const;
     template <class T> T* alloc(size_t count);
     void collectTimings();
@@ -152,6 +157,7 @@ private:
     int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
     int physBlock_ = 0, physGrid_ = 0;
     int pruneEvery_ = PROJ_PRUNE_EVERY;
+    int projLanes_ = PROJ_LANES;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};
@@ -166,8 +172,16 @@ private:
     BatchView bt_{};
     uint32_t* ctrl_ = nullptr;
     int parity_ = 0;   // the half holding the current state
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graphExec_ = nullptr;
+    // the step's graphs (two steps, both parities), without and with the adjoint's loss terms
+    hipGraph_t graph_[2]{};
+    hipGraphExec_t graphExec_[2]{};
+    float lossGraphOvw_ = 0.0f;          // the overlap weight captured in graph_[1]
+    // the adjoint's graphs: a full segment's taped replay (by starting parity) and backward (by which half of the
+    // state adjoint's ping-pong holds the output), for the current segment length and loss terms
+    hipGraph_t replayGraph_[2]{}, backGraph_[2]{};
+    hipGraphExec_t replayExec_[2]{}, backExec_[2]{};
+    int* stateBase_ = nullptr;           // device: the first state of the segment a captured backward runs
+    float* adjG0_ = nullptr;             // adj_.gx as allocated (the ping-pong's reference half)
     // timing: events per stamp (start, sorted, choice, predict, projection, final) per recorded step
     bool timing_ = false;
     std::vector<std::vector<hipEvent_t>> pendingStamps_;

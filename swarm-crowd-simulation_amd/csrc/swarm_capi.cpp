@@ -270,6 +270,16 @@ int swarm_set_projection_prune(swarm_ctx* c, int every) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setPruneEvery(every); });
 }
+int swarm_set_projection_lanes(swarm_ctx* c, int lanes) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setProjectionLanes(lanes); });
+}
+int swarm_get_projection_lanes(swarm_ctx* c, int* lanes) {
+    CTX_CHECK(c);
+    if (!lanes) { g_err = "null argument"; return SWARM_EINVAL; }
+    *lanes = c->sim->projectionLanes();
+    return SWARM_OK;
+}
 int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid) {
     CTX_CHECK(c);
     if (block) *block = c->sim->physicsBlock();

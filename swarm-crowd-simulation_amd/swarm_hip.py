@@ -102,6 +102,8 @@ def lib():
     L.swarm_read_evac_counters.argtypes = [vp, _u32p]
     L.swarm_set_physics_shape.argtypes = [vp, C.c_int, C.c_int]
     L.swarm_set_projection_prune.argtypes = [vp, C.c_int]
+    L.swarm_set_projection_lanes.argtypes = [vp, C.c_int]
+    L.swarm_get_projection_lanes.argtypes = [vp, C.c_void_p]
     L.swarm_get_physics_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.swarm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.swarm_destroy.argtypes = [vp]
@@ -316,6 +318,15 @@ class Sim:
     def set_projection_prune(self, every=8):
         """Candidate pruning every `every` projection iterations (0: never); results are bit-identical."""
         check(lib().swarm_set_projection_prune(self.h, every))
+
+    def set_projection_lanes(self, lanes=1):
+        """Lanes per slot in the projection's list iterations (1, 2, 4, 8); bit-identical results."""
+        check(lib().swarm_set_projection_lanes(self.h, lanes))
+
+    def projection_lanes(self):
+        v = C.c_int()
+        check(lib().swarm_get_projection_lanes(self.h, C.byref(v)))
+        return v.value
 
     def physics_shape(self):
         b, g = C.c_int(), C.c_int()

@@ -62,3 +62,4 @@ def test_batch_create_rejects_bad_arguments():
     assert L.swarm_autotune(None, 3, None) == -1
     assert L.swarm_set_physics_shape(None, 128, 0) == -1
     assert L.swarm_set_projection_prune(None, 8) == -1
+    assert L.swarm_set_projection_lanes(None, 4) == -1

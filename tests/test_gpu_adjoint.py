@@ -208,3 +208,32 @@ def test_torch_autograd_wrapper(room_run):
     assert float(loss) == pytest.approx(room_run["loss"], rel=1e-6)
     np.testing.assert_allclose(v0.grad.numpy(), room_run["gv0"], rtol=1e-5, atol=1e-6 * np.abs(room_run["gv0"]).max())
     np.testing.assert_allclose(glob.grad.numpy(), room_run["g"][:3], rtol=1e-5)
+
+
+def test_captured_segments_match_eager():
+    # full segments replay and run backwards as captured graphs (one per parity), the partial last one eagerly;
+    # dL/dv0 equals the all-eager one bit for bit, also on a second backward (cached graphs) and with crossing seeds
+    # and a truncation window; the per-scenario globals (relaxation time, wall gain) sum wave partials with float
+    # atomics, whose order differs from run to run
+    n, K = 1024, 45
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, 5)
+    rng = np.random.default_rng(2)
+    steps = rng.integers(-1, K + 1, n).astype(np.int32)
+    gin = rng.standard_normal((n, 2)).astype(np.float32)
+    gout = rng.standard_normal((n, 2)).astype(np.float32)
+    out = {}
+    for graph in (True, False):
+        sim = sh.Sim(scene, n, use_graph=graph)
+        sim.upload(a)
+        loss = sim.adjoint_forward(K, 20, 1.0, 0.1)
+        r1 = sim.adjoint_backward()
+        r2 = sim.adjoint_backward()
+        sim.set_adjoint_terms(steps, gin, gout, exit_term=False, window=10)
+        r3 = sim.adjoint_backward()
+        out[graph] = (loss, r1, r2, r3)
+    assert out[True][0] == out[False][0]
+    for i in (1, 2, 3):
+        assert_bits(out[True][i][0], out[False][i][0], f"dL/dv0, backward {i}")
+        np.testing.assert_allclose(out[True][i][1], out[False][i][1], rtol=1e-5)
+    assert_bits(out[True][1][0], out[True][2][0], "second backward")

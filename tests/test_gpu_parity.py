@@ -153,19 +153,21 @@ def test_graph_and_eager_identical():
 
 @pytest.mark.parametrize("layout,active,total,steps", [(sh.LAYOUT_ROOM, 4096, 4096, 120), (sh.LAYOUT_BOWL, 63424, 65536, 12)])
 def test_candidate_pruning_is_exact(layout, active, total, steps):
-    """Pruning the projection's contact candidates (every 1 or 8 iterations) gives the unpruned run's bits."""
+    """Pruning the projection's contact candidates (every 1, 4 or 8 iterations) and 2, 4 or 8 lanes per slot give
+    the single-lane, unpruned run's bits."""
     scene = sh.Scene.layout(layout)
     a = sh.place_agents(layout, active, total, 8)
     runs = []
-    for every in (0, 1, 8):
+    for every, lanes in ((0, 1), (1, 1), (4, 1), (8, 1), (0, 2), (0, 4), (8, 4), (0, 8)):
         s = sh.Sim(scene, total)
         s.set_projection_prune(every)
+        s.set_projection_lanes(lanes)
         s.upload(a)
         s.step(steps)
-        runs.append((every, s.state(), s.assign()))
-    for every, st, asg in runs[1:]:
+        runs.append(((every, lanes), s.state(), s.assign()))
+    for cfg, st, asg in runs[1:]:
         for f in FIELDS:
-            assert_bits(st[f], runs[0][1][f], f"prune every {every}: {f}")
+            assert_bits(st[f], runs[0][1][f], f"prune every {cfg[0]}, {cfg[1]} lanes: {f}")
         assert np.array_equal(asg, runs[0][2])
 
 
