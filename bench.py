@@ -71,13 +71,13 @@ def cpu_baseline(scene, a, steps):
                 ms_per_step=1000.0 * dt / steps)
 
 
-def batch_bench(steps, warmup, loop_steps):
+def batch_bench(steps, warmup, loop_steps, use_graph=True):
     """The ensemble (256 scenarios of the room, corridor, bowl-eighth and pillar floors, about 1.4 M agents) as one
     batched step, against the same scenarios stepped by a loop over plain contexts (each scenario's steps, then a
     synchronisation, one scenario after another)."""
     scen_layout, totals, actives, agents = sh.batch_workload()
     scenes = [sh.Scene.layout(l) for l in sh.BATCH_LAYOUTS]
-    b = sh.BatchSim(scenes, scen_layout, totals)
+    b = sh.BatchSim(scenes, scen_layout, totals, use_graph=use_graph)
     b.upload(agents)
     tune_ms = b.autotune(5)
     shape = b.physics_shape()
@@ -93,7 +93,7 @@ def batch_bench(steps, warmup, loop_steps):
     plains = []
     for s in range(len(totals)):
         lo = int(np.sum(totals[:s]))
-        p = sh.Sim(scenes[scen_layout[s]], int(totals[s]))
+        p = sh.Sim(scenes[scen_layout[s]], int(totals[s]), use_graph=use_graph)
         p.upload({k: v[lo:lo + int(totals[s])] for k, v in agents.items()})
         p.step(2)
         plains.append(p)
@@ -118,13 +118,13 @@ def batch_bench(steps, warmup, loop_steps):
             "speedup_vs_looped_plain": batched / looped}
 
 
-def adjoint_bench(K=200, every=20, n=4096):
+def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     """The differentiable configuration: 4,096 agents in the 40 x 30 m room, K steps. The forward is K plain steps
     (graph replays); the adjoint is the adjoint's forward (the same steps with the loss terms and checkpoints) plus
     its backward (each segment replayed with the tape, then the adjoint kernels)."""
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
     a = sh.place_agents(sh.LAYOUT_ROOM, n, n, SEED)
-    sim = sh.Sim(scene, n)
+    sim = sh.Sim(scene, n, use_graph=use_graph)
     sim.upload(a)
     tune_ms = sim.autotune(5)   # launch shape and projection lanes for this crowd (the state is restored)
     sim.step(2)
@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches in every leg (for profilers that do not follow graph launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
@@ -176,7 +178,7 @@ def main():
     scene = sh.Scene.layout(sh.LAYOUT_BOWL)
     a = sh.place_agents(sh.LAYOUT_BOWL, ACTIVE, TOTAL, SEED)
     build_s = time.perf_counter() - t0
-    sim = sh.Sim(scene, TOTAL)
+    sim = sh.Sim(scene, TOTAL, use_graph=not args.no_graph)
     sim.upload(a)
     print(f"# scene built in {build_s:.1f} s; hash grid {scene.info.hash_nx}x{scene.info.hash_ny}, "
           f"SDF {scene.info.sdf_nx}x{scene.info.sdf_ny}, nav {scene.info.nav_res_x}x{scene.info.nav_res_y} x {scene.info.E} exits",
@@ -220,14 +222,14 @@ def main():
     batch = None
     if not args.no_batch:
         sim.close()
-        batch = batch_bench(args.batch_steps, 5, args.loop_steps)
+        batch = batch_bench(args.batch_steps, 5, args.loop_steps, use_graph=not args.no_graph)
         print(f"# batched ensemble: {batch['ms_per_step']:.3f} ms/step, {batch['value'] / 1e6:.1f} M agent-steps/s; "
               f"looped plain {batch['looped_plain']['value'] / 1e6:.1f} M; x{batch['speedup_vs_looped_plain']:.2f}",
               file=sys.stderr, flush=True)
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_bench()
+        adjoint = adjoint_bench(use_graph=not args.no_graph)
         print(f"# adjoint K={adjoint['K']}: forward {adjoint['forward_ms']:.1f} ms, adjoint {adjoint['adjoint_forward_ms']:.1f} + "
               f"{adjoint['adjoint_backward_ms']:.1f} ms ({adjoint['cost_vs_forward']:.2f}x)", file=sys.stderr, flush=True)
 
