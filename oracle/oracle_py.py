@@ -236,3 +236,67 @@ class AdjRef:
             self.close()
         except Exception:
             pass
+
+
+# ---------------- safety metrics (oracle/metrics_oracle.cpp) ----------------
+COUNTER_WORDS = 68   # step, active, evacuated, activeAccum, exits[64]
+CELL_WORDS = 16      # lastMass, lastSpeedMass, peakMass, occupiedSteps, stepsAbove[4], massSteps, speedMassSteps, exposure (u64 each), pad
+
+
+def _metrics_sigs(L):
+    if getattr(L, "_metrics_sigs", False):
+        return L
+    u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    L.oracle_metrics_cic.argtypes = [C.c_int, f32, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, u32, u32]
+    L.oracle_metrics_speed_mass.argtypes = [C.c_int, u32, f32, u32]
+    L.oracle_metrics_speed_per_unit.restype = C.c_float
+    L.oracle_metrics_thresholds.argtypes = [C.c_int, f32, u8, u32]
+    L.oracle_metrics_record.argtypes = ([C.c_int] + [f32] * 8 + [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, u32, C.c_int, f32, u32,
+                                                                 u32, u32])
+    L._metrics_sigs = True
+    return L
+
+
+def metrics_cic(xy, res_x, res_y, cells_per_unit, off_x, off_y):
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    n = xy.shape[0]
+    cells, w = np.zeros((n, 4), np.uint32), np.zeros((n, 4), np.uint32)
+    _metrics_sigs(lib()).oracle_metrics_cic(n, xy.reshape(-1), res_x, res_y, cells_per_unit, off_x, off_y, cells.reshape(-1), w.reshape(-1))
+    return cells, w
+
+
+def metrics_speed_mass(weight, mps):
+    weight = np.ascontiguousarray(weight, np.uint32)
+    mps = np.ascontiguousarray(mps, np.float32)
+    out = np.zeros(weight.size, np.uint32)
+    _metrics_sigs(lib()).oracle_metrics_speed_mass(weight.size, weight, mps, out)
+    return out
+
+
+def metrics_thresholds(walkable, included):
+    walkable = np.ascontiguousarray(walkable, np.float32)
+    inc = np.ascontiguousarray(included, np.uint8)
+    out = np.zeros((walkable.size, 4), np.uint32)
+    _metrics_sigs(lib()).oracle_metrics_thresholds(walkable.size, walkable, inc, out.reshape(-1))
+    return out
+
+
+class MetricsOracle:
+    """The metric passes' state on the CPU: counters, per-agent evacuation steps, per-cell accumulators."""
+
+    def __init__(self, n, res_x, res_y, cells_per_unit, off_x, off_y, thresholds, rects=None):
+        self.n, self.grid = n, (res_x, res_y, float(cells_per_unit), float(off_x), float(off_y))
+        self.thr = np.ascontiguousarray(thresholds, np.uint32).reshape(-1)
+        self.rects = np.ascontiguousarray(np.zeros(4, np.float32) if rects is None else rects, np.float32).reshape(-1)
+        self.E = 0 if rects is None else self.rects.size // 4
+        self.counters = np.zeros(COUNTER_WORDS, np.uint32)
+        self.evac_step = np.zeros(n, np.uint32)
+        self.cells = np.zeros((res_x * res_y, CELL_WORDS), np.uint32)
+
+    def record(self, prev, cur):
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        _metrics_sigs(lib()).oracle_metrics_record(self.n, f(prev["x"]), f(prev["y"]), f(prev["w"]), f(cur["x"]), f(cur["y"]), f(cur["w"]),
+                                                    f(cur["vx"]), f(cur["vy"]), *self.grid, self.thr, self.E, self.rects, self.counters,
+                                                    self.evac_step, self.cells.reshape(-1))

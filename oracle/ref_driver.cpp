@@ -8,6 +8,7 @@
 // u32 ndim, u64 dims[ndim], raw little-endian data.
 #include <swarm/cpu_crowd_sim.hpp>
 #include <swarm/exit_choice.hpp>
+#include <swarm/metrics.hpp>
 #include <swarm/nav_field.hpp>
 #include <swarm/sdf.hpp>
 #include <swarm/venue.hpp>
@@ -292,6 +293,106 @@ void choiceCases() {
         putChoice("choice_c/final/", c);
     }
 }
+// Safety metrics (src/metrics.cpp: cicDeposit, speedMass, MetricVenue::cellInfo, MetricsReference::record) on
+// synthetic states over the reference's default 64 x 64 grid of 1 m cells: a dense cluster (past the crush
+// threshold), agents spread over and beyond the grid (clamped to its edge), agents on exact cell and quantum
+// boundaries, inactive agents and evacuations in every step
+void metricsCases() {
+    using namespace swarm::metrics;
+    Rng rng{ 777 };
+    {   // cicDeposit / speedMass known answers
+        std::vector<float> xy, sp;
+        std::vector<uint32_t> cells, weights, wts, sms;
+        for (int k = 0; k < 2000; ++k) {
+            float x = (rng.uniform() * 2.4f - 1.2f), y = (rng.uniform() * 2.4f - 1.2f);
+            if (k % 10 == 0) { x = float(int(x * 31.25f)) / 31.25f; }             // integer metres (cell corners)
+            if (k % 10 == 1) { y = (float(int(y * 31.25f)) + 0.5f) / 31.25f; }    // cell centres
+            if (k % 10 == 2) { x = (float(int(x * 8000.0f)) + 0.5f) / 8000.0f; }  // near the 1/256 quantum's half
+            xy.push_back(x); xy.push_back(y);
+            const CicDeposit d = cicDeposit(glm::vec2(x, y), metrics::GRID_RES);
+            for (int j = 0; j < 4; ++j) { cells.push_back(uint32_t(d.cell[j])); weights.push_back(d.weight[j]); }
+        }
+        for (int k = 0; k < 2000; ++k) {
+            const uint32_t w = uint32_t(rng.next() % (MASS_ONE + 1));
+            const float s = rng.uniform() * 3.0f;
+            wts.push_back(w); sp.push_back(s); sms.push_back(speedMass(w, s));
+        }
+        put("metrics/cic/xy", 'f', xy, { 2000, 2 });
+        put("metrics/cic/cell", 'u', cells, { 2000, 4 });
+        put("metrics/cic/weight", 'u', weights, { 2000, 4 });
+        put("metrics/speed/weight", 'u', wts);
+        put("metrics/speed/mps", 'f', sp);
+        put("metrics/speed/mass", 'u', sms);
+    }
+    // the cell table: walkable areas with partial and excluded cells
+    MetricVenue mv;
+    mv.res = metrics::GRID_RES;
+    mv.walkableM2.assign(CELLS, 1.0f);
+    mv.zone.assign(CELLS, 0u);
+    mv.included.assign(CELLS, 1u);
+    for (int c = 0; c < CELLS; ++c) {
+        const uint64_t r = rng.next() % 10;
+        if (r == 0) mv.walkableM2[size_t(c)] = rng.uniform();          // partial
+        if (r == 1) mv.walkableM2[size_t(c)] = 0.05f * rng.uniform();  // below MIN_WALKABLE_M2
+        mv.included[size_t(c)] = mv.walkableM2[size_t(c)] >= float(MIN_WALKABLE_M2) ? 1u : 0u;
+    }
+    const std::vector<GpuMetricCellInfo> info = mv.cellInfo();
+    std::vector<float> walk(mv.walkableM2);
+    std::vector<uint32_t> thr;
+    for (const GpuMetricCellInfo& i : info) for (int k = 0; k < 4; ++k) thr.push_back(i.thresholdMass[k]);
+    put("metrics/info/walkable_m2", 'f', walk);
+    put("metrics/info/threshold_mass", 'u', thr, { uint64_t(CELLS), 4 });
+    // the states
+    const int n = 3000, steps = 6;
+    std::vector<GpuBoid> st(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        float x, y;
+        if (i < 1200) { x = 0.10f + 0.12f * rng.uniform(); y = -0.20f + 0.12f * rng.uniform(); }   // 3.75 m square: ~85 per m^2
+        else if (i < 1300) { x = float(int(rng.uniform() * 60.0f) - 30) / 31.25f; y = float(int(rng.uniform() * 60.0f) - 30) / 31.25f; }
+        else { x = rng.uniform() * 2.3f - 1.15f; y = rng.uniform() * 2.3f - 1.15f; }                // beyond +-1.024: clamped
+        const float vx = rng.uniform() * 0.2f - 0.1f, vy = rng.uniform() * 0.2f - 0.1f;
+        st[size_t(i)].position = glm::vec4(x, y, 0.0f, (i % 11 == 0) ? 0.0f : 1.0f);
+        st[size_t(i)].velocity = glm::vec4(vx, vy, 0.0f, std::sqrt(vx * vx + vy * vy));
+    }
+    MetricsReference ref(n, info, nullptr);
+    std::vector<float> states;
+    auto putState = [&](const std::vector<GpuBoid>& v) {
+        for (const GpuBoid& b : v) { states.push_back(b.position.x); states.push_back(b.position.y); states.push_back(b.position.w);
+                                     states.push_back(b.velocity.x); states.push_back(b.velocity.y); }
+    };
+    putState(st);
+    std::vector<uint32_t> counters, lastMass, lastSpeed;
+    for (int k = 0; k < steps; ++k) {
+        std::vector<GpuBoid> nx = st;
+        for (int i = 0; i < n; ++i) {
+            GpuBoid& b = nx[size_t(i)];
+            if (b.position.w == 0.0f) continue;
+            b.position.x += b.velocity.x * 0.01f;
+            b.position.y += b.velocity.y * 0.01f;
+            if (rng.next() % 23 == 0) b.position.w = 0.0f;   // leaves in this step
+        }
+        ref.record(st, nx);
+        st = nx;
+        putState(st);
+        const GpuMetricCounters& c = ref.counters;
+        counters.push_back(c.step); counters.push_back(c.active); counters.push_back(c.evacuated); counters.push_back(c.activeAccum);
+        uint32_t exitSum = 0;
+        for (uint32_t e : c.exits) exitSum += e;
+        counters.push_back(exitSum);
+        for (const GpuMetricCell& cell : ref.cells) { lastMass.push_back(cell.lastMass); lastSpeed.push_back(cell.lastSpeedMass); }
+    }
+    put("metrics/run/states", 'f', states, { uint64_t(steps + 1), uint64_t(n), 5 });
+    put("metrics/run/counters", 'u', counters, { uint64_t(steps), 5 });
+    put("metrics/run/evac_step", 'u', ref.evacStep);
+    put("metrics/run/last_mass", 'u', lastMass, { uint64_t(steps), uint64_t(CELLS) });
+    put("metrics/run/last_speed_mass", 'u', lastSpeed, { uint64_t(steps), uint64_t(CELLS) });
+    std::vector<uint32_t> cells;
+    for (const GpuMetricCell& cell : ref.cells) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(&cell);
+        for (size_t j = 0; j < sizeof(GpuMetricCell) / 4; ++j) cells.push_back(w[j]);
+    }
+    put("metrics/run/cells", 'u', cells, { uint64_t(CELLS), uint64_t(sizeof(GpuMetricCell) / 4) });
+}
 } // namespace
 
 int main(int argc, char** argv) {
@@ -304,6 +405,7 @@ int main(int argc, char** argv) {
     terraceCase("t_sparse", vol, 512, 0.10f, 0.011f, 6, 1337, 0, false, 0);
     terraceCase("t_dense", vol, 1024, 0.085f, 0.0085f, 8, 4242, 37, true, 5);
     choiceCases();
+    metricsCases();
     {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents
         const std::vector<Rect> walls = { { { -0.3f, -0.3f }, { 20.3f, 0.0f } }, { { -0.3f, 0.0f }, { 0.0f, 2.3f } }, { { 0.0f, 2.0f }, { 18.0f, 2.3f } },
                                           { { 17.7f, 2.3f }, { 18.0f, 15.0f } }, { { 20.0f, 0.0f }, { 20.3f, 15.0f } } };
