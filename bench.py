@@ -196,6 +196,19 @@ def main():
     ms_per_step = ms / args.steps
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e6:.1f} M agent-steps/s", file=sys.stderr, flush=True)
 
+    # the safety-metric passes (density deposit + per-cell accumulators) inside the same step, on the same crowd
+    sim.metrics_enable(True)
+    sim.step(2)
+    sim.sync()
+    ms_m = sim.time_steps(args.steps)
+    mc = sim.metric_counters()
+    rx, ry = sim.metrics_grid()[:2]
+    sim.metrics_enable(False)
+    metrics = {"ms_per_step": ms_m / args.steps, "overhead_vs_plain": ms_m / ms - 1.0, "grid_cells": [rx, ry],
+               "recorded_steps": int(mc[0]), "active_last": int(mc[1])}
+    print(f"# with the safety metrics: {metrics['ms_per_step']:.4f} ms/step ({100 * metrics['overhead_vs_plain']:+.1f}%)",
+          file=sys.stderr, flush=True)
+
     # per-pass timing (eager launches with events between passes) for the dominant kernel's launch duration
     sim.set_timing(True)
     sim.step(max(1, args.profile_steps))
@@ -261,6 +274,7 @@ def main():
                      "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
+        "metrics": metrics,
         "batched": batch,
         "adjoint": adjoint,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

@@ -82,6 +82,13 @@ int swarm_scene_info_get(const swarm_scene* s, swarm_scene_info* info);
  * nav_res_x*nav_res_y; nav_slot, nav_dist cells*K; inv_cap_s E; exit_rects 4E. */
 int swarm_scene_read(const swarm_scene* s, float* sdf, float* sdf_min, uint32_t* nav_info, uint32_t* nav_slot,
                      uint16_t* nav_dist, float* inv_cap_s, float* stair, float* exit_rects);
+/* The floor's safety-metric grid (MetricVenue, metrics.hpp:145-162; its walkable area per cell as
+ * src/metrics.cpp:143-164 samples it, there over the terrace's centred 64 x 64 m square): 1 m cells covering the
+ * floor from whole metres, g = xy * cells_per_unit + off per axis, and each cell's walkable area in m^2 from
+ * 40 x 40 samples against the floor's walls. Host only; any
+ * pointer may be NULL; walkable_m2 holds res_x * res_y values, row-major in y. */
+int swarm_scene_metric_grid(const swarm_scene* s, int* res_x, int* res_y, float* cells_per_unit, float* off_x, float* off_y,
+                            float* walkable_m2);
 void swarm_scene_destroy(swarm_scene* s);
 
 /* Seeded agents on a benchmark floor (replaces CrowdSim's spawners, src/crowd_sim.cpp:485-808): 'active' of its
@@ -188,6 +195,28 @@ int swarm_get_projection_lanes(swarm_ctx* c, int* lanes);
  * per slot at the fastest shape; keeps the fastest and restores the state; *best_ms: its mean ms per step. No
  * counterpart in the reference. */
 int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);
+
+/* ---- the safety metrics (class CrowdMetrics, include/swarm/metrics.hpp:181-227; plain contexts) ----
+ * Write-only side outputs of each step: evacuations per exit (the floor's exit rectangles, then 'none'), each
+ * agent's evacuation step, cloud-in-cell density and speed on a floor grid of 1 m cells in fixed point (one person =
+ * 65536), and per cell the last, peak, occupied steps, steps at or above the LoS E / LoS F / crush threshold masses,
+ * the 64-bit sums of mass and speed mass and the crush exposure. Counters: METRIC_COUNTER_WORDS = 68 words (step,
+ * active, evacuated, activeAccum, exits[64]); cells: 16 words each (GpuMetricCell, gpu_types.hpp:127-137). */
+/* On: every swarm_step is followed by the deposit and finalize passes (metrics_deposit.comp, metrics_finalize.comp;
+ * CrowdMetrics::afterStep, metrics.hpp:197-199). The first enable makes the scene's grid (walkable areas sampled as
+ * computeMetricVenue does) and clears it. */
+int swarm_metrics_enable(swarm_ctx* c, int on);
+int swarm_metrics_reset(swarm_ctx* c);                                       /* CrowdMetrics::reset (metrics.hpp:196) */
+/* another grid: res_x x res_y cells, g = xy * cells_per_unit + off, 4 threshold masses per cell; then reset */
+int swarm_metrics_configure(swarm_ctx* c, int res_x, int res_y, float cells_per_unit, float off_x, float off_y, const uint32_t* thresholds4);
+int swarm_metrics_grid(swarm_ctx* c, int* res_x, int* res_y, float* cells_per_unit, float* off_x, float* off_y);
+/* the two passes on the caller's states, n each, agent order (CrowdMetrics::record, metrics.hpp:201) */
+int swarm_metrics_record(swarm_ctx* c, const float* px, const float* py, const float* pw, const float* cx, const float* cy, const float* cw,
+                         const float* cvx, const float* cvy);
+int swarm_read_metric_counters(swarm_ctx* c, uint32_t* out68);      /* readCounters (metrics.hpp:212) */
+int swarm_read_metric_cells(swarm_ctx* c, uint32_t* out16);         /* readCells (metrics.hpp:214) */
+int swarm_read_metric_evac_steps(swarm_ctx* c, uint32_t* out_n);    /* readEvacSteps (metrics.hpp:213) */
+int swarm_read_metric_cell_info(swarm_ctx* c, float* walkable_m2, uint32_t* thresholds4);   /* MetricVenue (metrics.hpp:145-153) */
 
 /* the context's HIP stream (hipStream_t), for events around swarm_step */
 int swarm_get_stream(swarm_ctx* c, void** stream);

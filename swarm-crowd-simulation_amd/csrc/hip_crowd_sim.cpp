@@ -117,6 +117,21 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     SWARM_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     scenes_.resize(layouts.size());
     for (size_t l = 0; l < layouts.size(); ++l) scenes_[l].upload(*layouts[l]);
+    if (!batched) {   // the safety metrics' floor grid and its threshold masses (MetricVenue::cellInfo's rule)
+        const Scene& sc = *layouts[0];
+        mgrid_ = MetricGrid{ sc.metricResX, sc.metricResY, sc.metricCellsPerUnit, sc.metricOffX, sc.metricOffY,
+                             float(1.0 * double(WORLD_SCALE_M_PER_UNIT) / double(TIME_SCALE_S_PER_UNIT)) };
+        mWalkable_ = sc.metricWalkableM2;
+        static constexpr double THRESHOLD_P_M2[3] = { 1.08, 2.17, 5.0 };   // LoS E, LoS F, crush (metrics.hpp:82-94)
+        mThr_.assign(mWalkable_.size() * 4, METRIC_NEVER);
+        for (size_t c = 0; c < mWalkable_.size(); ++c) {
+            if (!(double(mWalkable_[c]) >= 0.10)) continue;   // MIN_WALKABLE_M2 (metrics.hpp:138)
+            for (int k = 0; k < 3; ++k) {
+                const double m = std::ceil(THRESHOLD_P_M2[k] * double(mWalkable_[c]) * double(METRIC_MASS_ONE));
+                mThr_[4 * c + size_t(k)] = uint32_t(std::min(std::max(m, 1.0), double(METRIC_NEVER)));
+            }
+        }
+    }
     const size_t n = size_t(n_);
     const size_t S = size_t(S_);
     for (int h = 0; h < 2; ++h) {
@@ -196,6 +211,7 @@ HipCrowdSim::~HipCrowdSim() {
     dropGraph();
     for (void* p : bufs_) (void)hipFree(p);
     for (void* p : adjBufs_) (void)hipFree(p);
+    for (void* p : mBufs_) (void)hipFree(p);
     for (char* p : ckpt_) (void)hipFree(p);
     for (DeviceScene& d : scenes_) d.release();
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -399,6 +415,11 @@ void HipCrowdSim::enqueueStep(int parity) {
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
     launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
     if (lossOn_) launchOverlap(stream_, L, b, wr, overlapWeight_, lossAcc_, nullptr);
+    if (metricsOn_) {   // the step's input half rd, its output wr; in this step's slot order
+        launchMetricsDeposit(stream_, n_, mgrid_, b.sid, rd.x, rd.y, rd.w, wr.x, wr.y, wr.w, wr.vx, wr.vy, scenes_[0].view.E,
+                             scenes_[0].view.exitRects, mbuf_);
+        launchMetricsFinalize(stream_, mgrid_, mbuf_);
+    }
     if (timing_) {
         SWARM_HIP_CHECK(hipEventRecord(ev[5], stream_));
         pendingStamps_.push_back(ev);
@@ -967,6 +988,110 @@ void HipCrowdSim::readEvacPoints(int32_t* step, int32_t* exitId, float* inside, 
     if (inside) SWARM_HIP_CHECK(hipMemcpyAsync(inside, ev_.inside, n * 8, hipMemcpyDeviceToHost, stream_));
     if (before) SWARM_HIP_CHECK(hipMemcpyAsync(before, ev_.before, n * 8, hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+// ---- the safety metrics ----
+void HipCrowdSim::allocMetrics() {
+    for (void* p : mBufs_) (void)hipFree(p);
+    mBufs_.clear();
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
+        mBufs_.push_back(p);
+        return p;
+    };
+    const size_t cells = size_t(mgrid_.resX) * size_t(mgrid_.resY);
+    mbuf_.counters = static_cast<uint32_t*>(dalloc(METRIC_COUNTER_WORDS * 4));
+    mbuf_.evacStep = static_cast<uint32_t*>(dalloc(size_t(n_) * 4));
+    mbuf_.mass = static_cast<uint32_t*>(dalloc(cells * 8));
+    mbuf_.cells = static_cast<uint32_t*>(dalloc(cells * METRIC_CELL_WORDS * 4));
+    uint32_t* thr = static_cast<uint32_t*>(dalloc(cells * 16));
+    SWARM_HIP_CHECK(hipMemcpyAsync(thr, mThr_.data(), cells * 16, hipMemcpyHostToDevice, stream_));
+    mbuf_.thr = thr;
+    mScratch_ = static_cast<float*>(dalloc(size_t(n_) * 8 * 4));
+    resetMetrics();
+}
+
+void HipCrowdSim::setMetricsEnabled(bool on) {
+    if (on && batched_) throw std::invalid_argument("setMetricsEnabled: the safety metrics run on plain contexts");
+    if (on && !mbuf_.counters) {
+        if (mgrid_.resX <= 0 || mgrid_.resY <= 0) throw std::invalid_argument("setMetricsEnabled: the scene has no metric grid");
+        allocMetrics();
+    }
+    if (on != metricsOn_) dropGraph();
+    metricsOn_ = on;
+}
+
+void HipCrowdSim::resetMetrics() {
+    if (!mbuf_.counters) return;
+    const size_t cells = size_t(mgrid_.resX) * size_t(mgrid_.resY);
+    SWARM_HIP_CHECK(hipMemsetAsync(mbuf_.counters, 0, METRIC_COUNTER_WORDS * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(mbuf_.evacStep, 0, size_t(n_) * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(mbuf_.mass, 0, cells * 8, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(mbuf_.cells, 0, cells * METRIC_CELL_WORDS * 4, stream_));
+}
+
+void HipCrowdSim::configureMetrics(int resX, int resY, float cellsPerUnit, float offX, float offY, const uint32_t* thresholds4) {
+    if (batched_) throw std::invalid_argument("configureMetrics: the safety metrics run on plain contexts");
+    if (resX <= 0 || resY <= 0 || size_t(resX) * size_t(resY) > (size_t(1) << 26) || !(cellsPerUnit > 0.0f) || !thresholds4)
+        throw std::invalid_argument("configureMetrics: a grid of 1 .. 2^26 cells, positive cells per unit and thresholds");
+    mgrid_.resX = resX; mgrid_.resY = resY;
+    mgrid_.cellsPerUnit = cellsPerUnit; mgrid_.offX = offX; mgrid_.offY = offY;
+    const size_t cells = size_t(resX) * size_t(resY);
+    mThr_.assign(thresholds4, thresholds4 + cells * 4);
+    mWalkable_.assign(cells, 0.0f);
+    allocMetrics();
+    dropGraph();
+}
+
+void HipCrowdSim::metricGrid(int* resX, int* resY, float* cellsPerUnit, float* offX, float* offY) const {
+    if (resX) *resX = mgrid_.resX;
+    if (resY) *resY = mgrid_.resY;
+    if (cellsPerUnit) *cellsPerUnit = mgrid_.cellsPerUnit;
+    if (offX) *offX = mgrid_.offX;
+    if (offY) *offY = mgrid_.offY;
+}
+
+void HipCrowdSim::recordMetrics(const float* px, const float* py, const float* pw, const float* cx, const float* cy, const float* cw,
+                                const float* cvx, const float* cvy) {
+    if (batched_) throw std::invalid_argument("recordMetrics: the safety metrics run on plain contexts");
+    if (!mbuf_.counters) allocMetrics();
+    const size_t n = size_t(n_);
+    const float* src[8] = { px, py, pw, cx, cy, cw, cvx, cvy };
+    for (int k = 0; k < 8; ++k) {
+        if (!src[k]) throw std::invalid_argument("recordMetrics: null state array");
+        SWARM_HIP_CHECK(hipMemcpyAsync(mScratch_ + size_t(k) * n, src[k], n * 4, hipMemcpyHostToDevice, stream_));
+    }
+    const float* d = mScratch_;
+    launchMetricsDeposit(stream_, n_, mgrid_, nullptr, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, d + 6 * n, d + 7 * n,
+                         scenes_[0].view.E, scenes_[0].view.exitRects, mbuf_);
+    launchMetricsFinalize(stream_, mgrid_, mbuf_);
+    SWARM_HIP_CHECK(hipGetLastError());
+    sync();
+}
+
+void HipCrowdSim::readMetricCounters(uint32_t* out) {
+    if (!mbuf_.counters) throw std::invalid_argument("readMetricCounters: the metrics are not enabled");
+    SWARM_HIP_CHECK(hipMemcpyAsync(out, mbuf_.counters, METRIC_COUNTER_WORDS * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readMetricCells(uint32_t* out) {
+    if (!mbuf_.counters) throw std::invalid_argument("readMetricCells: the metrics are not enabled");
+    const size_t cells = size_t(mgrid_.resX) * size_t(mgrid_.resY);
+    SWARM_HIP_CHECK(hipMemcpyAsync(out, mbuf_.cells, cells * METRIC_CELL_WORDS * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readMetricEvacSteps(uint32_t* out) {
+    if (!mbuf_.counters) throw std::invalid_argument("readMetricEvacSteps: the metrics are not enabled");
+    SWARM_HIP_CHECK(hipMemcpyAsync(out, mbuf_.evacStep, size_t(n_) * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::readMetricCellInfo(float* walkableM2, uint32_t* thresholds4) {
+    if (walkableM2) std::copy(mWalkable_.begin(), mWalkable_.end(), walkableM2);
+    if (thresholds4) std::copy(mThr_.begin(), mThr_.end(), thresholds4);
 }
 
 } // namespace swarm

@@ -18,6 +18,7 @@
 
 #include "adjoint.h"
 #include "crowd_kernels.h"
+#include "metrics.h"
 #include "scene.hpp"
 
 namespace swarm {
@@ -123,6 +124,25 @@ public:
     // per agent: the step it left at, its exit, its position at that step's input and one step earlier (2 floats each)
     void readEvacPoints(int32_t* step, int32_t* exitId, float* inside, float* before);
 
+    // ---- the safety metrics (CrowdMetrics, swarm/metrics.hpp:168-214; plain contexts) ----
+    // On: every step is followed by the deposit and finalize passes (in the captured graph). The first enable
+    // allocates the floor grid of the scene (1 m cells over its domain, thresholds from its walkable areas) and
+    // clears it.
+    void setMetricsEnabled(bool on);
+    bool metricsEnabled() const { return metricsOn_; }
+    // Zeroes every counter and accumulator and the metric step (CrowdMetrics::reset)
+    void resetMetrics();
+    // Another grid: resX x resY cells, g = xy * cellsPerUnit + off, the caller's 4 threshold masses per cell (then reset)
+    void configureMetrics(int resX, int resY, float cellsPerUnit, float offX, float offY, const uint32_t* thresholds4);
+    void metricGrid(int* resX, int* resY, float* cellsPerUnit, float* offX, float* offY) const;
+    // The two passes on the caller's states (CrowdMetrics::record; agent order, n values each)
+    void recordMetrics(const float* px, const float* py, const float* pw, const float* cx, const float* cy, const float* cw,
+                       const float* cvx, const float* cvy);
+    void readMetricCounters(uint32_t* out);    // METRIC_COUNTER_WORDS
+    void readMetricCells(uint32_t* out);       // METRIC_CELL_WORDS per cell
+    void readMetricEvacSteps(uint32_t* out);   // n
+    void readMetricCellInfo(float* walkableM2, uint32_t* thresholds4);
+
     int numAgents() const { return n_; }
     int numCells() const { return C_; }
     int numScenarios() const { return S_; }
@@ -224,6 +244,16 @@ private:
     } tape_{};
     AdjBufs adj_{};
     std::vector<void*> adjBufs_;         // the adjoint's own allocations
+
+    // the safety metrics
+    void allocMetrics();
+    bool metricsOn_ = false;
+    MetricGrid mgrid_{};
+    MetricBuffers mbuf_{};
+    std::vector<float> mWalkable_;       // the scene's walkable area per cell (host)
+    std::vector<uint32_t> mThr_;         // 4 threshold masses per cell (host)
+    float* mScratch_ = nullptr;          // recordMetrics: 8 arrays of n
+    std::vector<void*> mBufs_;           // the metrics' own allocations
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

@@ -275,6 +275,55 @@ void lazyThetaStar(const NavField& f, int exitFilter, std::vector<double>& g, st
     }
 }
 
+namespace {
+// The metric grid and each cell's walkable area: 1 m cells from the whole metre at or below the domain's lower
+// corner; samples at the centres of a 40 x 40 sub-grid per cell, walkable when inside the domain and outside every
+// wall rectangle. One strip of cells (40 sample rows) is rasterised at a time.
+void buildMetricGrid(const SceneDesc& d, Scene& s) {
+    constexpr int SUB = 40;
+    const double mpu = d.metresPerUnit;
+    const double x0m = std::floor(double(d.xmin) * mpu), y0m = std::floor(double(d.ymin) * mpu);
+    const int rx = int(std::ceil(double(d.xmax) * mpu - x0m)), ry = int(std::ceil(double(d.ymax) * mpu - y0m));
+    s.metricResX = std::max(rx, 1);
+    s.metricResY = std::max(ry, 1);
+    s.metricCellsPerUnit = float(mpu);                 // 1 m cells
+    s.metricOffX = float(-x0m - 0.5);                  // whole metres - 0.5: exact in float
+    s.metricOffY = float(-y0m - 0.5);
+    s.metricWalkableM2.assign(size_t(s.metricResX) * size_t(s.metricResY), 0.0f);
+    const int W = s.metricResX * SUB;
+    std::vector<uint8_t> free(size_t(W) * SUB);
+    const double step = 1.0 / SUB;                     // metres between samples
+    for (int cy = 0; cy < s.metricResY; ++cy) {
+        // samples of this strip: x = x0m + (i + 0.5) step, y = y0m + cy + (j + 0.5) step (metres)
+        for (int j = 0; j < SUB; ++j) {
+            const double ym = y0m + cy + (j + 0.5) * step;
+            const float yu = float(ym / mpu);
+            for (int i = 0; i < W; ++i) {
+                const float xu = float((x0m + (i + 0.5) * step) / mpu);
+                free[size_t(j) * W + i] = xu >= d.xmin && xu <= d.xmax && yu >= d.ymin && yu <= d.ymax;
+            }
+        }
+        const double stripLo = y0m + cy, stripHi = stripLo + 1.0;
+        for (const Rect& r : d.walls) {
+            const double lo = double(r.loy) * mpu, hi = double(r.hiy) * mpu;
+            if (hi < stripLo || lo > stripHi) continue;
+            const int j0 = std::max(0, int(std::ceil((lo - stripLo) / step - 0.5)));
+            const int j1 = std::min(SUB - 1, int(std::floor((hi - stripLo) / step - 0.5)));
+            const int i0 = std::max(0, int(std::ceil((double(r.lox) * mpu - x0m) / step - 0.5)));
+            const int i1 = std::min(W - 1, int(std::floor((double(r.hix) * mpu - x0m) / step - 0.5)));
+            for (int j = j0; j <= j1; ++j)
+                for (int i = i0; i <= i1; ++i) free[size_t(j) * W + i] = 0;
+        }
+        for (int cx = 0; cx < s.metricResX; ++cx) {
+            int count = 0;
+            for (int j = 0; j < SUB; ++j)
+                for (int i = cx * SUB; i < (cx + 1) * SUB; ++i) count += free[size_t(j) * W + i];
+            s.metricWalkableM2[size_t(cy) * s.metricResX + cx] = float(count * step * step);
+        }
+    }
+}
+} // namespace
+
 Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) {
     const auto t0 = std::chrono::steady_clock::now();
     threads = resolveThreads(threads);
@@ -415,6 +464,7 @@ Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) 
                 if (d.stairs[k].contains(cx, cy)) { s.stair[size_t(d.nav.index(ix, iy))] = d.stairFactor[k]; break; }
         }
     s.exits = d.exits;
+    buildMetricGrid(d, s);
     s.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return s;
 }

@@ -69,6 +69,13 @@ struct Scene {
     std::vector<float> stair;              // per nav cell: free-speed factor
     std::vector<Rect> exits;
     float maxRadius = 0.0f, maxV0 = 0.0f;  // used for the rings and sdfMin window
+    // the safety metrics' floor grid (metrics.hpp:49-67, generalised to a rectangle off the origin): 1 m cells,
+    // g = xy * metricCellsPerUnit + metricOff per axis is the position in cells from the centre of cell 0;
+    // per cell its walkable floor area (m^2), sampled on 40 x 40 points as computeMetricVenue samples
+    // (src/metrics.cpp:150-165): inside the domain and outside every wall
+    int metricResX = 0, metricResY = 0;
+    float metricCellsPerUnit = 0.0f, metricOffX = 0.0f, metricOffY = 0.0f;
+    std::vector<float> metricWalkableM2;
     double buildSeconds = 0.0;
 };
 

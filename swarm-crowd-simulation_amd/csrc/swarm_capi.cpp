@@ -103,6 +103,19 @@ int swarm_scene_read(const swarm_scene* sc, float* sdf, float* sdf_min, uint32_t
     return SWARM_OK;
 }
 
+int swarm_scene_metric_grid(const swarm_scene* sc, int* res_x, int* res_y, float* cells_per_unit, float* off_x, float* off_y,
+                            float* walkable_m2) {
+    if (!sc) { g_err = "null argument"; return SWARM_EINVAL; }
+    const swarm::Scene& s = sc->s;
+    if (res_x) *res_x = s.metricResX;
+    if (res_y) *res_y = s.metricResY;
+    if (cells_per_unit) *cells_per_unit = s.metricCellsPerUnit;
+    if (off_x) *off_x = s.metricOffX;
+    if (off_y) *off_y = s.metricOffY;
+    if (walkable_m2) std::memcpy(walkable_m2, s.metricWalkableM2.data(), s.metricWalkableM2.size() * 4);
+    return SWARM_OK;
+}
+
 void swarm_scene_destroy(swarm_scene* s) { delete s; }
 
 int swarm_place_agents(int layout, size_t active, size_t total, uint64_t seed, double mean, double sd, float* x, float* y,
@@ -279,6 +292,47 @@ int swarm_get_projection_lanes(swarm_ctx* c, int* lanes) {
     if (!lanes) { g_err = "null argument"; return SWARM_EINVAL; }
     *lanes = c->sim->projectionLanes();
     return SWARM_OK;
+}
+int swarm_metrics_enable(swarm_ctx* c, int on) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setMetricsEnabled(on != 0); });
+}
+int swarm_metrics_reset(swarm_ctx* c) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->resetMetrics(); });
+}
+int swarm_metrics_configure(swarm_ctx* c, int res_x, int res_y, float cells_per_unit, float off_x, float off_y, const uint32_t* thresholds4) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->configureMetrics(res_x, res_y, cells_per_unit, off_x, off_y, thresholds4); });
+}
+int swarm_metrics_grid(swarm_ctx* c, int* res_x, int* res_y, float* cells_per_unit, float* off_x, float* off_y) {
+    CTX_CHECK(c);
+    c->sim->metricGrid(res_x, res_y, cells_per_unit, off_x, off_y);
+    return SWARM_OK;
+}
+int swarm_metrics_record(swarm_ctx* c, const float* px, const float* py, const float* pw, const float* cx, const float* cy, const float* cw,
+                         const float* cvx, const float* cvy) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->recordMetrics(px, py, pw, cx, cy, cw, cvx, cvy); });
+}
+int swarm_read_metric_counters(swarm_ctx* c, uint32_t* out) {
+    CTX_CHECK(c);
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readMetricCounters(out); });
+}
+int swarm_read_metric_cells(swarm_ctx* c, uint32_t* out) {
+    CTX_CHECK(c);
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readMetricCells(out); });
+}
+int swarm_read_metric_evac_steps(swarm_ctx* c, uint32_t* out) {
+    CTX_CHECK(c);
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->readMetricEvacSteps(out); });
+}
+int swarm_read_metric_cell_info(swarm_ctx* c, float* walkable_m2, uint32_t* thresholds4) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readMetricCellInfo(walkable_m2, thresholds4); });
 }
 int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid) {
     CTX_CHECK(c);

@@ -182,6 +182,17 @@ class Scene:
                                      t["inv_cap_s"], t["stair"], t["exit_rects"]))
         return t
 
+    def metric_grid(self):
+        """The floor's safety-metric grid: (res_x, res_y, cells_per_unit, off_x, off_y, walkable m^2 per cell)."""
+        L = lib()
+        L.swarm_scene_metric_grid.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        rx, ry = C.c_int(), C.c_int()
+        cpu, ox, oy = C.c_float(), C.c_float(), C.c_float()
+        check(L.swarm_scene_metric_grid(self.h, C.byref(rx), C.byref(ry), C.byref(cpu), C.byref(ox), C.byref(oy), None))
+        walk = np.zeros(rx.value * ry.value, np.float32)
+        check(L.swarm_scene_metric_grid(self.h, None, None, None, None, None, walk.ctypes.data))
+        return rx.value, ry.value, cpu.value, ox.value, oy.value, walk.reshape(ry.value, rx.value)
+
     def close(self):
         if self.h:
             lib().swarm_scene_destroy(self.h)
@@ -524,3 +535,95 @@ def _sim_evac_points(self):
 
 Sim.set_adjoint_terms = _sim_set_adjoint_terms
 Sim.evac_points = _sim_evac_points
+
+
+# ---------------- the safety metrics (swarm_hip.h) ----------------
+METRIC_COUNTER_WORDS = 68   # step, active, evacuated, activeAccum, exits[64]
+METRIC_CELL_WORDS = 16      # lastMass, lastSpeedMass, peakMass, occupiedSteps, stepsAbove[4], 3 x 64-bit sums, pad[2]
+
+
+def _metrics_lib():
+    L = lib()
+    if not getattr(L, "_metrics_sigs", False):
+        vp = C.c_void_p
+        L.swarm_metrics_enable.argtypes = [vp, C.c_int]
+        L.swarm_metrics_reset.argtypes = [vp]
+        L.swarm_metrics_configure.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp]
+        L.swarm_metrics_grid.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.swarm_metrics_record.argtypes = [vp] + [vp] * 8
+        L.swarm_read_metric_counters.argtypes = [vp, vp]
+        L.swarm_read_metric_cells.argtypes = [vp, vp]
+        L.swarm_read_metric_evac_steps.argtypes = [vp, vp]
+        L.swarm_read_metric_cell_info.argtypes = [vp, vp, vp]
+        L._metrics_sigs = True
+    return L
+
+
+def _sim_metrics_enable(self, on=True):
+    """Every step followed by the metric passes (in the captured graph); the first enable clears the grid."""
+    check(_metrics_lib().swarm_metrics_enable(self.h, 1 if on else 0))
+
+
+def _sim_metrics_reset(self):
+    check(_metrics_lib().swarm_metrics_reset(self.h))
+
+
+def _sim_metrics_configure(self, res_x, res_y, cells_per_unit, off_x, off_y, thresholds):
+    thr = np.ascontiguousarray(thresholds, np.uint32).reshape(-1)
+    if thr.size != 4 * res_x * res_y:
+        raise ValueError("4 threshold masses per cell")
+    check(_metrics_lib().swarm_metrics_configure(self.h, res_x, res_y, cells_per_unit, off_x, off_y, thr.ctypes.data))
+
+
+def _sim_metrics_grid(self):
+    rx, ry = C.c_int(), C.c_int()
+    cpu, ox, oy = C.c_float(), C.c_float(), C.c_float()
+    check(_metrics_lib().swarm_metrics_grid(self.h, C.byref(rx), C.byref(ry), C.byref(cpu), C.byref(ox), C.byref(oy)))
+    return rx.value, ry.value, cpu.value, ox.value, oy.value
+
+
+def _sim_metrics_record(self, prev, cur):
+    """The metric passes on the caller's states (agent order): prev x, y, w; cur x, y, w, vx, vy."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (prev["x"], prev["y"], prev["w"], cur["x"], cur["y"], cur["w"], cur["vx"], cur["vy"])]
+    for a in arrs:
+        if a.size != self.n:
+            raise ValueError("metrics_record: n values per array")
+    check(_metrics_lib().swarm_metrics_record(self.h, *[a.ctypes.data for a in arrs]))
+
+
+def _sim_metric_counters(self):
+    out = np.zeros(METRIC_COUNTER_WORDS, np.uint32)
+    check(_metrics_lib().swarm_read_metric_counters(self.h, out.ctypes.data))
+    return out
+
+
+def _sim_metric_cells(self):
+    rx, ry = self.metrics_grid()[:2]
+    out = np.zeros((rx * ry, METRIC_CELL_WORDS), np.uint32)
+    check(_metrics_lib().swarm_read_metric_cells(self.h, out.ctypes.data))
+    return out
+
+
+def _sim_metric_evac_steps(self):
+    out = np.zeros(self.n, np.uint32)
+    check(_metrics_lib().swarm_read_metric_evac_steps(self.h, out.ctypes.data))
+    return out
+
+
+def _sim_metric_cell_info(self):
+    rx, ry = self.metrics_grid()[:2]
+    walk = np.zeros(rx * ry, np.float32)
+    thr = np.zeros((rx * ry, 4), np.uint32)
+    check(_metrics_lib().swarm_read_metric_cell_info(self.h, walk.ctypes.data, thr.ctypes.data))
+    return walk, thr
+
+
+Sim.metrics_enable = _sim_metrics_enable
+Sim.metrics_reset = _sim_metrics_reset
+Sim.metrics_configure = _sim_metrics_configure
+Sim.metrics_grid = _sim_metrics_grid
+Sim.metrics_record = _sim_metrics_record
+Sim.metric_counters = _sim_metric_counters
+Sim.metric_cells = _sim_metric_cells
+Sim.metric_evac_steps = _sim_metric_evac_steps
+Sim.metric_cell_info = _sim_metric_cell_info

@@ -63,3 +63,18 @@ def test_batch_create_rejects_bad_arguments():
     assert L.swarm_set_physics_shape(None, 128, 0) == -1
     assert L.swarm_set_projection_prune(None, 8) == -1
     assert L.swarm_set_projection_lanes(None, 4) == -1
+
+
+def test_metrics_entries_reject_null_contexts():
+    L = sh.lib()
+    out = np.zeros(68, np.uint32)
+    assert L.swarm_metrics_enable(None, 1) == -1
+    assert L.swarm_metrics_reset(None) == -1
+    assert L.swarm_metrics_configure(None, 4, 4, C.c_float(1.0), C.c_float(0.0), C.c_float(0.0), out.ctypes.data) == -1
+    assert L.swarm_metrics_grid(None, None, None, None, None, None) == -1
+    assert L.swarm_metrics_record(None, *([None] * 8)) == -1
+    assert L.swarm_read_metric_counters(None, out.ctypes.data) == -1
+    assert L.swarm_read_metric_cells(None, out.ctypes.data) == -1
+    assert L.swarm_read_metric_evac_steps(None, out.ctypes.data) == -1
+    assert L.swarm_read_metric_cell_info(None, None, None) == -1
+    assert L.swarm_scene_metric_grid(None, None, None, None, None, None, None) == -1
