@@ -196,9 +196,11 @@ def main():
     ms_per_step = ms / args.steps
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e6:.1f} M agent-steps/s", file=sys.stderr, flush=True)
 
-    # the safety-metric passes (density deposit + per-cell accumulators) inside the same step, on the same crowd
+    # the safety-metric passes (density deposit + per-cell accumulators) inside the same steps: the same upload and
+    # warm-up replay the same trajectory, so the difference is the passes' cost
+    sim.upload(a)
+    sim.step(args.warmup)
     sim.metrics_enable(True)
-    sim.step(2)
     sim.sync()
     ms_m = sim.time_steps(args.steps)
     mc = sim.metric_counters()

@@ -41,9 +41,27 @@ __device__ inline void waveCountAdd(uint32_t* base, uint32_t key, bool valid) {
     }
 }
 
+// As waveCountAdd for keys that are mostly distinct within a wave (the exit choice's distance bins): the first
+// 'rounds' keys go through the wave's leaders, every lane still pending adds its own one. Each valid lane is counted
+// exactly once either way.
+__device__ inline void waveCountAddFew(uint32_t* base, uint32_t key, bool valid, int rounds) {
+    uint64_t pending = __ballot(valid);
+    const int lane = __lane_id();
+    for (int r = 0; r < rounds && pending; ++r) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(valid && key == k) & pending;
+        if (lane == leader) atomicAdd(&base[k], uint32_t(__popcll(same)));
+        pending &= ~same;
+    }
+    if ((pending >> lane) & 1ull) atomicAdd(&base[key], 1u);
+}
+
 __device__ inline uint32_t bucketOf(uint32_t key, uint32_t agent, uint32_t C) {
     return key == INACTIVE_CELL ? C + agent / uint32_t(HASH_BLOCK) : key;
 }
+
+constexpr int CHOICE_AHEAD_ROUNDS = 2;   // wave leaders for the distance bins before lanes add their own
 
 // ---------------- grid passes ----------------
 // Agent i's offset in its bucket. The block's inactive agents are its own bucket (C + block): their offsets are
@@ -204,8 +222,9 @@ __device__ inline ChoiceRef choiceRef(uint32_t sc, int stride) {
                       sc * uint32_t(stride) * uint32_t(CHOICE_GAIN_BINS), sc * uint32_t(THRESHOLD_STRIDE) };
 }
 
-// prefix stage, one block per scenario: the read parity's queue-ahead histograms to exclusive prefix sums, the
-// write parity cleared, the write parity's queue counts cleared
+// prefix stage, one block per scenario: the read parity's queue-ahead histograms to exclusive prefix sums, each
+// exit's histogram total added to the read parity's queue count (the apply stage counts there only the agents it does
+// not bin), the write parity cleared, the write parity's queue counts cleared
 template <bool B>
 __global__ void __launch_bounds__(1024) k_choice_prefix(SceneView s0, BatchView bt, ChoiceBuffers c, const uint32_t* __restrict__ ctrl) {
     const uint32_t sc = blockIdx.x;
@@ -231,6 +250,8 @@ __global__ void __launch_bounds__(1024) k_choice_prefix(SceneView s0, BatchView 
         uint32_t run = incl - local;
 #pragma unroll
         for (int k = 0; k < PER; ++k) { a[k] = run; run += v[k]; }
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (lane == 0) c.queue[r.queueBase + par * uint32_t(CHOICE_QUEUE_STRIDE) + uint32_t(e)] += total;
         uint32_t* wrp = aWrite + size_t(e) * CHOICE_AHEAD_BINS + lane * PER;
 #pragma unroll
         for (int k = 0; k < PER; ++k) wrp[k] = 0u;
@@ -367,8 +388,9 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
         egx[slot] = e.x;
         egy[slot] = e.y;
     }
-    waveCountAdd(c.queue, qkey, counted);
-    waveCountAdd(c.ahead, akey, binned);
+    // an exit's queue count is its binned agents (the next prefix stage sums its histogram) plus these
+    waveCountAdd(c.queue, qkey, counted && !binned);
+    waveCountAddFew(c.ahead, akey, binned, CHOICE_AHEAD_ROUNDS);
     waveCountAdd(c.counters, ckey + uint32_t(CNT_FORCED), forced);
     waveCountAdd(c.counters, ckey + uint32_t(CNT_SWITCHES), switched);
 }

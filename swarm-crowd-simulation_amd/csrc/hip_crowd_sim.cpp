@@ -517,10 +517,41 @@ void HipCrowdSim::stepExitChoice() {
     SWARM_HIP_CHECK(hipGetLastError());
 }
 
+// The queue counts on the device: the half the next step reads (the step counter's parity) holds only the agents
+// whose exit is not among their cell's slots, and that step's prefix pass adds each exit's queue-ahead histogram
+// total. aheadTotals sums those histograms of parity p, per scenario and exit (QUEUE_STRIDE words a scenario).
+std::vector<uint32_t> HipCrowdSim::aheadTotals(uint32_t p) {
+    const size_t S = size_t(S_), stride = size_t(exitStride_), bins = size_t(CHOICE_AHEAD_BINS);
+    std::vector<uint32_t> a(S * 2 * stride * bins);
+    SWARM_HIP_CHECK(hipMemcpyAsync(a.data(), ch_.ahead, a.size() * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    std::vector<uint32_t> t(S * QUEUE_STRIDE, 0u);
+    for (size_t sc = 0; sc < S; ++sc)
+        for (size_t e = 0; e < std::min(stride, size_t(CHOICE_QUEUE_STRIDE)); ++e) {
+            const uint32_t* h = a.data() + (sc * 2 + p) * stride * bins + e * bins;
+            uint32_t sum = 0;
+            for (size_t b = 0; b < bins; ++b) sum += h[b];
+            t[sc * QUEUE_STRIDE + p * CHOICE_QUEUE_STRIDE + e] = sum;
+        }
+    return t;
+}
+
+std::vector<uint32_t> HipCrowdSim::queueCounts() {
+    uint32_t step = 0;
+    readControl(&step, nullptr, nullptr);
+    std::vector<uint32_t> q(size_t(S_) * QUEUE_STRIDE);
+    SWARM_HIP_CHECK(hipMemcpyAsync(q.data(), ch_.queue, q.size() * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    const std::vector<uint32_t> t = aheadTotals(step & 1u);
+    for (size_t k = 0; k < q.size(); ++k) q[k] += t[k];
+    return q;
+}
+
 void HipCrowdSim::setChoiceState(uint32_t step, const uint32_t* assign, const uint32_t* queue128, const uint32_t* ahead) {
+    std::vector<uint32_t> q = queueCounts();   // the counts as they stand (scenario 0's replaced by queue128)
+    if (queue128) std::copy(queue128, queue128 + 128, q.begin());
     SWARM_HIP_CHECK(hipMemcpyAsync(ctrl_ + CTRL_STEP, &step, 4, hipMemcpyHostToDevice, stream_));
     if (assign) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.assign, assign, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
-    if (queue128) SWARM_HIP_CHECK(hipMemcpyAsync(ch_.queue, queue128, 128 * 4, hipMemcpyHostToDevice, stream_));
     if (ahead) {   // the harness's layout: parity x 64 exits x 1024 bins, into scenario 0's exit stride
         for (int par = 0; par < 2; ++par)
             SWARM_HIP_CHECK(hipMemcpyAsync(ch_.ahead + size_t(par) * size_t(exitStride_) * CHOICE_AHEAD_BINS,
@@ -528,6 +559,10 @@ void HipCrowdSim::setChoiceState(uint32_t step, const uint32_t* assign, const ui
                                            size_t(std::min(exitStride_, CHOICE_QUEUE_STRIDE)) * CHOICE_AHEAD_BINS * 4, hipMemcpyHostToDevice,
                                            stream_));
     }
+    // the device form for the new parity: the next prefix pass adds the histogram totals back (modulo 2^32)
+    const std::vector<uint32_t> t = aheadTotals(step & 1u);
+    for (size_t k = 0; k < q.size(); ++k) q[k] -= t[k];
+    SWARM_HIP_CHECK(hipMemcpyAsync(ch_.queue, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream_));
     sync();
 }
 
@@ -593,14 +628,15 @@ void HipCrowdSim::readAssign(uint32_t* assign) {
 }
 
 void HipCrowdSim::readChoiceCounters(uint32_t* out) {
-    SWARM_HIP_CHECK(hipMemcpyAsync(out, ch_.queue, 128 * 4, hipMemcpyDeviceToHost, stream_));
+    const std::vector<uint32_t> q = queueCounts();
+    std::copy(q.begin(), q.begin() + 128, out);
     SWARM_HIP_CHECK(hipMemcpyAsync(out + 128, ch_.counters, 4 * 4, hipMemcpyDeviceToHost, stream_));
     sync();
 }
 
 void HipCrowdSim::readChoiceCountersAll(uint32_t* out) {
-    std::vector<uint32_t> q(size_t(S_) * QUEUE_STRIDE), c(size_t(S_) * COUNTER_STRIDE);
-    SWARM_HIP_CHECK(hipMemcpyAsync(q.data(), ch_.queue, q.size() * 4, hipMemcpyDeviceToHost, stream_));
+    const std::vector<uint32_t> q = queueCounts();
+    std::vector<uint32_t> c(size_t(S_) * COUNTER_STRIDE);
     SWARM_HIP_CHECK(hipMemcpyAsync(c.data(), ch_.counters, c.size() * 4, hipMemcpyDeviceToHost, stream_));
     sync();
     for (size_t s = 0; s < size_t(S_); ++s) {

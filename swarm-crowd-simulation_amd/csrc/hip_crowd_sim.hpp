@@ -96,6 +96,7 @@ public:
     void readGrid(uint32_t* key, uint32_t* id);
     void readCells(uint32_t* start, uint32_t* count);       // the reference's cell table over the global cells
     void readAssign(uint32_t* assign);
+    std::vector<uint32_t> queueCounts();                     // every scenario's queue counts (QUEUE_STRIDE words each)
     void readChoiceCounters(uint32_t* out132);               // scenario 0: queue[128], switches, forced, evaluations, eligible
     void readChoiceCountersAll(uint32_t* out);               // every scenario: 132 words each
     void readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit);   // perExit: scenario 0's E + 1 counters
@@ -151,6 +152,7 @@ public:
     const SceneView& view() const { return scenes_[0].view; }
 
 private:
+    std::vector<uint32_t> aheadTotals(uint32_t parity);
     void init(const std::vector<const Scene*>& layouts, const std::vector<int>& scenLayout, const std::vector<int>& scenAgents, bool batched);
     void enqueueStep(int parity);
     void buildGraph();

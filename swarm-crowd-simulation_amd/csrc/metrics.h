@@ -1,10 +1,11 @@
 // The safety metrics of a step (metrics_deposit.comp and metrics_finalize.comp, swarm/metrics.hpp:1-33): write-only
 // side outputs that read the step's input and output agent state and never write it.
-//   deposit   per agent (taken in the step's slot order, so neighbouring lanes deposit into neighbouring cells):
+//   deposit   per agent (taken in the step's slot order, so a block's agents stand in a few rows of the floor):
 //             an agent active in the input and inactive in the output is an evacuation (its exit's counter, the
 //             total, its evacuation step); every active output agent deposits one person onto the 4 floor cells
 //             around it (cloud-in-cell, positions quantised to 1/256 of a cell, integer weights summing to 65536)
-//             and the same weights times its speed in m/s; the active count, one atomic per block
+//             and the same weights times its speed in m/s, summed per block in LDS and added to each touched cell
+//             once; the active count, one atomic per block
 //   finalize  per floor cell: last, peak, occupied steps, steps at or above each threshold mass, the 64-bit sums of
 //             mass and speed mass, exposure (mass while at or above the crush threshold); then the step's mass is
 //             cleared and the counters published
@@ -24,6 +25,8 @@ constexpr uint32_t METRIC_SUBCELL = 256u;                    // metrics.hpp:77
 constexpr uint32_t METRIC_MASS_ONE = METRIC_SUBCELL * METRIC_SUBCELL;
 constexpr uint32_t METRIC_NEVER = 0xFFFFFFFFu;
 constexpr int METRIC_CRUSH_INDEX = 2;
+constexpr int METRIC_DEPOSIT_PER_THREAD = 1;                 // slots per deposit thread (256 threads a block)
+constexpr int METRIC_DEPOSIT_WINDOW = 4096;                  // cells a deposit block sums in LDS (2 words each)
 
 struct MetricGrid {
     int resX, resY;
