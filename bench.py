@@ -145,11 +145,25 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     gv0, g = sim.adjoint_backward()
     t2 = time.perf_counter()
     total = t2 - t0
+    finite = bool(np.isfinite(gv0).all() and np.isfinite(g).all())
+    # the same K steps with the state adjoint cleared every 25 steps (the truncated estimator long horizons use)
+    sim.set_adjoint_terms(window=25)
+    sim.upload(a)
+    t3 = time.perf_counter()
+    sim.adjoint_forward(K, every, 1.0, 0.1)
+    wv0, wg = sim.adjoint_backward()
+    t4 = time.perf_counter()
+    sim.set_adjoint_terms(window=0)
+    windowed = {"window": 25, "ms": 1000.0 * (t4 - t3), "gradient_finite": bool(np.isfinite(wv0).all() and np.isfinite(wg).all()),
+                "grad_v0_norm": float(np.linalg.norm(wv0.astype(np.float64)))}
     return {"K": K, "checkpoint_every": every, "agents": n, "physics_shape": list(sim.physics_shape()),
             "projection_lanes": sim.projection_lanes(), "autotune_ms_per_step": tune_ms, "forward_ms": 1000.0 * fwd,
             "adjoint_forward_ms": 1000.0 * (t1 - t0), "adjoint_backward_ms": 1000.0 * (t2 - t1),
             "cost_vs_forward": total / fwd, "agent_steps_per_s": steps_sum / total, "loss": loss,
-            "gradient_finite": bool(np.isfinite(gv0).all() and np.isfinite(g).all())}
+            "gradient_finite": finite,
+            "gradient_note": None if finite else "the exact pathwise gradient of a 200-step loss overflows FP32 in this crowd "
+                                                 "(its sensitivity grows about exp(0.46 K)); the timing is the measured quantity",
+            "windowed": windowed}
 
 
 def main():

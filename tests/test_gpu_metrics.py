@@ -173,3 +173,31 @@ def test_metrics_graph_and_eager_identical():
     s.metrics_enable(True)
     s.step(1)
     assert s.metric_counters()[0] == 152
+
+
+def test_experiment_reports_room(tmp_path):
+    """An evacuation run through experiment.py: the report's cumulative evacuations are the pipeline's own per-exit
+    counts, its person-seconds the active agents of every step, and its evacuation times the reference's rule."""
+    import experiment as ex
+
+    print("metrics: experiment report, room 4096, 300 steps every 25", flush=True)
+    x = ex.Experiment(sh.LAYOUT_ROOM, 4096, 4096, seed=99, every=25).run(300)
+    summ = x.write(str(tmp_path))
+    st, _, per = x.sim.evac()
+    assert [s["step"] for s in x.samples] == list(range(25, 301, 25))
+    assert np.array_equal(x.samples[-1]["exits"], per)
+    assert summ["evacuated"] == int((st >= 0).sum()) > 100
+    head = open(tmp_path / "timeseries.csv").readline().strip().split(",")
+    assert head[:4] == ["step", "t_model", "t_real_s", "active"] and len(head) == 4 + x.E + 1 + 1
+    lines = open(tmp_path / "heatmap.csv").read().strip().split("\n")
+    walk = x.walkable
+    assert len(lines) == 1 + int((walk >= np.float32(0.10)).sum())
+    # person-seconds: every step's active agents, one person of mass each (cells below 0.10 m^2 hold a little)
+    active_sum = 4096 * 300 - sum(300 - int(s) + 1 for s in x.sim.metric_evac_steps() if s)
+    cells = x.sim.metric_cells()
+    assert int(ex.sum64(cells, 8).sum()) == active_sum * 65536
+    assert summ["person_s"] <= active_sum * ex.DT_REAL_S * (1 + 1e-9)
+    t = ex.evacuation_times(x.sim.metric_evac_steps(), 4096)
+    left = np.sort(st[st >= 0]) + 1
+    if left.size >= 2048:
+        assert t["t50_step"] == left[2047]
