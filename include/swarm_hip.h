@@ -191,6 +191,15 @@ int swarm_set_projection_prune(swarm_ctx* c, int every);
  * reference. */
 int swarm_set_projection_lanes(swarm_ctx* c, int lanes);
 int swarm_get_projection_lanes(swarm_ctx* c, int* lanes);
+/* Lanes per slot (1, 2, 4 or 8; default 4) in the projection's first iteration, the ring scan that also collects
+ * each slot's contact candidates: a slot's lanes take interleaved slots of each ring row, add the contact terms in
+ * ring order and write the candidates at their rank in ring order, so results are the same bit for bit. No
+ * counterpart in the reference. */
+int swarm_set_projection_collect_lanes(swarm_ctx* c, int lanes);
+/* The projection's contact candidates after the last step, in slot order: count[n] (NBR_OVERFLOW 0xFFFFFFFF: more
+ * than 48, the ring scan is used instead), index and contact distance [48 x n], candidate-major (entry j of slot s at
+ * j * n + s). Any pointer may be NULL. No counterpart in the reference (its projection rescans the ring). */
+int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float* dist);
 /* Times each candidate physics launch shape over 'steps' steps from the current state, then the projection's lanes
  * per slot at the fastest shape; keeps the fastest and restores the state; *best_ms: its mean ms per step. No
  * counterpart in the reference. */

@@ -867,6 +867,102 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
     }
 }
 
+// The first iteration (ring scan + candidate collection, projectRing with 'collect') with Q lanes per slot. In each
+// ring row lane l of a slot takes the row's slots k0 + c Q + l (c < PROJ_CHUNK), so the (c, l) order of a round is
+// the row's slot order: the contact terms are added in that order through shuffles within the slot's lanes, and the
+// candidates within reach are written at their rank in that order (the slot's ballot bits below the lane), so the
+// sum, the list and its count are projectRing's bit for bit. Lane 0 finishes and writes. A slot's Q lanes are
+// adjacent lanes of one wave and follow the same path.
+template <bool B, int Q>
+__global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
+                                                                    ProjBuffers out, int P) {
+    static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
+    const int total = n * Q;
+    const size_t stride = size_t(n);
+    const int groupLo = __lane_id() & ~(Q - 1);   // the slot's first lane in the wave (blockDim is a multiple of 64)
+    const float Pf = float(P);
+    for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int slot = t / Q;
+        const int lane = t % Q;
+        const uint32_t key = b.skey[slot];
+        if (key == INACTIVE_CELL) continue;
+        const float2 bp = in.p[slot];
+        const float acc = in.acc[slot];
+        if (b.pw[slot] == 0.0f) {   // parked this step: carried
+            if (lane == 0) { out.p[slot] = bp; out.acc[slot] = acc; }
+            continue;
+        }
+        const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
+        const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
+        const uint32_t self = b.sid[slot] - sr.agentBase;
+        const float ri = b.sr[slot];
+        const uint32_t cid = key - sr.cellBase;
+        const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
+        const int R = s.projRing;
+        const int x0 = cx - R < 0 ? 0 : cx - R;
+        const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
+        V2 corr{ 0.0f, 0.0f };
+        int nContacts = 0;
+        uint32_t cnt = 0;
+        for (int yy = -R; yy <= R; yy++) {
+            const int ny = cy + yy;
+            if (ny < 0 || ny >= s.hashNY) continue;
+            const uint32_t row = sr.cellBase + uint32_t(ny * s.hashNX);
+            const uint32_t st = b.bstart[row + uint32_t(x0)], en = b.bstart[row + uint32_t(x1) + 1u];
+            for (uint32_t k0 = st; k0 < en; k0 += uint32_t(Q * PROJ_CHUNK)) {
+                float2 qp[PROJ_CHUNK];
+                float qw[PROJ_CHUNK], qr[PROJ_CHUNK];
+#pragma unroll
+                for (int c = 0; c < PROJ_CHUNK; ++c) {
+                    const uint32_t k = k0 + uint32_t(c * Q + lane);
+                    const uint32_t kc = k < en ? k : en - 1u;
+                    qp[c] = in.p[kc]; qw[c] = b.pw[kc]; qr[c] = b.sr[kc];
+                }
+#pragma unroll
+                for (int c = 0; c < PROJ_CHUNK; ++c) {
+                    if (k0 + uint32_t(c * Q) >= en) break;   // uniform over the slot's lanes
+                    const uint32_t k = k0 + uint32_t(c * Q + lane);
+                    const bool live = k < en && k != uint32_t(slot) && qw[c] != 0.0f;
+                    const V2 offset = V2{ bp.x, bp.y } - V2{ qp[c].x, qp[c].y };
+                    const float D = ri + qr[c];
+                    const float dxy2 = dot2(offset, offset);
+                    const bool hit = live && dxy2 < contact2(D);
+                    V2 term{ 0.0f, 0.0f };
+                    if (hit) term = overlapCorrection(offset, dxy2, D, self, b.sid, k, sr.agentBase);
+                    const float reach = (D + Pf * (PROJ_MAX_SHIFT_RADII * D)) * 1.001f + 1e-6f;
+                    const bool take = live && dxy2 < reach * reach;
+                    const uint32_t bits = uint32_t(__ballot(take) >> groupLo) & ((1u << Q) - 1u);
+                    if (take) {
+                        const uint32_t at = cnt + uint32_t(__popc(bits & ((1u << lane) - 1u)));
+                        if (at < uint32_t(NBR_CAP)) {
+                            b.nbr[size_t(at) * stride + size_t(slot)] = k;
+                            b.nbrD[size_t(at) * stride + size_t(slot)] = D;
+                        }
+                    }
+                    cnt += uint32_t(__popc(bits));
+#pragma unroll
+                    for (int l = 0; l < Q; ++l) {
+                        const int h = __shfl(int(hit), l, Q);
+                        const float tx = __shfl(term.x, l, Q);
+                        const float ty = __shfl(term.y, l, Q);
+                        if (h) {
+                            corr = corr + V2{ tx, ty };
+                            ++nContacts;
+                        }
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
+            float x, y, oacc;
+            finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc);
+            out.p[slot] = make_float2(x, y);
+            out.acc[slot] = oacc;
+        }
+    }
+}
+
 // iteration 'first' (1: ring scan + candidates) or a later one (the candidates), slot order in -> out. REC: iteration
 // 'it' of the adjoint's taped forward (contact masks, cap and plane bits)
 // 'prune' > 0 (not REC): the candidates are pruned for the 'prune' iterations left, this one included
@@ -1156,7 +1252,14 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
     for (int it = 1; it < P; ++it) {
         // pruning at iterations 1 + k pruneEvery (k >= 1); the final pass is iteration P
         const int prune = L.pruneEvery > 0 && it > 1 && (it - 1) % L.pruneEvery == 0 ? P - it + 1 : 0;
-        if (it == 1 || prune > 0 || L.projLanes <= 1) {
+        if (it == 1 && L.collectLanes > 1) {
+            const int q = L.collectLanes;
+            const int full = blocks(n * q, bs);
+            const int gq = L.physGrid > 0 && L.physGrid < full ? L.physGrid : full;
+            if (q == 2) k_project_collect<B, 2><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+            else if (q == 4) k_project_collect<B, 4><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+            else k_project_collect<B, 8><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
+        } else if (it == 1 || prune > 0 || L.projLanes <= 1) {
             k_project<B><<<g, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P, it == 1 ? 1 : 0, TapeStep{}, 0, prune);
         } else {
             const int q = L.projLanes;

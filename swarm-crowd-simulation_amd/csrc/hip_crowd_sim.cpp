@@ -203,6 +203,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     // the overlap loss term reads the candidate lists after the step: they stay whole while it is on
     L.pruneEvery = lossOn_ ? 0 : pruneEvery_;
     L.projLanes = projLanes_;
+    L.collectLanes = collectLanes_;
     return L;
 }
 
@@ -306,6 +307,12 @@ void HipCrowdSim::setProjectionLanes(int lanes) {
     if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setProjectionLanes: 1, 2, 4 or 8");
     if (lanes != projLanes_) dropGraph();
     projLanes_ = lanes;
+}
+
+void HipCrowdSim::setCollectLanes(int lanes) {
+    if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setCollectLanes: 1, 2, 4 or 8");
+    if (lanes != collectLanes_) dropGraph();
+    collectLanes_ = lanes;
 }
 
 void HipCrowdSim::setPruneEvery(int every) {
@@ -601,6 +608,14 @@ void HipCrowdSim::sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t*
     launchSortKeys(stream_, launchDesc(), half_[parity_], grid_, b, rad_, ctrl_);
     SWARM_HIP_CHECK(hipGetLastError());
     readGrid(sortedKeys, ids);
+}
+
+void HipCrowdSim::readCandidates(uint32_t* count, uint32_t* index, float* dist) {
+    const size_t n = size_t(n_);
+    if (count) SWARM_HIP_CHECK(hipMemcpyAsync(count, sb_.nbrCount, n * 4, hipMemcpyDeviceToHost, stream_));
+    if (index) SWARM_HIP_CHECK(hipMemcpyAsync(index, sb_.nbr, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
+    if (dist) SWARM_HIP_CHECK(hipMemcpyAsync(dist, sb_.nbrD, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
 }
 
 void HipCrowdSim::readGrid(uint32_t* key, uint32_t* id) {

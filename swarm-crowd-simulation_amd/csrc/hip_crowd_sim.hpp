@@ -76,6 +76,8 @@ public:
     // Lanes per slot in the projection's non-pruning iterations (1, 2, 4, 8); results are the same bit for bit
     void setProjectionLanes(int lanes);
     int projectionLanes() const { return projLanes_; }
+    void setCollectLanes(int lanes);
+    int collectLanes() const { return collectLanes_; }
     // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, then the
     // projection's lanes per slot at the fastest shape; keeps the fastest and puts the state back as it was (a
     // slot's result depends on neither). Returns the fastest setting's mean ms per step.
@@ -94,6 +96,7 @@ public:
     // INACTIVE_CELL; returns the sorted keys and the permutation (the grid readback then holds this sort)
     void sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t* ids);
     void readGrid(uint32_t* key, uint32_t* id);
+    void readCandidates(uint32_t* count, uint32_t* index, float* dist);   // slot order; NBR_CAP x n, candidate-major
     void readCells(uint32_t* start, uint32_t* count);       // the reference's cell table over the global cells
     void readAssign(uint32_t* assign);
     std::vector<uint32_t> queueCounts();                     // every scenario's queue counts (QUEUE_STRIDE words each)
@@ -180,6 +183,7 @@ private:
     int physBlock_ = 0, physGrid_ = 0;
     int pruneEvery_ = PROJ_PRUNE_EVERY;
     int projLanes_ = PROJ_LANES;
+    int collectLanes_ = PROJ_COLLECT_LANES;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};

@@ -287,6 +287,14 @@ int swarm_set_projection_lanes(swarm_ctx* c, int lanes) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setProjectionLanes(lanes); });
 }
+int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float* dist) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->readCandidates(count, index, dist); });
+}
+int swarm_set_projection_collect_lanes(swarm_ctx* c, int lanes) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setCollectLanes(lanes); });
+}
 int swarm_get_projection_lanes(swarm_ctx* c, int* lanes) {
     CTX_CHECK(c);
     if (!lanes) { g_err = "null argument"; return SWARM_EINVAL; }

@@ -334,6 +334,20 @@ class Sim:
         """Lanes per slot in the projection's list iterations (1, 2, 4, 8); bit-identical results."""
         check(lib().swarm_set_projection_lanes(self.h, lanes))
 
+    def candidate_lists(self):
+        """(count[n], index[48, n], contact distance[48, n]) of the projection's candidates, slot order."""
+        cnt = np.zeros(self.n, np.uint32)
+        idx = np.zeros((48, self.n), np.uint32)
+        d = np.zeros((48, self.n), np.float32)
+        lib().swarm_read_candidates.argtypes = [C.c_void_p] * 4
+        check(lib().swarm_read_candidates(self.h, cnt.ctypes.data, idx.ctypes.data, d.ctypes.data))
+        return cnt, idx, d
+
+    def set_projection_collect_lanes(self, lanes=4):
+        """Lanes per slot in the projection's first iteration (ring scan + candidates); bit-identical results."""
+        lib().swarm_set_projection_collect_lanes.argtypes = [C.c_void_p, C.c_int]
+        check(lib().swarm_set_projection_collect_lanes(self.h, lanes))
+
     def projection_lanes(self):
         v = C.c_int()
         check(lib().swarm_get_projection_lanes(self.h, C.byref(v)))

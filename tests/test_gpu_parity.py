@@ -153,22 +153,53 @@ def test_graph_and_eager_identical():
 
 @pytest.mark.parametrize("layout,active,total,steps", [(sh.LAYOUT_ROOM, 4096, 4096, 120), (sh.LAYOUT_BOWL, 63424, 65536, 12)])
 def test_candidate_pruning_is_exact(layout, active, total, steps):
-    """Pruning the projection's contact candidates (every 1, 4 or 8 iterations) and 2, 4 or 8 lanes per slot give
-    the single-lane, unpruned run's bits."""
+    """Pruning the projection's contact candidates (every 1, 4 or 8 iterations), 2, 4 or 8 lanes per slot in the
+    list iterations and 2, 4 or 8 in the first (collecting) iteration give the single-lane, unpruned run's bits,
+    candidate lists included."""
     scene = sh.Scene.layout(layout)
     a = sh.place_agents(layout, active, total, 8)
     runs = []
-    for every, lanes in ((0, 1), (1, 1), (4, 1), (8, 1), (0, 2), (0, 4), (8, 4), (0, 8)):
+    for every, lanes, collect in ((0, 1, 1), (1, 1, 1), (4, 1, 1), (8, 1, 1), (0, 2, 1), (0, 4, 1), (8, 4, 1), (0, 8, 1),
+                                  (0, 1, 2), (0, 1, 4), (0, 1, 8), (8, 1, 4)):
         s = sh.Sim(scene, total)
         s.set_projection_prune(every)
         s.set_projection_lanes(lanes)
+        s.set_projection_collect_lanes(collect)
         s.upload(a)
         s.step(steps)
-        runs.append(((every, lanes), s.state(), s.assign()))
+        runs.append(((every, lanes, collect), s.state(), s.assign()))
     for cfg, st, asg in runs[1:]:
         for f in FIELDS:
-            assert_bits(st[f], runs[0][1][f], f"prune every {cfg[0]}, {cfg[1]} lanes: {f}")
+            assert_bits(st[f], runs[0][1][f], f"prune every {cfg[0]}, {cfg[1]} lanes, {cfg[2]} collecting lanes: {f}")
         assert np.array_equal(asg, runs[0][2])
+
+
+def test_collecting_lanes_write_the_same_lists():
+    """After the first iteration the candidate lists (count, indices, contact distances) of 2, 4 and 8 collecting
+    lanes are the single lane's, entry for entry, on the bowl and in the dense room."""
+    for layout, active, total in ((sh.LAYOUT_BOWL, 63424, 65536), (sh.LAYOUT_ROOM, 4096, 4096)):
+        scene = sh.Scene.layout(layout)
+        a = sh.place_agents(layout, active, total, 3)
+        lists = []
+        for collect in (1, 2, 4, 8):
+            s = sh.Sim(scene, total, proj_iters=2)   # the first iteration, then the final pass (which reads the lists)
+            s.set_projection_prune(0)
+            s.set_projection_collect_lanes(collect)
+            s.upload(a)
+            s.step(3)
+            key, ids = s.grid()
+            moving = (key != sh.INACTIVE_CELL) & (s.state()["w"][ids] != 0)   # the slots that collected this step
+            lists.append((s.candidate_lists(), s.state(), moving))
+        (c0, k0, d0), st0, mv = lists[0]
+        c0 = np.where(mv, c0, 0)
+        for (c, k, d), st, _ in lists[1:]:
+            assert np.array_equal(np.where(mv, c, 0), c0)
+            live = np.arange(k.shape[0])[:, None] < np.where(c0 == 0xFFFFFFFF, 0, c0)[None, :]
+            assert np.array_equal(np.where(live, k, 0), np.where(live, k0, 0))
+            assert np.array_equal(np.where(live, d.view(np.uint32), 0), np.where(live, d0.view(np.uint32), 0))
+            for f in FIELDS:
+                assert_bits(st[f], st0[f], f"collecting lanes {f}")
+        assert ((c0 > 0) & (c0 != 0xFFFFFFFF)).sum() > 100   # lists were written and compared
 
 
 def test_sort_grid_on_given_keys():
