@@ -21,6 +21,7 @@ constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projecti
 constexpr int PROJ_PRUNE_EVERY = 8; // projection iterations between candidate prunings (0: never)
 constexpr int PROJ_LANES = 1;       // lanes per slot in the projection iterations that do not prune (autotuned)
 constexpr int PROJ_COLLECT_LANES = 4;   // lanes per slot in the first projection iteration (ring scan + candidates)
+constexpr int PRED_LANES = 4;           // lanes per slot in predict (its perception ring)
 constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
 
@@ -171,6 +172,7 @@ struct StepLaunch {
     int pruneEvery = 0;   // > 0: the projection drops candidates out of reach every that many iterations
     int projLanes = 1;    // lanes per slot in the projection iterations that do not prune (1, 2, 4 or 8)
     int collectLanes = 1; // lanes per slot in the first projection iteration (1, 2, 4 or 8)
+    int predictLanes = 1; // lanes per slot in predict (1, 2, 4 or 8)
 };
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,

@@ -407,9 +407,14 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_goal_drive(int n, const uint32_t
 }
 
 // ---------------- PHYSICS_PASS 1: predict ----------------
-template <bool B, bool REC = false>
+// Q > 1 (not REC): the slot's Q adjacent lanes share the perception ring. Lane l takes the row's slots
+// k0 + u Q + l, so (u, l) order is the row's slot order: every lane adds the separation terms in that order through
+// shuffles within the slot's lanes (the single lane's sum bit for bit); the headway gap is a minimum over the
+// agents ahead, the same in any order, combined across the lanes. Lane 0 then finishes the slot alone.
+template <bool B, bool REC = false, int Q = 1>
 __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchView& bt, const ModelParams& mp, const StepBuffers& b,
-                                   const ProjBuffers& out, const TapeStep& tp = TapeStep{}) {
+                                   const ProjBuffers& out, const TapeStep& tp = TapeStep{}, int lane = 0) {
+    static_assert(Q == 1 || !REC, "the taped predict runs one lane per slot");
     const uint32_t key = b.skey[slot];
     swarm_predict_rec* rec = nullptr;
     if constexpr (REC) {
@@ -451,15 +456,46 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
         const uint32_t row = sr.cellBase + uint32_t(ny * s.hashNX);
         const uint32_t st = b.bstart[row + uint32_t(x0)], en = b.bstart[row + uint32_t(x1) + 1u];
         // the row's slots in chunks: a chunk's loads are in flight together; the sums keep the ring order
-        for (uint32_t k0 = st; k0 < en; k0 += PRED_CHUNK) {
+        for (uint32_t k0 = st; k0 < en; k0 += uint32_t(Q * PRED_CHUNK)) {
             float qx[PRED_CHUNK], qy[PRED_CHUNK], qr[PRED_CHUNK];
 #pragma unroll
             for (int u = 0; u < PRED_CHUNK; ++u) {
-                const uint32_t k = k0 + uint32_t(u) < en ? k0 + uint32_t(u) : en - 1u;
+                const uint32_t kk = k0 + uint32_t(u * Q + lane);
+                const uint32_t k = kk < en ? kk : en - 1u;
                 qx[u] = b.sx[k]; qy[u] = b.sy[k]; qr[u] = b.sr[k];
             }
 #pragma unroll
             for (int u = 0; u < PRED_CHUNK; ++u) {
+                if constexpr (Q > 1) {
+                    if (k0 + uint32_t(u * Q) >= en) break;   // uniform over the slot's lanes
+                    const uint32_t k = k0 + uint32_t(u * Q + lane);
+                    const V2 offset = p - V2{ qx[u], qy[u] };
+                    const float dxy2 = dot2(offset, offset);
+                    const bool hit = k < en && k != uint32_t(slot) && dxy2 < R_PERC2;
+                    V2 term{ 0.0f, 0.0f };
+                    if (hit) {
+                        const uint32_t other = b.sid[k] - sr.agentBase;
+                        const float D = ri + qr[u];
+                        term = separationForce(offset, self, other, D + SEP_MARGIN);
+                        const float ahead = -dot2(offset, eGoal);
+                        const float lateral = fabsf(offset.x * eGoal.y - offset.y * eGoal.x);
+                        const float gap = sqrtf(dxy2) - D;
+                        if (ahead > 0.0f && lateral < D && gap < sAhead) {
+                            const V2 eOther{ b.egx[k], b.egy[k] };
+                            const float aheadOther = dot2(offset, eOther);
+                            const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < D;
+                            if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = gap;
+                        }
+                    }
+#pragma unroll
+                    for (int l = 0; l < Q; ++l) {
+                        const int h = __shfl(int(hit), l, Q);
+                        const float tx = __shfl(term.x, l, Q);
+                        const float ty = __shfl(term.y, l, Q);
+                        if (h) sep = sep + V2{ tx, ty };
+                    }
+                    continue;
+                }
                 const uint32_t k = k0 + uint32_t(u);
                 if (k >= en) break;
                 if (k == uint32_t(slot)) continue;
@@ -485,6 +521,14 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
                 }
             }
         }
+    }
+    if constexpr (Q > 1) {
+#pragma unroll
+        for (int d = 1; d < Q; d <<= 1) {
+            const float o = __shfl_xor(sAhead, d, Q);
+            sAhead = o < sAhead ? o : sAhead;
+        }
+        if (lane != 0) return;
     }
     const float stair = stairFactor(s, p);
     const float vFree = stair * b.v0[agent];
@@ -567,6 +611,16 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0,
                                                             TapeStep tp = TapeStep{}) {
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x)
         predictSlot<B, REC>(slot, s0, bt, mp, b, out, tp);
+}
+
+// predict with Q lanes per slot (the perception ring shared, then lane 0 alone)
+template <bool B, int Q>
+__global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict_lanes(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b,
+                                                                  ProjBuffers out) {
+    static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
+    const int total = n * Q;
+    for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
+        predictSlot<B, false, Q>(t / Q, s0, bt, mp, b, out, TapeStep{}, t % Q);
 }
 
 // ---------------- PHYSICS_PASS 2: the Jacobi projection ----------------
@@ -1246,7 +1300,16 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
     const int full = blocks(n, bs);
     const int g = L.physGrid > 0 && L.physGrid < full ? L.physGrid : full;   // a persistent grid, or one slot per thread
     const int P = L.mp.projIters;
-    k_predict<B><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, pa);
+    if (L.predictLanes > 1) {
+        const int q = L.predictLanes;
+        const int full = blocks(n * q, bs);
+        const int gq = L.physGrid > 0 && L.physGrid < full ? L.physGrid : full;
+        if (q == 2) k_predict_lanes<B, 2><<<gq, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, pa);
+        else if (q == 4) k_predict_lanes<B, 4><<<gq, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, pa);
+        else k_predict_lanes<B, 8><<<gq, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, pa);
+    } else {
+        k_predict<B><<<g, bs, 0, st>>>(n, L.s, L.bt, L.mp, b, pa);
+    }
     if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < P; ++it) {

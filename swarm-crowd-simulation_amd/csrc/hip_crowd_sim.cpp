@@ -204,6 +204,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.pruneEvery = lossOn_ ? 0 : pruneEvery_;
     L.projLanes = projLanes_;
     L.collectLanes = collectLanes_;
+    L.predictLanes = predictLanes_;
     return L;
 }
 
@@ -313,6 +314,12 @@ void HipCrowdSim::setCollectLanes(int lanes) {
     if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setCollectLanes: 1, 2, 4 or 8");
     if (lanes != collectLanes_) dropGraph();
     collectLanes_ = lanes;
+}
+
+void HipCrowdSim::setPredictLanes(int lanes) {
+    if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setPredictLanes: 1, 2, 4 or 8");
+    if (lanes != predictLanes_) dropGraph();
+    predictLanes_ = lanes;
 }
 
 void HipCrowdSim::setPruneEvery(int every) {

@@ -77,6 +77,7 @@ public:
     void setProjectionLanes(int lanes);
     int projectionLanes() const { return projLanes_; }
     void setCollectLanes(int lanes);
+    void setPredictLanes(int lanes);
     int collectLanes() const { return collectLanes_; }
     // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, then the
     // projection's lanes per slot at the fastest shape; keeps the fastest and puts the state back as it was (a
@@ -184,6 +185,7 @@ private:
     int pruneEvery_ = PROJ_PRUNE_EVERY;
     int projLanes_ = PROJ_LANES;
     int collectLanes_ = PROJ_COLLECT_LANES;
+    int predictLanes_ = PRED_LANES;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};

@@ -343,6 +343,11 @@ class Sim:
         check(lib().swarm_read_candidates(self.h, cnt.ctypes.data, idx.ctypes.data, d.ctypes.data))
         return cnt, idx, d
 
+    def set_predict_lanes(self, lanes=4):
+        """Lanes per slot in predict's perception ring; bit-identical results."""
+        lib().swarm_set_predict_lanes.argtypes = [C.c_void_p, C.c_int]
+        check(lib().swarm_set_predict_lanes(self.h, lanes))
+
     def set_projection_collect_lanes(self, lanes=4):
         """Lanes per slot in the projection's first iteration (ring scan + candidates); bit-identical results."""
         lib().swarm_set_projection_collect_lanes.argtypes = [C.c_void_p, C.c_int]
