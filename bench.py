@@ -156,6 +156,22 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     sim.set_adjoint_terms(window=0)
     windowed = {"window": 25, "ms": 1000.0 * (t4 - t3), "gradient_finite": bool(np.isfinite(wv0).all() and np.isfinite(wg).all()),
                 "grad_v0_norm": float(np.linalg.norm(wv0.astype(np.float64)))}
+    # the whole run taped in its forward (one segment of K steps, no recompute): the tape's memory for the speed
+    sim.upload(a)
+    sim.adjoint_forward(K, K, 1.0, 0.1)   # warm-up (the K-step tape and its graphs)
+    sim.adjoint_backward()
+    sim.upload(a)
+    t5 = time.perf_counter()
+    loss_w = sim.adjoint_forward(K, K, 1.0, 0.1)
+    t6 = time.perf_counter()
+    tv0, tg = sim.adjoint_backward()
+    t7 = time.perf_counter()
+    finite_both = np.isfinite(tv0) & np.isfinite(gv0)
+    rel = float(np.max(np.abs(tv0[finite_both].astype(np.float64) - gv0[finite_both]) / np.maximum(np.abs(gv0[finite_both]), 1e-30))) \
+        if finite_both.any() else None
+    whole = {"checkpoint_every": K, "adjoint_forward_ms": 1000.0 * (t6 - t5), "adjoint_backward_ms": 1000.0 * (t7 - t6),
+             "cost_vs_forward": (t7 - t5) / fwd, "adjoint_memory_mb": sim.tape_bytes() / 1e6, "loss_equal": loss_w == loss,
+             "grad_v0_max_rel_diff_vs_checkpointed": rel}
     return {"K": K, "checkpoint_every": every, "agents": n, "physics_shape": list(sim.physics_shape()),
             "projection_lanes": sim.projection_lanes(), "autotune_ms_per_step": tune_ms, "forward_ms": 1000.0 * fwd,
             "adjoint_forward_ms": 1000.0 * (t1 - t0), "adjoint_backward_ms": 1000.0 * (t2 - t1),
@@ -163,7 +179,7 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
             "gradient_finite": finite,
             "gradient_note": None if finite else "the exact pathwise gradient of a 200-step loss overflows FP32 in this crowd "
                                                  "(its sensitivity grows about exp(0.46 K)); the timing is the measured quantity",
-            "windowed": windowed}
+            "windowed": windowed, "whole_tape": whole}
 
 
 def main():

@@ -213,6 +213,7 @@ HipCrowdSim::~HipCrowdSim() {
     dropGraph();
     for (void* p : bufs_) (void)hipFree(p);
     for (void* p : adjBufs_) (void)hipFree(p);
+    for (void* p : tapeBufs_) (void)hipFree(p);
     for (void* p : mBufs_) (void)hipFree(p);
     for (char* p : ckpt_) (void)hipFree(p);
     for (DeviceScene& d : scenes_) d.release();
@@ -727,6 +728,7 @@ void HipCrowdSim::allocAdjoint(int every) {
         void* p = nullptr;
         SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
         adjBufs_.push_back(p);
+        adjBytes_ += bytes;
         return p;
     };
     const size_t n = size_t(n_);
@@ -753,25 +755,38 @@ void HipCrowdSim::allocAdjoint(int every) {
     }
     if (every <= adjCap_) return;
     dropAdjointGraphs();   // they hold the old tape's addresses
+    SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (void* p : tapeBufs_) (void)hipFree(p);   // the smaller tape
+    tapeBufs_.clear();
+    adjBytes_ -= tapeBytes_;
+    tapeBytes_ = 0;
+    auto talloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
+        tapeBufs_.push_back(p);
+        tapeBytes_ += bytes;
+        adjBytes_ += bytes;
+        return p;
+    };
     // a segment's tape: 'every' steps
     const size_t C = size_t(every), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
     float** f1[] = { &tape_.inx, &tape_.iny, &tape_.invx, &tape_.invy, &tape_.inw, &tape_.outx, &tape_.outy, &tape_.sx, &tape_.sy,
                      &tape_.sr, &tape_.egx, &tape_.egy, &tape_.pw, &tape_.pvx, &tape_.pvy };
-    for (float** f : f1) *f = static_cast<float*>(dalloc(C * n * 4));
-    tape_.sid = static_cast<uint32_t*>(dalloc(C * n * 4));
-    tape_.skey = static_cast<uint32_t*>(dalloc(C * n * 4));
-    tape_.bstart = static_cast<uint32_t*>(dalloc(C * nbs * 4));
-    tape_.planes = static_cast<float4*>(dalloc(C * n * 16));
-    tape_.nbr = static_cast<uint32_t*>(dalloc(C * size_t(NBR_CAP) * n * 4));
-    tape_.nbrD = static_cast<float*>(dalloc(C * size_t(NBR_CAP) * n * 4));
-    tape_.nbrCount = static_cast<uint32_t*>(dalloc(C * n * 4));
-    tape_.pred = static_cast<swarm_predict_rec*>(dalloc(C * n * sizeof(swarm_predict_rec)));
-    tape_.fin = static_cast<swarm_final_rec*>(dalloc(C * n * sizeof(swarm_final_rec)));
-    tape_.mask = static_cast<uint64_t*>(dalloc(C * P * n * 8));
-    tape_.capBits = static_cast<uint64_t*>(dalloc(C * n * 8));
-    tape_.planeBits = static_cast<uint64_t*>(dalloc(C * n * 8));
-    tape_.overlap = static_cast<uint64_t*>(dalloc(C * n * 8));
-    tape_.q = static_cast<float2*>(dalloc(C * (P + 1) * n * 8));
+    for (float** f : f1) *f = static_cast<float*>(talloc(C * n * 4));
+    tape_.sid = static_cast<uint32_t*>(talloc(C * n * 4));
+    tape_.skey = static_cast<uint32_t*>(talloc(C * n * 4));
+    tape_.bstart = static_cast<uint32_t*>(talloc(C * nbs * 4));
+    tape_.planes = static_cast<float4*>(talloc(C * n * 16));
+    tape_.nbr = static_cast<uint32_t*>(talloc(C * size_t(NBR_CAP) * n * 4));
+    tape_.nbrD = static_cast<float*>(talloc(C * size_t(NBR_CAP) * n * 4));
+    tape_.nbrCount = static_cast<uint32_t*>(talloc(C * n * 4));
+    tape_.pred = static_cast<swarm_predict_rec*>(talloc(C * n * sizeof(swarm_predict_rec)));
+    tape_.fin = static_cast<swarm_final_rec*>(talloc(C * n * sizeof(swarm_final_rec)));
+    tape_.mask = static_cast<uint64_t*>(talloc(C * P * n * 8));
+    tape_.capBits = static_cast<uint64_t*>(talloc(C * n * 8));
+    tape_.planeBits = static_cast<uint64_t*>(talloc(C * n * 8));
+    tape_.overlap = static_cast<uint64_t*>(talloc(C * n * 8));
+    tape_.q = static_cast<float2*>(talloc(C * (P + 1) * n * 8));
     const uint32_t sentinel = uint32_t(n_);
     for (size_t k = 0; k < C; ++k)
         SWARM_HIP_CHECK(hipMemcpyAsync(tape_.bstart + k * nbs + size_t(nb_), &sentinel, 4, hipMemcpyHostToDevice, stream_));
@@ -803,6 +818,7 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
         char* p = nullptr;
         SWARM_HIP_CHECK(hipMalloc(&p, snapBytes_));
         ckpt_.push_back(p);
+        adjBytes_ += snapBytes_;
     }
     ckptParity_.assign(size_t(adjSegs_), 0);
     SWARM_HIP_CHECK(hipMemsetAsync(lossAcc_, 0, 8 * size_t(S_), stream_));
@@ -814,7 +830,10 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     for (int seg = 0; seg < adjSegs_; ++seg) {
         snapshotTo(ckpt_[size_t(seg)]);
         ckptParity_[size_t(seg)] = parity_;
-        step(std::min(adjEvery_, K - seg * adjEvery_));
+        // the last segment runs taped: the backward starts there and needs no replay of it (a run of one segment,
+        // every >= K, is taped whole and never replayed)
+        if (seg == adjSegs_ - 1) recordSegment(seg);
+        else step(std::min(adjEvery_, K - seg * adjEvery_));
     }
     lossOn_ = false;
     dropGraph();
@@ -864,10 +883,17 @@ void HipCrowdSim::tapeSegment(int seg) {
     if (seg < 0 || seg >= adjSegs_) throw std::invalid_argument("adjoint: no such segment");
     restoreFrom(ckpt_[size_t(seg)]);
     parity_ = ckptParity_[size_t(seg)];
+    recordSegment(seg);
+}
+
+// segment 'seg''s taped steps from the current state (its checkpoint, or the forward's own state at its start)
+void HipCrowdSim::recordSegment(int seg) {
     SWARM_HIP_CHECK(hipMemsetAsync(tape_.err, 0, 4, stream_));
     const int s0 = seg * adjEvery_, s1 = std::min(adjK_, s0 + adjEvery_);
     const int len = s1 - s0;
-    if (opt_.useGraph && len == adjEvery_) {   // a full segment: one captured replay per starting parity
+    // a full segment of at most ADJ_GRAPH_MAX_STEPS steps: one captured replay per starting parity (longer graphs
+    // launch slower than the same kernels launched eagerly)
+    if (opt_.useGraph && len == adjEvery_ && len <= ADJ_GRAPH_MAX_STEPS) {
         const int p = parity_;
         if (!replayExec_[p]) {
             SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -924,11 +950,11 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.gtau, 0, S * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.ggain, 0, S * 4, stream_));
     for (int seg = adjSegs_ - 1; seg >= 0; --seg) {
-        tapeSegment(seg);
+        if (tapedSeg_ != seg) tapeSegment(seg);
         const int len = std::min(adjK_, (seg + 1) * adjEvery_) - seg * adjEvery_;
         // a full segment whose clearing pattern repeats from segment to segment: one captured backward per
         // ping-pong half; the segment's first state comes from the device word stateBase_
-        if (opt_.useGraph && len == adjEvery_ && (window_ == 0 || adjEvery_ % window_ == 0)) {
+        if (opt_.useGraph && len == adjEvery_ && len <= ADJ_GRAPH_MAX_STEPS && (window_ == 0 || adjEvery_ % window_ == 0)) {
             launchAdjointSetBase(stream_, stateBase_, seg * adjEvery_);
             const int h = adj_.gx == adjG0_ ? 0 : 1;
             const AdjBufs before = adj_;

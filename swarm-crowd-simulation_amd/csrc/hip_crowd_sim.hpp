@@ -174,6 +174,10 @@ private:
     void allocAdjoint(int every);
     void tapedStep(int k);
     void tapeSegment(int seg);
+    void recordSegment(int seg);
+public:
+    size_t adjointBytes() const { return adjBytes_; }   // device memory the adjoint holds (tape, buffers, checkpoints)
+private:
     AdjStep adjStepView(int k) const;
 
     std::vector<DeviceScene> scenes_;
@@ -252,6 +256,9 @@ private:
     } tape_{};
     AdjBufs adj_{};
     std::vector<void*> adjBufs_;         // the adjoint's own allocations
+    size_t adjBytes_ = 0;                // their bytes, checkpoints included
+    std::vector<void*> tapeBufs_;        // the tape's (freed when a longer segment needs a larger one)
+    size_t tapeBytes_ = 0;
 
     // the safety metrics
     void allocMetrics();

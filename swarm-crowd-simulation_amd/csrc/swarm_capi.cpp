@@ -365,6 +365,12 @@ int swarm_adjoint_forward(swarm_ctx* c, int K, int checkpoint_every, float radiu
     if (!loss) { g_err = "null argument"; return SWARM_EINVAL; }
     return guarded([&] { *loss = c->sim->adjointForward(K, checkpoint_every, radius_scale, overlap_weight); });
 }
+int swarm_adjoint_memory(swarm_ctx* c, size_t* bytes) {
+    CTX_CHECK(c);
+    if (!bytes) { g_err = "null argument"; return SWARM_EINVAL; }
+    *bytes = c->sim->adjointBytes();
+    return SWARM_OK;
+}
 int swarm_adjoint_backward(swarm_ctx* c, float* grad_v0, double* grad_globals) {
     CTX_CHECK(c);
     if (!grad_v0 || !grad_globals) { g_err = "null argument"; return SWARM_EINVAL; }

@@ -515,10 +515,20 @@ def _sim_adjoint_tape(self, step, tape, exit_recs=None):
     return tape
 
 
+def _sim_tape_bytes(self):
+    """Device bytes the adjoint holds (tape, buffers, checkpoints)."""
+    L = _adjoint_lib()
+    L.swarm_adjoint_memory.argtypes = [C.c_void_p, C.c_void_p]
+    b = C.c_size_t()
+    check(L.swarm_adjoint_memory(self.h, C.byref(b)))
+    return b.value
+
+
 def _sim_set_model(self, tau_s, obstacle_gain=20.0):
     check(_adjoint_lib().swarm_set_model(self.h, tau_s, obstacle_gain))
 
 
+Sim.tape_bytes = _sim_tape_bytes
 Sim.adjoint_forward = _sim_adjoint_forward
 Sim.adjoint_backward = _sim_adjoint_backward
 Sim.adjoint_tape = _sim_adjoint_tape

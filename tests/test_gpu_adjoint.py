@@ -296,3 +296,28 @@ def test_crossing_terms_match_differences():
     # percent on some agents
     assert len(errs) >= 5
     assert np.median(errs) < 1e-2 and max(errs) < 6e-2
+
+
+def test_taped_forward_segments_match_recompute():
+    # the forward tapes its last segment (the backward starts there without recomputing it); a checkpoint interval
+    # >= K tapes the whole run and recomputes nothing: every split gives the same loss and dL/dv0 bit for bit, also
+    # with a truncation window, and the tape read back for a step of the last segment is the recomputed one
+    n, K = 1024, 45
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, 6)
+    out = {}
+    for every in (20, 7, 45, 100):
+        for graph in (True, False):
+            sim = sh.Sim(scene, n, use_graph=graph)
+            sim.upload(a)
+            loss = sim.adjoint_forward(K, every, 1.0, 0.1)
+            r1 = sim.adjoint_backward()
+            sim.set_adjoint_terms(window=10)
+            r2 = sim.adjoint_backward()
+            out[(every, graph)] = (loss, r1[0], r2[0], sim.tape_bytes())
+    ref = out[(20, False)]
+    for key, (loss, g1, g2, nbytes) in out.items():
+        assert loss == ref[0], key
+        assert_bits(g1, ref[1], f"dL/dv0, checkpoint every {key[0]}, graph {key[1]}")
+        assert_bits(g2, ref[2], f"dL/dv0 with a window, checkpoint every {key[0]}, graph {key[1]}")
+    assert out[(45, True)][3] > out[(7, True)][3]   # the whole run's tape is the larger
