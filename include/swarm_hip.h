@@ -200,9 +200,11 @@ int swarm_set_projection_collect_lanes(swarm_ctx* c, int lanes);
  * than 48, the ring scan is used instead), index and contact distance [48 x n], candidate-major (entry j of slot s at
  * j * n + s). Any pointer may be NULL. No counterpart in the reference (its projection rescans the ring). */
 int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float* dist);
-/* Lanes per slot (1, 2, 4 or 8; default 4) in predict's perception-ring scan (PHYSICS_PASS 1, cpu_crowd_sim.cpp:685-723): a
- * slot's lanes take interleaved slots of each ring row and add the separation terms in ring order; the headway gap is
- * a minimum, the same in any order. Results are the same bit for bit. No counterpart in the reference. */
+/* Lanes per slot (1, 2, 4 or 8; default 4) in predict (PHYSICS_PASS 1, cpu_crowd_sim.cpp:655-777): a slot's lanes
+ * take interleaved slots of each perception-ring row and add the separation terms in ring order (the headway gap is
+ * a minimum, the same in any order); then every lane carries the same state and the wall contacts split their 16
+ * feelers and two contact measurements over the lanes. Results are the same bit for bit. No counterpart in the
+ * reference. */
 int swarm_set_predict_lanes(swarm_ctx* c, int lanes);
 /* Times each candidate physics launch shape over 'steps' steps from the current state, then the projection's lanes
  * per slot at the fastest shape; keeps the fastest and restores the state; *best_ms: its mean ms per step. No

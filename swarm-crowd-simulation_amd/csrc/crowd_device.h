@@ -415,6 +415,117 @@ __device__ inline bool resolveStatic(const SceneView& s, V2 pOld, V2& np, V2& nv
     NoRec r;
     return resolveStatic(s, pOld, np, nv, rad, wall, r);
 }
+
+// ---- the same wall contacts shared by a slot's Q adjacent lanes ----
+// Every lane of the slot calls these with the same arguments and gets the same results (the state stays identical
+// across the lanes); only the work splits. The feelers: lane l samples feelers l, l + Q, ...; the clearing pass's
+// choice (the deepest feeler below zero, then the next, a tie going to the lower feeler index: the sequential scan's
+// rule) is the two smallest (value, index) pairs, merged across the lanes. The two contacts of a pass are measured
+// on lanes 0 and 1 and exchanged.
+struct Deep { float v; int i; };   // i < 0: none
+__device__ inline bool deeper(Deep a, Deep b) { return a.i >= 0 && (b.i < 0 || a.v < b.v || (a.v == b.v && a.i < b.i)); }
+
+template <int Q>
+__device__ inline void deepestFeelers(const SceneView& s, V2 np, float reach, int lane, Deep& d1, Deep& d2) {
+    d1 = Deep{ 0.0f, -1 };
+    d2 = Deep{ 0.0f, -1 };
+    for (int i = lane; i < FEELER_COUNT; i += Q) {   // ascending within the lane: the sequential rule on its share
+        const V2 f = np + V2{ FEELER_DIRS[i].x * reach, FEELER_DIRS[i].y * reach };
+        const float v = mapSdf(s, f);
+        if (!(v < 0.0f)) continue;
+        const Deep c{ v, i };
+        if (deeper(c, d1)) { d2 = d1; d1 = c; }
+        else if (deeper(c, d2)) d2 = c;
+    }
+#pragma unroll
+    for (int m = 1; m < Q; m <<= 1) {
+        const Deep o1{ __shfl_xor(d1.v, m, Q), __shfl_xor(d1.i, m, Q) };
+        const Deep o2{ __shfl_xor(d2.v, m, Q), __shfl_xor(d2.i, m, Q) };
+        // the two smallest of d1 <= d2 and o1 <= o2
+        if (deeper(o1, d1)) { d2 = deeper(d1, o2) ? d1 : o2; d1 = o1; }
+        else if (deeper(o1, d2)) d2 = o1;
+    }
+}
+
+template <int Q>
+__device__ inline bool clearWallsQ(const SceneView& s, V2& np, V2& nv, float rad, float wall[3], int lane) {
+    bool moved = false;
+    wall[0] = wall[1] = wall[2] = 0.0f;
+    const float reach = FEELER_REACH_RADII * rad;
+    const float recastTol = WALL_RECAST_TOL_RADII * rad;
+    for (int pass = 0; pass < WALL_PASSES; ++pass) {
+        const float fc = mapSdf(s, np);
+        Deep d1, d2;
+        deepestFeelers<Q>(s, np, reach, lane, d1, d2);
+        if (d1.i < 0) break;
+        V2 hNorm;
+        float clearance;
+        if (fc > 0.0f) {
+            // lane 0 the deepest feeler's contact, lane 1 the next one's (the other lanes repeat lane 0's)
+            const Deep d = (lane == 1 && d2.i >= 0) ? d2 : d1;
+            V2 h;
+            const float c = wallContact<false>(s, np, fc, V2{ FEELER_DIRS[d.i].x, FEELER_DIRS[d.i].y }, d.v, reach, recastTol, h);
+            clearance = __shfl(c, 0, Q);
+            hNorm = V2{ __shfl(h.x, 0, Q), __shfl(h.y, 0, Q) };
+            if (d2.i >= 0) {
+                const float c2 = __shfl(c, 1, Q);
+                const V2 h2{ __shfl(h.x, 1, Q), __shfl(h.y, 1, Q) };
+                if (c2 < clearance) { clearance = c2; hNorm = h2; }
+            }
+        } else {
+            const G3 g = mapExactGrad(s, np);
+            hNorm = safeNormalize(V2{ g.gx, g.gy });
+            const float gl = len2(V2{ g.gx, g.gy });
+            const float L = gl > CENTRE_GRAD_MIN ? gl : CENTRE_GRAD_MIN;
+            clearance = g.w / L;
+        }
+        if (clearance >= rad) {
+            if (clearance < NO_CONTACT) { wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = clearance - rad; }
+            break;
+        }
+        np = np + hNorm * (rad - clearance);
+        moved = true;
+        wall[0] = hNorm.x; wall[1] = hNorm.y; wall[2] = 0.0f;
+        const float vDotN = dot2(nv, hNorm);
+        if (vDotN < 0.0f) nv = nv - hNorm * vDotN;
+    }
+    return moved;
+}
+
+template <int Q>
+__device__ inline bool resolveStaticQ(const SceneView& s, V2 pOld, V2& np, V2& nv, float rad, float wall[3], int lane) {
+    if (wallFree(s, np)) {
+        wall[0] = wall[1] = wall[2] = 0.0f;
+        return true;
+    }
+    NoRec r;
+    bool grounded = !solidAt(s, np);
+    if (!grounded && sweepBack(s, pOld, np, nv, r)) grounded = !solidAt(s, np);
+    const bool walls = clearWallsQ<Q>(s, np, nv, rad, wall, lane);
+    if (walls && grounded) {
+        float w2[3];
+        clearWallsQ<Q>(s, np, nv, rad, w2, lane);
+        if (w2[0] != 0.0f || w2[1] != 0.0f) { wall[0] = w2[0]; wall[1] = w2[1]; wall[2] = w2[2]; }
+    }
+    return grounded;
+}
+
+template <int Q>
+__device__ inline void wallPlaneProbeQ(const SceneView& s, V2 np, float rad, float wall[3], int lane) {
+    wall[0] = wall[1] = wall[2] = 0.0f;
+    const float reach = PLANE_PROBE_RADII * rad;
+    const float fc = mapSdf(s, np);
+    Deep d1, d2;
+    deepestFeelers<Q>(s, np, reach, lane, d1, d2);
+    if (d1.i < 0 || fc <= 0.0f) return;
+    V2 away;
+    const float c = wallContact<false>(s, np, fc, V2{ FEELER_DIRS[d1.i].x, FEELER_DIRS[d1.i].y }, d1.v, reach, WALL_RECAST_TOL_RADII * rad, away);
+    if (c < NO_CONTACT) {
+        wall[0] = away.x; wall[1] = away.y;
+        const float m = c - rad;
+        wall[2] = m > 0.0f ? m : 0.0f;
+    }
+}
 __device__ inline void wallPlaneProbe(const SceneView& s, V2 np, float rad, float wall[3]) {
     NoRec r;
     wallPlaneProbe(s, np, rad, wall, r);

@@ -21,7 +21,7 @@ constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projecti
 constexpr int PROJ_PRUNE_EVERY = 8; // projection iterations between candidate prunings (0: never)
 constexpr int PROJ_LANES = 1;       // lanes per slot in the projection iterations that do not prune (autotuned)
 constexpr int PROJ_COLLECT_LANES = 4;   // lanes per slot in the first projection iteration (ring scan + candidates)
-constexpr int PRED_LANES = 4;           // lanes per slot in predict (its perception ring)
+constexpr int PRED_LANES = 4;           // lanes per slot in predict (its perception ring and wall contacts)
 constexpr int ADJ_GRAPH_MAX_STEPS = 32; // adjoint segments up to this long replay and run backwards as captured graphs
 constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;

@@ -410,7 +410,8 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_goal_drive(int n, const uint32_t
 // Q > 1 (not REC): the slot's Q adjacent lanes share the perception ring. Lane l takes the row's slots
 // k0 + u Q + l, so (u, l) order is the row's slot order: every lane adds the separation terms in that order through
 // shuffles within the slot's lanes (the single lane's sum bit for bit); the headway gap is a minimum over the
-// agents ahead, the same in any order, combined across the lanes. Lane 0 then finishes the slot alone.
+// agents ahead, the same in any order, combined across the lanes. Every lane then carries the same state through
+// the kinematics; the wall contacts split their feelers and contacts over the lanes (resolveStaticQ); lane 0 writes.
 template <bool B, bool REC = false, int Q = 1>
 __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchView& bt, const ModelParams& mp, const StepBuffers& b,
                                    const ProjBuffers& out, const TapeStep& tp = TapeStep{}, int lane = 0) {
@@ -528,7 +529,6 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
             const float o = __shfl_xor(sAhead, d, Q);
             sAhead = o < sAhead ? o : sAhead;
         }
-        if (lane != 0) return;
     }
     const float stair = stairFactor(s, p);
     const float vFree = stair * b.v0[agent];
@@ -594,6 +594,10 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
         rec->plane_nx = pl.n.x; rec->plane_ny = pl.n.y;
         rec->pc_ux = pl.c.u.x; rec->pc_uy = pl.c.u.y; rec->pc_gx = pl.c.g.x; rec->pc_gy = pl.c.g.y;
         rec->pc_x0 = pl.c.x0.x; rec->pc_y0 = pl.c.x0.y; rec->pc_t0 = pl.c.t0;
+    } else if constexpr (Q > 1) {   // every lane holds the same state; the wall contacts share the lanes' work
+        grounded = resolveStaticQ<Q>(s, p, np, nv, ri, wall, lane);
+        if (wall[0] == 0.0f && wall[1] == 0.0f && !wallFree(s, np)) wallPlaneProbeQ<Q>(s, np, ri, wall, lane);
+        if (lane != 0) return;
     } else {
         grounded = resolveStatic(s, p, np, nv, ri, wall);
         if (wall[0] == 0.0f && wall[1] == 0.0f && !wallFree(s, np)) wallPlaneProbe(s, np, ri, wall);

@@ -154,13 +154,14 @@ def test_graph_and_eager_identical():
 @pytest.mark.parametrize("layout,active,total,steps", [(sh.LAYOUT_ROOM, 4096, 4096, 120), (sh.LAYOUT_BOWL, 63424, 65536, 12)])
 def test_candidate_pruning_is_exact(layout, active, total, steps):
     """Pruning the projection's contact candidates (every 1, 4 or 8 iterations), 2, 4 or 8 lanes per slot in the
-    list iterations, in the first (collecting) iteration and in predict give the single-lane, unpruned run's bits."""
+    list iterations, in the first (collecting) iteration and in predict (its ring and its wall contacts) give the
+    single-lane, unpruned run's bits (evacuation records included)."""
     scene = sh.Scene.layout(layout)
     a = sh.place_agents(layout, active, total, 8)
     runs = []
-    for every, lanes, collect, pred in ((0, 1, 1, 1), (1, 1, 1, 1), (4, 1, 1, 1), (8, 1, 1, 1), (0, 2, 1, 1), (0, 4, 1, 1),
-                                        (8, 4, 1, 1), (0, 8, 1, 1), (0, 1, 2, 1), (0, 1, 4, 1), (0, 1, 8, 1), (8, 1, 4, 1),
-                                        (0, 1, 1, 2), (0, 1, 1, 4), (0, 1, 1, 8), (8, 1, 4, 4)):
+    for cfg in ((0, 1, 1, 1), (1, 1, 1, 1), (4, 1, 1, 1), (8, 1, 1, 1), (0, 2, 1, 1), (0, 4, 1, 1), (8, 4, 1, 1), (0, 8, 1, 1),
+                (0, 1, 2, 1), (0, 1, 4, 1), (0, 1, 8, 1), (8, 1, 4, 1), (0, 1, 1, 2), (0, 1, 1, 4), (0, 1, 1, 8), (8, 1, 4, 4)):
+        every, lanes, collect, pred = cfg
         s = sh.Sim(scene, total)
         s.set_projection_prune(every)
         s.set_projection_lanes(lanes)
@@ -168,11 +169,14 @@ def test_candidate_pruning_is_exact(layout, active, total, steps):
         s.set_predict_lanes(pred)
         s.upload(a)
         s.step(steps)
-        runs.append(((every, lanes, collect, pred), s.state(), s.assign()))
-    for cfg, st, asg in runs[1:]:
+        st, ev = s.state(), s.evac()
+        runs.append((cfg, st, s.assign(), ev))
+    for cfg, st, asg, ev in runs[1:]:
         for f in FIELDS:
-            assert_bits(st[f], runs[0][1][f], f"prune every {cfg[0]}, {cfg[1]} lanes, {cfg[2]} collecting, {cfg[3]} predict lanes: {f}")
+            assert_bits(st[f], runs[0][1][f], f"prune every {cfg[0]}, lanes: {cfg[1]} list, {cfg[2]} collecting, {cfg[3]} predict: {f}")
         assert np.array_equal(asg, runs[0][2])
+        for u, v in zip(ev, runs[0][3]):
+            assert np.array_equal(u, v), f"evacuations, {cfg}"
 
 
 def test_collecting_lanes_write_the_same_lists():
