@@ -255,7 +255,7 @@ def main():
     b_step = step_bytes(TOTAL, cells, mean_active)
     step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
     print(f"# passes ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in passes.items()) +
-          f"; one projection launch {proj_launch_ms * 1000:.2f} us", file=sys.stderr, flush=True)
+          f"; one projection iteration after the first {proj_launch_ms * 1000:.2f} us (event pair per launch)", file=sys.stderr, flush=True)
 
     copy_gbs = None
     if args.copy_probe:   # a plain copy of 1 GiB: the measured HBM copy bandwidth (read + write bytes / time)
@@ -300,7 +300,7 @@ def main():
                    "agents_active": ACTIVE, "agents_total": TOTAL, "hash_cells": cells, "proj_iterations": 32,
                    "exits": scene.info.E, "parallelism": "single GPU", "physics_shape": shape, "projection_lanes": lanes,
                    "autotune_ms_per_step": tune_ms},
-        "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration after the first, over the candidate lists)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,
                      "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,

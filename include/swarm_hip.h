@@ -153,7 +153,8 @@ int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, u
 int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
 /* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:142-143): per-pass HIP events (steps then run
  * without graph capture). ms8: mean ms per step of grid passes (hash, sort, cells, gather), exit choice,
- * predict, projection iterations, final, total; the mean ms of one projection launch; the steps timed */
+ * predict, projection iterations, final, total; the mean ms of one projection iteration after the first (an
+ * event pair around each launch); the steps timed */
 int swarm_set_timing(swarm_ctx* c, int on);
 int swarm_read_timings(swarm_ctx* c, double* ms8);
 /* nsteps steps between two HIP events on the context's stream (synchronises); *ms: their elapsed time */

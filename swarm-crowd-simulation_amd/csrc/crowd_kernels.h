@@ -184,8 +184,10 @@ void launchSortKeys(hipStream_t st, const StepLaunch& L, AgentHalf rd, const Gri
 void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, const float* driveX,
                   const float* driveY, uint32_t* ctrl);
 // stamps (optional, 2 events): recorded after predict and after the projection iterations before the final pass
+// stamps: events after predict and after the projection; iterStamps: a pair around each projection iteration after
+// the first (2 (P - 2) events)
 void launchPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuffers pa, ProjBuffers pb, AgentHalf rd, AgentHalf wr,
-                   EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps = nullptr);
+                   EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps = nullptr, hipEvent_t* iterStamps = nullptr);
 // The physics passes of the adjoint's taped forward: positions of every iteration into q ((P + 1) x n: q[0]
 // predict's output, q[it] iteration it's), the accumulator through acc0 / acc1, the records into tp
 void launchPhysicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, float2* q, float* acc0, float* acc1, AgentHalf rd,

@@ -24,6 +24,8 @@
 // Data is SoA FP32. The kernels are latency-bound gathers and integer atomics: no MFMA.
 #include "crowd_kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace swarm {
 
 // base[key] += the number of lanes of this wave that pass 'valid' with this key: one atomic per distinct key of
@@ -1298,7 +1300,7 @@ void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const Choic
 
 template <bool B>
 static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuffers pa, ProjBuffers pb, AgentHalf rd, AgentHalf wr,
-                          EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
+                          EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps, hipEvent_t* iterStamps) {
     const int n = L.n;
     const int bs = L.physBlock > 0 ? L.physBlock : PHYS_BLOCK;
     const int full = blocks(n, bs);
@@ -1319,6 +1321,14 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
     for (int it = 1; it < P; ++it) {
         // pruning at iterations 1 + k pruneEvery (k >= 1); the final pass is iteration P
         const int prune = L.pruneEvery > 0 && it > 1 && (it - 1) % L.pruneEvery == 0 ? P - it + 1 : 0;
+        if (iterStamps && it > 1 && (prune > 0 || L.projLanes <= 1)) {
+            // timed: the dispatch itself records the pair of events, so they bracket the kernel's execution
+            hipExtLaunchKernelGGL(k_project<B, false>, dim3(g), dim3(bs), 0, st, iterStamps[2 * (it - 2)], iterStamps[2 * (it - 2) + 1], 0u,
+                                  n, L.s, L.bt, b, in, out, P, 0, TapeStep{}, 0, prune);
+            const ProjBuffers t = in; in = out; out = t;
+            continue;
+        }
+        if (iterStamps && it > 1) (void)hipEventRecord(iterStamps[2 * (it - 2)], st);
         if (it == 1 && L.collectLanes > 1) {
             const int q = L.collectLanes;
             const int full = blocks(n * q, bs);
@@ -1336,6 +1346,7 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
             else if (q == 4) k_project_lanes<B, 4><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
             else k_project_lanes<B, 8><<<gq, bs, 0, st>>>(n, L.s, L.bt, b, in, out, P);
         }
+        if (iterStamps && it > 1) (void)hipEventRecord(iterStamps[2 * (it - 2) + 1], st);
         const ProjBuffers t = in; in = out; out = t;
     }
     if (stamps) (void)hipEventRecord(stamps[1], st);
@@ -1343,9 +1354,9 @@ static void physicsPasses(hipStream_t st, const StepLaunch& L, StepBuffers& b, P
 }
 
 void launchPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuffers pa, ProjBuffers pb, AgentHalf rd, AgentHalf wr,
-                   EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
-    if (L.batched) physicsPasses<true>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps);
-    else physicsPasses<false>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps);
+                   EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps, hipEvent_t* iterStamps) {
+    if (L.batched) physicsPasses<true>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps, iterStamps);
+    else physicsPasses<false>(st, L, b, pa, pb, rd, wr, ev, ctrl, stamps, iterStamps);
 }
 
 template <bool B>

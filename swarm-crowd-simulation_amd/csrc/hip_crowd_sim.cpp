@@ -418,8 +418,8 @@ void HipCrowdSim::enqueueStep(int parity) {
     b.avx = rd.vx;
     b.avy = rd.vy;
     std::vector<hipEvent_t> ev;
-    if (timing_) {   // stamps: start, sorted, choice, predict, projection, final
-        ev.resize(6);
+    if (timing_) {   // stamps: start, sorted, choice, predict, projection, final; then a pair per later iteration
+        ev.resize(6 + 2 * size_t(std::max(0, mp_.projIters - 2)));
         for (hipEvent_t& e : ev) SWARM_HIP_CHECK(hipEventCreate(&e));
         SWARM_HIP_CHECK(hipEventRecord(ev[0], stream_));
     }
@@ -428,7 +428,8 @@ void HipCrowdSim::enqueueStep(int parity) {
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[1], stream_));
     launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
-    launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
+    launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr,
+                  timing_ && mp_.projIters > 2 ? &ev[6] : nullptr);
     if (lossOn_) launchOverlap(stream_, L, b, wr, overlapWeight_, lossAcc_, nullptr);
     if (metricsOn_) {   // the step's input half rd, its output wr; in this step's slot order
         launchMetricsDeposit(stream_, n_, mgrid_, b.sid, rd.x, rd.y, rd.w, wr.x, wr.y, wr.w, wr.vx, wr.vy, scenes_[0].view.E,
@@ -444,13 +445,19 @@ void HipCrowdSim::enqueueStep(int parity) {
 
 void HipCrowdSim::collectTimings() {
     for (std::vector<hipEvent_t>& ev : pendingStamps_) {
-        SWARM_HIP_CHECK(hipEventSynchronize(ev.back()));
+        SWARM_HIP_CHECK(hipEventSynchronize(ev[5]));   // the final pass's stamp, recorded last
         float ms[5];
         for (int k = 0; k < 5; ++k) SWARM_HIP_CHECK(hipEventElapsedTime(&ms[k], ev[size_t(k)], ev[size_t(k) + 1]));
         float total = 0.0f;
         SWARM_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[5]));
         for (int k = 0; k < 5; ++k) timeSum_[k] += ms[k];   // grid passes, choice, predict, projection, final
         timeSum_[5] += total;
+        for (size_t k = 6; k + 1 < ev.size(); k += 2) {       // each later projection iteration, on its own
+            float it = 0.0f;
+            SWARM_HIP_CHECK(hipEventElapsedTime(&it, ev[k], ev[k + 1]));
+            timeSum_[6] += it;
+            timeSum_[7] += 1.0;
+        }
         ++timedSteps_;
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
@@ -470,7 +477,7 @@ int HipCrowdSim::readTimings(double* ms8) {
     collectTimings();
     const double n = timedSteps_ > 0 ? double(timedSteps_) : 1.0;
     for (int k = 0; k < 6; ++k) ms8[k] = timeSum_[k] / n;
-    ms8[6] = mp_.projIters > 1 ? ms8[3] / double(mp_.projIters - 1) : 0.0;
+    ms8[6] = timeSum_[7] > 0.0 ? timeSum_[6] / timeSum_[7] : 0.0;   // the mean of the later iterations' event pairs
     ms8[7] = double(timedSteps_);
     return timedSteps_;
 }
