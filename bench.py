@@ -81,6 +81,7 @@ def batch_bench(steps, warmup, loop_steps, use_graph=True):
     b.upload(agents)
     tune_ms = b.autotune(5)
     shape = b.physics_shape()
+    lane_cfg = dict(zip(("list", "collect", "predict"), b.lanes()))
     b.step(warmup)
     b.sync()
     _, _, s0 = b.control()
@@ -111,7 +112,7 @@ def batch_bench(steps, warmup, loop_steps, use_graph=True):
         p.close()
     return {"value": batched, "unit": "agent-steps/s", "ms_per_step": ms / steps, "steps": steps,
             "scenarios": int(len(totals)), "agents_total": int(totals.sum()), "agents_active": int(actives.sum()),
-            "physics_shape": list(shape), "autotune_ms_per_step": tune_ms,
+            "physics_shape": list(shape), "lanes_per_slot": lane_cfg, "autotune_ms_per_step": tune_ms,
             "step_algorithmic_bytes": b_step, "step_frac": b_step / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "looped_plain": {"value": looped, "unit": "agent-steps/s", "steps": loop_steps,
                              "ms_per_round": 1000.0 * dt / loop_steps},
@@ -214,8 +215,9 @@ def main():
           f"SDF {scene.info.sdf_nx}x{scene.info.sdf_ny}, nav {scene.info.nav_res_x}x{scene.info.nav_res_y} x {scene.info.E} exits",
           file=sys.stderr, flush=True)
 
-    tune_ms = sim.autotune(5)   # launch shape and projection lanes (the state is restored)
+    tune_ms = sim.autotune(5)   # launch shape and lanes per slot (the state is restored)
     shape, lanes = list(sim.physics_shape()), sim.projection_lanes()
+    lane_cfg = dict(zip(("list", "collect", "predict"), sim.lanes()))
     sim.step(args.warmup)
     sim.sync()
     _, _, sum0 = sim.control()
@@ -241,13 +243,16 @@ def main():
     print(f"# with the safety metrics: {metrics['ms_per_step']:.4f} ms/step ({100 * metrics['overhead_vs_plain']:+.1f}%)",
           file=sys.stderr, flush=True)
 
-    # per-pass timing (eager launches with events between passes) for the dominant kernel's launch duration
+    # per-pass timing (eager launches with events between passes), then the dominant kernel's launch duration (a pair
+    # of events recorded by each later projection iteration's dispatch, a separate run: they slow the pass totals)
     sim.set_timing(True)
     sim.step(max(1, args.profile_steps))
     t = sim.timings()
+    sim.set_timing(True, launches=True)
+    sim.step(max(1, args.profile_steps))
+    proj_launch_ms = sim.timings()[6]
     sim.set_timing(False)
     passes = dict(grid=t[0], choice=t[1], predict=t[2], projection=t[3], final=t[4], total_eager=t[5])
-    proj_launch_ms = t[6]
     mean_active = agent_steps / args.steps
     proj_bytes = PROJ_BYTES_PER_SLOT * mean_active
     achieved = proj_bytes / (proj_launch_ms * 1e-3) / 1e9
@@ -299,6 +304,7 @@ def main():
         "config": {"workload": "plain step, 63,424 agents on the 200 x 150 m seated bowl, sort size 65,536",
                    "agents_active": ACTIVE, "agents_total": TOTAL, "hash_cells": cells, "proj_iterations": 32,
                    "exits": scene.info.E, "parallelism": "single GPU", "physics_shape": shape, "projection_lanes": lanes,
+                   "lanes_per_slot": lane_cfg,
                    "autotune_ms_per_step": tune_ms},
         "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration after the first, over the candidate lists)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

@@ -153,8 +153,9 @@ int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, u
 int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
 /* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:142-143): per-pass HIP events (steps then run
  * without graph capture). ms8: mean ms per step of grid passes (hash, sort, cells, gather), exit choice,
- * predict, projection iterations, final, total; the mean ms of one projection iteration after the first (an
- * event pair around each launch); the steps timed */
+ * predict, projection iterations, final, total; the mean ms of one projection iteration after the first (on = 2
+ * only: a pair of events recorded by each of those dispatches, which slows the pass totals); the steps timed.
+ * Switching the mode starts a new measurement. */
 int swarm_set_timing(swarm_ctx* c, int on);
 int swarm_read_timings(swarm_ctx* c, double* ms8);
 /* nsteps steps between two HIP events on the context's stream (synchronises); *ms: their elapsed time */
@@ -207,9 +208,12 @@ int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float*
  * feelers and two contact measurements over the lanes. Results are the same bit for bit. No counterpart in the
  * reference. */
 int swarm_set_predict_lanes(swarm_ctx* c, int lanes);
-/* Times each candidate physics launch shape over 'steps' steps from the current state, then the projection's lanes
- * per slot at the fastest shape; keeps the fastest and restores the state; *best_ms: its mean ms per step. No
- * counterpart in the reference. */
+/* The lanes per slot in force: the projection's list iterations, its first (collecting) iteration, predict. */
+int swarm_get_lanes(swarm_ctx* c, int* list_lanes, int* collect_lanes, int* predict_lanes);
+/* Times each candidate physics launch shape over 'steps' steps from the current state, then at the fastest shape
+ * the lanes per slot of the projection's list iterations (1, 2, 4, 8), of predict and of the collecting iteration
+ * (1, 2, 4 each); keeps the fastest and restores the state; *best_ms: its mean ms per step. No counterpart in the
+ * reference. */
 int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);
 
 /* ---- the safety metrics (class CrowdMetrics, include/swarm/metrics.hpp:181-227; plain contexts) ----

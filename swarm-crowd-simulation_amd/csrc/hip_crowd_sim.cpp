@@ -390,7 +390,7 @@ double HipCrowdSim::autotune(int steps) {
             const double ms = trial();
             if (best == 0.0 || ms < best) { best = ms; bestShape = sh; }
         }
-        // then, at the fastest shape, the projection's lanes per slot
+        // then, at the fastest shape, the projection's lanes per slot, predict's and the collecting iteration's
         setPhysicsShape(bestShape.first, bestShape.second);
         for (int lanes : { 1, 2, 4, 8 }) {
             if (lanes == bestLanes) continue;
@@ -399,6 +399,22 @@ double HipCrowdSim::autotune(int steps) {
             if (ms < best) { best = ms; bestLanes = lanes; }
         }
         setProjectionLanes(bestLanes);
+        int bestPredict = predictLanes_;
+        for (int lanes : { 1, 2, 4 }) {
+            if (lanes == bestPredict) continue;
+            setPredictLanes(lanes);
+            const double ms = trial();
+            if (ms < best) { best = ms; bestPredict = lanes; }
+        }
+        setPredictLanes(bestPredict);
+        int bestCollect = collectLanes_;
+        for (int lanes : { 1, 2, 4 }) {
+            if (lanes == bestCollect) continue;
+            setCollectLanes(lanes);
+            const double ms = trial();
+            if (ms < best) { best = ms; bestCollect = lanes; }
+        }
+        setCollectLanes(bestCollect);
     } catch (...) {
         (void)hipStreamSynchronize(stream_);
         (void)hipFree(snap);
@@ -419,7 +435,7 @@ void HipCrowdSim::enqueueStep(int parity) {
     b.avy = rd.vy;
     std::vector<hipEvent_t> ev;
     if (timing_) {   // stamps: start, sorted, choice, predict, projection, final; then a pair per later iteration
-        ev.resize(6 + 2 * size_t(std::max(0, mp_.projIters - 2)));
+        ev.resize(6 + (timeLaunches_ ? 2 * size_t(std::max(0, mp_.projIters - 2)) : 0));
         for (hipEvent_t& e : ev) SWARM_HIP_CHECK(hipEventCreate(&e));
         SWARM_HIP_CHECK(hipEventRecord(ev[0], stream_));
     }
@@ -429,7 +445,7 @@ void HipCrowdSim::enqueueStep(int parity) {
     launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
     launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr,
-                  timing_ && mp_.projIters > 2 ? &ev[6] : nullptr);
+                  timing_ && timeLaunches_ && mp_.projIters > 2 ? &ev[6] : nullptr);
     if (lossOn_) launchOverlap(stream_, L, b, wr, overlapWeight_, lossAcc_, nullptr);
     if (metricsOn_) {   // the step's input half rd, its output wr; in this step's slot order
         launchMetricsDeposit(stream_, n_, mgrid_, b.sid, rd.x, rd.y, rd.w, wr.x, wr.y, wr.w, wr.vx, wr.vy, scenes_[0].view.E,
@@ -464,9 +480,10 @@ void HipCrowdSim::collectTimings() {
     pendingStamps_.clear();
 }
 
-void HipCrowdSim::setTimingEnabled(bool on) {
-    if (!on && timing_) collectTimings();
+void HipCrowdSim::setTimingEnabled(bool on, bool launches) {
+    if (timing_) collectTimings();
     timing_ = on;
+    timeLaunches_ = on && launches;
     if (on) {
         for (double& t : timeSum_) t = 0.0;
         timedSteps_ = 0;

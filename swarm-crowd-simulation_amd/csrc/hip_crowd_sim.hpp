@@ -78,6 +78,7 @@ public:
     int projectionLanes() const { return projLanes_; }
     void setCollectLanes(int lanes);
     void setPredictLanes(int lanes);
+    int predictLanes() const { return predictLanes_; }
     int collectLanes() const { return collectLanes_; }
     // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, then the
     // projection's lanes per slot at the fastest shape; keeps the fastest and puts the state back as it was (a
@@ -88,7 +89,7 @@ public:
     // step runs eagerly with HIP events between its passes on the context's stream. readTimings: the mean ms per
     // step of the grid passes (hash, sort, cell ranges, gather), exit choice, predict, projection (all
     // PROJ_ITERATIONS - 1 launches), final, total, and the mean ms of one projection launch; returns the steps timed
-    void setTimingEnabled(bool on);
+    void setTimingEnabled(bool on, bool launches = false);   // launches: also time each later projection iteration
     int readTimings(double* ms8);
 
     void readState(float* x, float* y, float* vx, float* vy, float* speed, float* w);
@@ -216,6 +217,7 @@ private:
     float* adjG0_ = nullptr;             // adj_.gx as allocated (the ping-pong's reference half)
     // timing: events per stamp (start, sorted, choice, predict, projection, final) per recorded step
     bool timing_ = false;
+    bool timeLaunches_ = false;          // timing mode 2: an event pair in each later projection iteration's dispatch
     std::vector<std::vector<hipEvent_t>> pendingStamps_;
     double timeSum_[8] = {};
     int timedSteps_ = 0;

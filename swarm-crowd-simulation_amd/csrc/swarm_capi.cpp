@@ -206,7 +206,10 @@ int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_
     CTX_CHECK(c);
     return guarded([&] { c->sim->readControl(step, n_active, active_sum); });
 }
-int swarm_set_timing(swarm_ctx* c, int on) { CTX_CHECK(c); return guarded([&] { c->sim->setTimingEnabled(on != 0); }); }
+int swarm_set_timing(swarm_ctx* c, int on) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setTimingEnabled(on != 0, on == 2); });
+}
 int swarm_read_timings(swarm_ctx* c, double* ms8) {
     CTX_CHECK(c);
     if (!ms8) { g_err = "null argument"; return SWARM_EINVAL; }
@@ -298,6 +301,13 @@ int swarm_set_predict_lanes(swarm_ctx* c, int lanes) {
 int swarm_set_projection_collect_lanes(swarm_ctx* c, int lanes) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setCollectLanes(lanes); });
+}
+int swarm_get_lanes(swarm_ctx* c, int* list_lanes, int* collect_lanes, int* predict_lanes) {
+    CTX_CHECK(c);
+    if (list_lanes) *list_lanes = c->sim->projectionLanes();
+    if (collect_lanes) *collect_lanes = c->sim->collectLanes();
+    if (predict_lanes) *predict_lanes = c->sim->predictLanes();
+    return SWARM_OK;
 }
 int swarm_get_projection_lanes(swarm_ctx* c, int* lanes) {
     CTX_CHECK(c);

@@ -309,8 +309,9 @@ class Sim:
         check(lib().swarm_read_control(self.h, C.byref(s), C.byref(a), C.byref(t)))
         return s.value, a.value, t.value
 
-    def set_timing(self, on):
-        check(lib().swarm_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on, launches=False):
+        """Per-pass timing; launches=True also times each later projection iteration by its dispatch's events."""
+        check(lib().swarm_set_timing(self.h, (2 if launches else 1) if on else 0))
 
     def timings(self):
         """Mean ms per step: grid passes, exit choice, predict, projection, final, total; one projection launch; steps."""
@@ -352,6 +353,14 @@ class Sim:
         """Lanes per slot in the projection's first iteration (ring scan + candidates); bit-identical results."""
         lib().swarm_set_projection_collect_lanes.argtypes = [C.c_void_p, C.c_int]
         check(lib().swarm_set_projection_collect_lanes(self.h, lanes))
+
+    def lanes(self):
+        """(list iterations, collecting iteration, predict) lanes per slot in force."""
+        L = lib()
+        L.swarm_get_lanes.argtypes = [C.c_void_p] * 4
+        a, b_, c = C.c_int(), C.c_int(), C.c_int()
+        check(L.swarm_get_lanes(self.h, C.byref(a), C.byref(b_), C.byref(c)))
+        return a.value, b_.value, c.value
 
     def projection_lanes(self):
         v = C.c_int()
