@@ -120,14 +120,16 @@ def batch_bench(steps, warmup, loop_steps, use_graph=True):
 
 
 def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
-    """The differentiable configuration: 4,096 agents in the 40 x 30 m room, K steps. The forward is K plain steps
-    (graph replays); the adjoint is the adjoint's forward (the same steps with the loss terms and checkpoints) plus
-    its backward (each segment replayed with the tape, then the adjoint kernels)."""
+    """The differentiable configuration: 4,096 agents in the 40 x 30 m room, K steps, checkpoints every 20. The
+    forward is K plain steps (graph replays); the adjoint is the adjoint's forward (the same steps with the loss
+    terms) plus its backward. By default the whole run's tape fits the tape budget, so the forward tapes it and
+    nothing is recomputed; 'recompute' is the same run with a zero budget (each segment but the last replayed
+    from its checkpoint with the tape, then the adjoint kernels)."""
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
     a = sh.place_agents(sh.LAYOUT_ROOM, n, n, SEED)
     sim = sh.Sim(scene, n, use_graph=use_graph)
     sim.upload(a)
-    tune_ms = sim.autotune(5)   # launch shape and projection lanes for this crowd (the state is restored)
+    tune_ms = sim.autotune(5)   # launch shape and lanes per slot for this crowd (the state is restored)
     sim.step(2)
     sim.upload(a)
     sim.sync()
@@ -136,16 +138,21 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     sim.sync()
     fwd = time.perf_counter() - t0
     _, _, steps_sum = sim.control()
-    sim.upload(a)
-    sim.adjoint_forward(K, every, 1.0, 0.1)   # warm-up (allocations)
-    sim.adjoint_backward()
-    sim.upload(a)
-    t0 = time.perf_counter()
-    loss = sim.adjoint_forward(K, every, 1.0, 0.1)
-    t1 = time.perf_counter()
-    gv0, g = sim.adjoint_backward()
-    t2 = time.perf_counter()
-    total = t2 - t0
+
+    def timed_adjoint():
+        sim.upload(a)
+        sim.adjoint_forward(K, every, 1.0, 0.1)   # warm-up (allocations, graphs)
+        sim.adjoint_backward()
+        sim.upload(a)
+        t0 = time.perf_counter()
+        loss = sim.adjoint_forward(K, every, 1.0, 0.1)
+        t1 = time.perf_counter()
+        gv0, g = sim.adjoint_backward()
+        t2 = time.perf_counter()
+        return loss, gv0, g, t1 - t0, t2 - t1, sim.adjoint_segments(), sim.tape_bytes()
+
+    loss, gv0, g, tf, tb, segs, mem = timed_adjoint()
+    total = tf + tb
     finite = bool(np.isfinite(gv0).all() and np.isfinite(g).all())
     # the same K steps with the state adjoint cleared every 25 steps (the truncated estimator long horizons use)
     sim.set_adjoint_terms(window=25)
@@ -157,30 +164,21 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     sim.set_adjoint_terms(window=0)
     windowed = {"window": 25, "ms": 1000.0 * (t4 - t3), "gradient_finite": bool(np.isfinite(wv0).all() and np.isfinite(wg).all()),
                 "grad_v0_norm": float(np.linalg.norm(wv0.astype(np.float64)))}
-    # the whole run taped in its forward (one segment of K steps, no recompute): the tape's memory for the speed
-    sim.upload(a)
-    sim.adjoint_forward(K, K, 1.0, 0.1)   # warm-up (the K-step tape and its graphs)
-    sim.adjoint_backward()
-    sim.upload(a)
-    t5 = time.perf_counter()
-    loss_w = sim.adjoint_forward(K, K, 1.0, 0.1)
-    t6 = time.perf_counter()
-    tv0, tg = sim.adjoint_backward()
-    t7 = time.perf_counter()
-    finite_both = np.isfinite(tv0) & np.isfinite(gv0)
-    rel = float(np.max(np.abs(tv0[finite_both].astype(np.float64) - gv0[finite_both]) / np.maximum(np.abs(gv0[finite_both]), 1e-30))) \
-        if finite_both.any() else None
-    whole = {"checkpoint_every": K, "adjoint_forward_ms": 1000.0 * (t6 - t5), "adjoint_backward_ms": 1000.0 * (t7 - t6),
-             "cost_vs_forward": (t7 - t5) / fwd, "adjoint_memory_mb": sim.tape_bytes() / 1e6, "loss_equal": loss_w == loss,
-             "grad_v0_max_rel_diff_vs_checkpointed": rel}
-    return {"K": K, "checkpoint_every": every, "agents": n, "physics_shape": list(sim.physics_shape()),
-            "projection_lanes": sim.projection_lanes(), "autotune_ms_per_step": tune_ms, "forward_ms": 1000.0 * fwd,
-            "adjoint_forward_ms": 1000.0 * (t1 - t0), "adjoint_backward_ms": 1000.0 * (t2 - t1),
+    # checkpoints every 20 steps and recompute, whatever fits
+    sim.set_tape_budget(0)
+    loss_r, rv0, _, rf, rb, rsegs, _ = timed_adjoint()
+    same = bool(np.array_equal(rv0.view(np.uint32), gv0.view(np.uint32)) and loss_r == loss)
+    recompute = {"segments": rsegs, "adjoint_forward_ms": 1000.0 * rf, "adjoint_backward_ms": 1000.0 * rb,
+                 "cost_vs_forward": (rf + rb) / fwd, "same_loss_and_grad_v0_bits": same}
+    return {"K": K, "checkpoint_every": every, "segments": segs, "adjoint_memory_mb": mem / 1e6, "agents": n,
+            "physics_shape": list(sim.physics_shape()), "lanes_per_slot": dict(zip(("list", "collect", "predict"), sim.lanes())),
+            "autotune_ms_per_step": tune_ms, "forward_ms": 1000.0 * fwd,
+            "adjoint_forward_ms": 1000.0 * tf, "adjoint_backward_ms": 1000.0 * tb,
             "cost_vs_forward": total / fwd, "agent_steps_per_s": steps_sum / total, "loss": loss,
             "gradient_finite": finite,
             "gradient_note": None if finite else "the exact pathwise gradient of a 200-step loss overflows FP32 in this crowd "
                                                  "(its sensitivity grows about exp(0.46 K)); the timing is the measured quantity",
-            "windowed": windowed, "whole_tape": whole}
+            "windowed": windowed, "recompute": recompute}
 
 
 def main():
@@ -281,7 +279,8 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_bench(use_graph=not args.no_graph)
         print(f"# adjoint K={adjoint['K']}: forward {adjoint['forward_ms']:.1f} ms, adjoint {adjoint['adjoint_forward_ms']:.1f} + "
-              f"{adjoint['adjoint_backward_ms']:.1f} ms ({adjoint['cost_vs_forward']:.2f}x)", file=sys.stderr, flush=True)
+              f"{adjoint['adjoint_backward_ms']:.1f} ms ({adjoint['cost_vs_forward']:.2f}x, {adjoint['segments']} segment(s)); "
+              f"with recompute {adjoint['recompute']['cost_vs_forward']:.2f}x", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -97,14 +97,19 @@ typedef struct {
 
 /* Runs K steps from the uploaded state (the plain step, hipGraph replays), keeps the state every
  * checkpoint_every steps, and returns the loss. radius_scale multiplies the uploaded radii for this run. The last
- * segment runs taped, so the backward starts without recomputing it; checkpoint_every >= K tapes the whole run in
- * the forward (no recompute at all, the tape of K steps in device memory: swarm_adjoint_memory). */
+ * segment runs taped, so the backward starts without recomputing it; a run whose whole tape fits the tape budget
+ * (swarm_adjoint_set_tape_budget), or checkpoint_every >= K, is taped whole in the forward and never recomputed. */
 int swarm_adjoint_forward(swarm_ctx* c, int K, int checkpoint_every, float radius_scale, float overlap_weight, double* loss);
 /* After swarm_adjoint_forward: recomputes each earlier segment from its checkpoint with the tape and runs the
  * adjoint kernels backwards. grad_v0: n (agent order); grad_globals: dL/dtau_s, dL/dradius_scale, dL/dwall_gain. */
 int swarm_adjoint_backward(swarm_ctx* c, float* grad_v0, double* grad_globals);
 /* Device bytes the context's adjoint holds (tape, adjoint buffers, checkpoints). */
 int swarm_adjoint_memory(swarm_ctx* c, size_t* bytes);
+/* A forward whose whole tape (all K steps) fits 'bytes' is taped whole and never recomputed, whatever the
+ * checkpoint interval (default 4 GiB: K = 200 at 4,096 agents takes 1.3 GB); 0: always checkpoint and recompute. */
+int swarm_adjoint_set_tape_budget(swarm_ctx* c, size_t bytes);
+/* The number of segments of the last swarm_adjoint_forward (1: taped whole). */
+int swarm_adjoint_segments(swarm_ctx* c, int* segments);
 /* The tape of step 'step' (0-based) of the last forward (recomputed from its checkpoint), and for step K - 1 also
  * the exit records (exit may be NULL). Any pointer of the view may be NULL. */
 int swarm_adjoint_tape(swarm_ctx* c, int step, const swarm_tape_view* out, swarm_exit_rec* exit);

@@ -818,20 +818,28 @@ void HipCrowdSim::allocAdjoint(int every) {
     adjCap_ = every;
 }
 
+size_t HipCrowdSim::tapeBytesPerStep() const {   // allocAdjoint's tape, per step
+    const size_t n = size_t(n_), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    return n * (15 * 4 + 2 * 4 + 16 + 2 * size_t(NBR_CAP) * 4 + 4 + sizeof(swarm_predict_rec) + sizeof(swarm_final_rec) + P * 8 + 3 * 8 +
+                (P + 1) * 8) + nbs * 4;
+}
+
 double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float overlapWeight) {
     if (K <= 0 || every <= 0) throw std::invalid_argument("adjointForward: K and the checkpoint interval must be positive");
     if (mp_.projIters < 2 || mp_.projIters > SWARM_TAPE_MAX_ITERS)
         throw std::invalid_argument("adjointForward: the adjoint needs 2 to 32 projection iterations");
     if (r0_.size() != size_t(n_)) throw std::invalid_argument("adjointForward: upload a state first");
     if (!(radiusScale > 0.0f)) throw std::invalid_argument("adjointForward: the radius scale must be positive");
-    allocAdjoint(std::min(every, K));
+    // the segment length: the whole run when its tape fits the budget (no recompute), else the checkpoint interval
+    const int segLen = size_t(K) * tapeBytesPerStep() <= tapeBudget_ ? K : std::min(every, K);
+    allocAdjoint(segLen);
     // this run's radii
     std::vector<float> r(r0_.size());
     for (size_t i = 0; i < r.size(); ++i) r[i] = radiusScale * r0_[i];
     SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r.data(), r.size() * 4, hipMemcpyHostToDevice, stream_));
-    if (std::min(every, K) != adjEvery_ || overlapWeight != overlapWeight_) dropAdjointGraphs();
+    if (segLen != adjEvery_ || overlapWeight != overlapWeight_) dropAdjointGraphs();
     adjK_ = K;
-    adjEvery_ = std::min(every, K);
+    adjEvery_ = segLen;
     adjSegs_ = (K + adjEvery_ - 1) / adjEvery_;
     adjScale_ = radiusScale;
     overlapWeight_ = overlapWeight;

@@ -178,6 +178,10 @@ private:
     void recordSegment(int seg);
 public:
     size_t adjointBytes() const { return adjBytes_; }   // device memory the adjoint holds (tape, buffers, checkpoints)
+    // a run whose whole tape fits this many bytes is taped in its forward and never recomputed
+    void setTapeBudget(size_t bytes) { tapeBudget_ = bytes; }
+    size_t tapeBytesPerStep() const;
+    int adjointSegments() const { return adjSegs_; }
 private:
     AdjStep adjStepView(int k) const;
 
@@ -261,6 +265,7 @@ private:
     size_t adjBytes_ = 0;                // their bytes, checkpoints included
     std::vector<void*> tapeBufs_;        // the tape's (freed when a longer segment needs a larger one)
     size_t tapeBytes_ = 0;
+    size_t tapeBudget_ = ADJ_TAPE_BUDGET;
 
     // the safety metrics
     void allocMetrics();

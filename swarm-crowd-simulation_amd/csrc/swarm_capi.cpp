@@ -375,6 +375,16 @@ int swarm_adjoint_forward(swarm_ctx* c, int K, int checkpoint_every, float radiu
     if (!loss) { g_err = "null argument"; return SWARM_EINVAL; }
     return guarded([&] { *loss = c->sim->adjointForward(K, checkpoint_every, radius_scale, overlap_weight); });
 }
+int swarm_adjoint_set_tape_budget(swarm_ctx* c, size_t bytes) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setTapeBudget(bytes); });
+}
+int swarm_adjoint_segments(swarm_ctx* c, int* segments) {
+    CTX_CHECK(c);
+    if (!segments) { g_err = "null argument"; return SWARM_EINVAL; }
+    *segments = c->sim->adjointSegments();
+    return SWARM_OK;
+}
 int swarm_adjoint_memory(swarm_ctx* c, size_t* bytes) {
     CTX_CHECK(c);
     if (!bytes) { g_err = "null argument"; return SWARM_EINVAL; }

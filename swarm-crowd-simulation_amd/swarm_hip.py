@@ -524,6 +524,21 @@ def _sim_adjoint_tape(self, step, tape, exit_recs=None):
     return tape
 
 
+def _sim_set_tape_budget(self, nbytes):
+    """A forward whose whole tape fits nbytes is taped whole (no recompute); 0: always checkpoint and recompute."""
+    L = _adjoint_lib()
+    L.swarm_adjoint_set_tape_budget.argtypes = [C.c_void_p, C.c_size_t]
+    check(L.swarm_adjoint_set_tape_budget(self.h, nbytes))
+
+
+def _sim_adjoint_segments(self):
+    L = _adjoint_lib()
+    L.swarm_adjoint_segments.argtypes = [C.c_void_p, C.c_void_p]
+    k = C.c_int()
+    check(L.swarm_adjoint_segments(self.h, C.byref(k)))
+    return k.value
+
+
 def _sim_tape_bytes(self):
     """Device bytes the adjoint holds (tape, buffers, checkpoints)."""
     L = _adjoint_lib()
@@ -538,6 +553,8 @@ def _sim_set_model(self, tau_s, obstacle_gain=20.0):
 
 
 Sim.tape_bytes = _sim_tape_bytes
+Sim.set_tape_budget = _sim_set_tape_budget
+Sim.adjoint_segments = _sim_adjoint_segments
 Sim.adjoint_forward = _sim_adjoint_forward
 Sim.adjoint_backward = _sim_adjoint_backward
 Sim.adjoint_tape = _sim_adjoint_tape

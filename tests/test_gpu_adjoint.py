@@ -225,8 +225,10 @@ def test_captured_segments_match_eager():
     out = {}
     for graph in (True, False):
         sim = sh.Sim(scene, n, use_graph=graph)
+        sim.set_tape_budget(0)   # checkpoints and recompute, whatever fits
         sim.upload(a)
         loss = sim.adjoint_forward(K, 20, 1.0, 0.1)
+        assert sim.adjoint_segments() == 3
         r1 = sim.adjoint_backward()
         r2 = sim.adjoint_backward()
         sim.set_adjoint_terms(steps, gin, gout, exit_term=False, window=10)
@@ -300,24 +302,27 @@ def test_crossing_terms_match_differences():
 
 def test_taped_forward_segments_match_recompute():
     # the forward tapes its last segment (the backward starts there without recomputing it); a checkpoint interval
-    # >= K tapes the whole run and recomputes nothing: every split gives the same loss and dL/dv0 bit for bit, also
-    # with a truncation window, and the tape read back for a step of the last segment is the recomputed one
+    # >= K, or a run whose whole tape fits the tape budget (the default one here), tapes the whole run and
+    # recomputes nothing: every split gives the same loss and dL/dv0 bit for bit, also with a truncation window
     n, K = 1024, 45
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
     a = sh.place_agents(sh.LAYOUT_ROOM, n, n, 6)
     out = {}
-    for every in (20, 7, 45, 100):
+    for every, budget, segs in ((20, 0, 3), (7, 0, 7), (45, 0, 1), (100, 0, 1), (20, None, 1)):
         for graph in (True, False):
             sim = sh.Sim(scene, n, use_graph=graph)
+            if budget is not None:
+                sim.set_tape_budget(budget)
             sim.upload(a)
             loss = sim.adjoint_forward(K, every, 1.0, 0.1)
+            assert sim.adjoint_segments() == segs, (every, budget)
             r1 = sim.adjoint_backward()
             sim.set_adjoint_terms(window=10)
             r2 = sim.adjoint_backward()
-            out[(every, graph)] = (loss, r1[0], r2[0], sim.tape_bytes())
-    ref = out[(20, False)]
+            out[(every, budget, graph)] = (loss, r1[0], r2[0], sim.tape_bytes())
+    ref = out[(20, 0, False)]
     for key, (loss, g1, g2, nbytes) in out.items():
         assert loss == ref[0], key
-        assert_bits(g1, ref[1], f"dL/dv0, checkpoint every {key[0]}, graph {key[1]}")
-        assert_bits(g2, ref[2], f"dL/dv0 with a window, checkpoint every {key[0]}, graph {key[1]}")
-    assert out[(45, True)][3] > out[(7, True)][3]   # the whole run's tape is the larger
+        assert_bits(g1, ref[1], f"dL/dv0, checkpoint every {key[0]}, budget {key[1]}, graph {key[2]}")
+        assert_bits(g2, ref[2], f"dL/dv0 with a window, checkpoint every {key[0]}, budget {key[1]}, graph {key[2]}")
+    assert out[(45, 0, True)][3] > out[(7, 0, True)][3]   # the whole run's tape is the larger
