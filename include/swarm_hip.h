@@ -202,6 +202,11 @@ int swarm_set_projection_collect_lanes(swarm_ctx* c, int lanes);
  * than 48, the ring scan is used instead), index and contact distance [48 x n], candidate-major (entry j of slot s at
  * j * n + s). Any pointer may be NULL. No counterpart in the reference (its projection rescans the ring). */
 int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float* dist);
+/* The key sort (CpuCrowdSim::SortMode, cpu_crowd_sim.hpp:32-45): 0 (default) the stable counting sort, ascending
+ * (cell, agent) (SortMode::Counting); 1 the reference GPU's bitonic network (bitonic_sort.comp,
+ * bitonic_sort_shared.comp; SortMode::GpuBitonic), whose unstable within-cell order is the GPU reference's own and
+ * changes the order of every neighbour sum. Mode 1 needs a power-of-two agent count (crowd_sim.cpp:99-100). */
+int swarm_set_sort_mode(swarm_ctx* c, int mode);
 /* Lanes per slot (1, 2, 4 or 8; default 4) in predict (PHYSICS_PASS 1, cpu_crowd_sim.cpp:655-777): a slot's lanes
  * take interleaved slots of each perception-ring row and add the separation terms in ring order (the headway gap is
  * a minimum, the same in any order); then every lane carries the same state and the wall contacts split their 16

@@ -177,6 +177,7 @@ struct oracle_ctx {
     std::vector<float> x[2], y[2], vx[2], vy[2], spd[2], w[2];
     std::vector<float> r, v0;
     int readIdx = 0, writeIdx = 1;
+    bool bitonic = false;   // SortMode::GpuBitonic instead of the stable counting sort
     std::vector<float> driveX, driveY;
     bool drive = false;
     // grid
@@ -758,6 +759,37 @@ void oracle_set_choice_state(oracle_ctx* c, uint32_t step, const uint32_t* assig
     if (ahead) c->ahead.assign(ahead, ahead + size_t(2) * CHOICE_QUEUE_STRIDE * CHOICE_AHEAD_BINS);
 }
 
+// bitonic_sort.comp's network (cpu_crowd_sim.cpp:1012-1029) on (key, id) pairs, n a power of two
+static void bitonicNetwork(int n, uint32_t* key, uint32_t* id) {
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1)
+            for (int i = 0; i < n; ++i) {
+                const int p = i ^ j;
+                if (p <= i) continue;
+                const bool asc = (i & k) == 0;
+                if ((asc && key[i] > key[p]) || (!asc && key[i] < key[p])) {
+                    std::swap(key[i], key[p]);
+                    std::swap(id[i], id[p]);
+                }
+            }
+}
+
+// the network alone on keys in agent order: the sorted keys and agents; -1 unless n is a power of two
+int oracle_bitonic_sort(int n, const uint32_t* keys, uint32_t* sortedKeys, uint32_t* ids) {
+    if (n < 1 || (n & (n - 1)) != 0) return -1;
+    for (int i = 0; i < n; ++i) { sortedKeys[i] = keys[i]; ids[i] = uint32_t(i); }
+    bitonicNetwork(n, sortedKeys, ids);
+    return 0;
+}
+
+// 0: the stable counting sort (SortMode::Counting), 1: the GPU's bitonic network (SortMode::GpuBitonic, a
+// power-of-two agent count); returns -1 for a mode the agent count does not allow
+int oracle_set_sort_mode(oracle_ctx* c, int mode) {
+    if (mode == 1 && (c->n < 2 || (c->n & (c->n - 1)) != 0)) return -1;
+    c->bitonic = mode == 1;
+    return 0;
+}
+
 void oracle_set_drive(oracle_ctx* c, const float* dx, const float* dy) {
     if (!dx || !dy) { c->drive = false; return; }
     c->driveX.assign(dx, dx + c->n);
@@ -790,8 +822,14 @@ void oracle_step(oracle_ctx* c, int nsteps) {
             });
         }
         const auto t1 = Clock::now();
-        // stable counting sort (cpu_crowd_sim.cpp:997-1006): ascending (cell, agent), inactive at the tail
-        {
+        if (c->bitonic) {
+            // bitonic_sort.comp's network (cpu_crowd_sim.cpp:1012-1029): stages k = 2 .. n, passes j = k/2 .. 1; the
+            // lower index i of each pair (i, i ^ j) compares keys only and swaps the pair out of its stage's
+            // direction (ascending where (i & k) == 0)
+            std::copy(c->keyTmp.begin(), c->keyTmp.begin() + n, c->key.begin());
+            std::copy(c->idTmp.begin(), c->idTmp.begin() + n, c->id.begin());
+            bitonicNetwork(n, c->key.data(), c->id.data());
+        } else {   // stable counting sort (cpu_crowd_sim.cpp:997-1006): ascending (cell, agent), inactive at the tail
             const uint32_t cells = uint32_t(c->sc.hashNX) * uint32_t(c->sc.hashNY);
             auto bucketOf = [cells](uint32_t g) { return g == INACTIVE_CELL ? cells : g; };
             std::fill(c->bucketStart.begin(), c->bucketStart.end(), 0u);

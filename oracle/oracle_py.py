@@ -49,6 +49,7 @@ def lib():
         L.oracle_destroy.argtypes = [vp]
         L.oracle_set_state.argtypes = [vp] + [_f32p] * 7
         L.oracle_set_drive.argtypes = [vp, vp, vp]
+        L.oracle_set_sort_mode.argtypes = [vp, C.c_int]
         L.oracle_restore_state.argtypes = [vp] + [_f32p] * 5
         L.oracle_choice_step.argtypes = [vp]
         L.oracle_set_choice_state.argtypes = [vp, C.c_uint32, vp, vp, vp]
@@ -87,6 +88,17 @@ def scene_struct(info, tables):
     return s
 
 
+def bitonic_sort(keys):
+    """The reference GPU's bitonic network on keys in agent order: (sorted keys, agents); n a power of two."""
+    k = np.ascontiguousarray(keys, np.uint32)
+    sk, ids = np.zeros_like(k), np.zeros_like(k)
+    L = lib()
+    L.oracle_bitonic_sort.argtypes = [C.c_int, _u32p, _u32p, _u32p]
+    if L.oracle_bitonic_sort(k.size, k, sk, ids) != 0:
+        raise ValueError("the bitonic network needs a power-of-two count")
+    return sk, ids
+
+
 class Oracle:
     def __init__(self, info, tables, n, relax_step, tau, obstacle_gain=20.0, proj_iters=32, threads=0):
         self._tables = tables
@@ -113,6 +125,11 @@ class Oracle:
         keep = [None if a is None else np.ascontiguousarray(a, np.uint32) for a in (assign, queue, ahead)]
         ptr = [None if a is None else a.ctypes.data for a in keep]
         lib().oracle_set_choice_state(self.h, step, *ptr)
+
+    def set_sort_mode(self, bitonic):
+        """The stable counting sort (False) or the GPU's bitonic network (True; a power-of-two agent count)."""
+        if lib().oracle_set_sort_mode(self.h, 1 if bitonic else 0) != 0:
+            raise ValueError("the bitonic network needs a power-of-two agent count")
 
     def set_drive(self, dx=None, dy=None):
         if dx is None:

@@ -58,7 +58,7 @@ std::vector<float> flatVolume() {
 
 // Terrace pitch scenario: n agents in a box on the fenced pitch, forced drive directions (counterflow), K steps
 void terraceCase(const std::string& tag, const std::vector<float>& vol, int n, float half, float minDist, int steps, uint64_t seed,
-                 int inactive, bool nanAgent, int outside) {
+                 int inactive, bool nanAgent, int outside, CpuCrowdSim::SortMode mode = CpuCrowdSim::SortMode::Counting) {
     Rng rng{ seed };
     std::vector<GpuBoid> b(static_cast<size_t>(n));
     std::vector<glm::vec2> drive(static_cast<size_t>(n));
@@ -103,7 +103,7 @@ void terraceCase(const std::string& tag, const std::vector<float>& vol, int n, f
         const glm::vec2 d = glm::normalize(glm::vec2(dir, tilt));
         drive[size_t(i)] = d;
     }
-    CpuCrowdSim sim(vol, 4, CpuCrowdSim::SortMode::Counting);
+    CpuCrowdSim sim(vol, 4, mode);
     sim.setState(b);
     sim.setDriveTable(&drive);
     auto dumpState = [&](const std::vector<GpuBoid>& s, const std::string& p) {
@@ -404,6 +404,9 @@ int main(int argc, char** argv) {
     // (|x|, |y| < 0.30) and, in the first step, farther than R_PERC from the origin (the NaN agent's reset)
     terraceCase("t_sparse", vol, 512, 0.10f, 0.011f, 6, 1337, 0, false, 0);
     terraceCase("t_dense", vol, 1024, 0.085f, 0.0085f, 8, 4242, 37, true, 5);
+    // the same scenes sorted by the GPU's bitonic network (SortMode::GpuBitonic: its unstable permutation)
+    terraceCase("t_bsparse", vol, 512, 0.10f, 0.011f, 6, 1337, 0, false, 0, CpuCrowdSim::SortMode::GpuBitonic);
+    terraceCase("t_bdense", vol, 1024, 0.085f, 0.0085f, 8, 4242, 37, true, 5, CpuCrowdSim::SortMode::GpuBitonic);
     choiceCases();
     metricsCases();
     {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents

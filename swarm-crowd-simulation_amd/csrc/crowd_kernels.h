@@ -23,6 +23,8 @@ constexpr int PROJ_LANES = 1;       // lanes per slot in the projection iteratio
 constexpr int PROJ_COLLECT_LANES = 4;   // lanes per slot in the first projection iteration (ring scan + candidates)
 constexpr int PRED_LANES = 4;           // lanes per slot in predict (its perception ring and wall contacts)
 constexpr int ADJ_GRAPH_MAX_STEPS = 32; // adjoint segments up to this long replay and run backwards as captured graphs
+constexpr int BITONIC_TILE = 4096;      // pairs a block sorts in LDS in the bitonic sort mode (32 KB)
+constexpr int BITONIC_BLOCK = 1024;
 constexpr size_t ADJ_TAPE_BUDGET = size_t(4) << 30;   // bytes of tape a run may keep instead of recomputing segments
 constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
@@ -50,6 +52,7 @@ struct GridBuffers {
     uint32_t* bstart;   // per bucket: exclusive start (+ sentinel n); cells first: a cell's slots are [bstart[c], bstart[c + 1])
     uint32_t* tileSum;
     uint32_t* tmp;      // slot order, atomic order within buckets
+    uint64_t* pairs;    // the bitonic sort mode's (key << 32 | agent) pairs, n
 };
 
 // the batched step's per-scenario tables (unused by the plain step)
@@ -175,6 +178,7 @@ struct StepLaunch {
     int projLanes = 1;    // lanes per slot in the projection iterations that do not prune (1, 2, 4 or 8)
     int collectLanes = 1; // lanes per slot in the first projection iteration (1, 2, 4 or 8)
     int predictLanes = 1; // lanes per slot in predict (1, 2, 4 or 8)
+    bool bitonic = false; // the sort: the reference GPU's bitonic network instead of the stable counting sort
 };
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,

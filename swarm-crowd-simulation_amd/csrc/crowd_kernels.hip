@@ -211,6 +211,84 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_fix_gather(int n, BatchView bt, 
     }
 }
 
+// ---------------- the GPU reference's bitonic network (the optional sort mode) ----------------
+// bitonic_sort.comp / bitonic_sort_shared.comp over the hash output in agent order: stages k = 2 .. n, passes
+// j = k/2 .. 1; each pair (lo, lo + j), lo with bit j clear, is ascending where (lo & k) == 0; the keys are compared
+// and the pair swapped when out of its direction. The network is the reference's pass for pass, so the
+// permutation (unstable within a cell) is its own. A key rides in the high word of a 64-bit pair with its agent in
+// the low word; only high words are compared. Passes with j < BITONIC_TILE run on a block's tile in LDS.
+__device__ inline void bitonicCx(uint64_t* __restrict__ t, uint32_t base, uint32_t p, uint32_t k, uint32_t j) {
+    const uint32_t lo = ((p & ~(j - 1u)) << 1u) | (p & (j - 1u));   // pair p: a 0 inserted at bit log2(j)
+    const uint32_t hi = lo | j;
+    const bool asc = ((base + lo) & k) == 0u;
+    const uint64_t a = t[lo], b = t[hi];
+    const uint32_t ka = uint32_t(a >> 32), kb = uint32_t(b >> 32);
+    if ((asc && ka > kb) || (!asc && ka < kb)) { t[lo] = b; t[hi] = a; }
+}
+
+// the stages 2 .. tile of each tile of 'tile' pairs (tile = min(n, BITONIC_TILE)), from the keys in agent order
+__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_tile(int tile, const uint32_t* __restrict__ key, uint64_t* __restrict__ pairs) {
+    __shared__ uint64_t t[BITONIC_TILE];
+    const uint32_t base = blockIdx.x * uint32_t(tile);
+    for (int i = threadIdx.x; i < tile; i += blockDim.x) t[i] = (uint64_t(key[base + uint32_t(i)]) << 32) | uint64_t(base + uint32_t(i));
+    __syncthreads();
+    for (uint32_t k = 2; k <= uint32_t(tile); k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = threadIdx.x; p < uint32_t(tile) / 2u; p += blockDim.x) bitonicCx(t, base, p, k, j);
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < tile; i += blockDim.x) pairs[base + uint32_t(i)] = t[i];
+}
+
+// stage k's passes j = BITONIC_TILE/2 .. 1 on each tile (n > BITONIC_TILE)
+__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_merge(uint32_t k, uint64_t* __restrict__ pairs) {
+    __shared__ uint64_t t[BITONIC_TILE];
+    const uint32_t base = blockIdx.x * uint32_t(BITONIC_TILE);
+    for (int i = threadIdx.x; i < BITONIC_TILE; i += blockDim.x) t[i] = pairs[base + uint32_t(i)];
+    __syncthreads();
+    for (uint32_t j = BITONIC_TILE / 2; j > 0; j >>= 1) {
+        for (uint32_t p = threadIdx.x; p < uint32_t(BITONIC_TILE) / 2u; p += blockDim.x) bitonicCx(t, base, p, k, j);
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < BITONIC_TILE; i += blockDim.x) pairs[base + uint32_t(i)] = t[i];
+}
+
+// one pass (k, j >= BITONIC_TILE) over all n/2 pairs in HBM
+__global__ void __launch_bounds__(256) k_bitonic_pass(uint32_t half, uint32_t k, uint32_t j, uint64_t* __restrict__ pairs) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p < half) bitonicCx(pairs, 0u, p, k, j);
+}
+
+// the sorted pairs to the slot order's ids and keys, and the state gathered into slot order (as k_fix_gather)
+template <bool B>
+__global__ void __launch_bounds__(HASH_BLOCK) k_bitonic_gather(int n, BatchView bt, const uint64_t* __restrict__ pairs, uint32_t* __restrict__ sid,
+                                                               uint32_t* __restrict__ skey, const float* __restrict__ ax,
+                                                               const float* __restrict__ ay, const float* __restrict__ rad,
+                                                               float* __restrict__ sx, float* __restrict__ sy, float* __restrict__ sr) {
+    const int f = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    if (f >= n) return;
+    const uint64_t v = pairs[f];
+    const uint32_t id = uint32_t(v), k = uint32_t(v >> 32);
+    sid[f] = id;
+    skey[f] = k;
+    if constexpr (B) bt.slotScen[f] = bt.agentScen[id];
+    if (k != INACTIVE_CELL) {
+        sx[f] = ax[id];
+        sy[f] = ay[id];
+        sr[f] = rad[id];
+    }
+}
+
+static void launchBitonic(hipStream_t st, int n, const uint32_t* key, uint64_t* pairs) {
+    const int tile = n < BITONIC_TILE ? n : BITONIC_TILE;   // n is a power of two
+    k_bitonic_tile<<<n / tile, BITONIC_BLOCK, 0, st>>>(tile, key, pairs);
+    for (uint32_t k = 2u * uint32_t(tile); k <= uint32_t(n); k <<= 1) {
+        for (uint32_t j = k >> 1; j >= uint32_t(tile); j >>= 1)
+            k_bitonic_pass<<<(n / 2 + 255) / 256, 256, 0, st>>>(uint32_t(n) / 2u, k, j, pairs);
+        k_bitonic_merge<<<n / BITONIC_TILE, BITONIC_BLOCK, 0, st>>>(k, pairs);
+    }
+}
+
 // ---------------- exit choice (PHYSICS_PASS 0) ----------------
 __device__ inline bool evalStep(uint32_t step) { return step > 0u && step % uint32_t(CHOICE_STEPS) == 0u; }
 
@@ -1250,6 +1328,11 @@ static void gridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const 
     k_hash<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, g.key, g.loc, g.bcnt, C);
     k_scan_reduce<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, g.tileSum);
     k_scan_apply<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, C, g.tileSum, g.bstart, ctrl);
+    if (L.bitonic) {   // the cell ranges from the counts, the permutation from the reference's network
+        launchBitonic(st, n, g.key, g.pairs);
+        k_bitonic_gather<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.pairs, b.sid, b.skey, rd.x, rd.y, rad, b.sx, b.sy, b.sr);
+        return;
+    }
     k_scatter<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bstart, g.tmp, C);
     k_fix_gather<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.key, g.tmp, g.bstart, C, b.sid, b.skey, rd.x, rd.y, rad,
                                                                    b.sx, b.sy, b.sr);
@@ -1264,6 +1347,14 @@ void launchSortKeys(hipStream_t st, const StepLaunch& L, AgentHalf rd, const Gri
     k_key_count<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bcnt, C);
     k_scan_reduce<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, g.tileSum);
     k_scan_apply<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, C, g.tileSum, g.bstart, ctrl);
+    if (L.bitonic) {
+        launchBitonic(st, n, g.key, g.pairs);
+        if (L.batched)
+            k_bitonic_gather<true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.pairs, b.sid, b.skey, rd.x, rd.y, rad, b.sx, b.sy, b.sr);
+        else
+            k_bitonic_gather<false><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.pairs, b.sid, b.skey, rd.x, rd.y, rad, b.sx, b.sy, b.sr);
+        return;
+    }
     k_scatter<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, g.key, g.loc, g.bstart, g.tmp, C);
     if (L.batched)
         k_fix_gather<true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.bt, g.key, g.tmp, g.bstart, C, b.sid, b.skey, rd.x, rd.y, rad,

@@ -143,6 +143,7 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     grid_.key = alloc<uint32_t>(n); grid_.loc = alloc<uint32_t>(n);
     grid_.bcnt = alloc<uint32_t>(size_t(nb_)); grid_.bstart = alloc<uint32_t>(size_t(nb_) + 1);
     grid_.tileSum = alloc<uint32_t>(size_t((nb_ + SCAN_TILE - 1) / SCAN_TILE)); grid_.tmp = alloc<uint32_t>(n);
+    grid_.pairs = alloc<uint64_t>(n);
     sb_.sid = alloc<uint32_t>(n); sb_.skey = alloc<uint32_t>(n);
     sb_.bstart = grid_.bstart;
     sb_.sx = alloc<float>(n); sb_.sy = alloc<float>(n); sb_.sr = alloc<float>(n);
@@ -205,6 +206,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.projLanes = projLanes_;
     L.collectLanes = collectLanes_;
     L.predictLanes = predictLanes_;
+    L.bitonic = bitonic_;
     return L;
 }
 
@@ -321,6 +323,14 @@ void HipCrowdSim::setPredictLanes(int lanes) {
     if (!(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8)) throw std::invalid_argument("setPredictLanes: 1, 2, 4 or 8");
     if (lanes != predictLanes_) dropGraph();
     predictLanes_ = lanes;
+}
+
+void HipCrowdSim::setSortMode(int mode) {
+    if (mode != 0 && mode != 1) throw std::invalid_argument("setSortMode: 0 (counting) or 1 (bitonic network)");
+    if (mode == 1 && (n_ < 2 || (n_ & (n_ - 1)) != 0))
+        throw std::invalid_argument("setSortMode: the bitonic network needs a power-of-two agent count");
+    if ((mode == 1) != bitonic_) dropGraph();
+    bitonic_ = mode == 1;
 }
 
 void HipCrowdSim::setPruneEvery(int every) {

@@ -78,6 +78,7 @@ public:
     int projectionLanes() const { return projLanes_; }
     void setCollectLanes(int lanes);
     void setPredictLanes(int lanes);
+    void setSortMode(int mode);   // 0: stable counting sort (SortMode::Counting), 1: the GPU's bitonic network
     int predictLanes() const { return predictLanes_; }
     int collectLanes() const { return collectLanes_; }
     // Times every candidate launch shape of the physics passes over 'steps' steps from the current state, then the
@@ -195,6 +196,7 @@ private:
     int projLanes_ = PROJ_LANES;
     int collectLanes_ = PROJ_COLLECT_LANES;
     int predictLanes_ = PRED_LANES;
+    bool bitonic_ = false;
     hipStream_t stream_ = nullptr;
     std::vector<void*> bufs_;
     AgentHalf half_[2]{};

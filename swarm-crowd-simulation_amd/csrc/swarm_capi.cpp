@@ -294,6 +294,10 @@ int swarm_read_candidates(swarm_ctx* c, uint32_t* count, uint32_t* index, float*
     CTX_CHECK(c);
     return guarded([&] { c->sim->readCandidates(count, index, dist); });
 }
+int swarm_set_sort_mode(swarm_ctx* c, int mode) {
+    CTX_CHECK(c);
+    return guarded([&] { c->sim->setSortMode(mode); });
+}
 int swarm_set_predict_lanes(swarm_ctx* c, int lanes) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setPredictLanes(lanes); });

@@ -344,6 +344,11 @@ class Sim:
         check(lib().swarm_read_candidates(self.h, cnt.ctypes.data, idx.ctypes.data, d.ctypes.data))
         return cnt, idx, d
 
+    def set_sort_mode(self, bitonic):
+        """The stable counting sort (False) or the reference GPU's bitonic network (True; power-of-two n)."""
+        lib().swarm_set_sort_mode.argtypes = [C.c_void_p, C.c_int]
+        check(lib().swarm_set_sort_mode(self.h, 1 if bitonic else 0))
+
     def set_predict_lanes(self, lanes=4):
         """Lanes per slot in predict's perception ring; bit-identical results."""
         lib().swarm_set_predict_lanes.argtypes = [C.c_void_p, C.c_int]

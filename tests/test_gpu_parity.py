@@ -30,13 +30,15 @@ def compare_step(sim, o, what, cells=True):
 
 
 # ---------------- against the reference's own golden vectors ----------------
-@pytest.mark.parametrize("tag,steps", [("t_sparse", 6), ("t_dense", 8)])
+@pytest.mark.parametrize("tag,steps", [("t_sparse", 6), ("t_dense", 8), ("t_bsparse", 6), ("t_bdense", 8)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_gpu_matches_reference_terrace(tag, steps, graph):
+    """t_b*: the reference sorted by its GPU's bitonic network (SortMode::GpuBitonic), this path in sort mode 1."""
     kat = golden("terrace_kat.npz")
     a = agents_from_golden(kat, tag)
     n = a["x"].size
     sim = sh.Sim(terrace_scene(), n, use_graph=graph)
+    sim.set_sort_mode(tag.startswith("t_b"))
     sim.upload(a)
     sim.set_drive(kat[tag + "__in__dx"], kat[tag + "__in__dy"])
     for k in range(1, steps + 1):
@@ -242,3 +244,56 @@ def test_previous_state_is_last_input():
         prev = sim.prev_state()
         for f in FIELDS:
             assert_bits(prev[f], before[f], f"previous state after step {k + 1} {f}")
+
+
+# ---------------- the bitonic sort mode ----------------
+@pytest.mark.parametrize("layout,active,total,steps,every", [(sh.LAYOUT_ROOM, 4096, 4096, 60, 1), (sh.LAYOUT_BOWL, 63424, 65536, 3, 1)])
+def test_bitonic_mode_bit_exact(layout, active, total, steps, every):
+    """The reference GPU's bitonic network as the sort (its unstable within-cell order changes every neighbour sum):
+    the whole step against the oracle in the same mode, every quantity every step."""
+    scene = sh.Scene.layout(layout)
+    a = sh.place_agents(layout, active, total, 1337)
+    sim = sh.Sim(scene, total)
+    sim.set_sort_mode(True)
+    sim.upload(a)
+    o = oracle_for(scene, total)
+    o.set_sort_mode(True)
+    o.set_state(a)
+    first = None
+    for k in range(steps):
+        sim.step(1)
+        o.step(1)
+        compare_step(sim, o, f"bitonic layout {layout} step {k + 1}")
+        assert np.array_equal(sim.assign(), o.assign())
+        assert np.array_equal(sim.choice_counters(), o.choice_counters())
+        if first is None:
+            first = sim.grid()[1]
+    counting = sh.Sim(scene, total)
+    counting.upload(a)
+    counting.step(1)
+    assert not np.array_equal(counting.grid()[1], first)   # the two sorts do differ within cells
+
+
+def test_bitonic_sort_grid_on_given_keys():
+    """CrowdSim::sortGrid in the bitonic mode: the network's permutation on keys of the caller's own (few distinct
+    keys, all equal, descending, inactive), sizes 2 .. 65,536, against the oracle's network."""
+    import oracle_py as op
+
+    rng = np.random.default_rng(3)
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    for n in (2, 256, 1024, 4096, 8192, 65536):
+        sim = sh.Sim(scene, n)
+        sim.upload(sh.place_agents(sh.LAYOUT_ROOM, min(n, 100), n, 1))
+        sim.set_sort_mode(True)
+        cases = [rng.integers(0, 7, n), np.zeros(n, np.int64), np.arange(n)[::-1] % sim.cells, rng.integers(0, sim.cells, n)]
+        for keys in cases:
+            keys = keys.astype(np.uint32)
+            keys[rng.random(n) < 0.1] = sh.INACTIVE_CELL
+            sk, ids = sim.sort_grid(keys)
+            ok, oi = op.bitonic_sort(keys)
+            assert np.array_equal(sk, ok) and np.array_equal(ids, oi), f"n {n}"
+            st, cn = sim.cell_table()
+            act = keys[keys != sh.INACTIVE_CELL]
+            assert np.array_equal(cn, np.bincount(act, minlength=sim.cells).astype(np.uint32))
+    with pytest.raises(RuntimeError):
+        sh.Sim(scene, 300).set_sort_mode(True)   # not a power of two
