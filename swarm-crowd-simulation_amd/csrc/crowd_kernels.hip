@@ -226,31 +226,76 @@ __device__ inline void bitonicCx(uint64_t* __restrict__ t, uint32_t base, uint32
     if ((asc && ka > kb) || (!asc && ka < kb)) { t[lo] = b; t[hi] = a; }
 }
 
-// the stages 2 .. tile of each tile of 'tile' pairs (tile = min(n, BITONIC_TILE)), from the keys in agent order
-__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_tile(int tile, const uint32_t* __restrict__ key, uint64_t* __restrict__ pairs) {
-    __shared__ uint64_t t[BITONIC_TILE];
-    const uint32_t base = blockIdx.x * uint32_t(tile);
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) t[i] = (uint64_t(key[base + uint32_t(i)]) << 32) | uint64_t(base + uint32_t(i));
-    __syncthreads();
-    for (uint32_t k = 2; k <= uint32_t(tile); k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t p = threadIdx.x; p < uint32_t(tile) / 2u; p += blockDim.x) bitonicCx(t, base, p, k, j);
-            __syncthreads();
-        }
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) pairs[base + uint32_t(i)] = t[i];
+// Inside a tile a thread owns its elements 2t and 2t + 1 in registers. A pass with j = 1 pairs them; a pass with
+// 2 <= j <= 64 pairs them with those of thread t ^ (j / 2), in the same wave (read by shuffles, each thread keeping
+// its side of the compare-exchange); a pass with j >= 128 goes through the tile in LDS.
+__device__ inline uint64_t shflXor64(uint64_t v, int d) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), d)), hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), d));
+    return (uint64_t(hi) << 32) | uint64_t(lo);
+}
+__device__ inline void regPass(uint64_t& r0, uint64_t& r1, uint32_t base, uint32_t t, uint32_t k, uint32_t j) {
+    const uint32_t e0 = 2u * t;
+    if (j == 1u) {
+        const bool asc = ((base + e0) & k) == 0u;
+        const uint32_t k0 = uint32_t(r0 >> 32), k1 = uint32_t(r1 >> 32);
+        if ((asc && k0 > k1) || (!asc && k0 < k1)) { const uint64_t x = r0; r0 = r1; r1 = x; }
+        return;
+    }
+    const int d = int(j >> 1);
+    const uint64_t o0 = shflXor64(r0, d), o1 = shflXor64(r1, d);
+    const bool meLo = (e0 & j) == 0u;                    // the same for e0 + 1: j >= 2
+    const bool asc = ((base + (meLo ? e0 : (e0 ^ j))) & k) == 0u;   // k >= 4: bit 0 of the pair's lower index is moot
+    auto pick = [&](uint64_t mine, uint64_t other) {
+        const uint64_t lo = meLo ? mine : other, hi = meLo ? other : mine;
+        const uint32_t kl = uint32_t(lo >> 32), kh = uint32_t(hi >> 32);
+        return ((asc && kl > kh) || (!asc && kl < kh)) ? other : mine;
+    };
+    r0 = pick(r0, o0);
+    r1 = pick(r1, o1);
 }
 
-// stage k's passes j = BITONIC_TILE/2 .. 1 on each tile (n > BITONIC_TILE)
-__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_merge(uint32_t k, uint64_t* __restrict__ pairs) {
-    __shared__ uint64_t t[BITONIC_TILE];
-    const uint32_t base = blockIdx.x * uint32_t(BITONIC_TILE);
-    for (int i = threadIdx.x; i < BITONIC_TILE; i += blockDim.x) t[i] = pairs[base + uint32_t(i)];
-    __syncthreads();
-    for (uint32_t j = BITONIC_TILE / 2; j > 0; j >>= 1) {
-        for (uint32_t p = threadIdx.x; p < uint32_t(BITONIC_TILE) / 2u; p += blockDim.x) bitonicCx(t, base, p, k, j);
+// stage k's passes j = jTop .. 1 on the tile at 'base' held in registers (r0, r1); the passes j >= 128 in LDS
+__device__ inline void tileStage(uint64_t* __restrict__ lds, uint64_t& r0, uint64_t& r1, uint32_t base, uint32_t t, uint32_t k, uint32_t jTop) {
+    uint32_t j = jTop;
+    if (j >= 128u) {
+        lds[2u * t] = r0;
+        lds[2u * t + 1u] = r1;
         __syncthreads();
+        for (; j >= 128u; j >>= 1) {
+            bitonicCx(lds, base, t, k, j);   // pair t of the tile (blockDim = tile / 2 pairs)
+            __syncthreads();
+        }
+        r0 = lds[2u * t];
+        r1 = lds[2u * t + 1u];
+        __syncthreads();   // the tile is read before the next stage writes it
     }
-    for (int i = threadIdx.x; i < BITONIC_TILE; i += blockDim.x) pairs[base + uint32_t(i)] = t[i];
+    for (; j > 0; j >>= 1) regPass(r0, r1, base, t, k, j);
+}
+
+// the stages 2 .. tile of each tile of 'tile' pairs (tile = min(n, BITONIC_TILE), tile / 2 threads a block), from
+// the keys in agent order
+__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_tile(int tile, const uint32_t* __restrict__ key, uint64_t* __restrict__ pairs) {
+    __shared__ uint64_t lds[BITONIC_TILE];
+    const uint32_t t = threadIdx.x;
+    const uint32_t base = blockIdx.x * uint32_t(tile);
+    const uint32_t e = base + 2u * t;
+    uint64_t r0 = (uint64_t(key[e]) << 32) | uint64_t(e);
+    uint64_t r1 = (uint64_t(key[e + 1u]) << 32) | uint64_t(e + 1u);
+    for (uint32_t k = 2; k <= uint32_t(tile); k <<= 1) tileStage(lds, r0, r1, base, t, k, k >> 1);
+    pairs[e] = r0;
+    pairs[e + 1u] = r1;
+}
+
+// stage k's passes j = BITONIC_TILE/2 .. 1 on each tile (n > BITONIC_TILE; BITONIC_TILE / 2 threads a block)
+__global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_merge(uint32_t k, uint64_t* __restrict__ pairs) {
+    __shared__ uint64_t lds[BITONIC_TILE];
+    const uint32_t t = threadIdx.x;
+    const uint32_t base = blockIdx.x * uint32_t(BITONIC_TILE);
+    const uint32_t e = base + 2u * t;
+    uint64_t r0 = pairs[e], r1 = pairs[e + 1u];
+    tileStage(lds, r0, r1, base, t, k, BITONIC_TILE / 2);
+    pairs[e] = r0;
+    pairs[e + 1u] = r1;
 }
 
 // one pass (k, j >= BITONIC_TILE) over all n/2 pairs in HBM
@@ -281,11 +326,11 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_bitonic_gather(int n, BatchView 
 
 static void launchBitonic(hipStream_t st, int n, const uint32_t* key, uint64_t* pairs) {
     const int tile = n < BITONIC_TILE ? n : BITONIC_TILE;   // n is a power of two
-    k_bitonic_tile<<<n / tile, BITONIC_BLOCK, 0, st>>>(tile, key, pairs);
+    k_bitonic_tile<<<n / tile, tile / 2, 0, st>>>(tile, key, pairs);
     for (uint32_t k = 2u * uint32_t(tile); k <= uint32_t(n); k <<= 1) {
         for (uint32_t j = k >> 1; j >= uint32_t(tile); j >>= 1)
             k_bitonic_pass<<<(n / 2 + 255) / 256, 256, 0, st>>>(uint32_t(n) / 2u, k, j, pairs);
-        k_bitonic_merge<<<n / BITONIC_TILE, BITONIC_BLOCK, 0, st>>>(k, pairs);
+        k_bitonic_merge<<<n / BITONIC_TILE, BITONIC_TILE / 2, 0, st>>>(k, pairs);
     }
 }
 
