@@ -177,25 +177,52 @@ __global__ void __launch_bounds__(256) k_adj_project(AdjStep t, BatchView bt, Ad
     const float ri = t.sr[i];
     const V2 wi{ wv[i].x, wv[i].y };
     float gr = g.gr[i];
-    for (uint64_t m = t.local ? 0ull : t.mask[size_t(it - 1) * n + size_t(i)]; m; m &= m - 1ull) {
-        const int k = __ffsll((unsigned long long)m) - 1;
-        const uint32_t j = t.nbr[size_t(k) * n + size_t(i)];
-        const float2 qj = t.q[size_t(it - 1) * n + size_t(j)];
-        const V2 off{ qi.x - qj.x, qi.y - qj.y };
-        const float d2 = dot2(off, off);
-        const float D = ri + t.sr[j];
-        const V2 dw = wi - V2{ wv[j].x, wv[j].y };
-        if (d2 <= SEP_STACKED_XY2) {
-            const V2 dir = pairDirection(self, t.sid[j] - sr.agentBase);
-            const float dd = dot2(dir, dw);
-            gr += 0.5f * dd;
-            if (d2 > 0.0f) gq = gq - off * (0.5f * dd / sqrtf(d2));
-        } else {
-            const float d = sqrtf(d2);
-            const V2 nh = off / d;
-            const float nd = dot2(nh, dw);
-            gq = gq + (dw - nh * nd) * (0.5f * (D / d - 1.0f)) - nh * (0.5f * nd);
-            gr += 0.5f * nd;
+    // the contacts in chunks of ADJ_CHUNK: a chunk's index loads, then its position, radius and adjoint loads, are
+    // all in flight together; the sums still run in contact order
+    uint64_t m = t.local ? 0ull : t.mask[size_t(it - 1) * n + size_t(i)];
+    while (m) {
+        uint32_t jj[ADJ_CHUNK];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < ADJ_CHUNK; ++u) {
+            if (m) {
+                const int k = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1ull;
+                jj[u] = t.nbr[size_t(k) * n + size_t(i)];
+                ++cnt;
+            } else {
+                jj[u] = uint32_t(i);
+            }
+        }
+        float2 qq[ADJ_CHUNK], ww[ADJ_CHUNK];
+        float rr[ADJ_CHUNK];
+#pragma unroll
+        for (int u = 0; u < ADJ_CHUNK; ++u) {
+            qq[u] = t.q[size_t(it - 1) * n + size_t(jj[u])];
+            rr[u] = t.sr[jj[u]];
+            ww[u] = wv[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < ADJ_CHUNK; ++u) {
+            if (u >= cnt) break;
+            const uint32_t j = jj[u];
+            const float2 qj = qq[u];
+            const V2 off{ qi.x - qj.x, qi.y - qj.y };
+            const float d2 = dot2(off, off);
+            const float D = ri + rr[u];
+            const V2 dw = wi - V2{ ww[u].x, ww[u].y };
+            if (d2 <= SEP_STACKED_XY2) {
+                const V2 dir = pairDirection(self, t.sid[j] - sr.agentBase);
+                const float dd = dot2(dir, dw);
+                gr += 0.5f * dd;
+                if (d2 > 0.0f) gq = gq - off * (0.5f * dd / sqrtf(d2));
+            } else {
+                const float d = sqrtf(d2);
+                const V2 nh = off / d;
+                const float nd = dot2(nh, dw);
+                gq = gq + (dw - nh * nd) * (0.5f * (D / d - 1.0f)) - nh * (0.5f * nd);
+                gr += 0.5f * nd;
+            }
         }
     }
     if (it > 1) {

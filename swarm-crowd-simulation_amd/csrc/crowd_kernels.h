@@ -25,6 +25,7 @@ constexpr int PRED_LANES = 4;           // lanes per slot in predict (its percep
 constexpr int ADJ_GRAPH_MAX_STEPS = 32; // adjoint segments up to this long replay and run backwards as captured graphs
 constexpr int BITONIC_TILE = 2048;      // pairs a block sorts in LDS in the bitonic sort mode (16 KB)
 constexpr int BITONIC_BLOCK = 1024;
+constexpr int ADJ_CHUNK = 4;            // contacts whose loads the adjoint projection issues together
 constexpr size_t ADJ_TAPE_BUDGET = size_t(4) << 30;   // bytes of tape a run may keep instead of recomputing segments
 constexpr int PRED_CHUNK = 4;      // ring slots loaded together by one predict thread
 constexpr uint32_t NBR_OVERFLOW = 0xFFFFFFFFu;
