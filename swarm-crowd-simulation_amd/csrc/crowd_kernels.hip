@@ -755,7 +755,8 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict_lanes(int n, SceneVi
 // ---------------- PHYSICS_PASS 2: the Jacobi projection ----------------
 // The end of projectSlot (cpu_crowd_sim.cpp:805-821): the mean correction, its cap, the wall plane, the accumulator
 // capped / planed: whether the cap and the wall plane applied (the adjoint's tape)
-__device__ inline void finishProjection(const StepBuffers& b, int slot, float bx, float by, float acc, V2 corr, int nContacts,
+// pl: the slot's wall plane (planes[slot])
+__device__ inline void finishProjection(const float4 pl, float bx, float by, float acc, V2 corr, int nContacts,
                                         float ri, float& ox, float& oy, float& oacc, bool* capped = nullptr, bool* planed = nullptr) {
     corr = corr * (PROJ_RELAX / float(nContacts > 1 ? nContacts : 1));
     const float maxShift = PROJ_MAX_SHIFT_RADII * ri;
@@ -763,7 +764,6 @@ __device__ inline void finishProjection(const StepBuffers& b, int slot, float bx
     const bool cap = len > maxShift;
     if (cap) corr = corr * (maxShift / len);
     V2 q = V2{ bx, by } + corr;
-    const float4 pl = b.planes[slot];
     bool pld = false;
     if (int(pl.w) & 1) {
         const float sd = pl.z - dot2(q, V2{ pl.x, pl.y });
@@ -798,6 +798,7 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
     const float acc = in.acc[slot];
     const uint32_t self = b.sid[slot] - sr.agentBase;
     const float ri = b.sr[slot];
+    const float4 pl = b.planes[slot];
     const uint32_t cid = key - sr.cellBase;
     const int cx = int(cid % uint32_t(s.hashNX)), cy = int(cid / uint32_t(s.hashNX));
     V2 corr{ 0.0f, 0.0f };
@@ -848,32 +849,52 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
         }
     }
     if (collect) b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
-    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
+    finishProjection(pl, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
-// One iteration for one slot over its contact candidates
+// A slot's own words for one projection iteration. An iteration over the candidate lists is a short chain of
+// dependent load rounds per slot (the slot's words, the candidates' indices, the candidates' positions); loading
+// every word of the slot at once, ahead of the kernel's branches on them, makes the first link one round.
+struct SlotHead {
+    float2 p;
+    float acc, ri;
+    uint32_t sid;
+    float4 plane;
+};
+__device__ inline SlotHead loadHead(const StepBuffers& b, const ProjBuffers& in, int slot) {
+    return SlotHead{ in.p[slot], in.acc[slot], b.sr[slot], b.sid[slot], b.planes[slot] };
+}
+// The compiler otherwise moves each load into the branch that first uses it, one round per branch: this empty
+// statement uses every value at the point of the call, so all the loads issue before it and the wave waits once.
+__device__ inline void holdHead(uint32_t key, float pw, uint32_t listed, const SlotHead& h) {
+    asm volatile("" ::"v"(key), "v"(pw), "v"(listed), "v"(h.p.x), "v"(h.p.y), "v"(h.acc), "v"(h.ri), "v"(h.sid), "v"(h.plane.x),
+                 "v"(h.plane.y), "v"(h.plane.z), "v"(h.plane.w));
+}
+
+// One iteration for one slot over its contact candidates (cnt >= 1)
 __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
-                                   float& ox, float& oy, float& oacc, uint64_t* mask = nullptr, bool* capped = nullptr,
-                                   bool* planed = nullptr) {
-    const float2 bp = in.p[slot];
-    const float acc = in.acc[slot];
-    const uint32_t self = b.sid[slot] - sr.agentBase;
-    const float ri = b.sr[slot];
+                                   const SlotHead& h, float& ox, float& oy, float& oacc, uint64_t* mask = nullptr,
+                                   bool* capped = nullptr, bool* planed = nullptr) {
+    const float2 bp = h.p;
+    const float acc = h.acc;
+    const uint32_t self = h.sid - sr.agentBase;
+    const float ri = h.ri;
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     const size_t stride = size_t(n);
     // candidates in chunks: the chunk's index and position loads are all in flight before the first use; the
-    // sum still runs in candidate order
+    // sum still runs in candidate order. A chunk's entries past the count load the last candidate again (no
+    // branch around the loads) and are never used.
     for (uint32_t j0 = 0; j0 < cnt; j0 += PROJ_CHUNK) {
         uint32_t kk[PROJ_CHUNK];
         float dd[PROJ_CHUNK];
         float2 pp[PROJ_CHUNK];
 #pragma unroll
         for (int u = 0; u < PROJ_CHUNK; ++u) {
-            const bool in_ = j0 + uint32_t(u) < cnt;
-            const size_t at = size_t(j0 + uint32_t(u)) * stride + size_t(slot);
-            kk[u] = in_ ? b.nbr[at] : uint32_t(slot);
-            dd[u] = in_ ? b.nbrD[at] : 0.0f;
+            const uint32_t j = j0 + uint32_t(u) < cnt ? j0 + uint32_t(u) : cnt - 1u;
+            const size_t at = size_t(j) * stride + size_t(slot);
+            kk[u] = b.nbr[at];
+            dd[u] = b.nbrD[at];
         }
 #pragma unroll
         for (int u = 0; u < PROJ_CHUNK; ++u) pp[u] = in.p[kk[u]];
@@ -889,7 +910,7 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
             }
         }
     }
-    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
+    finishProjection(h.plane, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
 // One iteration over the candidates that also drops those out of reach for the rest of the step: with 'rem'
@@ -897,11 +918,11 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
 // than D + rem PROJ_MAX_SHIFT_RADII D (with the collection's slack) cannot touch again. The kept candidates keep
 // their order, so every later sum is the unpruned sum bit for bit (the dropped terms are never contacts).
 __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
-                                        int rem, float& ox, float& oy, float& oacc) {
-    const float2 bp = in.p[slot];
-    const float acc = in.acc[slot];
-    const uint32_t self = b.sid[slot] - sr.agentBase;
-    const float ri = b.sr[slot];
+                                        const SlotHead& h, int rem, float& ox, float& oy, float& oacc) {
+    const float2 bp = h.p;
+    const float acc = h.acc;
+    const uint32_t self = h.sid - sr.agentBase;
+    const float ri = h.ri;
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     const size_t stride = size_t(n);
@@ -940,7 +961,7 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
         }
     }
     if (kept != cnt) b.nbrCount[slot] = kept;
-    finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
+    finishProjection(h.plane, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
 }
 
 // A later iteration with Q lanes per slot (the production step's iterations that do not prune). Lane l of a slot
@@ -1038,7 +1059,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
         if (lane == 0) {
             float x, y, oacc;
             bool capped = false, planed = false;
-            finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, b.sr[slot], x, y, oacc, &capped, &planed);
+            finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, b.sr[slot], x, y, oacc, &capped, &planed);
             out.p[slot] = make_float2(x, y);
             out.acc[slot] = oacc;
             if constexpr (REC) {
@@ -1139,7 +1160,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, Scene
         if (lane == 0) {
             b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
             float x, y, oacc;
-            finishProjection(b, slot, bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc);
+            finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc);
             out.p[slot] = make_float2(x, y);
             out.acc[slot] = oacc;
         }
@@ -1154,28 +1175,32 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
                                                             ProjBuffers out, int P, int first, TapeStep tp = TapeStep{}, int it = 0,
                                                             int prune = 0) {
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+        // every word of the slot in one load round, before the branches on them
         const uint32_t key = b.skey[slot];
+        const float pw = b.pw[slot];
+        const uint32_t listed = b.nbrCount[slot];
+        const SlotHead h = loadHead(b, in, slot);
+        holdHead(key, pw, listed, h);
         if (key == INACTIVE_CELL) continue;
         float x, y, acc;
         uint64_t mask = 0ull;
         bool capped = false, planed = false;
-        if (b.pw[slot] == 0.0f) {   // parked this step: carried
-            const float2 bp = in.p[slot];
-            x = bp.x; y = bp.y; acc = in.acc[slot];
+        if (pw == 0.0f) {   // parked this step: carried
+            x = h.p.x; y = h.p.y; acc = h.acc;
         } else {
             const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
-            const uint32_t cnt = first ? NBR_OVERFLOW : b.nbrCount[slot];
+            const uint32_t cnt = first ? NBR_OVERFLOW : listed;
             if constexpr (REC) {
                 if (cnt == NBR_OVERFLOW) {
                     projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc, &mask, &capped, &planed);
                     if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
                 } else {
-                    projectList(b, in, n, slot, cnt, sr, x, y, acc, &mask, &capped, &planed);
+                    projectList(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                 }
             } else {
                 if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc);
-                else if (prune > 0) projectListPrune(b, in, n, slot, cnt, sr, prune, x, y, acc);
-                else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+                else if (prune > 0) projectListPrune(b, in, n, slot, cnt, sr, h, prune, x, y, acc);
+                else projectList(b, in, n, slot, cnt, sr, h, x, y, acc);
             }
         }
         out.p[slot] = make_float2(x, y);
@@ -1219,7 +1244,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                         projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc, &mask, &capped, &planed);
                         atomicOr(tp.err, 1u);
                     } else {
-                        projectList(b, in, n, slot, cnt, sr, x, y, acc, &mask, &capped, &planed);
+                        projectList(b, in, n, slot, cnt, sr, loadHead(b, in, slot), x, y, acc, &mask, &capped, &planed);
                     }
                     const int it = mp.projIters;
                     tp.mask[size_t(it - 1) * size_t(n) + size_t(slot)] = mask;
@@ -1227,7 +1252,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                     if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
                 } else {
                     if (cnt == NBR_OVERFLOW) projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc);
-                    else projectList(b, in, n, slot, cnt, sr, x, y, acc);
+                    else projectList(b, in, n, slot, cnt, sr, loadHead(b, in, slot), x, y, acc);
                 }
             }
             if constexpr (REC) { last.p[slot] = make_float2(x, y); last.acc[slot] = acc; }
