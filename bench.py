@@ -213,7 +213,7 @@ def main():
           f"SDF {scene.info.sdf_nx}x{scene.info.sdf_ny}, nav {scene.info.nav_res_x}x{scene.info.nav_res_y} x {scene.info.E} exits",
           file=sys.stderr, flush=True)
 
-    tune_ms = sim.autotune(5)   # launch shape and lanes per slot (the state is restored)
+    tune_ms = sim.autotune(10)   # launch shape and lanes per slot (the state is restored)
     shape, lanes = list(sim.physics_shape()), sim.projection_lanes()
     lane_cfg = dict(zip(("list", "collect", "predict"), sim.lanes()))
     sim.step(args.warmup)

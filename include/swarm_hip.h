@@ -217,8 +217,8 @@ int swarm_set_predict_lanes(swarm_ctx* c, int lanes);
 int swarm_get_lanes(swarm_ctx* c, int* list_lanes, int* collect_lanes, int* predict_lanes);
 /* Times each candidate physics launch shape over 'steps' steps from the current state, then at the fastest shape
  * the lanes per slot of the projection's list iterations (1, 2, 4, 8), of predict and of the collecting iteration
- * (1, 2, 4 each); keeps the fastest and restores the state; *best_ms: its mean ms per step. No counterpart in the
- * reference. */
+ * (1, 2, 4 each), one setting at a time: every option twice, in interleaved rounds, each keeping its faster run;
+ * keeps the fastest and restores the state; *best_ms: its mean ms per step. No counterpart in the reference. */
 int swarm_autotune(swarm_ctx* c, int steps, double* best_ms);
 
 /* ---- the safety metrics (class CrowdMetrics, include/swarm/metrics.hpp:181-227; plain contexts) ----

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 namespace swarm {
 
@@ -385,7 +386,6 @@ double HipCrowdSim::autotune(int steps) {
     timing_ = false;
     double best = 0.0;
     std::pair<int, int> bestShape = { physBlock_, physGrid_ };
-    int bestLanes = projLanes_;
     auto trial = [&]() {
         step(2);   // graph build and warm-up
         const double ms = timeSteps(steps) / double(steps);
@@ -393,38 +393,32 @@ double HipCrowdSim::autotune(int steps) {
         parity_ = parity0;
         return ms;
     };
+    // One setting at a time: every option, the one in force included, timed twice in interleaved rounds, each
+    // keeping its faster run, so that a drift of the clock between trials does not decide the choice
+    auto pick = [&](int count, const std::function<void(int)>& apply) {
+        std::vector<double> ms(size_t(count), 0.0);
+        for (int round = 0; round < 2; ++round)
+            for (int i = 0; i < count; ++i) {
+                apply(i);
+                const double t = trial();
+                if (round == 0 || t < ms[size_t(i)]) ms[size_t(i)] = t;
+            }
+        int bi = 0;
+        for (int i = 1; i < count; ++i)
+            if (ms[size_t(i)] < ms[size_t(bi)]) bi = i;
+        apply(bi);
+        best = ms[size_t(bi)];
+        return bi;
+    };
     try {
         copyAll(true);
-        for (const auto& sh : shapes) {
-            setPhysicsShape(sh.first, sh.second);
-            const double ms = trial();
-            if (best == 0.0 || ms < best) { best = ms; bestShape = sh; }
-        }
+        const int shapeCount = int(sizeof(shapes) / sizeof(shapes[0]));
+        bestShape = shapes[pick(shapeCount, [&](int i) { setPhysicsShape(shapes[i].first, shapes[i].second); })];
         // then, at the fastest shape, the projection's lanes per slot, predict's and the collecting iteration's
-        setPhysicsShape(bestShape.first, bestShape.second);
-        for (int lanes : { 1, 2, 4, 8 }) {
-            if (lanes == bestLanes) continue;
-            setProjectionLanes(lanes);
-            const double ms = trial();
-            if (ms < best) { best = ms; bestLanes = lanes; }
-        }
-        setProjectionLanes(bestLanes);
-        int bestPredict = predictLanes_;
-        for (int lanes : { 1, 2, 4 }) {
-            if (lanes == bestPredict) continue;
-            setPredictLanes(lanes);
-            const double ms = trial();
-            if (ms < best) { best = ms; bestPredict = lanes; }
-        }
-        setPredictLanes(bestPredict);
-        int bestCollect = collectLanes_;
-        for (int lanes : { 1, 2, 4 }) {
-            if (lanes == bestCollect) continue;
-            setCollectLanes(lanes);
-            const double ms = trial();
-            if (ms < best) { best = ms; bestCollect = lanes; }
-        }
-        setCollectLanes(bestCollect);
+        const int listLanes[] = { 1, 2, 4, 8 }, fewLanes[] = { 1, 2, 4 };
+        pick(4, [&](int i) { setProjectionLanes(listLanes[i]); });
+        pick(3, [&](int i) { setPredictLanes(fewLanes[i]); });
+        pick(3, [&](int i) { setCollectLanes(fewLanes[i]); });
     } catch (...) {
         (void)hipStreamSynchronize(stream_);
         (void)hipFree(snap);
