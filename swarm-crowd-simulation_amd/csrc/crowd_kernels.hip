@@ -1221,8 +1221,15 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                                                           TapeStep tp = TapeStep{}, ProjBuffers last = ProjBuffers{}) {
     const uint32_t step = ctrl[CTRL_STEP];
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+        // every word of the slot in one load round, before the branches on them
         const uint32_t key = b.skey[slot];
-        const uint32_t agent = b.sid[slot];
+        const float pw = b.pw[slot];
+        const uint32_t listed = b.nbrCount[slot];
+        const SlotHead h = loadHead(b, in, slot);
+        const float sx = b.sx[slot], sy = b.sy[slot], pvx = b.pvx[slot], pvy = b.pvy[slot];
+        holdHead(key, pw, listed, h);
+        asm volatile("" ::"v"(sx), "v"(sy), "v"(pvx), "v"(pvy));
+        const uint32_t agent = h.sid;
         if constexpr (REC) { tp.fin[slot].flags = 0u; tp.fin[slot].n_events = 0u; }
         if (key == INACTIVE_CELL) {
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.w[agent] = rd.w[agent];
@@ -1231,12 +1238,11 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
             const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
             const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
             float x, y, acc;
-            if (b.pw[slot] == 0.0f || mp.projIters == 0) {
-                const float2 bp = in.p[slot];
-                x = bp.x; y = bp.y; acc = in.acc[slot];
+            if (pw == 0.0f || mp.projIters == 0) {
+                x = h.p.x; y = h.p.y; acc = h.acc;
                 if constexpr (REC) tp.mask[size_t(mp.projIters - 1) * size_t(n) + size_t(slot)] = 0ull;
             } else {
-                const uint32_t cnt = useList ? b.nbrCount[slot] : NBR_OVERFLOW;
+                const uint32_t cnt = useList ? listed : NBR_OVERFLOW;
                 if constexpr (REC) {
                     uint64_t mask = 0ull;
                     bool capped = false, planed = false;
@@ -1244,7 +1250,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                         projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc, &mask, &capped, &planed);
                         atomicOr(tp.err, 1u);
                     } else {
-                        projectList(b, in, n, slot, cnt, sr, loadHead(b, in, slot), x, y, acc, &mask, &capped, &planed);
+                        projectList(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                     }
                     const int it = mp.projIters;
                     tp.mask[size_t(it - 1) * size_t(n) + size_t(slot)] = mask;
@@ -1252,24 +1258,24 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                     if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
                 } else {
                     if (cnt == NBR_OVERFLOW) projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc);
-                    else projectList(b, in, n, slot, cnt, sr, loadHead(b, in, slot), x, y, acc);
+                    else projectList(b, in, n, slot, cnt, sr, h, x, y, acc);
                 }
             }
             if constexpr (REC) { last.p[slot] = make_float2(x, y); last.acc[slot] = acc; }
-            if (b.pw[slot] == 0.0f) {   // parked this step: an evacuation event
+            if (pw == 0.0f) {   // parked this step: an evacuation event
                 // the crossing: the input position (inside) and the previous step's input, which the write half
                 // still holds
-                ev.inside[agent] = make_float2(b.sx[slot], b.sy[slot]);
+                ev.inside[agent] = make_float2(sx, sy);
                 ev.before[agent] = make_float2(wr.x[agent], wr.y[agent]);
                 wr.x[agent] = x; wr.y[agent] = y; wr.w[agent] = 0.0f;
                 wr.vx[agent] = 0.0f; wr.vy[agent] = 0.0f; wr.spd[agent] = acc;
-                const int e = exitOf(s, V2{ b.sx[slot], b.sy[slot] });
+                const int e = exitOf(s, V2{ sx, sy });
                 const int ex = e >= 0 ? e : s.E;
                 ev.step[agent] = int(step);
                 ev.exit[agent] = ex;
                 atomicAdd(&ev.perExit[sr.sc * uint32_t(PER_EXIT_STRIDE) + uint32_t(ex)], 1u);
             } else {
-                V2 np{ x, y }, nv{ b.pvx[slot], b.pvy[slot] };
+                V2 np{ x, y }, nv{ pvx, pvy };
                 if (acc > 0.0f) {
                     float wall[3];
                     if constexpr (REC) {
@@ -1277,9 +1283,9 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                         fr->flags = SWARM_REC_MOVED;
                         WallTape wt{ &fr->flags, &fr->n_events, fr->ev, &fr->sweep_lo, tp.err };
                         PlaneRec pl{};
-                        resolveStatic(s, V2{ b.sx[slot], b.sy[slot] }, np, nv, b.sr[slot], wall, wt, &pl);
+                        resolveStatic(s, V2{ sx, sy }, np, nv, h.ri, wall, wt, &pl);
                     } else {
-                        resolveStatic(s, V2{ b.sx[slot], b.sy[slot] }, np, nv, b.sr[slot], wall);
+                        resolveStatic(s, V2{ sx, sy }, np, nv, h.ri, wall);
                     }
                 }
                 wr.x[agent] = np.x; wr.y[agent] = np.y; wr.w[agent] = 1.0f;

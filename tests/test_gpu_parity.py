@@ -97,9 +97,44 @@ def test_room_evacuation_curve():
     assert gper.sum() == (gs >= 0).sum()
 
 
+def test_room_full_evacuation():
+    """The small plain config run to its end (every agent out, within 5,000 steps; graph replays): positions,
+    velocities and exit choices every 500 steps, then the evacuated-count curve step by step, the final
+    evacuation step and every agent's (step, exit), identical to the oracle's."""
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, 4096, 4096, 99)
+    sim = sh.Sim(scene, 4096)
+    sim.upload(a)
+    o = oracle_for(scene, 4096)
+    o.set_state(a)
+    done = 0
+    while done < 5000:
+        sim.step(500)
+        o.step(500)
+        done += 500
+        compare_step(sim, o, f"room evacuation step {done}", cells=False)
+        assert np.array_equal(sim.assign(), o.assign()), f"room evacuation step {done} exit choices"
+        if (sim.evac()[0] >= 0).all():
+            break
+    gs, ge, gper = sim.evac()
+    cs, ce = o.evac()
+    assert (gs >= 0).all(), f"agents left after {done} steps: {(gs < 0).sum()}"
+    assert np.array_equal(gs, cs) and np.array_equal(ge, ce)
+    assert gs.max() == cs.max() and gs.max() < done
+    curve = lambda s: np.cumsum(np.bincount(s, minlength=done))
+    assert np.array_equal(curve(gs), curve(cs))
+    assert np.array_equal(gper[: scene.info.E + 1], np.bincount(ce, minlength=scene.info.E + 1))
+
+
 def test_bowl_63424_agents_bit_exact():
     """The large plain config: 63,424 agents on the 200 x 150 m bowl, sort size 65,536 (2,112 parked)."""
     run_pair(sh.LAYOUT_BOWL, 63424, 65536, 1337, 3)
+
+
+def test_bowl_63424_agents_150_steps():
+    """The large plain config over 150 steps (the crowd leaves its seats; an exit-choice evaluation every 17
+    steps), every quantity at steps 50, 100 and 150."""
+    run_pair(sh.LAYOUT_BOWL, 63424, 65536, 1337, 150, check_every=50)
 
 
 @pytest.mark.parametrize("layout", [sh.LAYOUT_CORRIDOR, sh.LAYOUT_PILLAR, sh.LAYOUT_BOWL_EIGHTH])
