@@ -79,6 +79,9 @@ struct StepBuffers {
     // the projection's contact candidates, slot-interleaved: nbr[j * n + slot], their contact distances nbrD;
     // nbrCount[slot] (NBR_OVERFLOW: more than NBR_CAP, the slot scans its ring instead)
     uint32_t* nbr; float* nbrD; uint32_t* nbrCount;
+    // the step's control words; ctrl[CTRL_NACTIVE] is the active slot count: the inactive key sorts last, so
+    // slots [ctrl[CTRL_NACTIVE], n) are the inactive ones
+    const uint32_t* ctrl;
 };
 
 // the projection's slot-order ping-pong

@@ -1174,7 +1174,8 @@ template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
                                                             ProjBuffers out, int P, int first, TapeStep tp = TapeStep{}, int it = 0,
                                                             int prune = 0) {
-    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+    const int active = int(b.ctrl[CTRL_NACTIVE]);   // the inactive slots follow the active ones and are left alone
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
         // every word of the slot in one load round, before the branches on them
         const uint32_t key = b.skey[slot];
         const float pw = b.pw[slot];
@@ -1220,7 +1221,15 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                                                           int useList, AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* __restrict__ ctrl,
                                                           TapeStep tp = TapeStep{}, ProjBuffers last = ProjBuffers{}) {
     const uint32_t step = ctrl[CTRL_STEP];
+    const int active = int(ctrl[CTRL_NACTIVE]);
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+        if constexpr (REC) { tp.fin[slot].flags = 0u; tp.fin[slot].n_events = 0u; }
+        if (slot >= active) {   // an inactive slot (the inactive key sorts last): its agent is carried over
+            const uint32_t agent = b.sid[slot];
+            wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.w[agent] = rd.w[agent];
+            wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.spd[agent] = rd.spd[agent];
+            continue;
+        }
         // every word of the slot in one load round, before the branches on them
         const uint32_t key = b.skey[slot];
         const float pw = b.pw[slot];
@@ -1230,7 +1239,6 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
         holdHead(key, pw, listed, h);
         asm volatile("" ::"v"(sx), "v"(sy), "v"(pvx), "v"(pvy));
         const uint32_t agent = h.sid;
-        if constexpr (REC) { tp.fin[slot].flags = 0u; tp.fin[slot].n_events = 0u; }
         if (key == INACTIVE_CELL) {
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.w[agent] = rd.w[agent];
             wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.spd[agent] = rd.spd[agent];

@@ -166,6 +166,7 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     ev_.step = alloc<int32_t>(n); ev_.exit = alloc<int32_t>(n); ev_.perExit = alloc<uint32_t>(S * PER_EXIT_STRIDE);
     ev_.inside = alloc<float2>(n); ev_.before = alloc<float2>(n);
     ctrl_ = alloc<uint32_t>(CTRL_WORDS);
+    sb_.ctrl = ctrl_;
     const uint32_t sentinel = uint32_t(n_);
     SWARM_HIP_CHECK(hipMemcpyAsync(grid_.bstart + nb_, &sentinel, 4, hipMemcpyHostToDevice, stream_));
     if (batched_) {   // the per-scenario tables
