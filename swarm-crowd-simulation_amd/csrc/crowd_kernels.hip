@@ -738,7 +738,9 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
 template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers out,
                                                             TapeStep tp = TapeStep{}) {
-    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x)
+    // the taped forward clears every slot's record; otherwise the inactive slots (the last ones) are left alone
+    const int end = REC ? n : int(b.ctrl[CTRL_NACTIVE]);
+    for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < end; slot += gridDim.x * blockDim.x)
         predictSlot<B, REC>(slot, s0, bt, mp, b, out, tp);
 }
 
@@ -747,7 +749,7 @@ template <bool B, int Q>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict_lanes(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b,
                                                                   ProjBuffers out) {
     static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
-    const int total = n * Q;
+    const int total = int(b.ctrl[CTRL_NACTIVE]) * Q;   // the inactive slots (the last ones) are left alone
     for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
         predictSlot<B, false, Q>(t / Q, s0, bt, mp, b, out, TapeStep{}, t % Q);
 }
@@ -974,7 +976,7 @@ template <bool B, int Q, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
                                                                   ProjBuffers out, int P, TapeStep tp = TapeStep{}, int it = 0) {
     static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
-    const int total = n * Q;
+    const int total = int(b.ctrl[CTRL_NACTIVE]) * Q;   // the inactive slots (the last ones) are left alone
     const size_t stride = size_t(n);
     // blockDim and the grid stride are multiples of Q, so a slot's lanes take every branch together
     for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
@@ -1081,7 +1083,7 @@ template <bool B, int Q>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, SceneView s0, BatchView bt, StepBuffers b, ProjBuffers in,
                                                                     ProjBuffers out, int P) {
     static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
-    const int total = n * Q;
+    const int total = int(b.ctrl[CTRL_NACTIVE]) * Q;   // the inactive slots (the last ones) are left alone
     const size_t stride = size_t(n);
     const int groupLo = __lane_id() & ~(Q - 1);   // the slot's first lane in the wave (blockDim is a multiple of 64)
     const float Pf = float(P);
