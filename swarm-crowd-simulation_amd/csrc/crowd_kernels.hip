@@ -786,6 +786,14 @@ __device__ inline V2 overlapCorrection(V2 offset, float dxy2, float D, uint32_t 
     const float shift = 0.5f * (D - dxy);
     return dxy2 <= SEP_STACKED_XY2 ? shift * pairDirection(self, sid[k] - agentBase) : shift * (offset / dxy);
 }
+// The same with the slot's own id read only for a stacked pair (the list iterations do not load it otherwise)
+__device__ inline V2 overlapCorrectionAt(V2 offset, float dxy2, float D, const uint32_t* __restrict__ sid, uint32_t slot, uint32_t k,
+                                         uint32_t agentBase) {
+    const float dxy = sqrtf(dxy2);
+    const float shift = 0.5f * (D - dxy);
+    if (dxy2 <= SEP_STACKED_XY2) return shift * pairDirection(sid[slot] - agentBase, sid[k] - agentBase);
+    return shift * (offset / dxy);
+}
 
 // One iteration for one slot by the ring scan (cpu_crowd_sim.cpp:781-822); with 'collect', also the slot's contact
 // candidates: every projection iteration moves an agent by at most PROJ_MAX_SHIFT_RADII r (the capped correction; the
@@ -854,23 +862,23 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
     finishProjection(pl, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc, capped, planed);
 }
 
-// A slot's own words for one projection iteration. An iteration over the candidate lists is a short chain of
+// A slot's own words for one projection iteration over its candidate list. The iteration is a short chain of
 // dependent load rounds per slot (the slot's words, the candidates' indices, the candidates' positions); loading
-// every word of the slot at once, ahead of the kernel's branches on them, makes the first link one round.
+// every word of the slot at once, ahead of the kernel's branches on them, makes the first link one round. The
+// slot's key (only for the ring scan) and id (only for a stacked pair) are read where they are needed.
 struct SlotHead {
     float2 p;
     float acc, ri;
-    uint32_t sid;
     float4 plane;
 };
 __device__ inline SlotHead loadHead(const StepBuffers& b, const ProjBuffers& in, int slot) {
-    return SlotHead{ in.p[slot], in.acc[slot], b.sr[slot], b.sid[slot], b.planes[slot] };
+    return SlotHead{ in.p[slot], in.acc[slot], b.sr[slot], b.planes[slot] };
 }
 // The compiler otherwise moves each load into the branch that first uses it, one round per branch: this empty
 // statement uses every value at the point of the call, so all the loads issue before it and the wave waits once.
-__device__ inline void holdHead(uint32_t key, float pw, uint32_t listed, const SlotHead& h) {
-    asm volatile("" ::"v"(key), "v"(pw), "v"(listed), "v"(h.p.x), "v"(h.p.y), "v"(h.acc), "v"(h.ri), "v"(h.sid), "v"(h.plane.x),
-                 "v"(h.plane.y), "v"(h.plane.z), "v"(h.plane.w));
+__device__ inline void holdHead(float pw, uint32_t listed, const SlotHead& h) {
+    asm volatile("" ::"v"(pw), "v"(listed), "v"(h.p.x), "v"(h.p.y), "v"(h.acc), "v"(h.ri), "v"(h.plane.x), "v"(h.plane.y),
+                 "v"(h.plane.z), "v"(h.plane.w));
 }
 
 // One iteration for one slot over its contact candidates (cnt >= 1)
@@ -879,7 +887,6 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
                                    bool* capped = nullptr, bool* planed = nullptr) {
     const float2 bp = h.p;
     const float acc = h.acc;
-    const uint32_t self = h.sid - sr.agentBase;
     const float ri = h.ri;
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
@@ -906,7 +913,7 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
             const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
             const float dxy2 = dot2(offset, offset);
             if (dxy2 < contact2(dd[u])) {
-                corr = corr + overlapCorrection(offset, dxy2, dd[u], self, b.sid, kk[u], sr.agentBase);
+                corr = corr + overlapCorrectionAt(offset, dxy2, dd[u], b.sid, uint32_t(slot), kk[u], sr.agentBase);
                 ++nContacts;
                 if (mask) *mask |= 1ull << (j0 + uint32_t(u));
             }
@@ -923,7 +930,6 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
                                         const SlotHead& h, int rem, float& ox, float& oy, float& oacc) {
     const float2 bp = h.p;
     const float acc = h.acc;
-    const uint32_t self = h.sid - sr.agentBase;
     const float ri = h.ri;
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
@@ -949,7 +955,7 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
             const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
             const float dxy2 = dot2(offset, offset);
             if (dxy2 < contact2(dd[u])) {
-                corr = corr + overlapCorrection(offset, dxy2, dd[u], self, b.sid, kk[u], sr.agentBase);
+                corr = corr + overlapCorrectionAt(offset, dxy2, dd[u], b.sid, uint32_t(slot), kk[u], sr.agentBase);
                 ++nContacts;
             }
             const float reach = (dd[u] + remf * (PROJ_MAX_SHIFT_RADII * dd[u])) * 1.001f + 1e-6f;
@@ -1179,12 +1185,10 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
     const int active = int(b.ctrl[CTRL_NACTIVE]);   // the inactive slots follow the active ones and are left alone
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < active; slot += gridDim.x * blockDim.x) {
         // every word of the slot in one load round, before the branches on them
-        const uint32_t key = b.skey[slot];
         const float pw = b.pw[slot];
         const uint32_t listed = b.nbrCount[slot];
         const SlotHead h = loadHead(b, in, slot);
-        holdHead(key, pw, listed, h);
-        if (key == INACTIVE_CELL) continue;
+        holdHead(pw, listed, h);
         float x, y, acc;
         uint64_t mask = 0ull;
         bool capped = false, planed = false;
@@ -1195,13 +1199,14 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
             const uint32_t cnt = first ? NBR_OVERFLOW : listed;
             if constexpr (REC) {
                 if (cnt == NBR_OVERFLOW) {
-                    projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc, &mask, &capped, &planed);
+                    projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, b.skey[slot], sr, P, first != 0, x, y, acc, &mask, &capped,
+                                &planed);
                     if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
                 } else {
                     projectList(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                 }
             } else {
-                if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, first != 0, x, y, acc);
+                if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, b.skey[slot], sr, P, first != 0, x, y, acc);
                 else if (prune > 0) projectListPrune(b, in, n, slot, cnt, sr, h, prune, x, y, acc);
                 else projectList(b, in, n, slot, cnt, sr, h, x, y, acc);
             }
@@ -1234,13 +1239,13 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
         }
         // every word of the slot in one load round, before the branches on them
         const uint32_t key = b.skey[slot];
+        const uint32_t agent = b.sid[slot];
         const float pw = b.pw[slot];
         const uint32_t listed = b.nbrCount[slot];
         const SlotHead h = loadHead(b, in, slot);
         const float sx = b.sx[slot], sy = b.sy[slot], pvx = b.pvx[slot], pvy = b.pvy[slot];
-        holdHead(key, pw, listed, h);
-        asm volatile("" ::"v"(sx), "v"(sy), "v"(pvx), "v"(pvy));
-        const uint32_t agent = h.sid;
+        holdHead(pw, listed, h);
+        asm volatile("" ::"v"(key), "v"(agent), "v"(sx), "v"(sy), "v"(pvx), "v"(pvy));
         if (key == INACTIVE_CELL) {
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.w[agent] = rd.w[agent];
             wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.spd[agent] = rd.spd[agent];
