@@ -18,6 +18,7 @@ constexpr int PHYS_MAX_BLOCK = 256;
 constexpr int XCDS = 8;            // accelerator complex dies of an MI355X (one L2 each)
 constexpr int NBR_CAP = 48;        // contact candidates per slot
 constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projection thread
+constexpr int PROJ_CHUNK_ENSEMBLE = 4;   // the same in the batched step's list iterations (more waves per CU)
 constexpr int PROJ_PRUNE_EVERY = 8; // projection iterations between candidate prunings (0: never)
 constexpr int PROJ_LANES = 1;       // lanes per slot in the projection iterations that do not prune (autotuned)
 constexpr int PROJ_COLLECT_LANES = 4;   // lanes per slot in the first projection iteration (ring scan + candidates)

@@ -881,7 +881,8 @@ __device__ inline void holdHead(float pw, uint32_t listed, const SlotHead& h) {
                  "v"(h.plane.z), "v"(h.plane.w));
 }
 
-// One iteration for one slot over its contact candidates (cnt >= 1)
+// One iteration for one slot over its contact candidates (cnt >= 1), CH of them loaded together
+template <int CH = PROJ_CHUNK>
 __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
                                    const SlotHead& h, float& ox, float& oy, float& oacc, uint64_t* mask = nullptr,
                                    bool* capped = nullptr, bool* planed = nullptr) {
@@ -894,21 +895,21 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
     // candidates in chunks: the chunk's index and position loads are all in flight before the first use; the
     // sum still runs in candidate order. A chunk's entries past the count load the last candidate again (no
     // branch around the loads) and are never used.
-    for (uint32_t j0 = 0; j0 < cnt; j0 += PROJ_CHUNK) {
-        uint32_t kk[PROJ_CHUNK];
-        float dd[PROJ_CHUNK];
-        float2 pp[PROJ_CHUNK];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += CH) {
+        uint32_t kk[CH];
+        float dd[CH];
+        float2 pp[CH];
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) {
+        for (int u = 0; u < CH; ++u) {
             const uint32_t j = j0 + uint32_t(u) < cnt ? j0 + uint32_t(u) : cnt - 1u;
             const size_t at = size_t(j) * stride + size_t(slot);
             kk[u] = b.nbr[at];
             dd[u] = b.nbrD[at];
         }
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) pp[u] = in.p[kk[u]];
+        for (int u = 0; u < CH; ++u) pp[u] = in.p[kk[u]];
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) {
+        for (int u = 0; u < CH; ++u) {
             if (j0 + uint32_t(u) >= cnt) break;
             const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
             const float dxy2 = dot2(offset, offset);
@@ -926,6 +927,7 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
 // iterations left (this one included) each agent moves by at most rem PROJ_MAX_SHIFT_RADII r, so a pair farther apart
 // than D + rem PROJ_MAX_SHIFT_RADII D (with the collection's slack) cannot touch again. The kept candidates keep
 // their order, so every later sum is the unpruned sum bit for bit (the dropped terms are never contacts).
+template <int CH = PROJ_CHUNK>
 __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t cnt, const ScnRef& sr,
                                         const SlotHead& h, int rem, float& ox, float& oy, float& oacc) {
     const float2 bp = h.p;
@@ -936,21 +938,21 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
     const size_t stride = size_t(n);
     const float remf = float(rem);
     uint32_t kept = 0;
-    for (uint32_t j0 = 0; j0 < cnt; j0 += PROJ_CHUNK) {
-        uint32_t kk[PROJ_CHUNK];
-        float dd[PROJ_CHUNK];
-        float2 pp[PROJ_CHUNK];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += CH) {
+        uint32_t kk[CH];
+        float dd[CH];
+        float2 pp[CH];
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) {
+        for (int u = 0; u < CH; ++u) {
             const bool in_ = j0 + uint32_t(u) < cnt;
             const size_t at = size_t(j0 + uint32_t(u)) * stride + size_t(slot);
             kk[u] = in_ ? b.nbr[at] : uint32_t(slot);
             dd[u] = in_ ? b.nbrD[at] : 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) pp[u] = in.p[kk[u]];
+        for (int u = 0; u < CH; ++u) pp[u] = in.p[kk[u]];
 #pragma unroll
-        for (int u = 0; u < PROJ_CHUNK; ++u) {
+        for (int u = 0; u < CH; ++u) {
             if (j0 + uint32_t(u) >= cnt) break;
             const V2 offset = V2{ bp.x, bp.y } - V2{ pp[u].x, pp[u].y };
             const float dxy2 = dot2(offset, offset);
@@ -971,6 +973,11 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
     if (kept != cnt) b.nbrCount[slot] = kept;
     finishProjection(h.plane, bp.x, bp.y, acc, corr, nContacts, ri, ox, oy, oacc);
 }
+
+// candidates loaded together in a list iteration: the plain step's launches hold one wave per SIMD and gain from
+// more loads in flight; the batched step's hold many and are bound by bandwidth (same-box: 8 for the plain bowl,
+// 313 us per step against 323 us at 4; 4 for the 256-scenario ensemble, 1.279 ms against 1.352 ms at 8)
+template <bool B> __host__ __device__ constexpr int listChunk() { return B ? PROJ_CHUNK_ENSEMBLE : PROJ_CHUNK; }
 
 // A later iteration with Q lanes per slot (the production step's iterations that do not prune). Lane l of a slot
 // loads and tests candidates j = l, l + Q, ..., PROJ_CHUNK of them per round, so Q x PROJ_CHUNK candidates are in
@@ -1203,12 +1210,12 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
                                 &planed);
                     if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
                 } else {
-                    projectList(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
+                    projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                 }
             } else {
                 if (cnt == NBR_OVERFLOW) projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, b.skey[slot], sr, P, first != 0, x, y, acc);
-                else if (prune > 0) projectListPrune(b, in, n, slot, cnt, sr, h, prune, x, y, acc);
-                else projectList(b, in, n, slot, cnt, sr, h, x, y, acc);
+                else if (prune > 0) projectListPrune<listChunk<B>()>(b, in, n, slot, cnt, sr, h, prune, x, y, acc);
+                else projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc);
             }
         }
         out.p[slot] = make_float2(x, y);
@@ -1265,7 +1272,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                         projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc, &mask, &capped, &planed);
                         atomicOr(tp.err, 1u);
                     } else {
-                        projectList(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
+                        projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                     }
                     const int it = mp.projIters;
                     tp.mask[size_t(it - 1) * size_t(n) + size_t(slot)] = mask;
@@ -1273,7 +1280,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                     if (planed) tp.planeBits[slot] |= 1ull << (it - 1);
                 } else {
                     if (cnt == NBR_OVERFLOW) projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc);
-                    else projectList(b, in, n, slot, cnt, sr, h, x, y, acc);
+                    else projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc);
                 }
             }
             if constexpr (REC) { last.p[slot] = make_float2(x, y); last.acc[slot] = acc; }
