@@ -45,6 +45,7 @@ extern "C" {
 #define SWARM_REC_SWEEP 0x800u     /* sweepBack moved the position */
 #define SWARM_REC_SWEEP_VEL 0x1000u
 #define SWARM_REC_MOVED 0x2000u    /* final: the projection moved the agent, resolveStatic ran again */
+#define SWARM_REC_GOAL_WALL 0x4000u /* e_goal's wall rule applied: e = unit(e0 - n (e0 . n)), n = goal_wall */
 
 /* A wall push: x' = x + a (r - t(x)), t(x) = t0 - g.(x - x0) / (g.u); vel: v' = v - a (a.v) */
 typedef struct {
@@ -63,6 +64,7 @@ typedef struct {
     uint32_t plane_kind;        /* 0 none; 1 offset n.x; 2 offset n.x - (t(x) - r) */
     float plane_nx, plane_ny;
     float pc_ux, pc_uy, pc_gx, pc_gy, pc_x0, pc_y0, pc_t0;   /* the plane's clearance model (kind 2) */
+    float goal_wall_x, goal_wall_y;   /* the wall direction of e_goal's wall rule (SWARM_REC_GOAL_WALL) */
     swarm_wall_event ev[SWARM_TAPE_MAX_WALL];
 } swarm_predict_rec;
 
@@ -101,7 +103,11 @@ typedef struct {
  * (swarm_adjoint_set_tape_budget), or checkpoint_every >= K, is taped whole in the forward and never recomputed. */
 int swarm_adjoint_forward(swarm_ctx* c, int K, int checkpoint_every, float radius_scale, float overlap_weight, double* loss);
 /* After swarm_adjoint_forward: recomputes each earlier segment from its checkpoint with the tape and runs the
- * adjoint kernels backwards. grad_v0: n (agent order); grad_globals: dL/dtau_s, dL/dradius_scale, dL/dwall_gain. */
+ * adjoint kernels backwards. grad_v0: n (agent order); grad_globals: dL/dtau_s, dL/dradius_scale, dL/dwall_gain.
+ * Afterwards the context holds the forward's final state again (a recompute puts it back). Returns
+ * SWARM_ENONFINITE, with both outputs written, when any gradient value is inf or NaN (the pathwise derivative
+ * overflowed FP32: a horizon beyond what FP32 holds in a chaotic crowd). swarm_upload, swarm_set_proj_iters and a
+ * swarm_set_model that changes the model end the forward's run: a backward then returns SWARM_EINVAL. */
 int swarm_adjoint_backward(swarm_ctx* c, float* grad_v0, double* grad_globals);
 /* Device bytes the context's adjoint holds (tape, adjoint buffers, checkpoints). */
 int swarm_adjoint_memory(swarm_ctx* c, size_t* bytes);

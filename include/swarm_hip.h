@@ -27,6 +27,7 @@ extern "C" {
 #define SWARM_ENOMEM (-3)   /* device allocation failed */
 #define SWARM_ESCENE (-4)   /* invalid scene */
 #define SWARM_EHIP (-5)     /* HIP runtime error or no device */
+#define SWARM_ENONFINITE (-6)  /* swarm_adjoint_backward: a gradient value is inf or NaN (the outputs are still written) */
 
 typedef struct swarm_scene swarm_scene;
 typedef struct swarm_ctx swarm_ctx;
@@ -70,6 +71,7 @@ typedef struct {
     int K, E;
     float seconds_per_unit, dist_quantum;
     double build_seconds;
+    int n_walls, wall_bin_entries;   /* wall boxes; entries of their index on the hash grid */
 } swarm_scene_info;
 
 /* nav::build + buildChoiceTable (src/nav_field.cpp:344-366, src/exit_choice.cpp:143-317) and the SDF bake
@@ -82,6 +84,10 @@ int swarm_scene_info_get(const swarm_scene* s, swarm_scene_info* info);
  * nav_res_x*nav_res_y; nav_slot, nav_dist cells*K; inv_cap_s E; exit_rects 4E. */
 int swarm_scene_read(const swarm_scene* s, float* sdf, float* sdf_min, uint32_t* nav_info, uint32_t* nav_slot,
                      uint16_t* nav_dist, float* inv_cap_s, float* stair, float* exit_rects);
+/* The wall boxes (n_walls x (lox, loy, hix, hiy)) and their index on the hash grid (per hash cell, ascending, the
+ * walls e_goal's wall rule can reach from it: bin_start hash_nx * hash_ny + 1, bin_list wall_bin_entries). Any
+ * pointer may be NULL. The reference queries its analytic venue's walls directly (seated::wall, src/venue.cpp:763-813). */
+int swarm_scene_read_walls(const swarm_scene* s, float* rects, uint32_t* bin_start, uint32_t* bin_list);
 /* The floor's safety-metric grid (MetricVenue, metrics.hpp:145-162; its walkable area per cell as
  * src/metrics.cpp:143-164 samples it, there over the terrace's centred 64 x 64 m square): 1 m cells covering the
  * floor from whole metres, g = xy * cells_per_unit + off per axis, and each cell's walkable area in m^2 from
@@ -149,6 +155,12 @@ int swarm_read_prev_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy
  * (< swarm_num_cells) or 0xFFFFFFFF (inactive); out: the sorted keys and the permutation (ascending (key, index),
  * the stable counting sort's order). Leaves the context's own grid readback holding this sort. */
 int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, uint32_t* ids);
+/* The step's navigation lookups at m caller points, on the GPU (the reference's probe dispatch of physics.comp's
+ * navChoiceCell / navChoiceDirection / navChoiceGoal, tools/test_gpu.cpp:4155-4191): per point its nav cell, the table's
+ * direction to exit_id[i] (dir: 2 floats) and e_goal with the wall rule at radius r[i] (goal: 2 floats). Scenario 0's
+ * scene; synchronises. */
+int swarm_nav_probe(swarm_ctx* c, int m, const float* x, const float* y, const float* r, const uint32_t* exit_id, int32_t* cell,
+                    float* dir, float* goal);
 /* step counter, active agents at the last step's input, running sum of active agents over steps */
 int swarm_read_control(swarm_ctx* c, uint32_t* step, uint32_t* n_active, uint64_t* active_sum);
 /* CrowdSim::setTimingEnabled / readTimings (crowd_sim.hpp:142-143): per-pass HIP events (steps then run

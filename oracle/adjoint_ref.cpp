@@ -149,6 +149,11 @@ void stepModel(const adjref& A, const StepRec& t, std::vector<Dual<N>>& X, std::
         else {
             const W2<N> d = cst2<N>(pr.anchor_x, pr.anchor_y) - p;
             e = d / len(d);
+            if (f & SWARM_REC_GOAL_WALL) {   // e_goal's wall rule with the recorded wall direction
+                const W2<N> nw = cst2<N>(pr.goal_wall_x, pr.goal_wall_y);
+                const W2<N> e1 = e - nw * dot(e, nw);
+                e = e1 / len(e1);
+            }
         }
         W2<N> sep = cst2<N>(0.0, 0.0);
         for (const SepLink& L : t.sep[size_t(i)]) {

@@ -34,6 +34,7 @@ constexpr float PROJ_RELAX = 1.0f;
 constexpr float PROJ_MAX_SHIFT_RADII = 0.05f;
 constexpr float W_OBS = 10.0f;
 constexpr float GOAL_DEADZONE = 1e-4f;
+constexpr float WALL_NONE = 1e9f;   // venue.hpp:251
 constexpr float OBSTACLE_RANGE = 0.025f;
 constexpr float OBSTACLE_MIN_NORMAL = 0.01f;
 constexpr float OBSTACLE_GRAD_H = 0.0015f;
@@ -168,7 +169,7 @@ struct SlotState { float x, y, w, vx, vy, acc; };   // position xy, active flag,
 struct oracle_ctx {
     // scene (copied)
     oracle_scene sc;
-    std::vector<float> sdf, invCapS, stair, exitRects;
+    std::vector<float> sdf, invCapS, stair, exitRects, walls;
     std::vector<uint32_t> navInfo, navSlot;
     std::vector<uint16_t> navDist;
     oracle_params par;
@@ -269,9 +270,53 @@ struct oracle_ctx {
         }
         return V2{ 0.0f, 0.0f };
     }
+    // seated::wall (venue.cpp:763-813) on a rectangle world: over every wall box, the nearest point within reach
+    // (strict <, so the first of equal distances); (direction away from it, clearance), WALL_NONE when none. On the
+    // box's boundary the direction is the normal of the face p lies on (venue.cpp:806-810); strictly inside a box, the
+    // normal of its nearest face and the negative depth.
+    G3 wallQuery(V2 p, float reach) const {
+        float best = reach * reach;
+        int bk = -1;
+        V2 bq{ 0.0f, 0.0f };
+        for (size_t k = 0; k < walls.size() / 4; ++k) {
+            const float* r = &walls[4 * k];
+            const V2 q{ std::min(std::max(p.x, r[0]), r[2]), std::min(std::max(p.y, r[1]), r[3]) };
+            const V2 dd = p - q;
+            const float d2 = dot(dd, dd);
+            if (d2 < best) { best = d2; bq = q; bk = int(k); }
+        }
+        if (bk < 0) return G3{ 0.0f, 0.0f, WALL_NONE };
+        const float d = std::sqrt(best);
+        if (d > 0.0f) { const V2 u = (p - bq) / d; return G3{ u.x, u.y, d }; }
+        const float* r = &walls[4 * size_t(bk)];
+        const V2 h{ p.x <= r[0] ? -1.0f : (p.x >= r[2] ? 1.0f : 0.0f), p.y <= r[1] ? -1.0f : (p.y >= r[3] ? 1.0f : 0.0f) };
+        if (h.x != 0.0f || h.y != 0.0f) { const V2 u = h / length(h); return G3{ u.x, u.y, 0.0f }; }
+        float dep = p.x - r[0];
+        V2 u{ -1.0f, 0.0f };
+        if (r[2] - p.x < dep) { dep = r[2] - p.x; u = V2{ 1.0f, 0.0f }; }
+        if (p.y - r[1] < dep) { dep = p.y - r[1]; u = V2{ 0.0f, -1.0f }; }
+        if (r[3] - p.y < dep) { dep = r[3] - p.y; u = V2{ 0.0f, 1.0f }; }
+        return G3{ u.x, u.y, -dep };
+    }
+    // choiceGoal (exit_choice.cpp:568-578): the direction's component into the nearest wall within FEELER_REACH
+    // (FEELER_REACH_RADII r per agent) removed, the rest renormalized
+    V2 choiceGoal(V2 p, float rad, uint32_t exit, bool* applied = nullptr) const {
+        V2 e = choiceDirection(p, exit);
+        const float reach = FEELER_REACH_RADII * rad;
+        const G3 w = wallQuery(p, reach);
+        const V2 n{ w.gx, w.gy };
+        const float into = dot(e, n);
+        if (w.w < reach && into < 0.0f) {
+            e = e - n * into;
+            const float l = length(e);
+            e = l > SAFE_NORMALIZE_EPS ? e / l : V2{ 0.0f, 0.0f };
+            if (applied) *applied = true;
+        }
+        return e;
+    }
     V2 goalDirection(V2 p, uint32_t agent) const {   // cpu_crowd_sim.cpp:213-218
         if (drive) return V2{ driveX[agent], driveY[agent] };
-        return choiceDirection(p, assign[agent]);
+        return choiceGoal(p, r[agent], assign[agent]);
     }
     float estimatedTime(float dist, uint32_t exit, uint32_t queued) const {   // exit_choice.cpp:580-584
         const float walk = dist * sc.secondsPerUnit;
@@ -681,6 +726,39 @@ double msBetween(std::chrono::steady_clock::time_point a, std::chrono::steady_cl
 
 extern "C" {
 
+void oracle_goal(const oracle_ctx* c, int n, const float* x, const float* y, const float* r, const uint32_t* exit, int wall_rule,
+                 float* ex, float* ey, uint8_t* applied) {
+    for (int i = 0; i < n; ++i) {
+        bool a = false;
+        const V2 p{ x[i], y[i] };
+        const V2 e = wall_rule ? c->choiceGoal(p, r[i], exit[i], &a) : c->choiceDirection(p, exit[i]);
+        ex[i] = e.x;
+        ey[i] = e.y;
+        if (applied) applied[i] = a ? 1 : 0;
+    }
+}
+
+void oracle_sincos(int n, const float* x, float* s, float* c) {
+    for (int i = 0; i < n; ++i) sincosCephes(x[i], &s[i], &c[i]);
+}
+
+void oracle_choice_cell(const oracle_ctx* c, int n, const float* x, const float* y, int32_t* cell) {
+    for (int i = 0; i < n; ++i) cell[i] = c->choiceCell(V2{ x[i], y[i] });
+}
+
+void oracle_exit_at(const oracle_ctx* c, int n, const float* x, const float* y, int32_t* exit) {
+    for (int i = 0; i < n; ++i) exit[i] = c->exitOf(V2{ x[i], y[i] });
+}
+
+void oracle_wall(const oracle_ctx* c, int n, const float* x, const float* y, float reach, float* wx, float* wy, float* wd) {
+    for (int i = 0; i < n; ++i) {
+        const G3 w = c->wallQuery(V2{ x[i], y[i] }, reach);
+        wx[i] = w.gx;
+        wy[i] = w.gy;
+        wd[i] = w.w;
+    }
+}
+
 oracle_ctx* oracle_create(const oracle_scene* s, const oracle_params* p, int n) {
     if (!s || !p || n <= 0) return nullptr;
     oracle_ctx* c = new oracle_ctx();
@@ -696,6 +774,8 @@ oracle_ctx* oracle_create(const oracle_scene* s, const oracle_params* p, int n) 
     c->stair.assign(s->stair, s->stair + nc);
     c->invCapS.assign(s->invCapS, s->invCapS + s->E);
     c->exitRects.assign(s->exitRects, s->exitRects + size_t(s->E) * 4);
+    if (s->nWalls > 0 && s->wallRects) c->walls.assign(s->wallRects, s->wallRects + size_t(s->nWalls) * 4);
+    c->sc.wallRects = nullptr;
     c->sc.sdf = nullptr; c->sc.navInfo = nullptr; c->sc.navSlot = nullptr; c->sc.navDist = nullptr;
     c->sc.stair = nullptr; c->sc.invCapS = nullptr; c->sc.exitRects = nullptr;
     for (int h = 0; h < 2; ++h) {

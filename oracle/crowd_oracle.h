@@ -34,6 +34,8 @@ typedef struct {
     float secondsPerUnit, distQuantum;
     const float* stair;                    /* per nav cell: free-speed factor */
     const float* exitRects;                /* E x (lox, loy, hix, hiy) */
+    int nWalls;                            /* wall boxes (e_goal's wall rule queries them exactly) */
+    const float* wallRects;                /* nWalls x (lox, loy, hix, hiy) */
 } oracle_scene;
 
 typedef struct {
@@ -71,6 +73,20 @@ void oracle_get_evac(const oracle_ctx* c, int32_t* step, int32_t* exitId);
 int oracle_num_steps(const oracle_ctx* c);
 /* wall clock of the last step per stage, ms: hash, sort, cells, choice, predict, project, final, total */
 void oracle_get_times(const oracle_ctx* c, double* t8);
+/* e_goal of agents at (x, y) with radius r assigned to exit[i]: the navigation table's direction (choiceDirection,
+ * exit_choice.cpp:557-566), with the wall rule when wall_rule != 0 (choiceGoal, :568-578), n points; applied[i]: the
+ * wall rule changed it. The choice state is not touched. */
+void oracle_goal(const oracle_ctx* c, int n, const float* x, const float* y, const float* r, const uint32_t* exit, int wall_rule,
+                 float* ex, float* ey, uint8_t* applied);
+/* The stacked-pair direction's sine and cosine (pairDirection's cos/sin, cpu_crowd_sim.cpp:123-129) at n angles */
+void oracle_sincos(int n, const float* x, float* s, float* c);
+/* choiceCell (exit_choice.cpp:512-548) at n points */
+void oracle_choice_cell(const oracle_ctx* c, int n, const float* x, const float* y, int32_t* cell);
+/* the exit rectangle holding each point (RectGeometry::exitAt, nav_field.cpp:155-159; -1: none), n points */
+void oracle_exit_at(const oracle_ctx* c, int n, const float* x, const float* y, int32_t* exit);
+/* The exact wall query (seated::wall, venue.cpp:763-813, over the scene's wall boxes): direction away from the
+ * nearest wall within reach and the clearance (WALL_NONE 1e9 when none), n points */
+void oracle_wall(const oracle_ctx* c, int n, const float* x, const float* y, float reach, float* wx, float* wy, float* wd);
 /* exported model constants, for the constant-parity test */
 int oracle_constants(float* out, int cap);
 

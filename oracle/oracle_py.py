@@ -26,7 +26,8 @@ class OracleScene(C.Structure):
                 ("navRX", C.c_int), ("navRY", C.c_int), ("navOX", C.c_float), ("navOY", C.c_float),
                 ("navCell", C.c_float), ("navInvCell", C.c_float), ("K", C.c_int), ("E", C.c_int),
                 ("navInfo", C.c_void_p), ("navSlot", C.c_void_p), ("navDist", C.c_void_p), ("invCapS", C.c_void_p),
-                ("secondsPerUnit", C.c_float), ("distQuantum", C.c_float), ("stair", C.c_void_p), ("exitRects", C.c_void_p)]
+                ("secondsPerUnit", C.c_float), ("distQuantum", C.c_float), ("stair", C.c_void_p), ("exitRects", C.c_void_p),
+                ("nWalls", C.c_int), ("wallRects", C.c_void_p)]
 
 
 class OracleParams(C.Structure):
@@ -85,6 +86,8 @@ def scene_struct(info, tables):
     s.invCapS = tables["inv_cap_s"].ctypes.data
     s.stair = tables["stair"].ctypes.data
     s.exitRects = tables["exit_rects"].ctypes.data
+    s.nWalls = int(getattr(i, "n_walls", 0))
+    s.wallRects = tables["walls"].ctypes.data if s.nWalls > 0 else None
     return s
 
 
@@ -155,6 +158,45 @@ class Oracle:
         st, cn = np.zeros(self.cells, np.uint32), np.zeros(self.cells, np.uint32)
         lib().oracle_get_cells(self.h, st, cn)
         return st, cn
+
+    def goal(self, x, y, r, exit, wall_rule=True):
+        """e_goal at points: the table's direction (choiceDirection), with the wall rule (choiceGoal) unless
+        wall_rule is False: (ex, ey, applied)."""
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        x, y = f(x), f(y)
+        r = f(np.broadcast_to(np.asarray(r, np.float32), x.shape))
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(exit, np.uint32), x.shape))
+        ex, ey, ap = np.zeros(x.size, np.float32), np.zeros(x.size, np.float32), np.zeros(x.size, np.uint8)
+        L = lib()
+        L.oracle_goal.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 3
+        L.oracle_goal(self.h, x.size, x.ctypes.data, y.ctypes.data, r.ctypes.data, e.ctypes.data, int(bool(wall_rule)),
+                      ex.ctypes.data, ey.ctypes.data, ap.ctypes.data)
+        return ex, ey, ap.astype(bool)
+
+    def choice_cell(self, x, y):
+        x, y = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32)
+        out = np.zeros(x.size, np.int32)
+        L = lib()
+        L.oracle_choice_cell.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        L.oracle_choice_cell(self.h, x.size, x.ctypes.data, y.ctypes.data, out.ctypes.data)
+        return out
+
+    def exit_at(self, x, y):
+        x, y = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32)
+        out = np.zeros(x.size, np.int32)
+        L = lib()
+        L.oracle_exit_at.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        L.oracle_exit_at(self.h, x.size, x.ctypes.data, y.ctypes.data, out.ctypes.data)
+        return out
+
+    def wall(self, x, y, reach):
+        """The exact wall query at points: (direction x, direction y, clearance; 1e9 when no wall within reach)."""
+        x, y = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32)
+        wx, wy, wd = (np.zeros(x.size, np.float32) for _ in range(3))
+        L = lib()
+        L.oracle_wall.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float] + [C.c_void_p] * 3
+        L.oracle_wall(self.h, x.size, x.ctypes.data, y.ctypes.data, reach, wx.ctypes.data, wy.ctypes.data, wd.ctypes.data)
+        return wx, wy, wd
 
     def assign(self):
         a = np.zeros(self.n, np.uint32)

@@ -171,6 +171,74 @@ void putRects(const std::string& p, const std::vector<Rect>& rs) {
     put(p, 'f', v, { uint64_t(rs.size()), 4 });
 }
 
+// The reference's own lookups at probe points of a rectangle world: nav::choiceCell, nav::choiceDirection for every
+// exit (src/exit_choice.cpp:512-566) and RectGeometry::exitAt (src/nav_field.cpp:155-159). Points: a seeded uniform
+// sample over the grid, points beside every wall face (the NOFIT band, its 3 x 3 and 5 x 5 fallbacks), anchor
+// centres and points within and just beyond GOAL_DEADZONE of them, and every exit rectangle's corners, edge midpoints
+// and their neighbours one float step inside and outside.
+void probeCase(const std::string& tag, const nav::RectGeometry& geo, const nav::Grid& grid, const std::vector<Rect>& walls,
+               const std::vector<Rect>& exits, const nav::ChoiceTable& t) {
+    std::vector<float> px, py;
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ uint64_t(tag.size());
+    auto uni = [&]() {   // splitmix64 -> [0, 1)
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return double(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const double lo = grid.origin, span = double(grid.res) * double(grid.cell);
+    for (int i = 0; i < 3000; ++i) { px.push_back(float(lo + span * uni())); py.push_back(float(lo + span * uni())); }
+    const float offs[] = { 0.01f, 0.04f, 0.09f, 0.16f, 0.26f, 0.41f };
+    for (const Rect& w : walls)
+        for (float o : offs)
+            for (int k = 0; k < 4; ++k) {
+                const float a = float(uni());
+                const float fx = w.lo.x + a * (w.hi.x - w.lo.x), fy = w.lo.y + a * (w.hi.y - w.lo.y);
+                if (k == 0) { px.push_back(w.lo.x - o); py.push_back(fy); }
+                if (k == 1) { px.push_back(w.hi.x + o); py.push_back(fy); }
+                if (k == 2) { px.push_back(fx); py.push_back(w.lo.y - o); }
+                if (k == 3) { px.push_back(fx); py.push_back(w.hi.y + o); }
+            }
+    for (int i = 0; i < 200; ++i) {   // anchors: the first candidate's anchor of a random reachable cell
+        const int c = int(uni() * double(t.index.size()));
+        if (c >= int(t.index.size()) || !t.reachable(size_t(c))) continue;
+        const uint32_t a = t.at(size_t(c)).slot[0] & nav::CHOICE_ANCHOR_MASK;
+        const glm::vec2 ac = nav::anchorCentre(t, a);
+        const float d[] = { 0.0f, 3e-5f, 9.9e-5f, 1.01e-4f, 3e-4f };
+        for (float dd : d) { px.push_back(ac.x + dd); py.push_back(ac.y - 0.5f * dd); }
+    }
+    for (const Rect& e : exits) {
+        const float xs[] = { e.lo.x, 0.5f * (e.lo.x + e.hi.x), e.hi.x }, ys[] = { e.lo.y, 0.5f * (e.lo.y + e.hi.y), e.hi.y };
+        for (float x : xs)
+            for (float y : ys)
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        px.push_back(dx < 0 ? std::nextafter(x, -1e9f) : dx > 0 ? std::nextafter(x, 1e9f) : x);
+                        py.push_back(dy < 0 ? std::nextafter(y, -1e9f) : dy > 0 ? std::nextafter(y, 1e9f) : y);
+                    }
+    }
+    const int E = geo.exitCount();
+    std::vector<int32_t> cell, at;
+    std::vector<float> dx, dy;
+    for (size_t i = 0; i < px.size(); ++i) {
+        const glm::vec2 q(px[i], py[i]);
+        cell.push_back(nav::choiceCell(t, q));
+        at.push_back(geo.exitAt(q));
+        for (int e = 0; e < E; ++e) {
+            const glm::vec2 d = nav::choiceDirection(t, q, uint32_t(e));
+            dx.push_back(d.x);
+            dy.push_back(d.y);
+        }
+    }
+    put(tag + "/probe/x", 'f', px);
+    put(tag + "/probe/y", 'f', py);
+    put(tag + "/probe/cell", 'i', cell);
+    put(tag + "/probe/exit_at", 'i', at);
+    put(tag + "/probe/dir_x", 'f', dx, { uint64_t(px.size()), uint64_t(E) });
+    put(tag + "/probe/dir_y", 'f', dy, { uint64_t(px.size()), uint64_t(E) });
+}
+
 // A rectangle world's grid (cls, out, exit region), per-exit fields and candidate table
 void navCase(const std::string& tag, const std::vector<Rect>& walls, const std::vector<Rect>& exits, const std::vector<float>& widths,
              const nav::Grid& grid, float body) {
@@ -201,6 +269,7 @@ void navCase(const std::string& tag, const std::vector<Rect>& walls, const std::
     put(tag + "/table/dists", 'u', dists, { uint64_t(t.entries.size()), uint64_t(nav::CHOICE_DIST_WORDS) });
     put(tag + "/table/scalars", 'f', std::vector<float>{ t.secondsPerUnit, t.distQuantum });
     put(tag + "/table/inv_cap_s", 'f', t.invCapS);
+    probeCase(tag, geo, grid, walls, exits, t);
 }
 
 GpuBoid agentAt(float x, float y, bool active = true) {

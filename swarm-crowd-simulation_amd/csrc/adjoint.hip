@@ -47,7 +47,7 @@ __device__ inline void projectPhaseA(const AdjStep& t, const BatchView& bt, int 
     const uint64_t mask = t.mask[size_t(it - 1) * n + size_t(i)];
     const float2 qi = t.q[size_t(it - 1) * n + size_t(i)];
     const float ri = t.sr[i];
-    const ScnRef sr = scnOfSlot<B>(bt, uint32_t(i));
+    const ScnRef sr = scnOfAgent<B>(bt, t.sid[i]);   // the tape's own slot order (the live slot map is the last step's)
     const uint32_t self = t.sid[i] - sr.agentBase;
     V2 u{ 0.0f, 0.0f };
     int nc = 0;
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_adj_project(AdjStep t, BatchView bt, Ad
         else g.gq[i] = make_float2(gq.x, gq.y);
         return;
     }
-    const ScnRef sr = scnOfSlot<B>(bt, uint32_t(i));
+    const ScnRef sr = scnOfAgent<B>(bt, t.sid[i]);   // the tape's own slot order (the live slot map is the last step's)
     const uint32_t self = t.sid[i] - sr.agentBase;
     const float2 qi = t.q[size_t(it - 1) * n + size_t(i)];
     const float ri = t.sr[i];
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) k_adj_predict_a(AdjStep t, SceneView s0, 
     if (live) {
         const swarm_predict_rec& pr = t.pred[i];
         const uint32_t f = pr.flags;
-        const ScnRef sr = scnOfSlot<B>(bt, uint32_t(i));
+        const ScnRef sr = scnOfAgent<B>(bt, t.sid[i]);   // the tape's own slot order (the live slot map is the last step's)
         scn = sr.sc;
         const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
         const uint32_t a = t.sid[i];
@@ -368,10 +368,18 @@ __global__ void __launch_bounds__(256) k_adj_predict_a(AdjStep t, SceneView s0, 
             gp = gp + nl * gs;
             gr -= gs;
         }
-        if (!(f & SWARM_REC_EFIXED)) {      // e = (anchor - p) / |anchor - p|
+        if (!(f & SWARM_REC_EFIXED)) {      // e0 = (anchor - p) / |anchor - p|
             const V2 d = V2{ pr.anchor_x, pr.anchor_y } - p;
             const float ld = len2(d);
             const V2 ee = d / ld;
+            if (f & SWARM_REC_GOAL_WALL) {  // e1 = e0 - n (e0 . n), e = e1 / |e1| (n frozen)
+                const V2 nw{ pr.goal_wall_x, pr.goal_wall_y };
+                const V2 e1 = ee - nw * dot2(ee, nw);
+                const float l1 = len2(e1);
+                const V2 eu = e1 / l1;
+                const V2 g1 = (ge - eu * dot2(eu, ge)) / l1;
+                ge = g1 - nw * dot2(nw, g1);
+            }
             gp = gp - (ge - ee * dot2(ee, ge)) / ld;
         }
         const float2 po = g.gpo[i];
@@ -393,7 +401,7 @@ __global__ void __launch_bounds__(256) k_adj_predict_b(AdjStep t, SceneView s0, 
     if (i >= t.n) return;
     if (!(g.flags[i] & SWARM_REC_ACTIVE)) return;
     const uint32_t a = t.sid[i];
-    const ScnRef sr = scnOfSlot<B>(bt, uint32_t(i));
+    const ScnRef sr = scnOfAgent<B>(bt, t.sid[i]);   // the tape's own slot order (the live slot map is the last step's)
     const SceneView& s = sceneOf<B>(s0, bt, sr.sc);
     const uint32_t self = a - sr.agentBase;
     const float2 gp0 = g.gp[i];

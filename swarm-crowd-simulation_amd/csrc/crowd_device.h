@@ -30,6 +30,9 @@ struct SceneView {
     float secondsPerUnit, distQuantum;
     const float* stair;
     const float* exitRects;   // E x 4
+    const float* wallRects;            // wall boxes, 4 floats each (lox, loy, hix, hiy)
+    const uint32_t* wallBinStart;      // per hash cell: its walls within reach, wallBinList[start[c] .. start[c + 1])
+    const uint32_t* wallBinList;
 };
 
 struct ModelParams {
@@ -172,6 +175,55 @@ __device__ inline V2 choiceDirection(const SceneView& s, int cell, V2 p, uint32_
         return l > GOAL_DEADZONE ? d / l : V2{ 0.0f, 0.0f };
     }
     return V2{ 0.0f, 0.0f };
+}
+// The exact wall query (seated::wall, src/venue.cpp:763-813, on a rectangle world): the nearest point of the wall
+// boxes within 'reach' of p, as (unit direction away from the wall, clearance); w = WALL_NONE when none lies closer
+// than reach. Candidates in ascending box order, the first of equal distances kept. On a box's boundary the
+// direction is the normal of the face p lies on; strictly inside a box, the normal of its nearest face and the
+// clearance the negative depth.
+__device__ inline G3 wallQuery(const SceneView& s, V2 p, float reach) {
+    const uint32_t c = hashCell(s, p.x, p.y);
+    const uint32_t k0 = s.wallBinStart[c], k1 = s.wallBinStart[c + 1];
+    float best = reach * reach;
+    uint32_t bk = 0xFFFFFFFFu;
+    V2 bq{ 0.0f, 0.0f };
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t wi = s.wallBinList[k];
+        const float4 r = *reinterpret_cast<const float4*>(s.wallRects + 4 * size_t(wi));
+        const V2 q{ fminf(fmaxf(p.x, r.x), r.z), fminf(fmaxf(p.y, r.y), r.w) };
+        const V2 dd = p - q;
+        const float d2 = dot2(dd, dd);
+        if (d2 < best) { best = d2; bq = q; bk = wi; }
+    }
+    if (bk == 0xFFFFFFFFu) return G3{ 0.0f, 0.0f, WALL_NONE };
+    const float d = sqrtf(best);
+    if (d > 0.0f) { const V2 u = (p - bq) / d; return G3{ u.x, u.y, d }; }
+    const float4 r = *reinterpret_cast<const float4*>(s.wallRects + 4 * size_t(bk));
+    const V2 h{ p.x <= r.x ? -1.0f : (p.x >= r.z ? 1.0f : 0.0f), p.y <= r.y ? -1.0f : (p.y >= r.w ? 1.0f : 0.0f) };
+    if (nonzero(h)) { const V2 u = h / len2(h); return G3{ u.x, u.y, 0.0f }; }
+    // strictly inside: the nearest face (left, right, bottom, top; the first of equal depths)
+    float dep = p.x - r.x;
+    V2 u{ -1.0f, 0.0f };
+    if (r.z - p.x < dep) { dep = r.z - p.x; u = V2{ 1.0f, 0.0f }; }
+    if (p.y - r.y < dep) { dep = p.y - r.y; u = V2{ 0.0f, -1.0f }; }
+    if (r.w - p.y < dep) { dep = r.w - p.y; u = V2{ 0.0f, 1.0f }; }
+    return G3{ u.x, u.y, -dep };
+}
+// e_goal's wall rule (physics.comp navChoiceGoal :719-729, CPU nav::choiceGoal, src/exit_choice.cpp:568-578): the
+// component of e into the nearest wall within reach removed, the rest renormalized (0 when nothing is left).
+// *applied: the rule changed e; *wn: the wall direction it used.
+__device__ inline V2 goalWallRule(const SceneView& s, V2 p, float reach, V2 e, bool* applied = nullptr, V2* wn = nullptr) {
+    const G3 w = wallQuery(s, p, reach);
+    const V2 n{ w.gx, w.gy };
+    const float into = dot2(e, n);
+    if (w.w < reach && into < 0.0f) {
+        e = e - n * into;
+        const float l = len2(e);
+        e = l > SAFE_NORMALIZE_EPS ? e / l : V2{ 0.0f, 0.0f };
+        if (applied) *applied = true;
+        if (wn) *wn = n;
+    }
+    return e;
 }
 __device__ inline float estimatedTime(const SceneView& s, float dist, uint32_t exit, uint32_t queued) {
     const float walk = dist * s.secondsPerUnit;

@@ -210,5 +210,9 @@ void launchOverlap(hipStream_t st, const StepLaunch& L, StepBuffers& b, AgentHal
 void launchExitLoss(hipStream_t st, const StepLaunch& L, AgentHalf s, const uint32_t* assign, swarm_exit_rec* out, double* loss);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes);
+// The navigation lookups of the step at caller points (the reference's probe dispatch of physics.comp's nav section,
+// tools/test_gpu.cpp:4155-4191): choiceCell, choiceDirection and e_goal with the wall rule for exit ex[i], radius r[i]
+void launchNavProbe(hipStream_t st, const SceneView& s, int n, const float* x, const float* y, const float* r, const uint32_t* ex,
+                    int32_t* cell, float* dir, float* goal);
 
 } // namespace swarm

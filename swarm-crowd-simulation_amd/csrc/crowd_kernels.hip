@@ -465,8 +465,8 @@ __global__ void k_choice_thresh(SceneView s0, BatchView bt, ChoiceBuffers c, con
 template <bool B>
 __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0, BatchView bt, const uint32_t* __restrict__ skey,
                                                              const uint32_t* __restrict__ sid, const float* __restrict__ sx,
-                                                             const float* __restrict__ sy, ChoiceBuffers c, float* __restrict__ egx,
-                                                             float* __restrict__ egy, const uint32_t* __restrict__ ctrl) {
+                                                             const float* __restrict__ sy, const float* __restrict__ srad, ChoiceBuffers c,
+                                                             float* __restrict__ egx, float* __restrict__ egy, const uint32_t* __restrict__ ctrl) {
     const int slot = xcdBlock() * HASH_BLOCK + threadIdx.x;
     const bool live = slot < n && skey[slot] != INACTIVE_CELL;
     const uint32_t step = ctrl[CTRL_STEP];
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
             akey = uint32_t(r.aheadBase + size_t(1u - par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS) + ex * uint32_t(CHOICE_AHEAD_BINS) +
                    aheadBin(s.navDist[size_t(cell) * size_t(s.K) + size_t(k)]);
         }
-        const V2 e = choiceDirection(s, cell, p, ex);
+        const V2 e = goalWallRule(s, p, FEELER_REACH_RADII * srad[slot], choiceDirection(s, cell, p, ex));
         egx[slot] = e.x;
         egy[slot] = e.y;
     }
@@ -704,6 +704,13 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
                 const V2 a = anchorCentre(s, s.navSlot[size_t(cell) * size_t(s.K) + size_t(kk)] & CHOICE_ANCHOR_MASK);
                 rec->anchor_x = a.x; rec->anchor_y = a.y;
                 if (!(len2(a - p) > GOAL_DEADZONE)) efixed = true;
+                else {   // the wall rule: e = unit(e0 - n (e0 . n)) with n frozen; nothing left: a constant 0
+                    bool wrule = false;
+                    V2 wn{ 0.0f, 0.0f };
+                    const V2 e1 = goalWallRule(s, p, FEELER_REACH_RADII * ri, choiceDirection(s, cell, p, ex), &wrule, &wn);
+                    if (wrule && !nonzero(e1)) efixed = true;
+                    else if (wrule) { f |= SWARM_REC_GOAL_WALL; rec->goal_wall_x = wn.x; rec->goal_wall_y = wn.y; }
+                }
             }
         }
         if (efixed) f |= SWARM_REC_EFIXED;
@@ -1412,6 +1419,25 @@ static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl) { k_step_increment<<<1, 64, 0, st>>>(ctrl); }
 
+__global__ void __launch_bounds__(HASH_BLOCK) k_nav_probe(int n, SceneView s, const float* __restrict__ x, const float* __restrict__ y,
+                                                          const float* __restrict__ r, const uint32_t* __restrict__ ex,
+                                                          int32_t* __restrict__ cell, float* __restrict__ dir, float* __restrict__ goal) {
+    const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const V2 p{ x[i], y[i] };
+    const int c = choiceCell(s, p);
+    const V2 d = choiceDirection(s, c, p, ex[i]);
+    const V2 g = goalWallRule(s, p, FEELER_REACH_RADII * r[i], d);
+    cell[i] = c;
+    dir[2 * i] = d.x; dir[2 * i + 1] = d.y;
+    goal[2 * i] = g.x; goal[2 * i + 1] = g.y;
+}
+
+void launchNavProbe(hipStream_t st, const SceneView& s, int n, const float* x, const float* y, const float* r, const uint32_t* ex,
+                    int32_t* cell, float* dir, float* goal) {
+    k_nav_probe<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, s, x, y, r, ex, cell, dir, goal);
+}
+
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes) {
     k_copy_probe<<<4096, 256, 0, st>>>(static_cast<const float4*>(src), static_cast<float4*>(dst), bytes / 16);
 }
@@ -1474,7 +1500,7 @@ static void choicePasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, cons
     k_choice_prefix<B><<<L.scenarios, 1024, 0, st>>>(L.s, L.bt, c, ctrl);
     k_choice_eval<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
     k_choice_thresh<B><<<L.scenarios, 64, 0, st>>>(L.s, L.bt, c, ctrl);
-    k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, c, b.egx, b.egy, ctrl);
+    k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c, b.egx, b.egy, ctrl);
 }
 
 void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, const float* driveX,

@@ -53,6 +53,11 @@ void DeviceScene::upload(const Scene& s) {
     std::vector<float> ex;
     for (const Rect& r : s.exits) { ex.push_back(r.lox); ex.push_back(r.loy); ex.push_back(r.hix); ex.push_back(r.hiy); }
     v.exitRects = static_cast<const float*>(put(ex.data(), ex.size() * 4));
+    std::vector<float> wr;
+    for (const Rect& r : s.walls) { wr.push_back(r.lox); wr.push_back(r.loy); wr.push_back(r.hix); wr.push_back(r.hiy); }
+    v.wallRects = static_cast<const float*>(put(wr.data(), wr.size() * 4));
+    v.wallBinStart = static_cast<const uint32_t*>(put(s.wallBinStart.data(), s.wallBinStart.size() * 4));
+    v.wallBinList = static_cast<const uint32_t*>(put(s.wallBinList.data(), s.wallBinList.size() * 4));
 }
 
 void DeviceScene::release() {
@@ -220,6 +225,7 @@ HipCrowdSim::~HipCrowdSim() {
     for (void* p : tapeBufs_) (void)hipFree(p);
     for (void* p : mBufs_) (void)hipFree(p);
     for (char* p : ckpt_) (void)hipFree(p);
+    if (finalSnap_) (void)hipFree(finalSnap_);
     for (DeviceScene& d : scenes_) d.release();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -245,6 +251,7 @@ void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const 
                          const float* v0) {
     const size_t b = size_t(n_) * 4;
     const size_t S = size_t(S_);
+    adjK_ = 0;   // a new state: the last adjoint forward's backward no longer applies
     restoreAgentState(x, y, vx, vy, w);
     SWARM_HIP_CHECK(hipMemcpyAsync(rad_, r, b, hipMemcpyHostToDevice, stream_));
     r0_.assign(r, r + n_);
@@ -297,7 +304,10 @@ void HipCrowdSim::setDrive(const float* dx, const float* dy) {
 
 void HipCrowdSim::setProjectionIterations(int iterations) {
     if (iterations < 0) throw std::invalid_argument("setProjectionIterations: iterations must be >= 0");
-    if (iterations != mp_.projIters) dropGraph();
+    if (iterations != mp_.projIters) {
+        dropGraph();
+        adjK_ = 0;   // the last adjoint forward ran another step: its backward would replay this one
+    }
     mp_.projIters = iterations;
 }
 
@@ -647,6 +657,41 @@ void HipCrowdSim::sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t*
     readGrid(sortedKeys, ids);
 }
 
+void HipCrowdSim::navProbe(int m, const float* x, const float* y, const float* r, const uint32_t* ex, int32_t* cell, float* dir, float* goal) {
+    if (m <= 0) return;
+    if (!x || !y || !r || !ex || !cell || !dir || !goal) throw std::invalid_argument("navProbe: null argument");
+    const SceneView& v = scenes_[0].view;
+    for (int i = 0; i < m; ++i)
+        if (v.E > 0 && ex[i] >= uint32_t(v.E) && ex[i] != CHOICE_NONE) throw std::invalid_argument("navProbe: no such exit");
+    const size_t b = size_t(m) * 4;
+    char* d = nullptr;
+    SWARM_HIP_CHECK(hipMalloc(&d, 9 * b));
+    try {
+        float* dx = reinterpret_cast<float*>(d);
+        float* dy = dx + m;
+        float* dr = dy + m;
+        uint32_t* de = reinterpret_cast<uint32_t*>(dr + m);
+        int32_t* dc = reinterpret_cast<int32_t*>(de + m);
+        float* ddir = reinterpret_cast<float*>(dc + m);
+        float* dgoal = ddir + 2 * size_t(m);
+        SWARM_HIP_CHECK(hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(dy, y, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(dr, r, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(de, ex, b, hipMemcpyHostToDevice, stream_));
+        launchNavProbe(stream_, v, m, dx, dy, dr, de, dc, ddir, dgoal);
+        SWARM_HIP_CHECK(hipGetLastError());
+        SWARM_HIP_CHECK(hipMemcpyAsync(cell, dc, b, hipMemcpyDeviceToHost, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(dir, ddir, 2 * b, hipMemcpyDeviceToHost, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(goal, dgoal, 2 * b, hipMemcpyDeviceToHost, stream_));
+        sync();
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipFree(d);
+        throw;
+    }
+    SWARM_HIP_CHECK(hipFree(d));
+}
+
 void HipCrowdSim::readCandidates(uint32_t* count, uint32_t* index, float* dist) {
     const size_t n = size_t(n_);
     if (count) SWARM_HIP_CHECK(hipMemcpyAsync(count, sb_.nbrCount, n * 4, hipMemcpyDeviceToHost, stream_));
@@ -731,7 +776,11 @@ void HipCrowdSim::setModel(float tauS, float obstacleGain) {
     if (!(tauS > 0.0f)) throw std::invalid_argument("setModel: the relaxation time must be positive");
     opt_.tauS = tauS;
     opt_.obstacleGain = obstacleGain;
-    modelParamsFor(tauS, mp_.tau, mp_.relaxStep);
+    float tau = 0.0f, relax = 0.0f;
+    modelParamsFor(tauS, tau, relax);
+    if (tau != mp_.tau || relax != mp_.relaxStep || obstacleGain != mp_.obstacleGain) adjK_ = 0;   // another model than the last forward's
+    mp_.tau = tau;
+    mp_.relaxStep = relax;
     mp_.obstacleGain = obstacleGain;
     dropGraph();
 }
@@ -782,7 +831,10 @@ void HipCrowdSim::allocAdjoint(int every) {
         stateBase_ = static_cast<int*>(dalloc(16));
         adjG0_ = adj_.gx;
     }
-    if (every <= adjCap_) return;
+    // the mask and position arrays hold P iterations per step: a tape made for fewer iterations is made again
+    const int Pnow = mp_.projIters;
+    if (every <= adjCap_ && Pnow <= tapeP_) return;
+    every = std::max(every, adjCap_);
     dropAdjointGraphs();   // they hold the old tape's addresses
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
     for (void* p : tapeBufs_) (void)hipFree(p);   // the smaller tape
@@ -798,7 +850,7 @@ void HipCrowdSim::allocAdjoint(int every) {
         return p;
     };
     // a segment's tape: 'every' steps
-    const size_t C = size_t(every), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    const size_t C = size_t(every), P = size_t(Pnow), nbs = size_t(nb_) + 1;
     float** f1[] = { &tape_.inx, &tape_.iny, &tape_.invx, &tape_.invy, &tape_.inw, &tape_.outx, &tape_.outy, &tape_.sx, &tape_.sy,
                      &tape_.sr, &tape_.egx, &tape_.egy, &tape_.pw, &tape_.pvx, &tape_.pvy };
     for (float** f : f1) *f = static_cast<float*>(talloc(C * n * 4));
@@ -821,6 +873,7 @@ void HipCrowdSim::allocAdjoint(int every) {
         SWARM_HIP_CHECK(hipMemcpyAsync(tape_.bstart + k * nbs + size_t(nb_), &sentinel, 4, hipMemcpyHostToDevice, stream_));
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
     adjCap_ = every;
+    tapeP_ = Pnow;
 }
 
 size_t HipCrowdSim::tapeBytesPerStep() const {   // allocAdjoint's tape, per step
@@ -851,6 +904,10 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     tapedSeg_ = -1;
     snapBytes_ = 0;
     for (const auto& s : persistentState()) snapBytes_ += s.second;
+    if (!finalSnap_) {
+        SWARM_HIP_CHECK(hipMalloc(&finalSnap_, snapBytes_));
+        adjBytes_ += snapBytes_;
+    }
     while (int(ckpt_.size()) < adjSegs_) {
         char* p = nullptr;
         SWARM_HIP_CHECK(hipMalloc(&p, snapBytes_));
@@ -878,10 +935,18 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     SWARM_HIP_CHECK(hipGetLastError());
     SWARM_HIP_CHECK(hipMemcpyAsync(finX_, half_[parity_].x, size_t(n_) * 4, hipMemcpyDeviceToDevice, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(finY_, half_[parity_].y, size_t(n_) * 4, hipMemcpyDeviceToDevice, stream_));
+    snapshotTo(finalSnap_);
+    finalParity_ = parity_;
     double loss = 0.0;
     SWARM_HIP_CHECK(hipMemcpyAsync(&loss, lossAcc_, 8, hipMemcpyDeviceToHost, stream_));
     sync();
     return loss;
+}
+
+// the forward's final state again, after a segment's replay from its checkpoint moved the state back
+void HipCrowdSim::restoreFinal() {
+    restoreFrom(finalSnap_);
+    parity_ = finalParity_;
 }
 
 void HipCrowdSim::tapedStep(int k) {
@@ -1012,6 +1077,7 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
         }
         SWARM_HIP_CHECK(hipGetLastError());
     }
+    if (adjSegs_ > 1) restoreFinal();   // a segment's replay restored an earlier state
     std::vector<float> gR(n), gtau(S), ggain(S);
     SWARM_HIP_CHECK(hipMemcpyAsync(gradV0, adj_.gv0, n * 4, hipMemcpyDeviceToHost, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(gR.data(), adj_.gR, n * 4, hipMemcpyDeviceToHost, stream_));
@@ -1034,6 +1100,12 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
         gradGlobals[3 * s + 1] = gs[s];
         gradGlobals[3 * s + 2] = double(ggain[s]);
     }
+    bool finite = true;
+    for (size_t i = 0; i < n && finite; ++i) finite = std::isfinite(gradV0[i]);
+    for (size_t i = 0; i < 3 * S && finite; ++i) finite = std::isfinite(gradGlobals[i]);
+    if (!finite)
+        throw NonFiniteGradient("adjointBackward: the gradient overflowed FP32 (inf or NaN); the pathwise derivative of this "
+                                "horizon exceeds the float range (shorten K or use the truncation window)");
 }
 
 // the backward of one segment's 'len' steps from its tape; seg < 0: the segment's first state is read from stateBase_
@@ -1074,6 +1146,7 @@ void HipCrowdSim::adjointTape(int step, const swarm_tape_view& out, swarm_exit_r
     get(out.plane_bits, tape_.planeBits + K * n, n * 8);
     get(out.overlap, tape_.overlap + K * n, n * 8);
     if (exit && step == adjK_ - 1) get(exit, exitRec_, n * sizeof(swarm_exit_rec));
+    if (adjSegs_ > 1) restoreFinal();
     sync();
 }
 

@@ -108,13 +108,17 @@ public:
     void readEvac(int32_t* step, int32_t* exitId, uint32_t* perExit);   // perExit: scenario 0's E + 1 counters
     void readEvacCounters(uint32_t* perExit64);              // every scenario: PER_EXIT_STRIDE words each
     void readControl(uint32_t* step, uint32_t* nActive, uint64_t* activeSum);
+    // The step's navigation lookups at m caller points (scenario 0's scene): choiceCell, choiceDirection for exit ex[i]
+    // (2 floats each) and e_goal with the wall rule at radius r[i] (2 floats each)
+    void navProbe(int m, const float* x, const float* y, const float* r, const uint32_t* ex, int32_t* cell, float* dir, float* goal);
 
     // ---- the adjoint of K steps (swarm_adjoint.h) ----
     // Runs K steps from the current state with the loss terms, keeping the state every 'every' steps; returns the
     // loss. The radii of the run are radiusScale x the uploaded radii.
     double adjointForward(int K, int every, float radiusScale, float overlapWeight);
     // dL/dv0 (n, agent order) and dL/d(tau_s, radius scale, wall gain) of the last adjointForward; per scenario in
-    // a batch (globals: 3 per scenario). Leaves the state unspecified.
+    // a batch (globals: 3 per scenario). The state afterwards is the forward's final state, as the forward left it.
+    // Throws NonFiniteGradient (after writing the outputs) when any gradient value is inf or NaN.
     void adjointBackward(float* gradV0, double* gradGlobals);
     // one step's tape of the last adjointForward (recomputed from its checkpoint)
     void adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit);
@@ -243,6 +247,10 @@ private:
     int window_ = 0;
     bool local_ = false;
     int adjK_ = 0, adjEvery_ = 0, adjSegs_ = 0, adjCap_ = 0, tapedSeg_ = -1;
+    int tapeP_ = 0;                      // the projection iterations the tape's mask and position arrays are sized for
+    char* finalSnap_ = nullptr;          // device: the forward's final state, put back after a recompute
+    int finalParity_ = 0;
+    void restoreFinal();
     float adjScale_ = 1.0f;
     size_t snapBytes_ = 0;
     std::vector<char*> ckpt_;            // device snapshots, one per segment
@@ -278,6 +286,11 @@ private:
     std::vector<uint32_t> mThr_;         // 4 threshold masses per cell (host)
     float* mScratch_ = nullptr;          // recordMetrics: 8 arrays of n
     std::vector<void*> mBufs_;           // the metrics' own allocations
+};
+
+// An adjoint whose gradient overflowed FP32 (the outputs are written; the C ABI returns SWARM_ENONFINITE)
+struct NonFiniteGradient : std::runtime_error {
+    using std::runtime_error::runtime_error;
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

@@ -322,6 +322,47 @@ void buildMetricGrid(const SceneDesc& d, Scene& s) {
         }
     }
 }
+// The wall index of the exact wall query: a hash cell lists (ascending) every wall whose box lies within
+// FEELER_REACH_RADII r_max + half a cell of the cell's rectangle, so a query from any point the hash puts in the cell
+// (rounding at the cell's edges included) finds every wall within its reach there
+void buildWallBins(const SceneDesc& d, Scene& s) {
+    s.walls = d.walls;
+    const size_t cells = size_t(s.hashNX) * size_t(s.hashNY);
+    const double cw = double(s.hashSpanX) / double(s.hashNX), ch = double(s.hashSpanY) / double(s.hashNY);
+    const double reach = double(FEELER_REACH_RADII) * double(s.maxRadius) + 0.5 * std::min(cw, ch);
+    std::vector<uint32_t> count(cells + 1, 0u);
+    auto span = [&](const Rect& r, int& x0, int& x1, int& y0, int& y1) {
+        x0 = std::max(0, int(std::floor((double(r.lox) - reach - double(s.xmin)) / cw)));
+        x1 = std::min(s.hashNX - 1, int(std::floor((double(r.hix) + reach - double(s.xmin)) / cw)));
+        y0 = std::max(0, int(std::floor((double(r.loy) - reach - double(s.ymin)) / ch)));
+        y1 = std::min(s.hashNY - 1, int(std::floor((double(r.hiy) + reach - double(s.ymin)) / ch)));
+    };
+    auto near = [&](const Rect& r, int cx, int cy) {   // the gap between the wall box and the cell's rectangle < reach
+        const double lx = double(s.xmin) + cx * cw, hx = lx + cw, ly = double(s.ymin) + cy * ch, hy = ly + ch;
+        const double gx = std::max({ 0.0, double(r.lox) - hx, lx - double(r.hix) });
+        const double gy = std::max({ 0.0, double(r.loy) - hy, ly - double(r.hiy) });
+        return gx * gx + gy * gy < reach * reach;
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            for (size_t c = 0; c < cells; ++c) count[c + 1] += count[c];
+            s.wallBinStart = count;
+            s.wallBinList.assign(count[cells], 0u);
+        }
+        std::vector<uint32_t> fill(pass == 1 ? s.wallBinStart : std::vector<uint32_t>());
+        for (size_t k = 0; k < d.walls.size(); ++k) {
+            int x0, x1, y0, y1;
+            span(d.walls[k], x0, x1, y0, y1);
+            for (int cy = y0; cy <= y1; ++cy)
+                for (int cx = x0; cx <= x1; ++cx) {
+                    if (!near(d.walls[k], cx, cy)) continue;
+                    const size_t c = size_t(cy) * size_t(s.hashNX) + size_t(cx);
+                    if (pass == 0) ++count[c + 1];
+                    else s.wallBinList[fill[c]++] = uint32_t(k);
+                }
+        }
+    }
+}
 } // namespace
 
 Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) {
@@ -464,6 +505,7 @@ Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) 
                 if (d.stairs[k].contains(cx, cy)) { s.stair[size_t(d.nav.index(ix, iy))] = d.stairFactor[k]; break; }
         }
     s.exits = d.exits;
+    buildWallBins(d, s);
     buildMetricGrid(d, s);
     s.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return s;

@@ -68,6 +68,11 @@ struct Scene {
     float secondsPerUnit = 0.0f, distQuantum = 0.0f;
     std::vector<float> stair;              // per nav cell: free-speed factor
     std::vector<Rect> exits;
+    // the wall boxes (the exact wall query of e_goal's wall rule) and their index on the hash grid: per hash cell,
+    // ascending, every wall within FEELER_REACH_RADII r_max + half a cell of the cell's rectangle
+    std::vector<Rect> walls;
+    std::vector<uint32_t> wallBinStart;    // hashNX * hashNY + 1
+    std::vector<uint32_t> wallBinList;
     float maxRadius = 0.0f, maxV0 = 0.0f;  // used for the rings and sdfMin window
     // the safety metrics' floor grid (metrics.hpp:49-67, generalised to a rectangle off the origin): 1 m cells,
     // g = xy * metricCellsPerUnit + metricOff per axis is the position in cells from the centre of cell 0;

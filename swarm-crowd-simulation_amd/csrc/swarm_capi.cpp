@@ -23,6 +23,9 @@ template <class F> int guarded(F&& f) {
     } catch (const std::invalid_argument& e) {
         g_err = e.what();
         return SWARM_EINVAL;
+    } catch (const swarm::NonFiniteGradient& e) {
+        g_err = e.what();
+        return SWARM_ENONFINITE;
     } catch (const std::bad_alloc& e) {
         g_err = e.what();
         return SWARM_ENOMEM;
@@ -81,6 +84,8 @@ int swarm_scene_info_get(const swarm_scene* sc, swarm_scene_info* i) {
     i->nav_cell = s.nav.cell; i->nav_inv_cell = s.nav.invCell;
     i->K = s.K; i->E = s.E; i->seconds_per_unit = s.secondsPerUnit; i->dist_quantum = s.distQuantum;
     i->build_seconds = s.buildSeconds;
+    i->n_walls = int(s.walls.size());
+    i->wall_bin_entries = int(s.wallBinList.size());
     return SWARM_OK;
 }
 
@@ -100,6 +105,19 @@ int swarm_scene_read(const swarm_scene* sc, float* sdf, float* sdf_min, uint32_t
             exit_rects[4 * e] = s.exits[e].lox; exit_rects[4 * e + 1] = s.exits[e].loy;
             exit_rects[4 * e + 2] = s.exits[e].hix; exit_rects[4 * e + 3] = s.exits[e].hiy;
         }
+    return SWARM_OK;
+}
+
+int swarm_scene_read_walls(const swarm_scene* sc, float* rects, uint32_t* bin_start, uint32_t* bin_list) {
+    if (!sc) { g_err = "null argument"; return SWARM_EINVAL; }
+    const swarm::Scene& s = sc->s;
+    if (rects)
+        for (size_t k = 0; k < s.walls.size(); ++k) {
+            rects[4 * k] = s.walls[k].lox; rects[4 * k + 1] = s.walls[k].loy;
+            rects[4 * k + 2] = s.walls[k].hix; rects[4 * k + 3] = s.walls[k].hiy;
+        }
+    if (bin_start) std::memcpy(bin_start, s.wallBinStart.data(), s.wallBinStart.size() * 4);
+    if (bin_list) std::memcpy(bin_list, s.wallBinList.data(), s.wallBinList.size() * 4);
     return SWARM_OK;
 }
 
@@ -193,6 +211,12 @@ int swarm_sort_grid(swarm_ctx* c, const uint32_t* keys, uint32_t* sorted_keys, u
     CTX_CHECK(c);
     if (!keys) { g_err = "null keys"; return SWARM_EINVAL; }
     return guarded([&] { c->sim->sortGrid(keys, sorted_keys, ids); });
+}
+int swarm_nav_probe(swarm_ctx* c, int m, const float* x, const float* y, const float* r, const uint32_t* exit_id, int32_t* cell,
+                    float* dir, float* goal) {
+    CTX_CHECK(c);
+    if (m < 0) { g_err = "negative point count"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->navProbe(m, x, y, r, exit_id, cell, dir, goal); });
 }
 int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id) { CTX_CHECK(c); return guarded([&] { c->sim->readGrid(key, id); }); }
 int swarm_read_cells(swarm_ctx* c, uint32_t* start, uint32_t* count) { CTX_CHECK(c); return guarded([&] { c->sim->readCells(start, count); }); }

@@ -61,6 +61,7 @@ constexpr float PROJ_MAX_SHIFT_RADII = 0.05f;
 // --- steering (sim_params.hpp:171-185) ---
 constexpr float W_OBS = 10.0f;
 constexpr float GOAL_DEADZONE = 1e-4f;
+constexpr float WALL_NONE = 1e9f;   // the wall query's clearance when no wall lies within reach (venue.hpp:251)
 constexpr float OBSTACLE_RANGE = 0.025f;
 constexpr float OBSTACLE_GAIN = 20.0f;          // the wall-avoidance stiffness (a differentiable global)
 constexpr float OBSTACLE_MIN_NORMAL = 0.01f;

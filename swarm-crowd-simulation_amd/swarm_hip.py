@@ -43,7 +43,7 @@ class SceneInfo(C.Structure):
                 ("sdf_min_win", C.c_int), ("nav_res_x", C.c_int), ("nav_res_y", C.c_int),
                 ("nav_origin_x", C.c_float), ("nav_origin_y", C.c_float), ("nav_cell", C.c_float), ("nav_inv_cell", C.c_float),
                 ("K", C.c_int), ("E", C.c_int), ("seconds_per_unit", C.c_float), ("dist_quantum", C.c_float),
-                ("build_seconds", C.c_double)]
+                ("build_seconds", C.c_double), ("n_walls", C.c_int), ("wall_bin_entries", C.c_int)]
 
 
 class Config(C.Structure):
@@ -110,6 +110,13 @@ def lib():
     L.swarm_destroy.restype = None
     _lib = L
     return L
+
+
+ENONFINITE = -6
+
+
+class NonFiniteGradient(RuntimeError):
+    """swarm_adjoint_backward's SWARM_ENONFINITE: the gradient overflowed FP32; .grad_v0 / .grad_globals hold it."""
 
 
 def check(rc):
@@ -180,6 +187,12 @@ class Scene:
                  stair=np.zeros(cells, np.float32), exit_rects=np.zeros(max(4 * i.E, 4), np.float32))
         check(lib().swarm_scene_read(self.h, t["sdf"], t["sdf_min"], t["nav_info"], t["nav_slot"], t["nav_dist"],
                                      t["inv_cap_s"], t["stair"], t["exit_rects"]))
+        t["walls"] = np.zeros(max(4 * i.n_walls, 4), np.float32)
+        t["wall_bin_start"] = np.zeros(i.hash_nx * i.hash_ny + 1, np.uint32)
+        t["wall_bin_list"] = np.zeros(max(i.wall_bin_entries, 1), np.uint32)
+        L = lib()
+        L.swarm_scene_read_walls.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+        check(L.swarm_scene_read_walls(self.h, t["walls"].ctypes.data, t["wall_bin_start"].ctypes.data, t["wall_bin_list"].ctypes.data))
         return t
 
     def metric_grid(self):
@@ -257,6 +270,20 @@ class Sim:
         keep = [None if a is None else np.ascontiguousarray(a, np.uint32) for a in (assign, queue, ahead)]
         ptr = [None if a is None else a.ctypes.data for a in keep]
         check(lib().swarm_set_choice_state(self.h, step, *ptr))
+
+    def nav_probe(self, x, y, r, exit):
+        """The GPU's navigation lookups at points: (cell, direction (m, 2), e_goal with the wall rule (m, 2))."""
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        x, y = f(x), f(y)
+        r = f(np.broadcast_to(np.asarray(r, np.float32), x.shape))
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(exit, np.uint32), x.shape))
+        cell = np.zeros(x.size, np.int32)
+        d, g = np.zeros((x.size, 2), np.float32), np.zeros((x.size, 2), np.float32)
+        L = lib()
+        L.swarm_nav_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 7
+        check(L.swarm_nav_probe(self.h, x.size, x.ctypes.data, y.ctypes.data, r.ctypes.data, e.ctypes.data, cell.ctypes.data,
+                                d.ctypes.data, g.ctypes.data))
+        return cell, d, g
 
     def set_proj_iters(self, it):
         check(lib().swarm_set_proj_iters(self.h, it))
@@ -465,7 +492,8 @@ class PredictRec(C.Structure):
                 [(f, C.c_float) for f in ("anchor_x", "anchor_y", "egoal_x", "egoal_y", "stair", "sep_x", "sep_y",
                                           "sweep_lo", "sweep_dx", "sweep_dy")] +
                 [("n_events", C.c_uint32), ("plane_kind", C.c_uint32)] +
-                [(f, C.c_float) for f in ("plane_nx", "plane_ny", "pc_ux", "pc_uy", "pc_gx", "pc_gy", "pc_x0", "pc_y0", "pc_t0")] +
+                [(f, C.c_float) for f in ("plane_nx", "plane_ny", "pc_ux", "pc_uy", "pc_gx", "pc_gy", "pc_x0", "pc_y0", "pc_t0",
+                                          "goal_wall_x", "goal_wall_y")] +
                 [("ev", WallEvent * 6)])
 
 
@@ -520,7 +548,12 @@ def _sim_adjoint_backward(self):
     """(dL/dv0 per agent, [dL/dtau_s, dL/dradius_scale, dL/dwall_gain] per scenario) of the last adjoint_forward."""
     gv0 = np.zeros(self.n, np.float32)
     g = np.zeros(3 * getattr(self, "scenarios", 1), np.float64)
-    check(_adjoint_lib().swarm_adjoint_backward(self.h, gv0, g))
+    rc = _adjoint_lib().swarm_adjoint_backward(self.h, gv0, g)
+    if rc == ENONFINITE:
+        e = NonFiniteGradient(f"swarm error {rc}: {lib().swarm_last_error().decode()}")
+        e.grad_v0, e.grad_globals = gv0, g
+        raise e
+    check(rc)
     return gv0, g
 
 
