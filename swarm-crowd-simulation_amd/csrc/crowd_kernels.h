@@ -39,6 +39,9 @@ constexpr int CTRL_NACTIVE = 1;         // active agents at the current step's i
 constexpr int CTRL_ACTIVE_SUM_LO = 2;   // running sum of active agents over steps (uint64, low word first)
 constexpr int CTRL_DONE_BLOCKS = 4;     // blocks of the final pass that have finished (reset by the last one)
 constexpr int CTRL_WORDS = 8;
+// The counting sort ranks an agent within its cell by scanning the cell's slots: at most this many (a step's cells
+// hold a few hard discs; swarm_sort_grid's caller keys beyond it take the stable network instead)
+constexpr uint32_t FIX_RANK_MAX = 64;
 // choice counters
 constexpr int CNT_SWITCHES = 0, CNT_FORCED = 1, CNT_EVALUATIONS = 2, CNT_ELIGIBLE = 3;
 // per-scenario strides of the exit-choice and evacuation state
@@ -210,6 +213,20 @@ void launchOverlap(hipStream_t st, const StepLaunch& L, StepBuffers& b, AgentHal
 void launchExitLoss(hipStream_t st, const StepLaunch& L, AgentHalf s, const uint32_t* assign, swarm_exit_rec* out, double* loss);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes);
+// The stable sort of n caller keys (cells < C, or INACTIVE_CELL) into ascending (key, index) order on the bitonic
+// network with whole 64-bit compares (a total order: no rank loops, O(n log^2 n) whatever the keys), and the cell
+// table from the counting pass. Scratch: loc n, bcnt keySortBuckets, bstart keySortBuckets (+ 1), tileSum
+// keySortTiles, pairs keySortPow2(n), ctrl CTRL_WORDS. Any output may be nullptr.
+struct KeySortScratch {
+    uint32_t* loc; uint32_t* bcnt; uint32_t* bstart; uint32_t* tileSum; uint64_t* pairs; uint32_t* ctrl;
+};
+uint32_t keySortPow2(uint32_t n);
+uint32_t keySortBuckets(uint32_t n, uint32_t C);
+uint32_t keySortTiles(uint32_t n, uint32_t C);
+void launchStableKeySort(hipStream_t st, uint32_t n, uint32_t C, const uint32_t* key, const KeySortScratch& s, uint32_t* sortedKeys,
+                         uint32_t* ids, uint32_t* cellStart, uint32_t* cellCount);
+// slot order from sorted (key << 32 | agent) pairs: ids, keys, and the state gathered (as the step's gather)
+void launchSortedGather(hipStream_t st, const StepLaunch& L, AgentHalf rd, const uint64_t* pairs, StepBuffers& b, const float* rad);
 // The navigation lookups of the step at caller points (the reference's probe dispatch of physics.comp's nav section,
 // tools/test_gpu.cpp:4155-4191): choiceCell, choiceDirection and e_goal with the wall rule for exit ex[i], radius r[i]
 void launchNavProbe(hipStream_t st, const SceneView& s, int n, const float* x, const float* y, const float* r, const uint32_t* ex,

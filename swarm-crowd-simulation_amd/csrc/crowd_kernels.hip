@@ -111,7 +111,8 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_hash(int n, SceneView s0, BatchV
 __global__ void __launch_bounds__(HASH_BLOCK) k_key_count(int n, const uint32_t* __restrict__ key, uint32_t* __restrict__ loc,
                                                           uint32_t* __restrict__ bcnt, uint32_t C) {
     const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
-    bucketOffsets(i, n, i < n ? key[i] : INACTIVE_CELL, loc, bcnt, C);
+    const uint32_t k = i < n ? key[i] : INACTIVE_CELL;
+    bucketOffsets(i, n, k < C ? k : INACTIVE_CELL, loc, bcnt, C);   // a key that is no cell counts in none
 }
 
 // tile sums of the bucket counts
@@ -217,13 +218,18 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_fix_gather(int n, BatchView bt, 
 // and the pair swapped when out of its direction. The network is the reference's pass for pass, so the
 // permutation (unstable within a cell) is its own. A key rides in the high word of a 64-bit pair with its agent in
 // the low word; only high words are compared. Passes with j < BITONIC_TILE run on a block's tile in LDS.
-__device__ inline void bitonicCx(uint64_t* __restrict__ t, uint32_t base, uint32_t p, uint32_t k, uint32_t j) {
+// FULL: the whole 64-bit pair is compared (a total order on (key, agent): the stable order of the counting sort)
+template <bool FULL> __device__ inline bool outOfOrder(uint64_t a, uint64_t b, bool asc) {
+    if constexpr (FULL) return asc ? a > b : a < b;
+    const uint32_t ka = uint32_t(a >> 32), kb = uint32_t(b >> 32);
+    return asc ? ka > kb : ka < kb;
+}
+template <bool FULL> __device__ inline void bitonicCx(uint64_t* __restrict__ t, uint32_t base, uint32_t p, uint32_t k, uint32_t j) {
     const uint32_t lo = ((p & ~(j - 1u)) << 1u) | (p & (j - 1u));   // pair p: a 0 inserted at bit log2(j)
     const uint32_t hi = lo | j;
     const bool asc = ((base + lo) & k) == 0u;
     const uint64_t a = t[lo], b = t[hi];
-    const uint32_t ka = uint32_t(a >> 32), kb = uint32_t(b >> 32);
-    if ((asc && ka > kb) || (!asc && ka < kb)) { t[lo] = b; t[hi] = a; }
+    if (outOfOrder<FULL>(a, b, asc)) { t[lo] = b; t[hi] = a; }
 }
 
 // Inside a tile a thread owns its elements 2t and 2t + 1 in registers. A pass with j = 1 pairs them; a pass with
@@ -233,12 +239,11 @@ __device__ inline uint64_t shflXor64(uint64_t v, int d) {
     const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), d)), hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), d));
     return (uint64_t(hi) << 32) | uint64_t(lo);
 }
-__device__ inline void regPass(uint64_t& r0, uint64_t& r1, uint32_t base, uint32_t t, uint32_t k, uint32_t j) {
+template <bool FULL> __device__ inline void regPass(uint64_t& r0, uint64_t& r1, uint32_t base, uint32_t t, uint32_t k, uint32_t j) {
     const uint32_t e0 = 2u * t;
     if (j == 1u) {
         const bool asc = ((base + e0) & k) == 0u;
-        const uint32_t k0 = uint32_t(r0 >> 32), k1 = uint32_t(r1 >> 32);
-        if ((asc && k0 > k1) || (!asc && k0 < k1)) { const uint64_t x = r0; r0 = r1; r1 = x; }
+        if (outOfOrder<FULL>(r0, r1, asc)) { const uint64_t x = r0; r0 = r1; r1 = x; }
         return;
     }
     const int d = int(j >> 1);
@@ -247,14 +252,14 @@ __device__ inline void regPass(uint64_t& r0, uint64_t& r1, uint32_t base, uint32
     const bool asc = ((base + (meLo ? e0 : (e0 ^ j))) & k) == 0u;   // k >= 4: bit 0 of the pair's lower index is moot
     auto pick = [&](uint64_t mine, uint64_t other) {
         const uint64_t lo = meLo ? mine : other, hi = meLo ? other : mine;
-        const uint32_t kl = uint32_t(lo >> 32), kh = uint32_t(hi >> 32);
-        return ((asc && kl > kh) || (!asc && kl < kh)) ? other : mine;
+        return outOfOrder<FULL>(lo, hi, asc) ? other : mine;
     };
     r0 = pick(r0, o0);
     r1 = pick(r1, o1);
 }
 
 // stage k's passes j = jTop .. 1 on the tile at 'base' held in registers (r0, r1); the passes j >= 128 in LDS
+template <bool FULL>
 __device__ inline void tileStage(uint64_t* __restrict__ lds, uint64_t& r0, uint64_t& r1, uint32_t base, uint32_t t, uint32_t k, uint32_t jTop) {
     uint32_t j = jTop;
     if (j >= 128u) {
@@ -262,46 +267,49 @@ __device__ inline void tileStage(uint64_t* __restrict__ lds, uint64_t& r0, uint6
         lds[2u * t + 1u] = r1;
         __syncthreads();
         for (; j >= 128u; j >>= 1) {
-            bitonicCx(lds, base, t, k, j);   // pair t of the tile (blockDim = tile / 2 pairs)
+            bitonicCx<FULL>(lds, base, t, k, j);   // pair t of the tile (blockDim = tile / 2 pairs)
             __syncthreads();
         }
         r0 = lds[2u * t];
         r1 = lds[2u * t + 1u];
         __syncthreads();   // the tile is read before the next stage writes it
     }
-    for (; j > 0; j >>= 1) regPass(r0, r1, base, t, k, j);
+    for (; j > 0; j >>= 1) regPass<FULL>(r0, r1, base, t, k, j);
 }
 
 // the stages 2 .. tile of each tile of 'tile' pairs (tile = min(n, BITONIC_TILE), tile / 2 threads a block), from
-// the keys in agent order
+// the keys in agent order (key != nullptr) or from the pairs as they are
+template <bool FULL>
 __global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_tile(int tile, const uint32_t* __restrict__ key, uint64_t* __restrict__ pairs) {
     __shared__ uint64_t lds[BITONIC_TILE];
     const uint32_t t = threadIdx.x;
     const uint32_t base = blockIdx.x * uint32_t(tile);
     const uint32_t e = base + 2u * t;
-    uint64_t r0 = (uint64_t(key[e]) << 32) | uint64_t(e);
-    uint64_t r1 = (uint64_t(key[e + 1u]) << 32) | uint64_t(e + 1u);
-    for (uint32_t k = 2; k <= uint32_t(tile); k <<= 1) tileStage(lds, r0, r1, base, t, k, k >> 1);
+    uint64_t r0 = key ? (uint64_t(key[e]) << 32) | uint64_t(e) : pairs[e];
+    uint64_t r1 = key ? (uint64_t(key[e + 1u]) << 32) | uint64_t(e + 1u) : pairs[e + 1u];
+    for (uint32_t k = 2; k <= uint32_t(tile); k <<= 1) tileStage<FULL>(lds, r0, r1, base, t, k, k >> 1);
     pairs[e] = r0;
     pairs[e + 1u] = r1;
 }
 
 // stage k's passes j = BITONIC_TILE/2 .. 1 on each tile (n > BITONIC_TILE; BITONIC_TILE / 2 threads a block)
+template <bool FULL>
 __global__ void __launch_bounds__(BITONIC_BLOCK) k_bitonic_merge(uint32_t k, uint64_t* __restrict__ pairs) {
     __shared__ uint64_t lds[BITONIC_TILE];
     const uint32_t t = threadIdx.x;
     const uint32_t base = blockIdx.x * uint32_t(BITONIC_TILE);
     const uint32_t e = base + 2u * t;
     uint64_t r0 = pairs[e], r1 = pairs[e + 1u];
-    tileStage(lds, r0, r1, base, t, k, BITONIC_TILE / 2);
+    tileStage<FULL>(lds, r0, r1, base, t, k, BITONIC_TILE / 2);
     pairs[e] = r0;
     pairs[e + 1u] = r1;
 }
 
 // one pass (k, j >= BITONIC_TILE) over all n/2 pairs in HBM
+template <bool FULL>
 __global__ void __launch_bounds__(256) k_bitonic_pass(uint32_t half, uint32_t k, uint32_t j, uint64_t* __restrict__ pairs) {
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    if (p < half) bitonicCx(pairs, 0u, p, k, j);
+    if (p < half) bitonicCx<FULL>(pairs, 0u, p, k, j);
 }
 
 // the sorted pairs to the slot order's ids and keys, and the state gathered into slot order (as k_fix_gather)
@@ -324,14 +332,74 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_bitonic_gather(int n, BatchView 
     }
 }
 
-static void launchBitonic(hipStream_t st, int n, const uint32_t* key, uint64_t* pairs) {
-    const int tile = n < BITONIC_TILE ? n : BITONIC_TILE;   // n is a power of two
-    k_bitonic_tile<<<n / tile, tile / 2, 0, st>>>(tile, key, pairs);
+// the network over n (a power of two >= 2) pairs: built from the keys (key != nullptr) or sorted in place
+template <bool FULL>
+static void launchBitonicNet(hipStream_t st, int n, const uint32_t* key, uint64_t* pairs) {
+    const int tile = n < BITONIC_TILE ? n : BITONIC_TILE;
+    k_bitonic_tile<FULL><<<n / tile, tile / 2, 0, st>>>(tile, key, pairs);
     for (uint32_t k = 2u * uint32_t(tile); k <= uint32_t(n); k <<= 1) {
         for (uint32_t j = k >> 1; j >= uint32_t(tile); j >>= 1)
-            k_bitonic_pass<<<(n / 2 + 255) / 256, 256, 0, st>>>(uint32_t(n) / 2u, k, j, pairs);
-        k_bitonic_merge<<<n / BITONIC_TILE, BITONIC_TILE / 2, 0, st>>>(k, pairs);
+            k_bitonic_pass<FULL><<<(n / 2 + 255) / 256, 256, 0, st>>>(uint32_t(n) / 2u, k, j, pairs);
+        k_bitonic_merge<FULL><<<n / BITONIC_TILE, BITONIC_TILE / 2, 0, st>>>(k, pairs);
     }
+}
+static void launchBitonic(hipStream_t st, int n, const uint32_t* key, uint64_t* pairs) { launchBitonicNet<false>(st, n, key, pairs); }
+
+// ---------------- the stable (key, agent) sort of caller keys on the network ----------------
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t n, uint32_t* __restrict__ p, uint32_t v) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+// (key << 32 | index) for the n keys, then all-ones padding to the network's power of two (it sorts last)
+__global__ void __launch_bounds__(256) k_pairs_init(uint32_t n, uint32_t npow, const uint32_t* __restrict__ key, uint64_t* __restrict__ pairs) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e < npow) pairs[e] = e < n ? (uint64_t(key[e]) << 32) | uint64_t(e) : ~0ull;
+}
+__global__ void __launch_bounds__(256) k_pairs_out(uint32_t n, const uint64_t* __restrict__ pairs, uint32_t* __restrict__ sortedKeys,
+                                                   uint32_t* __restrict__ ids) {
+    const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+    if (f >= n) return;
+    const uint64_t v = pairs[f];
+    if (sortedKeys) sortedKeys[f] = uint32_t(v >> 32);
+    if (ids) ids[f] = uint32_t(v);
+}
+// the reference's cell table (reorder.comp:20-28: start 0 for an empty cell) from the bucket starts
+__global__ void __launch_bounds__(256) k_cells_out(uint32_t C, const uint32_t* __restrict__ bstart, uint32_t* __restrict__ start,
+                                                   uint32_t* __restrict__ count) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t a = bstart[c], cnt = bstart[c + 1u] - a;
+    if (start) start[c] = cnt ? a : 0u;
+    if (count) count[c] = cnt;
+}
+
+uint32_t keySortPow2(uint32_t n) {
+    uint32_t p = 2u;
+    while (p < n) p <<= 1;
+    return p;
+}
+uint32_t keySortBuckets(uint32_t n, uint32_t C) { return C + uint32_t((int(n) + HASH_BLOCK - 1) / HASH_BLOCK); }
+uint32_t keySortTiles(uint32_t n, uint32_t C) { return uint32_t((int(keySortBuckets(n, C)) + SCAN_TILE - 1) / SCAN_TILE); }
+
+void launchStableKeySort(hipStream_t st, uint32_t n, uint32_t C, const uint32_t* key, const KeySortScratch& s, uint32_t* sortedKeys,
+                         uint32_t* ids, uint32_t* cellStart, uint32_t* cellCount) {
+    const uint32_t nb = keySortBuckets(n, C), npow = keySortPow2(n);
+    const int tiles = int(keySortTiles(n, C));
+    k_fill_u32<<<(nb + 255) / 256, 256, 0, st>>>(nb, s.bcnt, 0u);
+    k_key_count<<<(int(n) + HASH_BLOCK - 1) / HASH_BLOCK, HASH_BLOCK, 0, st>>>(int(n), key, s.loc, s.bcnt, C);
+    k_scan_reduce<<<tiles, SCAN_BLOCK, 0, st>>>(s.bcnt, nb, s.tileSum);
+    k_scan_apply<<<tiles, SCAN_BLOCK, 0, st>>>(s.bcnt, nb, C, s.tileSum, s.bstart, s.ctrl);
+    k_pairs_init<<<(npow + 255) / 256, 256, 0, st>>>(n, npow, key, s.pairs);
+    launchBitonicNet<true>(st, int(npow), nullptr, s.pairs);
+    k_pairs_out<<<(n + 255) / 256, 256, 0, st>>>(n, s.pairs, sortedKeys, ids);
+    if (cellStart || cellCount) k_cells_out<<<(C + 255) / 256, 256, 0, st>>>(C, s.bstart, cellStart, cellCount);
+}
+
+void launchSortedGather(hipStream_t st, const StepLaunch& L, AgentHalf rd, const uint64_t* pairs, StepBuffers& b, const float* rad) {
+    if (L.batched)
+        k_bitonic_gather<true><<<(L.n + HASH_BLOCK - 1) / HASH_BLOCK, HASH_BLOCK, 0, st>>>(L.n, L.bt, pairs, b.sid, b.skey, rd.x, rd.y, rad, b.sx, b.sy, b.sr);
+    else
+        k_bitonic_gather<false><<<(L.n + HASH_BLOCK - 1) / HASH_BLOCK, HASH_BLOCK, 0, st>>>(L.n, L.bt, pairs, b.sid, b.skey, rd.x, rd.y, rad, b.sx, b.sy, b.sr);
 }
 
 // ---------------- exit choice (PHYSICS_PASS 0) ----------------

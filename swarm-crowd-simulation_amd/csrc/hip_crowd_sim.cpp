@@ -652,8 +652,34 @@ void HipCrowdSim::sortGrid(const uint32_t* keys, uint32_t* sortedKeys, uint32_t*
         if (keys[i] != INACTIVE_CELL && keys[i] >= uint32_t(C_)) throw std::invalid_argument("sortGrid: a key is neither a cell nor inactive");
     SWARM_HIP_CHECK(hipMemcpyAsync(grid_.key, keys, size_t(n_) * 4, hipMemcpyHostToDevice, stream_));
     StepBuffers b = sb_;
-    launchSortKeys(stream_, launchDesc(), half_[parity_], grid_, b, rad_, ctrl_);
-    SWARM_HIP_CHECK(hipGetLastError());
+    // the counting sort ranks each agent within its cell by a scan of the cell's slots: a step's cells hold a few
+    // agents (hard discs), caller keys may put any number in one; beyond FIX_RANK_MAX the stable order comes from
+    // the network with whole-pair compares instead (the same order, O(n log^2 n))
+    std::vector<uint32_t> cnt(size_t(C_) + 1, 0u);
+    uint32_t most = 0;
+    for (size_t i = 0; i < size_t(n_); ++i)
+        if (keys[i] != INACTIVE_CELL) most = std::max(most, ++cnt[keys[i]]);
+    if (!bitonic_ && most > FIX_RANK_MAX) {
+        const uint32_t npow = keySortPow2(uint32_t(n_));
+        uint64_t* pairs = grid_.pairs;
+        void* tmp = nullptr;
+        if (npow != uint32_t(n_)) {
+            SWARM_HIP_CHECK(hipMalloc(&tmp, size_t(npow) * 8));
+            pairs = static_cast<uint64_t*>(tmp);
+        }
+        const KeySortScratch sc{ grid_.loc, grid_.bcnt, grid_.bstart, grid_.tileSum, pairs, ctrl_ };
+        launchStableKeySort(stream_, uint32_t(n_), uint32_t(C_), grid_.key, sc, nullptr, nullptr, nullptr, nullptr);
+        launchSortedGather(stream_, launchDesc(), half_[parity_], pairs, b, rad_);
+        const hipError_t e = hipGetLastError();
+        if (tmp) {
+            (void)hipStreamSynchronize(stream_);
+            (void)hipFree(tmp);
+        }
+        SWARM_HIP_CHECK(e);
+    } else {
+        launchSortKeys(stream_, launchDesc(), half_[parity_], grid_, b, rad_, ctrl_);
+        SWARM_HIP_CHECK(hipGetLastError());
+    }
     readGrid(sortedKeys, ids);
 }
 
