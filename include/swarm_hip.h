@@ -255,6 +255,24 @@ int swarm_read_metric_cells(swarm_ctx* c, uint32_t* out16);         /* readCells
 int swarm_read_metric_evac_steps(swarm_ctx* c, uint32_t* out_n);    /* readEvacSteps (metrics.hpp:213) */
 int swarm_read_metric_cell_info(swarm_ctx* c, float* walkable_m2, uint32_t* thresholds4);   /* MetricVenue (metrics.hpp:145-153) */
 
+/* ---- the sort stage without allocation ----
+ * No counterpart in the reference: a variant of swarm_sort_grid (CrowdSim::sortGrid) for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Every pointer is
+ * DEVICE memory of the caller: keys[n] (a cell < cells, or 0xFFFFFFFF inactive; a value neither is not counted in
+ * any cell), outputs sorted_keys[n] and ids[n] in ascending (key, index) order (swarm_sort_grid's), cell_start[cells]
+ * and cell_count[cells] (swarm_read_cells'); any output may be NULL. stream: a hipStream_t, or NULL for the default
+ * stream; the call is asynchronous on it, launches kernels only and allocates nothing: its working memory is one
+ * static device array of SWARM_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on that GPU to the end of the
+ * process. Limits: n <= SWARM_NOALLOC_MAX_KEYS, cells <= SWARM_NOALLOC_MAX_CELLS (SWARM_EINVAL beyond). Calls that
+ * overlap on one GPU (other streams, other threads) share that array and race: serialise them. */
+#define SWARM_NOALLOC_MAX_KEYS 65536u
+#define SWARM_NOALLOC_MAX_CELLS 262144u
+#define SWARM_NOALLOC_SCRATCH_BYTES 2886912u
+int swarm_sort_keys_noalloc(const uint32_t* keys, uint32_t n, uint32_t cells, uint32_t* sorted_keys, uint32_t* ids,
+                            uint32_t* cell_start, uint32_t* cell_count, void* stream);
+/* the message of this thread's last failed swarm_sort_keys_noalloc */
+const char* swarm_noalloc_last_error(void);
+
 /* the context's HIP stream (hipStream_t), for events around swarm_step */
 int swarm_get_stream(swarm_ctx* c, void** stream);
 int swarm_num_cells(swarm_ctx* c);
