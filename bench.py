@@ -35,7 +35,12 @@ def step_bytes(n_total, cells, active, iters=32):
     return 40 * n_total + 8 * cells + (236 + 36 * (iters - 1)) * active
 
 
-PROJ_BYTES_PER_SLOT = 52   # one projection launch per active slot: key 4, x/y/acc 12, flag 4, radius 4, plane 16 read; x/y/acc 12 written
+# one projection iteration's algorithmic bytes per active slot (the per-step accounting's 36 A per iteration: position
+# and displacement accumulator read and written, radius, the wall plane's offset)
+PROJ_BYTES_PER_SLOT = 36
+# the same launch's bytes counted word by word as this build's kernel touches them once (key 4, x/y/acc 12, flag 4,
+# radius 4, plane 16 read; x/y/acc 12 written): a secondary figure
+PROJ_WORDS_PER_SLOT = 52
 TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "traffic_bowl_63424.md")
 
 
@@ -51,24 +56,93 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(scene, a, steps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+CPU_STAGES = ("hash", "sort", "cells", "choice", "predict", "project", "final", "total")
+
+
+def cpu_snapshot_rate(scene, a, n, threads, warmup, measure, rounds=3):
+    """The oracle (a CPU restatement of the reference's CpuCrowdSim: counting sort, OpenMP dynamic,64) in the
+    reference benchmark's snapshot mode (tools/benchmark/benchmark_crowd.cpp:16-24): every step, warm-up and timed,
+    starts from the same agent state, restored outside the timed region (the exit choice runs on); the median of the
+    per-round medians over `rounds` rounds, with the median per-stage wall clock of the timed steps."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py as op
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     tau, relax = sh.model_params(0.5)
-    o = op.Oracle(scene.info, scene.tables(), TOTAL, relax, tau, threads=threads)
+    o = op.Oracle(scene.info, scene.tables(), n, relax, tau, threads=threads)
     o.set_state(a)
-    o.step(1)   # warm-up
-    active = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        active += int(o.state()["w"].sum())
-        o.step(1)
-    dt = time.perf_counter() - t0
-    return dict(value=active / dt, unit="agent-steps/s", cores=threads, kind="port",
-                sample=f"{steps} consecutive steps of the 63,424-agent bowl after one warm-up step, oracle (OpenMP dynamic,64)",
-                ms_per_step=1000.0 * dt / steps)
+    active = int(np.sum(a["w"] > 0))
+    medians, stages = [], []
+    for _ in range(rounds):
+        for _ in range(warmup):
+            o.restore(a)
+            o.step(1)
+        times = []
+        for _ in range(measure):
+            o.restore(a)
+            t0 = time.perf_counter()
+            o.step(1)
+            times.append(time.perf_counter() - t0)
+            stages.append(o.times())
+        medians.append(float(np.median(times)))
+    ms = 1000.0 * float(np.median(medians))
+    st = np.median(np.array(stages), axis=0)
+    return {"agent_steps_per_s": active / (ms * 1e-3), "ms_per_step": ms, "threads": threads, "agents_active": active,
+            "rounds": rounds, "warmup": warmup, "measured_per_round": measure,
+            "stages_ms": {k: float(v) for k, v in zip(CPU_STAGES, st)}}
+
+
+def gpu_plain_rate(scene, a, n, steps=100, warmup=10):
+    """The plain step on the GPU for the same state (graph replays, HIP events), as agent-steps/s."""
+    sim = sh.Sim(scene, n)
+    sim.upload(a)
+    sim.autotune(5)
+    sim.step(warmup)
+    sim.sync()
+    _, _, s0 = sim.control()
+    ms = sim.time_steps(steps)
+    _, _, s1 = sim.control()
+    sim.close()
+    return {"agent_steps_per_s": (s1 - s0) / (ms * 1e-3), "ms_per_step": ms / steps}
+
+
+def cpu_baseline(bowl_scene, bowl_agents, full=True):
+    """The CPU baseline: the oracle on the host cores of this box, all logical cores and one thread, on the large
+    floor (63,424 agents), the small room (4,096 agents) and one scenario of the ensemble (the one of median agent
+    count), beside the GPU's rate for the same state."""
+    cores = os.cpu_count() or 1
+    out = {"cpu_model": cpu_model(), "logical_cores": cores, "protocol": "snapshot mode (benchmark_crowd.cpp:16-24): "
+           "every step from the same restored agent state; median of per-round medians over 3 rounds", "configs": {}}
+    bowl = {"all_cores": cpu_snapshot_rate(bowl_scene, bowl_agents, TOTAL, cores, 2, 5)}
+    if full:
+        bowl["one_thread"] = cpu_snapshot_rate(bowl_scene, bowl_agents, TOTAL, 1, 1, 3)
+    out["configs"]["bowl_63424"] = bowl
+    if full:
+        room_scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+        room = sh.place_agents(sh.LAYOUT_ROOM, 4096, 4096, SEED)
+        out["configs"]["room_4096"] = {"all_cores": cpu_snapshot_rate(room_scene, room, 4096, cores, 5, 20),
+                                       "one_thread": cpu_snapshot_rate(room_scene, room, 4096, 1, 2, 8),
+                                       "gpu": gpu_plain_rate(room_scene, room, 4096)}
+        scen_layout, totals, actives, agents = sh.batch_workload()
+        s = int(np.argsort(totals)[len(totals) // 2])
+        lo = int(np.sum(totals[:s]))
+        lay = sh.BATCH_LAYOUTS[scen_layout[s]]
+        sc = sh.Scene.layout(lay)
+        part = {k: v[lo:lo + int(totals[s])] for k, v in agents.items()}
+        out["configs"]["ensemble_scenario"] = {"scenario": s, "layout": int(lay), "agents": int(totals[s]),
+                                               "all_cores": cpu_snapshot_rate(sc, part, int(totals[s]), cores, 5, 20),
+                                               "one_thread": cpu_snapshot_rate(sc, part, int(totals[s]), 1, 2, 8),
+                                               "gpu": gpu_plain_rate(sc, part, int(totals[s]))}
+    return out
 
 
 def batch_bench(steps, warmup, loop_steps, use_graph=True):
@@ -139,15 +213,21 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     fwd = time.perf_counter() - t0
     _, _, steps_sum = sim.control()
 
+    def backward():
+        try:
+            return sim.adjoint_backward()
+        except sh.NonFiniteGradient as e:   # SWARM_ENONFINITE: the outputs are written all the same
+            return e.grad_v0, e.grad_globals
+
     def timed_adjoint():
         sim.upload(a)
         sim.adjoint_forward(K, every, 1.0, 0.1)   # warm-up (allocations, graphs)
-        sim.adjoint_backward()
+        backward()
         sim.upload(a)
         t0 = time.perf_counter()
         loss = sim.adjoint_forward(K, every, 1.0, 0.1)
         t1 = time.perf_counter()
-        gv0, g = sim.adjoint_backward()
+        gv0, g = backward()
         t2 = time.perf_counter()
         return loss, gv0, g, t1 - t0, t2 - t1, sim.adjoint_segments(), sim.tape_bytes()
 
@@ -159,7 +239,7 @@ def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
     sim.upload(a)
     t3 = time.perf_counter()
     sim.adjoint_forward(K, every, 1.0, 0.1)
-    wv0, wg = sim.adjoint_backward()
+    wv0, wg = backward()
     t4 = time.perf_counter()
     sim.set_adjoint_terms(window=0)
     windowed = {"window": 25, "ms": 1000.0 * (t4 - t3), "gradient_finite": bool(np.isfinite(wv0).all() and np.isfinite(wg).all()),
@@ -186,8 +266,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-quick", action="store_true", help="the CPU baseline on the large floor at all cores only")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true", default=True)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false")
@@ -229,8 +309,8 @@ def main():
     # the safety-metric passes (density deposit + per-cell accumulators) inside the same steps: the same upload and
     # warm-up replay the same trajectory, so the difference is the passes' cost
     sim.upload(a)
-    sim.step(args.warmup)
     sim.metrics_enable(True)
+    sim.step(args.warmup)   # the same warm-up, now with the passes (their graph is built and warm before timing)
     sim.sync()
     ms_m = sim.time_steps(args.steps)
     mc = sim.metric_counters()
@@ -254,6 +334,7 @@ def main():
     mean_active = agent_steps / args.steps
     proj_bytes = PROJ_BYTES_PER_SLOT * mean_active
     achieved = proj_bytes / (proj_launch_ms * 1e-3) / 1e9
+    achieved_words = PROJ_WORDS_PER_SLOT * mean_active / (proj_launch_ms * 1e-3) / 1e9
     cells = scene.info.hash_nx * scene.info.hash_ny
     b_step = step_bytes(TOTAL, cells, mean_active)
     step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
@@ -284,8 +365,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, a, args.cpu_steps)
-        print(f"# cpu baseline {cpu['ms_per_step']:.1f} ms/step on {cpu['cores']} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scene, a, full=not args.cpu_quick)
+        for name, c in cpu["configs"].items():
+            print(f"# cpu baseline {name}: " + ", ".join(f"{k} {v['ms_per_step']:.2f} ms/step ({v['agent_steps_per_s'] / 1e6:.2f} M agent-steps/s)"
+                                                         for k, v in c.items() if isinstance(v, dict) and "ms_per_step" in v),
+                  file=sys.stderr, flush=True)
 
     line = {
         "metric": "agent-steps/s @63,424 agents + %HBM roofline; batched & adjoint vs plain",
@@ -307,14 +391,21 @@ def main():
                    "autotune_ms_per_step": tune_ms},
         "roofline": {"bound": "hbm", "kernel": "k_project (one Jacobi iteration after the first, over the candidate lists)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "bytes_per_launch": proj_bytes, "launch_us": proj_launch_ms * 1000.0,
+                     "traffic_source": "FETCH_SIZE + WRITE_SIZE counter runs, scaled by a plain copy's counts (profiles/traffic_bowl_63424.md)",
+                     "bytes_per_launch": proj_bytes, "bytes_per_slot": PROJ_BYTES_PER_SLOT, "launch_us": proj_launch_ms * 1000.0,
+                     "launch_timing": "HIP events recorded by each later iteration's own dispatch, mean over the run",
+                     "achieved_word_bytes": achieved_words, "word_bytes_per_slot": PROJ_WORDS_PER_SLOT,
                      "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
         "metrics": metrics,
         "batched": batch,
         "adjoint": adjoint,
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else dict(
+            value=cpu["configs"]["bowl_63424"]["all_cores"]["agent_steps_per_s"], unit="agent-steps/s", cores=cpu["logical_cores"],
+            kind="port", sample="the 63,424-agent bowl: 3 rounds of 2 warm-up + 5 timed steps, each from the same restored state "
+                                "(snapshot mode), median of the per-round medians; oracle = CPU restatement of CpuCrowdSim",
+            cpu_model=cpu["cpu_model"], protocol=cpu["protocol"], configs=cpu["configs"]),
     }
     print(json.dumps(line))
 

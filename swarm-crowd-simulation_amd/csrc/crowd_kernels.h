@@ -28,6 +28,7 @@ constexpr int BITONIC_TILE = 2048;      // pairs a block sorts in LDS in the bit
 constexpr int BITONIC_BLOCK = 1024;
 constexpr int ADJ_CHUNK = 4;            // contacts whose loads the adjoint projection issues together
 constexpr size_t ADJ_TAPE_BUDGET = size_t(4) << 30;   // bytes of tape a run may keep instead of recomputing segments
+constexpr size_t ADJ_DUAL_LIMIT = size_t(16) << 30;   // two segment tapes up to this many bytes (the pipelined backward)
 // ring slots loaded together by one predict thread (same-box against 4: the 256-scenario ensemble 1.296 -> 1.279 ms
 // per step, the plain bowl unchanged; 8 slower in both)
 constexpr int PRED_CHUNK = 2;

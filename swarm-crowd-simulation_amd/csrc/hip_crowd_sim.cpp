@@ -227,6 +227,15 @@ HipCrowdSim::~HipCrowdSim() {
     for (char* p : ckpt_) (void)hipFree(p);
     if (finalSnap_) (void)hipFree(finalSnap_);
     for (DeviceScene& d : scenes_) d.release();
+    if (stream2_) {
+        (void)hipStreamSynchronize(stream2_);
+        (void)hipStreamDestroy(stream2_);
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (evReplay_[b]) (void)hipEventDestroy(evReplay_[b]);
+        if (evBack_[b]) (void)hipEventDestroy(evBack_[b]);
+    }
+    if (evMain_) (void)hipEventDestroy(evMain_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -239,12 +248,13 @@ void HipCrowdSim::dropGraph() {
 }
 
 void HipCrowdSim::dropAdjointGraphs() {
-    for (int i = 0; i < 2; ++i) {
-        if (replayExec_[i]) { (void)hipGraphExecDestroy(replayExec_[i]); replayExec_[i] = nullptr; }
-        if (replayGraph_[i]) { (void)hipGraphDestroy(replayGraph_[i]); replayGraph_[i] = nullptr; }
-        if (backExec_[i]) { (void)hipGraphExecDestroy(backExec_[i]); backExec_[i] = nullptr; }
-        if (backGraph_[i]) { (void)hipGraphDestroy(backGraph_[i]); backGraph_[i] = nullptr; }
-    }
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 2; ++i) {
+            if (replayExec_[b][i]) { (void)hipGraphExecDestroy(replayExec_[b][i]); replayExec_[b][i] = nullptr; }
+            if (replayGraph_[b][i]) { (void)hipGraphDestroy(replayGraph_[b][i]); replayGraph_[b][i] = nullptr; }
+            if (backExec_[b][i]) { (void)hipGraphExecDestroy(backExec_[b][i]); backExec_[b][i] = nullptr; }
+            if (backGraph_[b][i]) { (void)hipGraphDestroy(backGraph_[b][i]); backGraph_[b][i] = nullptr; }
+        }
 }
 
 void HipCrowdSim::upload(const float* x, const float* y, const float* vx, const float* vy, const float* w, const float* r,
@@ -819,15 +829,15 @@ void HipCrowdSim::snapshotTo(char* dst) {
     }
 }
 
-void HipCrowdSim::restoreFrom(const char* src) {
+void HipCrowdSim::restoreFrom(const char* src, hipStream_t st) {
     size_t off = 0;
     for (const auto& s : persistentState()) {
-        SWARM_HIP_CHECK(hipMemcpyAsync(s.first, src + off, s.second, hipMemcpyDeviceToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(s.first, src + off, s.second, hipMemcpyDeviceToDevice, st));
         off += s.second;
     }
 }
 
-void HipCrowdSim::allocAdjoint(int every) {
+void HipCrowdSim::allocAdjoint(int every, bool dual) {
     auto dalloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
         SWARM_HIP_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 16));
@@ -851,18 +861,20 @@ void HipCrowdSim::allocAdjoint(int every) {
         adj_.lead = static_cast<uint32_t*>(dalloc(n * 4));
         adj_.gtau = static_cast<float*>(dalloc(4 * size_t(S_)));
         adj_.ggain = static_cast<float*>(dalloc(4 * size_t(S_)));
-        tape_.acc0 = static_cast<float*>(dalloc(n * 4));
-        tape_.acc1 = static_cast<float*>(dalloc(n * 4));
-        tape_.err = static_cast<uint32_t*>(dalloc(16));
+        tape_[0].acc0 = static_cast<float*>(dalloc(n * 4));
+        tape_[0].acc1 = static_cast<float*>(dalloc(n * 4));
+        tape_[0].err = static_cast<uint32_t*>(dalloc(16));
         stateBase_ = static_cast<int*>(dalloc(16));
         adjG0_ = adj_.gx;
     }
     // the mask and position arrays hold P iterations per step: a tape made for fewer iterations is made again
     const int Pnow = mp_.projIters;
-    if (every <= adjCap_ && Pnow <= tapeP_) return;
+    const bool haveDual = tape_[1].inx != nullptr;
+    if (every <= adjCap_ && Pnow <= tapeP_ && (haveDual || !dual)) return;
     every = std::max(every, adjCap_);
     dropAdjointGraphs();   // they hold the old tape's addresses
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (stream2_) SWARM_HIP_CHECK(hipStreamSynchronize(stream2_));
     for (void* p : tapeBufs_) (void)hipFree(p);   // the smaller tape
     tapeBufs_.clear();
     adjBytes_ -= tapeBytes_;
@@ -875,29 +887,39 @@ void HipCrowdSim::allocAdjoint(int every) {
         adjBytes_ += bytes;
         return p;
     };
-    // a segment's tape: 'every' steps
+    // a segment's tape: 'every' steps (two of them for the pipelined backward)
     const size_t C = size_t(every), P = size_t(Pnow), nbs = size_t(nb_) + 1;
-    float** f1[] = { &tape_.inx, &tape_.iny, &tape_.invx, &tape_.invy, &tape_.inw, &tape_.outx, &tape_.outy, &tape_.sx, &tape_.sy,
-                     &tape_.sr, &tape_.egx, &tape_.egy, &tape_.pw, &tape_.pvx, &tape_.pvy };
-    for (float** f : f1) *f = static_cast<float*>(talloc(C * n * 4));
-    tape_.sid = static_cast<uint32_t*>(talloc(C * n * 4));
-    tape_.skey = static_cast<uint32_t*>(talloc(C * n * 4));
-    tape_.bstart = static_cast<uint32_t*>(talloc(C * nbs * 4));
-    tape_.planes = static_cast<float4*>(talloc(C * n * 16));
-    tape_.nbr = static_cast<uint32_t*>(talloc(C * size_t(NBR_CAP) * n * 4));
-    tape_.nbrD = static_cast<float*>(talloc(C * size_t(NBR_CAP) * n * 4));
-    tape_.nbrCount = static_cast<uint32_t*>(talloc(C * n * 4));
-    tape_.pred = static_cast<swarm_predict_rec*>(talloc(C * n * sizeof(swarm_predict_rec)));
-    tape_.fin = static_cast<swarm_final_rec*>(talloc(C * n * sizeof(swarm_final_rec)));
-    tape_.mask = static_cast<uint64_t*>(talloc(C * P * n * 8));
-    tape_.capBits = static_cast<uint64_t*>(talloc(C * n * 8));
-    tape_.planeBits = static_cast<uint64_t*>(talloc(C * n * 8));
-    tape_.overlap = static_cast<uint64_t*>(talloc(C * n * 8));
-    tape_.q = static_cast<float2*>(talloc(C * (P + 1) * n * 8));
     const uint32_t sentinel = uint32_t(n_);
-    for (size_t k = 0; k < C; ++k)
-        SWARM_HIP_CHECK(hipMemcpyAsync(tape_.bstart + k * nbs + size_t(nb_), &sentinel, 4, hipMemcpyHostToDevice, stream_));
+    for (int tb = 0; tb < 2; ++tb) {
+        Tape& T = tape_[tb];
+        float* acc0 = tape_[0].acc0;
+        float* acc1 = tape_[0].acc1;
+        uint32_t* err = tape_[0].err;
+        T = Tape{};
+        T.acc0 = acc0; T.acc1 = acc1; T.err = err;   // the replay's scratch and error word are shared
+        if (tb == 1 && !dual) continue;
+        float** f1[] = { &T.inx, &T.iny, &T.invx, &T.invy, &T.inw, &T.outx, &T.outy, &T.sx, &T.sy,
+                         &T.sr, &T.egx, &T.egy, &T.pw, &T.pvx, &T.pvy };
+        for (float** f : f1) *f = static_cast<float*>(talloc(C * n * 4));
+        T.sid = static_cast<uint32_t*>(talloc(C * n * 4));
+        T.skey = static_cast<uint32_t*>(talloc(C * n * 4));
+        T.bstart = static_cast<uint32_t*>(talloc(C * nbs * 4));
+        T.planes = static_cast<float4*>(talloc(C * n * 16));
+        T.nbr = static_cast<uint32_t*>(talloc(C * size_t(NBR_CAP) * n * 4));
+        T.nbrD = static_cast<float*>(talloc(C * size_t(NBR_CAP) * n * 4));
+        T.nbrCount = static_cast<uint32_t*>(talloc(C * n * 4));
+        T.pred = static_cast<swarm_predict_rec*>(talloc(C * n * sizeof(swarm_predict_rec)));
+        T.fin = static_cast<swarm_final_rec*>(talloc(C * n * sizeof(swarm_final_rec)));
+        T.mask = static_cast<uint64_t*>(talloc(C * P * n * 8));
+        T.capBits = static_cast<uint64_t*>(talloc(C * n * 8));
+        T.planeBits = static_cast<uint64_t*>(talloc(C * n * 8));
+        T.overlap = static_cast<uint64_t*>(talloc(C * n * 8));
+        T.q = static_cast<float2*>(talloc(C * (P + 1) * n * 8));
+        for (size_t k = 0; k < C; ++k)
+            SWARM_HIP_CHECK(hipMemcpyAsync(T.bstart + k * nbs + size_t(nb_), &sentinel, 4, hipMemcpyHostToDevice, stream_));
+    }
     SWARM_HIP_CHECK(hipStreamSynchronize(stream_));
+    tapedSeg_[0] = tapedSeg_[1] = -1;
     adjCap_ = every;
     tapeP_ = Pnow;
 }
@@ -914,9 +936,12 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
         throw std::invalid_argument("adjointForward: the adjoint needs 2 to 32 projection iterations");
     if (r0_.size() != size_t(n_)) throw std::invalid_argument("adjointForward: upload a state first");
     if (!(radiusScale > 0.0f)) throw std::invalid_argument("adjointForward: the radius scale must be positive");
-    // the segment length: the whole run when its tape fits the budget (no recompute), else the checkpoint interval
+    // the segment length: the whole run when its tape fits the budget (no recompute), else the checkpoint interval;
+    // with several segments, a second segment tape when the two fit ADJ_DUAL_LIMIT (the pipelined backward)
     const int segLen = size_t(K) * tapeBytesPerStep() <= tapeBudget_ ? K : std::min(every, K);
-    allocAdjoint(segLen);
+    const int segs = (K + segLen - 1) / segLen;
+    const bool dual = segs > 1 && 2 * size_t(segLen) * tapeBytesPerStep() <= ADJ_DUAL_LIMIT;
+    allocAdjoint(segLen, dual);
     // this run's radii
     std::vector<float> r(r0_.size());
     for (size_t i = 0; i < r.size(); ++i) r[i] = radiusScale * r0_[i];
@@ -924,10 +949,11 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     if (segLen != adjEvery_ || overlapWeight != overlapWeight_) dropAdjointGraphs();
     adjK_ = K;
     adjEvery_ = segLen;
-    adjSegs_ = (K + adjEvery_ - 1) / adjEvery_;
+    adjSegs_ = segs;
+    dual_ = dual;
     adjScale_ = radiusScale;
     overlapWeight_ = overlapWeight;
-    tapedSeg_ = -1;
+    tapedSeg_[0] = tapedSeg_[1] = -1;
     snapBytes_ = 0;
     for (const auto& s : persistentState()) snapBytes_ += s.second;
     if (!finalSnap_) {
@@ -942,6 +968,7 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     }
     ckptParity_.assign(size_t(adjSegs_), 0);
     SWARM_HIP_CHECK(hipMemsetAsync(lossAcc_, 0, 8 * size_t(S_), stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(tape_[0].err, 0, 4, stream_));
     lossOn_ = true;   // graph_[1]: the step with the loss terms, for this overlap weight
     if (graphExec_[1] && lossGraphOvw_ != overlapWeight_) {
         (void)hipGraphExecDestroy(graphExec_[1]); graphExec_[1] = nullptr;
@@ -952,7 +979,7 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
         ckptParity_[size_t(seg)] = parity_;
         // the last segment runs taped: the backward starts there and needs no replay of it (a run of one segment,
         // every >= K, is taped whole and never replayed)
-        if (seg == adjSegs_ - 1) recordSegment(seg);
+        if (seg == adjSegs_ - 1) recordSegment(seg, tapeBuf(seg), stream_);
         else step(std::min(adjEvery_, K - seg * adjEvery_));
     }
     lossOn_ = false;
@@ -965,97 +992,105 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
     finalParity_ = parity_;
     double loss = 0.0;
     SWARM_HIP_CHECK(hipMemcpyAsync(&loss, lossAcc_, 8, hipMemcpyDeviceToHost, stream_));
-    sync();
+    checkTapeErrors();   // synchronises
     return loss;
 }
 
 // the forward's final state again, after a segment's replay from its checkpoint moved the state back
 void HipCrowdSim::restoreFinal() {
-    restoreFrom(finalSnap_);
+    restoreFrom(finalSnap_, stream_);
     parity_ = finalParity_;
 }
 
-void HipCrowdSim::tapedStep(int k) {
-    const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
-    const AgentHalf rd = half_[parity_], wr = half_[1 - parity_];
-    const size_t b = n * 4;
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.inx + K * n, rd.x, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.iny + K * n, rd.y, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.invx + K * n, rd.vx, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.invy + K * n, rd.vy, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.inw + K * n, rd.w, b, hipMemcpyDeviceToDevice, stream_));
-    StepBuffers sb = sb_;
-    sb.sid = tape_.sid + K * n; sb.skey = tape_.skey + K * n; sb.bstart = tape_.bstart + K * nbs;
-    sb.sx = tape_.sx + K * n; sb.sy = tape_.sy + K * n; sb.sr = tape_.sr + K * n;
-    sb.egx = tape_.egx + K * n; sb.egy = tape_.egy + K * n;
-    sb.pw = tape_.pw + K * n; sb.pvx = tape_.pvx + K * n; sb.pvy = tape_.pvy + K * n;
-    sb.planes = tape_.planes + K * n;
-    sb.nbr = tape_.nbr + K * size_t(NBR_CAP) * n; sb.nbrD = tape_.nbrD + K * size_t(NBR_CAP) * n; sb.nbrCount = tape_.nbrCount + K * n;
-    sb.avx = rd.vx; sb.avy = rd.vy;
-    GridBuffers gb = grid_;
-    gb.bstart = tape_.bstart + K * nbs;
-    const StepLaunch L = launchDesc();
-    launchGridPasses(stream_, L, rd, gb, sb, rad_, ctrl_);
-    launchChoice(stream_, L, rd, ch_, sb, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
-    TapeStep tp{ tape_.pred + K * n, tape_.fin + K * n, tape_.mask + K * P * n, tape_.capBits + K * n, tape_.planeBits + K * n,
-                 tape_.overlap + K * n, ch_.assign, (drive_ || (!batched_ && L.s.E == 0)) ? 1 : 0, tape_.err };
-    launchPhysicsRecord(stream_, L, sb, tape_.q + K * (P + 1) * n, tape_.acc0, tape_.acc1, rd, wr, ev_, ctrl_, tp);
-    launchOverlap(stream_, L, sb, wr, overlapWeight_, lossAcc_ + 0, &tp);
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.outx + K * n, wr.x, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipMemcpyAsync(tape_.outy + K * n, wr.y, b, hipMemcpyDeviceToDevice, stream_));
-    SWARM_HIP_CHECK(hipGetLastError());
-    parity_ = 1 - parity_;
-}
-
-void HipCrowdSim::tapeSegment(int seg) {
-    if (seg < 0 || seg >= adjSegs_) throw std::invalid_argument("adjoint: no such segment");
-    restoreFrom(ckpt_[size_t(seg)]);
-    parity_ = ckptParity_[size_t(seg)];
-    recordSegment(seg);
-}
-
-// segment 'seg''s taped steps from the current state (its checkpoint, or the forward's own state at its start)
-void HipCrowdSim::recordSegment(int seg) {
-    SWARM_HIP_CHECK(hipMemsetAsync(tape_.err, 0, 4, stream_));
-    const int s0 = seg * adjEvery_, s1 = std::min(adjK_, s0 + adjEvery_);
-    const int len = s1 - s0;
-    // a full segment of at most ADJ_GRAPH_MAX_STEPS steps: one captured replay per starting parity (longer graphs
-    // launch slower than the same kernels launched eagerly)
-    if (opt_.useGraph && len == adjEvery_ && len <= ADJ_GRAPH_MAX_STEPS) {
-        const int p = parity_;
-        if (!replayExec_[p]) {
-            SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < len; ++k) tapedStep(k);
-            SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &replayGraph_[p]));
-            SWARM_HIP_CHECK(hipGraphInstantiate(&replayExec_[p], replayGraph_[p], nullptr, nullptr, 0));
-        }
-        SWARM_HIP_CHECK(hipGraphLaunch(replayExec_[p], stream_));
-        parity_ = (len & 1) ? 1 - p : p;
-    } else {
-        for (int k = 0; k < len; ++k) tapedStep(k);
-    }
+// the replays' error word (synchronises the context's stream)
+void HipCrowdSim::checkTapeErrors() {
     uint32_t err = 0;
-    SWARM_HIP_CHECK(hipMemcpyAsync(&err, tape_.err, 4, hipMemcpyDeviceToHost, stream_));
+    SWARM_HIP_CHECK(hipMemcpyAsync(&err, tape_[0].err, 4, hipMemcpyDeviceToHost, stream_));
     sync();
-    tapedSeg_ = seg;
     if (err & 1u) throw std::runtime_error("adjoint: a slot has more than 48 contact candidates (unsupported by the adjoint)");
     if (err & 2u) throw std::runtime_error("adjoint: more wall pushes in one resolveStatic than the tape holds");
 }
 
-AdjStep HipCrowdSim::adjStepView(int k) const {
+// one taped step into tape buffer 'buf' (its step k), on stream st
+void HipCrowdSim::tapedStep(int buf, int k, hipStream_t st) {
+    const Tape& T = tape_[buf];
+    const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
+    const AgentHalf rd = half_[parity_], wr = half_[1 - parity_];
+    const size_t b = n * 4;
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.inx + K * n, rd.x, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.iny + K * n, rd.y, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.invx + K * n, rd.vx, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.invy + K * n, rd.vy, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.inw + K * n, rd.w, b, hipMemcpyDeviceToDevice, st));
+    StepBuffers sb = sb_;
+    sb.sid = T.sid + K * n; sb.skey = T.skey + K * n; sb.bstart = T.bstart + K * nbs;
+    sb.sx = T.sx + K * n; sb.sy = T.sy + K * n; sb.sr = T.sr + K * n;
+    sb.egx = T.egx + K * n; sb.egy = T.egy + K * n;
+    sb.pw = T.pw + K * n; sb.pvx = T.pvx + K * n; sb.pvy = T.pvy + K * n;
+    sb.planes = T.planes + K * n;
+    sb.nbr = T.nbr + K * size_t(NBR_CAP) * n; sb.nbrD = T.nbrD + K * size_t(NBR_CAP) * n; sb.nbrCount = T.nbrCount + K * n;
+    sb.avx = rd.vx; sb.avy = rd.vy;
+    GridBuffers gb = grid_;
+    gb.bstart = T.bstart + K * nbs;
+    const StepLaunch L = launchDesc();
+    launchGridPasses(st, L, rd, gb, sb, rad_, ctrl_);
+    launchChoice(st, L, rd, ch_, sb, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
+    TapeStep tp{ T.pred + K * n, T.fin + K * n, T.mask + K * P * n, T.capBits + K * n, T.planeBits + K * n,
+                 T.overlap + K * n, ch_.assign, (drive_ || (!batched_ && L.s.E == 0)) ? 1 : 0, T.err };
+    launchPhysicsRecord(st, L, sb, T.q + K * (P + 1) * n, T.acc0, T.acc1, rd, wr, ev_, ctrl_, tp);
+    launchOverlap(st, L, sb, wr, overlapWeight_, lossAcc_ + 0, &tp);
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.outx + K * n, wr.x, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipMemcpyAsync(T.outy + K * n, wr.y, b, hipMemcpyDeviceToDevice, st));
+    SWARM_HIP_CHECK(hipGetLastError());
+    parity_ = 1 - parity_;
+}
+
+// segment 'seg' replayed from its checkpoint into its tape buffer, on the context's stream (synchronous)
+void HipCrowdSim::tapeSegment(int seg) {
+    if (seg < 0 || seg >= adjSegs_) throw std::invalid_argument("adjoint: no such segment");
+    restoreFrom(ckpt_[size_t(seg)], stream_);
+    parity_ = ckptParity_[size_t(seg)];
+    recordSegment(seg, tapeBuf(seg), stream_);
+}
+
+// segment 'seg''s taped steps from the current state (its checkpoint, or the forward's own state at its start) into
+// tape buffer 'buf', enqueued on stream st; the replays' errors are read by checkTapeErrors
+void HipCrowdSim::recordSegment(int seg, int buf, hipStream_t st) {
+    const int s0 = seg * adjEvery_, s1 = std::min(adjK_, s0 + adjEvery_);
+    const int len = s1 - s0;
+    // a full segment of at most ADJ_GRAPH_MAX_STEPS steps: one captured replay per tape buffer and starting parity
+    // (longer graphs launch slower than the same kernels launched eagerly)
+    if (opt_.useGraph && len == adjEvery_ && len <= ADJ_GRAPH_MAX_STEPS) {
+        const int p = parity_;
+        if (!replayExec_[buf][p]) {
+            SWARM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < len; ++k) tapedStep(buf, k, st);
+            SWARM_HIP_CHECK(hipStreamEndCapture(st, &replayGraph_[buf][p]));
+            SWARM_HIP_CHECK(hipGraphInstantiate(&replayExec_[buf][p], replayGraph_[buf][p], nullptr, nullptr, 0));
+        }
+        SWARM_HIP_CHECK(hipGraphLaunch(replayExec_[buf][p], st));
+        parity_ = (len & 1) ? 1 - p : p;
+    } else {
+        for (int k = 0; k < len; ++k) tapedStep(buf, k, st);
+    }
+    tapedSeg_[buf] = seg;
+}
+
+AdjStep HipCrowdSim::adjStepView(int buf, int k) const {
+    const Tape& T = tape_[buf];
     const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
     AdjStep t{};
     t.n = n_; t.P = mp_.projIters;
-    t.sid = tape_.sid + K * n; t.skey = tape_.skey + K * n; t.bstart = tape_.bstart + K * nbs;
-    t.sx = tape_.sx + K * n; t.sy = tape_.sy + K * n; t.sr = tape_.sr + K * n;
-    t.invx = tape_.invx + K * n; t.invy = tape_.invy + K * n;
-    t.outx = tape_.outx + K * n; t.outy = tape_.outy + K * n;
-    t.pred = tape_.pred + K * n; t.fin = tape_.fin + K * n;
-    t.nbr = tape_.nbr + K * size_t(NBR_CAP) * n; t.cnt = tape_.nbrCount + K * n;
-    t.mask = tape_.mask + K * P * n; t.capBits = tape_.capBits + K * n; t.planeBits = tape_.planeBits + K * n;
-    t.overlap = tape_.overlap + K * n;
-    t.q = tape_.q + K * (P + 1) * n;
-    t.planes = tape_.planes + K * n;
+    t.sid = T.sid + K * n; t.skey = T.skey + K * n; t.bstart = T.bstart + K * nbs;
+    t.sx = T.sx + K * n; t.sy = T.sy + K * n; t.sr = T.sr + K * n;
+    t.invx = T.invx + K * n; t.invy = T.invy + K * n;
+    t.outx = T.outx + K * n; t.outy = T.outy + K * n;
+    t.pred = T.pred + K * n; t.fin = T.fin + K * n;
+    t.nbr = T.nbr + K * size_t(NBR_CAP) * n; t.cnt = T.nbrCount + K * n;
+    t.mask = T.mask + K * P * n; t.capBits = T.capBits + K * n; t.planeBits = T.planeBits + K * n;
+    t.overlap = T.overlap + K * n;
+    t.q = T.q + K * (P + 1) * n;
+    t.planes = T.planes + K * n;
     t.v0 = v0_;
     t.local = local_ ? 1 : 0;
     return t;
@@ -1077,39 +1112,80 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.gR, 0, n * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.gtau, 0, S * 4, stream_));
     SWARM_HIP_CHECK(hipMemsetAsync(adj_.ggain, 0, S * 4, stream_));
+    SWARM_HIP_CHECK(hipMemsetAsync(tape_[0].err, 0, 4, stream_));
+    // Pipelined (two tape buffers): segment s runs backwards on the context's stream while segment s - 1 replays from
+    // its checkpoint into the other buffer on stream2_. A replay writes the live state and its own buffer only; the
+    // backward reads its buffer, the adjoint's buffers and constants. Events order the buffer reuse: a replay waits
+    // for the backward that last read its buffer, a backward for the replay that filled it.
+    const bool pipe = dual_ && adjSegs_ > 1;
+    bool replayPending[2] = { false, false }, backDone[2] = { false, false };
+    if (pipe) {
+        if (!stream2_) SWARM_HIP_CHECK(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b) {
+            if (!evReplay_[b]) SWARM_HIP_CHECK(hipEventCreateWithFlags(&evReplay_[b], hipEventDisableTiming));
+            if (!evBack_[b]) SWARM_HIP_CHECK(hipEventCreateWithFlags(&evBack_[b], hipEventDisableTiming));
+        }
+        if (!evMain_) SWARM_HIP_CHECK(hipEventCreateWithFlags(&evMain_, hipEventDisableTiming));
+        SWARM_HIP_CHECK(hipEventRecord(evMain_, stream_));
+        SWARM_HIP_CHECK(hipStreamWaitEvent(stream2_, evMain_, 0));
+    }
+    auto enqueueReplay = [&](int seg) {   // on stream2_
+        const int b = tapeBuf(seg);
+        if (tapedSeg_[b] == seg) return;
+        if (backDone[b]) SWARM_HIP_CHECK(hipStreamWaitEvent(stream2_, evBack_[b], 0));
+        restoreFrom(ckpt_[size_t(seg)], stream2_);
+        parity_ = ckptParity_[size_t(seg)];
+        recordSegment(seg, b, stream2_);
+        SWARM_HIP_CHECK(hipEventRecord(evReplay_[b], stream2_));
+        replayPending[b] = true;
+    };
     for (int seg = adjSegs_ - 1; seg >= 0; --seg) {
-        if (tapedSeg_ != seg) tapeSegment(seg);
+        const int buf = tapeBuf(seg);
+        if (pipe) {
+            enqueueReplay(seg);   // (already there: the forward's last segment, or enqueued one segment earlier)
+            if (replayPending[buf]) {
+                SWARM_HIP_CHECK(hipStreamWaitEvent(stream_, evReplay_[buf], 0));
+                replayPending[buf] = false;
+            }
+            if (seg > 0) enqueueReplay(seg - 1);   // runs while this segment goes backwards
+        } else if (tapedSeg_[buf] != seg) {
+            tapeSegment(seg);
+        }
         const int len = std::min(adjK_, (seg + 1) * adjEvery_) - seg * adjEvery_;
-        // a full segment whose clearing pattern repeats from segment to segment: one captured backward per
-        // ping-pong half; the segment's first state comes from the device word stateBase_
+        // a full segment whose clearing pattern repeats from segment to segment: one captured backward per tape
+        // buffer and ping-pong half; the segment's first state comes from the device word stateBase_
         if (opt_.useGraph && len == adjEvery_ && len <= ADJ_GRAPH_MAX_STEPS && (window_ == 0 || adjEvery_ % window_ == 0)) {
             launchAdjointSetBase(stream_, stateBase_, seg * adjEvery_);
             const int h = adj_.gx == adjG0_ ? 0 : 1;
             const AdjBufs before = adj_;
-            if (!backExec_[h]) {
+            if (!backExec_[buf][h]) {
                 SWARM_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-                backwardSegment(-1, len);
-                SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &backGraph_[h]));
-                SWARM_HIP_CHECK(hipGraphInstantiate(&backExec_[h], backGraph_[h], nullptr, nullptr, 0));
+                backwardSegment(-1, len, buf);
+                SWARM_HIP_CHECK(hipStreamEndCapture(stream_, &backGraph_[buf][h]));
+                SWARM_HIP_CHECK(hipGraphInstantiate(&backExec_[buf][h], backGraph_[buf][h], nullptr, nullptr, 0));
                 adj_ = before;
             }
-            SWARM_HIP_CHECK(hipGraphLaunch(backExec_[h], stream_));
+            SWARM_HIP_CHECK(hipGraphLaunch(backExec_[buf][h], stream_));
             if (len & 1) {
                 std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
                 std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
             }
         } else {
-            backwardSegment(seg, len);
+            backwardSegment(seg, len, buf);
         }
         SWARM_HIP_CHECK(hipGetLastError());
+        if (pipe) {
+            SWARM_HIP_CHECK(hipEventRecord(evBack_[buf], stream_));
+            backDone[buf] = true;
+        }
     }
-    if (adjSegs_ > 1) restoreFinal();   // a segment's replay restored an earlier state
+    if (adjSegs_ > 1) restoreFinal();   // a segment's replay restored an earlier state (the last replay ran before this)
     std::vector<float> gR(n), gtau(S), ggain(S);
     SWARM_HIP_CHECK(hipMemcpyAsync(gradV0, adj_.gv0, n * 4, hipMemcpyDeviceToHost, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(gR.data(), adj_.gR, n * 4, hipMemcpyDeviceToHost, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(gtau.data(), adj_.gtau, S * 4, hipMemcpyDeviceToHost, stream_));
     SWARM_HIP_CHECK(hipMemcpyAsync(ggain.data(), adj_.ggain, S * 4, hipMemcpyDeviceToHost, stream_));
-    sync();
+    checkTapeErrors();   // synchronises
     // per scenario: dL/dtau_s (tau = tau_s / T), dL/dscale = sum r0 dL/dr, dL/dgain
     std::vector<double> gs(S, 0.0);
     std::vector<uint32_t> base(S + 1, 0);
@@ -1134,9 +1210,9 @@ void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
                                 "horizon exceeds the float range (shorten K or use the truncation window)");
 }
 
-// the backward of one segment's 'len' steps from its tape; seg < 0: the segment's first state is read from stateBase_
-// (the captured form), and the clearing pattern is that of a segment starting at a multiple of the window
-void HipCrowdSim::backwardSegment(int seg, int len) {
+// the backward of one segment's 'len' steps from tape buffer 'buf'; seg < 0: the segment's first state is read from
+// stateBase_ (the captured form), and the clearing pattern is that of a segment starting at a multiple of the window
+void HipCrowdSim::backwardSegment(int seg, int len, int buf) {
     const StepLaunch L = launchDesc();
     for (int k = len - 1; k >= 0; --k) {
         const int rel = seg < 0 ? k : seg * adjEvery_ + k;   // the step (relative to the segment when captured)
@@ -1144,7 +1220,7 @@ void HipCrowdSim::backwardSegment(int seg, int len) {
             if (seg < 0) launchAdjointEventSeedRel(stream_, n_, stateBase_, k + 1, seedStep_, seedIn_, seedOut_, adj_);
             else launchAdjointEventSeed(stream_, n_, rel + 1, seedStep_, seedIn_, seedOut_, adj_);
         }
-        launchAdjointStep(stream_, L, adjStepView(k), adj_, overlapWeight_);
+        launchAdjointStep(stream_, L, adjStepView(buf, k), adj_, overlapWeight_);
         std::swap(adj_.gx, adj_.hx); std::swap(adj_.gy, adj_.hy);
         std::swap(adj_.gvx, adj_.hvx); std::swap(adj_.gvy, adj_.hvy);
         if (window_ > 0 && rel % window_ == 0) launchAdjointZero(stream_, n_, adj_);
@@ -1154,26 +1230,28 @@ void HipCrowdSim::backwardSegment(int seg, int len) {
 void HipCrowdSim::adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit) {
     if (adjK_ <= 0 || step < 0 || step >= adjK_) throw std::invalid_argument("adjointTape: no such step of the last forward");
     const int seg = step / adjEvery_;
-    if (tapedSeg_ != seg) tapeSegment(seg);
+    const int buf = tapeBuf(seg);
+    if (tapedSeg_[buf] != seg) tapeSegment(seg);
+    const Tape& T = tape_[buf];
     const size_t n = size_t(n_), K = size_t(step - seg * adjEvery_), P = size_t(mp_.projIters);
     auto get = [&](void* dst, const void* src, size_t bytes) {
         if (dst) SWARM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
     };
-    get(out.x, tape_.inx + K * n, n * 4); get(out.y, tape_.iny + K * n, n * 4);
-    get(out.vx, tape_.invx + K * n, n * 4); get(out.vy, tape_.invy + K * n, n * 4); get(out.w, tape_.inw + K * n, n * 4);
-    get(out.sid, tape_.sid + K * n, n * 4);
-    get(out.pred, tape_.pred + K * n, n * sizeof(swarm_predict_rec));
-    get(out.fin, tape_.fin + K * n, n * sizeof(swarm_final_rec));
-    get(out.sx, tape_.sx + K * n, n * 4); get(out.sy, tape_.sy + K * n, n * 4); get(out.sr, tape_.sr + K * n, n * 4);
-    get(out.nbr, tape_.nbr + K * size_t(NBR_CAP) * n, size_t(NBR_CAP) * n * 4);
-    get(out.nbr_count, tape_.nbrCount + K * n, n * 4);
-    get(out.mask, tape_.mask + K * P * n, P * n * 8);
-    get(out.cap_bits, tape_.capBits + K * n, n * 8);
-    get(out.plane_bits, tape_.planeBits + K * n, n * 8);
-    get(out.overlap, tape_.overlap + K * n, n * 8);
+    get(out.x, T.inx + K * n, n * 4); get(out.y, T.iny + K * n, n * 4);
+    get(out.vx, T.invx + K * n, n * 4); get(out.vy, T.invy + K * n, n * 4); get(out.w, T.inw + K * n, n * 4);
+    get(out.sid, T.sid + K * n, n * 4);
+    get(out.pred, T.pred + K * n, n * sizeof(swarm_predict_rec));
+    get(out.fin, T.fin + K * n, n * sizeof(swarm_final_rec));
+    get(out.sx, T.sx + K * n, n * 4); get(out.sy, T.sy + K * n, n * 4); get(out.sr, T.sr + K * n, n * 4);
+    get(out.nbr, T.nbr + K * size_t(NBR_CAP) * n, size_t(NBR_CAP) * n * 4);
+    get(out.nbr_count, T.nbrCount + K * n, n * 4);
+    get(out.mask, T.mask + K * P * n, P * n * 8);
+    get(out.cap_bits, T.capBits + K * n, n * 8);
+    get(out.plane_bits, T.planeBits + K * n, n * 8);
+    get(out.overlap, T.overlap + K * n, n * 8);
     if (exit && step == adjK_ - 1) get(exit, exitRec_, n * sizeof(swarm_exit_rec));
     if (adjSegs_ > 1) restoreFinal();
-    sync();
+    checkTapeErrors();   // synchronises
 }
 
 } // namespace swarm

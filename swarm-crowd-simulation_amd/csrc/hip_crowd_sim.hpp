@@ -168,7 +168,7 @@ private:
     void buildGraph();
     void dropGraph();
     void dropAdjointGraphs();
-    void backwardSegment(int seg, int len);
+    void backwardSegment(int seg, int len, int buf);
     StepLaunch launchDesc() const;
     template <class T> T* alloc(size_t count);
     void collectTimings();
@@ -176,11 +176,13 @@ private:
     std::vector<std::pair<void*, size_t>> persistentState() const;
     // adjoint helpers
     void snapshotTo(char* dst);
-    void restoreFrom(const char* src);
-    void allocAdjoint(int every);
-    void tapedStep(int k);
+    void restoreFrom(const char* src, hipStream_t st);
+    void allocAdjoint(int every, bool dual);
+    void tapedStep(int buf, int k, hipStream_t st);
     void tapeSegment(int seg);
-    void recordSegment(int seg);
+    void recordSegment(int seg, int buf, hipStream_t st);
+    int tapeBuf(int seg) const { return dual_ ? (seg & 1) : 0; }
+    void checkTapeErrors();
 public:
     size_t adjointBytes() const { return adjBytes_; }   // device memory the adjoint holds (tape, buffers, checkpoints)
     // a run whose whole tape fits this many bytes is taped in its forward and never recomputed
@@ -188,7 +190,7 @@ public:
     size_t tapeBytesPerStep() const;
     int adjointSegments() const { return adjSegs_; }
 private:
-    AdjStep adjStepView(int k) const;
+    AdjStep adjStepView(int buf, int k) const;
 
     std::vector<DeviceScene> scenes_;
     SimOptions opt_;
@@ -221,8 +223,9 @@ private:
     float lossGraphOvw_ = 0.0f;          // the overlap weight captured in graph_[1]
     // the adjoint's graphs: a full segment's taped replay (by starting parity) and backward (by which half of the
     // state adjoint's ping-pong holds the output), for the current segment length and loss terms
-    hipGraph_t replayGraph_[2]{}, backGraph_[2]{};
-    hipGraphExec_t replayExec_[2]{}, backExec_[2]{};
+    // (per tape buffer, then parity / half)
+    hipGraph_t replayGraph_[2][2]{}, backGraph_[2][2]{};
+    hipGraphExec_t replayExec_[2][2]{}, backExec_[2][2]{};
     int* stateBase_ = nullptr;           // device: the first state of the segment a captured backward runs
     float* adjG0_ = nullptr;             // adj_.gx as allocated (the ping-pong's reference half)
     // timing: events per stamp (start, sorted, choice, predict, projection, final) per recorded step
@@ -246,7 +249,13 @@ private:
     bool seedsOn_ = false, exitTermOn_ = true;
     int window_ = 0;
     bool local_ = false;
-    int adjK_ = 0, adjEvery_ = 0, adjSegs_ = 0, adjCap_ = 0, tapedSeg_ = -1;
+    int adjK_ = 0, adjEvery_ = 0, adjSegs_ = 0, adjCap_ = 0;
+    int tapedSeg_[2] = { -1, -1 };      // the segment each tape buffer holds (-1: none)
+    // two segment tapes: the backward of a segment runs on the context's stream while the replay of the segment before
+    // it, from its checkpoint into the other tape, runs on stream2_ (the replays keep to stream2_, one after another)
+    bool dual_ = false;
+    hipStream_t stream2_ = nullptr;
+    hipEvent_t evReplay_[2]{}, evBack_[2]{}, evMain_ = nullptr;
     int tapeP_ = 0;                      // the projection iterations the tape's mask and position arrays are sized for
     char* finalSnap_ = nullptr;          // device: the forward's final state, put back after a recompute
     int finalParity_ = 0;
@@ -269,7 +278,7 @@ private:
         float2* q;
         float *acc0, *acc1;
         uint32_t* err;
-    } tape_{};
+    } tape_[2]{};
     AdjBufs adj_{};
     std::vector<void*> adjBufs_;         // the adjoint's own allocations
     size_t adjBytes_ = 0;                // their bytes, checkpoints included
