@@ -72,6 +72,7 @@ typedef struct {
     float seconds_per_unit, dist_quantum;
     double build_seconds;
     int n_walls, wall_bin_entries;   /* wall boxes; entries of their index on the hash grid */
+    int from_cache;                  /* 1: the tables were loaded from the scene cache */
 } swarm_scene_info;
 
 /* nav::build + buildChoiceTable (src/nav_field.cpp:344-366, src/exit_choice.cpp:143-317) and the SDF bake
@@ -79,6 +80,14 @@ typedef struct {
 int swarm_scene_build(const swarm_scene_desc* desc, float max_radius, float max_v0, int threads, swarm_scene** out);
 /* The same for a benchmark floor (SWARM_LAYOUT_*) */
 int swarm_scene_build_layout(int layout, float max_radius, float max_v0, int threads, swarm_scene** out);
+/* The same through the checksummed scene cache in cache_dir (the reference's navigation / choice caches,
+ * src/nav_field.cpp:468-688, src/exit_choice.cpp:335-445): the file scene_<key>.bin, key = FNV-1a 64 of everything the
+ * tables depend on, holds magic + payload + the payload's FNV-1a checksum; a valid file is loaded (info.from_cache 1),
+ * an absent, foreign or damaged one is replaced by a fresh build (written to a temporary file, then renamed). NULL or
+ * "": no cache. The tables are the same bit for bit either way. */
+int swarm_scene_build_cached(const swarm_scene_desc* desc, float max_radius, float max_v0, int threads, const char* cache_dir,
+                             swarm_scene** out);
+int swarm_scene_build_layout_cached(int layout, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out);
 int swarm_scene_info_get(const swarm_scene* s, swarm_scene_info* info);
 /* Copies of the tables; any pointer may be NULL. Sizes: sdf, sdf_min sdf_nx*sdf_ny; nav_info, stair
  * nav_res_x*nav_res_y; nav_slot, nav_dist cells*K; inv_cap_s E; exit_rects 4E. */

@@ -82,11 +82,21 @@ struct Scene {
     float metricCellsPerUnit = 0.0f, metricOffX = 0.0f, metricOffY = 0.0f;
     std::vector<float> metricWalkableM2;
     double buildSeconds = 0.0;
+    int fromCache = 0;                     // the tables came from the scene cache
 };
 
 // Build every table of a scene. maxRadius / maxV0: the largest agent radius and free speed the scene
 // will carry (they size the projection ring and the SDF-min window).
 Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads = 0);
+
+// --- the checksummed scene cache (the reference's navigation and choice caches, src/nav_field.cpp:468-688,
+// src/exit_choice.cpp:335-445) ---
+uint64_t sceneCacheKey(const SceneDesc& d, float maxRadius, float maxV0);
+std::string sceneCacheFile(const std::string& dir, uint64_t key);
+bool saveScene(const Scene& s, uint64_t key, const std::string& file);
+bool loadScene(Scene& s, uint64_t key, const std::string& file);   // false: absent, foreign or damaged
+// buildScene through the cache in 'dir' (empty: no cache): a valid file is loaded, else the scene is built and saved
+Scene buildSceneCached(const SceneDesc& d, float maxRadius, float maxV0, int threads, const std::string& dir, int* fromCache);
 
 // --- the navigation builder, exposed for known-answer tests ---
 enum NavClass : uint8_t { NAV_CLASS_FIT = 0, NAV_CLASS_NOFIT = 1 };

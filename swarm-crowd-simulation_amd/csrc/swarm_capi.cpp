@@ -67,6 +67,41 @@ int swarm_scene_build(const swarm_scene_desc* d, float max_radius, float max_v0,
     });
 }
 
+namespace {
+swarm::SceneDesc descOf(const swarm_scene_desc* d) {
+    swarm::SceneDesc sd;
+    sd.xmin = d->xmin; sd.ymin = d->ymin; sd.xmax = d->xmax; sd.ymax = d->ymax;
+    sd.hashCell = d->hash_cell; sd.sdfSpacing = d->sdf_spacing; sd.sdfClamp = d->sdf_clamp;
+    sd.nav.resX = d->nav_res_x; sd.nav.resY = d->nav_res_y; sd.nav.originX = d->nav_origin_x; sd.nav.originY = d->nav_origin_y;
+    sd.nav.cell = d->nav_cell; sd.nav.invCell = d->nav_inv_cell; sd.navBody = d->nav_body;
+    sd.metresPerUnit = d->metres_per_unit;
+    sd.walls = rects(d->walls, d->n_walls);
+    sd.exits = rects(d->exits, d->n_exits);
+    sd.stairs = rects(d->stairs, d->n_stairs);
+    for (int i = 0; i < d->n_exits; ++i) sd.exitWidthM.push_back(d->exit_width_m ? d->exit_width_m[i] : 0.0f);
+    for (int i = 0; i < d->n_stairs; ++i) sd.stairFactor.push_back(d->stair_factor[i]);
+    if (sd.nav.resX <= 0 || sd.nav.resY <= 0 || !(sd.nav.cell > 0.0f) || !(sd.sdfSpacing > 0.0f))
+        throw std::invalid_argument("swarm_scene_build: empty grid");
+    return sd;
+}
+}  // namespace
+
+int swarm_scene_build_cached(const swarm_scene_desc* d, float max_radius, float max_v0, int threads, const char* cache_dir,
+                             swarm_scene** out) {
+    if (!d || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        *out = new swarm_scene{ swarm::buildSceneCached(descOf(d), max_radius, max_v0, threads, cache_dir ? cache_dir : "", nullptr) };
+    });
+}
+
+int swarm_scene_build_layout_cached(int layout, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out) {
+    if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        *out = new swarm_scene{ swarm::buildSceneCached(swarm::layoutDesc(layout), max_radius, max_v0, threads, cache_dir ? cache_dir : "",
+                                                        nullptr) };
+    });
+}
+
 int swarm_scene_build_layout(int layout, float max_radius, float max_v0, int threads, swarm_scene** out) {
     if (!out) { g_err = "null argument"; return SWARM_EINVAL; }
     return guarded([&] { *out = new swarm_scene{ swarm::buildScene(swarm::layoutDesc(layout), max_radius, max_v0, threads) }; });
@@ -86,6 +121,7 @@ int swarm_scene_info_get(const swarm_scene* sc, swarm_scene_info* i) {
     i->build_seconds = s.buildSeconds;
     i->n_walls = int(s.walls.size());
     i->wall_bin_entries = int(s.wallBinList.size());
+    i->from_cache = s.fromCache;
     return SWARM_OK;
 }
 

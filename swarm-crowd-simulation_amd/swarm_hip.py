@@ -43,7 +43,7 @@ class SceneInfo(C.Structure):
                 ("sdf_min_win", C.c_int), ("nav_res_x", C.c_int), ("nav_res_y", C.c_int),
                 ("nav_origin_x", C.c_float), ("nav_origin_y", C.c_float), ("nav_cell", C.c_float), ("nav_inv_cell", C.c_float),
                 ("K", C.c_int), ("E", C.c_int), ("seconds_per_unit", C.c_float), ("dist_quantum", C.c_float),
-                ("build_seconds", C.c_double), ("n_walls", C.c_int), ("wall_bin_entries", C.c_int)]
+                ("build_seconds", C.c_double), ("n_walls", C.c_int), ("wall_bin_entries", C.c_int), ("from_cache", C.c_int)]
 
 
 class Config(C.Structure):
@@ -152,9 +152,15 @@ class Scene:
         self.info = inf
 
     @classmethod
-    def layout(cls, layout, max_radius=0.25 / M_PER_UNIT, max_v0=2.0 * S_PER_UNIT / M_PER_UNIT, threads=0):
+    def layout(cls, layout, max_radius=0.25 / M_PER_UNIT, max_v0=2.0 * S_PER_UNIT / M_PER_UNIT, threads=0, cache_dir=None):
+        """A benchmark floor's tables; cache_dir: through the checksummed scene cache there."""
         h = C.c_void_p()
-        check(lib().swarm_scene_build_layout(layout, max_radius, max_v0, threads, C.byref(h)))
+        if cache_dir:
+            L = lib()
+            L.swarm_scene_build_layout_cached.argtypes = [C.c_int, C.c_float, C.c_float, C.c_int, C.c_char_p, C.c_void_p]
+            check(L.swarm_scene_build_layout_cached(layout, max_radius, max_v0, threads, str(cache_dir).encode(), C.byref(h)))
+        else:
+            check(lib().swarm_scene_build_layout(layout, max_radius, max_v0, threads, C.byref(h)))
         return cls(h)
 
     @classmethod
