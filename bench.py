@@ -128,7 +128,7 @@ def cpu_baseline(bowl_scene, bowl_agents, full=True):
     out["configs"]["bowl_63424"] = bowl
     if full:
         room_scene = sh.Scene.layout(sh.LAYOUT_ROOM)
-        room = sh.place_agents(sh.LAYOUT_ROOM, 4096, 4096, SEED)
+        room = sh.spawn_uniform(sh.LAYOUT_ROOM, 4096, 4096, SEED)[0]   # uniform with rejection, relaxed
         out["configs"]["room_4096"] = {"all_cores": cpu_snapshot_rate(room_scene, room, 4096, cores, 5, 20),
                                        "one_thread": cpu_snapshot_rate(room_scene, room, 4096, 1, 2, 8),
                                        "gpu": gpu_plain_rate(room_scene, room, 4096)}
@@ -194,13 +194,13 @@ def batch_bench(steps, warmup, loop_steps, use_graph=True):
 
 
 def adjoint_bench(K=200, every=20, n=4096, use_graph=True):
-    """The differentiable configuration: 4,096 agents in the 40 x 30 m room, K steps, checkpoints every 20. The
+    """The differentiable configuration: 4,096 agents placed uniformly in the 40 x 30 m room, K steps, checkpoints every 20. The
     forward is K plain steps (graph replays); the adjoint is the adjoint's forward (the same steps with the loss
     terms) plus its backward. By default the whole run's tape fits the tape budget, so the forward tapes it and
     nothing is recomputed; 'recompute' is the same run with a zero budget (each segment but the last replayed
     from its checkpoint with the tape, then the adjoint kernels)."""
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
-    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, SEED)
+    a = sh.spawn_uniform(sh.LAYOUT_ROOM, n, n, SEED)[0]   # the small floor's placement: uniform with rejection, relaxed
     sim = sh.Sim(scene, n, use_graph=use_graph)
     sim.upload(a)
     tune_ms = sim.autotune(5)   # launch shape and lanes per slot for this crowd (the state is restored)

@@ -110,6 +110,16 @@ void swarm_scene_destroy(swarm_scene* s);
  * slots, padded to 'total' with parked agents; r ~ U[0.20, 0.25] m, v0 ~ N(mean, sd) m/s clipped to [0.5, 2.0]. */
 int swarm_place_agents(int layout, size_t active, size_t total, uint64_t seed, double speed_mean, double speed_sd,
                        float* x, float* y, float* r, float* v0, float* w);
+/* The reference's seat spawn (seatedSpawnSeats, src/crowd_sim.cpp:652-674): round(load * seat_count) of seat_count seats
+ * chosen by a partial Fisher-Yates on mt19937's raw output from seed, in seat order, padded with -1 to n (chosen[n]);
+ * SWARM_EINVAL when more seats are chosen than n. The same seats as the reference for the same arguments. */
+int swarm_spawn_seats(int seat_count, int n, double load, uint32_t seed, int32_t* chosen);
+/* An open floor (SWARM_LAYOUT_ROOM, _CORRIDOR, _PILLAR): 'active' agents placed uniformly with rejection (mt19937's raw
+ * output), then relaxed by Gauss-Seidel sweeps over overlapping pairs (relaxSpawn, src/crowd_sim.cpp:544-597, with
+ * per-agent radii), padded to 'total' with parked agents; r ~ U[0.20, 0.25] m, v0 ~ N(mean, sd) m/s clipped to
+ * [0.5, 2.0]. sweeps / max_overlap (may be NULL): the relaxation's sweeps and its last largest overlap (model units). */
+int swarm_spawn_uniform(int layout, size_t active, size_t total, uint32_t seed, double speed_mean, double speed_sd, float* x, float* y,
+                        float* r, float* v0, float* w, int* sweeps, float* max_overlap);
 /* Slots of a benchmark floor (capacity query; x, y may be NULL) */
 int swarm_layout_slots(int layout, float* x, float* y, size_t cap, size_t* count);
 

@@ -7,6 +7,7 @@
 // Record format: u32 name length, name, u8 dtype ('f' float32, 'u' uint32, 'i' int32, 'b' uint8, 'd' float64),
 // u32 ndim, u64 dims[ndim], raw little-endian data.
 #include <swarm/cpu_crowd_sim.hpp>
+#include <swarm/crowd_sim.hpp>
 #include <swarm/exit_choice.hpp>
 #include <swarm/metrics.hpp>
 #include <swarm/nav_field.hpp>
@@ -462,6 +463,31 @@ void metricsCases() {
     }
     put("metrics/run/cells", 'u', cells, { uint64_t(CELLS), uint64_t(sizeof(GpuMetricCell) / 4) });
 }
+// The reference's seat spawn (seatedSpawnSeats, src/crowd_sim.cpp:652-674): a partial Fisher-Yates over the seat indices
+// on mt19937's raw output, the chosen seats in seat order, padded with -1, on venues S, M and L
+void spawnCases() {
+    const char* names[3] = { "S", "M", "L" };
+    for (int v = 0; v < 3; ++v) {
+        const Venue& venue = venueByName(names[v]);
+        const int seats = int(seated::seats(venue.layout).size());
+        int pow2 = 256;
+        while (pow2 < seats) pow2 <<= 1;
+        struct Case { int n; double load; unsigned seed; };
+        const Case cases[] = { { seats, 1.0, 1337u }, { pow2, 1.0, 1337u }, { pow2, 0.5, 7u }, { seats, 0.3, 42u }, { pow2, 0.05, 2024u } };
+        std::vector<int32_t> out, meta;
+        for (const Case& c : cases) {
+            const std::vector<int> ch = seatedSpawnSeats(venue, c.n, c.load, c.seed);
+            meta.push_back(c.n);
+            meta.push_back(int32_t(std::lround(c.load * 1000.0)));
+            meta.push_back(int32_t(c.seed));
+            out.insert(out.end(), ch.begin(), ch.end());
+        }
+        put(std::string("spawn/") + names[v] + "/seats", 'i', std::vector<int32_t>{ seats });
+        put(std::string("spawn/") + names[v] + "/cases", 'i', meta);
+        put(std::string("spawn/") + names[v] + "/chosen", 'i', out);
+    }
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
@@ -478,6 +504,7 @@ int main(int argc, char** argv) {
     terraceCase("t_bdense", vol, 1024, 0.085f, 0.0085f, 8, 4242, 37, true, 5, CpuCrowdSim::SortMode::GpuBitonic);
     choiceCases();
     metricsCases();
+    spawnCases();
     {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents
         const std::vector<Rect> walls = { { { -0.3f, -0.3f }, { 20.3f, 0.0f } }, { { -0.3f, 0.0f }, { 0.0f, 2.3f } }, { { 0.0f, 2.0f }, { 18.0f, 2.3f } },
                                           { { 17.7f, 2.3f }, { 18.0f, 15.0f } }, { { 20.0f, 0.0f }, { 20.3f, 15.0f } } };

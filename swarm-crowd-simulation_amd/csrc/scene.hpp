@@ -9,6 +9,7 @@
 // plus seeded agent placements for the benchmark floors.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -135,5 +136,16 @@ void layoutSlots(int layout, std::vector<float>& sx, std::vector<float>& sy);
 // sites with jitter; bowls: the first 'active' seats in slot order, jittered), padded with parked agents to
 // 'total'. Radii U[0.20, 0.25] m, free speed N(1.34, 0.26) m/s clipped to [0.5, 2.0].
 Agents placeAgents(int layout, size_t active, size_t total, uint64_t seed, double speedMean = 1.34, double speedSd = 0.26);
+
+// --- spawners (spawn.cpp) ---
+// The reference's seat spawn (seatedSpawnSeats, src/crowd_sim.cpp:652-674): the chosen seat per agent (-1: parked)
+std::vector<int> spawnSeats(int seatCount, int numAgents, double load, uint32_t seed);
+// Gauss-Seidel relaxation of overlapping discs in agent order (relaxSpawn, src/crowd_sim.cpp:544-597, with radii r_i):
+// clampTo(i, x, y) keeps agent i on its band; returns the sweeps run, maxOverlap the last sweep's largest overlap
+int relaxSpawnDiscs(std::vector<float>& x, std::vector<float>& y, const std::vector<float>& r,
+                    const std::function<void(size_t, float&, float&)>& clampTo, float& maxOverlap);
+// An open floor (room, corridor, pillar room): uniform placement with rejection, then the relaxation
+Agents spawnUniform(int layout, size_t active, size_t total, uint32_t seed, double speedMean, double speedSd, int* sweeps,
+                    float* maxOverlap);
 
 } // namespace swarm

@@ -183,6 +183,24 @@ int swarm_place_agents(int layout, size_t active, size_t total, uint64_t seed, d
     });
 }
 
+int swarm_spawn_seats(int seat_count, int n, double load, uint32_t seed, int32_t* chosen) {
+    if (!chosen || seat_count < 0 || n < 0) { g_err = "swarm_spawn_seats: bad argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const std::vector<int> c = swarm::spawnSeats(seat_count, n, load, seed);
+        for (int i = 0; i < n; ++i) chosen[i] = c[size_t(i)];
+    });
+}
+
+int swarm_spawn_uniform(int layout, size_t active, size_t total, uint32_t seed, double speed_mean, double speed_sd, float* x, float* y,
+                        float* r, float* v0, float* w, int* sweeps, float* max_overlap) {
+    if (!x || !y || !r || !v0 || !w) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::Agents a = swarm::spawnUniform(layout, active, total, seed, speed_mean, speed_sd, sweeps, max_overlap);
+        std::memcpy(x, a.x.data(), total * 4); std::memcpy(y, a.y.data(), total * 4); std::memcpy(r, a.r.data(), total * 4);
+        std::memcpy(v0, a.v0.data(), total * 4); std::memcpy(w, a.w.data(), total * 4);
+    });
+}
+
 int swarm_layout_slots(int layout, float* x, float* y, size_t cap, size_t* count) {
     return guarded([&] {
         std::vector<float> sx, sy;

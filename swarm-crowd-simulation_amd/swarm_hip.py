@@ -232,6 +232,29 @@ def place_agents(layout, active, total, seed, speed_mean=1.34, speed_sd=0.26):
     return a
 
 
+def spawn_seats(seat_count, n, load=1.0, seed=1337):
+    """The reference's seat spawn: the seat of each of n agents (-1: parked)."""
+    out = np.zeros(n, np.int32)
+    L = lib()
+    L.swarm_spawn_seats.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint32, C.c_void_p]
+    check(L.swarm_spawn_seats(seat_count, n, load, seed, out.ctypes.data))
+    return out
+
+
+def spawn_uniform(layout, active, total, seed=1337, speed_mean=1.34, speed_sd=0.26):
+    """An open floor: uniform placement with rejection, relaxed (agent dict as place_agents, plus the relaxation's
+    sweeps and last largest overlap)."""
+    a = {k: np.zeros(total, np.float32) for k in ("x", "y", "r", "v0", "w")}
+    sweeps, mo = C.c_int(), C.c_float()
+    L = lib()
+    L.swarm_spawn_uniform.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_uint32, C.c_double, C.c_double] + [C.c_void_p] * 7
+    check(L.swarm_spawn_uniform(layout, active, total, seed, speed_mean, speed_sd, a["x"].ctypes.data, a["y"].ctypes.data,
+                                a["r"].ctypes.data, a["v0"].ctypes.data, a["w"].ctypes.data, C.byref(sweeps), C.byref(mo)))
+    a["vx"] = np.zeros(total, np.float32)
+    a["vy"] = np.zeros(total, np.float32)
+    return a, sweeps.value, mo.value
+
+
 def layout_capacity(layout):
     n = C.c_size_t()
     check(lib().swarm_layout_slots(layout, None, None, 0, C.byref(n)))

@@ -6,6 +6,7 @@ packs its records into compressed .npz files:
                    cell table after every step)
   nav_kat.npz      nav::buildGrid / buildExitField / buildChoiceTable on rectangle worlds
   choice_kat.npz   nav::choiceStep on the two-door room (test_gpu nav.exit_choice's known answers)
+  spawn_kat.npz    seatedSpawnSeats (mt19937's raw output, partial Fisher-Yates) on venues S, M and L
   metrics_kat.npz  metrics::cicDeposit / speedMass known answers, MetricVenue::cellInfo thresholds and six steps of
                    MetricsReference::record on synthetic states (64 x 64 grid of 1 m cells)
 
@@ -67,7 +68,9 @@ def main():
     np.savez_compressed(os.path.join(HERE, "choice_kat.npz"), **choice)
     metrics = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("metrics/")}
     np.savez_compressed(os.path.join(HERE, "metrics_kat.npz"), **metrics)
-    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz"):
+    spawn = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("spawn/")}
+    np.savez_compressed(os.path.join(HERE, "spawn_kat.npz"), **spawn)
+    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz", "spawn_kat.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
 
 
