@@ -126,6 +126,52 @@ int swarm_layout_slots(int layout, float* x, float* y, size_t cap, size_t* count
 /* tau (model time) and exp(-DT / tau) for a relaxation time in seconds, folded from decimals (sim_params.hpp:128-129,510) */
 int swarm_model_params(float tau_s, float* tau, float* relax_step);
 
+/* ---------------- the analytic seated venues S, M and L (include/swarm/venue.hpp, src/venue.cpp) ---------------- */
+
+#define SWARM_VENUE_MAX_GANGWAYS 8
+/* SeatedLayout (venue.hpp:103-145) and the Venue fields around it (venue.hpp:149-166), model units; the exit sizing
+ * rule's figures in metres. Fields up to corner_door_lo are the layout the heightfield queries read. */
+typedef struct {
+    float stand_front, row0, row_depth, clearway;
+    int rows;
+    float conc_front, conc_back, outer_face, landing_end;
+    float avenue_half, lat0, gangway_lo, gangway_hi, exit_lo, exit_hi, corner_inset;
+    float seat_inset, face_margin, wall_above, wall_clear;
+    float first_row, row_rise, seat_top, barrier_top, concourse_z, wall_top, rake;
+    int max_band_steps;
+    float seat_width, domain_half;
+    int gangways;
+    float gw_lo[SWARM_VENUE_MAX_GANGWAYS], gw_hi[SWARM_VENUE_MAX_GANGWAYS];
+    int gw_row[SWARM_VENUE_MAX_GANGWAYS], gw_seated[SWARM_VENUE_MAX_GANGWAYS];
+    int doors;
+    float door_lo[SWARM_VENUE_MAX_GANGWAYS], door_hi[SWARM_VENUE_MAX_GANGWAYS];
+    float corner_door_lo;   /* >= 1e8: no corner door */
+    float domain_min, domain_span, exit_depth, exit_max_z;
+    int grid_res, metric_grid_res, exit_count, gangway_count, seat_total;
+    double door_width_m, corner_door_width_m, total_exit_width_m, required_exit_width_m;
+} swarm_venue_layout;
+
+/* venueByName + buildSeatedLayout (src/venue.cpp:121-327): name "S", "M" or "L" (SWARM_EINVAL otherwise) */
+int swarm_venue_layout_get(const char* name, swarm_venue_layout* out);
+/* seated::seats (src/venue.cpp:1054-1083): every seat in the order quadrant, wing, row, segment, index; x, y, z the spawn
+ * position (clearway centre, rest height), where 5 per seat (quadrant, wing, row, segment, index). Any array may be NULL
+ * (count only); otherwise it holds cap entries and SWARM_ESIZE is returned when the venue has more seats. */
+int swarm_venue_seats(const char* name, float* x, float* y, float* z, int32_t* where, size_t cap, size_t* count);
+/* The heightfield queries of src/venue.cpp:481-935 (the GLSL venue_seated.glsl inside physics.comp), evaluated on the GPU
+ * at n points p (n x 3) with plan directions e (n x 2): sdf_grad n x 4 (seated::sdfGrad: gradient, signed distance),
+ * floor_h n (floorHeight), cell n x 3 (cellAt: kind, row, band), wall n x 9 (seated::wall at reach[0], [1], [2]: unit
+ * direction away from the wall, clearance; 1e9: none within reach), slope n (floorSlope), exit_id n (exitByLat),
+ * gangway n (gangwayAt). Output pointers may be NULL. Synchronous on the current device. */
+int swarm_venue_probe(const char* name, size_t n, const float* p, const float* e, const float* reach3, float* sdf_grad, float* floor_h,
+                      int32_t* cell, float* wall, float* slope, int32_t* exit_id, int32_t* gangway);
+/* The venue's plan as a rectangle world for the step (the 2D cut of the analytic venue): walls the footprints of the seat
+ * blocks, barrier, side and outer walls and the fenced pitch; the domain ends at the outer wall's outer face (the reference's
+ * exit test, max(|x|, |y|) > exitDepth); exit rectangles beyond it, one per avenue and door, the L-shaped corner door as
+ * two (its second halves after the reference's exits); the gangway inclines as stair bands. Built as swarm_scene_build_cached. */
+int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out);
+/* The reference's exit id of each plan exit (ref_exit[cap]; count: the plan's exits) */
+int swarm_venue_exit_map(const char* name, int32_t* ref_exit, int cap, int* count);
+
 /* ---------------- the plain step (CrowdSim) ---------------- */
 
 typedef struct {

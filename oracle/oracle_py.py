@@ -359,3 +359,50 @@ class MetricsOracle:
         _metrics_sigs(lib()).oracle_metrics_record(self.n, f(prev["x"]), f(prev["y"]), f(prev["w"]), f(cur["x"]), f(cur["y"]), f(cur["w"]),
                                                     f(cur["vx"]), f(cur["vy"]), *self.grid, self.thr, self.E, self.rects, self.counters,
                                                     self.evac_step, self.cells.reshape(-1))
+
+
+# --- the analytic seated venue (venue_oracle.cpp) ---
+VENUE_REACHES = (0.008, 0.016, 0.02)   # FEELER_REACH, PLANE_PROBE_REACH (model units) and a wide reach
+
+
+def venue_layout_arrays(kat, name):
+    """The layout of venue name as the oracle's inputs, from the reference's dump in venue_kat.npz."""
+    g = lambda k: np.ascontiguousarray(kat["venue__%s__%s" % (name, k)])
+    return (g("layout_f").astype(np.float32), g("layout_i").astype(np.int32), g("layout_gw").astype(np.float32),
+            g("layout_gw_hi").astype(np.float32), g("layout_gw_row").astype(np.int32), g("layout_gw_seated").astype(np.int32),
+            g("layout_door_lo").astype(np.float32), g("layout_door_hi").astype(np.float32))
+
+
+def _venue_sigs(L):
+    L.oracle_venue_cells.argtypes = [C.c_void_p] * 8 + [C.c_int, C.c_void_p, C.c_void_p]
+    L.oracle_venue_cells.restype = C.c_int
+    L.oracle_venue_eval.argtypes = [C.c_void_p] * 8 + [C.c_int] + [C.c_void_p] * 10
+
+
+def venue_cells(layout):
+    """The oracle's cell table: (band, index, kind, row) int32 and (l0, l1, lo, hi) float32 rows."""
+    L = lib()
+    _venue_sigs(L)
+    ptrs = [a.ctypes.data for a in layout]
+    n = L.oracle_venue_cells(*ptrs, 0, None, None)
+    ci = np.zeros((n, 4), np.int32)
+    cf = np.zeros((n, 4), np.float32)
+    L.oracle_venue_cells(*ptrs, n, ci.ctypes.data, cf.ctypes.data)
+    return ci, cf
+
+
+def venue_eval(layout, p, e, reach3=VENUE_REACHES):
+    """Every heightfield query at points p (n, 3) with directions e (n, 2), as the GPU probe returns them."""
+    L = lib()
+    _venue_sigs(L)
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(e, np.float32).reshape(-1, 2)
+    r = np.ascontiguousarray(reach3, np.float32)
+    n = len(p)
+    o = dict(sdf_grad=np.zeros((n, 4), np.float32), floor=np.zeros(n, np.float32), cell=np.zeros((n, 3), np.int32),
+             wall=np.zeros((n, 3, 3), np.float32), slope=np.zeros(n, np.float32), exit=np.zeros(n, np.int32),
+             gangway=np.zeros(n, np.int32))
+    L.oracle_venue_eval(*[a.ctypes.data for a in layout], n, p.ctypes.data, e.ctypes.data, r.ctypes.data, o["sdf_grad"].ctypes.data,
+                        o["floor"].ctypes.data, o["cell"].ctypes.data, o["wall"].ctypes.data, o["slope"].ctypes.data,
+                        o["exit"].ctypes.data, o["gangway"].ctypes.data)
+    return o

@@ -488,6 +488,153 @@ void spawnCases() {
     }
 }
 
+// The analytic seated venues S, M and L (src/venue.cpp): the layout buildSeatedLayout makes (every scalar and array), the cells
+// of every band, the seats, and the heightfield queries (sdfGrad, floorHeight, cellAt, wall, floorSlope, exitByLat, gangwayAt)
+// at seeded probe points: uniform over the plan at the step's wall plane, above and below the surface, exactly on it, on cell
+// and band boundaries (unfolded into a random octant), on the diagonals and on the axes
+void venueCases() {
+    const char* names[3] = { "S", "M", "L" };
+    for (int v = 0; v < 3; ++v) {
+        const Venue& venue = venueByName(names[v]);
+        const SeatedLayout& L = venue.layout;
+        const std::string p = std::string("venue/") + names[v] + "/";
+        put(p + "layout_f", 'f', std::vector<float>{ L.standFront, L.row0, L.rowDepth, L.clearway, L.concFront, L.concBack, L.outerFace,
+                                                     L.landingEnd, L.avenueHalf, L.lat0, L.gangwayLo, L.gangwayHi, L.exitLo, L.exitHi,
+                                                     L.cornerInset, L.seatInset, L.faceMargin, L.wallAbove, L.wallClear, L.firstRow,
+                                                     L.rowRise, L.seatTop, L.barrierTop, L.concourseZ, L.wallTop, L.rake, L.seatWidth,
+                                                     L.domainHalf, L.cornerDoorLo });
+        put(p + "layout_i", 'i', std::vector<int32_t>{ L.rows, L.maxBandSteps, L.gangways, L.doors, L.seatTotal });
+        put(p + "layout_gw", 'f', std::vector<float>(L.gwLo.begin(), L.gwLo.end()));
+        put(p + "layout_gw_hi", 'f', std::vector<float>(L.gwHi.begin(), L.gwHi.end()));
+        put(p + "layout_gw_row", 'i', std::vector<int32_t>(L.gwRow.begin(), L.gwRow.end()));
+        put(p + "layout_gw_seated", 'i', std::vector<int32_t>(L.gwSeated.begin(), L.gwSeated.end()));
+        put(p + "layout_door_lo", 'f', std::vector<float>(L.doorLo.begin(), L.doorLo.end()));
+        put(p + "layout_door_hi", 'f', std::vector<float>(L.doorHi.begin(), L.doorHi.end()));
+        put(p + "layout_d", 'd', std::vector<double>{ L.doorWidthM, L.cornerDoorWidthM, L.totalExitWidthM, L.requiredExitWidthM });
+        put(p + "venue_f", 'f', std::vector<float>{ venue.domainMin, venue.domainSpan, venue.exitDepth, venue.exitMaxZ });
+        put(p + "venue_i", 'i', std::vector<int32_t>{ venue.gridRes, venue.metricGridRes, seated::exitCount(L), seated::gangwayCount(L) });
+        {
+            std::vector<double> w;
+            for (int e = 0; e < seated::exitCount(L); ++e) w.push_back(seated::exitWidthM(L, e));
+            put(p + "exit_width_m", 'd', w);
+        }
+        // cells: (band, c, kind, row) and (l0, l1, lo, hi)
+        std::vector<int32_t> ci;
+        std::vector<float> cf;
+        const int nb = seated::bandCount(L);
+        for (int j = 0; j < nb; ++j) {
+            float lo, hi, a, b;
+            int k, r;
+            seated::band(L, j, lo, hi);
+            for (int c = 0; seated::cell(L, j, c, a, b, k, r); ++c) {
+                ci.insert(ci.end(), { j, c, k, r });
+                cf.insert(cf.end(), { a, b, lo, hi });
+            }
+        }
+        put(p + "cells_i", 'i', ci, { uint64_t(ci.size() / 4), 4 });
+        put(p + "cells_f", 'f', cf, { uint64_t(cf.size() / 4), 4 });
+        // seats (venue L: every 17th)
+        const std::vector<seated::Seat> seats = seated::seats(L);
+        const size_t stride = v == 2 ? 17 : 1;
+        std::vector<float> sp;
+        std::vector<int32_t> si;
+        for (size_t i = 0; i < seats.size(); i += stride) {
+            const seated::Seat& s = seats[i];
+            sp.insert(sp.end(), { s.pos.x, s.pos.y, s.pos.z, s.lat });
+            si.insert(si.end(), { int32_t(i), s.quadrant, s.wing, s.row, s.segment, s.index });
+        }
+        put(p + "seat_count", 'i', std::vector<int32_t>{ int32_t(seats.size()) });
+        put(p + "seats_f", 'f', sp, { uint64_t(sp.size() / 4), 4 });
+        put(p + "seats_i", 'i', si, { uint64_t(si.size() / 6), 6 });
+
+        // probe points
+        Rng rng{ 0x5EA7ED00ull + uint64_t(v) };
+        const float H = L.landingEnd + 0.02f;
+        auto U = [&](float a, float b) { return a + (b - a) * rng.uniform(); };
+        auto octant = [&](float lat, float dep) {   // unfold (lat, dep) into a random octant
+            const uint64_t o = rng.next();
+            float x = (o & 1) ? dep : lat, y = (o & 1) ? lat : dep;
+            if (o & 2) x = -x;
+            if (o & 4) y = -y;
+            return glm::vec2(x, y);
+        };
+        const int per = v == 0 ? 1024 : 512;
+        std::vector<float> pts, es;
+        std::vector<int32_t> kinds;
+        for (int kind = 0; kind < 6; ++kind)
+            for (int i = 0; i < per; ++i) {
+                glm::vec2 xy;
+                if (kind <= 2) xy = glm::vec2(U(-H, H), U(-H, H));
+                else if (kind == 3) {   // a cell boundary: lat at l0 / l1 / inside, dep at lo / hi / just inside
+                    int j = int(rng.next() % uint64_t(nb));
+                    float lo, hi;
+                    seated::band(L, j, lo, hi);
+                    if (hi > H) hi = H;
+                    int cn = 0;
+                    float a, b;
+                    int k, r;
+                    while (seated::cell(L, j, cn, a, b, k, r)) ++cn;
+                    seated::cell(L, j, int(rng.next() % uint64_t(cn)), a, b, k, r);
+                    const uint64_t m = rng.next();
+                    const float dep = (m % 3) == 0 ? lo : (m % 3) == 1 ? hi : U(lo, lo + std::min(0.05f, hi - lo));
+                    const float bb = std::min(b, dep);
+                    const float lat = ((m / 3) % 3) == 0 ? std::min(a, dep) : ((m / 3) % 3) == 1 ? bb : U(std::min(a, dep), bb);
+                    xy = octant(lat, dep);
+                } else if (kind == 4) {   // the diagonals
+                    const float t = U(0.0f, H);
+                    const uint64_t o = rng.next();
+                    xy = glm::vec2((o & 1) ? -t : t, (o & 2) ? -t : t);
+                } else {   // the axes
+                    const float t = U(-H, H);
+                    xy = (rng.next() & 1) ? glm::vec2(t, 0.0f) : glm::vec2(0.0f, t);
+                }
+                const float f = seated::floorHeight(L, xy);
+                float z;
+                const uint64_t zc = rng.next() % 4;
+                if (kind == 0) z = f + BODY_RADIUS + L.wallAbove;   // the step's wall plane
+                else if (kind == 1) z = f + U(-0.03f, 0.06f);
+                else if (kind == 2) z = f;
+                else z = zc == 0 ? f : zc == 1 ? f + BODY_RADIUS : zc == 2 ? f + BODY_RADIUS + L.wallAbove : f + U(-0.02f, 0.04f);
+                const float ang = U(0.0f, 6.2831853f);
+                pts.insert(pts.end(), { xy.x, xy.y, z });
+                es.insert(es.end(), { std::cos(ang), std::sin(ang) });
+                kinds.push_back(kind);
+            }
+        const size_t n = kinds.size();
+        std::vector<float> grad, floorH, wallF, wallP, wallW, slope;
+        std::vector<int32_t> cellK, exitL, gangway;
+        for (size_t i = 0; i < n; ++i) {
+            const glm::vec3 q(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+            const glm::vec2 xy(q);
+            const glm::vec4 g = seated::sdfGrad(L, q);
+            grad.insert(grad.end(), { g.x, g.y, g.z, g.w });
+            floorH.push_back(seated::floorHeight(L, xy));
+            int row = 0, bandIndex = 0;
+            const int kind = seated::cellAt(L, xy, &row, &bandIndex);
+            cellK.insert(cellK.end(), { kind, row, bandIndex });
+            const glm::vec3 wf = seated::wall(L, q, FEELER_REACH), wp = seated::wall(L, q, PLANE_PROBE_REACH), ww = seated::wall(L, q, 0.02f);
+            wallF.insert(wallF.end(), { wf.x, wf.y, wf.z });
+            wallP.insert(wallP.end(), { wp.x, wp.y, wp.z });
+            wallW.insert(wallW.end(), { ww.x, ww.y, ww.z });
+            slope.push_back(seated::floorSlope(L, xy, glm::vec2(es[2 * i], es[2 * i + 1])));
+            exitL.push_back(seated::exitByLat(L, xy));
+            gangway.push_back(seated::gangwayAt(L, xy));
+        }
+        put(p + "probe/kind", 'i', kinds);
+        put(p + "probe/p", 'f', pts, { uint64_t(n), 3 });
+        put(p + "probe/e", 'f', es, { uint64_t(n), 2 });
+        put(p + "probe/sdf_grad", 'f', grad, { uint64_t(n), 4 });
+        put(p + "probe/floor", 'f', floorH);
+        put(p + "probe/cell", 'i', cellK, { uint64_t(n), 3 });
+        put(p + "probe/wall_feeler", 'f', wallF, { uint64_t(n), 3 });
+        put(p + "probe/wall_plane", 'f', wallP, { uint64_t(n), 3 });
+        put(p + "probe/wall_wide", 'f', wallW, { uint64_t(n), 3 });
+        put(p + "probe/slope", 'f', slope);
+        put(p + "probe/exit", 'i', exitL);
+        put(p + "probe/gangway", 'i', gangway);
+    }
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
@@ -505,6 +652,7 @@ int main(int argc, char** argv) {
     choiceCases();
     metricsCases();
     spawnCases();
+    venueCases();
     {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents
         const std::vector<Rect> walls = { { { -0.3f, -0.3f }, { 20.3f, 0.0f } }, { { -0.3f, 0.0f }, { 0.0f, 2.3f } }, { { 0.0f, 2.0f }, { 18.0f, 2.3f } },
                                           { { 17.7f, 2.3f }, { 18.0f, 15.0f } }, { { 20.0f, 0.0f }, { 20.3f, 15.0f } } };

@@ -3,12 +3,16 @@
 #include "../../include/swarm_adjoint.h"
 #include "../../include/swarm_hip.h"
 
+#include <cstddef>
 #include <cstring>
 #include <new>
+#include <stdexcept>
 #include <string>
 
 #include "hip_crowd_sim.hpp"
 #include "scene.hpp"
+#include "venue.hpp"
+#include "venue_probe.h"
 
 struct swarm_scene { swarm::Scene s; };
 struct swarm_ctx { swarm::HipCrowdSim* sim; };
@@ -16,10 +20,17 @@ struct swarm_ctx { swarm::HipCrowdSim* sim; };
 namespace {
 thread_local std::string g_err;
 
+struct SizeError : std::runtime_error {   // a caller's array is too small: SWARM_ESIZE
+    using std::runtime_error::runtime_error;
+};
+
 template <class F> int guarded(F&& f) {
     try {
         f();
         return SWARM_OK;
+    } catch (const SizeError& e) {
+        g_err = e.what();
+        return SWARM_ESIZE;
     } catch (const std::invalid_argument& e) {
         g_err = e.what();
         return SWARM_EINVAL;
@@ -217,6 +228,82 @@ int swarm_model_params(float tau_s, float* tau, float* relax) {
     if (!tau || !relax || !(tau_s > 0.0f)) { g_err = "invalid relaxation time"; return SWARM_EINVAL; }
     swarm::modelParamsFor(tau_s, *tau, *relax);
     return SWARM_OK;
+}
+
+int swarm_venue_layout_get(const char* name, swarm_venue_layout* out) {
+    if (!name || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::Venue& v = swarm::venueByName(name);
+        static_assert(offsetof(swarm_venue_layout, corner_door_lo) == offsetof(swarm::seated::Layout, cornerDoorLo) &&
+                          sizeof(swarm::seated::Layout) == offsetof(swarm_venue_layout, corner_door_lo) + sizeof(float),
+                      "swarm_venue_layout begins with the layout");
+        std::memset(out, 0, sizeof *out);
+        std::memcpy(out, &v.layout, sizeof v.layout);
+        out->domain_min = v.domainMin; out->domain_span = v.domainSpan; out->exit_depth = v.exitDepth; out->exit_max_z = v.exitMaxZ;
+        out->grid_res = v.gridRes; out->metric_grid_res = v.metricGridRes;
+        out->exit_count = swarm::seated::exitCount(v.layout);
+        out->gangway_count = swarm::seated::gangwayCount(v.layout);
+        out->seat_total = v.seatTotal;
+        out->door_width_m = v.doorWidthM; out->corner_door_width_m = v.cornerDoorWidthM;
+        out->total_exit_width_m = v.totalExitWidthM; out->required_exit_width_m = v.requiredExitWidthM;
+    });
+}
+
+int swarm_venue_seats(const char* name, float* x, float* y, float* z, int32_t* where, size_t cap, size_t* count) {
+    if (!name) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const std::vector<swarm::VenueSeat> s = swarm::venueSeats(swarm::venueByName(name).layout);
+        if (count) *count = s.size();
+        if (!x && !y && !z && !where) return;
+        if (s.size() > cap) throw SizeError("swarm_venue_seats: the venue has " + std::to_string(s.size()) + " seats");
+        for (size_t i = 0; i < s.size(); ++i) {
+            if (x) x[i] = s[i].x;
+            if (y) y[i] = s[i].y;
+            if (z) z[i] = s[i].z;
+            if (where) {
+                int32_t* w = where + 5 * i;
+                w[0] = s[i].quadrant; w[1] = s[i].wing; w[2] = s[i].row; w[3] = s[i].segment; w[4] = s[i].index;
+            }
+        }
+    });
+}
+
+int swarm_venue_probe(const char* name, size_t n, const float* p, const float* e, const float* reach3, float* sdf_grad, float* floor_h,
+                      int32_t* cell, float* wall, float* slope, int32_t* exit_id, int32_t* gangway) {
+    if (!name || (n > 0 && (!p || !e || !reach3))) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::Venue& v = swarm::venueByName(name);
+        if (n == 0) return;
+        std::vector<float> out(n * swarm::VENUE_PROBE_WORDS);
+        swarm::venueProbe(v.layout, n, p, e, reach3, out.data());
+        auto copy = [&](void* dst, size_t word0, size_t words) { if (dst) std::memcpy(dst, out.data() + word0 * n, words * n * 4); };
+        copy(sdf_grad, 0, 4);
+        copy(floor_h, 4, 1);
+        copy(cell, 5, 3);
+        copy(wall, 8, 9);
+        copy(slope, 17, 1);
+        copy(exit_id, 18, 1);
+        copy(gangway, 19, 1);
+    });
+}
+
+int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out) {
+    if (!name || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::VenuePlan plan = swarm::venuePlan(swarm::venueByName(name));
+        *out = new swarm_scene{ swarm::buildSceneCached(plan.desc, max_radius, max_v0, threads, cache_dir ? cache_dir : "", nullptr) };
+    });
+}
+
+int swarm_venue_exit_map(const char* name, int32_t* ref_exit, int cap, int* count) {
+    if (!name) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        const swarm::VenuePlan plan = swarm::venuePlan(swarm::venueByName(name));
+        if (count) *count = int(plan.refExit.size());
+        if (!ref_exit) return;
+        if (plan.refExit.size() > size_t(cap)) throw SizeError("swarm_venue_exit_map: the plan has " + std::to_string(plan.refExit.size()) + " exits");
+        for (size_t i = 0; i < plan.refExit.size(); ++i) ref_exit[i] = plan.refExit[i];
+    });
 }
 
 int swarm_create(const swarm_scene* scene, int n, const swarm_config* cfg, swarm_ctx** out) {

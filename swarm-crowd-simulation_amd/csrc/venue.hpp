@@ -1,0 +1,68 @@
+// The reference's analytic seated venues S, M and L on the host: the layout generator, the seats, and the venue's plan
+// as a rectangle world for the per-step pipeline.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "scene.hpp"
+#include "seated_venue.h"
+
+namespace swarm {
+
+// SeatedVenueParams (include/swarm/venue.hpp:60-99 of the reference): metres unless stated
+struct VenueParams {
+    std::string name;
+    double walkwayWidthM, barrierThicknessM, avenueWidthM, sideWallThicknessM;
+    int rows;
+    double rowDepthM, clearwayM, seatWidthM, gangwayWidthM, seatBlockInsetM;
+    int maxSeatsBetweenGangways, maxSeatsDeadEnd;
+    double concourseWidthM, outerWallThicknessM, landingDepthM, rowRiseM, seatBlockTopM, barrierTopM, wallTopAboveConcourseM;
+    int exitsPerQuadrant;
+    double exitRatePerMMin, egressTimeMin, minExitWidthM;
+    bool doorPerGangwayHead;   // false: one door per wing (venue S)
+    int gridRes;               // neighbour grid cells per axis (0: the terrace's 40)
+    int metricGridRes;         // 1 m metric cells per axis (0: 64)
+};
+
+// A venue: its parameters, the neighbour-grid domain and its layout, with the exit sizing rule's figures (metres)
+struct Venue {
+    VenueParams params;
+    float domainMin, domainSpan;
+    int gridRes, metricGridRes;
+    float exitDepth, exitMaxZ;
+    seated::Layout layout;
+    double doorWidthM, cornerDoorWidthM, totalExitWidthM, requiredExitWidthM;
+    int seatTotal;
+};
+
+// "S", "M" or "L" (either case); std::invalid_argument otherwise
+const Venue& venueByName(const std::string& name);
+// The layout of a parameter set in a domain of half-width domainHalf, with the generator's design rules checked
+// (std::runtime_error naming the rule on a violation)
+Venue makeVenue(const VenueParams& p);
+
+// A seat: the spawn position (clearway centre, the tread's rest height) and where it is
+struct VenueSeat {
+    float x, y, z, lat;
+    int quadrant, wing, row, segment, index;
+};
+// Every seat, in the order quadrant, wing, row, segment, index (seated::seats, src/venue.cpp:1054-1083)
+std::vector<VenueSeat> venueSeats(const seated::Layout& L);
+int seatCountIn(float a, float b, float seatWidth);
+
+// The exit clear width (metres) of reference exit e (seated::exitWidthM, src/venue.cpp:867-874)
+double venueExitWidthM(const Venue& v, int e);
+
+// The venue's plan for the per-step pipeline: walls are the footprints of the seat blocks, the barrier, the side and
+// outer walls (each stands higher than a body above every floor beside it) and the fenced pitch; agents leave when
+// max(|x|, |y|) passes the outer wall's outer face (the domain's edge). Exit rectangles lie beyond that face, one per
+// avenue and door; the corner door, L-shaped there, is two rectangles, the second appended after the reference's exits.
+// refExit[e] is the reference's exit id of plan exit e. The gangway inclines are stair bands.
+struct VenuePlan {
+    SceneDesc desc;
+    std::vector<int> refExit;
+};
+VenuePlan venuePlan(const Venue& v);
+
+}  // namespace swarm

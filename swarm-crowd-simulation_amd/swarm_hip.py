@@ -164,6 +164,16 @@ class Scene:
         return cls(h)
 
     @classmethod
+    def venue(cls, name, max_radius=0.0064, max_v0=0.75, threads=0, cache_dir=None):
+        """The plan of the reference's seated venue S, M or L (walls, exits past the outer face, gangway stair bands)."""
+        h = C.c_void_p()
+        L = lib()
+        L.swarm_scene_build_venue.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_int, C.c_char_p, C.c_void_p]
+        check(L.swarm_scene_build_venue(name.encode(), max_radius, max_v0, threads,
+                                        str(cache_dir).encode() if cache_dir else None, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def rects(cls, domain, hash_cell, sdf_spacing, sdf_clamp, nav, nav_body, walls, exits, exit_width_m=None,
               stairs=None, stair_factor=None, metres_per_unit=M_PER_UNIT, max_radius=0.0064, max_v0=0.75, threads=0):
         """A rectangle world. nav = (res_x, res_y, origin_x, origin_y, cell, inv_cell)."""
@@ -238,6 +248,96 @@ def spawn_seats(seat_count, n, load=1.0, seed=1337):
     L = lib()
     L.swarm_spawn_seats.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint32, C.c_void_p]
     check(L.swarm_spawn_seats(seat_count, n, load, seed, out.ctypes.data))
+    return out
+
+
+VENUE_MAX_GANGWAYS = 8
+
+
+class VenueLayout(C.Structure):
+    """swarm_venue_layout: SeatedLayout and the venue around it."""
+    G = VENUE_MAX_GANGWAYS
+    _fields_ = [(n, C.c_float) for n in ("stand_front", "row0", "row_depth", "clearway")] + [("rows", C.c_int)] + \
+        [(n, C.c_float) for n in ("conc_front", "conc_back", "outer_face", "landing_end", "avenue_half", "lat0", "gangway_lo",
+                                  "gangway_hi", "exit_lo", "exit_hi", "corner_inset", "seat_inset", "face_margin", "wall_above",
+                                  "wall_clear", "first_row", "row_rise", "seat_top", "barrier_top", "concourse_z", "wall_top",
+                                  "rake")] + \
+        [("max_band_steps", C.c_int), ("seat_width", C.c_float), ("domain_half", C.c_float), ("gangways", C.c_int),
+         ("gw_lo", C.c_float * G), ("gw_hi", C.c_float * G), ("gw_row", C.c_int * G), ("gw_seated", C.c_int * G),
+         ("doors", C.c_int), ("door_lo", C.c_float * G), ("door_hi", C.c_float * G), ("corner_door_lo", C.c_float)] + \
+        [(n, C.c_float) for n in ("domain_min", "domain_span", "exit_depth", "exit_max_z")] + \
+        [(n, C.c_int) for n in ("grid_res", "metric_grid_res", "exit_count", "gangway_count", "seat_total")] + \
+        [(n, C.c_double) for n in ("door_width_m", "corner_door_width_m", "total_exit_width_m", "required_exit_width_m")]
+
+
+def venue_layout(name):
+    """The layout of the reference's seated venue S, M or L."""
+    out = VenueLayout()
+    L = lib()
+    L.swarm_venue_layout_get.argtypes = [C.c_char_p, C.c_void_p]
+    check(L.swarm_venue_layout_get(name.encode(), C.byref(out)))
+    return out
+
+
+def venue_seats(name):
+    """Every seat: x, y, z (n,) float32 and where (n, 5) int32 = quadrant, wing, row, segment, index."""
+    L = lib()
+    L.swarm_venue_seats.argtypes = [C.c_char_p] + [C.c_void_p] * 4 + [C.c_size_t, C.c_void_p]
+    n = C.c_size_t()
+    check(L.swarm_venue_seats(name.encode(), None, None, None, None, 0, C.byref(n)))
+    x, y, z = (np.zeros(n.value, np.float32) for _ in range(3))
+    where = np.zeros((n.value, 5), np.int32)
+    check(L.swarm_venue_seats(name.encode(), x.ctypes.data, y.ctypes.data, z.ctypes.data, where.ctypes.data, n.value, C.byref(n)))
+    return x, y, z, where
+
+
+PARK_COORD = 1000.0
+
+
+def venue_agents(name, n, load=1.0, seed=1337):
+    """The reference's seated crowd (spawnSeatedCrowd, src/crowd_sim.cpp:677-697) on venue name: n agents, round(load x
+    seats) of them on seats chosen by the seat spawn, at rest, the rest parked; the reference's global radius and free
+    speed (BODY_RADIUS 0.2 m, V_MAX 1.34 m/s)."""
+    x, y, _, _ = venue_seats(name)
+    chosen = spawn_seats(len(x), n, load, seed)
+    on = chosen >= 0
+    a = {k: np.zeros(n, np.float32) for k in ("x", "y", "vx", "vy", "r", "v0", "w")}
+    a["x"][:] = PARK_COORD
+    a["y"][:] = PARK_COORD
+    a["x"][on] = x[chosen[on]]
+    a["y"][on] = y[chosen[on]]
+    a["w"][on] = 1.0
+    a["r"][:] = np.float32(0.0064)
+    a["v0"][:] = np.float32(0.75)
+    return a
+
+
+def venue_probe(name, p, e, reach3):
+    """The heightfield queries on the GPU at points p (n, 3) with directions e (n, 2): a dict of sdf_grad (n, 4),
+    floor (n,), cell (n, 3), wall (n, 3, 3) (one row per reach), slope, exit, gangway (n,)."""
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(e, np.float32).reshape(-1, 2)
+    r = np.ascontiguousarray(reach3, np.float32).reshape(3)
+    n = len(p)
+    o = dict(sdf_grad=np.zeros((n, 4), np.float32), floor=np.zeros(n, np.float32), cell=np.zeros((n, 3), np.int32),
+             wall=np.zeros((n, 3, 3), np.float32), slope=np.zeros(n, np.float32), exit=np.zeros(n, np.int32),
+             gangway=np.zeros(n, np.int32))
+    L = lib()
+    L.swarm_venue_probe.argtypes = [C.c_char_p, C.c_size_t] + [C.c_void_p] * 10
+    check(L.swarm_venue_probe(name.encode(), n, p.ctypes.data, e.ctypes.data, r.ctypes.data, o["sdf_grad"].ctypes.data,
+                              o["floor"].ctypes.data, o["cell"].ctypes.data, o["wall"].ctypes.data, o["slope"].ctypes.data,
+                              o["exit"].ctypes.data, o["gangway"].ctypes.data))
+    return o
+
+
+def venue_exit_map(name):
+    """The reference's exit id of each exit of the venue's plan."""
+    L = lib()
+    L.swarm_venue_exit_map.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_void_p]
+    n = C.c_int()
+    check(L.swarm_venue_exit_map(name.encode(), None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.int32)
+    check(L.swarm_venue_exit_map(name.encode(), out.ctypes.data, n.value, C.byref(n)))
     return out
 
 

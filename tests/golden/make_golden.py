@@ -7,6 +7,9 @@ packs its records into compressed .npz files:
   nav_kat.npz      nav::buildGrid / buildExitField / buildChoiceTable on rectangle worlds
   choice_kat.npz   nav::choiceStep on the two-door room (test_gpu nav.exit_choice's known answers)
   spawn_kat.npz    seatedSpawnSeats (mt19937's raw output, partial Fisher-Yates) on venues S, M and L
+  venue_kat.npz    the analytic seated venues S, M and L: buildSeatedLayout's layout, the cells of every band, the seats
+                   (venue L: every 17th) and seated::sdfGrad / floorHeight / cellAt / wall / floorSlope / exitByLat /
+                   gangwayAt at seeded probe points
   metrics_kat.npz  metrics::cicDeposit / speedMass known answers, MetricVenue::cellInfo thresholds and six steps of
                    MetricsReference::record on synthetic states (64 x 64 grid of 1 m cells)
 
@@ -70,7 +73,9 @@ def main():
     np.savez_compressed(os.path.join(HERE, "metrics_kat.npz"), **metrics)
     spawn = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("spawn/")}
     np.savez_compressed(os.path.join(HERE, "spawn_kat.npz"), **spawn)
-    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz", "spawn_kat.npz"):
+    venue = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("venue/")}
+    np.savez_compressed(os.path.join(HERE, "venue_kat.npz"), **venue)
+    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz", "spawn_kat.npz", "venue_kat.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
 
 
