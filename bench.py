@@ -145,6 +145,68 @@ def cpu_baseline(bowl_scene, bowl_agents, full=True):
     return out
 
 
+# The reference's published GPU step on its venue L (AMD Radeon RX 7800 XT, OpenGL compute; its 3D model): ms per step and
+# per pass, at spawn and congested (the state at t50 of its full-load run) (results/benchmarks/thesis/perf_tables.md:17-18,45-46)
+REFERENCE_VENUE_L = {
+    "hardware": "AMD Radeon RX 7800 XT (OpenGL 4.3 compute), the reference's 3D seated model",
+    "source": "results/benchmarks/thesis/perf_tables.md:17-18,45-46",
+    "spawn": {"active": 63424, "ms_per_step": 1.861, "passes_ms": {"hash": 0.0036, "sort": 0.1064, "cells_gather": 0.0087,
+                                                                    "exit_choice": 0.2039, "predict": 0.7722, "projection": 0.6953,
+                                                                    "final": 0.0663}},
+    "congested": {"active": 31703, "ms_per_step": 3.928, "passes_ms": {"hash": 0.0050, "sort": 0.0919, "cells_gather": 0.0088,
+                                                                        "exit_choice": 0.4564, "predict": 2.289, "projection": 0.6879,
+                                                                        "final": 0.3720}},
+}
+
+
+def venue_bench(name="L", n=65536, steps=100, warmup=5, use_graph=True, cache_dir=None):
+    """The plan of the reference's venue (its own seat spawn at full load, its global radius and free speed) at spawn and
+    congested: the state when half of the seated crowd has left (t50), reached by stepping from spawn. Each state: warm-up,
+    then 'steps' timed steps (graph replays, HIP events), and the per-pass split of an eager run."""
+    t0 = time.perf_counter()
+    scene = sh.Scene.venue(name, cache_dir=cache_dir)
+    build_s = time.perf_counter() - t0
+    a = sh.venue_agents(name, n, 1.0, SEED)
+    seated = int(a["w"].sum())
+    sim = sh.Sim(scene, n, use_graph=use_graph)
+    sim.upload(a)
+    sim.autotune(5)
+
+    def timed(state):
+        sim.step(warmup)
+        sim.sync()
+        _, act0, s0 = sim.control()
+        ms = sim.time_steps(steps)
+        _, _, s1 = sim.control()
+        sim.set_timing(True)
+        sim.step(5)
+        t = sim.timings()
+        sim.set_timing(False)
+        out = {"active_at_start": int(act0), "ms_per_step": ms / steps, "agent_steps_per_s": (s1 - s0) / (ms * 1e-3),
+               "passes_ms": dict(grid=t[0], choice=t[1], predict=t[2], projection=t[3], final=t[4], total_eager=t[5])}
+        ref = REFERENCE_VENUE_L.get(state) if name == "L" else None
+        if ref:
+            out["speedup_vs_reference_gpu"] = ref["ms_per_step"] / out["ms_per_step"]
+        return out
+
+    res = {"venue": name, "agents_total": n, "seated": seated, "scene_build_s": build_s, "exits": scene.info.E,
+           "nav_cells": scene.info.nav_res_x * scene.info.nav_res_y, "hash_cells": scene.info.hash_nx * scene.info.hash_ny}
+    res["spawn"] = timed("spawn")
+    sim.upload(a)
+    chunk, stepped = 50, 0
+    while stepped < 20000:   # to t50: half of the seated crowd gone
+        sim.step(chunk)
+        stepped += chunk
+        if sim.control()[1] <= seated // 2:
+            break
+    res["congested_after_steps"] = stepped
+    res["congested"] = timed("congested")
+    if name == "L":
+        res["reference"] = REFERENCE_VENUE_L
+    sim.close()
+    return res
+
+
 def batch_bench(steps, warmup, loop_steps, use_graph=True):
     """The ensemble (256 scenarios of the room, corridor, bowl-eighth and pillar floors, about 1.4 M agents) as one
     batched step, against the same scenarios stepped by a loop over plain contexts (each scenario's steps, then a
@@ -275,6 +337,7 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-venue", action="store_true", help="skip the reference's venue L leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches in every leg (for profilers that do not follow graph launches)")
     args = ap.parse_args()
@@ -363,6 +426,16 @@ def main():
               f"{adjoint['adjoint_backward_ms']:.1f} ms ({adjoint['cost_vs_forward']:.2f}x, {adjoint['segments']} segment(s)); "
               f"with recompute {adjoint['recompute']['cost_vs_forward']:.2f}x", file=sys.stderr, flush=True)
 
+    venue = None
+    if not args.no_venue:
+        import tempfile
+        venue = venue_bench("L", use_graph=not args.no_graph, cache_dir=os.path.join(tempfile.gettempdir(), "swarm_scene_cache"))
+        for st in ("spawn", "congested"):
+            v = venue[st]
+            print(f"# venue L plan, {st}: {v['active_at_start']} active, {v['ms_per_step']:.4f} ms/step, "
+                  f"{v['agent_steps_per_s'] / 1e6:.1f} M agent-steps/s; the reference's GPU {REFERENCE_VENUE_L[st]['ms_per_step']} ms "
+                  f"(x{v['speedup_vs_reference_gpu']:.2f})", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, a, full=not args.cpu_quick)
@@ -401,6 +474,7 @@ def main():
         "metrics": metrics,
         "batched": batch,
         "adjoint": adjoint,
+        "venue_l": venue,
         "cpu_baseline": None if cpu is None else dict(
             value=cpu["configs"]["bowl_63424"]["all_cores"]["agent_steps_per_s"], unit="agent-steps/s", cores=cpu["logical_cores"],
             kind="port", sample="the 63,424-agent bowl: 3 rounds of 2 warm-up + 5 timed steps, each from the same restored state "

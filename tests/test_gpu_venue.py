@@ -6,6 +6,9 @@ own src/venue.cpp) and the oracle's at fresh random points. The step on the plan
 reference's seated crowd (its seat spawn, global radius and free speed) and is bit for bit the oracle's: state, sorted
 grid, cell table and exit choices every step, then the evacuation curve and every agent's (step, exit) to the end.
 """
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -97,3 +100,20 @@ def test_venue_m_spawn_steps_bit_exact():
         o.step(1)
         compare_step(sim, o, f"venue M step {k}")
         assert np.array_equal(sim.assign(), o.assign()), f"venue M step {k} exit choices"
+
+
+def test_venue_l_spawn_steps_bit_exact():
+    """Venue L's plan at spawn, 63,424 of 65,536 agents seated (the workload of the reference's venue L timings): 3 steps,
+    every quantity every step."""
+    scene = sh.Scene.venue("L", cache_dir=os.path.join(tempfile.gettempdir(), "swarm_scene_cache"))
+    a = sh.venue_agents("L", 65536, 1.0, 1337)
+    assert int(a["w"].sum()) == 63424
+    sim = sh.Sim(scene, 65536)
+    sim.upload(a)
+    o = oracle_for(scene, 65536)
+    o.set_state(a)
+    for k in range(1, 4):
+        sim.step(1)
+        o.step(1)
+        compare_step(sim, o, f"venue L step {k}")
+        assert np.array_equal(sim.assign(), o.assign()), f"venue L step {k} exit choices"
