@@ -151,3 +151,41 @@ def test_ensemble_workload_sampled_scenarios():
         ps = p.state()
         for f in FIELDS:
             assert_bits(st[f][lo:lo + n], ps[f], f"ensemble scenario {s} {f}")
+
+
+def test_batch_scenarios_match_the_oracle_directly():
+    """Each scenario of a batch (room, corridor, bowl eighth, pillar room; uneven counts, parked agents) against the
+    oracle run on that scenario's own inputs: state every step for 20 steps, then exit choices and evacuation events,
+    bit for bit. This pins the batched path to the CPU restatement without going through a plain GPU run."""
+    from common import oracle_for
+
+    layouts = [sh.LAYOUT_ROOM, sh.LAYOUT_CORRIDOR, sh.LAYOUT_BOWL_EIGHTH, sh.LAYOUT_PILLAR]
+    scen_layout = np.array([0, 1, 2, 3, 2], np.int32)
+    totals = np.array([1500, 1024, 2300, 1800, 1100], np.int32)
+    actives = np.array([1400, 1024, 2000, 1700, 900], np.int32)
+    scenes = [scene(l) for l in layouts]
+    parts = [sh.place_agents(layouts[scen_layout[s]], int(actives[s]), int(totals[s]), 4242 + s) for s in range(len(totals))]
+    agents = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    b = sh.BatchSim(scenes, scen_layout, totals)
+    b.upload(agents)
+    oracles = []
+    for s, p in enumerate(parts):
+        o = oracle_for(scenes[scen_layout[s]], int(totals[s]))
+        o.set_state(p)
+        oracles.append(o)
+    for k in range(1, 21):
+        b.step(1)
+        st = b.state()
+        assign = b.assign()
+        for s, o in enumerate(oracles):
+            o.step(1)
+            lo, hi = int(b.agent_base[s]), int(b.agent_base[s]) + int(totals[s])
+            os_ = o.state()
+            for f in FIELDS:
+                assert_bits(st[f][lo:hi], os_[f], f"step {k} scenario {s} {f}")
+            assert np.array_equal(assign[lo:hi], o.assign()), f"step {k} scenario {s} exit choices"
+    ev_step, ev_exit, _ = b.evac()
+    for s, o in enumerate(oracles):
+        lo, hi = int(b.agent_base[s]), int(b.agent_base[s]) + int(totals[s])
+        cs, ce = o.evac()
+        assert np.array_equal(ev_step[lo:hi], cs) and np.array_equal(ev_exit[lo:hi], ce), f"scenario {s} evacuation"
