@@ -57,13 +57,20 @@ def measured_traffic():
 
 
 def cpu_model():
+    """The host CPU as lscpu names it: model name, family and model (a guest may report only the vendor's line)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model") and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info:
+        name += f" (family {info['cpu family']}, model {info.get('model', '?')})"
+    return name
 
 
 CPU_STAGES = ("hash", "sort", "cells", "choice", "predict", "project", "final", "total")
