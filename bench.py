@@ -376,6 +376,18 @@ def main():
     ms_per_step = ms / args.steps
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e6:.1f} M agent-steps/s", file=sys.stderr, flush=True)
 
+    # a caller that hands the state over through host memory every step: read it back after each step and write it
+    # again before the next (the PCIe-inclusive rate; the timed steps above keep the state in HBM)
+    st = sim.state()
+    t0 = time.perf_counter()
+    rt_steps = 20
+    for _ in range(rt_steps):
+        sim.restore(st)
+        sim.step(1)
+        st = sim.state()
+    host_rt_ms = 1000.0 * (time.perf_counter() - t0) / rt_steps
+    print(f"# with the state through host memory every step: {host_rt_ms:.3f} ms/step", file=sys.stderr, flush=True)
+
     # the safety-metric passes (density deposit + per-cell accumulators) inside the same steps: the same upload and
     # warm-up replay the same trajectory, so the difference is the passes' cost
     sim.upload(a)
@@ -478,6 +490,8 @@ def main():
                      "step_algorithmic_bytes": b_step, "step_achieved_gbs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "measured_copy_gbs": copy_gbs},
         "passes_ms": passes,
+        "host_roundtrip": {"ms_per_step": host_rt_ms, "steps": rt_steps,
+                           "what": "restore the state from host arrays, one step, read the state back (wall clock)"},
         "metrics": metrics,
         "batched": batch,
         "adjoint": adjoint,

@@ -218,7 +218,7 @@ def test_torch_autograd_wrapper(room_run):
     glob = torch.tensor([0.5, 1.0, 20.0], dtype=torch.float64, requires_grad=True)
     loss = CrowdLoss.apply(v0, glob)
     loss.backward()
-    assert float(loss) == pytest.approx(room_run["loss"], rel=1e-6)
+    assert float(loss.detach()) == pytest.approx(room_run["loss"], rel=1e-6)
     np.testing.assert_allclose(v0.grad.numpy(), room_run["gv0"], rtol=1e-5, atol=1e-6 * np.abs(room_run["gv0"]).max())
     np.testing.assert_allclose(glob.grad.numpy(), room_run["g"][:3], rtol=1e-5)
 
