@@ -112,7 +112,7 @@ def lib():
     return L
 
 
-ENONFINITE = -6
+OK, EINVAL, ESIZE, ENOMEM, ESCENE, EHIP, ENONFINITE = 0, -1, -2, -3, -4, -5, -6   # swarm_hip.h status codes
 
 
 class NonFiniteGradient(RuntimeError):

@@ -78,3 +78,23 @@ def test_metrics_entries_reject_null_contexts():
     assert L.swarm_read_metric_evac_steps(None, out.ctypes.data) == -1
     assert L.swarm_read_metric_cell_info(None, None, None) == -1
     assert L.swarm_scene_metric_grid(None, None, None, None, None, None, None) == -1
+
+
+def test_venue_entries_reject_bad_arguments():
+    L = C.CDLL(sh.LIB_PATH)
+    L.swarm_venue_seats.argtypes = [C.c_char_p] + [C.c_void_p] * 4 + [C.c_size_t, C.c_void_p]
+    L.swarm_venue_exit_map.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.swarm_venue_layout_get.argtypes = [C.c_char_p, C.c_void_p]
+    L.swarm_venue_probe.argtypes = [C.c_char_p, C.c_size_t] + [C.c_void_p] * 10
+    out = sh.VenueLayout()
+    assert L.swarm_venue_layout_get(b"XL", C.byref(out)) == sh.EINVAL   # unknown venue
+    assert L.swarm_venue_layout_get(None, C.byref(out)) == sh.EINVAL
+    n = C.c_size_t()
+    assert L.swarm_venue_seats(b"S", None, None, None, None, 0, C.byref(n)) == 0 and n.value == 3728
+    x = np.zeros(100, np.float32)
+    assert L.swarm_venue_seats(b"S", x.ctypes.data, None, None, None, 100, C.byref(n)) == sh.ESIZE   # too small
+    m = C.c_int()
+    e = np.zeros(4, np.int32)
+    assert L.swarm_venue_exit_map(b"M", e.ctypes.data, 4, C.byref(m)) == sh.ESIZE and m.value == 36
+    # points without their arrays: refused before any device work
+    assert L.swarm_venue_probe(b"S", 10, None, None, None, None, None, None, None, None, None, None) == sh.EINVAL
