@@ -92,14 +92,24 @@ def test_venue_s_full_evacuation():
     assert gper[:-1].sum() == active.sum() and (gper[:-1] > 0).all()
 
 
-def test_venue_m_spawn_steps_bit_exact():
-    """Venue M's plan at spawn, 15,832 of 16,384 agents seated: 20 steps, every quantity every step."""
+def test_venue_m_steps_bit_exact():
+    """Venue M's plan, 15,832 of 16,384 agents seated: 20 steps with every quantity every step, then on into the
+    congested regime (the gangways and doors queueing) to step 400, checked every 40 steps, with the evacuation record."""
     sim, o, _ = venue_pair("M", 16384)
     for k in range(1, 21):
         sim.step(1)
         o.step(1)
         compare_step(sim, o, f"venue M step {k}")
         assert np.array_equal(sim.assign(), o.assign()), f"venue M step {k} exit choices"
+    for k in range(60, 401, 40):
+        sim.step(40)
+        o.step(40)
+        compare_step(sim, o, f"venue M step {k}", cells=False)
+        assert np.array_equal(sim.assign(), o.assign()), f"venue M step {k} exit choices"
+    gs, ge, _ = sim.evac()
+    cs, ce = o.evac()
+    assert np.array_equal(gs, cs) and np.array_equal(ge, ce)
+    assert (gs >= 0).sum() > 1000   # people did leave
 
 
 def test_venue_l_spawn_steps_bit_exact():
