@@ -167,7 +167,8 @@ int swarm_venue_probe(const char* name, size_t n, const float* p, const float* e
 /* The venue's plan as a rectangle world for the step (the 2D cut of the analytic venue): walls the footprints of the seat
  * blocks, barrier, side and outer walls and the fenced pitch; the domain ends at the outer wall's outer face (the reference's
  * exit test, max(|x|, |y|) > exitDepth); exit rectangles beyond it, one per avenue and door, the L-shaped corner door as
- * two (its second halves after the reference's exits); the gangway inclines as stair bands. Built as swarm_scene_build_cached. */
+ * two (its second halves after the reference's exits); no stair bands (the reference's default model walks the inclines at
+ * the level speed, exit_choice.hpp:84-85). Built as swarm_scene_build_cached. */
 int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out);
 /* The reference's exit id of each plan exit (ref_exit[cap]; count: the plan's exits) */
 int swarm_venue_exit_map(const char* name, int32_t* ref_exit, int cap, int* count);

@@ -25,9 +25,6 @@ constexpr float MAX_STEP_UP_M = 0.30f;
 constexpr float WALL_MAX_NZ = 0.6f;
 constexpr int TERRACE_GRID_RES = 40;            // the terrace's neighbour grid: 40 cells over [-1, 1]
 constexpr float TERRACE_DOMAIN_MIN = -1.0f, TERRACE_DOMAIN_SPAN = 2.0f;
-// the descending incline speed at the gangways' rake relative to the level free speed (physics.comp inclineFreeSpeed
-// at g = -rake: V_STAIR_DOWN 0.655 m/s reached from the reference stair's slope tan 27 deg on, smoothstep below)
-constexpr double V_LEVEL_M_S = 1.34, V_STAIR_DOWN_M_S = 0.655, INCLINE_REF_SLOPE = 0.50952545;
 
 // The double of f's shortest round-trip decimal: the value the reference's shader compiler folds a literal to
 double decimalOf(float f) {
@@ -342,7 +339,9 @@ VenuePlan venuePlan(const Venue& v) {
     d.nav.resX = d.nav.resY = 2 * half;
     d.nav.originX = d.nav.originY = float(-double(half) * double(d.nav.cell));
 
-    // walls: the seat blocks, the barrier, the side and outer walls; the fenced pitch
+    // walls: the seat blocks, the barrier, the side and outer walls; the fenced pitch. The inclines are level floor for
+    // the free speed, as in the reference's default model (ModelOptions::inclineSpeed and stairFlow off,
+    // include/swarm/exit_choice.hpp:84-85)
     const int nb = seated::bandCount(L);
     for (int j = 1; j < nb - 1; ++j) {
         float lo, hi;
@@ -351,13 +350,6 @@ VenuePlan venuePlan(const Venue& v) {
         for (int i = 0; seated::cellAt(L, j, i, c); ++i) {
             if (c.kind == seated::SEAT || c.kind == seated::BARRIER || c.kind == seated::WALL || c.kind == seated::SIDE)
                 unfold(c.l0, c.l1, lo, hi, d.walls);
-            else if (c.kind == seated::INCLINE) {
-                const size_t before = d.stairs.size();
-                unfold(c.l0, c.l1, lo, hi, d.stairs);
-                const double u = std::min(1.0, double(L.rake) / INCLINE_REF_SLOPE), t = u * u * (3.0 - 2.0 * u);
-                const float factor = float(1.0 + (V_STAIR_DOWN_M_S / V_LEVEL_M_S - 1.0) * t);
-                d.stairFactor.insert(d.stairFactor.end(), d.stairs.size() - before, factor);
-            }
         }
     }
     d.walls.push_back(Rect{ -FENCE_HALF_OUTER, -FENCE_HALF_OUTER, FENCE_HALF_OUTER, FENCE_HALF_OUTER });
