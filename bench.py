@@ -85,7 +85,9 @@ def cpu_snapshot_rate(scene, a, n, threads, warmup, measure, rounds=3):
     import oracle_py as op
 
     tau, relax = sh.model_params(0.5)
-    o = op.Oracle(scene.info, scene.tables(), n, relax, tau, threads=threads)
+    lay = scene.venue_layout()
+    o = op.Oracle(scene.info, scene.tables(), n, relax, tau, threads=threads,
+                  venue=None if lay is None else op.venue_layout_arrays_of(lay))
     o.set_state(a)
     active = int(np.sum(a["w"] > 0))
     medians, stages = [], []

@@ -167,9 +167,14 @@ int swarm_venue_probe(const char* name, size_t n, const float* p, const float* e
 /* The venue's plan as a rectangle world for the step (the 2D cut of the analytic venue): walls the footprints of the seat
  * blocks, barrier, side and outer walls and the fenced pitch; the domain ends at the outer wall's outer face (the reference's
  * exit test, max(|x|, |y|) > exitDepth); exit rectangles beyond it, one per avenue and door, the L-shaped corner door as
- * two (its second halves after the reference's exits); no stair bands (the reference's default model walks the inclines at
- * the level speed, exit_choice.hpp:84-85). Built as swarm_scene_build_cached. */
+ * two (its second halves after the reference's exits). The plan carries the layout: the step applies the reference's
+ * default venue model (defaultModelOptions, src/exit_choice.cpp:51-58) on the gangway inclines, the free speed and the
+ * headway rule's time gap by the floor's slope along e_goal (cpu_crowd_sim.cpp:727-733, the agent's v0 in place of
+ * V_MAX). Built as swarm_scene_build_cached. */
 int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out);
+/* A scene's venue: has_venue 1 for a venue plan (swarm_scene_build_venue), then layout (may be NULL) holds its layout
+ * (the fields up to corner_door_lo, the rest zero); 0 otherwise (layout zeroed). */
+int swarm_scene_read_venue(const swarm_scene* s, int* has_venue, swarm_venue_layout* layout);
 /* The reference's exit id of each plan exit (ref_exit[cap]; count: the plan's exits) */
 int swarm_venue_exit_map(const char* name, int32_t* ref_exit, int cap, int* count);
 

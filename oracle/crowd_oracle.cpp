@@ -30,6 +30,14 @@ constexpr float R_SEP = 0.016f, BODY_DIAM = 0.0128f;
 constexpr float SEP_MARGIN = R_SEP - BODY_DIAM;   // exact (Sterbenz)
 constexpr float HEADWAY_RATE = 0x1.d26afcp+5f;     // 1.0 / T_HEADWAY (sim_params.hpp:511)
 constexpr float R_PERC2 = 0x1.096bbap-9f;          // R_PERC * R_PERC (sim_params.hpp:503)
+// the seated venues' incline rules (sim_params.hpp:441-481; EMPIRICAL's stair time gaps, exit_choice.cpp:43-48)
+constexpr float V_MAX = 0.75f;
+constexpr float T_HEADWAY = 0x1.1904b4p-6f;        // 0.3 s / TIME_SCALE_S_PER_UNIT
+constexpr float V_STAIR_UP = 0x1.4c6afcp-2f;       // 0.58 m/s in model units
+constexpr float V_STAIR_DOWN = 0x1.77672ap-2f;     // 0.655 m/s
+constexpr float INCLINE_INV_REF_SLOPE = 0x1.f66da2p+0f;   // 1 / tan(27 deg)
+constexpr float T_HEADWAY_STAIR_UP = 0x1.24ba3cp-5f;      // 0.625 s
+constexpr float T_HEADWAY_STAIR_DOWN = 0x1.306fc4p-5f;    // 0.65 s
 constexpr float PROJ_RELAX = 1.0f;
 constexpr float PROJ_MAX_SHIFT_RADII = 0.05f;
 constexpr float W_OBS = 10.0f;
@@ -554,6 +562,25 @@ struct oracle_ctx {
         }
         return -1;
     }
+    // seated::inclineFreeSpeed / inclineHeadwayTime (src/venue.cpp:652-670): the smoothstep of |g| / INCLINE_REF_SLOPE
+    // blends the level value into the stair value of the slope's sign
+    static float inclineBlend(float g) {
+        const float u = std::clamp(std::abs(g) * INCLINE_INV_REF_SLOPE, 0.0f, 1.0f);
+        const float uu = u * u, w = 3.0f - 2.0f * u;
+        return uu * w;
+    }
+    static float inclineFreeSpeed(float g) {
+        const float t = inclineBlend(g);
+        const float dv = (g > 0.0f ? V_STAIR_UP : V_STAIR_DOWN) - V_MAX;
+        const float k = dv * t;
+        return V_MAX + k;
+    }
+    static float inclineHeadwayTime(float g) {
+        const float t = inclineBlend(g);
+        const float dt = (g > 0.0f ? T_HEADWAY_STAIR_UP : T_HEADWAY_STAIR_DOWN) - T_HEADWAY;
+        const float k = dt * t;
+        return T_HEADWAY + k;
+    }
     float stairFactor(V2 p) const {
         const float gx = (p.x - sc.navOX) * sc.navInvCell, gy = (p.y - sc.navOY) * sc.navInvCell;
         const int cx = std::clamp(int(std::floor(gx)), 0, sc.navRX - 1), cy = std::clamp(int(std::floor(gy)), 0, sc.navRY - 1);
@@ -609,8 +636,16 @@ struct oracle_ctx {
                 }
             }
         }
-        const float vFree = stairFactor(p) * v0[self];
-        const float vDes = std::clamp(sAhead * HEADWAY_RATE, 0.0f, vFree) * length(eGoal);
+        float vFree, vDes;
+        if (sc.venue) {   // the default venue model's incline speed and stair spacing (cpu_crowd_sim.cpp:727-733)
+            const float slope = oracle_venue_slope(sc.venue, p.x, p.y, eGoal.x, eGoal.y);
+            vFree = inclineFreeSpeed(slope);
+            vDes = (slope != 0.0f ? std::clamp(sAhead / inclineHeadwayTime(slope), 0.0f, vFree)
+                                  : std::clamp(sAhead * HEADWAY_RATE, 0.0f, vFree)) * length(eGoal);
+        } else {
+            vFree = stairFactor(p) * v0[self];
+            vDes = std::clamp(sAhead * HEADWAY_RATE, 0.0f, vFree) * length(eGoal);
+        }
         const V2 eDir = safeNormalize(eGoal + sep);
 
         V2 aObs{ 0.0f, 0.0f };   // obstacle avoidance (:739-748)
@@ -1013,12 +1048,20 @@ void oracle_get_evac(const oracle_ctx* c, int32_t* step, int32_t* exitId) {
 int oracle_num_steps(const oracle_ctx* c) { return c->steps; }
 void oracle_get_times(const oracle_ctx* c, double* t8) { std::memcpy(t8, c->times, sizeof(c->times)); }
 
+void oracle_incline(int n, const float* g, float* v_free, float* t_headway) {
+    for (int i = 0; i < n; ++i) {
+        v_free[i] = oracle_ctx::inclineFreeSpeed(g[i]);
+        t_headway[i] = oracle_ctx::inclineHeadwayTime(g[i]);
+    }
+}
+
 int oracle_constants(float* out, int cap) {
     const float v[] = { DT, K_SEP, R_SEP, BODY_DIAM, SEP_MARGIN, HEADWAY_RATE, R_PERC2, PROJ_MAX_SHIFT_RADII, W_OBS,
                         GOAL_DEADZONE, OBSTACLE_RANGE, OBSTACLE_MIN_NORMAL, OBSTACLE_GRAD_H, OBSTACLE_GRAD_SCALE,
                         SAFE_NORMALIZE_EPS, SEP_STACKED_XY2, FEELER_REACH_RADII, WALL_RECAST_TOL_RADII, CENTRE_GRAD_MIN,
                         PLANE_PROBE_RADII, PARK_COORD, HASH_TO_RADIANS, CHOICE_REL_MARGIN, CHOICE_ABS_MARGIN_S,
-                        CHOICE_SWITCH_SHARE, float(CHOICE_STEPS) };
+                        CHOICE_SWITCH_SHARE, float(CHOICE_STEPS), V_MAX, T_HEADWAY, V_STAIR_UP, V_STAIR_DOWN,
+                        INCLINE_INV_REF_SLOPE, T_HEADWAY_STAIR_UP, T_HEADWAY_STAIR_DOWN };
     const int m = int(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < m && i < cap; ++i) out[i] = v[i];
     return m;

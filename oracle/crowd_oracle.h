@@ -36,6 +36,7 @@ typedef struct {
     const float* exitRects;                /* E x (lox, loy, hix, hiy) */
     int nWalls;                            /* wall boxes (e_goal's wall rule queries them exactly) */
     const float* wallRects;                /* nWalls x (lox, loy, hix, hiy) */
+    const void* venue;                     /* a seated venue's plan: its layout (oracle_venue_new), else NULL */
 } oracle_scene;
 
 typedef struct {
@@ -45,6 +46,17 @@ typedef struct {
     int projIters;        /* Jacobi iterations (PROJ_ITERATIONS) */
     int threads;          /* OpenMP threads, <= 0: all */
 } oracle_params;
+
+/* The analytic seated venue (venue_oracle.cpp) for the step: on a venue plan the free speed and the headway rule's time gap
+ * follow the floor's slope along e_goal (the reference's default venue model, cpu_crowd_sim.cpp:727-733), at the
+ * reference's global free speed V_MAX (a venue plan does not read v0). The layout arrays are those of
+ * oracle_venue_eval. */
+void* oracle_venue_new(const float* lf, const int* li, const float* gwLo, const float* gwHi, const int* gwRow, const int* gwSeated,
+                       const float* doorLo, const float* doorHi);
+void oracle_venue_free(void* venue);
+float oracle_venue_slope(const void* venue, float x, float y, float ex, float ey);
+/* seated::inclineFreeSpeed and inclineHeadwayTime at EMPIRICAL's time gaps (src/venue.cpp:652-670) at n slopes */
+void oracle_incline(int n, const float* g, float* v_free, float* t_headway);
 
 oracle_ctx* oracle_create(const oracle_scene* scene, const oracle_params* params, int n);
 void oracle_destroy(oracle_ctx* c);

@@ -27,7 +27,7 @@ class OracleScene(C.Structure):
                 ("navCell", C.c_float), ("navInvCell", C.c_float), ("K", C.c_int), ("E", C.c_int),
                 ("navInfo", C.c_void_p), ("navSlot", C.c_void_p), ("navDist", C.c_void_p), ("invCapS", C.c_void_p),
                 ("secondsPerUnit", C.c_float), ("distQuantum", C.c_float), ("stair", C.c_void_p), ("exitRects", C.c_void_p),
-                ("nWalls", C.c_int), ("wallRects", C.c_void_p)]
+                ("nWalls", C.c_int), ("wallRects", C.c_void_p), ("venue", C.c_void_p)]
 
 
 class OracleParams(C.Structure):
@@ -64,6 +64,12 @@ def lib():
         L.oracle_num_steps.argtypes = [vp]
         L.oracle_get_times.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
         L.oracle_constants.argtypes = [_f32p, C.c_int]
+        L.oracle_venue_new.argtypes = [vp] * 8
+        L.oracle_venue_new.restype = vp
+        L.oracle_venue_free.argtypes = [vp]
+        L.oracle_venue_slope.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.oracle_venue_slope.restype = C.c_float
+        L.oracle_incline.argtypes = [C.c_int, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -103,9 +109,16 @@ def bitonic_sort(keys):
 
 
 class Oracle:
-    def __init__(self, info, tables, n, relax_step, tau, obstacle_gain=20.0, proj_iters=32, threads=0):
+    def __init__(self, info, tables, n, relax_step, tau, obstacle_gain=20.0, proj_iters=32, threads=0, venue=None):
+        """venue: a seated venue plan's layout arrays (venue_layout_arrays), the incline rules' input; None elsewhere."""
         self._tables = tables
+        self.h = None
+        self._venue = None
         sc = scene_struct(info, tables)
+        if venue is not None:
+            self._venue_arrays = [np.ascontiguousarray(a) for a in venue]
+            self._venue = lib().oracle_venue_new(*[a.ctypes.data for a in self._venue_arrays])
+            sc.venue = self._venue
         p = OracleParams(relax_step, tau, obstacle_gain, proj_iters, threads)
         self.h = lib().oracle_create(C.byref(sc), C.byref(p), n)
         if not self.h:
@@ -222,6 +235,9 @@ class Oracle:
         if self.h:
             lib().oracle_destroy(self.h)
             self.h = None
+        if self._venue:
+            lib().oracle_venue_free(self._venue)
+            self._venue = None
 
     def __del__(self):
         try:
@@ -371,6 +387,29 @@ def venue_layout_arrays(kat, name):
     return (g("layout_f").astype(np.float32), g("layout_i").astype(np.int32), g("layout_gw").astype(np.float32),
             g("layout_gw_hi").astype(np.float32), g("layout_gw_row").astype(np.int32), g("layout_gw_seated").astype(np.int32),
             g("layout_door_lo").astype(np.float32), g("layout_door_hi").astype(np.float32))
+
+
+_LAYOUT_F = ("stand_front", "row0", "row_depth", "clearway", "conc_front", "conc_back", "outer_face", "landing_end", "avenue_half",
+             "lat0", "gangway_lo", "gangway_hi", "exit_lo", "exit_hi", "corner_inset", "seat_inset", "face_margin", "wall_above",
+             "wall_clear", "first_row", "row_rise", "seat_top", "barrier_top", "concourse_z", "wall_top", "rake", "seat_width",
+             "domain_half", "corner_door_lo")
+
+
+def venue_layout_arrays_of(lay):
+    """The oracle's layout arrays from a swarm_venue_layout (its field names), in venue_layout_arrays' order."""
+    f = lambda v: np.ascontiguousarray(np.asarray(v, np.float32))
+    i = lambda v: np.ascontiguousarray(np.asarray(v, np.int32))
+    return (f([getattr(lay, k) for k in _LAYOUT_F]), i([lay.rows, lay.max_band_steps, lay.gangways, lay.doors, 0]),
+            f(list(lay.gw_lo)), f(list(lay.gw_hi)), i(list(lay.gw_row)), i(list(lay.gw_seated)), f(list(lay.door_lo)),
+            f(list(lay.door_hi)))
+
+
+def incline(g):
+    """seated::inclineFreeSpeed and inclineHeadwayTime (EMPIRICAL) at slopes g: (v_free, t_headway) float32."""
+    g = np.ascontiguousarray(g, np.float32)
+    v, t = np.zeros_like(g), np.zeros_like(g)
+    lib().oracle_incline(len(g), g.ctypes.data, v.ctypes.data, t.ctypes.data)
+    return v, t
 
 
 def _venue_sigs(L):

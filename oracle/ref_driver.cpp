@@ -633,6 +633,22 @@ void venueCases() {
         put(p + "probe/exit", 'i', exitL);
         put(p + "probe/gangway", 'i', gangway);
     }
+    {   // the default model's incline rules (seated::inclineFreeSpeed / inclineHeadwayTime at EMPIRICAL's time gaps)
+        std::vector<float> g = { 0.0f, -0.0f, 1e-30f, -1e-30f, 0.1f, -0.1f, 0.25f, -0.25f, 0.5f, -0.5f, INCLINE_REF_SLOPE, -INCLINE_REF_SLOPE,
+                                 0.6f, -0.6f, 2.0f, -2.0f };
+        Rng rng{ 0x1C11E5ull };
+        for (int i = 0; i < 1024; ++i) g.push_back(-0.7f + 1.4f * float(rng.uniform()));
+        const glm::vec2 t = stairHeadwayTimes(StairFlow::Empirical);
+        std::vector<float> v, h;
+        for (float x : g) {
+            v.push_back(seated::inclineFreeSpeed(x));
+            h.push_back(seated::inclineHeadwayTime(x, t.x, t.y));
+        }
+        put("venue/incline/g", 'f', g);
+        put("venue/incline/v_free", 'f', v);
+        put("venue/incline/t_headway", 'f', h);
+        put("venue/incline/consts", 'f', std::vector<float>{ V_MAX, V_STAIR_UP, V_STAIR_DOWN, T_HEADWAY, t.x, t.y, INCLINE_INV_REF_SLOPE });
+    }
 }
 
 } // namespace

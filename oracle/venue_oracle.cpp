@@ -359,6 +359,18 @@ int gangwayAt(const OVenue& V, float x, float y) {
     return straight ? base + (gabs(y) >= gabs(x) ? 0 : V.gangways) + i : base + 2 * V.gangways;
 }
 
+float floorSlope(const OVenue& V, float x, float y, float ex, float ey) {   // seated::floorSlope
+    float lat, dep;
+    foldXY(x, y, lat, dep);
+    const int j = V.bandOf(dep);
+    const OCell& c = V.bands[size_t(j)].cells[size_t(V.own(j, lat))];
+    if (c.kind != INCLINE) return 0.0f;
+    float gx, gy;
+    unfold(x, y, 0.0f, V.rake, gx, gy);
+    const float a = gx * ex, b = gy * ey;
+    return a + b;
+}
+
 OVenue make(const float* lf, const int* li, const float* gwLo, const float* gwHi, const int* gwRow, const int* gwSeated,
             const float* doorLo, const float* doorHi) {
     OVenue V{};
@@ -412,16 +424,21 @@ void oracle_venue_eval(const float* lf, const int* li, const float* gwLo, const 
         floorH[i] = V.top(c.kind, c.row, dep);
         cell[3 * i] = c.kind; cell[3 * i + 1] = c.row; cell[3 * i + 2] = j;
         for (int k = 0; k < 3; ++k) wall(V, x, y, z, reach3[k], wallOut + 9 * i + 3 * k);
-        if (c.kind != INCLINE) slope[i] = 0.0f;
-        else {
-            float gx, gy;
-            unfold(x, y, 0.0f, V.rake, gx, gy);
-            const float a = gx * e[2 * i], b = gy * e[2 * i + 1];
-            slope[i] = a + b;
-        }
+        slope[i] = floorSlope(V, x, y, e[2 * i], e[2 * i + 1]);
         exitId[i] = exitByLat(V, x, y);
         gangway[i] = gangwayAt(V, x, y);
     }
+}
+
+// A venue for the step's incline rules (oracle_scene.venue): made once, read by oracle_venue_slope, freed by
+// oracle_venue_free
+void* oracle_venue_new(const float* lf, const int* li, const float* gwLo, const float* gwHi, const int* gwRow, const int* gwSeated,
+                       const float* doorLo, const float* doorHi) {
+    return new OVenue(make(lf, li, gwLo, gwHi, gwRow, gwSeated, doorLo, doorHi));
+}
+void oracle_venue_free(void* v) { delete static_cast<OVenue*>(v); }
+float oracle_venue_slope(const void* v, float x, float y, float ex, float ey) {
+    return floorSlope(*static_cast<const OVenue*>(v), x, y, ex, ey);
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "seated_venue.h"
 #include "swarm_params.h"
 
 namespace swarm {
@@ -33,6 +34,7 @@ struct SceneView {
     const float* wallRects;            // wall boxes, 4 floats each (lox, loy, hix, hiy)
     const uint32_t* wallBinStart;      // per hash cell: its walls within reach, wallBinList[start[c] .. start[c + 1])
     const uint32_t* wallBinList;
+    const seated::Layout* venue;       // a seated venue's plan: its layout (the incline rules), else null
 };
 
 struct ModelParams {
@@ -265,6 +267,30 @@ __device__ inline int exitOf(const SceneView& s, V2 p) {
 }
 __device__ inline bool parkTest(const SceneView& s, V2 p) {
     return exitOf(s, p) >= 0 || p.x < s.xmin || p.x > s.xmax || p.y < s.ymin || p.y > s.ymax;
+}
+// The seated venues' incline rules (physics.comp inclineFreeSpeed / inclineHeadwayTime, src/venue.cpp:652-670), with
+// the agent's own free speed v0 in place of V_MAX (v0 R_STAIR is V_STAIR bit for bit at v0 = V_MAX): the free speed and
+// the headway rule's time gap for the floor's slope g along the travel direction, blended by the smoothstep of
+// |g| / INCLINE_REF_SLOPE
+__device__ inline float inclineFreeSpeed(float v0, float g) {
+    const float a = fabsf(g) * INCLINE_INV_REF_SLOPE;
+    const float u = a < 0.0f ? 0.0f : (1.0f < a ? 1.0f : a);
+    const float uu = u * u, w = 3.0f - 2.0f * u;
+    const float t = uu * w;
+    const float vStair = v0 * (g > 0.0f ? R_STAIR_UP : R_STAIR_DOWN);
+    const float dv = vStair - v0;
+    const float k = dv * t;
+    return v0 + k;
+}
+__device__ inline float inclineHeadwayTime(float g) {
+    const float a = fabsf(g) * INCLINE_INV_REF_SLOPE;
+    const float u = a < 0.0f ? 0.0f : (1.0f < a ? 1.0f : a);
+    const float uu = u * u, w = 3.0f - 2.0f * u;
+    const float t = uu * w;
+    const float tStair = g > 0.0f ? T_HEADWAY_STAIR_UP : T_HEADWAY_STAIR_DOWN;
+    const float dt = tStair - T_HEADWAY;
+    const float k = dt * t;
+    return T_HEADWAY + k;
 }
 __device__ inline float stairFactor(const SceneView& s, V2 p) {
     const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;

@@ -724,8 +724,11 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
         }
     }
     const float stair = stairFactor(s, p);
-    const float vFree = stair * b.v0[agent];
-    float vc = sAhead * HEADWAY_RATE;
+    // a seated venue's plan: the free speed and the headway time gap by the floor's slope along e_goal on the gangway
+    // inclines (physics.comp VENUE_INCLINE_SPEED / VENUE_STAIR_HEADWAY; cpu_crowd_sim.cpp:727-733)
+    const float slope = s.venue ? seated::floorSlope(*s.venue, seated::F2{ p.x, p.y }, seated::F2{ eGoal.x, eGoal.y }) : 0.0f;
+    const float vFree = s.venue ? inclineFreeSpeed(b.v0[agent], slope) : stair * b.v0[agent];
+    float vc = s.venue && slope != 0.0f ? sAhead / inclineHeadwayTime(slope) : sAhead * HEADWAY_RATE;
     const bool vcZero = vc < 0.0f, vcFree = !vcZero && vFree < vc;
     vc = vcZero ? 0.0f : (vcFree ? vFree : vc);
     const float vDes = vc * len2(eGoal);

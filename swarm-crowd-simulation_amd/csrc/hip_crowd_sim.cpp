@@ -58,6 +58,7 @@ void DeviceScene::upload(const Scene& s) {
     v.wallRects = static_cast<const float*>(put(wr.data(), wr.size() * 4));
     v.wallBinStart = static_cast<const uint32_t*>(put(s.wallBinStart.data(), s.wallBinStart.size() * 4));
     v.wallBinList = static_cast<const uint32_t*>(put(s.wallBinList.data(), s.wallBinList.size() * 4));
+    v.venue = s.hasVenue ? static_cast<const seated::Layout*>(put(&s.venue, sizeof s.venue)) : nullptr;
 }
 
 void DeviceScene::release() {
@@ -936,6 +937,8 @@ double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float ov
         throw std::invalid_argument("adjointForward: the adjoint needs 2 to 32 projection iterations");
     if (r0_.size() != size_t(n_)) throw std::invalid_argument("adjointForward: upload a state first");
     if (!(radiusScale > 0.0f)) throw std::invalid_argument("adjointForward: the radius scale must be positive");
+    if (scenes_[0].view.venue)
+        throw std::invalid_argument("adjointForward: a venue plan's incline rules have no adjoint (a rectangle world's step only)");
     // the segment length: the whole run when its tape fits the budget (no recompute), else the checkpoint interval;
     // with several segments, a second segment tape when the two fit ADJ_DUAL_LIMIT (the pipelined backward)
     const int segLen = size_t(K) * tapeBytesPerStep() <= tapeBudget_ ? K : std::min(every, K);

@@ -372,6 +372,8 @@ Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) 
     if (!(d.xmax > d.xmin && d.ymax > d.ymin && d.hashCell > 0.0f)) throw std::invalid_argument("buildScene: empty domain");
     Scene s;
     s.xmin = d.xmin; s.ymin = d.ymin; s.xmax = d.xmax; s.ymax = d.ymax;
+    s.hasVenue = d.hasVenue ? 1 : 0;
+    s.venue = d.venue;
     s.hashNX = int(std::lround(double(d.xmax - d.xmin) / double(d.hashCell)));
     s.hashNY = int(std::lround(double(d.ymax - d.ymin) / double(d.hashCell)));
     s.hashSpanX = d.xmax - d.xmin;

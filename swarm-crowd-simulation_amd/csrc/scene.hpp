@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include "seated_venue.h"
+
 namespace swarm {
 
 // Closed axis-aligned rectangle (RectGeometry's Rect, include/swarm/nav_field.hpp:135-138)
@@ -44,6 +46,10 @@ struct SceneDesc {
     std::vector<Rect> walls, exits, stairs;
     std::vector<float> stairFactor;   // per stair rect
     std::vector<float> exitWidthM;    // per exit, metres (0: the exit rect's longer side)
+    // a seated venue's plan: its layout, for the incline rules of the step (the free speed and the headway time gap by the
+    // floor's slope along e_goal on the gangway inclines)
+    bool hasVenue = false;
+    seated::Layout venue{};
 };
 
 // The tables the step reads (uploaded to the GPU as they are)
@@ -84,6 +90,8 @@ struct Scene {
     std::vector<float> metricWalkableM2;
     double buildSeconds = 0.0;
     int fromCache = 0;                     // the tables came from the scene cache
+    int hasVenue = 0;                      // a seated venue's plan: the layout of its incline rules
+    seated::Layout venue{};
 };
 
 // Build every table of a scene. maxRadius / maxV0: the largest agent radius and free speed the scene

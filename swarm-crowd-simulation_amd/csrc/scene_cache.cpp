@@ -24,7 +24,7 @@ namespace swarm {
 
 namespace {
 
-constexpr uint32_t SCENE_CACHE_VERSION = 1;
+constexpr uint32_t SCENE_CACHE_VERSION = 2;
 constexpr char MAGIC[8] = { 'S', 'W', 'S', 'C', 'E', 'N', 'E', '1' };
 
 struct Fnv1a {   // FNV-1a, 64 bits (Fowler, Noll, Vo)
@@ -129,6 +129,8 @@ void write(Writer& w, const Scene& s) {
     w.pod(s.metricOffX);
     w.pod(s.metricOffY);
     w.vec(s.metricWalkableM2);
+    w.pod(s.hasVenue);
+    w.pod(s.venue);
 }
 
 bool read(Reader& r, Scene& s) {
@@ -141,7 +143,7 @@ bool read(Reader& r, Scene& s) {
          r.vec(s.navSlot, LIM) && r.vec(s.navDist, LIM) && r.vec(s.invCapS, LIM) && r.pod(s.secondsPerUnit) && r.pod(s.distQuantum) &&
          r.vec(s.stair, LIM) && r.vec(s.exits, LIM) && r.vec(s.walls, LIM) && r.vec(s.wallBinStart, LIM) && r.vec(s.wallBinList, LIM) &&
          r.pod(s.maxRadius) && r.pod(s.maxV0) && r.pod(s.metricResX) && r.pod(s.metricResY) && r.pod(s.metricCellsPerUnit) &&
-         r.pod(s.metricOffX) && r.pod(s.metricOffY) && r.vec(s.metricWalkableM2, LIM);
+         r.pod(s.metricOffX) && r.pod(s.metricOffY) && r.vec(s.metricWalkableM2, LIM) && r.pod(s.hasVenue) && r.pod(s.venue);
     if (!ok || !r.done()) return false;
     // the sizes every reader of the tables relies on
     const size_t sdfN = size_t(s.sdfNX) * size_t(s.sdfNY), navN = s.nav.cells(), hashN = size_t(s.hashNX) * size_t(s.hashNY);
@@ -169,6 +171,15 @@ uint64_t sceneCacheKey(const SceneDesc& d, float maxRadius, float maxV0) {
     for (float v : d.stairFactor) s << bits(v) << "|";
     s << "w|";
     for (float v : d.exitWidthM) s << bits(v) << "|";
+    s << "v|" << (d.hasVenue ? 1 : 0) << "|";
+    if (d.hasVenue) {   // the layout's bytes, in hex
+        const unsigned char* b = reinterpret_cast<const unsigned char*>(&d.venue);
+        char h[3];
+        for (size_t i = 0; i < sizeof d.venue; ++i) {
+            std::snprintf(h, sizeof h, "%02x", b[i]);
+            s << h;
+        }
+    }
     const std::string text = s.str();
     Fnv1a h;
     h.bytes(text.data(), text.size());

@@ -168,6 +168,17 @@ int swarm_scene_read_walls(const swarm_scene* sc, float* rects, uint32_t* bin_st
     return SWARM_OK;
 }
 
+int swarm_scene_read_venue(const swarm_scene* sc, int* has_venue, swarm_venue_layout* layout) {
+    if (!sc || !has_venue) { g_err = "null argument"; return SWARM_EINVAL; }
+    const swarm::Scene& s = sc->s;
+    *has_venue = s.hasVenue ? 1 : 0;
+    if (layout) {
+        std::memset(layout, 0, sizeof *layout);
+        if (s.hasVenue) std::memcpy(layout, &s.venue, sizeof s.venue);
+    }
+    return SWARM_OK;
+}
+
 int swarm_scene_metric_grid(const swarm_scene* sc, int* res_x, int* res_y, float* cells_per_unit, float* off_x, float* off_y,
                             float* walkable_m2) {
     if (!sc) { g_err = "null argument"; return SWARM_EINVAL; }

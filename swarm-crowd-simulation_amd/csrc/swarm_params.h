@@ -50,6 +50,13 @@ constexpr float TAU_RELAX = 0x1.d45d2cp-6f;  // 0.5 / 17.490671 folded (0.028586
 constexpr float T_HEADWAY = 0x1.1904b4p-6f;  // 0.3 / 17.490671 folded (0.017152)
 constexpr float HEADWAY_RATE = 0x1.d26afcp+5f;   // 1.0 / T_HEADWAY folded (58.30224)
 constexpr float RELAX_STEP = 0x1.68de1cp-1f;     // exp(-DT / TAU_RELAX) folded in double (0.70481956)
+// --- the incline speed and stair spacing of the seated venues' default model (sim_params.hpp:429-481; ModelOptions
+// {QueueAware, inclineSpeed, StairFlow::Empirical}, exit_choice.cpp:51-58) ---
+constexpr float INCLINE_INV_REF_SLOPE = 0x1.f66da2p+0f;   // 1.0 / 0.50952545 (tan 27 deg) folded (1.9626104)
+constexpr float R_STAIR_UP = 0x1.bb3950p-2f;              // 0.58 / 1.34 folded: V_STAIR_UP / V_MAX, per agent (0.43283582)
+constexpr float R_STAIR_DOWN = 0x1.f4898ep-2f;            // 0.655 / 1.34 folded: V_STAIR_DOWN / V_MAX (0.48880598)
+constexpr float T_HEADWAY_STAIR_UP = 0x1.24ba3cp-5f;      // 0.625 / 17.490671 folded, EMPIRICAL (0.035733335)
+constexpr float T_HEADWAY_STAIR_DOWN = 0x1.306fc4p-5f;    // 0.65 / 17.490671 folded, EMPIRICAL (0.03716267)
 constexpr float R_PERC = 0.045f;
 constexpr float R_PERC2 = 0x1.096bbap-9f;        // R_PERC * R_PERC folded (0.002025)
 

@@ -332,6 +332,8 @@ VenuePlan venuePlan(const Venue& v) {
     d.xmin = -of; d.ymin = -of; d.xmax = of; d.ymax = of;
     d.metresPerUnit = M_PER_UNIT;
     d.navBody = BODY_RADIUS;
+    d.hasVenue = true;   // the step's incline rules read the layout
+    d.venue = L;
     // the navigation grid: 0.1 m cells, symmetric about 0, reaching the landings
     d.nav.cell = 0.0032f;
     d.nav.invCell = float(1.0 / double(d.nav.cell));
@@ -339,9 +341,7 @@ VenuePlan venuePlan(const Venue& v) {
     d.nav.resX = d.nav.resY = 2 * half;
     d.nav.originX = d.nav.originY = float(-double(half) * double(d.nav.cell));
 
-    // walls: the seat blocks, the barrier, the side and outer walls; the fenced pitch. The inclines are level floor for
-    // the free speed, as in the reference's default model (ModelOptions::inclineSpeed and stairFlow off,
-    // include/swarm/exit_choice.hpp:84-85)
+    // walls: the seat blocks, the barrier, the side and outer walls; the fenced pitch
     const int nb = seated::bandCount(L);
     for (int j = 1; j < nb - 1; ++j) {
         float lo, hi;

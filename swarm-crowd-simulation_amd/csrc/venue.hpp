@@ -58,8 +58,9 @@ double venueExitWidthM(const Venue& v, int e);
 // outer walls (each stands higher than a body above every floor beside it) and the fenced pitch; agents leave when
 // max(|x|, |y|) passes the outer wall's outer face (the domain's edge). Exit rectangles lie beyond that face, one per
 // avenue and door; the corner door, L-shaped there, is two rectangles, the second appended after the reference's exits.
-// refExit[e] is the reference's exit id of plan exit e. No stair bands: the reference's default model walks the gangway
-// inclines at the level free speed.
+// refExit[e] is the reference's exit id of plan exit e. The plan carries the layout: the step applies the reference's
+// default model's incline rules on the gangway inclines (the free speed and the headway time gap by the floor's slope
+// along e_goal), no stair bands.
 struct VenuePlan {
     SceneDesc desc;
     std::vector<int> refExit;

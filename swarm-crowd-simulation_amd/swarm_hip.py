@@ -165,7 +165,8 @@ class Scene:
 
     @classmethod
     def venue(cls, name, max_radius=0.0064, max_v0=0.75, threads=0, cache_dir=None):
-        """The plan of the reference's seated venue S, M or L (walls, exits past the outer face, gangway stair bands)."""
+        """The plan of the reference's seated venue S, M or L (walls, exits past the outer face; the layout for the step's
+        incline rules)."""
         h = C.c_void_p()
         L = lib()
         L.swarm_scene_build_venue.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_int, C.c_char_p, C.c_void_p]
@@ -192,6 +193,15 @@ class Scene:
         h = C.c_void_p()
         check(lib().swarm_scene_build(C.byref(d), max_radius, max_v0, threads, C.byref(h)))
         return cls(h)
+
+    def venue_layout(self):
+        """The layout a venue plan carries (a VenueLayout up to corner_door_lo), None for other scenes."""
+        L = lib()
+        L.swarm_scene_read_venue.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        has = C.c_int()
+        out = VenueLayout()
+        check(L.swarm_scene_read_venue(self.h, C.byref(has), C.byref(out)))
+        return out if has.value else None
 
     def tables(self):
         """Copies of every table, as numpy arrays (the oracle's inputs)."""

@@ -39,9 +39,12 @@ def kat_rect_scene(g, prefix, body=None):
 
 
 def oracle_for(scene, n, proj_iters=32, threads=0, tau_s=0.5, obstacle_gain=20.0):
+    """The oracle over the scene's tables (and, for a venue plan, the layout its incline rules read)."""
     tau, relax = sh.model_params(tau_s)
     t = scene.tables()
-    return op.Oracle(scene.info, t, n, relax, tau, obstacle_gain=obstacle_gain, proj_iters=proj_iters, threads=threads)
+    lay = scene.venue_layout()
+    venue = None if lay is None else op.venue_layout_arrays_of(lay)
+    return op.Oracle(scene.info, t, n, relax, tau, obstacle_gain=obstacle_gain, proj_iters=proj_iters, threads=threads, venue=venue)
 
 
 def agents_from_golden(g, tag):

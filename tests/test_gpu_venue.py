@@ -2,9 +2,10 @@
 
 The probe kernel (swarm_venue_probe) evaluates seated::sdfGrad / floorHeight / cellAt / wall / floorSlope / exitByLat /
 gangwayAt on the GPU: bit for bit the reference's answers at its probe points (venue_kat.npz, written by the reference's
-own src/venue.cpp) and the oracle's at fresh random points. The step on the plan of venue S (and M at spawn) runs the
-reference's seated crowd (its seat spawn, global radius and free speed) and is bit for bit the oracle's: state, sorted
-grid, cell table and exit choices every step, then the evacuation curve and every agent's (step, exit) to the end.
+own src/venue.cpp) and the oracle's at fresh random points. The step on the plans of venues S, M and L runs the
+reference's seated crowd (its seat spawn, global radius and free speed) under the reference's default venue model (the
+incline speed and the stair spacing on the gangways) and is bit for bit the oracle's: state, sorted grid, cell table and
+exit choices every step, then the evacuation curve and every agent's (step, exit) to the end.
 """
 import os
 import tempfile
@@ -90,6 +91,63 @@ def test_venue_s_full_evacuation():
     active = a["w"] > 0
     assert (gs[active] >= 0).all(), "every seated agent leaves within 3,000 steps"
     assert gper[:-1].sum() == active.sum() and (gper[:-1] > 0).all()
+
+
+def test_venue_s_incline_walkers_bit_exact():
+    """Single agents on venue S's gangways, driven up and down the inclines and along the level concourse in all four
+    quadrants, two of them behind a standing agent: 60 steps, every quantity every step, against the oracle (whose
+    incline rules are pinned to the reference's inclineFreeSpeed / inclineHeadwayTime); the free walkers settle at the
+    stair speeds of the rake and at the level speed."""
+    scene = sh.Scene.venue("S")
+    # per quadrant, one walk in each of its two wings (x <-> y): up the gangway, down it, up and down off-centre, and
+    # along the level concourse; no two walkers share a gangway
+    walks = {(1, 1): [(0.3498, 0.50, 0.0, 1.0)] * 2, (-1, 1): [(0.3498, 0.80, 0.0, -1.0)] * 2,
+             (-1, -1): [(0.3402, 0.62, 0.0, 1.0), (0.3594, 0.70, 0.0, -1.0)], (1, -1): [(0.10, 0.912, 1.0, 0.0)] * 2}
+    starts = []
+    for (sx, sy), pair in walks.items():
+        for swap, (lat, dep, dl, dd) in zip((False, True), pair):
+            x, y, dx, dy = (dep, lat, dd, dl) if swap else (lat, dep, dl, dd)
+            starts.append((sx * x, sy * y, sx * dx, sy * dy))
+    # a standing agent just ahead of two of them: the stair time gap in the headway rule
+    n = 64
+    a = dict(x=np.full(n, 1000.0, np.float32), y=np.full(n, 1000.0, np.float32), vx=np.zeros(n, np.float32),
+             vy=np.zeros(n, np.float32), w=np.zeros(n, np.float32), r=np.full(n, 0.0064, np.float32),
+             v0=np.full(n, 0.75, np.float32))
+    dx = np.zeros(n, np.float32)
+    dy = np.zeros(n, np.float32)
+    for i, (x, y, ex, ey) in enumerate(starts):
+        a["x"][i], a["y"][i], a["w"][i], dx[i], dy[i] = x, y, 1.0, ex, ey
+    for j, i in enumerate((0, 2)):
+        a["x"][40 + j] = a["x"][i] + 0.02 * dx[i]
+        a["y"][40 + j] = a["y"][i] + 0.02 * dy[i]
+        a["w"][40 + j] = 1.0
+    sim = sh.Sim(scene, n)
+    sim.upload(a)
+    sim.set_drive(dx, dy)
+    o = oracle_for(scene, n)
+    o.set_state(a)
+    o.set_drive(dx, dy)
+    print(f"venue S: {len(starts)} incline walkers", flush=True)
+    for k in range(1, 61):
+        sim.step(1)
+        o.step(1)
+        compare_step(sim, o, f"venue S incline walkers step {k}")
+    (v_up, v_down), _ = op.incline(np.array([0.5, -0.5], np.float32))
+    st = sim.state()
+    speed = np.hypot(st["vx"], st["vy"])
+    assert abs(speed[1] - v_up) < 1e-3 * v_up   # up the gangway, nobody ahead
+    assert abs(speed[3] - v_down) < 1e-3 * v_down   # down it
+    assert abs(speed[6] - 0.75) < 1e-3 * 0.75   # along the concourse
+
+
+def test_venue_adjoint_is_refused():
+    """The adjoint differentiates a rectangle world's step; a venue plan's incline rules have none, so it refuses."""
+    scene = sh.Scene.venue("S")
+    a = sh.venue_agents("S", 4096, 1.0, 1337)
+    sim = sh.Sim(scene, 4096)
+    sim.upload(a)
+    with pytest.raises(RuntimeError, match="incline"):
+        sim.adjoint_forward(4)
 
 
 def test_venue_m_steps_bit_exact():
