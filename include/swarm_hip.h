@@ -172,9 +172,17 @@ int swarm_venue_probe(const char* name, size_t n, const float* p, const float* e
  * headway rule's time gap by the floor's slope along e_goal (cpu_crowd_sim.cpp:727-733, the agent's v0 in place of
  * V_MAX). Built as swarm_scene_build_cached. */
 int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, int threads, const char* cache_dir, swarm_scene** out);
-/* A scene's venue: has_venue 1 for a venue plan (swarm_scene_build_venue), then layout (may be NULL) holds its layout
- * (the fields up to corner_door_lo, the rest zero); 0 otherwise (layout zeroed). */
+/* A scene's venue: has_venue 1 for a venue plan (swarm_scene_build_venue), 2 for a venue's own 3D step
+ * (swarm_scene_build_seated), 0 otherwise (layout zeroed). layout (may be NULL): the fields up to corner_door_lo, and
+ * for a 3D scene also domain_min, domain_span, exit_depth, exit_max_z, grid_res and exit_count; the rest zero. */
 int swarm_scene_read_venue(const swarm_scene* s, int* has_venue, swarm_venue_layout* layout);
+/* The reference's own 3D step on its seated venue S, M or L (CrowdSim on a seated Venue with defaultModelOptions): the
+ * venue's neighbour grid (gridRes cells over its domain), its navigation field on the heightfield (nav::forVenue: 0.05 m
+ * cells, SeatedGeometry, src/nav_field.cpp:41-101) and candidate table with the exit-type rule (nav::choiceForVenue,
+ * src/exit_choice.cpp:143-333), bit for bit the reference's; the heightfield is evaluated exactly in the step. Agents
+ * carry z (swarm_upload3 / swarm_state3) and the reference's global radius and free speed. Through the scene cache in
+ * cache_dir (NULL or "": none). */
+int swarm_scene_build_seated(const char* name, int threads, const char* cache_dir, swarm_scene** out);
 /* The reference's exit id of each plan exit (ref_exit[cap]; count: the plan's exits) */
 int swarm_venue_exit_map(const char* name, int32_t* ref_exit, int cap, int* count);
 

@@ -11,6 +11,7 @@
 // bilinearly with exact weights, exits as rectangles, a stair speed-factor field, and a configurable
 // projection ring (the reference searches 3x3: its cell is large enough, sim_params.hpp:533).
 #include "crowd_oracle.h"
+#include "venue_oracle.h"
 
 #include <omp.h>
 
@@ -170,7 +171,75 @@ float newtonStep(float t, G3 g, V2 u) {   // cpu_crowd_sim.cpp:180-183
     return du < 0.0f ? t - g.w / du : t;
 }
 
-struct SlotState { float x, y, w, vx, vy, acc; };   // position xy, active flag, velocity xy, projection displacement (velocity.w)
+struct SlotState { float x, y, w, vx, vy, acc, z, vz; };   // position xy, active flag, velocity xy, projection displacement (velocity.w); the seated step's z
+
+// ---- the seated 3D step's constants (sim_params.hpp; folded like the reference) ----
+constexpr float BODY_RADIUS = 0.0064f;
+constexpr float PROJ_MAX_SHIFT = 0x1.4f8b58p-12f;        // PROJ_MAX_SHIFT_RADII * BODY_RADIUS (0.00032)
+constexpr float FEELER_REACH = 0x1.0624dep-7f;           // FEELER_REACH_RADII * BODY_RADIUS (0.008)
+constexpr float PLANE_PROBE_REACH = 0x1.0624dep-6f;      // PLANE_PROBE_RADII * BODY_RADIUS (0.016)
+constexpr float GRAVITY_DT = 0x1.ebb3b0p-1f;             // GRAVITY * DT (0.9603553)
+constexpr float FLOOR_SETTLE_REACH = 0x1.3a92a4p-7f;     // FLOOR_SETTLE_RADII * BODY_RADIUS (0.0096)
+constexpr float STEP_UP_FLOOR_REACH = 0x1.3a92a4p-5f;    // STEP_UP_FLOOR_REACH_RADII * BODY_RADIUS (0.0384)
+constexpr float STEP_UP_TOL = 1e-6f;
+constexpr float TORSO_HEIGHT = 0.05f;
+constexpr float TAU_RELAX_REF = 0x1.d45d2cp-6f, RELAX_STEP_REF = 0x1.68de1cp-1f, OBSTACLE_GAIN_REF = 20.0f;
+constexpr float FENCE_HALF_MID = 0.5f * (0.328395f + 0.335432f);
+constexpr float FENCE_HALF_THK_COLL = 0.5f * (0.335432f - 0.328395f) + BODY_RADIUS;
+constexpr int FENCE_CCD_BISECTIONS = 8, STEP_UP_SLIDE_TRIES = 3;
+const float STEP_UP_SLIDE_DIRS[3][2] = { { 1.0f, 0.0f }, { 0.86602540f, 0.5f }, { 0.5f, 0.86602540f } };
+enum : uint8_t { C_EXIT = 1, C_FENCE_CCD = 2, C_FENCE = 4, C_FLOOR = 8, C_WALL = 16, C_AGENT = 32, C_SWEEP = 64, C_STEP_UP = 128 };
+
+struct V3 { float x, y, z; };
+inline V3 v3(V2 a, float z) { return V3{ a.x, a.y, z }; }
+inline V2 xy(V3 a) { return V2{ a.x, a.y }; }
+inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }   // glm: (x x' + y y') + z z'
+inline float length3(V3 a) { return std::sqrt(dot3(a, a)); }
+
+float sdBox2(V2 p, float h) {   // sdBox2 (src/sdf.cpp:45-48), half size (h, h)
+    const V2 q{ std::abs(p.x) - h, std::abs(p.y) - h };
+    const V2 m{ std::max(q.x, 0.0f), std::max(q.y, 0.0f) };
+    return length(m) + std::min(std::max(q.x, q.y), 0.0f);
+}
+V2 sdBox2Grad(V2 p, float h) {   // src/sdf.cpp:50-56
+    const V2 q{ std::abs(p.x) - h, std::abs(p.y) - h };
+    const V2 sg{ p.x < 0.0f ? -1.0f : 1.0f, p.y < 0.0f ? -1.0f : 1.0f };
+    V2 g;
+    if (std::max(q.x, q.y) > 0.0f) {
+        const V2 m{ std::max(q.x, 0.0f), std::max(q.y, 0.0f) };
+        g = m / length(m);
+    } else {
+        g = q.x > q.y ? V2{ 1.0f, 0.0f } : V2{ 0.0f, 1.0f };
+    }
+    return V2{ sg.x * g.x, sg.y * g.y };
+}
+// the pitch fence (src/sdf.cpp:64-82: the closed ring about the origin, no gates), inflated by BODY_RADIUS
+float sdFence(V3 p) { return std::abs(sdBox2(V2{ p.x - 0.0f, p.y - 0.0f }, FENCE_HALF_MID)) - FENCE_HALF_THK_COLL; }
+V2 sdFenceGradient(V3 p) {
+    const V2 rel{ p.x - 0.0f, p.y - 0.0f };
+    const float box = sdBox2(rel, FENCE_HALF_MID);
+    return (box < 0.0f ? -1.0f : 1.0f) * sdBox2Grad(rel, FENCE_HALF_MID);
+}
+bool fenceCCD(V3 pOld, V3& pNew, V3& vel) {   // cpu_crowd_sim.cpp:94-112
+    const float sOld = sdBox2(xy(pOld), FENCE_HALF_MID), sNew = sdBox2(xy(pNew), FENCE_HALF_MID);
+    if (sOld * sNew >= 0.0f) return false;
+    const float side = sOld > 0.0f ? 1.0f : -1.0f;
+    float lo = 0.0f;
+    if (side * sOld > FENCE_HALF_THK_COLL) {
+        float hi = 1.0f;
+        for (int i = 0; i < FENCE_CCD_BISECTIONS; ++i) {
+            const float t = 0.5f * (lo + hi);
+            if (side * sdBox2(mix2(xy(pOld), xy(pNew), t), FENCE_HALF_MID) > FENCE_HALF_THK_COLL) lo = t;
+            else hi = t;
+        }
+    }
+    const V2 q = mix2(xy(pOld), xy(pNew), lo);
+    pNew.x = q.x; pNew.y = q.y;
+    const V2 n = sdFenceGradient(pNew);
+    const float vn = dot(xy(vel), n);
+    if (vn < 0.0f) { const V2 u = xy(vel) - n * vn; vel.x = u.x; vel.y = u.y; }
+    return true;
+}
 
 } // namespace
 
@@ -184,6 +253,8 @@ struct oracle_ctx {
     int n = 0, threads = 1;
     // agent state, ping-pong in agent order
     std::vector<float> x[2], y[2], vx[2], vy[2], spd[2], w[2];
+    std::vector<float> z[2], vz[2];   // the seated step's heights and vertical velocities
+    std::vector<uint8_t> contacts;    // the seated step's contact bits per agent
     std::vector<float> r, v0;
     int readIdx = 0, writeIdx = 1;
     bool bitonic = false;   // SortMode::GpuBitonic instead of the stable counting sort
@@ -253,7 +324,30 @@ struct oracle_ctx {
                         if (d2 < best) { best = d2; i = j; }
                     }
         }
+        if (sc.venue3d && !surfaceReaches(p, anchorCentre(uint32_t(i)))) {   // the surface rule (exit_choice.cpp:533-546)
+            float best = 1e9f;
+            int pick = i;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int mx = cx + dx, my = cy + dy;
+                    if (mx < 0 || my < 0 || mx >= sc.navRX || my >= sc.navRY) continue;
+                    const int j = mx + my * sc.navRX;
+                    if (j == i || !fit(size_t(j)) || !has(size_t(j))) continue;
+                    const float qx = gx - (float(mx) + 0.5f), qy = gy - (float(my) + 0.5f);
+                    const float d2 = qx * qx + qy * qy;
+                    if (d2 < best && surfaceReaches(p, anchorCentre(uint32_t(j)))) { best = d2; pick = j; }
+                }
+            i = pick;
+        }
         return i;
+    }
+    // surfaceReaches (nav_field.cpp:381-385): the same venue cell, or a move the step-up scan allows
+    bool surfaceReaches(V2 p, V2 c) const {
+        float kc[2], kp[2], n[2];
+        vo::cellKey(sc.venue, c.x, c.y, kc);
+        vo::cellKey(sc.venue, p.x, p.y, kp);
+        if (kc[0] == kp[0] && kc[1] == kp[1]) return true;
+        return vo::stepUpScan(sc.venue, p.x, p.y, c.x, c.y, vo::floorHeight(sc.venue, p.x, p.y), n) >= 1.0f;
     }
     uint32_t slotExit(size_t cell, int k) const { return (navSlot[cell * size_t(sc.K) + size_t(k)] >> CHOICE_EXIT_SHIFT) & 63u; }
     uint32_t slotAnchor(size_t cell, int k) const { return navSlot[cell * size_t(sc.K) + size_t(k)] & CHOICE_ANCHOR_MASK; }
@@ -324,7 +418,22 @@ struct oracle_ctx {
     }
     V2 goalDirection(V2 p, uint32_t agent) const {   // cpu_crowd_sim.cpp:213-218
         if (drive) return V2{ driveX[agent], driveY[agent] };
+        if (sc.venue3d) return seatedChoiceGoal(V3{ p.x, p.y, z[readIdx][agent] }, assign[agent]);
         return choiceGoal(p, r[agent], assign[agent]);
+    }
+    // choiceGoal on the venue (exit_choice.cpp:568-578): its exact wall at the centre within FEELER_REACH
+    V2 seatedChoiceGoal(V3 p, uint32_t exit) const {
+        V2 e = choiceDirection(xy(p), exit);
+        float w[3];
+        vo::wall(sc.venue, p.x, p.y, p.z, FEELER_REACH, w);
+        const V2 n{ w[0], w[1] };
+        const float into = dot(e, n);
+        if (w[2] < FEELER_REACH && into < 0.0f) {
+            e = e - n * into;
+            const float l = length(e);
+            e = l > SAFE_NORMALIZE_EPS ? e / l : V2{ 0.0f, 0.0f };
+        }
+        return e;
     }
     float estimatedTime(float dist, uint32_t exit, uint32_t queued) const {   // exit_choice.cpp:580-584
         const float walk = dist * sc.secondsPerUnit;
@@ -741,6 +850,370 @@ struct oracle_ctx {
         x[wi][self] = np.x; y[wi][self] = np.y; w[wi][self] = 1.0f;
         vx[wi][self] = nv.x; vy[wi][self] = nv.y; spd[wi][self] = length(nv);
     }
+
+    // ================ the seated 3D step (cpu_crowd_sim.cpp, the VENUE_ANALYTIC branches) ================
+    static void setXY(V3& v, V2 a) { v.x = a.x; v.y = a.y; }
+    float seatedMap(V3 p) const { float g[4]; vo::sdfGrad(sc.venue, p.x, p.y, p.z, g); return g[3]; }   // map: seated::sdf
+    // calcGradient (cpu_crowd_sim.cpp:297-303): the tetrahedral stencil k = (1, -1) in 3D
+    V3 seatedGradient(V3 p) const {
+        const float H = OBSTACLE_GRAD_H;
+        const float m1 = seatedMap(V3{ p.x + H, p.y + -H, p.z + -H }), m2 = seatedMap(V3{ p.x + -H, p.y + -H, p.z + H });
+        const float m3 = seatedMap(V3{ p.x + -H, p.y + H, p.z + -H }), m4 = seatedMap(V3{ p.x + H, p.y + H, p.z + H });
+        // kxyy m1 + kyyx m2 + kyxy m3 + kxxx m4, then * OBSTACLE_GRAD_SCALE
+        const V3 g{ ((1.0f * m1 + -1.0f * m2) + -1.0f * m3) + 1.0f * m4, ((-1.0f * m1 + -1.0f * m2) + 1.0f * m3) + 1.0f * m4,
+                    ((-1.0f * m1 + 1.0f * m2) + -1.0f * m3) + 1.0f * m4 };
+        return V3{ g.x * OBSTACLE_GRAD_SCALE, g.y * OBSTACLE_GRAD_SCALE, g.z * OBSTACLE_GRAD_SCALE };
+    }
+    // floorUnderReach's venue branch (cpu_crowd_sim.cpp:265-271)
+    bool floorUnderReach(V3 c, float zFrom, float down, float& zFloor) const {
+        const float h = vo::floorHeight(sc.venue, c.x, c.y);
+        zFloor = h;
+        if (c.z <= h) return h <= std::max(zFrom, c.z) + BODY_RADIUS;
+        return h >= c.z - down;
+    }
+    bool restOnFloor(V3& np, V3& nv, float zFrom) const {   // cpu_crowd_sim.cpp:318-326
+        float zF;
+        if (floorUnderReach(np, zFrom, BODY_RADIUS, zF) && np.z < zF + BODY_RADIUS) { np.z = zF + BODY_RADIUS; nv.z = 0.0f; return true; }
+        return false;
+    }
+    bool settleOnFloor(V3& np, V3& nv, float zFrom) const {   // cpu_crowd_sim.cpp:328-336
+        float zF;
+        if (floorUnderReach(np, zFrom, FLOOR_SETTLE_REACH, zF)) { np.z = zF + BODY_RADIUS; nv.z = 0.0f; return true; }
+        return false;
+    }
+    bool bakedSolid(V3 p) const { return sdFence(p) >= 0.0f && seatedMap(p) <= 0.0f; }   // cpu_crowd_sim.cpp:445
+    bool sweepBack(V3 pOld, V3& np, V3& nv) const {   // cpu_crowd_sim.cpp:447-463
+        if (!bakedSolid(np)) return false;
+        float lo = 0.0f, hi = 1.0f;
+        if (!bakedSolid(V3{ np.x, np.y, pOld.z })) lo = 1.0f;
+        for (int i = 0; i < STATIC_SWEEP_BISECTIONS; ++i) {
+            const float t = 0.5f * (lo + hi);
+            if (bakedSolid(v3(mix2(xy(pOld), xy(np), t), pOld.z))) hi = t;
+            else lo = t;
+        }
+        const V2 d = safeNormalize(xy(np) - xy(pOld));
+        np = v3(mix2(xy(pOld), xy(np), lo), pOld.z);
+        const float vd = dot(xy(nv), d);
+        if (vd > 0.0f) setXY(nv, xy(nv) - d * vd);
+        nv.z = 0.0f;
+        return true;
+    }
+    // clearWalls' venue branch (cpu_crowd_sim.cpp:367-381): the exact wall at the body's top
+    bool clearWalls(V3& np, V3& nv, V3& wall) const {
+        bool moved = false;
+        wall = V3{ 0.0f, 0.0f, 0.0f };
+        const float above = vo::wallAbove(sc.venue), clear = vo::wallClear(sc.venue);
+        for (int pass = 0; pass < WALL_PASSES; ++pass) {
+            float w[3];
+            vo::wall(sc.venue, np.x, np.y, np.z + above, FEELER_REACH, w);
+            const V2 n{ w[0], w[1] };
+            if (w[2] >= clear) {
+                if (w[2] < WALL_NONE) wall = v3(n, w[2] - clear);
+                break;
+            }
+            setXY(np, xy(np) + n * (clear - w[2]));
+            moved = true;
+            wall = v3(n, 0.0f);
+            const float vDotN = dot(xy(nv), n);
+            if (vDotN < 0.0f) setXY(nv, xy(nv) - n * vDotN);
+        }
+        return moved;
+    }
+    V3 wallPlaneProbe(V3 np) const {   // cpu_crowd_sim.cpp:420-424
+        float w[3];
+        vo::wall(sc.venue, np.x, np.y, np.z + vo::wallAbove(sc.venue), PLANE_PROBE_REACH, w);
+        return w[2] >= 0.0f && w[2] < WALL_NONE ? V3{ w[0], w[1], std::max(w[2] - vo::wallClear(sc.venue), 0.0f) } : V3{ 0.0f, 0.0f, 0.0f };
+    }
+    // resolveStatic (cpu_crowd_sim.cpp:467-500)
+    bool resolveStatic3(V3 pOld, V3& np, V3& nv, bool settle, V3& wall, uint8_t& contact) const {
+        if (fenceCCD(pOld, np, nv)) contact |= C_FENCE_CCD;
+        const float dFence = sdFence(np);
+        if (dFence < 0.0f) {
+            const V2 n = sdFenceGradient(np);
+            setXY(np, xy(np) - n * dFence);
+            const float vDotN = dot(xy(nv), n);
+            if (vDotN < 0.0f) setXY(nv, xy(nv) - n * vDotN);
+            contact |= C_FENCE;
+        }
+        bool grounded = settle ? settleOnFloor(np, nv, pOld.z) : restOnFloor(np, nv, pOld.z);
+        if (!grounded && sweepBack(pOld, np, nv)) {
+            contact |= C_SWEEP;
+            grounded = restOnFloor(np, nv, pOld.z);
+        }
+        if (grounded) contact |= C_FLOOR;
+        const bool walls = clearWalls(np, nv, wall);
+        if (walls) {
+            contact |= C_WALL;
+            if (grounded) {
+                if (settleOnFloor(np, nv, pOld.z)) contact |= C_FLOOR;
+                V3 w2;
+                const bool again = clearWalls(np, nv, w2);
+                if (w2.x != 0.0f || w2.y != 0.0f) wall = w2;
+                if (again && restOnFloor(np, nv, pOld.z)) contact |= C_FLOOR;
+            } else if (restOnFloor(np, nv, pOld.z)) {
+                contact |= C_FLOOR;
+            }
+        }
+        return grounded;
+    }
+    float scan(V2 a, V2 b, float f0, V2& n) const {
+        float nn[2];
+        const float t = vo::stepUpScan(sc.venue, a.x, a.y, b.x, b.y, f0, nn);
+        n = V2{ nn[0], nn[1] };
+        return t;
+    }
+    // venueBlockedWall (cpu_crowd_sim.cpp:591-605)
+    void venueBlockedWall(float f0, V3 pOld, V3& np, V3& nv) const {
+        float w[3];
+        vo::wall(sc.venue, np.x, np.y, np.z + vo::wallAbove(sc.venue), FEELER_REACH, w);
+        const float clear = vo::wallClear(sc.venue);
+        if (w[2] >= clear) return;
+        const V2 n{ w[0], w[1] };
+        const V2 pushed = xy(np) + n * (clear - w[2]);
+        V2 nn;
+        if (scan(xy(np), pushed, f0, nn) < 1.0f) return;
+        const float vDotN = dot(xy(nv), n);
+        if (vDotN < 0.0f) setXY(nv, xy(nv) - n * vDotN);
+        float zF;
+        if (floorUnderReach(V3{ pushed.x, pushed.y, pOld.z }, pOld.z, FLOOR_SETTLE_REACH, zF)) {
+            np = v3(pushed, zF + BODY_RADIUS);
+            nv.z = 0.0f;
+        } else {
+            setXY(np, pushed);
+        }
+    }
+    // venueEdgeMargin (cpu_crowd_sim.cpp:610-636)
+    void venueEdgeMargin(V3 pOld, float f0, V3& np, V3& nv) const {
+        const float margin = vo::faceMargin(sc.venue);
+        float feet = np.z - BODY_RADIUS;
+        const float fl = vo::floorHeight(sc.venue, np.x, np.y);
+        float w[3];
+        if (feet > fl + margin) {
+            vo::wall(sc.venue, np.x, np.y, fl + STEP_UP_TOL, margin, w);
+            if (w[2] < margin) {
+                const V2 into = xy(np) - V2{ w[0], w[1] } * (w[2] + margin);
+                const float top = vo::floorHeight(sc.venue, into.x, into.y);
+                if (std::abs(top - feet) <= margin && top <= f0 + STEP_UP_TOL) { setXY(np, into); return; }
+            }
+            if (np.z <= pOld.z) {
+                vo::wall(sc.venue, np.x, np.y, feet + STEP_UP_TOL, margin, w);
+                if (w[2] < margin) setXY(np, xy(np) + V2{ w[0], w[1] } * (margin - w[2]));
+                return;
+            }
+            float zF;
+            if (!floorUnderReach(V3{ np.x, np.y, pOld.z }, pOld.z, FLOOR_SETTLE_REACH, zF)) { np.z = pOld.z; return; }
+            np.z = zF + BODY_RADIUS;
+            nv.z = 0.0f;
+            feet = zF;
+        }
+        vo::wall(sc.venue, np.x, np.y, feet + STEP_UP_TOL, margin, w);
+        if (w[2] >= margin) return;
+        setXY(np, xy(np) + V2{ w[0], w[1] } * (margin - w[2]));
+        np.z = vo::floorHeight(sc.venue, np.x, np.y) + BODY_RADIUS;
+    }
+    // limitStepUp with the venue's scan (cpu_crowd_sim.cpp:555-587)
+    bool limitStepUp(V3 pOld, V3& np, V3& nv) const {
+        float f0;
+        if (!floorUnderReach(pOld, pOld.z, STEP_UP_FLOOR_REACH, f0)) f0 = pOld.z - STEP_UP_FLOOR_REACH;
+        V2 n;
+        const float t = scan(xy(pOld), xy(np), f0, n);
+        if (t >= 1.0f) {
+            venueEdgeMargin(pOld, f0, np, nv);
+            return false;
+        }
+        V2 q = mix2(xy(pOld), xy(np), t);
+        const V2 rest = xy(np) - q;
+        const V2 tangent = rest - n * dot(rest, n);
+        const float lt = length(tangent);
+        if (lt > 0.0f) {
+            for (int k = 0; k < STEP_UP_SLIDE_TRIES; ++k) {
+                const V2 slide = (q + tangent * STEP_UP_SLIDE_DIRS[k][0]) + n * (lt * STEP_UP_SLIDE_DIRS[k][1]);
+                V2 n2;
+                if (sdFence(V3{ slide.x, slide.y, 0.0f }) >= 0.0f && scan(q, slide, f0, n2) >= 1.0f) { q = slide; break; }
+            }
+        }
+        const float vn = dot(xy(nv), n);
+        if (vn < 0.0f) setXY(nv, xy(nv) - n * vn);
+        float zF;
+        if (floorUnderReach(V3{ q.x, q.y, pOld.z }, pOld.z, FLOOR_SETTLE_REACH, zF)) {
+            np = v3(q, zF + BODY_RADIUS);
+            nv.z = 0.0f;
+        } else {
+            np = v3(q, std::min(np.z, pOld.z));
+        }
+        venueBlockedWall(f0, pOld, np, nv);
+        venueEdgeMargin(pOld, f0, np, nv);
+        return true;
+    }
+    // separationForce / contactCorrection over 3D offsets (cpu_crowd_sim.cpp:131-148): the reference's global radius
+    static V2 separation3(V3 offset, uint32_t self, uint32_t other) {
+        const float d2 = dot3(offset, offset);
+        if (d2 >= R_SEP2) return V2{ 0.0f, 0.0f };
+        const float w = 1.0f - std::sqrt(d2) / R_SEP;
+        const float magnitude = K_SEP * w * w;
+        const float dxy2 = dot(xy(offset), xy(offset));
+        if (dxy2 <= SEP_STACKED_XY2) return magnitude * pairDirection(self, other);
+        return magnitude * xy(offset) * (1.0f / std::sqrt(dxy2));
+    }
+    static V2 contact3(V3 offset, uint32_t self, uint32_t other) {
+        const float dxy2 = dot(xy(offset), xy(offset));
+        if (std::abs(offset.z) >= BODY_DIAM || dxy2 >= BODY_DIAM2) return V2{ 0.0f, 0.0f };
+        const float dxy = std::sqrt(dxy2);
+        const float shift = 0.5f * (BODY_DIAM - dxy);
+        if (dxy2 <= SEP_STACKED_XY2) return shift * pairDirection(self, other);
+        return shift * (xy(offset) / dxy);
+    }
+    // PHYSICS_PASS 1 on the venue (cpu_crowd_sim.cpp:655-777)
+    void predict3(uint32_t slot) {
+        const uint32_t self = id[slot];
+        const uint32_t cid = key[slot];
+        uint8_t& contact = contacts[self];
+        contact = 0;
+        if (cid == INACTIVE_CELL) return;
+        const int ri = readIdx;
+        const V3 p{ x[ri][self], y[ri][self], z[ri][self] };
+        const V3 v{ vx[ri][self], vy[ri][self], vz[ri][self] };
+        const float m = std::max(std::abs(p.x), std::abs(p.y));
+        if ((m > sc.exitDepth && p.z < sc.exitMaxZ) || m > sc.domainMax) {
+            slotA[slot] = SlotState{ PARK_COORD, PARK_COORD, 0.0f, 0.0f, 0.0f, 0.0f, PARK_COORD, 0.0f };
+            contact = C_EXIT;
+            return;
+        }
+        const V2 eGoal{ egX[self], egY[self] };
+        const int cx = int(cid % uint32_t(sc.hashNX)), cy = int(cid / uint32_t(sc.hashNX));
+        V2 sep{ 0.0f, 0.0f };
+        float sAhead = 1e9f;
+        for (int yy = -sc.percRing; yy <= sc.percRing; yy++)
+            for (int xx = -sc.percRing; xx <= sc.percRing; xx++) {
+                const int nx = cx + xx, ny = cy + yy;
+                if (nx < 0 || nx >= sc.hashNX || ny < 0 || ny >= sc.hashNY) continue;
+                const size_t c = size_t(nx) + size_t(ny) * size_t(sc.hashNX);
+                for (uint32_t k = cellStart[c]; k < cellStart[c] + cellCount[c]; k++) {
+                    if (k == slot) continue;
+                    const uint32_t other = id[k];
+                    const V3 op{ x[ri][other], y[ri][other], z[ri][other] };
+                    const V3 offset{ p.x - op.x, p.y - op.y, p.z - op.z };
+                    const float dxy2 = dot(xy(offset), xy(offset));
+                    if (dxy2 < R_PERC2) {
+                        sep = sep + separation3(offset, self, other);
+                        const float ahead = -dot(xy(offset), eGoal);
+                        const float lateral = std::abs(offset.x * eGoal.y - offset.y * eGoal.x);
+                        const float dxy = std::sqrt(dxy2);
+                        if (ahead > 0.0f && lateral < BODY_DIAM && dxy < sAhead) {
+                            const V2 eOther{ egX[other], egY[other] };
+                            const float aheadOther = dot(xy(offset), eOther);
+                            const bool mutual = aheadOther > 0.0f && std::abs(offset.x * eOther.y - offset.y * eOther.x) < BODY_DIAM;
+                            if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = dxy;
+                        }
+                    }
+                }
+            }
+        const float slope = vo::floorSlope(sc.venue, p.x, p.y, eGoal.x, eGoal.y);
+        const float vFree = inclineFreeSpeed(slope);
+        const float vDes = (slope != 0.0f ? std::clamp((sAhead - BODY_DIAM) / inclineHeadwayTime(slope), 0.0f, vFree)
+                                          : std::clamp((sAhead - BODY_DIAM) * HEADWAY_RATE, 0.0f, vFree)) * length(eGoal);
+        const V2 eDir = safeNormalize(eGoal + sep);
+        // obstacle avoidance at torso height: the venue or the fence, the nearer (sceneDist / sceneNormal, :307-316)
+        V2 aObs{ 0.0f, 0.0f };
+        const V3 torso{ p.x, p.y, p.z + TORSO_HEIGHT };
+        const float dStadium = seatedMap(torso);
+        const float dFence = sdFence(torso) + BODY_RADIUS;
+        const bool onFence = dFence < dStadium;
+        const float dTorso = onFence ? dFence : dStadium;
+        if (dTorso < OBSTACLE_RANGE) {
+            V2 norm;
+            if (onFence) norm = sdFenceGradient(torso);
+            else {
+                const V3 g = seatedGradient(torso);
+                const float l = length3(g);
+                norm = l > SAFE_NORMALIZE_EPS ? V2{ g.x / l, g.y / l } : V2{ 0.0f, 0.0f };
+            }
+            if (length(norm) > OBSTACLE_MIN_NORMAL) aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * OBSTACLE_GAIN_REF;
+        }
+        const V2 vEq = eDir * vDes + aObs * TAU_RELAX_REF;
+        V3 nv = v;
+        setXY(nv, vEq + (xy(nv) - vEq) * RELAX_STEP_REF);
+        const float along = dot(xy(nv), eGoal);
+        if (along > vDes) setXY(nv, xy(nv) - eGoal * (along - vDes));
+        nv.z -= GRAVITY_DT;
+        nv.z = std::min(nv.z, 0.0f);
+        if (length(xy(nv)) > V_MAX) setXY(nv, safeNormalize(xy(nv)) * V_MAX);
+        V3 np{ p.x + nv.x * DT, p.y + nv.y * DT, p.z + nv.z * DT };
+        V3 wall;
+        const bool grounded = resolveStatic3(p, np, nv, false, wall, contact);
+        if (wall.x == 0.0f && wall.y == 0.0f) wall = wallPlaneProbe(np);
+        slotA[slot] = SlotState{ np.x, np.y, 1.0f, nv.x, nv.y, 0.0f, np.z, nv.z };
+        const V2 wn{ wall.x, wall.y };
+        float* pl = &planes[size_t(slot) * 4];
+        pl[0] = wn.x; pl[1] = wn.y; pl[2] = dot(xy(np), wn) - wall.z;
+        pl[3] = (nonzero(wn) ? PLANE_WALL : 0.0f) + (grounded ? PLANE_GROUNDED : 0.0f);
+    }
+    // PHYSICS_PASS 2 on the venue (cpu_crowd_sim.cpp:780-820)
+    SlotState projectSlot3(const std::vector<SlotState>& in, uint32_t slot, uint32_t self, uint32_t cid) {
+        SlotState b = in[slot];
+        if (b.w == 0.0f) return b;
+        const int cx = int(cid % uint32_t(sc.hashNX)), cy = int(cid / uint32_t(sc.hashNX));
+        V2 corr{ 0.0f, 0.0f };
+        int nContacts = 0;
+        for (int yy = -sc.projRing; yy <= sc.projRing; yy++)
+            for (int xx = -sc.projRing; xx <= sc.projRing; xx++) {
+                const int nx = cx + xx, ny = cy + yy;
+                if (nx < 0 || nx >= sc.hashNX || ny < 0 || ny >= sc.hashNY) continue;
+                const size_t c = size_t(nx) + size_t(ny) * size_t(sc.hashNX);
+                for (uint32_t k = cellStart[c]; k < cellStart[c] + cellCount[c]; k++) {
+                    if (k == slot) continue;
+                    const SlotState& o = in[k];
+                    const V3 offset{ b.x - o.x, b.y - o.y, b.z - o.z };
+                    if (o.w != 0.0f && std::abs(offset.z) < BODY_DIAM && dot(xy(offset), xy(offset)) < BODY_DIAM2) {
+                        corr = corr + contact3(offset, self, id[k]);
+                        ++nContacts;
+                    }
+                }
+            }
+        corr = corr * (PROJ_RELAX / float(std::max(nContacts, 1)));
+        const float len = length(corr);
+        if (len > PROJ_MAX_SHIFT) corr = corr * (PROJ_MAX_SHIFT / len);
+        V3 q{ b.x, b.y, b.z };
+        setXY(q, xy(q) + corr);
+        const float dFence = sdFence(q);
+        if (dFence < 0.0f) setXY(q, xy(q) - sdFenceGradient(q) * dFence);
+        const float* pl = &planes[size_t(slot) * 4];
+        if (std::fmod(pl[3], 2.0f) == PLANE_WALL) {
+            const float sgap = pl[2] - dot(xy(q), V2{ pl[0], pl[1] });
+            if (sgap > 0.0f) setXY(q, xy(q) + V2{ pl[0], pl[1] } * sgap);
+        }
+        const V2 d{ q.x - b.x, q.y - b.y };
+        if (nonzero(d)) contacts[self] |= C_AGENT;
+        b.x = q.x; b.y = q.y;
+        b.acc += length(d);
+        return b;
+    }
+    // PHYSICS_PASS 3 on the venue (cpu_crowd_sim.cpp:835-866)
+    void finalPass3(const std::vector<SlotState>& in, uint32_t slot, bool projectHere) {
+        const uint32_t self = id[slot];
+        const uint32_t cid = key[slot];
+        const int wi = writeIdx, ri0 = readIdx;
+        if (cid == INACTIVE_CELL) {
+            x[wi][self] = x[ri0][self]; y[wi][self] = y[ri0][self]; z[wi][self] = z[ri0][self]; w[wi][self] = w[ri0][self];
+            vx[wi][self] = vx[ri0][self]; vy[wi][self] = vy[ri0][self]; vz[wi][self] = vz[ri0][self]; spd[wi][self] = spd[ri0][self];
+            return;
+        }
+        const SlotState b = projectHere ? projectSlot3(in, slot, self, cid) : in[slot];
+        if (b.w == 0.0f) {
+            x[wi][self] = b.x; y[wi][self] = b.y; z[wi][self] = b.z; w[wi][self] = 0.0f;
+            vx[wi][self] = b.vx; vy[wi][self] = b.vy; vz[wi][self] = b.vz; spd[wi][self] = b.acc;
+            return;
+        }
+        V3 np{ b.x, b.y, b.z }, nv{ b.vx, b.vy, b.vz };
+        const V3 pOld{ x[ri0][self], y[ri0][self], z[ri0][self] };
+        if (b.acc > 0.0f) {
+            V3 wall;
+            resolveStatic3(pOld, np, nv, planes[size_t(slot) * 4 + 3] >= PLANE_GROUNDED, wall, contacts[self]);
+        }
+        if (limitStepUp(pOld, np, nv)) contacts[self] |= C_STEP_UP;
+        x[wi][self] = np.x; y[wi][self] = np.y; z[wi][self] = np.z; w[wi][self] = 1.0f;
+        vx[wi][self] = nv.x; vy[wi][self] = nv.y; vz[wi][self] = nv.z; spd[wi][self] = length3(nv);
+    }
 };
 
 namespace {
@@ -816,7 +1289,9 @@ oracle_ctx* oracle_create(const oracle_scene* s, const oracle_params* p, int n) 
     for (int h = 0; h < 2; ++h) {
         c->x[h].assign(size_t(n), 0.0f); c->y[h].assign(size_t(n), 0.0f); c->vx[h].assign(size_t(n), 0.0f);
         c->vy[h].assign(size_t(n), 0.0f); c->spd[h].assign(size_t(n), 0.0f); c->w[h].assign(size_t(n), 0.0f);
+        c->z[h].assign(size_t(n), 0.0f); c->vz[h].assign(size_t(n), 0.0f);
     }
+    c->contacts.assign(size_t(n), 0);
     c->r.assign(size_t(n), 0.0f); c->v0.assign(size_t(n), 0.0f);
     const size_t cells = size_t(s->hashNX) * s->hashNY;
     c->key.assign(size_t(n), 0u); c->id.assign(size_t(n), 0u); c->keyTmp.assign(size_t(n), 0u); c->idTmp.assign(size_t(n), 0u);
@@ -924,8 +1399,9 @@ void oracle_step(oracle_ctx* c, int nsteps) {
             float* py = c->y[c->readIdx].data();
             float* pw = c->w[c->readIdx].data();
             const oracle_scene& sc = c->sc;
+            float* pz = c->z[c->readIdx].data();
             parallelFor(T, n, false, [&](int i) {
-                if (std::isnan(px[i])) { px[i] = 0.0f; py[i] = 0.0f; pw[i] = 1.0f; }
+                if (std::isnan(px[i])) { px[i] = 0.0f; py[i] = 0.0f; pz[i] = 0.0f; pw[i] = 1.0f; }
                 uint32_t k = INACTIVE_CELL;
                 if (pw[i] != 0.0f) {
                     const int gx = std::clamp(int((px[i] - sc.xmin) / sc.hashSpanX * float(sc.hashNX)), 0, sc.hashNX - 1);
@@ -981,20 +1457,29 @@ void oracle_step(oracle_ctx* c, int nsteps) {
             });
         }
         const auto t4 = Clock::now();
-        parallelFor(T, n, true, [&](int slot) { c->predict(uint32_t(slot)); });
+        const bool seated = c->sc.venue3d != 0;
+        parallelFor(T, n, true, [&](int slot) {
+            if (seated) c->predict3(uint32_t(slot));
+            else c->predict(uint32_t(slot));
+        });
         const auto t5 = Clock::now();
         std::vector<SlotState>* in = &c->slotA;
         std::vector<SlotState>* out = &c->slotB;
         for (int it = 1; it < c->par.projIters; ++it) {
             parallelFor(T, n, true, [&](int slot) {
                 const uint32_t cid = c->key[size_t(slot)];
-                if (cid != INACTIVE_CELL) (*out)[size_t(slot)] = c->projectSlot(*in, uint32_t(slot), c->id[size_t(slot)], cid);
+                if (cid == INACTIVE_CELL) return;
+                (*out)[size_t(slot)] = seated ? c->projectSlot3(*in, uint32_t(slot), c->id[size_t(slot)], cid)
+                                              : c->projectSlot(*in, uint32_t(slot), c->id[size_t(slot)], cid);
             });
             std::swap(in, out);
         }
         const auto t6 = Clock::now();
         const bool projectInFinal = c->par.projIters > 0;
-        parallelFor(T, n, true, [&](int slot) { c->finalPass(*in, uint32_t(slot), projectInFinal); });
+        parallelFor(T, n, true, [&](int slot) {
+            if (seated) c->finalPass3(*in, uint32_t(slot), projectInFinal);
+            else c->finalPass(*in, uint32_t(slot), projectInFinal);
+        });
         // evacuation events: active at the step's input, parked at its output
         {
             const float* wIn = c->w[c->readIdx].data();
@@ -1004,7 +1489,9 @@ void oracle_step(oracle_ctx* c, int nsteps) {
             for (int i = 0; i < n; ++i)
                 if (wIn[i] != 0.0f && wOut[i] == 0.0f) {
                     c->evacStep[size_t(i)] = c->steps;
-                    const int e = c->exitOf(V2{ px[i], py[i] });
+                    // the exit: the rectangle it entered; on the venue its opening by the last active position
+                    // (metrics_deposit.comp exitSlot, seated::exitByLat)
+                    const int e = seated ? vo::exitByLat(c->sc.venue, px[i], py[i]) : c->exitOf(V2{ px[i], py[i] });
                     c->evacExit[size_t(i)] = e >= 0 ? e : c->sc.E;
                 }
         }
@@ -1016,6 +1503,35 @@ void oracle_step(oracle_ctx* c, int nsteps) {
         c->times[6] = msBetween(t6, t7); c->times[7] = msBetween(t0, t7);
     }
 }
+
+void oracle_set_state3(oracle_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                       const float* w) {
+    const size_t n = size_t(c->n);
+    std::vector<float> r(n, BODY_RADIUS), v0(n, V_MAX);
+    oracle_set_state(c, x, y, vx, vy, w, r.data(), v0.data());
+    for (int h = 0; h < 2; ++h) {
+        c->z[h].assign(z, z + n);
+        c->vz[h].assign(vz, vz + n);
+        for (size_t i = 0; i < n; ++i) c->spd[h][i] = std::sqrt((vx[i] * vx[i] + vy[i] * vy[i]) + vz[i] * vz[i]);
+    }
+}
+
+void oracle_get_state3(const oracle_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w) {
+    const int h = c->readIdx;
+    std::copy(c->x[h].begin(), c->x[h].end(), x); std::copy(c->y[h].begin(), c->y[h].end(), y);
+    std::copy(c->z[h].begin(), c->z[h].end(), z); std::copy(c->vx[h].begin(), c->vx[h].end(), vx);
+    std::copy(c->vy[h].begin(), c->vy[h].end(), vy); std::copy(c->vz[h].begin(), c->vz[h].end(), vz);
+    std::copy(c->spd[h].begin(), c->spd[h].end(), speed); std::copy(c->w[h].begin(), c->w[h].end(), w);
+}
+
+uint64_t oracle_fnv64(const void* data, uint64_t bytes) {   // FNV-1a 64 (the reference's cache checksum)
+    const unsigned char* b = static_cast<const unsigned char*>(data);
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (uint64_t i = 0; i < bytes; ++i) { h ^= b[i]; h *= 0x100000001b3ull; }
+    return h;
+}
+
+void oracle_get_contacts(const oracle_ctx* c, uint8_t* contacts) { std::copy(c->contacts.begin(), c->contacts.end(), contacts); }
 
 void oracle_get_state(const oracle_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w) {
     const int h = c->readIdx;

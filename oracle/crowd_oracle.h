@@ -37,6 +37,8 @@ typedef struct {
     int nWalls;                            /* wall boxes (e_goal's wall rule queries them exactly) */
     const float* wallRects;                /* nWalls x (lox, loy, hix, hiy) */
     const void* venue;                     /* a seated venue's plan: its layout (oracle_venue_new), else NULL */
+    int venue3d;                           /* 1: the venue's own 3D step (below); needs venue */
+    float exitDepth, exitMaxZ, domainMax;  /* its exit test: max(|x|, |y|) > exitDepth below exitMaxZ, or > domainMax */
 } oracle_scene;
 
 typedef struct {
@@ -57,6 +59,20 @@ void oracle_venue_free(void* venue);
 float oracle_venue_slope(const void* venue, float x, float y, float ex, float ey);
 /* seated::inclineFreeSpeed and inclineHeadwayTime at EMPIRICAL's time gaps (src/venue.cpp:652-670) at n slopes */
 void oracle_incline(int n, const float* g, float* v_free, float* t_headway);
+
+/* The seated 3D step (venue3d): the reference's CpuCrowdSim on an analytic venue (src/cpu_crowd_sim.cpp, the seated
+ * branches: the exit test, e_goal with the surface rule and the wall term, separation and the headway rule over 3D
+ * offsets, the incline rules, obstacle avoidance at torso height against the venue and the pitch fence, gravity, the static
+ * contacts (fence CCD, floor rest and settle, sweep-back, the exact walls), the wall plane, the projection with the fence
+ * and the height test, the final contacts and the step-up limit with the venue's face margin). Global BODY_RADIUS and
+ * V_MAX (r and v0 are not read). Positions and velocities carry z. */
+void oracle_set_state3(oracle_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                       const float* w);
+void oracle_get_state3(const oracle_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w);
+/* FNV-1a 64 of a byte range (the checksum the reference's caches use) */
+uint64_t oracle_fnv64(const void* data, uint64_t bytes);
+/* the contact bits of the last step per agent (CpuCrowdSim::ContactBits) */
+void oracle_get_contacts(const oracle_ctx* c, uint8_t* contacts);
 
 oracle_ctx* oracle_create(const oracle_scene* scene, const oracle_params* params, int n);
 void oracle_destroy(oracle_ctx* c);

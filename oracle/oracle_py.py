@@ -27,7 +27,8 @@ class OracleScene(C.Structure):
                 ("navCell", C.c_float), ("navInvCell", C.c_float), ("K", C.c_int), ("E", C.c_int),
                 ("navInfo", C.c_void_p), ("navSlot", C.c_void_p), ("navDist", C.c_void_p), ("invCapS", C.c_void_p),
                 ("secondsPerUnit", C.c_float), ("distQuantum", C.c_float), ("stair", C.c_void_p), ("exitRects", C.c_void_p),
-                ("nWalls", C.c_int), ("wallRects", C.c_void_p), ("venue", C.c_void_p)]
+                ("nWalls", C.c_int), ("wallRects", C.c_void_p), ("venue", C.c_void_p), ("venue3d", C.c_int),
+                ("exitDepth", C.c_float), ("exitMaxZ", C.c_float), ("domainMax", C.c_float)]
 
 
 class OracleParams(C.Structure):
@@ -70,6 +71,11 @@ def lib():
         L.oracle_venue_slope.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
         L.oracle_venue_slope.restype = C.c_float
         L.oracle_incline.argtypes = [C.c_int, vp, vp, vp]
+        L.oracle_set_state3.argtypes = [vp] + [_f32p] * 7
+        L.oracle_get_state3.argtypes = [vp] + [_f32p] * 8
+        L.oracle_get_contacts.argtypes = [vp, vp]
+        L.oracle_fnv64.argtypes = [vp, C.c_uint64]
+        L.oracle_fnv64.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -377,6 +383,63 @@ class MetricsOracle:
                                                     self.evac_step, self.cells.reshape(-1))
 
 
+class SeatedOracle(Oracle):
+    """The oracle's seated 3D step on an analytic venue: venue = its layout arrays (venue_layout_arrays), grid = (domain_min,
+    domain_span, grid_res), exit = (exit_depth, exit_max_z, domain_max); nav = (info, tables) of the venue's navigation
+    scene for the exit choice, or None (forced drive only)."""
+
+    def __init__(self, venue, grid, exit, n, nav=None, threads=0, proj_iters=32):
+        self._tables = None
+        self.h = None
+        self._venue_arrays = [np.ascontiguousarray(a) for a in venue]
+        self._venue = lib().oracle_venue_new(*[a.ctypes.data for a in self._venue_arrays])
+        dmin, span, res = grid
+        if nav is not None:
+            info, tables = nav
+            self._tables = tables
+            sc = scene_struct(info, tables)
+        else:
+            sc = OracleScene()
+            self._dummy = np.zeros(4, np.float32)
+            sc.sdfNX = sc.sdfNY = 2
+            sc.sdf = self._dummy.ctypes.data
+            sc.sdfInvH = 1.0
+            sc.navRX = sc.navRY = 0
+        sc.xmin = sc.ymin = dmin
+        sc.xmax = sc.ymax = dmin + span
+        sc.hashSpanX = sc.hashSpanY = span
+        sc.hashNX = sc.hashNY = res
+        sc.percRing = sc.projRing = 1
+        sc.venue = self._venue
+        sc.venue3d = 1
+        sc.exitDepth, sc.exitMaxZ, sc.domainMax = exit
+        p = OracleParams(0.0, 0.0, 20.0, proj_iters, threads)
+        self.h = lib().oracle_create(C.byref(sc), C.byref(p), n)
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+        self.n = n
+        self.cells = res * res
+
+    def set_state3(self, pos, vel):
+        """pos, vel: (n, 4) as GpuBoid's position (xyz, w) and velocity (xyz, speed)."""
+        pos = np.asarray(pos, np.float32)
+        vel = np.asarray(vel, np.float32)
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        lib().oracle_set_state3(self.h, f(pos[:, 0]), f(pos[:, 1]), f(pos[:, 2]), f(vel[:, 0]), f(vel[:, 1]), f(vel[:, 2]), f(pos[:, 3]))
+
+    def state3(self):
+        """(pos, vel) as (n, 4) arrays like GpuBoid's."""
+        a = [np.zeros(self.n, np.float32) for _ in range(8)]
+        lib().oracle_get_state3(self.h, *a)
+        x, y, z, vx, vy, vz, sp, w = a
+        return np.stack([x, y, z, w], 1), np.stack([vx, vy, vz, sp], 1)
+
+    def contacts(self):
+        out = np.zeros(self.n, np.uint8)
+        lib().oracle_get_contacts(self.h, out.ctypes.data)
+        return out
+
+
 # --- the analytic seated venue (venue_oracle.cpp) ---
 VENUE_REACHES = (0.008, 0.016, 0.02)   # FEELER_REACH, PLANE_PROBE_REACH (model units) and a wide reach
 
@@ -402,6 +465,12 @@ def venue_layout_arrays_of(lay):
     return (f([getattr(lay, k) for k in _LAYOUT_F]), i([lay.rows, lay.max_band_steps, lay.gangways, lay.doors, 0]),
             f(list(lay.gw_lo)), f(list(lay.gw_hi)), i(list(lay.gw_row)), i(list(lay.gw_seated)), f(list(lay.door_lo)),
             f(list(lay.door_hi)))
+
+
+def fnv64(a):
+    """FNV-1a 64 of an array's bytes."""
+    a = np.ascontiguousarray(a)
+    return int(lib().oracle_fnv64(a.ctypes.data, a.nbytes))
 
 
 def incline(g):

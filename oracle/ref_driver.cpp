@@ -651,6 +651,130 @@ void venueCases() {
     }
 }
 
+uint64_t fnv64(const void* data, size_t bytes, uint64_t h = 0xcbf29ce484222325ull) {   // FNV-1a 64
+    const unsigned char* b = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < bytes; ++i) { h ^= b[i]; h *= 0x100000001b3ull; }
+    return h;
+}
+
+void putBoids(const std::string& p, const std::vector<GpuBoid>& st) {
+    std::vector<float> pos, vel;
+    for (const GpuBoid& b : st) {
+        pos.insert(pos.end(), { b.position.x, b.position.y, b.position.z, b.position.w });
+        vel.insert(vel.end(), { b.velocity.x, b.velocity.y, b.velocity.z, b.velocity.w });
+    }
+    put(p + "pos", 'f', pos, { uint64_t(st.size()), 4 });
+    put(p + "vel", 'f', vel, { uint64_t(st.size()), 4 });
+}
+
+// The reference's seated 3D step (CpuCrowdSim on an analytic venue: floors, gravity, the fence, the exact walls, the
+// step-up limit) on venue S, its 100 % seat spawn in 4,096 slots (3,728 seated), counting sort:
+//   seated/S/drive/    a forced drive per agent (setDriveTable: seeded random directions), 40 steps
+//   seated/S/default/  the default model (its navigation field, queue-aware choice, incline speed, EMPIRICAL stairs), 200 steps
+// States (position, velocity) and contacts at chosen steps; the default run's choice state; its navigation field and
+// choice table as FNV-1a checksums with every 997th cell written out
+void seatedCases() {
+    const Venue& venue = venueByName("S");
+    const int n = 4096;
+    int active = 0;
+    const std::vector<GpuBoid> init = spawnSeatedCrowd(venue, n, 1.0, 1337u, &active);
+    putBoids("seated/S/init/", init);
+    {
+        CpuCrowdSim sim(venue, std::vector<float>{}, 0, CpuCrowdSim::SortMode::Counting);
+        std::vector<glm::vec2> drive(static_cast<size_t>(n));
+        Rng rng{ 0xD21BEull };
+        for (int i = 0; i < n; ++i) {
+            const float a = 6.2831853f * rng.uniform();
+            drive[size_t(i)] = glm::vec2(std::cos(a), std::sin(a));
+        }
+        std::vector<float> dv;
+        for (const glm::vec2& d : drive) dv.insert(dv.end(), { d.x, d.y });
+        put("seated/S/drive/dir", 'f', dv, { uint64_t(n), 2 });
+        sim.setDriveTable(&drive);
+        sim.setState(init);
+        const int keep[] = { 1, 2, 3, 5, 10, 20, 40 };
+        int k = 0;
+        for (int step = 1; step <= 40; ++step) {
+            sim.step();
+            if (step == keep[k]) {
+                const std::string p = "seated/S/drive/step" + std::to_string(step) + "/";
+                putBoids(p, sim.state());
+                put(p + "contacts", 'b', sim.contacts());
+                ++k;
+            }
+        }
+    }
+    {
+        CpuCrowdSim sim(venue, std::vector<float>{}, 0, CpuCrowdSim::SortMode::Counting);
+        sim.setState(init);
+        const int keep[] = { 1, 2, 3, 5, 10, 25, 50, 100, 200 };
+        int k = 0;
+        for (int step = 1; step <= 200; ++step) {
+            sim.step();
+            if (step == 1) {
+                std::vector<uint32_t> key, id, start, count;
+                for (const GpuGridInfo& g : sim.grid()) { key.push_back(g.gridID); id.push_back(g.boidID); }
+                for (const GpuCellData& c : sim.cells()) { start.push_back(c.startIndex); count.push_back(c.agentCount); }
+                put("seated/S/default/grid_key", 'u', key);
+                put("seated/S/default/grid_id", 'u', id);
+                put("seated/S/default/cell_start", 'u', start);
+                put("seated/S/default/cell_count", 'u', count);
+            }
+            if (step == keep[k]) {
+                const std::string p = "seated/S/default/step" + std::to_string(step) + "/";
+                putBoids(p, sim.state());
+                put(p + "contacts", 'b', sim.contacts());
+                putChoice(p, sim.choiceState());
+                ++k;
+            }
+        }
+        // the navigation field and the choice table the run used
+        const nav::Field& f = nav::forVenue(venue);
+        const nav::ChoiceTable& t = nav::choiceForVenue(venue);
+        const nav::Grid& g = f.grid;
+        put("seated/S/nav/grid", 'd', std::vector<double>{ double(g.res), double(g.origin), double(g.cell), double(g.invCell) });
+        std::vector<uint32_t> slots, dists;
+        for (const GpuNavChoiceCell& c : t.entries) {
+            for (int j = 0; j < nav::CHOICE_K; ++j) slots.push_back(c.slot[j]);
+            for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) dists.push_back(c.dist[j]);
+        }
+        const uint64_t h[] = { fnv64(f.cls.data(), f.cls.size()), fnv64(f.out.data(), f.out.size()),
+                               fnv64(f.exitRegion.data(), f.exitRegion.size()), fnv64(f.floorZ.data(), f.floorZ.size() * 4),
+                               fnv64(t.index.data(), t.index.size() * 4), fnv64(slots.data(), slots.size() * 4),
+                               fnv64(dists.data(), dists.size() * 4) };
+        std::vector<uint32_t> hw;
+        for (uint64_t v : h) { hw.push_back(uint32_t(v)); hw.push_back(uint32_t(v >> 32)); }
+        put("seated/S/nav/fnv", 'u', hw);
+        put("seated/S/nav/sizes", 'i', std::vector<int32_t>{ int32_t(f.cls.size()), int32_t(t.entries.size()) });
+        put("seated/S/nav/scalars", 'f', std::vector<float>{ t.secondsPerUnit, t.distQuantum });
+        put("seated/S/nav/inv_cap_s", 'f', t.invCapS);
+        std::vector<int32_t> cells;
+        std::vector<uint32_t> info, sl, ds;
+        std::vector<uint8_t> cls, out;
+        std::vector<int32_t> er;
+        std::vector<float> fz;
+        for (size_t i = 0; i < f.cls.size(); i += 997) {
+            cells.push_back(int32_t(i));
+            cls.push_back(f.cls[i]);
+            out.push_back(f.out[i]);
+            er.push_back(f.exitRegion[i]);
+            fz.push_back(f.floorZ[i]);
+            info.push_back(t.index[i]);
+            const GpuNavChoiceCell& c = t.at(i);
+            for (int j = 0; j < nav::CHOICE_K; ++j) sl.push_back(c.slot[j]);
+            for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) ds.push_back(c.dist[j]);
+        }
+        put("seated/S/nav/sample_cell", 'i', cells);
+        put("seated/S/nav/sample_cls", 'b', cls);
+        put("seated/S/nav/sample_out", 'b', out);
+        put("seated/S/nav/sample_exit_region", 'i', er);
+        put("seated/S/nav/sample_floor", 'f', fz);
+        put("seated/S/nav/sample_index", 'u', info);
+        put("seated/S/nav/sample_slots", 'u', sl, { uint64_t(cells.size()), uint64_t(nav::CHOICE_K) });
+        put("seated/S/nav/sample_dists", 'u', ds, { uint64_t(cells.size()), uint64_t(nav::CHOICE_DIST_WORDS) });
+    }
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
@@ -669,6 +793,7 @@ int main(int argc, char** argv) {
     metricsCases();
     spawnCases();
     venueCases();
+    seatedCases();
     {   // the L corridor (tools/nav_check.hpp:166-177) at 0.1 m, point agents
         const std::vector<Rect> walls = { { { -0.3f, -0.3f }, { 20.3f, 0.0f } }, { { -0.3f, 0.0f }, { 0.0f, 2.3f } }, { { 0.0f, 2.0f }, { 18.0f, 2.3f } },
                                           { { 17.7f, 2.3f }, { 18.0f, 15.0f } }, { { 20.0f, 0.0f }, { 20.3f, 15.0f } } };

@@ -8,9 +8,12 @@
 // FP32 in the reference's operation order; min / max / abs / clamp with glm's and std's tie and signed-zero rules.
 // Pinned: the cell table against the reference's own cells, every query against the reference's answers at probe
 // points (tests/golden/venue_kat.npz), the distance and the wall query against brute force over the unfolded cells.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
+
+#include "venue_oracle.h"
 
 namespace {
 
@@ -389,6 +392,102 @@ OVenue make(const float* lf, const int* li, const float* gwLo, const float* gwHi
 }
 
 }  // namespace
+
+namespace vo {
+
+namespace {
+const OVenue& ven(const void* v) { return *static_cast<const OVenue*>(v); }
+const OCell& ownCellOf(const OVenue& V, float x, float y, int& band, float& dep) {
+    float lat;
+    foldXY(x, y, lat, dep);
+    band = V.bandOf(dep);
+    return V.bands[size_t(band)].cells[size_t(V.own(band, lat))];
+}
+constexpr float STEP_UP_SAMPLE = 0x1.a36e2ep-11f;   // STEP_UP_SAMPLE_RADII * BODY_RADIUS folded (0.125 x 0.0064 = 0.0008)
+constexpr float STEP_UP_TOL = 1e-6f;
+constexpr int STEP_UP_MAX_SAMPLES = 32, SCAN_CROSSINGS = 4, SCAN_BISECTIONS = 12;
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }   // glm::mix
+}  // namespace
+
+void sdfGrad(const void* v, float x, float y, float z, float out[4]) { ::sdfGrad(ven(v), x, y, z, out); }
+void wall(const void* v, float x, float y, float z, float reach, float out[3]) { ::wall(ven(v), x, y, z, reach, out); }
+float floorHeight(const void* v, float x, float y) {
+    int j;
+    float dep;
+    const OCell& c = ownCellOf(ven(v), x, y, j, dep);
+    return ven(v).top(c.kind, c.row, dep);
+}
+int cellAt(const void* v, float x, float y, int* row) {
+    int j;
+    float dep;
+    const OCell& c = ownCellOf(ven(v), x, y, j, dep);
+    if (row) *row = c.row;
+    return c.kind;
+}
+void cellKey(const void* v, float x, float y, float out[2]) {
+    int j;
+    float dep;
+    const OCell& c = ownCellOf(ven(v), x, y, j, dep);
+    out[0] = float(j);
+    out[1] = c.l0;
+}
+float floorSlope(const void* v, float x, float y, float ex, float ey) { return ::floorSlope(ven(v), x, y, ex, ey); }
+int exitByLat(const void* v, float x, float y) { return ::exitByLat(ven(v), x, y); }
+float rake(const void* v) { return ven(v).rake; }
+float wallAbove(const void* v) { return ven(v).wallAbove; }
+float wallClear(const void* v) { return ven(v).wallClear; }
+float faceMargin(const void* v) { return ven(v).faceMargin; }
+
+float stepUpScan(const void* v, float p0x, float p0y, float p1x, float p1y, float f0, float n[2]) {
+    n[0] = 0.0f;
+    n[1] = 0.0f;
+    const float dx = p1x - p0x, dy = p1y - p0y;
+    const float len = std::sqrt(dx * dx + dy * dy);
+    const int ns = std::min(std::max(int(std::ceil(len / STEP_UP_SAMPLE)), 1), STEP_UP_MAX_SAMPLES);
+    float lo = 0.0f;
+    float key[2], k2[2];
+    cellKey(v, p0x, p0y, key);
+    auto at = [&](float t, float& x, float& y) { x = mixf(p0x, p1x, t); y = mixf(p0y, p1y, t); };
+    auto same = [&](float t) {
+        float x, y;
+        at(t, x, y);
+        cellKey(v, x, y, k2);
+        return k2[0] == key[0] && k2[1] == key[1];
+    };
+    for (int i = 1; i <= ns; ++i) {
+        const float t = float(i) / float(ns);
+        for (int k = 0; k < SCAN_CROSSINGS; ++k) {
+            if (same(t)) break;
+            float a = lo, b = t;
+            for (int m = 0; m < SCAN_BISECTIONS; ++m) {
+                const float tm = 0.5f * (a + b);
+                if (same(tm)) a = tm;
+                else b = tm;
+            }
+            float ax, ay, bx, by;
+            at(a, ax, ay);
+            at(b, bx, by);
+            const float ha = floorHeight(v, ax, ay), hb = floorHeight(v, bx, by);
+            if (hb > f0 + STEP_UP_TOL && hb - ha > rake(v) * len * (b - a) + STEP_UP_TOL) {
+                float w[3];
+                ::wall(ven(v), ax, ay, ha + STEP_UP_TOL, STEP_UP_SAMPLE, w);
+                if (w[2] < NONE) { n[0] = w[0]; n[1] = w[1]; }
+                else {
+                    const float l = std::max(len, 1e-30f);
+                    n[0] = (p0x - p1x) / l;
+                    n[1] = (p0y - p1y) / l;
+                }
+                return a;
+            }
+            lo = b;
+            cellKey(v, bx, by, key);
+        }
+        lo = t;
+    }
+    return 1.0f;
+}
+
+}  // namespace vo
 
 extern "C" {
 

@@ -365,6 +365,200 @@ void buildWallBins(const SceneDesc& d, Scene& s) {
 }
 } // namespace
 
+NavField buildSeatedNavGrid(const seated::Layout& L, const NavGrid& g, int threads, std::vector<float>* floorZ) {
+    threads = resolveThreads(threads);
+    NavField f;
+    f.grid = g;
+    const size_t n = g.cells();
+    std::vector<float> fz(n);
+    f.cls.assign(n, NAV_CLASS_NOFIT);
+    f.out.assign(n, 0);
+    f.exitRegion.assign(n, -1);
+    f.exitCount = seated::exitCount(L);
+    const float mid = seated::FENCE_HALF_MID;
+    parallelFor(threads, g.resY, 8, [&](long long iyL) {
+        const int iy = int(iyL);
+        for (int ix = 0; ix < g.resX; ++ix) {
+            const size_t i = size_t(g.index(ix, iy));
+            float cx, cy;
+            g.centre(ix, iy, cx, cy);
+            const seated::F2 c{ cx, cy };
+            const float z = seated::floorHeight(L, c);
+            fz[i] = z;
+            // fits: outside the fence's collision band, and the exact walls of the plane at the body's top (the centre at
+            // floor + BODY_RADIUS, the plane wallAbove higher) none closer than BODY_RADIUS
+            bool fit = seated::sdFence(c) >= 0.0f;
+            if (fit) fit = seated::wall(L, seated::F3{ cx, cy, z + BODY_RADIUS + L.wallAbove }, BODY_RADIUS).z >= BODY_RADIUS;
+            f.cls[i] = fit ? NAV_CLASS_FIT : NAV_CLASS_NOFIT;
+            // exitAt: past the outer wall's outer face, the avenue to the domain's edge, a door up to its landing's end
+            const seated::F2 fo = seated::fold(c);
+            int e = -1;
+            if (fo.y > L.outerFace) {
+                e = seated::exitByLat(L, c);
+                if (e >= 4 && !(fo.y <= L.landingEnd)) e = -1;
+            }
+            f.exitRegion[i] = int8_t(e);
+        }
+    });
+    auto moveLegal = [&](seated::F2 from, float fromZ, seated::F2 to, float toZ) {
+        if (seated::sdBox2(from, mid) * seated::sdBox2(to, mid) < 0.0f) return false;   // crosses the fence's midline
+        if (toZ < fromZ - MAX_STEP_UP) return false;                                     // a drop deeper than a step
+        if (toZ > fromZ + MAX_STEP_UP) return false;                                     // a rise above a step
+        seated::F2 nrm;
+        if (seated::stepUpScan(L, from, to, fromZ, nrm) < 1.0f) return false;            // a climb over a vertical face
+        int rowTo = -1, rowFrom = -1, band = 0;
+        if (seated::kindAt(L, to, &rowTo, &band) == seated::TREAD) {                      // rows are exits only
+            if (seated::kindAt(L, from, &rowFrom, &band) != seated::TREAD || rowFrom != rowTo) return false;
+        }
+        return true;
+    };
+    parallelFor(threads, g.resY, 8, [&](long long iyL) {
+        const int iy = int(iyL);
+        for (int ix = 0; ix < g.resX; ++ix) {
+            const size_t i = size_t(g.index(ix, iy));
+            float cx, cy;
+            g.centre(ix, iy, cx, cy);
+            uint8_t mask = 0;
+            for (int k = 0; k < 8; ++k) {
+                const int nx = ix + DIR_X[k], ny = iy + DIR_Y[k];
+                if (nx < 0 || ny < 0 || nx >= g.resX || ny >= g.resY) continue;
+                const size_t j = size_t(g.index(nx, ny));
+                if (DIR_X[k] != 0 && DIR_Y[k] != 0 && f.cls[i] == NAV_CLASS_FIT && f.cls[j] == NAV_CLASS_FIT &&
+                    (f.cls[size_t(g.index(nx, iy))] != NAV_CLASS_FIT || f.cls[size_t(g.index(ix, ny))] != NAV_CLASS_FIT))
+                    continue;
+                float tx, ty;
+                g.centre(nx, ny, tx, ty);
+                if (moveLegal(seated::F2{ cx, cy }, fz[i], seated::F2{ tx, ty }, fz[j])) mask = uint8_t(mask | (1u << k));
+            }
+            f.out[i] = mask;
+        }
+    });
+    if (floorZ) *floorZ = std::move(fz);
+    return f;
+}
+
+void buildCandidateTable(const NavField& f, const std::vector<double>& widthM, const std::vector<int>& exitType, double metresPerUnit,
+                         int threads, Scene& s) {
+    threads = resolveThreads(threads);
+    const size_t n = f.grid.cells();
+    const int E = f.exitCount;
+    if (E > CHOICE_MAX_EXITS) throw std::invalid_argument("buildCandidateTable: more than 63 exits");
+    if (n > size_t(CHOICE_ANCHOR_MASK) + 1) throw std::invalid_argument("buildCandidateTable: more nav cells than a 25-bit anchor holds");
+    s.E = E;
+    s.K = std::max(1, std::min(CHOICE_K, E));
+    const int K = s.K;
+    int types = 1;
+    for (int e = 0; e < E; ++e) types = std::max(types, exitType[size_t(e)] + 1);
+    if (types > 2) throw std::invalid_argument("buildCandidateTable: at most two exit types");
+    constexpr int TM = CHOICE_TYPE_MIN_MAX;
+    std::vector<float> dist(n * size_t(K), std::numeric_limits<float>::infinity());
+    s.navSlot.assign(n * size_t(K), CHOICE_EMPTY << CHOICE_EXIT_SHIFT);
+    std::vector<float> typeDist;
+    std::vector<uint32_t> typeSlot;
+    if (types == 2) {
+        typeDist.assign(n * 2 * TM, std::numeric_limits<float>::infinity());
+        typeSlot.assign(n * 2 * TM, 0u);
+    }
+    // the per-exit searches in batches (each holds about 13 bytes per cell), merged in exit order: a later exit goes
+    // before the first slot it beats, so ties keep the lower exit and the table does not depend on the batching
+    const int batch = std::max(1, std::min(std::min(threads, E), int(std::max<size_t>(1, (size_t(4) << 30) / std::max<size_t>(n * 13, 1)))));
+    std::vector<std::vector<double>> gE(static_cast<size_t>(batch));
+    std::vector<std::vector<int32_t>> parE(static_cast<size_t>(batch));
+    std::vector<std::vector<uint8_t>> closedE(static_cast<size_t>(batch));
+    for (int e0 = 0; e0 < E; e0 += batch) {
+        const int e1 = std::min(E, e0 + batch);
+        parallelFor(std::max(1, std::min(threads, e1 - e0)), e1 - e0, 1,
+                    [&](long long k) { lazyThetaStar(f, e0 + int(k), gE[size_t(k)], parE[size_t(k)], closedE[size_t(k)]); });
+        for (int e = e0; e < e1; ++e) {
+            const std::vector<double>& g = gE[size_t(e - e0)];
+            const std::vector<int32_t>& par = parE[size_t(e - e0)];
+            const std::vector<uint8_t>& closed = closedE[size_t(e - e0)];
+            const int ty = exitType[size_t(e)], mt = CHOICE_TYPE_MIN[ty];
+            parallelFor(threads, (long long)n, 4096, [&](long long ii) {
+                const size_t i = size_t(ii);
+                if (f.cls[i] != NAV_CLASS_FIT || !closed[i] || !std::isfinite(g[i])) return;
+                const float dd = float(g[i]);
+                const uint32_t slot = (uint32_t(e) << CHOICE_EXIT_SHIFT) | uint32_t(par[i]);
+                float* cd = &dist[i * size_t(K)];
+                uint32_t* cs = &s.navSlot[i * size_t(K)];
+                int m = K;
+                while (m > 0 && dd < cd[m - 1]) --m;
+                if (m < K) {
+                    for (int r = K - 1; r > m; --r) { cs[r] = cs[r - 1]; cd[r] = cd[r - 1]; }
+                    cs[m] = slot;
+                    cd[m] = dd;
+                }
+                if (types == 2) {   // the nearest of this exit's type
+                    float* td = &typeDist[(i * 2 + size_t(ty)) * TM];
+                    uint32_t* ts = &typeSlot[(i * 2 + size_t(ty)) * TM];
+                    int q = mt;
+                    while (q > 0 && dd < td[q - 1]) --q;
+                    if (q < mt) {
+                        for (int r = mt - 1; r > q; --r) { td[r] = td[r - 1]; ts[r] = ts[r - 1]; }
+                        td[q] = dd;
+                        ts[q] = slot;
+                    }
+                }
+            });
+        }
+    }
+    // the type rule on every cell whose CHOICE_K slots are all taken: the guaranteed nearest of each type first, then the
+    // nearest others, sorted by (distance, exit)
+    if (types == 2 && K == CHOICE_K) {
+        parallelFor(threads, (long long)n, 4096, [&](long long ii) {
+            const size_t i = size_t(ii);
+            uint32_t* cs = &s.navSlot[i * size_t(K)];
+            float* cd = &dist[i * size_t(K)];
+            if ((cs[K - 1] >> CHOICE_EXIT_SHIFT) == CHOICE_EMPTY) return;
+            struct Cand { float d; uint32_t s; };
+            Cand res[CHOICE_K];
+            int nres = 0;
+            auto taken = [&](uint32_t e) {
+                for (int m = 0; m < nres; ++m)
+                    if ((res[m].s >> CHOICE_EXIT_SHIFT) == e) return true;
+                return false;
+            };
+            for (int ty = 0; ty < 2; ++ty)
+                for (int m = 0; m < CHOICE_TYPE_MIN[ty] && nres < K; ++m) {
+                    const size_t j = (i * 2 + size_t(ty)) * TM + size_t(m);
+                    if (std::isfinite(typeDist[j])) res[nres++] = { typeDist[j], typeSlot[j] };
+                }
+            for (int m = 0; m < K && nres < K; ++m)
+                if (!taken(cs[m] >> CHOICE_EXIT_SHIFT)) res[nres++] = { cd[m], cs[m] };
+            std::sort(res, res + nres, [](const Cand& a, const Cand& b) {
+                return a.d < b.d || (a.d == b.d && (a.s >> CHOICE_EXIT_SHIFT) < (b.s >> CHOICE_EXIT_SHIFT));
+            });
+            for (int m = 0; m < nres; ++m) { cs[m] = res[m].s; cd[m] = res[m].d; }
+        });
+    }
+    // capacities (setChoiceCapacities, exit_choice.cpp:128-141)
+    s.invCapS.assign(size_t(E), 0.0f);
+    for (int e = 0; e < E; ++e) {
+        const double w = widthM[size_t(e)];
+        s.invCapS[size_t(e)] = w > 0.0 ? float(60.0 / (w * 83.6)) : 0.0f;
+    }
+    const bool modelUnits = metresPerUnit == M_PER_UNIT;
+    s.secondsPerUnit = modelUnits ? CHOICE_S_PER_UNIT : float(metresPerUnit / 1.34);
+    s.distQuantum = modelUnits ? CHOICE_DIST_QUANTUM
+                               : std::ldexp(1.0f, -int(std::ceil(std::log2(metresPerUnit / (double(CHOICE_DIST_QUANTUM) * M_PER_UNIT)))));
+    s.navInfo.assign(n, 0u);
+    s.navDist.assign(n * size_t(K), uint16_t(CHOICE_DIST_EMPTY));
+    for (size_t i = 0; i < n; ++i) {
+        const bool fit = f.cls[i] == NAV_CLASS_FIT;
+        const bool has = fit && (s.navSlot[i * size_t(K)] >> CHOICE_EXIT_SHIFT) != CHOICE_EMPTY;
+        s.navInfo[i] = (fit ? NAV_FIT : 0u) | (has ? NAV_HAS : 0u);
+        for (int k = 0; k < K; ++k) {
+            const float dd = dist[i * size_t(K) + size_t(k)];
+            if (!has || !std::isfinite(dd)) continue;
+            const double q = std::nearbyint(double(dd) / double(s.distQuantum));
+            if (q >= double(CHOICE_DIST_EMPTY)) throw std::invalid_argument("buildCandidateTable: a candidate distance beyond the 16-bit range");
+            s.navDist[i * size_t(K) + size_t(k)] = uint16_t(q);
+        }
+        if (!has)
+            for (int k = 0; k < K; ++k) s.navSlot[i * size_t(K) + size_t(k)] = CHOICE_EMPTY << CHOICE_EXIT_SHIFT;
+    }
+}
+
 Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) {
     const auto t0 = std::chrono::steady_clock::now();
     threads = resolveThreads(threads);
@@ -437,66 +631,18 @@ Scene buildScene(const SceneDesc& d, float maxRadius, float maxV0, int threads) 
     // --- navigation grid, per-exit fields, candidate table (exit_choice.cpp:143-317, one exit type) ---
     const NavField f = buildNavGrid(d, threads);
     s.nav = d.nav;
-    const size_t n = d.nav.cells();
     const int E = int(d.exits.size());
-    s.E = E;
-    s.K = std::max(1, std::min(CHOICE_K, E));
-    const int K = s.K;
-    if (n > size_t(CHOICE_ANCHOR_MASK) + 1) throw std::invalid_argument("buildScene: more nav cells than a 25-bit anchor holds");
-    std::vector<float> dist(n * size_t(K), std::numeric_limits<float>::infinity());
-    s.navSlot.assign(n * size_t(K), CHOICE_EMPTY << CHOICE_EXIT_SHIFT);
-    std::vector<std::vector<double>> gE(static_cast<size_t>(E));
-    std::vector<std::vector<int32_t>> parE(static_cast<size_t>(E));
-    std::vector<std::vector<uint8_t>> closedE(static_cast<size_t>(E));
-    parallelFor(std::max(1, std::min(threads, E)), E, 1,
-                [&](long long e) { lazyThetaStar(f, int(e), gE[size_t(e)], parE[size_t(e)], closedE[size_t(e)]); });
-    for (int e = 0; e < E; ++e) {   // merged in exit order: before the first slot it beats (ties keep the lower exit)
-        parallelFor(threads, (long long)n, 4096, [&](long long ii) {
-            const size_t i = size_t(ii);
-            if (f.cls[i] != NAV_CLASS_FIT) return;
-            const bool reach = closedE[size_t(e)][i] && std::isfinite(gE[size_t(e)][i]);
-            if (!reach) return;
-            const float dd = float(gE[size_t(e)][i]);
-            float* cd = &dist[i * size_t(K)];
-            uint32_t* cs = &s.navSlot[i * size_t(K)];
-            int m = K;
-            while (m > 0 && dd < cd[m - 1]) --m;
-            if (m >= K) return;
-            for (int r = K - 1; r > m; --r) { cs[r] = cs[r - 1]; cd[r] = cd[r - 1]; }
-            cs[m] = (uint32_t(e) << CHOICE_EXIT_SHIFT) | uint32_t(parE[size_t(e)][i]);
-            cd[m] = dd;
-        });
-    }
-    // capacities (setChoiceCapacities, exit_choice.cpp:128-141)
-    s.invCapS.assign(size_t(E), 0.0f);
-    for (int e = 0; e < E; ++e) {
+    std::vector<double> widthM(size_t(E), 0.0);
+    for (int e = 0; e < E; ++e) {   // an exit's clear width: given, else its rectangle's longer side
         double w = e < int(d.exitWidthM.size()) ? double(d.exitWidthM[size_t(e)]) : 0.0;
         if (w <= 0.0) {
             const Rect& r = d.exits[size_t(e)];
             w = double(std::max(r.hix - r.lox, r.hiy - r.loy));
         }
-        s.invCapS[size_t(e)] = w > 0.0 ? float(60.0 / (w * 83.6)) : 0.0f;
+        widthM[size_t(e)] = w;
     }
-    const bool modelUnits = d.metresPerUnit == M_PER_UNIT;
-    s.secondsPerUnit = modelUnits ? CHOICE_S_PER_UNIT : float(d.metresPerUnit / 1.34);
-    s.distQuantum = modelUnits ? CHOICE_DIST_QUANTUM
-                               : std::ldexp(1.0f, -int(std::ceil(std::log2(d.metresPerUnit / (double(CHOICE_DIST_QUANTUM) * M_PER_UNIT)))));
-    s.navInfo.assign(n, 0u);
-    s.navDist.assign(n * size_t(K), uint16_t(CHOICE_DIST_EMPTY));
-    for (size_t i = 0; i < n; ++i) {
-        const bool fit = f.cls[i] == NAV_CLASS_FIT;
-        const bool has = fit && (s.navSlot[i * size_t(K)] >> CHOICE_EXIT_SHIFT) != CHOICE_EMPTY;
-        s.navInfo[i] = (fit ? NAV_FIT : 0u) | (has ? NAV_HAS : 0u);
-        for (int k = 0; k < K; ++k) {
-            const float dd = dist[i * size_t(K) + size_t(k)];
-            if (!has || !std::isfinite(dd)) continue;
-            const double q = std::nearbyint(double(dd) / double(s.distQuantum));
-            if (q >= double(CHOICE_DIST_EMPTY)) throw std::invalid_argument("buildScene: a candidate distance beyond the 16-bit range");
-            s.navDist[i * size_t(K) + size_t(k)] = uint16_t(q);
-        }
-        if (!has)
-            for (int k = 0; k < K; ++k) s.navSlot[i * size_t(K) + size_t(k)] = CHOICE_EMPTY << CHOICE_EXIT_SHIFT;
-    }
+    buildCandidateTable(f, widthM, std::vector<int>(size_t(E), 0), d.metresPerUnit, threads, s);
+    const size_t n = d.nav.cells();
     // --- stair field: per nav cell, the factor of the first stair band holding its centre ---
     s.stair.assign(n, 1.0f);
     for (int iy = 0; iy < d.nav.resY; ++iy)

@@ -92,6 +92,10 @@ struct Scene {
     int fromCache = 0;                     // the tables came from the scene cache
     int hasVenue = 0;                      // a seated venue's plan: the layout of its incline rules
     seated::Layout venue{};
+    // the venue's own 3D step (buildSeatedScene): the heightfield is the scene (no SDF grid, walls or exit rectangles);
+    // agents leave when max(|x|, |y|) passes exitDepth below exitMaxZ, or domainMax
+    int venue3d = 0;
+    float exitDepth = 0.0f, exitMaxZ = 0.0f, domainMax = 0.0f;
 };
 
 // Build every table of a scene. maxRadius / maxV0: the largest agent radius and free speed the scene
@@ -116,6 +120,15 @@ struct NavField {
     int exitCount = 0;
 };
 NavField buildNavGrid(const SceneDesc& d, int threads = 0);
+// nav::buildGrid on a seated venue's heightfield (SeatedGeometry, src/nav_field.cpp:55-101): per cell its floor, its
+// class (the fence band, the exact walls at the body's top), its exit region (past the outer face: the avenue, or a door
+// up to its landing's end) and its legal moves (no fence crossing, no drop or rise beyond MAX_STEP_UP, the step-up scan,
+// rows entered only from the same row). floorZ (may be null): the floors
+NavField buildSeatedNavGrid(const seated::Layout& L, const NavGrid& g, int threads = 0, std::vector<float>* floorZ = nullptr);
+// The candidate table of a navigation field (buildChoiceTable, src/exit_choice.cpp:143-317): widthM the exits' clear
+// widths (metres), exitType 0 / 1 per exit (two types: the type rule)
+void buildCandidateTable(const NavField& f, const std::vector<double>& widthM, const std::vector<int>& exitType, double metresPerUnit,
+                         int threads, Scene& s);
 // One exit's field: dist (grid units; +inf: no path) and anchor (-1: none); exitFilter -1: every exit
 void lazyThetaStar(const NavField& f, int exitFilter, std::vector<double>& g, std::vector<int32_t>& par,
                    std::vector<uint8_t>& closed);

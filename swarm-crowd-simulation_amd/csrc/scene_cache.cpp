@@ -19,12 +19,13 @@
 #include <vector>
 
 #include "scene.hpp"
+#include "venue.hpp"
 
 namespace swarm {
 
 namespace {
 
-constexpr uint32_t SCENE_CACHE_VERSION = 2;
+constexpr uint32_t SCENE_CACHE_VERSION = 3;
 constexpr char MAGIC[8] = { 'S', 'W', 'S', 'C', 'E', 'N', 'E', '1' };
 
 struct Fnv1a {   // FNV-1a, 64 bits (Fowler, Noll, Vo)
@@ -131,6 +132,10 @@ void write(Writer& w, const Scene& s) {
     w.vec(s.metricWalkableM2);
     w.pod(s.hasVenue);
     w.pod(s.venue);
+    w.pod(s.venue3d);
+    w.pod(s.exitDepth);
+    w.pod(s.exitMaxZ);
+    w.pod(s.domainMax);
 }
 
 bool read(Reader& r, Scene& s) {
@@ -143,13 +148,14 @@ bool read(Reader& r, Scene& s) {
          r.vec(s.navSlot, LIM) && r.vec(s.navDist, LIM) && r.vec(s.invCapS, LIM) && r.pod(s.secondsPerUnit) && r.pod(s.distQuantum) &&
          r.vec(s.stair, LIM) && r.vec(s.exits, LIM) && r.vec(s.walls, LIM) && r.vec(s.wallBinStart, LIM) && r.vec(s.wallBinList, LIM) &&
          r.pod(s.maxRadius) && r.pod(s.maxV0) && r.pod(s.metricResX) && r.pod(s.metricResY) && r.pod(s.metricCellsPerUnit) &&
-         r.pod(s.metricOffX) && r.pod(s.metricOffY) && r.vec(s.metricWalkableM2, LIM) && r.pod(s.hasVenue) && r.pod(s.venue);
+         r.pod(s.metricOffX) && r.pod(s.metricOffY) && r.vec(s.metricWalkableM2, LIM) && r.pod(s.hasVenue) && r.pod(s.venue) && r.pod(s.venue3d) && r.pod(s.exitDepth) &&
+         r.pod(s.exitMaxZ) && r.pod(s.domainMax);
     if (!ok || !r.done()) return false;
     // the sizes every reader of the tables relies on
     const size_t sdfN = size_t(s.sdfNX) * size_t(s.sdfNY), navN = s.nav.cells(), hashN = size_t(s.hashNX) * size_t(s.hashNY);
     return s.sdfNX > 1 && s.sdfNY > 1 && s.hashNX > 0 && s.hashNY > 0 && s.K > 0 && s.E >= 0 && s.sdf.size() == sdfN &&
            s.sdfMin.size() == sdfN && s.navInfo.size() == navN && s.stair.size() == navN && s.navSlot.size() == navN * size_t(s.K) &&
-           s.navDist.size() == navN * size_t(s.K) && s.invCapS.size() == size_t(s.E) && s.exits.size() == size_t(s.E) &&
+           s.navDist.size() == navN * size_t(s.K) && s.invCapS.size() == size_t(s.E) && s.exits.size() == (s.venue3d ? size_t(0) : size_t(s.E)) &&
            s.wallBinStart.size() == hashN + 1 && s.wallBinStart.back() == s.wallBinList.size() &&
            s.metricWalkableM2.size() == size_t(s.metricResX) * size_t(s.metricResY);
 }
@@ -229,6 +235,41 @@ bool loadScene(Scene& s, uint64_t key, const std::string& file) {
     if (!read(r, out)) return false;
     s = std::move(out);
     return true;
+}
+
+Scene buildSeatedSceneCached(const Venue& v, int threads, const std::string& dir, int* fromCache) {
+    if (fromCache) *fromCache = 0;
+    if (dir.empty()) return buildSeatedScene(v, threads);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::ostringstream t;   // everything the tables depend on: the layout's bytes, the domain, the grid
+    t << "swarm-seated-scene|" << SCENE_CACHE_VERSION << "|" << v.gridRes << "|" << bits(v.domainMin) << "|" << bits(v.domainSpan) << "|"
+      << bits(v.exitDepth) << "|" << bits(v.exitMaxZ) << "|";
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&v.layout);
+    char h[3];
+    for (size_t i = 0; i < sizeof v.layout; ++i) {
+        std::snprintf(h, sizeof h, "%02x", b[i]);
+        t << h;
+    }
+    for (int e = 0; e < seated::exitCount(v.layout); ++e) {
+        char w[40];
+        std::snprintf(w, sizeof w, "|%.17g", venueExitWidthM(v, e));
+        t << w;
+    }
+    const std::string text = t.str();
+    Fnv1a f;
+    f.bytes(text.data(), text.size());
+    const uint64_t key = f.h;
+    const std::string file = sceneCacheFile(dir, key);
+    Scene s;
+    if (loadScene(s, key, file)) {
+        s.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        s.fromCache = 1;
+        if (fromCache) *fromCache = 1;
+        return s;
+    }
+    s = buildSeatedScene(v, threads);
+    (void)saveScene(s, key, file);
+    return s;
 }
 
 Scene buildSceneCached(const SceneDesc& d, float maxRadius, float maxV0, int threads, const std::string& dir, int* fromCache) {

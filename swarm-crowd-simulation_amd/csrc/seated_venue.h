@@ -478,5 +478,99 @@ __host__ __device__ inline int gangwayAt(const Layout& L, F2 p) {
     return straight ? base + (ab(p.y) >= ab(p.x) ? 0 : L.gangways) + i : base + 2 * L.gangways;
 }
 
+// ---------------- the venue's own 3D step: cell keys, the step-up scan, the pitch fence ----------------
+
+// The cell under p as a key (its band and its lower lateral bound): two points share a cell iff their keys are equal
+// (seated::cellKey, src/venue.cpp:631-638)
+__host__ __device__ inline F2 cellKey(const Layout& L, F2 p) {
+    const F2 f = fold(p);
+    const int j = bandOf(L, f.y);
+    Cell c{};
+    ownCell(L, j, f.x, c);
+    return F2{ float(j), c.l0 };
+}
+
+constexpr float STEP_UP_SAMPLE = 0x1.a36e2ep-11f;   // 0.125 BODY_RADIUS folded (0.0008): the scan's sample spacing
+constexpr float STEP_UP_TOL = 1e-6f;
+constexpr int STEP_UP_MAX_SAMPLES = 32;
+constexpr int SCAN_CROSSINGS = 4;    // cell boundaries located per sample
+constexpr int SCAN_BISECTIONS = 12;  // bisections per boundary
+
+__host__ __device__ inline F2 lerp2(F2 a, F2 b, float t) {   // glm::mix: a (1 - t) + b t
+    const float u = 1.0f - t;
+    return F2{ a.x * u + b.x * t, a.y * u + b.y * t };
+}
+__host__ __device__ inline bool sameKey(F2 a, F2 b) { return a.x == b.x && a.y == b.y; }
+
+// The step-up limit's scan of a horizontal move p0 -> p1 from the start floor f0: walking the cells the move crosses
+// (every STEP_UP_SAMPLE, each boundary located by bisection), the move is blocked at the first boundary into a floor
+// above f0 + STEP_UP_TOL that rises faster than the steepest incline along the move; returns the allowed fraction (1: not
+// blocked) and n, the blocking face's normal away from it (seated::stepUpScan, src/venue.cpp:807-835)
+__host__ __device__ inline float stepUpScan(const Layout& L, F2 p0, F2 p1, float f0, F2& n) {
+    n = F2{ 0.0f, 0.0f };
+    const float dx = p1.x - p0.x, dy = p1.y - p0.y;
+    const float len = sqrtf(dx * dx + dy * dy);
+    const int ns = cli(int(ceilf(len / STEP_UP_SAMPLE)), 1, STEP_UP_MAX_SAMPLES);
+    float lo = 0.0f;
+    F2 key = cellKey(L, p0);
+    for (int i = 1; i <= ns; ++i) {
+        const float t = float(i) / float(ns);
+        for (int k = 0; k < SCAN_CROSSINGS; ++k) {
+            if (sameKey(cellKey(L, lerp2(p0, p1, t)), key)) break;
+            float a = lo, b = t;
+            for (int m = 0; m < SCAN_BISECTIONS; ++m) {
+                const float tm = 0.5f * (a + b);
+                if (sameKey(cellKey(L, lerp2(p0, p1, tm)), key)) a = tm;
+                else b = tm;
+            }
+            const F2 pa = lerp2(p0, p1, a), pb = lerp2(p0, p1, b);
+            const float ha = floorHeight(L, pa), hb = floorHeight(L, pb);
+            if (hb > f0 + STEP_UP_TOL && hb - ha > L.rake * len * (b - a) + STEP_UP_TOL) {
+                const F3 w = wall(L, F3{ pa.x, pa.y, ha + STEP_UP_TOL }, STEP_UP_SAMPLE);
+                if (w.z < NO_WALL) n = F2{ w.x, w.y };
+                else {
+                    const float l = mx(len, 1e-30f);
+                    n = F2{ (p0.x - p1.x) / l, (p0.y - p1.y) / l };
+                }
+                return a;
+            }
+            lo = b;
+            key = cellKey(L, pb);
+        }
+        lo = t;
+    }
+    return 1.0f;
+}
+
+// The pitch fence: a closed square ring about the origin at the measured faces, inflated by the body radius for
+// collision (sim_params.hpp:315-327; sdBox2 / sdBox2Grad / sdFence / sdFenceGradient, src/sdf.cpp:45-82)
+constexpr float FENCE_HALF_MID = 0.5f * (0.328395f + 0.335432f);
+constexpr float FENCE_HALF_THK_COLL = 0.5f * (0.335432f - 0.328395f) + 0.0064f;
+__host__ __device__ inline float sdBox2(F2 p, float h) {
+    const float qx = ab(p.x) - h, qy = ab(p.y) - h;
+    const float ox = mx(qx, 0.0f), oy = mx(qy, 0.0f);
+    return sqrtf(ox * ox + oy * oy) + mn(mx(qx, qy), 0.0f);
+}
+__host__ __device__ inline F2 sdBox2Grad(F2 p, float h) {
+    const float qx = ab(p.x) - h, qy = ab(p.y) - h;
+    const float sx = p.x < 0.0f ? -1.0f : 1.0f, sy = p.y < 0.0f ? -1.0f : 1.0f;
+    F2 g;
+    if (mx(qx, qy) > 0.0f) {
+        const float ox = mx(qx, 0.0f), oy = mx(qy, 0.0f);
+        const float l = sqrtf(ox * ox + oy * oy);
+        g = F2{ ox / l, oy / l };
+    } else {
+        g = qx > qy ? F2{ 1.0f, 0.0f } : F2{ 0.0f, 1.0f };
+    }
+    return F2{ sx * g.x, sy * g.y };
+}
+__host__ __device__ inline float sdFence(F2 p) { return ab(sdBox2(F2{ p.x - 0.0f, p.y - 0.0f }, FENCE_HALF_MID)) - FENCE_HALF_THK_COLL; }
+__host__ __device__ inline F2 sdFenceGradient(F2 p) {
+    const F2 rel{ p.x - 0.0f, p.y - 0.0f };
+    const float s = sdBox2(rel, FENCE_HALF_MID) < 0.0f ? -1.0f : 1.0f;
+    const F2 g = sdBox2Grad(rel, FENCE_HALF_MID);
+    return F2{ s * g.x, s * g.y };
+}
+
 }  // namespace seated
 }  // namespace swarm

@@ -171,10 +171,18 @@ int swarm_scene_read_walls(const swarm_scene* sc, float* rects, uint32_t* bin_st
 int swarm_scene_read_venue(const swarm_scene* sc, int* has_venue, swarm_venue_layout* layout) {
     if (!sc || !has_venue) { g_err = "null argument"; return SWARM_EINVAL; }
     const swarm::Scene& s = sc->s;
-    *has_venue = s.hasVenue ? 1 : 0;
+    *has_venue = s.venue3d ? 2 : (s.hasVenue ? 1 : 0);
     if (layout) {
         std::memset(layout, 0, sizeof *layout);
         if (s.hasVenue) std::memcpy(layout, &s.venue, sizeof s.venue);
+        if (s.venue3d) {
+            layout->domain_min = s.xmin;
+            layout->domain_span = s.hashSpanX;
+            layout->exit_depth = s.exitDepth;
+            layout->exit_max_z = s.exitMaxZ;
+            layout->grid_res = s.hashNX;
+            layout->exit_count = s.E;
+        }
     }
     return SWARM_OK;
 }
@@ -303,6 +311,13 @@ int swarm_scene_build_venue(const char* name, float max_radius, float max_v0, in
     return guarded([&] {
         const swarm::VenuePlan plan = swarm::venuePlan(swarm::venueByName(name));
         *out = new swarm_scene{ swarm::buildSceneCached(plan.desc, max_radius, max_v0, threads, cache_dir ? cache_dir : "", nullptr) };
+    });
+}
+
+int swarm_scene_build_seated(const char* name, int threads, const char* cache_dir, swarm_scene** out) {
+    if (!name || !out) { g_err = "null argument"; return SWARM_EINVAL; }
+    return guarded([&] {
+        *out = new swarm_scene{ swarm::buildSeatedSceneCached(swarm::venueByName(name), threads, cache_dir ? cache_dir : "", nullptr) };
     });
 }
 

@@ -89,6 +89,17 @@ constexpr int STATIC_SWEEP_BISECTIONS = 8;
 constexpr float PLANE_PROBE_RADII = 2.5f;
 constexpr float NO_CONTACT = 1e9f;            // wallContact's "no wall" clearance
 
+// --- the seated venues' own 3D step (sim_params.hpp:112-117,173-199,205-294; the reference's global radius) ---
+constexpr float FEELER_REACH = 0x1.0624dep-7f;          // FEELER_REACH_RADII * BODY_RADIUS folded (0.008)
+constexpr float PLANE_PROBE_REACH = 0x1.0624dep-6f;     // PLANE_PROBE_RADII * BODY_RADIUS folded (0.016)
+constexpr float PROJ_MAX_SHIFT = 0x1.4f8b58p-12f;       // PROJ_MAX_SHIFT_RADII * BODY_RADIUS folded (0.00032)
+constexpr float GRAVITY_DT = 0x1.ebb3b0p-1f;            // 9.81 m/s^2 in model units (96.03553) * DT, folded
+constexpr float TORSO_HEIGHT = 0.05f;                   // the obstacle avoidance's sample, above the centre
+constexpr float FLOOR_SETTLE_REACH = 0x1.3a92a4p-7f;    // 1.5 BODY_RADIUS folded (0.0096): a floor found this far down
+constexpr float STEP_UP_FLOOR_REACH = 0x1.3a92a4p-5f;   // 6 BODY_RADIUS folded (0.0384): the step-up's start floor search
+constexpr float MAX_STEP_UP = 0x1.3a92a4p-7f;           // 0.30 m / 31.25 folded (0.0096)
+constexpr int STEP_UP_SLIDE_TRIES = 3;                  // slides along a blocking face: 0, 30 and 60 degrees off it
+
 // --- exit / park (sim_params.hpp:351,393) ---
 constexpr float PARK_COORD = 1000.0f;
 constexpr uint32_t INACTIVE_CELL = 0xFFFFFFFFu;
@@ -107,6 +118,10 @@ constexpr float EXIT_FLOW_P_M_MIN = 83.6f;
 constexpr float CHOICE_S_PER_UNIT = 0x1.752264p+4f;   // 31.25 / 1.34 folded (23.320896)
 constexpr float CHOICE_DIST_QUANTUM = 1.0f / 4096.0f;   // 2^-12 model units
 constexpr int CHOICE_K = 24;                 // candidate slots at most
+// the exit-type rule of a two-type table (a seated venue's avenues, type 0, and doors, type 1): the 2 nearest avenues and the
+// nearest door are always candidates (exit_choice.hpp:111-115)
+constexpr int CHOICE_TYPE_MIN[2] = { 2, 1 };
+constexpr int CHOICE_TYPE_MIN_MAX = 2;
 constexpr int CHOICE_EXIT_SHIFT = 25;
 constexpr uint32_t CHOICE_ANCHOR_MASK = (1u << CHOICE_EXIT_SHIFT) - 1u;
 constexpr uint32_t CHOICE_EMPTY = 63u;

@@ -7,6 +7,7 @@
 #include "venue.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -400,6 +401,60 @@ VenuePlan venuePlan(const Venue& v) {
             add(r, 4 + q * dpq + 2 * L.doors, 0.5 * v.cornerDoorWidthM);
         }
     return plan;
+}
+
+NavGrid venueNavGrid(const Venue& v) {   // nav::venueGrid (src/nav_field.cpp:41-52): 0.05 m cells over the domain
+    const double cell = 0.05 / M_PER_UNIT;
+    NavGrid g;
+    g.resX = g.resY = int(std::lround(double(v.domainSpan) / cell));
+    if (float(double(g.resX) * cell) != v.domainSpan) throw std::logic_error("venueNavGrid: the domain is not a whole number of cells");
+    g.originX = g.originY = v.domainMin;
+    g.cell = float(double(v.domainSpan) / g.resX);
+    g.invCell = float(double(g.resX) / double(v.domainSpan));
+    return g;
+}
+
+Scene buildSeatedScene(const Venue& v, int threads) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const seated::Layout& L = v.layout;
+    Scene s;
+    s.xmin = s.ymin = v.domainMin;
+    s.xmax = s.ymax = v.domainMin + v.domainSpan;
+    s.hashSpanX = s.hashSpanY = v.domainSpan;
+    s.hashNX = s.hashNY = v.gridRes;
+    // the predict ring covers R_PERC (sim_params.hpp:516); the projection's 3 x 3 ring is the reference's (its grid
+    // cell bounds a step's moves, sim_params.hpp:533-534)
+    s.percRing = int(R_PERC / (v.domainSpan / float(v.gridRes))) + 1;
+    s.projRing = 1;
+    s.maxRadius = BODY_RADIUS;
+    s.maxV0 = V_MAX;
+    // no distance grid: the heightfield is evaluated exactly
+    s.sdfNX = s.sdfNY = 2;
+    s.sdfOX = s.sdfOY = 0.0f;
+    s.sdfH = s.sdfInvH = 1.0f;
+    s.sdf.assign(4, 1.0f);
+    s.sdfMin.assign(4, 1.0f);
+    s.sdfMinWin = 0;
+    s.nav = venueNavGrid(v);
+    const NavField f = buildSeatedNavGrid(L, s.nav, threads);
+    const int E = seated::exitCount(L);
+    std::vector<double> widthM(static_cast<size_t>(E));
+    std::vector<int> type(static_cast<size_t>(E));
+    for (int e = 0; e < E; ++e) {
+        widthM[size_t(e)] = venueExitWidthM(v, e);
+        type[size_t(e)] = e < 4 ? 0 : 1;   // the avenues, the doors (SeatedGeometry::exitType)
+    }
+    buildCandidateTable(f, widthM, type, M_PER_UNIT, threads, s);
+    s.stair.assign(s.nav.cells(), 1.0f);
+    s.wallBinStart.assign(size_t(s.hashNX) * size_t(s.hashNY) + 1, 0u);
+    s.hasVenue = 1;
+    s.venue = L;
+    s.venue3d = 1;
+    s.exitDepth = v.exitDepth;
+    s.exitMaxZ = v.exitMaxZ;
+    s.domainMax = v.domainMin + v.domainSpan;
+    s.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return s;
 }
 
 }  // namespace swarm

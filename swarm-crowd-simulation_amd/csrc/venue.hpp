@@ -67,4 +67,13 @@ struct VenuePlan {
 };
 VenuePlan venuePlan(const Venue& v);
 
+// The venue for its own 3D step: the reference's neighbour grid (gridRes cells over the domain), its navigation field on
+// the heightfield (0.05 m cells, SeatedGeometry) with the candidate table and the exit-type rule, its exit test (past the
+// outer face below exitMaxZ, or out of the domain); the heightfield itself is the scene (no distance grid, walls or exit
+// rectangles). Every table is the reference's bit for bit.
+NavGrid venueNavGrid(const Venue& v);
+Scene buildSeatedScene(const Venue& v, int threads = 0);
+// The same through the checksummed scene cache in dir (empty: no cache)
+Scene buildSeatedSceneCached(const Venue& v, int threads, const std::string& dir, int* fromCache);
+
 }  // namespace swarm

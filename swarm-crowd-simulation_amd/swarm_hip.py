@@ -175,6 +175,16 @@ class Scene:
         return cls(h)
 
     @classmethod
+    def seated(cls, name, threads=0, cache_dir=None):
+        """The reference's own 3D step on its seated venue S, M or L: the venue's grid, its navigation field and candidate
+        table on the heightfield (the reference's bit for bit), its exit test; agents carry z (Sim.upload3 / state3)."""
+        h = C.c_void_p()
+        L = lib()
+        L.swarm_scene_build_seated.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_void_p]
+        check(L.swarm_scene_build_seated(name.encode(), threads, str(cache_dir).encode() if cache_dir else None, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def rects(cls, domain, hash_cell, sdf_spacing, sdf_clamp, nav, nav_body, walls, exits, exit_width_m=None,
               stairs=None, stair_factor=None, metres_per_unit=M_PER_UNIT, max_radius=0.0064, max_v0=0.75, threads=0):
         """A rectangle world. nav = (res_x, res_y, origin_x, origin_y, cell, inv_cell)."""
@@ -195,13 +205,19 @@ class Scene:
         return cls(h)
 
     def venue_layout(self):
-        """The layout a venue plan carries (a VenueLayout up to corner_door_lo), None for other scenes."""
+        """The layout a venue scene carries (a VenueLayout up to corner_door_lo; a 3D scene also its grid and exit test),
+        None for other scenes."""
+        kind, out = self.venue_kind()
+        return out if kind else None
+
+    def venue_kind(self):
+        """(0 no venue, 1 a venue plan, 2 a venue's own 3D step; its VenueLayout)."""
         L = lib()
         L.swarm_scene_read_venue.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         has = C.c_int()
         out = VenueLayout()
         check(L.swarm_scene_read_venue(self.h, C.byref(has), C.byref(out)))
-        return out if has.value else None
+        return has.value, out
 
     def tables(self):
         """Copies of every table, as numpy arrays (the oracle's inputs)."""

@@ -10,6 +10,9 @@ packs its records into compressed .npz files:
   venue_kat.npz    the analytic seated venues S, M and L: buildSeatedLayout's layout, the cells of every band, the seats
                    (venue L: every 17th) and seated::sdfGrad / floorHeight / cellAt / wall / floorSlope / exitByLat /
                    gangwayAt at seeded probe points
+  seated_kat.npz   CpuCrowdSim's seated 3D step on venue S (its 100 % seat spawn, 4,096 slots): a forced random drive for 40
+                   steps and the default model for 200 steps (states, contacts, the choice state), with the navigation field
+                   and choice table it used (checksums and every 997th cell)
   metrics_kat.npz  metrics::cicDeposit / speedMass known answers, MetricVenue::cellInfo thresholds and six steps of
                    MetricsReference::record on synthetic states (64 x 64 grid of 1 m cells)
 
@@ -51,7 +54,9 @@ def main():
     subprocess.run([os.path.join(REPO, "oracle", "ref_build.sh")], check=True)
     with tempfile.TemporaryDirectory() as td:
         rec = os.path.join(td, "golden.bin")
-        subprocess.run([drv, rec], check=True)
+        # the reference keeps its venue navigation caches in SWARM_NAV_CACHE: a temporary folder, never the reference's tree
+        env = dict(os.environ, SWARM_NAV_CACHE=os.path.join(tempfile.gettempdir(), "swarm_ref_navcache"))
+        subprocess.run([drv, rec], check=True, env=env)
         r = read_records(rec)
     terrace = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("t_")}
     choice = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("choice_")}
@@ -75,7 +80,9 @@ def main():
     np.savez_compressed(os.path.join(HERE, "spawn_kat.npz"), **spawn)
     venue = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("venue/")}
     np.savez_compressed(os.path.join(HERE, "venue_kat.npz"), **venue)
-    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz", "spawn_kat.npz", "venue_kat.npz"):
+    seated = {k.replace("/", "__"): v for k, v in r.items() if k.startswith("seated/")}
+    np.savez_compressed(os.path.join(HERE, "seated_kat.npz"), **seated)
+    for f in ("terrace_kat.npz", "nav_kat.npz", "choice_kat.npz", "metrics_kat.npz", "spawn_kat.npz", "venue_kat.npz", "seated_kat.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
 
 

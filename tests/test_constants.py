@@ -49,6 +49,14 @@ def expected():
         "V_STAIR_DOWN": f32(dec(0.655) * dec(TS) / dec(31.25)),
         "T_HEADWAY_STAIR_UP": fold_div(0.625, TS),
         "T_HEADWAY_STAIR_DOWN": fold_div(0.65, TS),
+        # the seated venues' own 3D step (sim_params.hpp:116-117,163,199,206,227,271,277,504)
+        "FEELER_REACH": fold_mul(1.25, 0.0064),
+        "PLANE_PROBE_REACH": fold_mul(2.5, 0.0064),
+        "PROJ_MAX_SHIFT": fold_mul(0.05, 0.0064),
+        "GRAVITY_DT": fold_mul(f32(dec(9.81) * dec(TS) * dec(TS) / dec(31.25)), 0.01),
+        "FLOOR_SETTLE_REACH": fold_mul(1.5, 0.0064),
+        "STEP_UP_FLOOR_REACH": fold_mul(6.0, 0.0064),
+        "MAX_STEP_UP": fold_div(0.30, 31.25),
         # the product's per-agent form: the stair speed as a share of the level speed
         "R_STAIR_UP": fold_div(0.58, 1.34),
         "R_STAIR_DOWN": fold_div(0.655, 1.34),
@@ -72,7 +80,8 @@ def test_product_header_literals():
     e = expected()
     for name in ("TAU_RELAX", "T_HEADWAY", "HEADWAY_RATE", "RELAX_STEP", "R_PERC2", "OBSTACLE_GRAD_SCALE", "HASH_TO_RADIANS",
                  "CHOICE_S_PER_UNIT", "R_SEP2", "BODY_DIAM2", "INCLINE_INV_REF_SLOPE", "R_STAIR_UP", "R_STAIR_DOWN",
-                 "T_HEADWAY_STAIR_UP", "T_HEADWAY_STAIR_DOWN"):
+                 "T_HEADWAY_STAIR_UP", "T_HEADWAY_STAIR_DOWN", "FEELER_REACH", "PLANE_PROBE_REACH", "PROJ_MAX_SHIFT", "GRAVITY_DT",
+                 "FLOOR_SETTLE_REACH", "STEP_UP_FLOOR_REACH", "MAX_STEP_UP"):
         m = re.search(r"constexpr float " + name + r" = (0x[0-9a-fp.+-]+)f;", text)
         assert m, name
         assert f32(float.fromhex(m.group(1))) == e[name], name
