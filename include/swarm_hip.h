@@ -218,6 +218,13 @@ int swarm_sync(swarm_ctx* c);
 int swarm_set_proj_iters(swarm_ctx* c, int iterations);
 /* currentStateBuffer readback (crowd_sim.hpp:132; CpuCrowdSim::state, cpu_crowd_sim.hpp:129) */
 int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w);
+/* A seated venue's own 3D step (a context on a swarm_scene_build_seated scene): CrowdSim::upload / currentStateBuffer
+ * with GpuBoid's full position and velocity (gpu_types.hpp:17-20): x, y, z, w and vx, vy, vz, speed; radii and free
+ * speeds are the reference's global BODY_RADIUS and V_MAX. upload3 resets the exit choice and the evacuation record;
+ * either refuses (SWARM_EINVAL) a context on any other scene. Output pointers may be NULL. */
+int swarm_upload3(swarm_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                  const float* w);
+int swarm_read_state3(swarm_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w);
 /* gridBuffer readback (crowd_sim.hpp:134; CpuCrowdSim::grid, cpu_crowd_sim.hpp:131): sorted (cell, agent) pairs of the last step's input */
 int swarm_read_grid(swarm_ctx* c, uint32_t* key, uint32_t* id);
 /* cellBuffer readback (crowd_sim.hpp:135; CpuCrowdSim::cells, cpu_crowd_sim.hpp:132): start and count per cell */

@@ -34,7 +34,9 @@ struct SceneView {
     const float* wallRects;            // wall boxes, 4 floats each (lox, loy, hix, hiy)
     const uint32_t* wallBinStart;      // per hash cell: its walls within reach, wallBinList[start[c] .. start[c + 1])
     const uint32_t* wallBinList;
-    const seated::Layout* venue;       // a seated venue's plan: its layout (the incline rules), else null
+    const seated::Layout* venue;       // a seated venue: its layout (a plan's incline rules, or the 3D step), else null
+    int venue3d;                       // the venue's own 3D step: the surface rule in choiceCell, the venue's wall term
+    float exitDepth, exitMaxZ, domainMax;   // its exit test
 };
 
 struct ModelParams {
@@ -135,6 +137,17 @@ __device__ inline V2 contactCorrection(V2 offset, uint32_t self, uint32_t other,
 }
 
 // ---- navigation / exit choice (exit_choice.cpp:512-603) ----
+// On a seated venue's own step: whether the cell centre of nav cell a is reachable on the venue's surface from p (the
+// same venue cell, or a move the step-up scan allows; surfaceReaches, src/nav_field.cpp:375-385)
+__device__ inline bool surfaceReaches(const SceneView& s, V2 p, uint32_t a) {
+    const uint32_t res = uint32_t(s.navRX);
+    const float cx = float(a % res) + 0.5f, cy = float(a / res) + 0.5f;
+    const float px = cx * s.navCell, py = cy * s.navCell;
+    const seated::F2 c{ s.navOX + px, s.navOY + py }, x{ p.x, p.y };
+    if (seated::sameKey(seated::cellKey(*s.venue, c), seated::cellKey(*s.venue, x))) return true;
+    seated::F2 n;
+    return seated::stepUpScan(*s.venue, x, c, seated::floorHeight(*s.venue, x), n) >= 1.0f;
+}
 __device__ inline int choiceCell(const SceneView& s, V2 p) {
     const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;
     const int cx = clampi(int(floorf(gx)), 0, s.navRX - 1), cy = clampi(int(floorf(gy)), 0, s.navRY - 1);
@@ -154,6 +167,22 @@ __device__ inline int choiceCell(const SceneView& s, V2 p) {
                     const float d2 = qx * qx + qy * qy;
                     if (d2 < best) { best = d2; i = j; }
                 }
+    }
+    if (s.venue3d && !surfaceReaches(s, p, uint32_t(i))) {   // the surface rule (exit_choice.cpp:533-546)
+        float best = 1e9f;
+        int pick = i;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int mx = cx + dx, my = cy + dy;
+                if (mx < 0 || my < 0 || mx >= s.navRX || my >= s.navRY) continue;
+                const int j = mx + my * s.navRX;
+                const uint32_t info = s.navInfo[j];
+                if (j == i || !(info & NAV_FIT) || !(info & NAV_HAS)) continue;
+                const float qx = gx - (float(mx) + 0.5f), qy = gy - (float(my) + 0.5f);
+                const float d2 = qx * qx + qy * qy;
+                if (d2 < best && surfaceReaches(s, p, uint32_t(j))) { best = d2; pick = j; }
+            }
+        i = pick;
     }
     return i;
 }

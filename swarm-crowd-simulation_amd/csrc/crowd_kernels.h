@@ -51,6 +51,7 @@ constexpr int QUEUE_STRIDE = 128, COUNTER_STRIDE = 4, THRESHOLD_STRIDE = 64, PER
 // agent-order state, one half of the ping-pong
 struct AgentHalf {
     float* x; float* y; float* vx; float* vy; float* spd; float* w;
+    float* z; float* vz;   // heights and vertical velocities (a seated venue's own 3D step; unused elsewhere)
 };
 
 struct GridBuffers {

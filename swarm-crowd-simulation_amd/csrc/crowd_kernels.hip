@@ -23,6 +23,7 @@
 // sorted by cell, so the row's slots in slot order are the ring scan's order (dy, dx, slot).
 // Data is SoA FP32. The kernels are latency-bound gathers and integer atomics: no MFMA.
 #include "crowd_kernels.h"
+#include "seated_step.h"
 
 #include <hip/hip_ext.h>
 
@@ -88,15 +89,16 @@ __device__ inline void bucketOffsets(int i, int n, uint32_t k, uint32_t* __restr
 
 template <bool B>
 __global__ void __launch_bounds__(HASH_BLOCK) k_hash(int n, SceneView s0, BatchView bt, float* __restrict__ ax, float* __restrict__ ay,
-                                                     float* __restrict__ aw, uint32_t* __restrict__ key, uint32_t* __restrict__ loc,
-                                                     uint32_t* __restrict__ bcnt, uint32_t C) {
+                                                     float* __restrict__ aw, float* __restrict__ az, uint32_t* __restrict__ key,
+                                                     uint32_t* __restrict__ loc, uint32_t* __restrict__ bcnt, uint32_t C) {
     const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
     uint32_t k = INACTIVE_CELL;
     if (i < n) {
         float x = ax[i], y = ay[i], w = aw[i];
-        if (isnan(x)) {   // calc_grid.comp resets a NaN position in the input state
+        if (isnan(x)) {   // calc_grid.comp resets a NaN position in the input state (to (0, 0, 0), active)
             x = 0.0f; y = 0.0f; w = 1.0f;
             ax[i] = x; ay[i] = y; aw[i] = w;
+            if (az) az[i] = 0.0f;
         }
         if (w != 0.0f) {
             const ScnRef r = scnOfAgent<B>(bt, uint32_t(i));
@@ -534,7 +536,8 @@ template <bool B>
 __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0, BatchView bt, const uint32_t* __restrict__ skey,
                                                              const uint32_t* __restrict__ sid, const float* __restrict__ sx,
                                                              const float* __restrict__ sy, const float* __restrict__ srad, ChoiceBuffers c,
-                                                             float* __restrict__ egx, float* __restrict__ egy, const uint32_t* __restrict__ ctrl) {
+                                                             float* __restrict__ egx, float* __restrict__ egy, const uint32_t* __restrict__ ctrl,
+                                                             const float* __restrict__ az) {
     const int slot = xcdBlock() * HASH_BLOCK + threadIdx.x;
     const bool live = slot < n && skey[slot] != INACTIVE_CELL;
     const uint32_t step = ctrl[CTRL_STEP];
@@ -577,7 +580,10 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
             akey = uint32_t(r.aheadBase + size_t(1u - par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS) + ex * uint32_t(CHOICE_AHEAD_BINS) +
                    aheadBin(s.navDist[size_t(cell) * size_t(s.K) + size_t(k)]);
         }
-        const V2 e = goalWallRule(s, p, FEELER_REACH_RADII * srad[slot], choiceDirection(s, cell, p, ex));
+        const V2 dir = choiceDirection(s, cell, p, ex);
+        // the wall term: the plan's wall boxes, or on a venue's own step its exact wall at the centre's height
+        const V2 e = s.venue3d ? seatedstep::venueGoalWallRule(*s.venue, seatedstep::V3{ p.x, p.y, az[id] }, dir)
+                               : goalWallRule(s, p, FEELER_REACH_RADII * srad[slot], dir);
         egx[slot] = e.x;
         egy[slot] = e.y;
     }
@@ -1520,7 +1526,7 @@ static void gridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const 
     const uint32_t C = L.cells;
     const uint32_t nb = C + uint32_t(blocks(n, HASH_BLOCK));
     const int tiles = blocks(int(nb), SCAN_TILE);
-    k_hash<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, g.key, g.loc, g.bcnt, C);
+    k_hash<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, rd.z, g.key, g.loc, g.bcnt, C);
     k_scan_reduce<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, g.tileSum);
     k_scan_apply<<<tiles, SCAN_BLOCK, 0, st>>>(g.bcnt, nb, C, g.tileSum, g.bstart, ctrl);
     if (L.bitonic) {   // the cell ranges from the counts, the permutation from the reference's network
@@ -1571,7 +1577,8 @@ static void choicePasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, cons
     k_choice_prefix<B><<<L.scenarios, 1024, 0, st>>>(L.s, L.bt, c, ctrl);
     k_choice_eval<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
     k_choice_thresh<B><<<L.scenarios, 64, 0, st>>>(L.s, L.bt, c, ctrl);
-    k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c, b.egx, b.egy, ctrl);
+    k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c, b.egx, b.egy, ctrl,
+                                                                     rd.z);
 }
 
 void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, const float* driveX,

@@ -59,6 +59,10 @@ void DeviceScene::upload(const Scene& s) {
     v.wallBinStart = static_cast<const uint32_t*>(put(s.wallBinStart.data(), s.wallBinStart.size() * 4));
     v.wallBinList = static_cast<const uint32_t*>(put(s.wallBinList.data(), s.wallBinList.size() * 4));
     v.venue = s.hasVenue ? static_cast<const seated::Layout*>(put(&s.venue, sizeof s.venue)) : nullptr;
+    v.venue3d = s.venue3d;
+    v.exitDepth = s.exitDepth;
+    v.exitMaxZ = s.exitMaxZ;
+    v.domainMax = s.domainMax;
 }
 
 void DeviceScene::release() {
@@ -97,6 +101,10 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("HipCrowdSim: no HIP device");
     batched_ = batched;
+    for (const Scene* s : layouts)
+        if (s->venue3d && batched)
+            throw std::invalid_argument("HipCrowdSim: a seated venue's own 3D step runs on a plain context, not in a batch");
+    venue3d_ = !batched && layouts[0]->venue3d != 0;
     S_ = int(scenLayout.size());
     const size_t nScen = static_cast<size_t>(S_);
     std::vector<uint32_t> cellBase(nScen), agentBase(nScen), lay(nScen);
@@ -144,7 +152,10 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     for (int h = 0; h < 2; ++h) {
         half_[h].x = alloc<float>(n); half_[h].y = alloc<float>(n); half_[h].vx = alloc<float>(n);
         half_[h].vy = alloc<float>(n); half_[h].spd = alloc<float>(n); half_[h].w = alloc<float>(n);
+        half_[h].z = alloc<float>(n); half_[h].vz = alloc<float>(n);
     }
+    seatedSlots_.pz = alloc<float>(n);
+    seatedSlots_.pvz = alloc<float>(n);
     rad_ = alloc<float>(n); v0_ = alloc<float>(n);
     driveX_ = alloc<float>(n); driveY_ = alloc<float>(n);
     grid_.key = alloc<uint32_t>(n); grid_.loc = alloc<uint32_t>(n);
@@ -383,6 +394,7 @@ std::vector<std::pair<void*, size_t>> HipCrowdSim::persistentState() const {
 
 double HipCrowdSim::autotune(int steps) {
     if (steps <= 0) throw std::invalid_argument("autotune: steps must be positive");
+    if (venue3d_) return 0.0;   // the seated passes have one launch shape: nothing to tune
     int dev = 0, cus = 0;
     SWARM_HIP_CHECK(hipGetDevice(&dev));
     SWARM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -470,8 +482,10 @@ void HipCrowdSim::enqueueStep(int parity) {
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[1], stream_));
     launchChoice(stream_, L, rd, ch_, b, drive_ ? driveX_ : nullptr, drive_ ? driveY_ : nullptr, ctrl_);
     if (timing_) SWARM_HIP_CHECK(hipEventRecord(ev[2], stream_));
-    launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr,
-                  timing_ && timeLaunches_ && mp_.projIters > 2 ? &ev[6] : nullptr);
+    if (venue3d_) launchSeatedPhysics(stream_, L, b, seatedSlots_, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr);
+    else
+        launchPhysics(stream_, L, b, pa_, pb_, rd, wr, ev_, ctrl_, timing_ ? &ev[3] : nullptr,
+                      timing_ && timeLaunches_ && mp_.projIters > 2 ? &ev[6] : nullptr);
     if (lossOn_) launchOverlap(stream_, L, b, wr, overlapWeight_, lossAcc_, nullptr);
     if (metricsOn_) {   // the step's input half rd, its output wr; in this step's slot order
         launchMetricsDeposit(stream_, n_, mgrid_, b.sid, rd.x, rd.y, rd.w, wr.x, wr.y, wr.w, wr.vx, wr.vy, scenes_[0].view.E,
@@ -642,6 +656,30 @@ void HipCrowdSim::readState(float* x, float* y, float* vx, float* vy, float* spe
     if (vy) SWARM_HIP_CHECK(hipMemcpyAsync(vy, h.vy, b, hipMemcpyDeviceToHost, stream_));
     if (speed) SWARM_HIP_CHECK(hipMemcpyAsync(speed, h.spd, b, hipMemcpyDeviceToHost, stream_));
     if (w) SWARM_HIP_CHECK(hipMemcpyAsync(w, h.w, b, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void HipCrowdSim::upload3(const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                          const float* w) {
+    if (!venue3d_) throw std::invalid_argument("upload3: the scene is not a seated venue's own 3D step");
+    const size_t n = size_t(n_), b = n * 4;
+    std::vector<float> r(n, BODY_RADIUS), v0(n, V_MAX), spd(n);
+    upload(x, y, vx, vy, w, r.data(), v0.data());
+    for (size_t i = 0; i < n; ++i) spd[i] = std::sqrt((vx[i] * vx[i] + vy[i] * vy[i]) + vz[i] * vz[i]);
+    for (int h = 0; h < 2; ++h) {
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].z, z, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].vz, vz, b, hipMemcpyHostToDevice, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(half_[h].spd, spd.data(), b, hipMemcpyHostToDevice, stream_));
+    }
+    sync();
+}
+
+void HipCrowdSim::readState3(float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w) {
+    readState(x, y, vx, vy, speed, w);
+    const AgentHalf h = half_[parity_];
+    const size_t b = size_t(n_) * 4;
+    if (z) SWARM_HIP_CHECK(hipMemcpyAsync(z, h.z, b, hipMemcpyDeviceToHost, stream_));
+    if (vz) SWARM_HIP_CHECK(hipMemcpyAsync(vz, h.vz, b, hipMemcpyDeviceToHost, stream_));
     sync();
 }
 

@@ -20,6 +20,7 @@
 #include "crowd_kernels.h"
 #include "metrics.h"
 #include "scene.hpp"
+#include "seated_kernels.h"
 
 namespace swarm {
 
@@ -94,6 +95,11 @@ public:
     int readTimings(double* ms8);
 
     void readState(float* x, float* y, float* vx, float* vy, float* speed, float* w);
+    // A seated venue's own 3D step (a scene of buildSeatedScene): the state with heights and vertical velocities, at the
+    // reference's global radius and free speed (upload3 resets the exit choice and the evacuation record as upload does)
+    void upload3(const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz, const float* w);
+    void readState3(float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w);
+    bool seated3d() const { return venue3d_; }
     void readPrevState(float* x, float* y, float* vx, float* vy, float* speed, float* w);   // previousStateBuffer
     // The sort stage alone on the caller's keys (CrowdSim::sortGrid): n keys, each a cell (< numCells) or
     // INACTIVE_CELL; returns the sorted keys and the permutation (the grid readback then holds this sort)
@@ -196,6 +202,8 @@ private:
     SimOptions opt_;
     ModelParams mp_{};
     bool batched_ = false;
+    bool venue3d_ = false;          // the scene is a seated venue's own 3D step
+    SeatedSlots seatedSlots_{};
     int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
     int physBlock_ = 0, physGrid_ = 0;
     int pruneEvery_ = PROJ_PRUNE_EVERY;

@@ -1,0 +1,247 @@
+// The seated venues' own 3D step on the GPU (gfx950): predict, the projection's Jacobi iterations and the final pass
+// of the reference's step on an analytic venue (CpuCrowdSim / physics.comp with VENUE_ANALYTIC), one slot per thread.
+// The grid passes and the exit choice are the plain step's (crowd_kernels.hip, with the venue's surface rule and wall
+// term); agents carry a height and a vertical velocity (AgentHalf z, vz; the slots' predicted ones in SeatedSlots).
+// Bit-identical to the reference's CPU step: every expression is its FP32 operation order (seated_step.h).
+#include "crowd_kernels.h"
+#include "seated_kernels.h"
+#include "seated_step.h"
+
+namespace swarm {
+
+using namespace seatedstep;
+
+namespace {
+
+constexpr int SEATED_BLOCK = 128;
+
+inline int blocksOf(int n, int b) { return (n + b - 1) / b; }
+
+// One Jacobi iteration for one slot (cpu_crowd_sim.cpp:780-820): the contacts of the 3 x 3 ring's slots whose heights
+// overlap, the mean correction capped at PROJ_MAX_SHIFT, the fence, the slot's wall plane; p and acc are updated
+__device__ inline void projectSeated(const SceneView& s, const StepBuffers& b, const SeatedSlots& q, ProjBuffers in, int slot,
+                                     uint32_t key, V2& p, float& acc) {
+    const float z = q.pz[slot];
+    const uint32_t self = b.sid[slot];
+    const int cx = int(key % uint32_t(s.hashNX)), cy = int(key / uint32_t(s.hashNX));
+    V2 corr{ 0.0f, 0.0f };
+    int nContacts = 0;
+    for (int yy = -s.projRing; yy <= s.projRing; yy++)
+        for (int xx = -s.projRing; xx <= s.projRing; xx++) {
+            const int nx = cx + xx, ny = cy + yy;
+            if (nx < 0 || nx >= s.hashNX || ny < 0 || ny >= s.hashNY) continue;
+            const uint32_t c = uint32_t(nx + ny * s.hashNX);
+            const uint32_t k0 = b.bstart[c], k1 = b.bstart[c + 1];
+            for (uint32_t k = k0; k < k1; ++k) {
+                if (int(k) == slot) continue;
+                const float2 o = in.p[k];
+                const V3 offset{ p.x - o.x, p.y - o.y, z - q.pz[k] };
+                if (b.pw[k] != 0.0f && fabsf(offset.z) < BODY_DIAM && dot2(xy(offset), xy(offset)) < BODY_DIAM2) {
+                    corr = corr + contact3(offset, self, b.sid[k]);
+                    ++nContacts;
+                }
+            }
+        }
+    corr = corr * (PROJ_RELAX / float(nContacts > 1 ? nContacts : 1));
+    const float len = len2(corr);
+    if (len > PROJ_MAX_SHIFT) corr = corr * (PROJ_MAX_SHIFT / len);
+    V2 r = p + corr;
+    const float dFence = seated::sdFence(seated::F2{ r.x, r.y });
+    if (dFence < 0.0f) r = r - fenceGrad(V3{ r.x, r.y, z }) * dFence;
+    const float4 pl = b.planes[slot];
+    if (fmodf(pl.w, 2.0f) == 1.0f) {   // PLANE_WALL
+        const float sg = pl.z - dot2(r, V2{ pl.x, pl.y });
+        if (sg > 0.0f) r = r + V2{ pl.x, pl.y } * sg;
+    }
+    const V2 d = r - p;
+    p = r;
+    acc += len2(d);
+}
+
+// PHYSICS_PASS 1 on the venue (cpu_crowd_sim.cpp:655-777): the exit test, separation and the headway rule over the
+// 3 x 3 ring (3D offsets), the incline rules, obstacle avoidance at torso height against the venue and the fence, the
+// kinematics with gravity, the static contacts and the wall plane
+__global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneView s, ModelParams mp, StepBuffers b, AgentHalf rd,
+                                                                SeatedSlots q, ProjBuffers out) {
+    const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
+    if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
+    const uint32_t key = b.skey[slot];
+    if (key == INACTIVE_CELL) return;
+    const Layout& L = *s.venue;
+    const uint32_t self = b.sid[slot];
+    const V3 p{ b.sx[slot], b.sy[slot], rd.z[self] };
+    const V3 v{ rd.vx[self], rd.vy[self], rd.vz[self] };
+    const float m = mxf(fabsf(p.x), fabsf(p.y));
+    if ((m > s.exitDepth && p.z < s.exitMaxZ) || m > s.domainMax) {   // parked: it left the venue
+        out.p[slot] = make_float2(PARK_COORD, PARK_COORD);
+        out.acc[slot] = 0.0f;
+        b.pw[slot] = 0.0f;
+        b.pvx[slot] = 0.0f;
+        b.pvy[slot] = 0.0f;
+        q.pz[slot] = PARK_COORD;
+        q.pvz[slot] = 0.0f;
+        b.planes[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const V2 eGoal{ b.egx[slot], b.egy[slot] };
+    const int cx = int(key % uint32_t(s.hashNX)), cy = int(key / uint32_t(s.hashNX));
+    V2 sep{ 0.0f, 0.0f };
+    float sAhead = 1e9f;   // the horizontal centre distance to the nearest agent ahead
+    for (int yy = -s.percRing; yy <= s.percRing; yy++)
+        for (int xx = -s.percRing; xx <= s.percRing; xx++) {
+            const int nx = cx + xx, ny = cy + yy;
+            if (nx < 0 || nx >= s.hashNX || ny < 0 || ny >= s.hashNY) continue;
+            const uint32_t c = uint32_t(nx + ny * s.hashNX);
+            const uint32_t k0 = b.bstart[c], k1 = b.bstart[c + 1];
+            for (uint32_t k = k0; k < k1; ++k) {
+                if (int(k) == slot) continue;
+                const uint32_t other = b.sid[k];
+                const V3 offset{ p.x - b.sx[k], p.y - b.sy[k], p.z - rd.z[other] };
+                const float dxy2 = dot2(xy(offset), xy(offset));
+                if (dxy2 < R_PERC2) {
+                    sep = sep + separation3(offset, self, other);
+                    const float ahead = -dot2(xy(offset), eGoal);
+                    const float lateral = fabsf(offset.x * eGoal.y - offset.y * eGoal.x);
+                    const float dxy = sqrtf(dxy2);
+                    if (ahead > 0.0f && lateral < BODY_DIAM && dxy < sAhead) {
+                        const V2 eOther{ b.egx[k], b.egy[k] };
+                        const float aheadOther = dot2(xy(offset), eOther);
+                        const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < BODY_DIAM;
+                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = dxy;
+                    }
+                }
+            }
+        }
+    // the incline speed and the stair spacing by the floor's slope along e_goal (the reference's default venue model)
+    const float slope = seated::floorSlope(L, seated::F2{ p.x, p.y }, seated::F2{ eGoal.x, eGoal.y });
+    const float vFree = inclineFreeSpeed(V_MAX, slope);
+    const float vc = slope != 0.0f ? (sAhead - BODY_DIAM) / inclineHeadwayTime(slope) : (sAhead - BODY_DIAM) * HEADWAY_RATE;
+    const float vDes = seated::cl(vc, 0.0f, vFree) * len2(eGoal);
+    const V2 eDir = safeNormalize(eGoal + sep);
+    // obstacle avoidance TORSO_HEIGHT above the centre: the venue's distance or the fence's, the nearer
+    V2 aObs{ 0.0f, 0.0f };
+    const V3 torso{ p.x, p.y, p.z + TORSO_HEIGHT };
+    const float dStadium = venueMap(L, torso);
+    const float dFence = fence(torso) + BODY_RADIUS;
+    const bool onFence = dFence < dStadium;
+    const float dTorso = onFence ? dFence : dStadium;
+    if (dTorso < OBSTACLE_RANGE) {
+        const V2 norm = onFence ? fenceGrad(torso) : venueNormalXY(L, torso);
+        if (len2(norm) > OBSTACLE_MIN_NORMAL) aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * mp.obstacleGain;
+    }
+    const V2 vEq = eDir * vDes + aObs * mp.tau;
+    V3 nv = v;
+    setXY(nv, vEq + (xy(nv) - vEq) * mp.relaxStep);
+    const float along = dot2(xy(nv), eGoal);
+    if (along > vDes) setXY(nv, xy(nv) - eGoal * (along - vDes));
+    nv.z -= GRAVITY_DT;
+    nv.z = mnf(nv.z, 0.0f);   // no upward self-propulsion
+    if (len2(xy(nv)) > V_MAX) setXY(nv, safeNormalize(xy(nv)) * V_MAX);
+    V3 np{ p.x + nv.x * DT, p.y + nv.y * DT, p.z + nv.z * DT };
+    V3 wall;
+    const bool grounded = resolveStatic(L, p, np, nv, false, wall);
+    if (wall.x == 0.0f && wall.y == 0.0f) wall = wallPlaneProbe(L, np);
+    out.p[slot] = make_float2(np.x, np.y);
+    out.acc[slot] = 0.0f;
+    b.pw[slot] = 1.0f;
+    b.pvx[slot] = nv.x;
+    b.pvy[slot] = nv.y;
+    q.pz[slot] = np.z;
+    q.pvz[slot] = nv.z;
+    const V2 wn{ wall.x, wall.y };
+    b.planes[slot] = make_float4(wn.x, wn.y, dot2(xy(np), wn) - wall.z, (nonzero(wn) ? 1.0f : 0.0f) + (grounded ? 2.0f : 0.0f));
+}
+
+// PHYSICS_PASS 2 on the venue (cpu_crowd_sim.cpp:822-829): one Jacobi iteration, in -> out
+__global__ void __launch_bounds__(SEATED_BLOCK) k_seated_project(int n, SceneView s, StepBuffers b, SeatedSlots q, ProjBuffers in,
+                                                                ProjBuffers out) {
+    const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
+    if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
+    const uint32_t key = b.skey[slot];
+    if (key == INACTIVE_CELL) return;
+    const float2 p0 = in.p[slot];
+    V2 p{ p0.x, p0.y };
+    float acc = in.acc[slot];
+    if (b.pw[slot] != 0.0f) projectSeated(s, b, q, in, slot, key, p, acc);
+    out.p[slot] = make_float2(p.x, p.y);
+    out.acc[slot] = acc;
+}
+
+// PHYSICS_PASS 3 on the venue (cpu_crowd_sim.cpp:835-866): the last iteration, the final static contacts of every agent
+// the projection moved, the step-up limit of every agent; the scatter to agent order with the evacuation events
+__global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView s, StepBuffers b, SeatedSlots q, ProjBuffers in,
+                                                              int projectHere, AgentHalf rd, AgentHalf wr, EvacBuffers ev,
+                                                              uint32_t* __restrict__ ctrl) {
+    const uint32_t step = ctrl[CTRL_STEP];
+    const int active = int(ctrl[CTRL_NACTIVE]);
+    const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
+    if (slot < n) {
+        const uint32_t key = b.skey[slot];
+        const uint32_t agent = b.sid[slot];
+        if (slot >= active || key == INACTIVE_CELL) {   // carried over
+            wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.z[agent] = rd.z[agent]; wr.w[agent] = rd.w[agent];
+            wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.vz[agent] = rd.vz[agent]; wr.spd[agent] = rd.spd[agent];
+        } else {
+            const Layout& L = *s.venue;
+            const float2 p0 = in.p[slot];
+            V2 p{ p0.x, p0.y };
+            float acc = in.acc[slot];
+            const float pw = b.pw[slot];
+            if (pw != 0.0f && projectHere) projectSeated(s, b, q, in, slot, key, p, acc);
+            const float sx = b.sx[slot], sy = b.sy[slot];
+            if (pw == 0.0f) {   // parked this step: an evacuation event, its exit by the last active position
+                ev.inside[agent] = make_float2(sx, sy);
+                ev.before[agent] = make_float2(wr.x[agent], wr.y[agent]);
+                wr.x[agent] = p.x; wr.y[agent] = p.y; wr.z[agent] = q.pz[slot]; wr.w[agent] = 0.0f;
+                wr.vx[agent] = 0.0f; wr.vy[agent] = 0.0f; wr.vz[agent] = 0.0f; wr.spd[agent] = acc;
+                const int e = seated::exitByLat(L, seated::F2{ sx, sy });
+                const int ex = e >= 0 ? e : s.E;
+                ev.step[agent] = int(step);
+                ev.exit[agent] = ex;
+                atomicAdd(&ev.perExit[uint32_t(ex)], 1u);
+            } else {
+                V3 np{ p.x, p.y, q.pz[slot] }, nv{ b.pvx[slot], b.pvy[slot], q.pvz[slot] };
+                const V3 pOld{ sx, sy, rd.z[agent] };
+                if (acc > 0.0f) {
+                    V3 wall;
+                    resolveStatic(L, pOld, np, nv, b.planes[slot].w >= 2.0f, wall);   // PLANE_GROUNDED
+                }
+                limitStepUp(L, pOld, np, nv);
+                wr.x[agent] = np.x; wr.y[agent] = np.y; wr.z[agent] = np.z; wr.w[agent] = 1.0f;
+                wr.vx[agent] = nv.x; wr.vy[agent] = nv.y; wr.vz[agent] = nv.z; wr.spd[agent] = len3(nv);
+            }
+        }
+    }
+    // the last block to finish advances the step counter: every block has read it by then
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctrl[CTRL_DONE_BLOCKS], 1u) == gridDim.x - 1u) {
+            ctrl[CTRL_DONE_BLOCKS] = 0u;
+            ctrl[CTRL_STEP] = step + 1u;
+            *reinterpret_cast<unsigned long long*>(ctrl + CTRL_ACTIVE_SUM_LO) += ctrl[CTRL_NACTIVE];
+        }
+    }
+}
+
+}  // namespace
+
+void launchSeatedPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, SeatedSlots q, ProjBuffers pa, ProjBuffers pb,
+                         AgentHalf rd, AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, hipEvent_t* stamps) {
+    const int n = L.n;
+    const int g = blocksOf(n, SEATED_BLOCK);
+    const int P = L.mp.projIters;
+    k_seated_predict<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, L.mp, b, rd, q, pa);
+    if (stamps) (void)hipEventRecord(stamps[0], st);
+    ProjBuffers in = pa, out = pb;
+    for (int it = 1; it < P; ++it) {
+        k_seated_project<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, out);
+        const ProjBuffers t = in;
+        in = out;
+        out = t;
+    }
+    if (stamps) (void)hipEventRecord(stamps[1], st);
+    k_seated_final<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, P > 0 ? 1 : 0, rd, wr, ev, ctrl);
+}
+
+}  // namespace swarm

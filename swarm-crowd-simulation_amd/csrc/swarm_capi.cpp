@@ -366,6 +366,19 @@ int swarm_set_choice_state(swarm_ctx* c, uint32_t step, const uint32_t* assign, 
 }
 int swarm_sync(swarm_ctx* c) { CTX_CHECK(c); return guarded([&] { c->sim->sync(); }); }
 int swarm_set_proj_iters(swarm_ctx* c, int it) { CTX_CHECK(c); return guarded([&] { c->sim->setProjectionIterations(it); }); }
+int swarm_upload3(swarm_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                  const float* w) {
+    CTX_CHECK(c);
+    if (!x || !y || !z || !vx || !vy || !vz || !w) { g_err = "null state array"; return SWARM_EINVAL; }
+    return guarded([&] { c->sim->upload3(x, y, z, vx, vy, vz, w); });
+}
+int swarm_read_state3(swarm_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* speed, float* w) {
+    CTX_CHECK(c);
+    return guarded([&] {
+        if (!c->sim->seated3d()) throw std::invalid_argument("swarm_read_state3: the scene is not a seated venue's own 3D step");
+        c->sim->readState3(x, y, z, vx, vy, vz, speed, w);
+    });
+}
 int swarm_read_state(swarm_ctx* c, float* x, float* y, float* vx, float* vy, float* speed, float* w) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->readState(x, y, vx, vy, speed, w); });

@@ -405,6 +405,25 @@ class Sim:
         f = lambda k: np.ascontiguousarray(a[k], np.float32)
         check(lib().swarm_restore_state(self.h, f("x"), f("y"), f("vx"), f("vy"), f("w")))
 
+    def upload3(self, pos, vel):
+        """A seated venue's own 3D step: pos, vel (n, 4) as GpuBoid's position (x, y, z, w) and velocity (vx, vy, vz,
+        speed; the speed is recomputed)."""
+        pos = np.asarray(pos, np.float32)
+        vel = np.asarray(vel, np.float32)
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        L = lib()
+        L.swarm_upload3.argtypes = [C.c_void_p] + [np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")] * 7
+        check(L.swarm_upload3(self.h, f(pos[:, 0]), f(pos[:, 1]), f(pos[:, 2]), f(vel[:, 0]), f(vel[:, 1]), f(vel[:, 2]), f(pos[:, 3])))
+
+    def state3(self):
+        """(pos, vel) as (n, 4) arrays like GpuBoid's: (x, y, z, w) and (vx, vy, vz, speed)."""
+        a = [np.zeros(self.n, np.float32) for _ in range(8)]
+        L = lib()
+        L.swarm_read_state3.argtypes = [C.c_void_p] + [np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")] * 8
+        check(L.swarm_read_state3(self.h, *a))
+        x, y, z, vx, vy, vz, sp, w = a
+        return np.stack([x, y, z, w], 1), np.stack([vx, vy, vz, sp], 1)
+
     def set_drive(self, dx=None, dy=None):
         if dx is None:
             check(lib().swarm_set_drive(self.h, None, None))
