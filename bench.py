@@ -216,6 +216,61 @@ def venue_bench(name="L", n=65536, steps=100, warmup=5, use_graph=True, cache_di
     return res
 
 
+def seated_agents(name, n, load=1.0, seed=SEED):
+    """The reference's seated crowd (spawnSeatedCrowd, src/crowd_sim.cpp:677-697) for the venue's own 3D step: GpuBoid
+    positions (x, y, z, w) at the chosen seats, at rest; the other slots parked at (1000, 1000, 1000)."""
+    x, y, z, _ = sh.venue_seats(name)
+    chosen = sh.spawn_seats(len(x), n, load, seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = 1000.0
+    live = chosen >= 0
+    pos[live, 0], pos[live, 1], pos[live, 2], pos[live, 3] = x[chosen[live]], y[chosen[live]], z[chosen[live]], 1.0
+    return pos, np.zeros((n, 4), np.float32)
+
+
+def seated_bench(scene, name="L", n=65536, steps=100, warmup=5, use_graph=True, build_s=None):
+    """The reference's own 3D step on its venue (the default venue model: floors, gravity, fence, exact walls, step-up
+    limit, queue-aware choice with the surface rule, incline rules), its seat spawn at full load, at spawn and congested
+    (t50, reached by stepping from spawn): like for like with the reference's published venue timings."""
+    pos, vel = seated_agents(name, n)
+    seated = int(pos[:, 3].sum())
+    sim = sh.Sim(scene, n, use_graph=use_graph)
+    sim.upload3(pos, vel)
+
+    def timed(state):
+        sim.step(warmup)
+        sim.sync()
+        _, act0, s0 = sim.control()
+        ms = sim.time_steps(steps)
+        _, _, s1 = sim.control()
+        sim.set_timing(True)
+        sim.step(5)
+        t = sim.timings()
+        sim.set_timing(False)
+        out = {"active_at_start": int(act0), "ms_per_step": ms / steps, "agent_steps_per_s": (s1 - s0) / (ms * 1e-3),
+               "passes_ms": dict(grid=t[0], choice=t[1], predict=t[2], projection=t[3], final=t[4], total_eager=t[5])}
+        ref = REFERENCE_VENUE_L.get(state) if name == "L" else None
+        if ref:
+            out["speedup_vs_reference_gpu"] = ref["ms_per_step"] / out["ms_per_step"]
+        return out
+
+    res = {"venue": name, "model": "the reference's own 3D step, default venue model", "agents_total": n, "seated": seated,
+           "scene_build_s": build_s, "exits": scene.info.E, "nav_cells": scene.info.nav_res_x * scene.info.nav_res_y,
+           "hash_cells": scene.info.hash_nx * scene.info.hash_ny}
+    res["spawn"] = timed("spawn")
+    sim.upload3(pos, vel)
+    chunk, stepped = 50, 0
+    while stepped < 20000:   # to t50: half of the seated crowd gone
+        sim.step(chunk)
+        stepped += chunk
+        if sim.control()[1] <= seated // 2:
+            break
+    res["congested_after_steps"] = stepped
+    res["congested"] = timed("congested")
+    sim.close()
+    return res
+
+
 def batch_bench(steps, warmup, loop_steps, use_graph=True):
     """The ensemble (256 scenarios of the room, corridor, bowl-eighth and pillar floors, about 1.4 M agents) as one
     batched step, against the same scenarios stepped by a loop over plain contexts (each scenario's steps, then a
@@ -346,7 +401,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-venue", action="store_true", help="skip the reference's venue L leg")
+    ap.add_argument("--no-venue", action="store_true", help="skip the reference's venue L legs")
+    ap.add_argument("--no-venue-plan", action="store_true", help="skip venue L's plan leg (the seated 3D leg stays)")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches in every leg (for profilers that do not follow graph launches)")
     args = ap.parse_args()
@@ -354,6 +410,21 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the step does not shard across GPUs here",
               file=sys.stderr)
         sys.exit(2)
+
+    # venue L's own 3D scene (its navigation field on the heightfield, 16 M cells x 48 exits) is built on the host while
+    # the GPU legs run; ctypes releases the interpreter during the build
+    seated_build = {}
+    if not args.no_venue:
+        import tempfile
+        import threading
+
+        def build_seated():
+            t = time.perf_counter()
+            seated_build["scene"] = sh.Scene.seated("L", threads=12, cache_dir=os.path.join(tempfile.gettempdir(), "swarm_scene_cache"))
+            seated_build["seconds"] = time.perf_counter() - t
+
+        seated_thread = threading.Thread(target=build_seated, daemon=True)
+        seated_thread.start()
 
     t0 = time.perf_counter()
     scene = sh.Scene.layout(sh.LAYOUT_BOWL)
@@ -448,7 +519,16 @@ def main():
               f"with recompute {adjoint['recompute']['cost_vs_forward']:.2f}x", file=sys.stderr, flush=True)
 
     venue = None
+    seated = None
     if not args.no_venue:
+        seated_thread.join()
+        seated = seated_bench(seated_build["scene"], "L", use_graph=not args.no_graph, build_s=seated_build["seconds"])
+        for st in ("spawn", "congested"):
+            v = seated[st]
+            print(f"# venue L (the reference's own 3D step), {st}: {v['active_at_start']} active, {v['ms_per_step']:.4f} ms/step, "
+                  f"{v['agent_steps_per_s'] / 1e6:.1f} M agent-steps/s; the reference's GPU {REFERENCE_VENUE_L[st]['ms_per_step']} ms "
+                  f"(x{v['speedup_vs_reference_gpu']:.2f})", file=sys.stderr, flush=True)
+    if not args.no_venue and not args.no_venue_plan:
         import tempfile
         venue = venue_bench("L", use_graph=not args.no_graph, cache_dir=os.path.join(tempfile.gettempdir(), "swarm_scene_cache"))
         for st in ("spawn", "congested"):
@@ -497,6 +577,7 @@ def main():
         "metrics": metrics,
         "batched": batch,
         "adjoint": adjoint,
+        "venue_l_seated": seated,
         "venue_l": venue,
         "cpu_baseline": None if cpu is None else dict(
             value=cpu["configs"]["bowl_63424"]["all_cores"]["agent_steps_per_s"], unit="agent-steps/s", cores=cpu["logical_cores"],

@@ -667,6 +667,55 @@ void putBoids(const std::string& p, const std::vector<GpuBoid>& st) {
     put(p + "vel", 'f', vel, { uint64_t(st.size()), 4 });
 }
 
+// A seated venue's navigation field and candidate table (nav::forVenue, nav::choiceForVenue) as FNV-1a checksums of the
+// cell classes, edges, exit regions, floors, the index and every entry's slots and distance words, with every 997th cell
+void putNav(const std::string& p, const Venue& venue) {
+    // the navigation field and the choice table the run used
+    const nav::Field& f = nav::forVenue(venue);
+    const nav::ChoiceTable& t = nav::choiceForVenue(venue);
+    const nav::Grid& g = f.grid;
+    put(p + "grid", 'd', std::vector<double>{ double(g.res), double(g.origin), double(g.cell), double(g.invCell) });
+    std::vector<uint32_t> slots, dists;
+    for (const GpuNavChoiceCell& c : t.entries) {
+        for (int j = 0; j < nav::CHOICE_K; ++j) slots.push_back(c.slot[j]);
+        for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) dists.push_back(c.dist[j]);
+    }
+    const uint64_t h[] = { fnv64(f.cls.data(), f.cls.size()), fnv64(f.out.data(), f.out.size()),
+                   fnv64(f.exitRegion.data(), f.exitRegion.size()), fnv64(f.floorZ.data(), f.floorZ.size() * 4),
+                   fnv64(t.index.data(), t.index.size() * 4), fnv64(slots.data(), slots.size() * 4),
+                   fnv64(dists.data(), dists.size() * 4) };
+    std::vector<uint32_t> hw;
+    for (uint64_t v : h) { hw.push_back(uint32_t(v)); hw.push_back(uint32_t(v >> 32)); }
+    put(p + "fnv", 'u', hw);
+    put(p + "sizes", 'i', std::vector<int32_t>{ int32_t(f.cls.size()), int32_t(t.entries.size()) });
+    put(p + "scalars", 'f', std::vector<float>{ t.secondsPerUnit, t.distQuantum });
+    put(p + "inv_cap_s", 'f', t.invCapS);
+    std::vector<int32_t> cells;
+    std::vector<uint32_t> info, sl, ds;
+    std::vector<uint8_t> cls, out;
+    std::vector<int32_t> er;
+    std::vector<float> fz;
+    for (size_t i = 0; i < f.cls.size(); i += 997) {
+        cells.push_back(int32_t(i));
+        cls.push_back(f.cls[i]);
+        out.push_back(f.out[i]);
+        er.push_back(f.exitRegion[i]);
+        fz.push_back(f.floorZ[i]);
+        info.push_back(t.index[i]);
+        const GpuNavChoiceCell& c = t.at(i);
+        for (int j = 0; j < nav::CHOICE_K; ++j) sl.push_back(c.slot[j]);
+        for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) ds.push_back(c.dist[j]);
+    }
+    put(p + "sample_cell", 'i', cells);
+    put(p + "sample_cls", 'b', cls);
+    put(p + "sample_out", 'b', out);
+    put(p + "sample_exit_region", 'i', er);
+    put(p + "sample_floor", 'f', fz);
+    put(p + "sample_index", 'u', info);
+    put(p + "sample_slots", 'u', sl, { uint64_t(cells.size()), uint64_t(nav::CHOICE_K) });
+    put(p + "sample_dists", 'u', ds, { uint64_t(cells.size()), uint64_t(nav::CHOICE_DIST_WORDS) });
+}
+
 // The reference's seated 3D step (CpuCrowdSim on an analytic venue: floors, gravity, the fence, the exact walls, the
 // step-up limit) on venue S, its 100 % seat spawn in 4,096 slots (3,728 seated), counting sort:
 //   seated/S/drive/    a forced drive per agent (setDriveTable: seeded random directions), 40 steps
@@ -728,50 +777,19 @@ void seatedCases() {
                 ++k;
             }
         }
-        // the navigation field and the choice table the run used
-        const nav::Field& f = nav::forVenue(venue);
-        const nav::ChoiceTable& t = nav::choiceForVenue(venue);
-        const nav::Grid& g = f.grid;
-        put("seated/S/nav/grid", 'd', std::vector<double>{ double(g.res), double(g.origin), double(g.cell), double(g.invCell) });
-        std::vector<uint32_t> slots, dists;
-        for (const GpuNavChoiceCell& c : t.entries) {
-            for (int j = 0; j < nav::CHOICE_K; ++j) slots.push_back(c.slot[j]);
-            for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) dists.push_back(c.dist[j]);
-        }
-        const uint64_t h[] = { fnv64(f.cls.data(), f.cls.size()), fnv64(f.out.data(), f.out.size()),
-                               fnv64(f.exitRegion.data(), f.exitRegion.size()), fnv64(f.floorZ.data(), f.floorZ.size() * 4),
-                               fnv64(t.index.data(), t.index.size() * 4), fnv64(slots.data(), slots.size() * 4),
-                               fnv64(dists.data(), dists.size() * 4) };
-        std::vector<uint32_t> hw;
-        for (uint64_t v : h) { hw.push_back(uint32_t(v)); hw.push_back(uint32_t(v >> 32)); }
-        put("seated/S/nav/fnv", 'u', hw);
-        put("seated/S/nav/sizes", 'i', std::vector<int32_t>{ int32_t(f.cls.size()), int32_t(t.entries.size()) });
-        put("seated/S/nav/scalars", 'f', std::vector<float>{ t.secondsPerUnit, t.distQuantum });
-        put("seated/S/nav/inv_cap_s", 'f', t.invCapS);
-        std::vector<int32_t> cells;
-        std::vector<uint32_t> info, sl, ds;
-        std::vector<uint8_t> cls, out;
-        std::vector<int32_t> er;
-        std::vector<float> fz;
-        for (size_t i = 0; i < f.cls.size(); i += 997) {
-            cells.push_back(int32_t(i));
-            cls.push_back(f.cls[i]);
-            out.push_back(f.out[i]);
-            er.push_back(f.exitRegion[i]);
-            fz.push_back(f.floorZ[i]);
-            info.push_back(t.index[i]);
-            const GpuNavChoiceCell& c = t.at(i);
-            for (int j = 0; j < nav::CHOICE_K; ++j) sl.push_back(c.slot[j]);
-            for (int j = 0; j < nav::CHOICE_DIST_WORDS; ++j) ds.push_back(c.dist[j]);
-        }
-        put("seated/S/nav/sample_cell", 'i', cells);
-        put("seated/S/nav/sample_cls", 'b', cls);
-        put("seated/S/nav/sample_out", 'b', out);
-        put("seated/S/nav/sample_exit_region", 'i', er);
-        put("seated/S/nav/sample_floor", 'f', fz);
-        put("seated/S/nav/sample_index", 'u', info);
-        put("seated/S/nav/sample_slots", 'u', sl, { uint64_t(cells.size()), uint64_t(nav::CHOICE_K) });
-        put("seated/S/nav/sample_dists", 'u', ds, { uint64_t(cells.size()), uint64_t(nav::CHOICE_DIST_WORDS) });
+        putNav("seated/S/nav/", venue);
+    }
+    {   // venue M (two exit types beyond CHOICE_K exits: the type rule), its 100 % seat spawn in 16,384 slots: 10 steps
+        const Venue& m = venueByName("M");
+        const std::vector<GpuBoid> initM = spawnSeatedCrowd(m, 16384, 1.0, 1337u, &active);
+        putBoids("seated/M/init/", initM);
+        CpuCrowdSim sim(m, std::vector<float>{}, 0, CpuCrowdSim::SortMode::Counting);
+        sim.setState(initM);
+        for (int step = 1; step <= 10; ++step) sim.step();
+        putBoids("seated/M/default/step10/", sim.state());
+        put("seated/M/default/step10/contacts", 'b', sim.contacts());
+        putChoice("seated/M/default/step10/", sim.choiceState());
+        putNav("seated/M/nav/", m);
     }
 }
 

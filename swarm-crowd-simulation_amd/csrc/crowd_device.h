@@ -37,6 +37,7 @@ struct SceneView {
     const seated::Layout* venue;       // a seated venue: its layout (a plan's incline rules, or the 3D step), else null
     int venue3d;                       // the venue's own 3D step: the surface rule in choiceCell, the venue's wall term
     float exitDepth, exitMaxZ, domainMax;   // its exit test
+    seated::CellTable venueCells;      // the venue's cells, every band's (a venue scene)
 };
 
 struct ModelParams {
@@ -144,9 +145,10 @@ __device__ inline bool surfaceReaches(const SceneView& s, V2 p, uint32_t a) {
     const float cx = float(a % res) + 0.5f, cy = float(a / res) + 0.5f;
     const float px = cx * s.navCell, py = cy * s.navCell;
     const seated::F2 c{ s.navOX + px, s.navOY + py }, x{ p.x, p.y };
-    if (seated::sameKey(seated::cellKey(*s.venue, c), seated::cellKey(*s.venue, x))) return true;
+    const seated::Layout& L = *s.venue;
+    if (seated::sameKey(seated::cellKeyC(L, s.venueCells, c), seated::cellKeyC(L, s.venueCells, x))) return true;
     seated::F2 n;
-    return seated::stepUpScan(*s.venue, x, c, seated::floorHeight(*s.venue, x), n) >= 1.0f;
+    return seated::stepUpScanC(L, s.venueCells, x, c, seated::floorHeightC(L, s.venueCells, x), n) >= 1.0f;
 }
 __device__ inline int choiceCell(const SceneView& s, V2 p) {
     const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;

@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
         }
         const V2 dir = choiceDirection(s, cell, p, ex);
         // the wall term: the plan's wall boxes, or on a venue's own step its exact wall at the centre's height
-        const V2 e = s.venue3d ? seatedstep::venueGoalWallRule(*s.venue, seatedstep::V3{ p.x, p.y, az[id] }, dir)
+        const V2 e = s.venue3d ? seatedstep::venueGoalWallRule(seatedstep::Venue3{ *s.venue, s.venueCells }, seatedstep::V3{ p.x, p.y, az[id] }, dir)
                                : goalWallRule(s, p, FEELER_REACH_RADII * srad[slot], dir);
         egx[slot] = e.x;
         egy[slot] = e.y;

@@ -63,6 +63,25 @@ void DeviceScene::upload(const Scene& s) {
     v.exitDepth = s.exitDepth;
     v.exitMaxZ = s.exitMaxZ;
     v.domainMax = s.domainMax;
+    v.venueCells = seated::CellTable{ nullptr, nullptr, nullptr };
+    if (s.hasVenue) {   // every band's cells, generated once (the same values the generator gives)
+        std::vector<int> start;
+        std::vector<float> l1;
+        std::vector<int4> cell;
+        const int nb = seated::bandCount(s.venue);
+        for (int j = 0; j < nb; ++j) {
+            start.push_back(int(l1.size()));
+            seated::Cell c{};
+            for (int i = 0; seated::cellAt(s.venue, j, i, c); ++i) {
+                l1.push_back(c.l1);
+                cell.push_back(make_int4(__builtin_bit_cast(int, c.l0), __builtin_bit_cast(int, c.l1), c.kind, c.row));
+            }
+        }
+        start.push_back(int(l1.size()));
+        v.venueCells.start = static_cast<const int*>(put(start.data(), start.size() * sizeof(int)));
+        v.venueCells.l1 = static_cast<const float*>(put(l1.data(), l1.size() * sizeof(float)));
+        v.venueCells.cell = static_cast<const int4*>(put(cell.data(), cell.size() * sizeof(int4)));
+    }
 }
 
 void DeviceScene::release() {

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
     if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
     const uint32_t key = b.skey[slot];
     if (key == INACTIVE_CELL) return;
-    const Layout& L = *s.venue;
+    const Venue3 V{ *s.venue, s.venueCells };
     const uint32_t self = b.sid[slot];
     const V3 p{ b.sx[slot], b.sy[slot], rd.z[self] };
     const V3 v{ rd.vx[self], rd.vy[self], rd.vz[self] };
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
             }
         }
     // the incline speed and the stair spacing by the floor's slope along e_goal (the reference's default venue model)
-    const float slope = seated::floorSlope(L, seated::F2{ p.x, p.y }, seated::F2{ eGoal.x, eGoal.y });
+    const float slope = seated::floorSlopeC(V.L, V.cs, seated::F2{ p.x, p.y }, seated::F2{ eGoal.x, eGoal.y });
     const float vFree = inclineFreeSpeed(V_MAX, slope);
     const float vc = slope != 0.0f ? (sAhead - BODY_DIAM) / inclineHeadwayTime(slope) : (sAhead - BODY_DIAM) * HEADWAY_RATE;
     const float vDes = seated::cl(vc, 0.0f, vFree) * len2(eGoal);
@@ -121,12 +121,12 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
     // obstacle avoidance TORSO_HEIGHT above the centre: the venue's distance or the fence's, the nearer
     V2 aObs{ 0.0f, 0.0f };
     const V3 torso{ p.x, p.y, p.z + TORSO_HEIGHT };
-    const float dStadium = venueMap(L, torso);
+    const float dStadium = venueMap(V, torso);
     const float dFence = fence(torso) + BODY_RADIUS;
     const bool onFence = dFence < dStadium;
     const float dTorso = onFence ? dFence : dStadium;
     if (dTorso < OBSTACLE_RANGE) {
-        const V2 norm = onFence ? fenceGrad(torso) : venueNormalXY(L, torso);
+        const V2 norm = onFence ? fenceGrad(torso) : venueNormalXY(V, torso);
         if (len2(norm) > OBSTACLE_MIN_NORMAL) aObs = safeNormalize(norm) * (OBSTACLE_RANGE - dTorso) * W_OBS * mp.obstacleGain;
     }
     const V2 vEq = eDir * vDes + aObs * mp.tau;
@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
     if (len2(xy(nv)) > V_MAX) setXY(nv, safeNormalize(xy(nv)) * V_MAX);
     V3 np{ p.x + nv.x * DT, p.y + nv.y * DT, p.z + nv.z * DT };
     V3 wall;
-    const bool grounded = resolveStatic(L, p, np, nv, false, wall);
-    if (wall.x == 0.0f && wall.y == 0.0f) wall = wallPlaneProbe(L, np);
+    const bool grounded = resolveStatic(V, p, np, nv, false, wall);
+    if (wall.x == 0.0f && wall.y == 0.0f) wall = wallPlaneProbe(V, np);
     out.p[slot] = make_float2(np.x, np.y);
     out.acc[slot] = 0.0f;
     b.pw[slot] = 1.0f;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView 
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.z[agent] = rd.z[agent]; wr.w[agent] = rd.w[agent];
             wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.vz[agent] = rd.vz[agent]; wr.spd[agent] = rd.spd[agent];
         } else {
-            const Layout& L = *s.venue;
+            const Venue3 V{ *s.venue, s.venueCells };
             const float2 p0 = in.p[slot];
             V2 p{ p0.x, p0.y };
             float acc = in.acc[slot];
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView 
                 ev.before[agent] = make_float2(wr.x[agent], wr.y[agent]);
                 wr.x[agent] = p.x; wr.y[agent] = p.y; wr.z[agent] = q.pz[slot]; wr.w[agent] = 0.0f;
                 wr.vx[agent] = 0.0f; wr.vy[agent] = 0.0f; wr.vz[agent] = 0.0f; wr.spd[agent] = acc;
-                const int e = seated::exitByLat(L, seated::F2{ sx, sy });
+                const int e = seated::exitByLat(V.L, seated::F2{ sx, sy });
                 const int ex = e >= 0 ? e : s.E;
                 ev.step[agent] = int(step);
                 ev.exit[agent] = ex;
@@ -204,9 +204,9 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView 
                 const V3 pOld{ sx, sy, rd.z[agent] };
                 if (acc > 0.0f) {
                     V3 wall;
-                    resolveStatic(L, pOld, np, nv, b.planes[slot].w >= 2.0f, wall);   // PLANE_GROUNDED
+                    resolveStatic(V, pOld, np, nv, b.planes[slot].w >= 2.0f, wall);   // PLANE_GROUNDED
                 }
-                limitStepUp(L, pOld, np, nv);
+                limitStepUp(V, pOld, np, nv);
                 wr.x[agent] = np.x; wr.y[agent] = np.y; wr.z[agent] = np.z; wr.w[agent] = 1.0f;
                 wr.vx[agent] = nv.x; wr.vy[agent] = nv.y; wr.vz[agent] = nv.z; wr.spd[agent] = len3(nv);
             }

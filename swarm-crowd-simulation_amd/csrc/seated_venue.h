@@ -230,20 +230,52 @@ __host__ __device__ inline int ownCell(const Layout& L, int j, float lat, Cell& 
     return i - 1;
 }
 
-__host__ __device__ inline float floorHeight(const Layout& L, F2 p) {
+// Where the queries below take their cells from: the generator above (host and device), or a table of every band's cells
+// made from it once (CellTable: the same values, read instead of generated; the device step uses it)
+struct GenCells {
+    const Layout& L;
+    __host__ __device__ bool at(int j, int c, Cell& out) const { return cellAt(L, j, c, out); }
+    __host__ __device__ int own(int j, float lat, Cell& out) const { return ownCell(L, j, lat, out); }
+};
+// Every band's cells in order: band j's are [start[j], start[j + 1]); per cell its l1 alone (the lateral search) and its
+// whole record (l0, l1 as float bits, kind, row)
+struct CellTable {
+    const int* start;
+    const float* l1;
+    const int4* cell;
+    __host__ __device__ bool at(int j, int c, Cell& out) const {
+        const int i = start[j] + c;
+        if (i >= start[j + 1]) return false;
+        const int4 v = cell[i];
+        out.l0 = __builtin_bit_cast(float, v.x);
+        out.l1 = __builtin_bit_cast(float, v.y);
+        out.kind = v.z;
+        out.row = v.w;
+        return true;
+    }
+    __host__ __device__ int own(int j, float lat, Cell& out) const {
+        const int a = start[j], b = start[j + 1];
+        int i = a;
+        while (i < b - 1 && !(lat < l1[i])) ++i;
+        at(j, i - a, out);
+        return i - a;
+    }
+};
+
+template <class C> __host__ __device__ inline float floorHeightC(const Layout& L, const C& cs, F2 p) {
     const F2 f = fold(p);
     const int j = bandOf(L, f.y);
     Cell c{};
-    ownCell(L, j, f.x, c);
+    cs.own(j, f.x, c);
     return cellTop(L, c.kind, c.row, f.y);
 }
 
 // Kind, row and band of the cell under p
-__host__ __device__ inline int kindAt(const Layout& L, F2 p, int* row, int* bandIndex) {
+template <class C> __host__ __device__ inline int kindAtC(const Layout& L, const C& cs, F2 p, int* row, int* bandIndex) {
     const F2 f = fold(p);
     const int j = bandOf(L, f.y);
     Cell c{};
-    ownCell(L, j, f.x, c);
+    cs.own(j, f.x, c);
     *row = c.row;
     *bandIndex = j;
     return c.kind;
@@ -296,13 +328,13 @@ __host__ __device__ inline F4 nearestInCell(const Layout& L, float lo, float hi,
 // The exact signed distance to the heightfield {z <= floorHeight(xy)} and its unit gradient: (gx, gy, gz, d).
 // Bands are searched outward from p's own band until the band gap exceeds the best distance, at most maxBandSteps on
 // either side; the cells of a band are scanned in lat order, stopping at the first one beyond the best distance.
-__host__ __device__ inline F4 sdfGrad(const Layout& L, F3 p) {
+template <class C> __host__ __device__ inline F4 sdfGradC(const Layout& L, const C& cs, F3 p) {
     const F2 f = fold(F2{ p.x, p.y });
     const F3 pf{ f.x, f.y, p.z };
     const int nb = bandCount(L);
     const int jp = bandOf(L, pf.y);
     Cell own{};
-    ownCell(L, jp, pf.x, own);
+    cs.own(jp, pf.x, own);
     const bool inAir = pf.z > cellTop(L, own.kind, own.row, pf.y);
     float best = BIG;
     F3 bestQ = pf;
@@ -312,7 +344,7 @@ __host__ __device__ inline F4 sdfGrad(const Layout& L, F3 p) {
         float lo, hi;
         band(L, j, lo, hi);
         Cell c{};
-        for (int i = 0; cellAt(L, j, i, c); ++i) {
+        for (int i = 0; cs.at(j, i, c); ++i) {
             const float lg = mx(mx(c.l0 - pf.x, pf.x - c.l1), 0.0f);
             if (g2 + lg * lg >= best) {
                 if (c.l0 > pf.x) break;
@@ -371,12 +403,12 @@ __host__ __device__ inline F4 sdfGrad(const Layout& L, F3 p) {
 // The nearest wall in the horizontal plane z = c.z: walls are the footprints of the cells whose top lies above the
 // plane (inclines are walkable slopes, never walls). (unit direction away from the wall, clearance); clearance < 0:
 // c is inside a wall; NO_WALL: none within reach.
-__host__ __device__ inline F3 wall(const Layout& L, F3 cpt, float reach) {
+template <class C> __host__ __device__ inline F3 wallC(const Layout& L, const C& cs, F3 cpt, float reach) {
     const F2 f = fold(F2{ cpt.x, cpt.y });
     const int nb = bandCount(L);
     const int jp = bandOf(L, f.y);
     Cell own{};
-    ownCell(L, jp, f.x, own);
+    cs.own(jp, f.x, own);
     const bool inWall = own.kind != INCLINE && cellTop(L, own.kind, own.row, f.y) > cpt.z;
     float best = inWall ? BIG : reach * reach;
     F2 bestQ = f;
@@ -386,7 +418,7 @@ __host__ __device__ inline F3 wall(const Layout& L, F3 cpt, float reach) {
         float lo, hi;
         band(L, j, lo, hi);
         Cell c{};
-        for (int i = 0; cellAt(L, j, i, c); ++i) {
+        for (int i = 0; cs.at(j, i, c); ++i) {
             const bool isWall = c.kind != INCLINE && cellTop(L, c.kind, c.row, lo) > cpt.z;
             if (isWall == inWall) continue;
             const F2 q{ mn(mx(f.x, c.l0), c.l1), mn(mx(f.y, lo), hi) };
@@ -433,11 +465,11 @@ __host__ __device__ inline F3 wall(const Layout& L, F3 cpt, float reach) {
 
 // The floor's rise per unit of horizontal travel along the plan direction e at p: on an incline its gradient (rake
 // along dep) projected on e, 0 on every level floor
-__host__ __device__ inline float floorSlope(const Layout& L, F2 p, F2 e) {
+template <class C> __host__ __device__ inline float floorSlopeC(const Layout& L, const C& cs, F2 p, F2 e) {
     const F2 f = fold(p);
     const int j = bandOf(L, f.y);
     Cell c{};
-    ownCell(L, j, f.x, c);
+    cs.own(j, f.x, c);
     if (c.kind != INCLINE) return 0.0f;
     const F2 g = unfoldDir(p, F2{ 0.0f, L.rake });
     const float a = g.x * e.x, b = g.y * e.y;
@@ -462,12 +494,12 @@ __host__ __device__ inline int exitByLat(const Layout& L, F2 p) {
 
 // The gangway under p (an incline, or a barrier gap at a gangway's foot): per quadrant the y wing's straight gangways,
 // the x wing's, then the corner gangway (a straight gangway merged with it counts as the corner gangway); -1: none
-__host__ __device__ inline int gangwayAt(const Layout& L, F2 p) {
+template <class C> __host__ __device__ inline int gangwayAtC(const Layout& L, const C& cs, F2 p) {
     const F2 f = fold(p);
     const int j = bandOf(L, f.y);
     if (j < 1 || j >= bandCount(L) - 4) return -1;
     Cell c{};
-    const int ci = ownCell(L, j, f.x, c);
+    const int ci = cs.own(j, f.x, c);
     if (!(c.kind == INCLINE || (j == 1 && c.kind == GROUND && ci >= 2))) return -1;
     const int r = (j - 1) >> 1;
     const int n = activeGangways(L, r);
@@ -482,11 +514,11 @@ __host__ __device__ inline int gangwayAt(const Layout& L, F2 p) {
 
 // The cell under p as a key (its band and its lower lateral bound): two points share a cell iff their keys are equal
 // (seated::cellKey, src/venue.cpp:631-638)
-__host__ __device__ inline F2 cellKey(const Layout& L, F2 p) {
+template <class C> __host__ __device__ inline F2 cellKeyC(const Layout& L, const C& cs, F2 p) {
     const F2 f = fold(p);
     const int j = bandOf(L, f.y);
     Cell c{};
-    ownCell(L, j, f.x, c);
+    cs.own(j, f.x, c);
     return F2{ float(j), c.l0 };
 }
 
@@ -506,27 +538,27 @@ __host__ __device__ inline bool sameKey(F2 a, F2 b) { return a.x == b.x && a.y =
 // (every STEP_UP_SAMPLE, each boundary located by bisection), the move is blocked at the first boundary into a floor
 // above f0 + STEP_UP_TOL that rises faster than the steepest incline along the move; returns the allowed fraction (1: not
 // blocked) and n, the blocking face's normal away from it (seated::stepUpScan, src/venue.cpp:807-835)
-__host__ __device__ inline float stepUpScan(const Layout& L, F2 p0, F2 p1, float f0, F2& n) {
+template <class C> __host__ __device__ inline float stepUpScanC(const Layout& L, const C& cs, F2 p0, F2 p1, float f0, F2& n) {
     n = F2{ 0.0f, 0.0f };
     const float dx = p1.x - p0.x, dy = p1.y - p0.y;
     const float len = sqrtf(dx * dx + dy * dy);
     const int ns = cli(int(ceilf(len / STEP_UP_SAMPLE)), 1, STEP_UP_MAX_SAMPLES);
     float lo = 0.0f;
-    F2 key = cellKey(L, p0);
+    F2 key = cellKeyC(L, cs, p0);
     for (int i = 1; i <= ns; ++i) {
         const float t = float(i) / float(ns);
         for (int k = 0; k < SCAN_CROSSINGS; ++k) {
-            if (sameKey(cellKey(L, lerp2(p0, p1, t)), key)) break;
+            if (sameKey(cellKeyC(L, cs, lerp2(p0, p1, t)), key)) break;
             float a = lo, b = t;
             for (int m = 0; m < SCAN_BISECTIONS; ++m) {
                 const float tm = 0.5f * (a + b);
-                if (sameKey(cellKey(L, lerp2(p0, p1, tm)), key)) a = tm;
+                if (sameKey(cellKeyC(L, cs, lerp2(p0, p1, tm)), key)) a = tm;
                 else b = tm;
             }
             const F2 pa = lerp2(p0, p1, a), pb = lerp2(p0, p1, b);
-            const float ha = floorHeight(L, pa), hb = floorHeight(L, pb);
+            const float ha = floorHeightC(L, cs, pa), hb = floorHeightC(L, cs, pb);
             if (hb > f0 + STEP_UP_TOL && hb - ha > L.rake * len * (b - a) + STEP_UP_TOL) {
-                const F3 w = wall(L, F3{ pa.x, pa.y, ha + STEP_UP_TOL }, STEP_UP_SAMPLE);
+                const F3 w = wallC(L, cs, F3{ pa.x, pa.y, ha + STEP_UP_TOL }, STEP_UP_SAMPLE);
                 if (w.z < NO_WALL) n = F2{ w.x, w.y };
                 else {
                     const float l = mx(len, 1e-30f);
@@ -535,7 +567,7 @@ __host__ __device__ inline float stepUpScan(const Layout& L, F2 p0, F2 p1, float
                 return a;
             }
             lo = b;
-            key = cellKey(L, pb);
+            key = cellKeyC(L, cs, pb);
         }
         lo = t;
     }
@@ -571,6 +603,16 @@ __host__ __device__ inline F2 sdFenceGradient(F2 p) {
     const F2 g = sdBox2Grad(rel, FENCE_HALF_MID);
     return F2{ s * g.x, s * g.y };
 }
+
+// The queries on the generator (the host, the probe kernel)
+__host__ __device__ inline float floorHeight(const Layout& L, F2 p) { return floorHeightC(L, GenCells{ L }, p); }
+__host__ __device__ inline int kindAt(const Layout& L, F2 p, int* row, int* bandIndex) { return kindAtC(L, GenCells{ L }, p, row, bandIndex); }
+__host__ __device__ inline F4 sdfGrad(const Layout& L, F3 p) { return sdfGradC(L, GenCells{ L }, p); }
+__host__ __device__ inline F3 wall(const Layout& L, F3 cpt, float reach) { return wallC(L, GenCells{ L }, cpt, reach); }
+__host__ __device__ inline float floorSlope(const Layout& L, F2 p, F2 e) { return floorSlopeC(L, GenCells{ L }, p, e); }
+__host__ __device__ inline int gangwayAt(const Layout& L, F2 p) { return gangwayAtC(L, GenCells{ L }, p); }
+__host__ __device__ inline F2 cellKey(const Layout& L, F2 p) { return cellKeyC(L, GenCells{ L }, p); }
+__host__ __device__ inline float stepUpScan(const Layout& L, F2 p0, F2 p1, float f0, F2& n) { return stepUpScanC(L, GenCells{ L }, p0, p1, f0, n); }
 
 }  // namespace seated
 }  // namespace swarm

@@ -5,7 +5,8 @@ bit for bit by the HIP step: positions with heights, velocities with vertical co
 random drive of 40 steps and after the default model's first 200 steps (with its sorted grid, assigned exits and choice
 counters). Then venue S runs to its last evacuee against the oracle (whose seated step is the reference's, pinned by the
 same fixtures on the CPU): the state every 250 steps and every agent's evacuation step and exit, bit for bit; the last
-seated agent leaves at step index 1,360, the reference's published CPU run (results/RESULTS.md:90: step 1361).
+seated agent leaves at step index 1,360, the reference's published CPU run (results/RESULTS.md:90: step 1361). Venue M
+(32 exits: the exit-type rule of its candidate table) after 10 default steps, bit for bit the reference's.
 """
 import os
 import tempfile
@@ -100,6 +101,19 @@ def test_seated_s_evacuation_matches_oracle(kat, scene_s):
     assert np.array_equal(gs, cs) and np.array_equal(ge, ce)
     assert gs[active].max() == 1360   # the reference's CPU run: its last evacuee at step 1361 (1-based)
     assert per[:12].sum() == active.sum() and (per[:12] > 0).all()
+    sim.close()
+
+
+def test_seated_m_matches_reference(kat):
+    """Venue M (32 exits: the candidate table's exit-type rule), 15,832 seated in 16,384 slots: 10 default steps."""
+    print("venue M seated scene: building", flush=True)
+    scene = sh.Scene.seated("M", cache_dir=CACHE)
+    sim = sh.Sim(scene, 16384)
+    sim.upload3(kat["seated__M__init__pos"], kat["seated__M__init__vel"])
+    sim.step(10)
+    p = "seated__M__default__step10__"
+    assert_same(sim, kat, p, "venue M step 10")
+    assert np.array_equal(sim.assign(), kat[p + "assign"]) and np.array_equal(sim.choice_counters(), kat[p + "counters"])
     sim.close()
 
 
