@@ -39,6 +39,7 @@ constexpr int CTRL_STEP = 0;            // steps since the last upload (the exit
 constexpr int CTRL_NACTIVE = 1;         // active agents at the current step's input
 constexpr int CTRL_ACTIVE_SUM_LO = 2;   // running sum of active agents over steps (uint64, low word first)
 constexpr int CTRL_DONE_BLOCKS = 4;     // blocks of the final pass that have finished (reset by the last one)
+constexpr int CTRL_LIST_BROKEN = 5;     // a seated step's agent moved past its candidate lists' budget (cleared by predict)
 constexpr int CTRL_WORDS = 8;
 // The counting sort ranks an agent within its cell by scanning the cell's slots: at most this many (a step's cells
 // hold a few hard discs; swarm_sort_grid's caller keys beyond it take the stable network instead)
@@ -191,6 +192,7 @@ struct StepLaunch {
     int collectLanes = 1; // lanes per slot in the first projection iteration (1, 2, 4 or 8)
     int predictLanes = 1; // lanes per slot in predict (1, 2, 4 or 8)
     bool bitonic = false; // the sort: the reference GPU's bitonic network instead of the stable counting sort
+    float listBudget = 1.0f;  // a seated venue's own step: its candidate lists' displacement budget (0: no lists)
 };
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,

@@ -2,6 +2,7 @@
 // src/crowd_sim.cpp:316-412: hash, sort, cells + gather, exit choice, predict, PROJ_ITERATIONS - 1 projection
 // iterations, final; then the ping-pong halves swap).
 #include "hip_crowd_sim.hpp"
+#include "venue.hpp"
 
 #include <charconv>
 #include <cmath>
@@ -63,24 +64,14 @@ void DeviceScene::upload(const Scene& s) {
     v.exitDepth = s.exitDepth;
     v.exitMaxZ = s.exitMaxZ;
     v.domainMax = s.domainMax;
-    v.venueCells = seated::CellTable{ nullptr, nullptr, nullptr };
+    v.venueCells = seated::CellTable{ nullptr, nullptr, nullptr, nullptr, nullptr };
     if (s.hasVenue) {   // every band's cells, generated once (the same values the generator gives)
-        std::vector<int> start;
-        std::vector<float> l1;
-        std::vector<int4> cell;
-        const int nb = seated::bandCount(s.venue);
-        for (int j = 0; j < nb; ++j) {
-            start.push_back(int(l1.size()));
-            seated::Cell c{};
-            for (int i = 0; seated::cellAt(s.venue, j, i, c); ++i) {
-                l1.push_back(c.l1);
-                cell.push_back(make_int4(__builtin_bit_cast(int, c.l0), __builtin_bit_cast(int, c.l1), c.kind, c.row));
-            }
-        }
-        start.push_back(int(l1.size()));
-        v.venueCells.start = static_cast<const int*>(put(start.data(), start.size() * sizeof(int)));
-        v.venueCells.l1 = static_cast<const float*>(put(l1.data(), l1.size() * sizeof(float)));
-        v.venueCells.cell = static_cast<const int4*>(put(cell.data(), cell.size() * sizeof(int4)));
+        const CellTableData t = makeCellTable(s.venue);
+        v.venueCells.start = static_cast<const int*>(put(t.start.data(), t.start.size() * sizeof(int)));
+        v.venueCells.l1 = static_cast<const float*>(put(t.l1.data(), t.l1.size() * sizeof(float)));
+        v.venueCells.cell = static_cast<const int4*>(put(t.cell.data(), t.cell.size() * sizeof(int4)));
+        v.venueCells.bucket = static_cast<const int*>(put(t.bucket.data(), t.bucket.size() * sizeof(int)));
+        v.venueCells.invW = static_cast<const float*>(put(t.invW.data(), t.invW.size() * sizeof(float)));
     }
 }
 
@@ -245,6 +236,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.collectLanes = collectLanes_;
     L.predictLanes = predictLanes_;
     L.bitonic = bitonic_;
+    L.listBudget = listBudget_;
     return L;
 }
 
@@ -384,6 +376,12 @@ void HipCrowdSim::setSortMode(int mode) {
         throw std::invalid_argument("setSortMode: the bitonic network needs a power-of-two agent count");
     if ((mode == 1) != bitonic_) dropGraph();
     bitonic_ = mode == 1;
+}
+
+void HipCrowdSim::setListBudget(float fraction) {
+    if (!(fraction >= 0.0f && fraction <= 1.0f)) throw std::invalid_argument("setListBudget: fraction must be in [0, 1]");
+    if (fraction != listBudget_) dropGraph();
+    listBudget_ = fraction;
 }
 
 void HipCrowdSim::setPruneEvery(int every) {

@@ -74,6 +74,9 @@ public:
     int physicsGrid() const { return physGrid_; }
     // Candidate pruning in the projection every 'every' iterations (0: off); results are the same bit for bit
     void setPruneEvery(int every);
+    // A seated venue's own 3D step: the displacement budget of the projection's candidate lists, a fraction of P - 1
+    // maximal moves (1 default; 0: no lists, the ring every iteration); results are the same bit for bit
+    void setListBudget(float fraction);
     // Lanes per slot in the projection's non-pruning iterations (1, 2, 4, 8); results are the same bit for bit
     void setProjectionLanes(int lanes);
     int projectionLanes() const { return projLanes_; }
@@ -207,6 +210,7 @@ private:
     int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
     int physBlock_ = 0, physGrid_ = 0;
     int pruneEvery_ = PROJ_PRUNE_EVERY;
+    float listBudget_ = 1.0f;
     int projLanes_ = PROJ_LANES;
     int collectLanes_ = PROJ_COLLECT_LANES;
     int predictLanes_ = PRED_LANES;

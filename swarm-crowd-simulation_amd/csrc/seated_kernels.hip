@@ -17,15 +17,30 @@ constexpr int SEATED_BLOCK = 128;
 
 inline int blocksOf(int n, int b) { return (n + b - 1) / b; }
 
-// One Jacobi iteration for one slot (cpu_crowd_sim.cpp:780-820): the contacts of the 3 x 3 ring's slots whose heights
-// overlap, the mean correction capped at PROJ_MAX_SHIFT, the fence, the slot's wall plane; p and acc are updated
-__device__ inline void projectSeated(const SceneView& s, const StepBuffers& b, const SeatedSlots& q, ProjBuffers in, int slot,
-                                     uint32_t key, V2& p, float& acc) {
+constexpr int SEATED_LIST_CHUNK = 4;   // list candidates whose loads one thread issues together
+
+// The contact candidates (the plain step's list buffers, slot-interleaved: index nbr[j * n + slot], height offset
+// nbrD[j * n + slot]). The first iteration's ring scan keeps, in ring order, every slot with a height overlap (heights
+// stay fixed through the projection) whose centre lies within D + 2 B of the slot's (predict's positions), B the
+// displacement budget: an iteration's contacts are tested at positions at most P - 1 moves from predict's, a move is
+// at most PROJ_MAX_SHIFT unless the fence or the wall plane pushes further, and every iteration checks its slots'
+// path lengths (acc, the sum of the moves) against B. A slot past it sets CTRL_LIST_BROKEN and the iterations after
+// it scan the ring again; so a list holds, in the ring's order, every slot that can touch, and the sums are the ring
+// scan's bit for bit.
+struct ListBudget {
+    int on;         // 0: every iteration scans the ring (no lists)
+    float budget;   // B, model units
+    float reach2;   // the collection's squared reach, ((D + 2 B) 1.001 + 1e-6)^2: FP32 rounding of the tests absorbed
+};
+
+// The contacts of the 3 x 3 ring's slots whose heights overlap (cpu_crowd_sim.cpp:780-820's neighbour loop), in ring
+// order. COLLECT: also the slot's candidate list
+template <bool COLLECT>
+__device__ inline void ringContactsSeated(const SceneView& s, const StepBuffers& b, const SeatedSlots& q, ProjBuffers in, int n, int slot,
+                                          uint32_t key, uint32_t self, V2 p, V2& corr, int& nContacts, float reach2 = 0.0f) {
     const float z = q.pz[slot];
-    const uint32_t self = b.sid[slot];
     const int cx = int(key % uint32_t(s.hashNX)), cy = int(key / uint32_t(s.hashNX));
-    V2 corr{ 0.0f, 0.0f };
-    int nContacts = 0;
+    uint32_t cnt = 0;
     for (int yy = -s.projRing; yy <= s.projRing; yy++)
         for (int xx = -s.projRing; xx <= s.projRing; xx++) {
             const int nx = cx + xx, ny = cy + yy;
@@ -36,18 +51,68 @@ __device__ inline void projectSeated(const SceneView& s, const StepBuffers& b, c
                 if (int(k) == slot) continue;
                 const float2 o = in.p[k];
                 const V3 offset{ p.x - o.x, p.y - o.y, z - q.pz[k] };
-                if (b.pw[k] != 0.0f && fabsf(offset.z) < BODY_DIAM && dot2(xy(offset), xy(offset)) < BODY_DIAM2) {
+                if (b.pw[k] == 0.0f || !(fabsf(offset.z) < BODY_DIAM)) continue;
+                const float d2 = dot2(xy(offset), xy(offset));
+                if (d2 < BODY_DIAM2) {
                     corr = corr + contact3(offset, self, b.sid[k]);
                     ++nContacts;
                 }
+                if constexpr (COLLECT) {
+                    if (d2 < reach2) {
+                        if (cnt < uint32_t(NBR_CAP)) {
+                            b.nbr[size_t(cnt) * size_t(n) + size_t(slot)] = k;
+                            b.nbrD[size_t(cnt) * size_t(n) + size_t(slot)] = offset.z;
+                        }
+                        ++cnt;
+                    }
+                }
             }
         }
+    if constexpr (COLLECT) b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
+}
+
+// the same contacts from the slot's candidate list (cnt entries)
+__device__ inline void listContactsSeated(const StepBuffers& b, ProjBuffers in, int n, int slot, uint32_t self, uint32_t cnt, V2 p,
+                                          V2& corr, int& nContacts) {
+    for (uint32_t j0 = 0; j0 < cnt; j0 += SEATED_LIST_CHUNK) {
+        uint32_t kk[SEATED_LIST_CHUNK];
+        float dz[SEATED_LIST_CHUNK];
+        float2 o[SEATED_LIST_CHUNK];
+#pragma unroll
+        for (int u = 0; u < SEATED_LIST_CHUNK; ++u) {
+            kk[u] = uint32_t(slot);
+            dz[u] = 0.0f;
+            if (j0 + u < cnt) {
+                kk[u] = b.nbr[size_t(j0 + u) * size_t(n) + size_t(slot)];
+                dz[u] = b.nbrD[size_t(j0 + u) * size_t(n) + size_t(slot)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SEATED_LIST_CHUNK; ++u) o[u] = in.p[kk[u]];
+#pragma unroll
+        for (int u = 0; u < SEATED_LIST_CHUNK; ++u) {
+            if (j0 + u >= cnt) break;
+            const V3 offset{ p.x - o[u].x, p.y - o[u].y, dz[u] };
+            if (dot2(xy(offset), xy(offset)) < BODY_DIAM2) {
+                corr = corr + contact3(offset, self, b.sid[kk[u]]);
+                ++nContacts;
+            }
+        }
+    }
+}
+
+// The rest of one Jacobi iteration for one slot (cpu_crowd_sim.cpp:780-820): the mean correction capped at
+// PROJ_MAX_SHIFT, the fence, the slot's wall plane; p and acc are updated
+__device__ inline void finishSeated(const StepBuffers& b, int slot, V2 corr, int nContacts, V2& p, float& acc) {
     corr = corr * (PROJ_RELAX / float(nContacts > 1 ? nContacts : 1));
     const float len = len2(corr);
     if (len > PROJ_MAX_SHIFT) corr = corr * (PROJ_MAX_SHIFT / len);
     V2 r = p + corr;
     const float dFence = seated::sdFence(seated::F2{ r.x, r.y });
-    if (dFence < 0.0f) r = r - fenceGrad(V3{ r.x, r.y, z }) * dFence;
+    if (dFence < 0.0f) {
+        const seated::F2 g = seated::sdFenceGradient(seated::F2{ r.x, r.y });
+        r = r - V2{ g.x, g.y } * dFence;
+    }
     const float4 pl = b.planes[slot];
     if (fmodf(pl.w, 2.0f) == 1.0f) {   // PLANE_WALL
         const float sg = pl.z - dot2(r, V2{ pl.x, pl.y });
@@ -58,11 +123,30 @@ __device__ inline void projectSeated(const SceneView& s, const StepBuffers& b, c
     acc += len2(d);
 }
 
+// One Jacobi iteration for one slot: COLLECT scans the ring and writes the slot's candidate list (the first
+// iteration), LISTS walks the list (the ring again where the lists broke or the slot's overflowed), RING scans the ring
+enum class ProjMode { Ring, Collect, Lists };
+__device__ inline void projectSeated(ProjMode mode, bool broken, const SceneView& s, const StepBuffers& b, const SeatedSlots& q,
+                                     ProjBuffers in, int n, int slot, uint32_t key, float reach2, V2& p, float& acc) {
+    const uint32_t self = b.sid[slot];
+    V2 corr{ 0.0f, 0.0f };
+    int nContacts = 0;
+    if (mode == ProjMode::Collect) {
+        ringContactsSeated<true>(s, b, q, in, n, slot, key, self, p, corr, nContacts, reach2);
+    } else {
+        const uint32_t cnt = mode == ProjMode::Lists && !broken ? b.nbrCount[slot] : NBR_OVERFLOW;
+        if (cnt == NBR_OVERFLOW) ringContactsSeated<false>(s, b, q, in, n, slot, key, self, p, corr, nContacts);
+        else listContactsSeated(b, in, n, slot, self, cnt, p, corr, nContacts);
+    }
+    finishSeated(b, slot, corr, nContacts, p, acc);
+}
+
 // PHYSICS_PASS 1 on the venue (cpu_crowd_sim.cpp:655-777): the exit test, separation and the headway rule over the
 // 3 x 3 ring (3D offsets), the incline rules, obstacle avoidance at torso height against the venue and the fence, the
 // kinematics with gravity, the static contacts and the wall plane
 __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneView s, ModelParams mp, StepBuffers b, AgentHalf rd,
-                                                                SeatedSlots q, ProjBuffers out) {
+                                                                SeatedSlots q, ProjBuffers out, uint32_t* __restrict__ ctrl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[CTRL_LIST_BROKEN] = 0u;   // the step's candidate lists start whole
     const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
     if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
     const uint32_t key = b.skey[slot];
@@ -152,9 +236,10 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
     b.planes[slot] = make_float4(wn.x, wn.y, dot2(xy(np), wn) - wall.z, (nonzero(wn) ? 1.0f : 0.0f) + (grounded ? 2.0f : 0.0f));
 }
 
-// PHYSICS_PASS 2 on the venue (cpu_crowd_sim.cpp:822-829): one Jacobi iteration, in -> out
+// PHYSICS_PASS 2 on the venue (cpu_crowd_sim.cpp:822-829): Jacobi iteration it of P, in -> out. The first collects the
+// candidate lists; every iteration checks its slots' path lengths against the lists' budget
 __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_project(int n, SceneView s, StepBuffers b, SeatedSlots q, ProjBuffers in,
-                                                                ProjBuffers out) {
+                                                                ProjBuffers out, int it, ListBudget lb, uint32_t* __restrict__ ctrl) {
     const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
     if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
     const uint32_t key = b.skey[slot];
@@ -162,7 +247,15 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_project(int n, SceneVie
     const float2 p0 = in.p[slot];
     V2 p{ p0.x, p0.y };
     float acc = in.acc[slot];
-    if (b.pw[slot] != 0.0f) projectSeated(s, b, q, in, slot, key, p, acc);
+    if (b.pw[slot] != 0.0f) {
+        if (!lb.on) {
+            projectSeated(ProjMode::Ring, false, s, b, q, in, n, slot, key, 0.0f, p, acc);
+        } else {
+            const bool broken = it > 1 && ctrl[CTRL_LIST_BROKEN] != 0u;
+            projectSeated(it == 1 ? ProjMode::Collect : ProjMode::Lists, broken, s, b, q, in, n, slot, key, lb.reach2, p, acc);
+            if (!broken && acc > lb.budget) atomicOr(&ctrl[CTRL_LIST_BROKEN], 1u);
+        }
+    }
     out.p[slot] = make_float2(p.x, p.y);
     out.acc[slot] = acc;
 }
@@ -187,7 +280,10 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView 
             V2 p{ p0.x, p0.y };
             float acc = in.acc[slot];
             const float pw = b.pw[slot];
-            if (pw != 0.0f && projectHere) projectSeated(s, b, q, in, slot, key, p, acc);
+            if (pw != 0.0f && projectHere) {   // iteration P: the lists when earlier iterations collected them
+                const ProjMode mode = projectHere > 1 ? ProjMode::Lists : ProjMode::Ring;
+                projectSeated(mode, mode == ProjMode::Lists && ctrl[CTRL_LIST_BROKEN] != 0u, s, b, q, in, n, slot, key, 0.0f, p, acc);
+            }
             const float sx = b.sx[slot], sy = b.sy[slot];
             if (pw == 0.0f) {   // parked this step: an evacuation event, its exit by the last active position
                 ev.inside[agent] = make_float2(sx, sy);
@@ -231,17 +327,24 @@ void launchSeatedPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, Se
     const int n = L.n;
     const int g = blocksOf(n, SEATED_BLOCK);
     const int P = L.mp.projIters;
-    k_seated_predict<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, L.mp, b, rd, q, pa);
+    ListBudget lb{ L.listBudget > 0.0f && P >= 2 ? 1 : 0, 0.0f, 0.0f };
+    if (lb.on) {
+        lb.budget = float(P - 1) * PROJ_MAX_SHIFT * L.listBudget;
+        const float r = (BODY_DIAM + 2.0f * lb.budget) * 1.001f + 1e-6f;
+        lb.reach2 = r * r;
+    }
+    k_seated_predict<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, L.mp, b, rd, q, pa, ctrl);
     if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < P; ++it) {
-        k_seated_project<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, out);
+        k_seated_project<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, out, it, lb, ctrl);
         const ProjBuffers t = in;
         in = out;
         out = t;
     }
     if (stamps) (void)hipEventRecord(stamps[1], st);
-    k_seated_final<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, P > 0 ? 1 : 0, rd, wr, ev, ctrl);
+    // the final pass runs iteration P: on the lists when iteration 1 collected them, else on the ring
+    k_seated_final<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, lb.on ? 2 : (P > 0 ? 1 : 0), rd, wr, ev, ctrl);
 }
 
 }  // namespace swarm

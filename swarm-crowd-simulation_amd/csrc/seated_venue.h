@@ -233,16 +233,25 @@ __host__ __device__ inline int ownCell(const Layout& L, int j, float lat, Cell& 
 // Where the queries below take their cells from: the generator above (host and device), or a table of every band's cells
 // made from it once (CellTable: the same values, read instead of generated; the device step uses it)
 struct GenCells {
+    static constexpr bool sorted = false;   // the queries scan every cell of a band, as the reference does
     const Layout& L;
     __host__ __device__ bool at(int j, int c, Cell& out) const { return cellAt(L, j, c, out); }
     __host__ __device__ int own(int j, float lat, Cell& out) const { return ownCell(L, j, lat, out); }
 };
+constexpr int CELL_BUCKETS = 32;   // lateral buckets per band of a cell table
 // Every band's cells in order: band j's are [start[j], start[j + 1]); per cell its l1 alone (the lateral search) and its
-// whole record (l0, l1 as float bits, kind, row)
+// whole record (l0, l1 as float bits, kind, row). The table is built only from layouts whose bands are sorted (l0 <= l1,
+// each cell's l0 at or past the previous cell's l1; checked by the builder), so a band's l1 never decreases and the
+// queries may search it (sorted): bucket[j * CELL_BUCKETS + b] is a starting index for the lateral search of band j at
+// lat in [b w, (b + 1) w), w = 1 / invW[j]. Any start gives the same cell (the search walks both ways to the first
+// l1 above lat); a good one saves the walk.
 struct CellTable {
+    static constexpr bool sorted = true;
     const int* start;
     const float* l1;
     const int4* cell;
+    const int* bucket;
+    const float* invW;
     __host__ __device__ bool at(int j, int c, Cell& out) const {
         const int i = start[j] + c;
         if (i >= start[j + 1]) return false;
@@ -253,10 +262,14 @@ struct CellTable {
         out.row = v.w;
         return true;
     }
+    // ownCell's answer: the first cell with lat < l1, else the band's last (a NaN lat: the last)
     __host__ __device__ int own(int j, float lat, Cell& out) const {
-        const int a = start[j], b = start[j + 1];
-        int i = a;
-        while (i < b - 1 && !(lat < l1[i])) ++i;
+        const int a = start[j], last = start[j + 1] - 1;
+        const float t = lat * invW[j];
+        const int b = t >= 0.0f ? (t < float(CELL_BUCKETS) ? int(t) : CELL_BUCKETS - 1) : 0;
+        int i = bucket[j * CELL_BUCKETS + b];
+        while (i < last && !(lat < l1[i])) ++i;
+        while (i > a && lat < l1[i - 1]) --i;
         at(j, i - a, out);
         return i - a;
     }
@@ -327,7 +340,8 @@ __host__ __device__ inline F4 nearestInCell(const Layout& L, float lo, float hi,
 
 // The exact signed distance to the heightfield {z <= floorHeight(xy)} and its unit gradient: (gx, gy, gz, d).
 // Bands are searched outward from p's own band until the band gap exceeds the best distance, at most maxBandSteps on
-// either side; the cells of a band are scanned in lat order, stopping at the first one beyond the best distance.
+// either side; the cells of a band are scanned in lat order, stopping at the first one beyond the best distance (a
+// sorted cell table starts the scan at the first cell not skipped: the same cells are visited in the same order).
 template <class C> __host__ __device__ inline F4 sdfGradC(const Layout& L, const C& cs, F3 p) {
     const F2 f = fold(F2{ p.x, p.y });
     const F3 pf{ f.x, f.y, p.z };
@@ -344,7 +358,19 @@ template <class C> __host__ __device__ inline F4 sdfGradC(const Layout& L, const
         float lo, hi;
         band(L, j, lo, hi);
         Cell c{};
-        for (int i = 0; cs.at(j, i, c); ++i) {
+        int i = 0;
+        if constexpr (C::sorted) {
+            // the cells before this run are all skipped below (their lateral gap only grows away from p's own cell)
+            const float b0 = best;
+            i = cs.own(j, pf.x, c);
+            Cell t{};
+            while (i > 0 && cs.at(j, i - 1, t)) {
+                const float lg = mx(mx(t.l0 - pf.x, pf.x - t.l1), 0.0f);
+                if (g2 + lg * lg >= b0) break;
+                --i;
+            }
+        }
+        for (; cs.at(j, i, c); ++i) {
             const float lg = mx(mx(c.l0 - pf.x, pf.x - c.l1), 0.0f);
             if (g2 + lg * lg >= best) {
                 if (c.l0 > pf.x) break;
@@ -414,22 +440,42 @@ template <class C> __host__ __device__ inline F3 wallC(const Layout& L, const C&
     F2 bestQ = f;
     float rl0 = 0.0f, rl1 = 0.0f, rlo = 0.0f, rhi = 0.0f;
     bool found = false;
+    auto planeD2 = [&](const Cell& c, float lo, float hi) {   // the squared distance to a cell's footprint
+        const F2 q{ mn(mx(f.x, c.l0), c.l1), mn(mx(f.y, lo), hi) };
+        const float dx = f.x - q.x, dy = f.y - q.y;
+        return dx * dx + dy * dy;
+    };
+    auto consider = [&](const Cell& c, float lo, float hi) {
+        const bool isWall = c.kind != INCLINE && cellTop(L, c.kind, c.row, lo) > cpt.z;
+        if (isWall == inWall) return;
+        const F2 q{ mn(mx(f.x, c.l0), c.l1), mn(mx(f.y, lo), hi) };
+        const float dx = f.x - q.x, dy = f.y - q.y;
+        const float d2 = dx * dx + dy * dy;
+        if (d2 < best) {
+            best = d2;
+            bestQ = q;
+            rl0 = c.l0; rl1 = c.l1; rlo = lo; rhi = hi;
+            found = true;
+        }
+    };
     auto visit = [&](int j) {
         float lo, hi;
         band(L, j, lo, hi);
         Cell c{};
-        for (int i = 0; cs.at(j, i, c); ++i) {
-            const bool isWall = c.kind != INCLINE && cellTop(L, c.kind, c.row, lo) > cpt.z;
-            if (isWall == inWall) continue;
-            const F2 q{ mn(mx(f.x, c.l0), c.l1), mn(mx(f.y, lo), hi) };
-            const float dx = f.x - q.x, dy = f.y - q.y;
-            const float d2 = dx * dx + dy * dy;
-            if (d2 < best) {
-                best = d2;
-                bestQ = q;
-                rl0 = c.l0; rl1 = c.l1; rlo = lo; rhi = hi;
-                found = true;
+        if constexpr (C::sorted) {
+            // In a sorted band a footprint's distance grows with the cells' index distance from p's own cell, so the
+            // cells closer than the best so far are one run about it: visited in order, the others could not win
+            const float b0 = best;
+            const int io = cs.own(j, f.x, c);
+            int i = io;
+            Cell t{};
+            while (i > 0 && cs.at(j, i - 1, t) && planeD2(t, lo, hi) < b0) --i;
+            for (; cs.at(j, i, c); ++i) {
+                if (i > io && !(planeD2(c, lo, hi) < b0)) break;
+                consider(c, lo, hi);
             }
+        } else {
+            for (int i = 0; cs.at(j, i, c); ++i) consider(c, lo, hi);
         }
     };
     visit(jp);

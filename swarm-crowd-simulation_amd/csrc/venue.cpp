@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <string>
 
 #include "swarm_params.h"
 
@@ -401,6 +402,36 @@ VenuePlan venuePlan(const Venue& v) {
             add(r, 4 + q * dpq + 2 * L.doors, 0.5 * v.cornerDoorWidthM);
         }
     return plan;
+}
+
+CellTableData makeCellTable(const seated::Layout& L) {
+    CellTableData t;
+    const int nb = seated::bandCount(L);
+    for (int j = 0; j < nb; ++j) {
+        const int a = int(t.l1.size());
+        t.start.push_back(a);
+        seated::Cell c{};
+        for (int i = 0; seated::cellAt(L, j, i, c); ++i) {
+            if (!(c.l0 <= c.l1) || (i > 0 && !(c.l0 >= t.l1.back())))
+                throw std::logic_error("makeCellTable: band " + std::to_string(j) + " is not sorted at cell " + std::to_string(i));
+            t.l1.push_back(c.l1);
+            t.cell.push_back(make_int4(__builtin_bit_cast(int, c.l0), __builtin_bit_cast(int, c.l1), c.kind, c.row));
+        }
+        const int last = int(t.l1.size()) - 1;
+        if (last < a) throw std::logic_error("makeCellTable: band " + std::to_string(j) + " has no cell");
+        // buckets over [0, the last cell's l1): each starts at the first cell reaching past its left edge
+        const float w = t.l1[size_t(last)];
+        const float invW = w > 0.0f && std::isfinite(w) ? float(seated::CELL_BUCKETS) / w : 0.0f;
+        t.invW.push_back(invW);
+        for (int b = 0; b < seated::CELL_BUCKETS; ++b) {
+            const float left = invW > 0.0f ? float(b) / invW : 0.0f;
+            int i = a;
+            while (i < last && !(left < t.l1[size_t(i)])) ++i;
+            t.bucket.push_back(i);
+        }
+    }
+    t.start.push_back(int(t.l1.size()));
+    return t;
 }
 
 NavGrid venueNavGrid(const Venue& v) {   // nav::venueGrid (src/nav_field.cpp:41-52): 0.05 m cells over the domain

@@ -54,6 +54,19 @@ int seatCountIn(float a, float b, float seatWidth);
 // The exit clear width (metres) of reference exit e (seated::exitWidthM, src/venue.cpp:867-874)
 double venueExitWidthM(const Venue& v, int e);
 
+// A layout's cell table (seated::CellTable) in host memory: every band's cells as the generator gives them and the
+// lateral buckets of the table's search; std::logic_error when a band is not sorted (a cell with l0 > l1, or starting
+// before the previous cell's l1), since the table's queries search sorted bands
+struct CellTableData {
+    std::vector<int> start;
+    std::vector<float> l1;
+    std::vector<int4> cell;
+    std::vector<int> bucket;
+    std::vector<float> invW;
+    seated::CellTable view() const { return seated::CellTable{ start.data(), l1.data(), cell.data(), bucket.data(), invW.data() }; }
+};
+CellTableData makeCellTable(const seated::Layout& L);
+
 // The venue's plan for the per-step pipeline: walls are the footprints of the seat blocks, the barrier, the side and
 // outer walls (each stands higher than a body above every floor beside it) and the fenced pitch; agents leave when
 // max(|x|, |y|) passes the outer wall's outer face (the domain's edge). Exit rectangles lie beyond that face, one per

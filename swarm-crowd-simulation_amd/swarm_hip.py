@@ -102,6 +102,7 @@ def lib():
     L.swarm_read_evac_counters.argtypes = [vp, _u32p]
     L.swarm_set_physics_shape.argtypes = [vp, C.c_int, C.c_int]
     L.swarm_set_projection_prune.argtypes = [vp, C.c_int]
+    L.swarm_set_candidate_budget.argtypes = [vp, C.c_float]
     L.swarm_set_projection_lanes.argtypes = [vp, C.c_int]
     L.swarm_get_projection_lanes.argtypes = [vp, C.c_void_p]
     L.swarm_get_physics_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -531,6 +532,11 @@ class Sim:
     def set_projection_prune(self, every=8):
         """Candidate pruning every `every` projection iterations (0: never); results are bit-identical."""
         check(lib().swarm_set_projection_prune(self.h, every))
+
+    def set_candidate_budget(self, fraction=1.0):
+        """A seated venue's own 3D step: the candidate lists' displacement budget (0: the cell ring every iteration);
+        bit-identical results for every fraction."""
+        check(lib().swarm_set_candidate_budget(self.h, fraction))
 
     def set_projection_lanes(self, lanes=1):
         """Lanes per slot in the projection's list iterations (1, 2, 4, 8); bit-identical results."""

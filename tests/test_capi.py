@@ -62,6 +62,7 @@ def test_batch_create_rejects_bad_arguments():
     assert L.swarm_autotune(None, 3, None) == -1
     assert L.swarm_set_physics_shape(None, 128, 0) == -1
     assert L.swarm_set_projection_prune(None, 8) == -1
+    assert L.swarm_set_candidate_budget(None, C.c_float(1.0)) == -1
     assert L.swarm_set_projection_lanes(None, 4) == -1
 
 
