@@ -580,6 +580,41 @@ __host__ __device__ inline F2 lerp2(F2 a, F2 b, float t) {   // glm::mix: a (1 -
 }
 __host__ __device__ inline bool sameKey(F2 a, F2 b) { return a.x == b.x && a.y == b.y; }
 
+// "Is q's cell key this key?", asked of many points against one key. The generator compares the keys; a sorted cell
+// table answers from the key's lateral run in its band: the cells sharing the key's l0 are adjacent, and the lateral
+// search gives them exactly the lat in [l1 of the cell before them, l1 of their last cell), every lat beyond when their
+// last is the band's last (NaN included), every lat below when their first is the band's first.
+template <class C, bool SORTED = C::sorted> struct KeyMatch;
+template <class C> struct KeyMatch<C, false> {
+    F2 key;
+    __host__ __device__ void set(const Layout& L, const C& cs, F2 p) { key = cellKeyC(L, cs, p); }
+    __host__ __device__ bool test(const Layout& L, const C& cs, F2 q) const { return sameKey(cellKeyC(L, cs, q), key); }
+};
+template <class C> struct KeyMatch<C, true> {
+    int j;
+    float lo, hi;
+    bool toEnd, fromStart;
+    __host__ __device__ void set(const Layout& L, const C& cs, F2 p) {
+        const F2 f = fold(p);
+        j = bandOf(L, f.y);
+        Cell c{}, t{};
+        const int io = cs.own(j, f.x, c);
+        int i0 = io, i1 = io;
+        while (i0 > 0 && cs.at(j, i0 - 1, t) && t.l0 == c.l0) --i0;
+        while (cs.at(j, i1 + 1, t) && t.l0 == c.l0) ++i1;
+        fromStart = i0 == 0;
+        lo = 0.0f;
+        if (!fromStart) { cs.at(j, i0 - 1, t); lo = t.l1; }
+        cs.at(j, i1, t);
+        hi = t.l1;
+        toEnd = !cs.at(j, i1 + 1, t);
+    }
+    __host__ __device__ bool test(const Layout& L, const C&, F2 q) const {
+        const F2 f = fold(q);
+        return bandOf(L, f.y) == j && (fromStart || !(f.x < lo)) && (toEnd || f.x < hi);
+    }
+};
+
 // The step-up limit's scan of a horizontal move p0 -> p1 from the start floor f0: walking the cells the move crosses
 // (every STEP_UP_SAMPLE, each boundary located by bisection), the move is blocked at the first boundary into a floor
 // above f0 + STEP_UP_TOL that rises faster than the steepest incline along the move; returns the allowed fraction (1: not
@@ -590,15 +625,16 @@ template <class C> __host__ __device__ inline float stepUpScanC(const Layout& L,
     const float len = sqrtf(dx * dx + dy * dy);
     const int ns = cli(int(ceilf(len / STEP_UP_SAMPLE)), 1, STEP_UP_MAX_SAMPLES);
     float lo = 0.0f;
-    F2 key = cellKeyC(L, cs, p0);
+    KeyMatch<C> key;
+    key.set(L, cs, p0);
     for (int i = 1; i <= ns; ++i) {
         const float t = float(i) / float(ns);
         for (int k = 0; k < SCAN_CROSSINGS; ++k) {
-            if (sameKey(cellKeyC(L, cs, lerp2(p0, p1, t)), key)) break;
+            if (key.test(L, cs, lerp2(p0, p1, t))) break;
             float a = lo, b = t;
             for (int m = 0; m < SCAN_BISECTIONS; ++m) {
                 const float tm = 0.5f * (a + b);
-                if (sameKey(cellKeyC(L, cs, lerp2(p0, p1, tm)), key)) a = tm;
+                if (key.test(L, cs, lerp2(p0, p1, tm))) a = tm;
                 else b = tm;
             }
             const F2 pa = lerp2(p0, p1, a), pb = lerp2(p0, p1, b);
@@ -613,7 +649,7 @@ template <class C> __host__ __device__ inline float stepUpScanC(const Layout& L,
                 return a;
             }
             lo = b;
-            key = cellKeyC(L, cs, pb);
+            key.set(L, cs, pb);
         }
         lo = t;
     }
