@@ -38,7 +38,25 @@ struct SceneView {
     int venue3d;                       // the venue's own 3D step: the surface rule in choiceCell, the venue's wall term
     float exitDepth, exitMaxZ, domainMax;   // its exit test
     seated::CellTable venueCells;      // the venue's cells, every band's (a venue scene)
+    const char* venueBlob;             // the same table as one blob (venueCells points into it)
+    seated::CellBlob venueBlobLayout;
 };
+
+// The largest venue cell table a block copies into its LDS (the rest read it from global memory)
+constexpr int VENUE_LDS_MAX = 64 * 1024;
+__host__ __device__ inline int venueLdsBytes(const SceneView& s) {
+    return s.venueBlob && s.venueBlobLayout.bytes <= VENUE_LDS_MAX ? s.venueBlobLayout.bytes : 0;
+}
+// The venue's cell table copied into the block's LDS (dynamic shared memory of venueLdsBytes(s) bytes, launched so):
+// the searches then read LDS instead of global memory. Every thread of the block must call it.
+__device__ inline seated::CellTable venueCellsInLds(const SceneView& s) {
+    extern __shared__ int4 venueLds[];
+    const int n16 = s.venueBlobLayout.bytes / 16;
+    const int4* src = reinterpret_cast<const int4*>(s.venueBlob);
+    for (int i = int(threadIdx.x); i < n16; i += int(blockDim.x)) venueLds[i] = src[i];
+    __syncthreads();
+    return seated::cellTableAt(reinterpret_cast<const char*>(venueLds), s.venueBlobLayout);
+}
 
 struct ModelParams {
     float relaxStep;      // exp(-DT / tau)

@@ -455,13 +455,15 @@ __global__ void __launch_bounds__(1024) k_choice_prefix(SceneView s0, BatchView 
     if (threadIdx.x < CHOICE_QUEUE_STRIDE) c.queue[r.queueBase + qWrite + threadIdx.x] = 0u;
 }
 
-// evaluate stage (evaluation steps only), per agent
-template <bool B>
+// evaluate stage (evaluation steps only), per agent. LDS: a seated venue's own step with its cell table in LDS
+template <bool B, bool LDS = false>
 __global__ void __launch_bounds__(HASH_BLOCK) k_choice_eval(int n, SceneView s0, BatchView bt, const float* __restrict__ ax,
                                                             const float* __restrict__ ay, const float* __restrict__ aw, ChoiceBuffers c,
                                                             const uint32_t* __restrict__ ctrl) {
     const uint32_t step = ctrl[CTRL_STEP];
     if (!evalStep(step)) return;
+    static_assert(!(B && LDS), "the venue's own step runs unbatched");
+    if constexpr (LDS) s0.venueCells = venueCellsInLds(s0);
     const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
     bool evaluated = false, eligible = false;
     uint32_t hkey = 0, ckey = 0;
@@ -531,13 +533,16 @@ __global__ void k_choice_thresh(SceneView s0, BatchView bt, ChoiceBuffers c, con
     for (int b = 0; b < CHOICE_GAIN_BINS; ++b) h[b] = 0u;
 }
 
-// apply stage (every step), per slot: the agent's exit for this step, its counts for the next, and its e_goal
-template <bool B>
+// apply stage (every step), per slot: the agent's exit for this step, its counts for the next, and its e_goal. LDS: a
+// seated venue's own step with its cell table in LDS
+template <bool B, bool LDS = false>
 __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0, BatchView bt, const uint32_t* __restrict__ skey,
                                                              const uint32_t* __restrict__ sid, const float* __restrict__ sx,
                                                              const float* __restrict__ sy, const float* __restrict__ srad, ChoiceBuffers c,
                                                              float* __restrict__ egx, float* __restrict__ egy, const uint32_t* __restrict__ ctrl,
                                                              const float* __restrict__ az) {
+    static_assert(!(B && LDS), "the venue's own step runs unbatched");
+    if constexpr (LDS) s0.venueCells = venueCellsInLds(s0);
     const int slot = xcdBlock() * HASH_BLOCK + threadIdx.x;
     const bool live = slot < n && skey[slot] != INACTIVE_CELL;
     const uint32_t step = ctrl[CTRL_STEP];
@@ -1575,10 +1580,22 @@ template <bool B>
 static void choicePasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, uint32_t* ctrl) {
     const int n = L.n;
     k_choice_prefix<B><<<L.scenarios, 1024, 0, st>>>(L.s, L.bt, c, ctrl);
-    k_choice_eval<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
+    // a seated venue's own step reads its cell table (the surface rule, the venue's wall term) from LDS where it fits
+    int lds = 0;
+    if constexpr (!B) lds = L.s.venue3d ? venueLdsBytes(L.s) : 0;
+    if constexpr (!B) {
+        if (lds > 0) k_choice_eval<false, true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, lds, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
+    }
+    if (lds == 0) k_choice_eval<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
     k_choice_thresh<B><<<L.scenarios, 64, 0, st>>>(L.s, L.bt, c, ctrl);
-    k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c, b.egx, b.egy, ctrl,
-                                                                     rd.z);
+    if constexpr (!B) {
+        if (lds > 0)
+            k_choice_apply<false, true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, lds, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c,
+                                                                                         b.egx, b.egy, ctrl, rd.z);
+    }
+    if (lds == 0)
+        k_choice_apply<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c, b.egx, b.egy,
+                                                                         ctrl, rd.z);
 }
 
 void launchChoice(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, const float* driveX,

@@ -65,13 +65,13 @@ void DeviceScene::upload(const Scene& s) {
     v.exitMaxZ = s.exitMaxZ;
     v.domainMax = s.domainMax;
     v.venueCells = seated::CellTable{ nullptr, nullptr, nullptr, nullptr, nullptr };
-    if (s.hasVenue) {   // every band's cells, generated once (the same values the generator gives)
+    v.venueBlob = nullptr;
+    v.venueBlobLayout = seated::CellBlob{};
+    if (s.hasVenue) {   // every band's cells, generated once (the same values the generator gives), as one blob
         const CellTableData t = makeCellTable(s.venue);
-        v.venueCells.start = static_cast<const int*>(put(t.start.data(), t.start.size() * sizeof(int)));
-        v.venueCells.l1 = static_cast<const float*>(put(t.l1.data(), t.l1.size() * sizeof(float)));
-        v.venueCells.cell = static_cast<const int4*>(put(t.cell.data(), t.cell.size() * sizeof(int4)));
-        v.venueCells.bucket = static_cast<const int*>(put(t.bucket.data(), t.bucket.size() * sizeof(int)));
-        v.venueCells.invW = static_cast<const float*>(put(t.invW.data(), t.invW.size() * sizeof(float)));
+        v.venueBlob = static_cast<const char*>(put(t.blob.data(), t.blob.size()));
+        v.venueBlobLayout = t.blobLayout;
+        v.venueCells = seated::cellTableAt(v.venueBlob, t.blobLayout);
     }
 }
 

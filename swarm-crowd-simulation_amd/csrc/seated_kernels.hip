@@ -103,7 +103,7 @@ __device__ inline void listContactsSeated(const StepBuffers& b, ProjBuffers in, 
 
 // The rest of one Jacobi iteration for one slot (cpu_crowd_sim.cpp:780-820): the mean correction capped at
 // PROJ_MAX_SHIFT, the fence, the slot's wall plane; p and acc are updated
-__device__ inline void finishSeated(const StepBuffers& b, int slot, V2 corr, int nContacts, V2& p, float& acc) {
+__device__ inline void finishSeated(float4 pl, V2 corr, int nContacts, V2& p, float& acc) {
     corr = corr * (PROJ_RELAX / float(nContacts > 1 ? nContacts : 1));
     const float len = len2(corr);
     if (len > PROJ_MAX_SHIFT) corr = corr * (PROJ_MAX_SHIFT / len);
@@ -113,7 +113,6 @@ __device__ inline void finishSeated(const StepBuffers& b, int slot, V2 corr, int
         const seated::F2 g = seated::sdFenceGradient(seated::F2{ r.x, r.y });
         r = r - V2{ g.x, g.y } * dFence;
     }
-    const float4 pl = b.planes[slot];
     if (fmodf(pl.w, 2.0f) == 1.0f) {   // PLANE_WALL
         const float sg = pl.z - dot2(r, V2{ pl.x, pl.y });
         if (sg > 0.0f) r = r + V2{ pl.x, pl.y } * sg;
@@ -126,32 +125,37 @@ __device__ inline void finishSeated(const StepBuffers& b, int slot, V2 corr, int
 // One Jacobi iteration for one slot: COLLECT scans the ring and writes the slot's candidate list (the first
 // iteration), LISTS walks the list (the ring again where the lists broke or the slot's overflowed), RING scans the ring
 enum class ProjMode { Ring, Collect, Lists };
+// The slot's own words (self: its agent; listed: its candidate count; pl: its wall plane) are the caller's.
 __device__ inline void projectSeated(ProjMode mode, bool broken, const SceneView& s, const StepBuffers& b, const SeatedSlots& q,
-                                     ProjBuffers in, int n, int slot, uint32_t key, float reach2, V2& p, float& acc) {
-    const uint32_t self = b.sid[slot];
+                                     ProjBuffers in, int n, int slot, uint32_t self, uint32_t listed, float4 pl, float reach2, V2& p,
+                                     float& acc) {
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     if (mode == ProjMode::Collect) {
-        ringContactsSeated<true>(s, b, q, in, n, slot, key, self, p, corr, nContacts, reach2);
+        ringContactsSeated<true>(s, b, q, in, n, slot, b.skey[slot], self, p, corr, nContacts, reach2);
     } else {
-        const uint32_t cnt = mode == ProjMode::Lists && !broken ? b.nbrCount[slot] : NBR_OVERFLOW;
-        if (cnt == NBR_OVERFLOW) ringContactsSeated<false>(s, b, q, in, n, slot, key, self, p, corr, nContacts);
+        const uint32_t cnt = mode == ProjMode::Lists && !broken ? listed : NBR_OVERFLOW;
+        if (cnt == NBR_OVERFLOW) ringContactsSeated<false>(s, b, q, in, n, slot, b.skey[slot], self, p, corr, nContacts);
         else listContactsSeated(b, in, n, slot, self, cnt, p, corr, nContacts);
     }
-    finishSeated(b, slot, corr, nContacts, p, acc);
+    finishSeated(pl, corr, nContacts, p, acc);
 }
 
 // PHYSICS_PASS 1 on the venue (cpu_crowd_sim.cpp:655-777): the exit test, separation and the headway rule over the
 // 3 x 3 ring (3D offsets), the incline rules, obstacle avoidance at torso height against the venue and the fence, the
 // kinematics with gravity, the static contacts and the wall plane
+// LDS: the venue's cell table read from the block's LDS (venueCellsInLds)
+template <bool LDS>
 __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneView s, ModelParams mp, StepBuffers b, AgentHalf rd,
                                                                 SeatedSlots q, ProjBuffers out, uint32_t* __restrict__ ctrl) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[CTRL_LIST_BROKEN] = 0u;   // the step's candidate lists start whole
+    seated::CellTable cells = s.venueCells;
+    if constexpr (LDS) cells = venueCellsInLds(s);
     const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
     if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
     const uint32_t key = b.skey[slot];
     if (key == INACTIVE_CELL) return;
-    const Venue3 V{ *s.venue, s.venueCells };
+    const Venue3 V{ *s.venue, cells };
     const uint32_t self = b.sid[slot];
     const V3 p{ b.sx[slot], b.sy[slot], rd.z[self] };
     const V3 v{ rd.vx[self], rd.vy[self], rd.vz[self] };
@@ -237,22 +241,28 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
 }
 
 // PHYSICS_PASS 2 on the venue (cpu_crowd_sim.cpp:822-829): Jacobi iteration it of P, in -> out. The first collects the
-// candidate lists; every iteration checks its slots' path lengths against the lists' budget
+// candidate lists; every iteration checks its slots' path lengths against the lists' budget. Slots past the step's
+// active count are inactive (the inactive key sorts last) and left alone.
 __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_project(int n, SceneView s, StepBuffers b, SeatedSlots q, ProjBuffers in,
                                                                 ProjBuffers out, int it, ListBudget lb, uint32_t* __restrict__ ctrl) {
     const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
     if (slot >= n || slot >= int(b.ctrl[CTRL_NACTIVE])) return;
-    const uint32_t key = b.skey[slot];
-    if (key == INACTIVE_CELL) return;
+    const bool broken = it > 1 && ctrl[CTRL_LIST_BROKEN] != 0u;
+    // every word of the slot in one load round, before the branches on them
     const float2 p0 = in.p[slot];
+    const float acc0 = in.acc[slot];
+    const float pw = b.pw[slot];
+    const uint32_t self = b.sid[slot];
+    const uint32_t listed = b.nbrCount[slot];
+    const float4 pl = b.planes[slot];
+    asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(acc0), "v"(pw), "v"(self), "v"(listed), "v"(pl.x), "v"(pl.y), "v"(pl.z), "v"(pl.w));
     V2 p{ p0.x, p0.y };
-    float acc = in.acc[slot];
-    if (b.pw[slot] != 0.0f) {
+    float acc = acc0;
+    if (pw != 0.0f) {
         if (!lb.on) {
-            projectSeated(ProjMode::Ring, false, s, b, q, in, n, slot, key, 0.0f, p, acc);
+            projectSeated(ProjMode::Ring, false, s, b, q, in, n, slot, self, listed, pl, 0.0f, p, acc);
         } else {
-            const bool broken = it > 1 && ctrl[CTRL_LIST_BROKEN] != 0u;
-            projectSeated(it == 1 ? ProjMode::Collect : ProjMode::Lists, broken, s, b, q, in, n, slot, key, lb.reach2, p, acc);
+            projectSeated(it == 1 ? ProjMode::Collect : ProjMode::Lists, broken, s, b, q, in, n, slot, self, listed, pl, lb.reach2, p, acc);
             if (!broken && acc > lb.budget) atomicOr(&ctrl[CTRL_LIST_BROKEN], 1u);
         }
     }
@@ -262,29 +272,38 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_project(int n, SceneVie
 
 // PHYSICS_PASS 3 on the venue (cpu_crowd_sim.cpp:835-866): the last iteration, the final static contacts of every agent
 // the projection moved, the step-up limit of every agent; the scatter to agent order with the evacuation events
+template <bool LDS>
 __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView s, StepBuffers b, SeatedSlots q, ProjBuffers in,
                                                               int projectHere, AgentHalf rd, AgentHalf wr, EvacBuffers ev,
                                                               uint32_t* __restrict__ ctrl) {
+    seated::CellTable cells = s.venueCells;
+    if constexpr (LDS) cells = venueCellsInLds(s);
     const uint32_t step = ctrl[CTRL_STEP];
     const int active = int(ctrl[CTRL_NACTIVE]);
     const int slot = xcdBlock() * SEATED_BLOCK + threadIdx.x;
     if (slot < n) {
-        const uint32_t key = b.skey[slot];
         const uint32_t agent = b.sid[slot];
-        if (slot >= active || key == INACTIVE_CELL) {   // carried over
+        if (slot >= active) {   // an inactive slot (the inactive key sorts last): its agent is carried over
             wr.x[agent] = rd.x[agent]; wr.y[agent] = rd.y[agent]; wr.z[agent] = rd.z[agent]; wr.w[agent] = rd.w[agent];
             wr.vx[agent] = rd.vx[agent]; wr.vy[agent] = rd.vy[agent]; wr.vz[agent] = rd.vz[agent]; wr.spd[agent] = rd.spd[agent];
         } else {
-            const Venue3 V{ *s.venue, s.venueCells };
+            const Venue3 V{ *s.venue, cells };
+            // every word of the slot in one load round, before the branches on them
             const float2 p0 = in.p[slot];
-            V2 p{ p0.x, p0.y };
-            float acc = in.acc[slot];
+            const float acc0 = in.acc[slot];
             const float pw = b.pw[slot];
+            const uint32_t listed = b.nbrCount[slot];
+            const float4 pl = b.planes[slot];
+            const float sx = b.sx[slot], sy = b.sy[slot];
+            asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(acc0), "v"(pw), "v"(listed), "v"(pl.x), "v"(pl.y), "v"(pl.z), "v"(pl.w), "v"(sx),
+                         "v"(sy));
+            V2 p{ p0.x, p0.y };
+            float acc = acc0;
             if (pw != 0.0f && projectHere) {   // iteration P: the lists when earlier iterations collected them
                 const ProjMode mode = projectHere > 1 ? ProjMode::Lists : ProjMode::Ring;
-                projectSeated(mode, mode == ProjMode::Lists && ctrl[CTRL_LIST_BROKEN] != 0u, s, b, q, in, n, slot, key, 0.0f, p, acc);
+                projectSeated(mode, mode == ProjMode::Lists && ctrl[CTRL_LIST_BROKEN] != 0u, s, b, q, in, n, slot, agent, listed, pl, 0.0f,
+                              p, acc);
             }
-            const float sx = b.sx[slot], sy = b.sy[slot];
             if (pw == 0.0f) {   // parked this step: an evacuation event, its exit by the last active position
                 ev.inside[agent] = make_float2(sx, sy);
                 ev.before[agent] = make_float2(wr.x[agent], wr.y[agent]);
@@ -300,7 +319,7 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_final(int n, SceneView 
                 const V3 pOld{ sx, sy, rd.z[agent] };
                 if (acc > 0.0f) {
                     V3 wall;
-                    resolveStatic(V, pOld, np, nv, b.planes[slot].w >= 2.0f, wall);   // PLANE_GROUNDED
+                    resolveStatic(V, pOld, np, nv, pl.w >= 2.0f, wall);   // PLANE_GROUNDED
                 }
                 limitStepUp(V, pOld, np, nv);
                 wr.x[agent] = np.x; wr.y[agent] = np.y; wr.z[agent] = np.z; wr.w[agent] = 1.0f;
@@ -333,7 +352,9 @@ void launchSeatedPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, Se
         const float r = (BODY_DIAM + 2.0f * lb.budget) * 1.001f + 1e-6f;
         lb.reach2 = r * r;
     }
-    k_seated_predict<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, L.mp, b, rd, q, pa, ctrl);
+    const int lds = venueLdsBytes(L.s);   // the cell table in each block's LDS where it fits
+    if (lds > 0) k_seated_predict<true><<<g, SEATED_BLOCK, lds, st>>>(n, L.s, L.mp, b, rd, q, pa, ctrl);
+    else k_seated_predict<false><<<g, SEATED_BLOCK, 0, st>>>(n, L.s, L.mp, b, rd, q, pa, ctrl);
     if (stamps) (void)hipEventRecord(stamps[0], st);
     ProjBuffers in = pa, out = pb;
     for (int it = 1; it < P; ++it) {
@@ -344,7 +365,9 @@ void launchSeatedPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, Se
     }
     if (stamps) (void)hipEventRecord(stamps[1], st);
     // the final pass runs iteration P: on the lists when iteration 1 collected them, else on the ring
-    k_seated_final<<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, lb.on ? 2 : (P > 0 ? 1 : 0), rd, wr, ev, ctrl);
+    const int here = lb.on ? 2 : (P > 0 ? 1 : 0);
+    if (lds > 0) k_seated_final<true><<<g, SEATED_BLOCK, lds, st>>>(n, L.s, b, q, in, here, rd, wr, ev, ctrl);
+    else k_seated_final<false><<<g, SEATED_BLOCK, 0, st>>>(n, L.s, b, q, in, here, rd, wr, ev, ctrl);
 }
 
 }  // namespace swarm

@@ -250,7 +250,7 @@ struct CellTable {
     const int* start;
     const float* l1;
     const int4* cell;
-    const int* bucket;
+    const uint16_t* bucket;
     const float* invW;
     __host__ __device__ bool at(int j, int c, Cell& out) const {
         const int i = start[j] + c;
@@ -274,6 +274,17 @@ struct CellTable {
         return i - a;
     }
 };
+
+// A cell table stored as one blob (byte offsets from its start, each array 16-byte aligned, the size a multiple of 16),
+// so that a kernel can copy it whole into LDS and search it there
+struct CellBlob {
+    int bytes, oStart, oInvW, oL1, oBucket, oCell;
+};
+__host__ __device__ inline CellTable cellTableAt(const char* base, const CellBlob& b) {
+    return CellTable{ reinterpret_cast<const int*>(base + b.oStart), reinterpret_cast<const float*>(base + b.oL1),
+                      reinterpret_cast<const int4*>(base + b.oCell), reinterpret_cast<const uint16_t*>(base + b.oBucket),
+                      reinterpret_cast<const float*>(base + b.oInvW) };
+}
 
 template <class C> __host__ __device__ inline float floorHeightC(const Layout& L, const C& cs, F2 p) {
     const F2 f = fold(p);

@@ -427,10 +427,24 @@ CellTableData makeCellTable(const seated::Layout& L) {
             const float left = invW > 0.0f ? float(b) / invW : 0.0f;
             int i = a;
             while (i < last && !(left < t.l1[size_t(i)])) ++i;
-            t.bucket.push_back(i);
+            t.bucket.push_back(uint16_t(i));
         }
     }
     t.start.push_back(int(t.l1.size()));
+    if (t.l1.size() >= 65536) throw std::logic_error("makeCellTable: 65,536 cells or more");
+    // the blob: start, invW, l1, bucket, cell; each array at a 16-byte boundary
+    auto add = [&t](const void* p, size_t bytes) {
+        const int o = int(t.blob.size());
+        t.blob.insert(t.blob.end(), static_cast<const char*>(p), static_cast<const char*>(p) + bytes);
+        t.blob.resize((t.blob.size() + 15) & ~size_t(15), 0);
+        return o;
+    };
+    t.blobLayout.oStart = add(t.start.data(), t.start.size() * sizeof(int));
+    t.blobLayout.oInvW = add(t.invW.data(), t.invW.size() * sizeof(float));
+    t.blobLayout.oL1 = add(t.l1.data(), t.l1.size() * sizeof(float));
+    t.blobLayout.oBucket = add(t.bucket.data(), t.bucket.size() * sizeof(uint16_t));
+    t.blobLayout.oCell = add(t.cell.data(), t.cell.size() * sizeof(int4));
+    t.blobLayout.bytes = int(t.blob.size());
     return t;
 }
 

@@ -55,14 +55,17 @@ int seatCountIn(float a, float b, float seatWidth);
 double venueExitWidthM(const Venue& v, int e);
 
 // A layout's cell table (seated::CellTable) in host memory: every band's cells as the generator gives them and the
-// lateral buckets of the table's search; std::logic_error when a band is not sorted (a cell with l0 > l1, or starting
-// before the previous cell's l1), since the table's queries search sorted bands
+// lateral buckets of the table's search, as arrays (view()) and packed into one blob (blob, blobLayout; for the device);
+// std::logic_error when a band is not sorted (a cell with l0 > l1, or starting before the previous cell's l1), since the
+// table's queries search sorted bands, or when it has 65,536 cells or more (the buckets hold 16-bit indices)
 struct CellTableData {
     std::vector<int> start;
     std::vector<float> l1;
     std::vector<int4> cell;
-    std::vector<int> bucket;
+    std::vector<uint16_t> bucket;
     std::vector<float> invW;
+    std::vector<char> blob;
+    seated::CellBlob blobLayout{};
     seated::CellTable view() const { return seated::CellTable{ start.data(), l1.data(), cell.data(), bucket.data(), invW.data() }; }
 };
 CellTableData makeCellTable(const seated::Layout& L);
