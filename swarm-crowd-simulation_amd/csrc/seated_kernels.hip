@@ -18,6 +18,7 @@ constexpr int SEATED_BLOCK = 128;
 inline int blocksOf(int n, int b) { return (n + b - 1) / b; }
 
 constexpr int SEATED_LIST_CHUNK = 4;   // list candidates whose loads one thread issues together
+constexpr int SEATED_RING_CHUNK = 4;   // ring slots whose loads one thread issues together
 
 // The contact candidates (the plain step's list buffers, slot-interleaved: index nbr[j * n + slot], height offset
 // nbrD[j * n + slot]). The first iteration's ring scan keeps, in ring order, every slot with a height overlap (heights
@@ -41,17 +42,31 @@ __device__ inline void ringContactsSeated(const SceneView& s, const StepBuffers&
     const float z = q.pz[slot];
     const int cx = int(key % uint32_t(s.hashNX)), cy = int(key / uint32_t(s.hashNX));
     uint32_t cnt = 0;
-    for (int yy = -s.projRing; yy <= s.projRing; yy++)
-        for (int xx = -s.projRing; xx <= s.projRing; xx++) {
-            const int nx = cx + xx, ny = cy + yy;
-            if (nx < 0 || nx >= s.hashNX || ny < 0 || ny >= s.hashNY) continue;
-            const uint32_t c = uint32_t(nx + ny * s.hashNX);
-            const uint32_t k0 = b.bstart[c], k1 = b.bstart[c + 1];
-            for (uint32_t k = k0; k < k1; ++k) {
+    // a ring row's slots are one range in the ring's order (consecutive keys); loaded SEATED_RING_CHUNK at a time
+    const int x0 = cx - s.projRing < 0 ? 0 : cx - s.projRing;
+    const int x1 = cx + s.projRing >= s.hashNX ? s.hashNX - 1 : cx + s.projRing;
+    for (int yy = -s.projRing; yy <= s.projRing; yy++) {
+        const int ny = cy + yy;
+        if (ny < 0 || ny >= s.hashNY) continue;
+        const uint32_t row = uint32_t(ny * s.hashNX);
+        const uint32_t k0 = b.bstart[row + uint32_t(x0)], k1 = b.bstart[row + uint32_t(x1) + 1u];
+        for (uint32_t kc = k0; kc < k1; kc += SEATED_RING_CHUNK) {
+            float2 o[SEATED_RING_CHUNK];
+            float oz[SEATED_RING_CHUNK], ow[SEATED_RING_CHUNK];
+#pragma unroll
+            for (int u = 0; u < SEATED_RING_CHUNK; ++u) {
+                const uint32_t k = kc + uint32_t(u) < k1 ? kc + uint32_t(u) : kc;
+                o[u] = in.p[k];
+                oz[u] = q.pz[k];
+                ow[u] = b.pw[k];
+            }
+#pragma unroll
+            for (int u = 0; u < SEATED_RING_CHUNK; ++u) {
+                const uint32_t k = kc + uint32_t(u);
+                if (k >= k1) break;
                 if (int(k) == slot) continue;
-                const float2 o = in.p[k];
-                const V3 offset{ p.x - o.x, p.y - o.y, z - q.pz[k] };
-                if (b.pw[k] == 0.0f || !(fabsf(offset.z) < BODY_DIAM)) continue;
+                const V3 offset{ p.x - o[u].x, p.y - o[u].y, z - oz[u] };
+                if (ow[u] == 0.0f || !(fabsf(offset.z) < BODY_DIAM)) continue;
                 const float d2 = dot2(xy(offset), xy(offset));
                 if (d2 < BODY_DIAM2) {
                     corr = corr + contact3(offset, self, b.sid[k]);
@@ -68,6 +83,7 @@ __device__ inline void ringContactsSeated(const SceneView& s, const StepBuffers&
                 }
             }
         }
+    }
     if constexpr (COLLECT) b.nbrCount[slot] = cnt <= uint32_t(NBR_CAP) ? cnt : NBR_OVERFLOW;
 }
 
@@ -175,19 +191,36 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
     const int cx = int(key % uint32_t(s.hashNX)), cy = int(key / uint32_t(s.hashNX));
     V2 sep{ 0.0f, 0.0f };
     float sAhead = 1e9f;   // the horizontal centre distance to the nearest agent ahead
-    for (int yy = -s.percRing; yy <= s.percRing; yy++)
-        for (int xx = -s.percRing; xx <= s.percRing; xx++) {
-            const int nx = cx + xx, ny = cy + yy;
-            if (nx < 0 || nx >= s.hashNX || ny < 0 || ny >= s.hashNY) continue;
-            const uint32_t c = uint32_t(nx + ny * s.hashNX);
-            const uint32_t k0 = b.bstart[c], k1 = b.bstart[c + 1];
-            for (uint32_t k = k0; k < k1; ++k) {
+    // a ring row's cells are consecutive keys, so their slots are one range, in the ring's order; its slots' words are
+    // loaded SEATED_RING_CHUNK at a time, then used in order
+    const int x0 = cx - s.percRing < 0 ? 0 : cx - s.percRing;
+    const int x1 = cx + s.percRing >= s.hashNX ? s.hashNX - 1 : cx + s.percRing;
+    for (int yy = -s.percRing; yy <= s.percRing; yy++) {
+        const int ny = cy + yy;
+        if (ny < 0 || ny >= s.hashNY) continue;
+        const uint32_t row = uint32_t(ny * s.hashNX);
+        const uint32_t k0 = b.bstart[row + uint32_t(x0)], k1 = b.bstart[row + uint32_t(x1) + 1u];
+        for (uint32_t kc = k0; kc < k1; kc += SEATED_RING_CHUNK) {
+            uint32_t other[SEATED_RING_CHUNK];
+            float ox[SEATED_RING_CHUNK], oy[SEATED_RING_CHUNK], oz[SEATED_RING_CHUNK];
+#pragma unroll
+            for (int u = 0; u < SEATED_RING_CHUNK; ++u) {
+                const uint32_t k = kc + uint32_t(u) < k1 ? kc + uint32_t(u) : kc;
+                other[u] = b.sid[k];
+                ox[u] = b.sx[k];
+                oy[u] = b.sy[k];
+            }
+#pragma unroll
+            for (int u = 0; u < SEATED_RING_CHUNK; ++u) oz[u] = rd.z[other[u]];
+#pragma unroll
+            for (int u = 0; u < SEATED_RING_CHUNK; ++u) {
+                const uint32_t k = kc + uint32_t(u);
+                if (k >= k1) break;
                 if (int(k) == slot) continue;
-                const uint32_t other = b.sid[k];
-                const V3 offset{ p.x - b.sx[k], p.y - b.sy[k], p.z - rd.z[other] };
+                const V3 offset{ p.x - ox[u], p.y - oy[u], p.z - oz[u] };
                 const float dxy2 = dot2(xy(offset), xy(offset));
                 if (dxy2 < R_PERC2) {
-                    sep = sep + separation3(offset, self, other);
+                    sep = sep + separation3(offset, self, other[u]);
                     const float ahead = -dot2(xy(offset), eGoal);
                     const float lateral = fabsf(offset.x * eGoal.y - offset.y * eGoal.x);
                     const float dxy = sqrtf(dxy2);
@@ -195,11 +228,12 @@ __global__ void __launch_bounds__(SEATED_BLOCK) k_seated_predict(int n, SceneVie
                         const V2 eOther{ b.egx[k], b.egy[k] };
                         const float aheadOther = dot2(xy(offset), eOther);
                         const bool mutual = aheadOther > 0.0f && fabsf(offset.x * eOther.y - offset.y * eOther.x) < BODY_DIAM;
-                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other)) sAhead = dxy;
+                        if (!mutual || ahead > aheadOther || (ahead == aheadOther && self > other[u])) sAhead = dxy;
                     }
                 }
             }
         }
+    }
     // the incline speed and the stair spacing by the floor's slope along e_goal (the reference's default venue model)
     const float slope = seated::floorSlopeC(V.L, V.cs, seated::F2{ p.x, p.y }, seated::F2{ eGoal.x, eGoal.y });
     const float vFree = inclineFreeSpeed(V_MAX, slope);
