@@ -417,26 +417,30 @@ __device__ inline ChoiceRef choiceRef(uint32_t sc, int stride) {
                       sc * uint32_t(stride) * uint32_t(CHOICE_GAIN_BINS), sc * uint32_t(THRESHOLD_STRIDE) };
 }
 
-// prefix stage, one block per scenario: the read parity's queue-ahead histograms to exclusive prefix sums, each
-// exit's histogram total added to the read parity's queue count (the apply stage counts there only the agents it does
-// not bin), the write parity cleared, the write parity's queue counts cleared
+// prefix stage, blockIdx.x the scenario, one wave per exit (CHOICE_EXIT_WAVES exits per block, blockIdx.y their
+// group): the read parity's queue-ahead histograms to exclusive prefix sums, each exit's histogram total added to the read
+// parity's queue count (the apply stage counts there only the agents it does not bin), the write parity cleared, the
+// write parity's queue counts cleared (by the scenario's first group)
+constexpr int CHOICE_EXIT_WAVES = 4;
 template <bool B>
-__global__ void __launch_bounds__(1024) k_choice_prefix(SceneView s0, BatchView bt, ChoiceBuffers c, const uint32_t* __restrict__ ctrl) {
+__global__ void __launch_bounds__(64 * CHOICE_EXIT_WAVES) k_choice_prefix(SceneView s0, BatchView bt, ChoiceBuffers c,
+                                                                          const uint32_t* __restrict__ ctrl) {
     const uint32_t sc = blockIdx.x;
     const int E = sceneOf<B>(s0, bt, sc).E;
     const uint32_t step = ctrl[CTRL_STEP];
     const uint32_t par = step & 1u;
     const ChoiceRef r = choiceRef(sc, c.exitStride);
-    uint32_t* aRead = c.ahead + r.aheadBase + size_t(par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS;
-    uint32_t* aWrite = c.ahead + r.aheadBase + size_t(1u - par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int PER = CHOICE_AHEAD_BINS / 64;
-    for (int e = wave; e < E; e += 16) {
-        uint32_t* a = aRead + size_t(e) * CHOICE_AHEAD_BINS + lane * PER;
-        uint32_t v[PER];
+    const int e = int(blockIdx.y) * CHOICE_EXIT_WAVES + wave;
+    if (e < E) {
+        constexpr int PER = CHOICE_AHEAD_BINS / 64;   // a lane's run of bins, PER / 4 16-byte words
+        uint32_t* aRead = c.ahead + r.aheadBase + size_t(par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS;
+        uint32_t* aWrite = c.ahead + r.aheadBase + size_t(1u - par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS;
+        uint4* a = reinterpret_cast<uint4*>(aRead + size_t(e) * CHOICE_AHEAD_BINS + lane * PER);
+        uint4 v[PER / 4];
         uint32_t local = 0;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) { v[k] = a[k]; local += v[k]; }
+        for (int k = 0; k < PER / 4; ++k) { v[k] = a[k]; local += v[k].x + v[k].y + v[k].z + v[k].w; }
         uint32_t incl = local;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t o = __shfl_up(incl, off, 64);
@@ -444,15 +448,22 @@ __global__ void __launch_bounds__(1024) k_choice_prefix(SceneView s0, BatchView 
         }
         uint32_t run = incl - local;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) { a[k] = run; run += v[k]; }
+        for (int k = 0; k < PER / 4; ++k) {
+            uint4 w;
+            w.x = run; run += v[k].x;
+            w.y = run; run += v[k].y;
+            w.z = run; run += v[k].z;
+            w.w = run; run += v[k].w;
+            a[k] = w;
+        }
         const uint32_t total = __shfl(incl, 63, 64);
         if (lane == 0) c.queue[r.queueBase + par * uint32_t(CHOICE_QUEUE_STRIDE) + uint32_t(e)] += total;
-        uint32_t* wrp = aWrite + size_t(e) * CHOICE_AHEAD_BINS + lane * PER;
+        uint4* wrp = reinterpret_cast<uint4*>(aWrite + size_t(e) * CHOICE_AHEAD_BINS + lane * PER);
 #pragma unroll
-        for (int k = 0; k < PER; ++k) wrp[k] = 0u;
+        for (int k = 0; k < PER / 4; ++k) wrp[k] = make_uint4(0u, 0u, 0u, 0u);
     }
     const uint32_t qWrite = uint32_t(CHOICE_QUEUE_STRIDE) - par * uint32_t(CHOICE_QUEUE_STRIDE);
-    if (threadIdx.x < CHOICE_QUEUE_STRIDE) c.queue[r.queueBase + qWrite + threadIdx.x] = 0u;
+    if (blockIdx.y == 0 && threadIdx.x < CHOICE_QUEUE_STRIDE) c.queue[r.queueBase + qWrite + threadIdx.x] = 0u;
 }
 
 // evaluate stage (evaluation steps only), per agent. LDS: a seated venue's own step with its cell table in LDS
@@ -504,33 +515,54 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_eval(int n, SceneView s0,
     waveCountAdd(c.counters, ckey + uint32_t(CNT_ELIGIBLE), eligible);
 }
 
-// thresholds stage (evaluation steps only), one block per scenario, one thread per exit
+// thresholds stage (evaluation steps only), blockIdx.x the scenario, one wave per exit (groups as the prefix stage).
+// Walking an exit's gain histogram down from its top bin, the bins are taken while their running total stays within
+// the cap; the threshold is the lowest bin taken, or the top non-empty bin when that alone exceeds the cap (none: the
+// bin count). With S(b) the count in bins b and above, that is the lowest non-empty bin with S(b) <= cap, else the
+// highest non-empty bin: a suffix sum across the wave and two reductions. The histogram is then cleared.
 template <bool B>
-__global__ void k_choice_thresh(SceneView s0, BatchView bt, ChoiceBuffers c, const uint32_t* __restrict__ ctrl) {
+__global__ void __launch_bounds__(64 * CHOICE_EXIT_WAVES) k_choice_thresh(SceneView s0, BatchView bt, ChoiceBuffers c,
+                                                                          const uint32_t* __restrict__ ctrl) {
     const uint32_t step = ctrl[CTRL_STEP];
     if (!evalStep(step)) return;
     const uint32_t sc = blockIdx.x;
-    const int e = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = int(blockIdx.y) * CHOICE_EXIT_WAVES + wave;
     if (e >= sceneOf<B>(s0, bt, sc).E) return;
+    static_assert(CHOICE_GAIN_BINS == 4 * 64, "a lane holds four bins");
     const ChoiceRef r = choiceRef(sc, c.exitStride);
     const uint32_t qRead = (step & 1u) * uint32_t(CHOICE_QUEUE_STRIDE);
-    uint32_t* h = c.hist + r.histBase + uint32_t(e) * CHOICE_GAIN_BINS;
+    uint4* h = reinterpret_cast<uint4*>(c.hist + r.histBase + uint32_t(e) * CHOICE_GAIN_BINS) + lane;   // bins 4 lane .. 4 lane + 3
     const uint32_t queued = c.queue[r.queueBase + qRead + uint32_t(e)];
     const uint32_t share = uint32_t(float(queued) * CHOICE_SWITCH_SHARE);
     const uint32_t cap = share > CHOICE_MIN_SWITCHES ? share : CHOICE_MIN_SWITCHES;
-    uint32_t cum = 0, thr = uint32_t(CHOICE_GAIN_BINS);
-    for (int b = CHOICE_GAIN_BINS - 1; b >= 0; --b) {
-        const uint32_t cnt = h[b];
-        if (cnt == 0) continue;
-        if (cum + cnt > cap) {
-            if (cum == 0) thr = uint32_t(b);
-            break;
-        }
-        cum += cnt;
-        thr = uint32_t(b);
+    const uint4 v = *h;
+    const uint32_t cnt[4] = { v.x, v.y, v.z, v.w };
+    // the lanes above this one, summed (a suffix scan down the wave)
+    uint32_t above = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_down(above, off, 64);
+        if (lane + off < 64) above += o;
     }
-    c.threshold[r.thresholdBase + uint32_t(e)] = thr;
-    for (int b = 0; b < CHOICE_GAIN_BINS; ++b) h[b] = 0u;
+    above -= cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    int lowOk = CHOICE_GAIN_BINS, highNonEmpty = -1;
+    uint32_t S = above;
+    for (int j = 3; j >= 0; --j) {
+        S += cnt[j];
+        const int b = 4 * lane + j;
+        if (cnt[j] != 0u) {
+            if (S <= cap) lowOk = b;
+            if (highNonEmpty < 0) highNonEmpty = b;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const int lo = __shfl_xor(lowOk, off, 64), hi = __shfl_xor(highNonEmpty, off, 64);
+        lowOk = lo < lowOk ? lo : lowOk;
+        highNonEmpty = hi > highNonEmpty ? hi : highNonEmpty;
+    }
+    const uint32_t thr = lowOk < CHOICE_GAIN_BINS ? uint32_t(lowOk) : (highNonEmpty >= 0 ? uint32_t(highNonEmpty) : uint32_t(CHOICE_GAIN_BINS));
+    if (lane == 0) c.threshold[r.thresholdBase + uint32_t(e)] = thr;
+    *h = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // apply stage (every step), per slot: the agent's exit for this step, its counts for the next, and its e_goal. LDS: a
@@ -1579,7 +1611,8 @@ void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const G
 template <bool B>
 static void choicePasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const ChoiceBuffers& c, StepBuffers& b, uint32_t* ctrl) {
     const int n = L.n;
-    k_choice_prefix<B><<<L.scenarios, 1024, 0, st>>>(L.s, L.bt, c, ctrl);
+    const dim3 exitGrid(unsigned(L.scenarios), unsigned((std::max(L.maxExits, 1) + CHOICE_EXIT_WAVES - 1) / CHOICE_EXIT_WAVES));
+    k_choice_prefix<B><<<exitGrid, 64 * CHOICE_EXIT_WAVES, 0, st>>>(L.s, L.bt, c, ctrl);
     // a seated venue's own step reads its cell table (the surface rule, the venue's wall term) from LDS where it fits
     int lds = 0;
     if constexpr (!B) lds = L.s.venue3d ? venueLdsBytes(L.s) : 0;
@@ -1587,7 +1620,7 @@ static void choicePasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, cons
         if (lds > 0) k_choice_eval<false, true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, lds, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
     }
     if (lds == 0) k_choice_eval<B><<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, L.s, L.bt, rd.x, rd.y, rd.w, c, ctrl);
-    k_choice_thresh<B><<<L.scenarios, 64, 0, st>>>(L.s, L.bt, c, ctrl);
+    k_choice_thresh<B><<<exitGrid, 64 * CHOICE_EXIT_WAVES, 0, st>>>(L.s, L.bt, c, ctrl);
     if constexpr (!B) {
         if (lds > 0)
             k_choice_apply<false, true><<<blocks(n, HASH_BLOCK), HASH_BLOCK, lds, st>>>(n, L.s, L.bt, b.skey, b.sid, b.sx, b.sy, b.sr, c,
