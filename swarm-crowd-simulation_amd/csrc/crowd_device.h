@@ -24,16 +24,19 @@ struct SceneView {
     int navRX, navRY;
     float navOX, navOY, navCell, navInvCell;
     int K, E;
-    const uint32_t* navInfo;
-    const uint32_t* navSlot;
-    const uint16_t* navDist;
+    const uint32_t* navInfo;           // per nav cell: its class bits (the fallback search reads its neighbours here)
+    // per nav cell one record of navStride words: the class bits, the K candidate slots (anchor | exit << 25), the K
+    // 16-bit distance quanta; an agent's choice reads one record (one cache line where K <= 8) instead of three arrays
+    const uint32_t* navRec;
+    int navStride;
     const float* invCapS;
     float secondsPerUnit, distQuantum;
     const float* stair;
     const float* exitRects;   // E x 4
-    const float* wallRects;            // wall boxes, 4 floats each (lox, loy, hix, hiy)
-    const uint32_t* wallBinStart;      // per hash cell: its walls within reach, wallBinList[start[c] .. start[c + 1])
-    const uint32_t* wallBinList;
+    // per hash cell: its wall boxes within reach, in ascending box order, as the boxes themselves (lox, loy, hix, hiy):
+    // wallBinBox[wallBinStart[c] .. wallBinStart[c + 1])
+    const uint32_t* wallBinStart;
+    const float4* wallBinBox;
     const seated::Layout* venue;       // a seated venue: its layout (a plan's incline rules, or the 3D step), else null
     int venue3d;                       // the venue's own 3D step: the surface rule in choiceCell, the venue's wall term
     float exitDepth, exitMaxZ, domainMax;   // its exit test
@@ -66,6 +69,13 @@ struct ModelParams {
 };
 
 struct V2 { float x, y; };
+
+// a nav cell's record: its class bits, its candidate slots, its distance quanta
+__device__ inline const uint32_t* navRecOf(const SceneView& s, int cell) { return s.navRec + size_t(cell) * size_t(s.navStride); }
+__device__ inline const uint32_t* navSlots(const SceneView& s, int cell) { return navRecOf(s, cell) + 1; }
+__device__ inline const uint16_t* navDists(const SceneView& s, int cell) {
+    return reinterpret_cast<const uint16_t*>(navRecOf(s, cell) + 1 + s.K);
+}
 __device__ inline V2 operator+(V2 a, V2 b) { return { a.x + b.x, a.y + b.y }; }
 __device__ inline V2 operator-(V2 a, V2 b) { return { a.x - b.x, a.y - b.y }; }
 __device__ inline V2 operator-(V2 a) { return { -a.x, -a.y }; }
@@ -172,7 +182,7 @@ __device__ inline int choiceCell(const SceneView& s, V2 p) {
     const float gx = (p.x - s.navOX) * s.navInvCell, gy = (p.y - s.navOY) * s.navInvCell;
     const int cx = clampi(int(floorf(gx)), 0, s.navRX - 1), cy = clampi(int(floorf(gy)), 0, s.navRY - 1);
     int i = cx + cy * s.navRX;
-    if (!(s.navInfo[i] & NAV_FIT)) {
+    if (!(navRecOf(s, i)[0] & NAV_FIT)) {   // the record's own copy: the slots the caller reads next share its line
         float best = 1e9f;
         for (int r = 1; r <= 2 && best == 1e9f; r++)
             for (int dy = -r; dy <= r; dy++)
@@ -217,7 +227,7 @@ __device__ inline V2 anchorCentre(const SceneView& s, uint32_t a) {
     return V2{ s.navOX + px, s.navOY + py };
 }
 __device__ inline V2 choiceDirection(const SceneView& s, int cell, V2 p, uint32_t exit) {
-    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+    const uint32_t* sl = navSlots(s, cell);
     for (int k = 0; k < s.K; ++k) {
         const uint32_t w = sl[k];
         if (slotExitOf(w) != exit) continue;
@@ -236,20 +246,19 @@ __device__ inline G3 wallQuery(const SceneView& s, V2 p, float reach) {
     const uint32_t c = hashCell(s, p.x, p.y);
     const uint32_t k0 = s.wallBinStart[c], k1 = s.wallBinStart[c + 1];
     float best = reach * reach;
-    uint32_t bk = 0xFFFFFFFFu;
+    bool found = false;
     V2 bq{ 0.0f, 0.0f };
+    float4 r{ 0.0f, 0.0f, 0.0f, 0.0f };   // the nearest box
     for (uint32_t k = k0; k < k1; ++k) {
-        const uint32_t wi = s.wallBinList[k];
-        const float4 r = *reinterpret_cast<const float4*>(s.wallRects + 4 * size_t(wi));
-        const V2 q{ fminf(fmaxf(p.x, r.x), r.z), fminf(fmaxf(p.y, r.y), r.w) };
+        const float4 b = s.wallBinBox[k];
+        const V2 q{ fminf(fmaxf(p.x, b.x), b.z), fminf(fmaxf(p.y, b.y), b.w) };
         const V2 dd = p - q;
         const float d2 = dot2(dd, dd);
-        if (d2 < best) { best = d2; bq = q; bk = wi; }
+        if (d2 < best) { best = d2; bq = q; r = b; found = true; }
     }
-    if (bk == 0xFFFFFFFFu) return G3{ 0.0f, 0.0f, WALL_NONE };
+    if (!found) return G3{ 0.0f, 0.0f, WALL_NONE };
     const float d = sqrtf(best);
     if (d > 0.0f) { const V2 u = (p - bq) / d; return G3{ u.x, u.y, d }; }
-    const float4 r = *reinterpret_cast<const float4*>(s.wallRects + 4 * size_t(bk));
     const V2 h{ p.x <= r.x ? -1.0f : (p.x >= r.z ? 1.0f : 0.0f), p.y <= r.y ? -1.0f : (p.y >= r.w ? 1.0f : 0.0f) };
     if (nonzero(h)) { const V2 u = h / len2(h); return G3{ u.x, u.y, 0.0f }; }
     // strictly inside: the nearest face (left, right, bottom, top; the first of equal depths)
@@ -287,7 +296,7 @@ __device__ inline uint32_t aheadBin(uint32_t q) {
 }
 __device__ inline int slotOfExit(const SceneView& s, int cell, uint32_t exit) {
     int k = -1;
-    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+    const uint32_t* sl = navSlots(s, cell);
     for (int i = 0; i < s.K; ++i)
         if (slotExitOf(sl[i]) == exit) k = i;
     return k;
@@ -295,8 +304,8 @@ __device__ inline int slotOfExit(const SceneView& s, int cell, uint32_t exit) {
 __device__ inline int bestCandidate(const SceneView& s, int cell, const uint32_t* aRead, float& eBest) {
     int best = -1;
     eBest = 0.0f;
-    const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
-    const uint16_t* dq = s.navDist + size_t(cell) * size_t(s.K);
+    const uint32_t* sl = navSlots(s, cell);
+    const uint16_t* dq = navDists(s, cell);
     for (int k = 0; k < s.K; ++k) {
         const uint32_t ex = slotExitOf(sl[k]);
         if (ex == CHOICE_EMPTY) break;

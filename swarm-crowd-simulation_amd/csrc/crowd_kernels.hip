@@ -488,8 +488,8 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_eval(int n, SceneView s0,
             const uint32_t* aRead = c.ahead + r.aheadBase + size_t(step & 1u) * size_t(c.exitStride) * CHOICE_AHEAD_BINS;
             const V2 p{ ax[i], ay[i] };
             const int cell = choiceCell(s, p);
-            const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
-            const uint16_t* dq = s.navDist + size_t(cell) * size_t(s.K);
+            const uint32_t* sl = navSlots(s, cell);
+            const uint16_t* dq = navDists(s, cell);
             const uint32_t current = c.assign[i];
             const int curK = slotExitOf(sl[0]) == CHOICE_EMPTY ? -1 : slotOfExit(s, cell, current);
             if (curK >= 0) {
@@ -592,7 +592,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
         const uint32_t id = sid[slot];
         const V2 p{ sx[slot], sy[slot] };
         const int cell = choiceCell(s, p);
-        const uint32_t* sl = s.navSlot + size_t(cell) * size_t(s.K);
+        const uint32_t* sl = navSlots(s, cell);
         const uint32_t current = c.assign[id];
         uint32_t ex = current;
         const bool hasPath = slotExitOf(sl[0]) != CHOICE_EMPTY;
@@ -615,7 +615,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_choice_apply(int n, SceneView s0
         if (k >= 0) {
             binned = true;
             akey = uint32_t(r.aheadBase + size_t(1u - par) * size_t(c.exitStride) * CHOICE_AHEAD_BINS) + ex * uint32_t(CHOICE_AHEAD_BINS) +
-                   aheadBin(s.navDist[size_t(cell) * size_t(s.K) + size_t(k)]);
+                   aheadBin(navDists(s, cell)[k]);
         }
         const V2 dir = choiceDirection(s, cell, p, ex);
         // the wall term: the plan's wall boxes, or on a venue's own step its exact wall at the centre's height
@@ -815,7 +815,7 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
             const int kk = slotOfExit(s, cell, ex);
             if (kk < 0) efixed = true;
             else {
-                const V2 a = anchorCentre(s, s.navSlot[size_t(cell) * size_t(s.K) + size_t(kk)] & CHOICE_ANCHOR_MASK);
+                const V2 a = anchorCentre(s, navSlots(s, cell)[kk] & CHOICE_ANCHOR_MASK);
                 rec->anchor_x = a.x; rec->anchor_y = a.y;
                 if (!(len2(a - p) > GOAL_DEADZONE)) efixed = true;
                 else {   // the wall rule: e = unit(e0 - n (e0 . n)) with n frozen; nothing left: a constant 0
@@ -1505,9 +1505,9 @@ __global__ void __launch_bounds__(256) k_exit_loss(int n, SceneView s0, BatchVie
             const int cell = choiceCell(s, p);
             const int k = slotOfExit(s, cell, ex);
             if (k >= 0) {
-                const V2 a = anchorCentre(s, s.navSlot[size_t(cell) * size_t(s.K) + size_t(k)] & CHOICE_ANCHOR_MASK);
+                const V2 a = anchorCentre(s, navSlots(s, cell)[k] & CHOICE_ANCHOR_MASK);
                 const V2 c = anchorCentre(s, uint32_t(cell));
-                const float base = slotDistOf(s, s.navDist[size_t(cell) * size_t(s.K) + size_t(k)]) - len2(c - a);
+                const float base = slotDistOf(s, navDists(s, cell)[k]) - len2(c - a);
                 const float D = base + len2(p - a);
                 r = swarm_exit_rec{ a.x, a.y, base, 1u };
                 part = D * D;

@@ -46,19 +46,34 @@ void DeviceScene::upload(const Scene& s) {
     v.navRX = s.nav.resX; v.navRY = s.nav.resY; v.navOX = s.nav.originX; v.navOY = s.nav.originY;
     v.navCell = s.nav.cell; v.navInvCell = s.nav.invCell; v.K = s.K; v.E = s.E;
     v.navInfo = static_cast<const uint32_t*>(put(s.navInfo.data(), s.navInfo.size() * 4));
-    v.navSlot = static_cast<const uint32_t*>(put(s.navSlot.data(), s.navSlot.size() * 4));
-    v.navDist = static_cast<const uint16_t*>(put(s.navDist.data(), s.navDist.size() * 2));
+    {   // one record per nav cell: class bits, K slots, K distance quanta, padded to 16 bytes
+        const size_t cells = s.navInfo.size(), K = size_t(s.K);
+        const size_t stride = (1 + K + (K + 1) / 2 + 3) / 4 * 4;
+        if (s.navSlot.size() != cells * K || s.navDist.size() != cells * K)
+            throw std::invalid_argument("DeviceScene: navigation tables of inconsistent sizes");
+        std::vector<uint32_t> rec(cells * stride, 0u);
+        for (size_t c = 0; c < cells; ++c) {
+            uint32_t* r = rec.data() + c * stride;
+            r[0] = s.navInfo[c];
+            std::memcpy(r + 1, s.navSlot.data() + c * K, K * 4);
+            std::memcpy(r + 1 + K, s.navDist.data() + c * K, K * 2);
+        }
+        v.navRec = static_cast<const uint32_t*>(put(rec.data(), rec.size() * 4));
+        v.navStride = int(stride);
+    }
     v.invCapS = static_cast<const float*>(put(s.invCapS.data(), s.invCapS.size() * 4));
     v.secondsPerUnit = s.secondsPerUnit; v.distQuantum = s.distQuantum;
     v.stair = static_cast<const float*>(put(s.stair.data(), s.stair.size() * 4));
     std::vector<float> ex;
     for (const Rect& r : s.exits) { ex.push_back(r.lox); ex.push_back(r.loy); ex.push_back(r.hix); ex.push_back(r.hiy); }
     v.exitRects = static_cast<const float*>(put(ex.data(), ex.size() * 4));
-    std::vector<float> wr;
-    for (const Rect& r : s.walls) { wr.push_back(r.lox); wr.push_back(r.loy); wr.push_back(r.hix); wr.push_back(r.hiy); }
-    v.wallRects = static_cast<const float*>(put(wr.data(), wr.size() * 4));
+    std::vector<float4> wb(s.wallBinList.size());   // each bin entry's box, in the bins' (ascending box) order
+    for (size_t k = 0; k < wb.size(); ++k) {
+        const Rect& r = s.walls.at(s.wallBinList[k]);
+        wb[k] = make_float4(r.lox, r.loy, r.hix, r.hiy);
+    }
     v.wallBinStart = static_cast<const uint32_t*>(put(s.wallBinStart.data(), s.wallBinStart.size() * 4));
-    v.wallBinList = static_cast<const uint32_t*>(put(s.wallBinList.data(), s.wallBinList.size() * 4));
+    v.wallBinBox = static_cast<const float4*>(put(wb.data(), wb.size() * sizeof(float4)));
     v.venue = s.hasVenue ? static_cast<const seated::Layout*>(put(&s.venue, sizeof s.venue)) : nullptr;
     v.venue3d = s.venue3d;
     v.exitDepth = s.exitDepth;
