@@ -284,13 +284,16 @@ int swarm_get_physics_shape(swarm_ctx* c, int* block, int* grid);
  * candidates that cannot touch in the iterations left. Results are the same bit for bit either way. No counterpart
  * in the reference (its projection scans the 3 x 3 cell ring every iteration, physics.comp:1367-1418). */
 int swarm_set_projection_prune(swarm_ctx* c, int every);
-/* A seated venue's own 3D step (swarm_scene_build_seated): the projection's first iteration scans the 3 x 3 cell ring
- * and keeps, in ring order, every body with a height overlap that can still come into contact; the later iterations
- * walk that list. Its reach assumes each body moves at most 'fraction' x (P - 1) x PROJ_MAX_SHIFT over the step; every
- * iteration checks the bodies' path lengths against it, and after one that exceeds it the iterations scan the ring
- * again. fraction in [0, 1], default 1; 0: no lists (the ring every iteration, as physics.comp:1367-1418 does).
- * Results are the same bit for bit for every fraction. No counterpart in the reference. */
-int swarm_set_candidate_budget(swarm_ctx* c, float fraction);
+/* The projection's candidate lists' displacement budget, in maximal moves (PROJ_MAX_SHIFT_RADII r each) over a step.
+ * The first iteration scans the cell ring and keeps, in ring order, every candidate within D + moves (shift_i +
+ * shift_j) (a seated venue's own step: every body with a height overlap within D + 2 min(moves, P - 1) PROJ_MAX_SHIFT);
+ * the later iterations walk that list and check every body's path length against moves x its shift; after an
+ * iteration where one exceeds it, that scenario's iterations scan the ring again. moves >= 0 (infinity allowed):
+ * default 12 for the plain and batched steps (moves >= P: the lists assume P moves and need no check, as every
+ * iteration moves a body by at most one shift), infinity for a seated venue's own step (whose fence and wall plane can
+ * push further; P - 1 moves, checked); 0: no lists, the ring every iteration, as physics.comp:1367-1418 does. Results
+ * are the same bit for bit for every budget. No counterpart in the reference. */
+int swarm_set_candidate_budget(swarm_ctx* c, float moves);
 /* Lanes per slot in the projection's iterations over the candidate lists (1 default, 2, 4 or 8): the lanes load
  * and test a slot's candidates together and add the contact terms in candidate order, so results are the same bit
  * for bit. More lanes pay where slots are few and candidates many (4 lanes: 4,096 agents in the 40 x 30 m room,

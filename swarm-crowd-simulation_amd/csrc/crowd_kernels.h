@@ -20,6 +20,10 @@ constexpr int NBR_CAP = 48;        // contact candidates per slot
 constexpr int PROJ_CHUNK = 8;      // candidates loaded together by one projection thread
 constexpr int PROJ_CHUNK_ENSEMBLE = 4;   // the same in the batched step's list iterations (more waves per CU)
 constexpr int PROJ_PRUNE_EVERY = 8; // projection iterations between candidate prunings (0: never)
+// The plain and batched steps' candidate budget, in maximal moves per step (the lists' reach assumes no body moves
+// farther over the projection; checked every iteration). At the 63,424-agent bowl 99.99 % of the bodies' projection
+// paths stay under 2.75 moves, and a path past 12 comes about once in 200 steps (the oracle, 400 steps).
+constexpr float PROJ_LIST_MOVES = 12.0f;
 constexpr int PROJ_LANES = 1;       // lanes per slot in the projection iterations that do not prune (autotuned)
 constexpr int PROJ_COLLECT_LANES = 4;   // lanes per slot in the first projection iteration (ring scan + candidates)
 constexpr int PRED_LANES = 4;           // lanes per slot in predict (its perception ring and wall contacts)
@@ -91,6 +95,14 @@ struct StepBuffers {
     // the step's control words; ctrl[CTRL_NACTIVE] is the active slot count: the inactive key sorts last, so
     // slots [ctrl[CTRL_NACTIVE], n) are the inactive ones
     const uint32_t* ctrl;
+    // The candidate lists' displacement budget, in maximal moves (PROJ_MAX_SHIFT_RADII r each): > 0, the lists hold the
+    // candidates within D + listMoves (shift_i + shift_j) and every iteration checks its slots' path lengths against
+    // listMoves shift_i; a slot past it marks its scenario's listBroken word, and that scenario's later iterations scan
+    // the ring. < 0: the lists hold every candidate within reach over all P iterations (no checks needed); 0: no lists,
+    // the ring every iteration. listBroken (per scenario, nScen words) is set by predict: 0, or 1 for listMoves == 0.
+    float listMoves;
+    uint32_t* listBroken;
+    int nScen;
 };
 
 // the projection's slot-order ping-pong
@@ -192,7 +204,7 @@ struct StepLaunch {
     int collectLanes = 1; // lanes per slot in the first projection iteration (1, 2, 4 or 8)
     int predictLanes = 1; // lanes per slot in predict (1, 2, 4 or 8)
     bool bitonic = false; // the sort: the reference GPU's bitonic network instead of the stable counting sort
-    float listBudget = 1.0f;  // a seated venue's own step: its candidate lists' displacement budget (0: no lists)
+    float listMoves = 0.0f;   // a seated venue's own step: its candidate lists' budget in maximal moves (0: no lists)
 };
 
 void launchGridPasses(hipStream_t st, const StepLaunch& L, AgentHalf rd, const GridBuffers& g, StepBuffers& b, const float* rad,
@@ -216,6 +228,14 @@ void launchPhysicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, fl
 void launchOverlap(hipStream_t st, const StepLaunch& L, StepBuffers& b, AgentHalf wr, float weight, double* loss, const TapeStep* tp);
 void launchExitLoss(hipStream_t st, const StepLaunch& L, AgentHalf s, const uint32_t* assign, swarm_exit_rec* out, double* loss);
 void launchStepIncrement(hipStream_t st, uint32_t* ctrl);
+// count arrays of n 32-bit words, src[a] -> dst[a], in one launch
+constexpr int TAPE_COPY_MAX = 8;
+struct TapeCopy {
+    const float* src[TAPE_COPY_MAX];
+    float* dst[TAPE_COPY_MAX];
+    int count;
+};
+void launchTapeCopy(hipStream_t st, int n, const TapeCopy& c);
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes);
 // The stable sort of n caller keys (cells < C, or INACTIVE_CELL) into ascending (key, index) order on the bitonic
 // network with whole 64-bit compares (a total order: no rank loops, O(n log^2 n) whatever the keys), and the cell

@@ -855,10 +855,18 @@ __device__ inline void predictSlot(int slot, const SceneView& s0, const BatchVie
     b.planes[slot] = make_float4(wn.x, wn.y, dot2(np, wn) - wall[2], (nonzero(wn) ? 1.0f : 0.0f) + (grounded ? 2.0f : 0.0f));
 }
 
+// the step's candidate-budget words (per scenario): whole at the start, or broken from the start without lists
+__device__ inline void resetListBudget(const StepBuffers& b) {
+    if (b.listMoves < 0.0f) return;
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < b.nScen; i += int(gridDim.x * blockDim.x))
+        b.listBroken[i] = b.listMoves == 0.0f ? 1u : 0u;
+}
+
 // the physics kernels loop over slots with the grid's stride: one slot per thread, or a persistent grid
 template <bool B, bool REC = false>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b, ProjBuffers out,
                                                             TapeStep tp = TapeStep{}) {
+    resetListBudget(b);
     // the taped forward clears every slot's record; otherwise the inactive slots (the last ones) are left alone
     const int end = REC ? n : int(b.ctrl[CTRL_NACTIVE]);
     for (int slot = xcdBlock() * blockDim.x + threadIdx.x; slot < end; slot += gridDim.x * blockDim.x)
@@ -870,6 +878,7 @@ template <bool B, int Q>
 __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_predict_lanes(int n, SceneView s0, BatchView bt, ModelParams mp, StepBuffers b,
                                                                   ProjBuffers out) {
     static_assert(Q == 2 || Q == 4 || Q == 8, "lanes per slot");
+    resetListBudget(b);
     const int total = int(b.ctrl[CTRL_NACTIVE]) * Q;   // the inactive slots (the last ones) are left alone
     for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
         predictSlot<B, false, Q>(t / Q, s0, bt, mp, b, out, TapeStep{}, t % Q);
@@ -921,7 +930,9 @@ __device__ inline V2 overlapCorrectionAt(V2 offset, float dxy2, float D, const u
 // wall plane is a projection onto a half-space holding the predicted position, which moves it no farther), so over
 // the P iterations of a step a pair can touch only if its predicted distance is below D + P (shift_i + shift_j).
 // The candidates are the ring's active slots within that bound (with 0.1% of float slack), in ring order, so the
-// later iterations' sums over them are the ring scan's sums bit for bit.
+// later iterations' sums over them are the ring scan's sums bit for bit. With a displacement budget (b.listMoves > 0)
+// the bound is D + listMoves (shift_i + shift_j): it holds while no path is longer than listMoves shifts, which the
+// iterations check (a scenario past it scans the ring for the rest of its step).
 __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, const ProjBuffers& in, int n, int slot, uint32_t key,
                                    const ScnRef& sr, int P, bool collect, float& ox, float& oy, float& oacc, uint64_t* mask = nullptr,
                                    bool* capped = nullptr, bool* planed = nullptr) {
@@ -936,7 +947,7 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
     int nContacts = 0;
     uint32_t cnt = 0;
     const size_t stride = size_t(n);
-    const float Pf = float(P);
+    const float Pf = b.listMoves > 0.0f ? b.listMoves : float(P);   // the moves the candidates' reach allows for
     const int R = s.projRing;
     const int x0 = cx - R < 0 ? 0 : cx - R;
     const int x1 = cx + R > s.hashNX - 1 ? s.hashNX - 1 : cx + R;
@@ -1057,7 +1068,8 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     const size_t stride = size_t(n);
-    const float remf = float(rem);
+    // a budget caps what the rest of the step can move each agent (a broken budget never reaches here)
+    const float remf = b.listMoves > 0.0f ? fminf(float(rem), b.listMoves) : float(rem);
     uint32_t kept = 0;
     for (uint32_t j0 = 0; j0 < cnt; j0 += CH) {
         uint32_t kk[CH];
@@ -1128,7 +1140,8 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
             continue;
         }
         const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
-        const uint32_t cnt = b.nbrCount[slot];
+        const bool broken = b.listMoves >= 0.0f && b.listBroken[sr.sc] != 0u;
+        const uint32_t cnt = broken ? NBR_OVERFLOW : b.nbrCount[slot];
         if (cnt == NBR_OVERFLOW) {   // the ring scan, on lane 0
             if (lane == 0) {
                 float x, y, oacc;
@@ -1136,10 +1149,12 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
                     uint64_t mask = 0ull;
                     bool capped = false, planed = false;
                     projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, false, x, y, oacc, &mask, &capped, &planed);
-                    atomicOr(tp.err, 1u);
+                    atomicOr(tp.err, broken ? 4u : 1u);
                     tp.mask[size_t(it - 1) * stride + size_t(slot)] = mask;
                 } else {
                     projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, key, sr, P, false, x, y, oacc);
+                    if (!broken && b.listMoves > 0.0f && oacc > b.listMoves * (PROJ_MAX_SHIFT_RADII * b.sr[slot]))
+                        atomicOr(&b.listBroken[sr.sc], 1u);
                 }
                 out.p[slot] = make_float2(x, y);
                 out.acc[slot] = oacc;
@@ -1195,9 +1210,11 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
         if (lane == 0) {
             float x, y, oacc;
             bool capped = false, planed = false;
-            finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, b.sr[slot], x, y, oacc, &capped, &planed);
+            const float ri = b.sr[slot];
+            finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc, &capped, &planed);
             out.p[slot] = make_float2(x, y);
             out.acc[slot] = oacc;
+            if (b.listMoves > 0.0f && oacc > b.listMoves * (PROJ_MAX_SHIFT_RADII * ri)) atomicOr(&b.listBroken[sr.sc], 1u);
             if constexpr (REC) {
                 tp.mask[size_t(it - 1) * stride + size_t(slot)] = mask;
                 if (capped) tp.capBits[slot] |= 1ull << (it - 1);
@@ -1220,7 +1237,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, Scene
     const int total = int(b.ctrl[CTRL_NACTIVE]) * Q;   // the inactive slots (the last ones) are left alone
     const size_t stride = size_t(n);
     const int groupLo = __lane_id() & ~(Q - 1);   // the slot's first lane in the wave (blockDim is a multiple of 64)
-    const float Pf = float(P);
+    const float Pf = b.listMoves > 0.0f ? b.listMoves : float(P);   // the moves the candidates' reach allows for
     for (int t = xcdBlock() * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const int slot = t / Q;
         const int lane = t % Q;
@@ -1299,6 +1316,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, Scene
             finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc);
             out.p[slot] = make_float2(x, y);
             out.acc[slot] = oacc;
+            if (b.listMoves > 0.0f && oacc > b.listMoves * (PROJ_MAX_SHIFT_RADII * ri)) atomicOr(&b.listBroken[sr.sc], 1u);
         }
     }
 }
@@ -1324,12 +1342,16 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
             x = h.p.x; y = h.p.y; acc = h.acc;
         } else {
             const ScnRef sr = scnOfSlot<B>(bt, uint32_t(slot));
-            const uint32_t cnt = first ? NBR_OVERFLOW : listed;
+            const bool broken = !first && b.listMoves >= 0.0f && b.listBroken[sr.sc] != 0u;
+            const uint32_t cnt = first || broken ? NBR_OVERFLOW : listed;
             if constexpr (REC) {
                 if (cnt == NBR_OVERFLOW) {
                     projectRing(sceneOf<B>(s0, bt, sr.sc), b, in, n, slot, b.skey[slot], sr, P, first != 0, x, y, acc, &mask, &capped,
                                 &planed);
-                    if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
+                    // the tape's masks index the candidate lists: a broken budget (bit 2; taped again without one) or an
+                    // overflowing list (bit 0) leaves this step's untaped
+                    if (broken) atomicOr(tp.err, 4u);
+                    else if (!first || b.nbrCount[slot] == NBR_OVERFLOW) atomicOr(tp.err, 1u);
                 } else {
                     projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                 }
@@ -1338,6 +1360,8 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project(int n, SceneView s0,
                 else if (prune > 0) projectListPrune<listChunk<B>()>(b, in, n, slot, cnt, sr, h, prune, x, y, acc);
                 else projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc);
             }
+            // the budget: a path past listMoves shifts makes the scenario's later iterations scan the ring
+            if (!broken && b.listMoves > 0.0f && acc > b.listMoves * (PROJ_MAX_SHIFT_RADII * h.ri)) atomicOr(&b.listBroken[sr.sc], 1u);
         }
         out.p[slot] = make_float2(x, y);
         out.acc[slot] = acc;
@@ -1385,13 +1409,14 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_final(int n, SceneView s0, B
                 x = h.p.x; y = h.p.y; acc = h.acc;
                 if constexpr (REC) tp.mask[size_t(mp.projIters - 1) * size_t(n) + size_t(slot)] = 0ull;
             } else {
-                const uint32_t cnt = useList ? listed : NBR_OVERFLOW;
+                const bool broken = b.listMoves >= 0.0f && b.listBroken[sr.sc] != 0u;
+                const uint32_t cnt = useList && !broken ? listed : NBR_OVERFLOW;
                 if constexpr (REC) {
                     uint64_t mask = 0ull;
                     bool capped = false, planed = false;
                     if (cnt == NBR_OVERFLOW) {
                         projectRing(s, b, in, n, slot, key, sr, mp.projIters, false, x, y, acc, &mask, &capped, &planed);
-                        atomicOr(tp.err, 1u);
+                        atomicOr(tp.err, broken ? 4u : 1u);
                     } else {
                         projectList<listChunk<B>()>(b, in, n, slot, cnt, sr, h, x, y, acc, &mask, &capped, &planed);
                     }
@@ -1552,6 +1577,17 @@ void launchNavProbe(hipStream_t st, const SceneView& s, int n, const float* x, c
     k_nav_probe<<<blocks(n, HASH_BLOCK), HASH_BLOCK, 0, st>>>(n, s, x, y, r, ex, cell, dir, goal);
 }
 
+// up to TAPE_COPY_MAX arrays of n words copied in one launch (the taped step's state copies)
+__global__ void __launch_bounds__(256) k_copy_words(int n, TapeCopy c) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int a = int(blockIdx.y);
+    if (i < n && a < c.count) c.dst[a][i] = c.src[a][i];
+}
+void launchTapeCopy(hipStream_t st, int n, const TapeCopy& c) {
+    if (n <= 0 || c.count <= 0) return;
+    k_copy_words<<<dim3(unsigned(blocks(n, 256)), unsigned(c.count)), 256, 0, st>>>(n, c);
+}
+
 void launchCopyProbe(hipStream_t st, const void* src, void* dst, size_t bytes) {
     k_copy_probe<<<4096, 256, 0, st>>>(static_cast<const float4*>(src), static_cast<float4*>(dst), bytes / 16);
 }
@@ -1703,8 +1739,9 @@ void launchPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, ProjBuff
 }
 
 template <bool B>
-static void physicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& b, float2* q, float* acc0, float* acc1, AgentHalf rd,
+static void physicsRecord(hipStream_t st, const StepLaunch& L, StepBuffers& bIn, float2* q, float* acc0, float* acc1, AgentHalf rd,
                           AgentHalf wr, EvacBuffers ev, uint32_t* ctrl, const TapeStep& tp) {
+    StepBuffers b = bIn;   // its listMoves: the caller's budget for the taped forward
     const int n = L.n;
     const int bs = L.physBlock > 0 ? L.physBlock : PHYS_BLOCK;
     const int g = blocks(n, bs);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <functional>
 
 namespace swarm {
@@ -197,6 +198,10 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     sb_.nbr = alloc<uint32_t>(size_t(NBR_CAP) * n);
     sb_.nbrD = alloc<float>(size_t(NBR_CAP) * n);
     sb_.nbrCount = alloc<uint32_t>(n);
+    sb_.listBroken = alloc<uint32_t>(S);
+    sb_.nScen = S_;
+    sb_.listMoves = -1.0f;
+    listMoves_ = venue3d_ ? std::numeric_limits<float>::infinity() : PROJ_LIST_MOVES;
     pa_.p = alloc<float2>(n); pa_.acc = alloc<float>(n);
     pb_.p = alloc<float2>(n); pb_.acc = alloc<float>(n);
     ch_.assign = alloc<uint32_t>(n); ch_.pending = alloc<uint32_t>(n);
@@ -251,7 +256,7 @@ StepLaunch HipCrowdSim::launchDesc() const {
     L.collectLanes = collectLanes_;
     L.predictLanes = predictLanes_;
     L.bitonic = bitonic_;
-    L.listBudget = listBudget_;
+    L.listMoves = listMoves_;
     return L;
 }
 
@@ -393,10 +398,20 @@ void HipCrowdSim::setSortMode(int mode) {
     bitonic_ = mode == 1;
 }
 
-void HipCrowdSim::setListBudget(float fraction) {
-    if (!(fraction >= 0.0f && fraction <= 1.0f)) throw std::invalid_argument("setListBudget: fraction must be in [0, 1]");
-    if (fraction != listBudget_) dropGraph();
-    listBudget_ = fraction;
+void HipCrowdSim::setCandidateBudget(float moves) {
+    if (!(moves >= 0.0f)) throw std::invalid_argument("setCandidateBudget: moves must be >= 0 (infinity allowed)");
+    if (moves != listMoves_) {
+        dropGraph();
+        dropAdjointGraphs();
+    }
+    listMoves_ = moves;
+    recBudgetOff_ = false;
+}
+
+float HipCrowdSim::stepListMoves() const {
+    // the overlap loss reads the candidate lists after the step: they stay whole (every pair within reach over P moves)
+    if (lossOn_) return -1.0f;
+    return listMoves_ >= float(mp_.projIters) ? -1.0f : listMoves_;
 }
 
 void HipCrowdSim::setPruneEvery(int every) {
@@ -503,6 +518,7 @@ void HipCrowdSim::enqueueStep(int parity) {
     StepBuffers b = sb_;
     b.avx = rd.vx;
     b.avy = rd.vy;
+    b.listMoves = stepListMoves();
     std::vector<hipEvent_t> ev;
     if (timing_) {   // stamps: start, sorted, choice, predict, projection, final; then a pair per later iteration
         ev.resize(6 + (timeLaunches_ ? 2 * size_t(std::max(0, mp_.projIters - 2)) : 0));
@@ -1001,7 +1017,21 @@ size_t HipCrowdSim::tapeBytesPerStep() const {   // allocAdjoint's tape, per ste
                 (P + 1) * 8) + nbs * 4;
 }
 
+// A taped step whose candidate lists broke their budget (the tape's contact masks index the lists) makes the call run
+// again, from its start, with whole lists: rare, and the results are the same either way.
 double HipCrowdSim::adjointForward(int K, int every, float radiusScale, float overlapWeight) {
+    try {
+        return adjointForwardRun(K, every, radiusScale, overlapWeight);
+    } catch (const TapeBudgetBroken&) {
+        restoreFrom(ckpt_[0], stream_);   // the run's start state
+        parity_ = ckptParity_[0];
+        recBudgetOff_ = true;
+        dropAdjointGraphs();
+        return adjointForwardRun(K, every, radiusScale, overlapWeight);
+    }
+}
+
+double HipCrowdSim::adjointForwardRun(int K, int every, float radiusScale, float overlapWeight) {
     if (K <= 0 || every <= 0) throw std::invalid_argument("adjointForward: K and the checkpoint interval must be positive");
     if (mp_.projIters < 2 || mp_.projIters > SWARM_TAPE_MAX_ITERS)
         throw std::invalid_argument("adjointForward: the adjoint needs 2 to 32 projection iterations");
@@ -1082,6 +1112,12 @@ void HipCrowdSim::checkTapeErrors() {
     sync();
     if (err & 1u) throw std::runtime_error("adjoint: a slot has more than 48 contact candidates (unsupported by the adjoint)");
     if (err & 2u) throw std::runtime_error("adjoint: more wall pushes in one resolveStatic than the tape holds");
+    if (err & 4u) throw TapeBudgetBroken();
+}
+
+// the taped forward's candidate budget (StepBuffers::listMoves): the context's, unless a taped step broke it once
+float HipCrowdSim::recListMoves() const {
+    return !recBudgetOff_ && listMoves_ > 0.0f && listMoves_ < float(mp_.projIters) ? listMoves_ : -1.0f;
 }
 
 // one taped step into tape buffer 'buf' (its step k), on stream st
@@ -1089,13 +1125,16 @@ void HipCrowdSim::tapedStep(int buf, int k, hipStream_t st) {
     const Tape& T = tape_[buf];
     const size_t n = size_t(n_), K = size_t(k), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
     const AgentHalf rd = half_[parity_], wr = half_[1 - parity_];
-    const size_t b = n * 4;
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.inx + K * n, rd.x, b, hipMemcpyDeviceToDevice, st));
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.iny + K * n, rd.y, b, hipMemcpyDeviceToDevice, st));
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.invx + K * n, rd.vx, b, hipMemcpyDeviceToDevice, st));
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.invy + K * n, rd.vy, b, hipMemcpyDeviceToDevice, st));
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.inw + K * n, rd.w, b, hipMemcpyDeviceToDevice, st));
+    {   // the step's input state into the tape, one launch
+        TapeCopy c{};
+        const float* src[5] = { rd.x, rd.y, rd.vx, rd.vy, rd.w };
+        float* dst[5] = { T.inx + K * n, T.iny + K * n, T.invx + K * n, T.invy + K * n, T.inw + K * n };
+        for (int a = 0; a < 5; ++a) { c.src[a] = src[a]; c.dst[a] = dst[a]; }
+        c.count = 5;
+        launchTapeCopy(st, n_, c);
+    }
     StepBuffers sb = sb_;
+    sb.listMoves = recListMoves();
     sb.sid = T.sid + K * n; sb.skey = T.skey + K * n; sb.bstart = T.bstart + K * nbs;
     sb.sx = T.sx + K * n; sb.sy = T.sy + K * n; sb.sr = T.sr + K * n;
     sb.egx = T.egx + K * n; sb.egy = T.egy + K * n;
@@ -1112,8 +1151,13 @@ void HipCrowdSim::tapedStep(int buf, int k, hipStream_t st) {
                  T.overlap + K * n, ch_.assign, (drive_ || (!batched_ && L.s.E == 0)) ? 1 : 0, T.err };
     launchPhysicsRecord(st, L, sb, T.q + K * (P + 1) * n, T.acc0, T.acc1, rd, wr, ev_, ctrl_, tp);
     launchOverlap(st, L, sb, wr, overlapWeight_, lossAcc_ + 0, &tp);
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.outx + K * n, wr.x, b, hipMemcpyDeviceToDevice, st));
-    SWARM_HIP_CHECK(hipMemcpyAsync(T.outy + K * n, wr.y, b, hipMemcpyDeviceToDevice, st));
+    {   // and its output positions
+        TapeCopy c{};
+        c.src[0] = wr.x; c.dst[0] = T.outx + K * n;
+        c.src[1] = wr.y; c.dst[1] = T.outy + K * n;
+        c.count = 2;
+        launchTapeCopy(st, n_, c);
+    }
     SWARM_HIP_CHECK(hipGetLastError());
     parity_ = 1 - parity_;
 }
@@ -1170,6 +1214,17 @@ AdjStep HipCrowdSim::adjStepView(int buf, int k) const {
 }
 
 void HipCrowdSim::adjointBackward(float* gradV0, double* gradGlobals) {
+    try {
+        adjointBackwardRun(gradV0, gradGlobals);
+    } catch (const TapeBudgetBroken&) {   // a segment taped again broke its budget: every segment taped again without
+        recBudgetOff_ = true;
+        dropAdjointGraphs();
+        tapedSeg_[0] = tapedSeg_[1] = -1;
+        adjointBackwardRun(gradV0, gradGlobals);
+    }
+}
+
+void HipCrowdSim::adjointBackwardRun(float* gradV0, double* gradGlobals) {
     if (adjK_ <= 0) throw std::invalid_argument("adjointBackward: run adjointForward first");
     const size_t n = size_t(n_), S = size_t(S_);
     // the exit term's adjoint from the forward's final positions
@@ -1301,6 +1356,17 @@ void HipCrowdSim::backwardSegment(int seg, int len, int buf) {
 }
 
 void HipCrowdSim::adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit) {
+    try {
+        adjointTapeRun(step, out, exit);
+    } catch (const TapeBudgetBroken&) {
+        recBudgetOff_ = true;
+        dropAdjointGraphs();
+        tapedSeg_[0] = tapedSeg_[1] = -1;
+        adjointTapeRun(step, out, exit);
+    }
+}
+
+void HipCrowdSim::adjointTapeRun(int step, const swarm_tape_view& out, swarm_exit_rec* exit) {
     if (adjK_ <= 0 || step < 0 || step >= adjK_) throw std::invalid_argument("adjointTape: no such step of the last forward");
     const int seg = step / adjEvery_;
     const int buf = tapeBuf(seg);

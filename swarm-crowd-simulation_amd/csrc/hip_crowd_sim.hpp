@@ -74,9 +74,11 @@ public:
     int physicsGrid() const { return physGrid_; }
     // Candidate pruning in the projection every 'every' iterations (0: off); results are the same bit for bit
     void setPruneEvery(int every);
-    // A seated venue's own 3D step: the displacement budget of the projection's candidate lists, a fraction of P - 1
-    // maximal moves (1 default; 0: no lists, the ring every iteration); results are the same bit for bit
-    void setListBudget(float fraction);
+    // The displacement budget of the projection's candidate lists, in maximal moves over the step (default 12; a seated
+    // venue's own step: infinity, i.e. P - 1); 0: no lists, the ring every iteration; results are the same bit for bit
+    void setCandidateBudget(float moves);
+    // the plain and batched steps' budget for this step (StepBuffers::listMoves)
+    float stepListMoves() const;
     // Lanes per slot in the projection's non-pruning iterations (1, 2, 4, 8); results are the same bit for bit
     void setProjectionLanes(int lanes);
     int projectionLanes() const { return projLanes_; }
@@ -125,12 +127,17 @@ public:
     // Runs K steps from the current state with the loss terms, keeping the state every 'every' steps; returns the
     // loss. The radii of the run are radiusScale x the uploaded radii.
     double adjointForward(int K, int every, float radiusScale, float overlapWeight);
+    double adjointForwardRun(int K, int every, float radiusScale, float overlapWeight);
     // dL/dv0 (n, agent order) and dL/d(tau_s, radius scale, wall gain) of the last adjointForward; per scenario in
     // a batch (globals: 3 per scenario). The state afterwards is the forward's final state, as the forward left it.
     // Throws NonFiniteGradient (after writing the outputs) when any gradient value is inf or NaN.
     void adjointBackward(float* gradV0, double* gradGlobals);
+    void adjointBackwardRun(float* gradV0, double* gradGlobals);
     // one step's tape of the last adjointForward (recomputed from its checkpoint)
     void adjointTape(int step, const swarm_tape_view& out, swarm_exit_rec* exit);
+    void adjointTapeRun(int step, const swarm_tape_view& out, swarm_exit_rec* exit);
+    // the taped forward's candidate budget (StepBuffers::listMoves)
+    float recListMoves() const;
     // the relaxation time (seconds) and wall gain of the next steps
     void setModel(float tauS, float obstacleGain);
     // Loss terms of the next adjointBackward beyond the built-in ones: position adjoints at evacuation crossings
@@ -210,7 +217,8 @@ private:
     int n_ = 0, C_ = 0, nb_ = 0, S_ = 1, maxExits_ = 0, exitStride_ = CHOICE_QUEUE_STRIDE;
     int physBlock_ = 0, physGrid_ = 0;
     int pruneEvery_ = PROJ_PRUNE_EVERY;
-    float listBudget_ = 1.0f;
+    float listMoves_ = PROJ_LIST_MOVES;
+    bool recBudgetOff_ = false;     // a taped step broke the budget once: the adjoint tapes whole lists from then on
     int projLanes_ = PROJ_LANES;
     int collectLanes_ = PROJ_COLLECT_LANES;
     int predictLanes_ = PRED_LANES;
@@ -312,6 +320,12 @@ private:
 // An adjoint whose gradient overflowed FP32 (the outputs are written; the C ABI returns SWARM_ENONFINITE)
 struct NonFiniteGradient : std::runtime_error {
     using std::runtime_error::runtime_error;
+};
+
+// A taped step whose candidate lists broke their displacement budget (caught inside: the call runs again with whole
+// lists)
+struct TapeBudgetBroken : std::runtime_error {
+    TapeBudgetBroken() : std::runtime_error("adjoint: a taped step's candidate lists broke their budget") {}
 };
 
 #define SWARM_HIP_CHECK(expr)                                                                              \

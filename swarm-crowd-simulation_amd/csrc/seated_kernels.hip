@@ -23,9 +23,9 @@ constexpr int SEATED_RING_CHUNK = 4;   // ring slots whose loads one thread issu
 // The contact candidates (the plain step's list buffers, slot-interleaved: index nbr[j * n + slot], height offset
 // nbrD[j * n + slot]). The first iteration's ring scan keeps, in ring order, every slot with a height overlap (heights
 // stay fixed through the projection) whose centre lies within D + 2 B of the slot's (predict's positions), B the
-// displacement budget: an iteration's contacts are tested at positions at most P - 1 moves from predict's, a move is
-// at most PROJ_MAX_SHIFT unless the fence or the wall plane pushes further, and every iteration checks its slots'
-// path lengths (acc, the sum of the moves) against B. A slot past it sets CTRL_LIST_BROKEN and the iterations after
+// displacement budget (min(listMoves, P - 1) maximal moves: an iteration's contacts are tested at positions at most
+// P - 1 moves from predict's, a move is at most PROJ_MAX_SHIFT unless the fence or the wall plane pushes further), and
+// every iteration checks its slots' path lengths (acc, the sum of the moves) against B. A slot past it sets CTRL_LIST_BROKEN and the iterations after
 // it scan the ring again; so a list holds, in the ring's order, every slot that can touch, and the sums are the ring
 // scan's bit for bit.
 struct ListBudget {
@@ -380,9 +380,9 @@ void launchSeatedPhysics(hipStream_t st, const StepLaunch& L, StepBuffers& b, Se
     const int n = L.n;
     const int g = blocksOf(n, SEATED_BLOCK);
     const int P = L.mp.projIters;
-    ListBudget lb{ L.listBudget > 0.0f && P >= 2 ? 1 : 0, 0.0f, 0.0f };
+    ListBudget lb{ L.listMoves > 0.0f && P >= 2 ? 1 : 0, 0.0f, 0.0f };
     if (lb.on) {
-        lb.budget = float(P - 1) * PROJ_MAX_SHIFT * L.listBudget;
+        lb.budget = (L.listMoves < float(P - 1) ? L.listMoves : float(P - 1)) * PROJ_MAX_SHIFT;
         const float r = (BODY_DIAM + 2.0f * lb.budget) * 1.001f + 1e-6f;
         lb.reach2 = r * r;
     }

@@ -490,9 +490,9 @@ int swarm_set_projection_prune(swarm_ctx* c, int every) {
     CTX_CHECK(c);
     return guarded([&] { c->sim->setPruneEvery(every); });
 }
-int swarm_set_candidate_budget(swarm_ctx* c, float fraction) {
+int swarm_set_candidate_budget(swarm_ctx* c, float moves) {
     CTX_CHECK(c);
-    return guarded([&] { c->sim->setListBudget(fraction); });
+    return guarded([&] { c->sim->setCandidateBudget(moves); });
 }
 int swarm_set_projection_lanes(swarm_ctx* c, int lanes) {
     CTX_CHECK(c);

@@ -533,10 +533,10 @@ class Sim:
         """Candidate pruning every `every` projection iterations (0: never); results are bit-identical."""
         check(lib().swarm_set_projection_prune(self.h, every))
 
-    def set_candidate_budget(self, fraction=1.0):
-        """A seated venue's own 3D step: the candidate lists' displacement budget (0: the cell ring every iteration);
-        bit-identical results for every fraction."""
-        check(lib().swarm_set_candidate_budget(self.h, fraction))
+    def set_candidate_budget(self, moves=12.0):
+        """The projection's candidate lists' displacement budget in maximal moves per step (0: the cell ring every
+        iteration; default 12, a seated venue's own step infinity); bit-identical results for every budget."""
+        check(lib().swarm_set_candidate_budget(self.h, moves))
 
     def set_projection_lanes(self, lanes=1):
         """Lanes per slot in the projection's list iterations (1, 2, 4, 8); bit-identical results."""

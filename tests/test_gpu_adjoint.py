@@ -341,6 +341,36 @@ def test_taped_forward_segments_match_recompute():
     assert out[(45, 0, True)][3] > out[(7, 0, True)][3]   # the whole run's tape is the larger
 
 
+def test_candidate_budget_in_the_tape():
+    """The taped steps use the candidate lists' budget too: the default (12 moves), and 0.3 moves, which breaks in the
+    dense room (a broken taped step makes the call run again with whole lists), give the loss and dL/dv0 of whole lists
+    bit for bit, through the forward and through a backward that recomputes its segments, and keep giving them after."""
+    n, K = 2048, 40
+    scene = sh.Scene.layout(sh.LAYOUT_ROOM)
+    a = sh.place_agents(sh.LAYOUT_ROOM, n, n, 11)
+    out = {}
+    for moves in (float("inf"), 12.0, 0.3):
+        for tape_budget in (None, 0):   # the whole run taped by the forward; every segment recomputed
+            sim = sh.Sim(scene, n)
+            sim.set_candidate_budget(moves)
+            if tape_budget is not None:
+                sim.set_tape_budget(tape_budget)
+            sim.upload(a)
+            print(f"adjoint with candidate budget {moves}, tape budget {tape_budget}", flush=True)
+            loss = sim.adjoint_forward(K, 10, 1.0, 0.1)
+            g1 = sim.adjoint_backward()[0]
+            sim.upload(a)
+            loss2 = sim.adjoint_forward(K, 10, 1.0, 0.1)   # a second run on the same context
+            g2 = sim.adjoint_backward()[0]
+            out[(moves, tape_budget)] = (loss, g1, loss2, g2)
+            sim.close()
+    ref = out[(float("inf"), None)]
+    for key, (loss, g1, loss2, g2) in out.items():
+        assert loss == ref[0] and loss2 == ref[0], key
+        assert_bits(g1, ref[1], f"dL/dv0, budget {key}")
+        assert_bits(g2, ref[1], f"dL/dv0 of the second run, budget {key}")
+
+
 def run_k200(n, budget=0, seed=1337):
     scene = sh.Scene.layout(sh.LAYOUT_ROOM)
     a = sh.place_agents(sh.LAYOUT_ROOM, n, n, seed)

@@ -99,6 +99,20 @@ def test_mixed_batch_equals_plain_runs():
     run_and_compare(b, plains, [1, 2, 18, 24])
 
 
+def test_candidate_budget_per_scenario():
+    """A budget of half a move breaks the lists of the scenarios whose bodies move farther (their own flags: the other
+    scenarios keep their lists); every scenario still equals its plain run without lists (the ring every iteration)."""
+    layouts = list(sh.BATCH_LAYOUTS)
+    scen_layout = [0, 1, 2, 3, 2, 0]
+    totals = [2048, 1500, 3000, 1600, 2200, 4096]
+    actives = [1800, 1500, 2900, 1200, 1000, 4000]
+    b, plains = make_batch(layouts, scen_layout, totals, actives)
+    b.set_candidate_budget(0.5)
+    for p in plains:
+        p.set_candidate_budget(0.0)
+    run_and_compare(b, plains, [1, 5, 20, 40])
+
+
 def test_launch_shapes_and_autotune_keep_results():
     layouts = list(sh.BATCH_LAYOUTS)
     scen_layout = [0, 1, 2, 3]

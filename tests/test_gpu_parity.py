@@ -216,6 +216,47 @@ def test_candidate_pruning_is_exact(layout, active, total, steps):
             assert np.array_equal(u, v), f"evacuations, {cfg}"
 
 
+@pytest.mark.parametrize("layout,active,total,steps", [(sh.LAYOUT_ROOM, 4096, 4096, 150), (sh.LAYOUT_BOWL, 63424, 65536, 40)])
+def test_candidate_budget_is_exact(layout, active, total, steps):
+    """The candidate lists' displacement budget (swarm_set_candidate_budget): no budget (the lists reach over all P
+    moves, no checks), the default 12 moves, 1 and 0.5 moves (the lists break in most steps and those steps' later
+    iterations scan the ring), with pruning and 4 lanes per slot in the list and collecting iterations, and no lists at
+    all (the ring every iteration), give the same bits every 10 steps (evacuation records included); the default lists
+    are shorter than the unbudgeted ones."""
+    scene = sh.Scene.layout(layout)
+    a = sh.place_agents(layout, active, total, 21)
+    runs = []
+    counts = {}
+    for cfg in ((float("inf"), 0, 1), (float("inf"), 8, 1), (12.0, 8, 1), (1.0, 8, 4), (0.5, 0, 1), (0.0, 8, 1), (12.0, 0, 4)):
+        moves, every, lanes = cfg
+        s = sh.Sim(scene, total)
+        s.set_candidate_budget(moves)
+        s.set_projection_prune(every)
+        s.set_projection_lanes(lanes)
+        s.set_projection_collect_lanes(lanes)
+        s.upload(a)
+        print(f"candidate budget {cfg}", flush=True)
+        snaps = []
+        for k in range(steps // 10):
+            s.step(10)
+            snaps.append(s.state())
+        counts[cfg] = s.candidate_lists()[0]
+        runs.append((cfg, snaps, s.assign(), s.evac()))
+        s.close()
+    cfg0, snaps0, asg0, ev0 = runs[0]
+    for cfg, snaps, asg, ev in runs[1:]:
+        for k, (st, st0) in enumerate(zip(snaps, snaps0)):
+            for f in FIELDS:
+                assert_bits(st[f], st0[f], f"budget {cfg}, step {10 * (k + 1)}: {f}")
+        assert np.array_equal(asg, asg0)
+        for u, v in zip(ev, ev0):
+            assert np.array_equal(u, v), f"evacuations, {cfg}"
+    # the lists of the last step's collecting iteration (unpruned runs): shorter with the budget
+    c_inf, c_12 = counts[(float("inf"), 0, 1)], counts[(12.0, 0, 4)]
+    live = (c_inf != 0xFFFFFFFF) & (c_12 != 0xFFFFFFFF)
+    assert live.sum() > active // 2 and c_12[live].sum() < c_inf[live].sum()
+
+
 def test_collecting_lanes_write_the_same_lists():
     """After the first iteration the candidate lists (count, indices, contact distances) of 2, 4 and 8 collecting
     lanes are the single lane's, entry for entry, on the bowl and in the dense room."""
