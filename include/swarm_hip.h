@@ -288,11 +288,12 @@ int swarm_set_projection_prune(swarm_ctx* c, int every);
  * The first iteration scans the cell ring and keeps, in ring order, every candidate within D + moves (shift_i +
  * shift_j) (a seated venue's own step: every body with a height overlap within D + 2 min(moves, P - 1) PROJ_MAX_SHIFT);
  * the later iterations walk that list and check every body's path length against moves x its shift; after an
- * iteration where one exceeds it, that scenario's iterations scan the ring again. moves >= 0 (infinity allowed):
- * default 12 for the plain and batched steps (moves >= P: the lists assume P moves and need no check, as every
- * iteration moves a body by at most one shift), infinity for a seated venue's own step (whose fence and wall plane can
- * push further; P - 1 moves, checked); 0: no lists, the ring every iteration, as physics.comp:1367-1418 does. Results
- * are the same bit for bit for every budget. No counterpart in the reference. */
+ * iteration where one exceeds it, that scenario's iterations scan the ring again. moves >= 0 (infinity allowed),
+ * default 12 (at the 63,424-agent bowl and at venue L almost no body moves that far in one step's projection). On the
+ * plain and batched steps moves >= P needs no check (every iteration moves a body by at most one shift); a seated
+ * venue's own step uses at most P - 1 moves and always checks (its fence and wall plane can push further). 0: no lists,
+ * the ring every iteration, as physics.comp:1367-1418 does. Results are the same bit for bit for every budget. No
+ * counterpart in the reference. */
 int swarm_set_candidate_budget(swarm_ctx* c, float moves);
 /* Lanes per slot in the projection's iterations over the candidate lists (1 default, 2, 4 or 8): the lanes load
  * and test a slot's candidates together and add the contact terms in candidate order, so results are the same bit

@@ -201,7 +201,6 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     sb_.listBroken = alloc<uint32_t>(S);
     sb_.nScen = S_;
     sb_.listMoves = -1.0f;
-    listMoves_ = venue3d_ ? std::numeric_limits<float>::infinity() : PROJ_LIST_MOVES;
     pa_.p = alloc<float2>(n); pa_.acc = alloc<float>(n);
     pb_.p = alloc<float2>(n); pb_.acc = alloc<float>(n);
     ch_.assign = alloc<uint32_t>(n); ch_.pending = alloc<uint32_t>(n);

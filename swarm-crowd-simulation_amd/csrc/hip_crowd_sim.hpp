@@ -75,7 +75,7 @@ public:
     // Candidate pruning in the projection every 'every' iterations (0: off); results are the same bit for bit
     void setPruneEvery(int every);
     // The displacement budget of the projection's candidate lists, in maximal moves over the step (default 12; a seated
-    // venue's own step: infinity, i.e. P - 1); 0: no lists, the ring every iteration; results are the same bit for bit
+    // venue's own step at most P - 1); 0: no lists, the ring every iteration; results are the same bit for bit
     void setCandidateBudget(float moves);
     // the plain and batched steps' budget for this step (StepBuffers::listMoves)
     float stepListMoves() const;

@@ -535,7 +535,7 @@ class Sim:
 
     def set_candidate_budget(self, moves=12.0):
         """The projection's candidate lists' displacement budget in maximal moves per step (0: the cell ring every
-        iteration; default 12, a seated venue's own step infinity); bit-identical results for every budget."""
+        iteration; default 12); bit-identical results for every budget."""
         check(lib().swarm_set_candidate_budget(self.h, moves))
 
     def set_projection_lanes(self, lanes=1):

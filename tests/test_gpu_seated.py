@@ -119,11 +119,12 @@ def test_seated_m_matches_reference(kat):
 
 def test_seated_candidate_lists_match_ring_scan(kat, scene_s):
     """The projection's candidate lists (swarm_set_candidate_budget) against the ring scan every iteration (budget 0):
-    venue S from its seat spawn through the congested gangways, 400 steps, bit for bit at the default budget (P - 1
-    maximal moves) and at 0.62 moves (which breaks its lists early in most steps and falls back to the ring); the lists
-    are collected (counts within capacity for nearly every active slot) and shorter at the smaller budget."""
+    venue S from its seat spawn through the congested gangways, 400 steps, bit for bit with no budget beyond the
+    projection's P - 1 maximal moves (infinity), at the default 12 and at 0.62 moves (which breaks its lists early in most
+    steps and falls back to the ring); the lists are collected (counts within capacity for nearly every active slot)
+    and shorter at the smaller budget."""
     runs = {}
-    for budget in (0.0, float("inf"), 0.62):
+    for budget in (0.0, float("inf"), 12.0, 0.62):
         sim = sh.Sim(scene_s, N)
         sim.set_candidate_budget(budget)
         sim.upload3(kat["seated__S__init__pos"], kat["seated__S__init__vel"])
@@ -135,7 +136,7 @@ def test_seated_candidate_lists_match_ring_scan(kat, scene_s):
         cnt = sim.candidate_lists()[0]
         runs[budget] = (states, cnt)
         sim.close()
-    for budget in (float("inf"), 0.62):
+    for budget in (float("inf"), 12.0, 0.62):
         for k, ((p, v), (rp, rv)) in enumerate(zip(runs[budget][0], runs[0.0][0])):
             assert np.array_equal(p.view(np.uint32), rp.view(np.uint32)), f"budget {budget}, step {100 * (k + 1)}: positions"
             assert np.array_equal(v.view(np.uint32), rv.view(np.uint32)), f"budget {budget}, step {100 * (k + 1)}: velocities"
