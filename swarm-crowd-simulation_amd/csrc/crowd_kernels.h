@@ -89,7 +89,8 @@ struct StepBuffers {
     float4* planes;                         // wall plane (normal, offset, flags)
     const float* avx; const float* avy;     // agent order, start of step
     const float* v0;                        // agent order
-    // the projection's contact candidates, slot-interleaved: nbr[j * n + slot], their contact distances nbrD;
+    // the projection's contact candidates, slot-interleaved: nbr[j * n + slot] (a pair's contact distance is the radii's
+    // sum, formed where it is used); nbrD[j * n + slot]: on a seated venue's own step, the candidate's height offset;
     // nbrCount[slot] (NBR_OVERFLOW: more than NBR_CAP, the slot scans its ring instead)
     uint32_t* nbr; float* nbrD; uint32_t* nbrCount;
     // the step's control words; ctrl[CTRL_NACTIVE] is the active slot count: the inactive key sorts last, so

@@ -69,7 +69,8 @@ constexpr int CHOICE_AHEAD_ROUNDS = 2;   // wave leaders for the distance bins b
 // ---------------- grid passes ----------------
 // Agent i's offset in its bucket. The block's inactive agents are its own bucket (C + block): their offsets are
 // their rank by agent id, from the block's ballots, and the bucket's count is one store. Cell buckets take an
-// atomic offset. Every thread of the block must call it.
+// atomic offset. Every thread of the block must call it, with i = (the block's number in agent order) HASH_BLOCK +
+// threadIdx.x.
 __device__ inline void bucketOffsets(int i, int n, uint32_t k, uint32_t* __restrict__ loc, uint32_t* __restrict__ bcnt, uint32_t C) {
     __shared__ uint32_t waveInactive[HASH_BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -84,14 +85,14 @@ __device__ inline void bucketOffsets(int i, int n, uint32_t k, uint32_t* __restr
     }
     if (inactive) loc[i] = before + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
     else if (i < n) loc[i] = atomicAdd(&bcnt[k], 1u);
-    if (threadIdx.x == 0) bcnt[C + blockIdx.x] = total;
+    if (threadIdx.x == 0) bcnt[C + uint32_t(i) / uint32_t(HASH_BLOCK)] = total;   // the block of agents i belongs to
 }
 
 template <bool B>
 __global__ void __launch_bounds__(HASH_BLOCK) k_hash(int n, SceneView s0, BatchView bt, float* __restrict__ ax, float* __restrict__ ay,
                                                      float* __restrict__ aw, float* __restrict__ az, uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ loc, uint32_t* __restrict__ bcnt, uint32_t C) {
-    const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    const int i = xcdBlock() * HASH_BLOCK + threadIdx.x;
     uint32_t k = INACTIVE_CELL;
     if (i < n) {
         float x = ax[i], y = ay[i], w = aw[i];
@@ -182,7 +183,7 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(uint32_t* __restrict_
 
 __global__ void __launch_bounds__(HASH_BLOCK) k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ loc,
                                                         const uint32_t* __restrict__ bstart, uint32_t* __restrict__ tmp, uint32_t C) {
-    const int i = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    const int i = xcdBlock() * HASH_BLOCK + threadIdx.x;
     if (i >= n) return;
     tmp[bstart[bucketOf(key[i], uint32_t(i), C)] + loc[i]] = uint32_t(i);
 }
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(HASH_BLOCK) k_fix_gather(int n, BatchView bt, 
                                                            uint32_t* __restrict__ skey, const float* __restrict__ ax,
                                                            const float* __restrict__ ay, const float* __restrict__ rad,
                                                            float* __restrict__ sx, float* __restrict__ sy, float* __restrict__ sr) {
-    const int s = blockIdx.x * HASH_BLOCK + threadIdx.x;
+    const int s = xcdBlock() * HASH_BLOCK + threadIdx.x;
     if (s >= n) return;
     const uint32_t id = tmp[s];
     const uint32_t k = key[id];
@@ -980,10 +981,7 @@ __device__ inline void projectRing(const SceneView& s, const StepBuffers& b, con
                 if (collect) {
                     const float reach = (D + Pf * (PROJ_MAX_SHIFT_RADII * D)) * 1.001f + 1e-6f;
                     if (dxy2 < reach * reach) {
-                        if (cnt < uint32_t(NBR_CAP)) {
-                            b.nbr[size_t(cnt) * stride + size_t(slot)] = k;
-                            b.nbrD[size_t(cnt) * stride + size_t(slot)] = D;
-                        }
+                        if (cnt < uint32_t(NBR_CAP)) b.nbr[size_t(cnt) * stride + size_t(slot)] = k;
                         ++cnt;
                     }
                 }
@@ -1024,9 +1022,10 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
     V2 corr{ 0.0f, 0.0f };
     int nContacts = 0;
     const size_t stride = size_t(n);
-    // candidates in chunks: the chunk's index and position loads are all in flight before the first use; the
-    // sum still runs in candidate order. A chunk's entries past the count load the last candidate again (no
-    // branch around the loads) and are never used.
+    // candidates in chunks: the chunk's index loads, then its positions and radii, are all in flight before the first
+    // use; the sum still runs in candidate order. A chunk's entries past the count load the last candidate again (no
+    // branch around the loads) and are never used. A pair's contact distance is the radii's sum, as the collecting
+    // scan formed it (the lists hold indices only).
     for (uint32_t j0 = 0; j0 < cnt; j0 += CH) {
         uint32_t kk[CH];
         float dd[CH];
@@ -1034,12 +1033,13 @@ __device__ inline void projectList(const StepBuffers& b, const ProjBuffers& in, 
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const uint32_t j = j0 + uint32_t(u) < cnt ? j0 + uint32_t(u) : cnt - 1u;
-            const size_t at = size_t(j) * stride + size_t(slot);
-            kk[u] = b.nbr[at];
-            dd[u] = b.nbrD[at];
+            kk[u] = b.nbr[size_t(j) * stride + size_t(slot)];
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) pp[u] = in.p[kk[u]];
+        for (int u = 0; u < CH; ++u) {
+            pp[u] = in.p[kk[u]];
+            dd[u] = ri + b.sr[kk[u]];
+        }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             if (j0 + uint32_t(u) >= cnt) break;
@@ -1078,12 +1078,13 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const bool in_ = j0 + uint32_t(u) < cnt;
-            const size_t at = size_t(j0 + uint32_t(u)) * stride + size_t(slot);
-            kk[u] = in_ ? b.nbr[at] : uint32_t(slot);
-            dd[u] = in_ ? b.nbrD[at] : 0.0f;
+            kk[u] = in_ ? b.nbr[size_t(j0 + uint32_t(u)) * stride + size_t(slot)] : uint32_t(slot);
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) pp[u] = in.p[kk[u]];
+        for (int u = 0; u < CH; ++u) {
+            pp[u] = in.p[kk[u]];
+            dd[u] = ri + b.sr[kk[u]];
+        }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             if (j0 + uint32_t(u) >= cnt) break;
@@ -1095,10 +1096,8 @@ __device__ inline void projectListPrune(const StepBuffers& b, const ProjBuffers&
             }
             const float reach = (dd[u] + remf * (PROJ_MAX_SHIFT_RADII * dd[u])) * 1.001f + 1e-6f;
             if (dxy2 < reach * reach) {
-                if (kept != j0 + uint32_t(u)) {   // reads of this chunk are done; later chunks sit above 'kept'
-                    b.nbr[size_t(kept) * stride + size_t(slot)] = kk[u];
-                    b.nbrD[size_t(kept) * stride + size_t(slot)] = dd[u];
-                }
+                // reads of this chunk are done; later chunks sit above 'kept'
+                if (kept != j0 + uint32_t(u)) b.nbr[size_t(kept) * stride + size_t(slot)] = kk[u];
                 ++kept;
             }
         }
@@ -1162,6 +1161,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
             continue;
         }
         const uint32_t self = b.sid[slot] - sr.agentBase;
+        const float ri = b.sr[slot];
         V2 corr{ 0.0f, 0.0f };
         int nContacts = 0;
         uint64_t mask = 0ull;   // REC: the contacts' indices among the candidates
@@ -1172,12 +1172,13 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
 #pragma unroll
             for (int c = 0; c < PROJ_CHUNK; ++c) {
                 const uint32_t j = j0 + uint32_t(c * Q + lane);
-                const bool in_ = j < cnt;
-                kk[c] = in_ ? b.nbr[size_t(j) * stride + size_t(slot)] : uint32_t(slot);
-                dd[c] = in_ ? b.nbrD[size_t(j) * stride + size_t(slot)] : 0.0f;
+                kk[c] = j < cnt ? b.nbr[size_t(j) * stride + size_t(slot)] : uint32_t(slot);
             }
 #pragma unroll
-            for (int c = 0; c < PROJ_CHUNK; ++c) pp[c] = in.p[kk[c]];
+            for (int c = 0; c < PROJ_CHUNK; ++c) {
+                pp[c] = in.p[kk[c]];
+                dd[c] = ri + b.sr[kk[c]];
+            }
             float tx[PROJ_CHUNK], ty[PROJ_CHUNK];
             int hit[PROJ_CHUNK];
 #pragma unroll
@@ -1210,7 +1211,6 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_lanes(int n, SceneVi
         if (lane == 0) {
             float x, y, oacc;
             bool capped = false, planed = false;
-            const float ri = b.sr[slot];
             finishProjection(b.planes[slot], bp.x, bp.y, acc, corr, nContacts, ri, x, y, oacc, &capped, &planed);
             out.p[slot] = make_float2(x, y);
             out.acc[slot] = oacc;
@@ -1291,10 +1291,7 @@ __global__ void __launch_bounds__(PHYS_MAX_BLOCK) k_project_collect(int n, Scene
                     const uint32_t bits = uint32_t(__ballot(take) >> groupLo) & ((1u << Q) - 1u);
                     if (take) {
                         const uint32_t at = cnt + uint32_t(__popc(bits & ((1u << lane) - 1u)));
-                        if (at < uint32_t(NBR_CAP)) {
-                            b.nbr[size_t(at) * stride + size_t(slot)] = k;
-                            b.nbrD[size_t(at) * stride + size_t(slot)] = D;
-                        }
+                        if (at < uint32_t(NBR_CAP)) b.nbr[size_t(at) * stride + size_t(slot)] = k;
                     }
                     cnt += uint32_t(__popc(bits));
 #pragma unroll

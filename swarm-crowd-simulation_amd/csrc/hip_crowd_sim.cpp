@@ -4,6 +4,7 @@
 #include "hip_crowd_sim.hpp"
 #include "venue.hpp"
 
+#include <algorithm>
 #include <charconv>
 #include <cmath>
 #include <cstdlib>
@@ -196,7 +197,7 @@ void HipCrowdSim::init(const std::vector<const Scene*>& layouts, const std::vect
     sb_.planes = alloc<float4>(n);
     sb_.v0 = v0_;
     sb_.nbr = alloc<uint32_t>(size_t(NBR_CAP) * n);
-    sb_.nbrD = alloc<float>(size_t(NBR_CAP) * n);
+    sb_.nbrD = venue3d_ ? alloc<float>(size_t(NBR_CAP) * n) : nullptr;   // the seated step's height offsets
     sb_.nbrCount = alloc<uint32_t>(n);
     sb_.listBroken = alloc<uint32_t>(S);
     sb_.nScen = S_;
@@ -818,8 +819,28 @@ void HipCrowdSim::readCandidates(uint32_t* count, uint32_t* index, float* dist) 
     const size_t n = size_t(n_);
     if (count) SWARM_HIP_CHECK(hipMemcpyAsync(count, sb_.nbrCount, n * 4, hipMemcpyDeviceToHost, stream_));
     if (index) SWARM_HIP_CHECK(hipMemcpyAsync(index, sb_.nbr, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
-    if (dist) SWARM_HIP_CHECK(hipMemcpyAsync(dist, sb_.nbrD, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
+    if (dist && venue3d_) SWARM_HIP_CHECK(hipMemcpyAsync(dist, sb_.nbrD, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
+    // the plain step's lists hold indices only: a pair's contact distance is the radii's sum (slot order), as the
+    // collecting scan forms it
+    std::vector<uint32_t> cnt, idx;
+    std::vector<float> rad;
+    if (dist && !venue3d_) {
+        cnt.resize(n); idx.resize(size_t(NBR_CAP) * n); rad.resize(n);
+        SWARM_HIP_CHECK(hipMemcpyAsync(cnt.data(), sb_.nbrCount, n * 4, hipMemcpyDeviceToHost, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(idx.data(), sb_.nbr, size_t(NBR_CAP) * n * 4, hipMemcpyDeviceToHost, stream_));
+        SWARM_HIP_CHECK(hipMemcpyAsync(rad.data(), sb_.sr, n * 4, hipMemcpyDeviceToHost, stream_));
+    }
     sync();
+    if (dist && !venue3d_) {
+        std::fill(dist, dist + size_t(NBR_CAP) * n, 0.0f);
+        for (size_t s = 0; s < n; ++s) {
+            const uint32_t c = cnt[s] == NBR_OVERFLOW ? 0u : std::min(cnt[s], uint32_t(NBR_CAP));
+            for (uint32_t j = 0; j < c; ++j) {
+                const uint32_t k = idx[size_t(j) * n + s];
+                if (k < n) dist[size_t(j) * n + s] = rad[s] + rad[k];
+            }
+        }
+    }
 }
 
 void HipCrowdSim::readGrid(uint32_t* key, uint32_t* id) {
@@ -992,7 +1013,6 @@ void HipCrowdSim::allocAdjoint(int every, bool dual) {
         T.bstart = static_cast<uint32_t*>(talloc(C * nbs * 4));
         T.planes = static_cast<float4*>(talloc(C * n * 16));
         T.nbr = static_cast<uint32_t*>(talloc(C * size_t(NBR_CAP) * n * 4));
-        T.nbrD = static_cast<float*>(talloc(C * size_t(NBR_CAP) * n * 4));
         T.nbrCount = static_cast<uint32_t*>(talloc(C * n * 4));
         T.pred = static_cast<swarm_predict_rec*>(talloc(C * n * sizeof(swarm_predict_rec)));
         T.fin = static_cast<swarm_final_rec*>(talloc(C * n * sizeof(swarm_final_rec)));
@@ -1139,7 +1159,7 @@ void HipCrowdSim::tapedStep(int buf, int k, hipStream_t st) {
     sb.egx = T.egx + K * n; sb.egy = T.egy + K * n;
     sb.pw = T.pw + K * n; sb.pvx = T.pvx + K * n; sb.pvy = T.pvy + K * n;
     sb.planes = T.planes + K * n;
-    sb.nbr = T.nbr + K * size_t(NBR_CAP) * n; sb.nbrD = T.nbrD + K * size_t(NBR_CAP) * n; sb.nbrCount = T.nbrCount + K * n;
+    sb.nbr = T.nbr + K * size_t(NBR_CAP) * n; sb.nbrD = nullptr; sb.nbrCount = T.nbrCount + K * n;
     sb.avx = rd.vx; sb.avy = rd.vy;
     GridBuffers gb = grid_;
     gb.bstart = T.bstart + K * nbs;

@@ -290,7 +290,6 @@ private:
         float *sx, *sy, *sr, *egx, *egy, *pw, *pvx, *pvy;
         float4* planes;
         uint32_t* nbr;
-        float* nbrD;
         uint32_t* nbrCount;
         swarm_predict_rec* pred;
         swarm_final_rec* fin;

@@ -111,8 +111,11 @@ __host__ __device__ inline int bandOf(const Layout& L, float dep) {
     if (dep < L.standFront) return 0;
     if (dep < L.row0) return 1;
     if (dep < L.concFront) {
-        int k = cli(int(floorf((dep - L.row0) / L.rowDepth)), 0, L.rows - 1);
-        if (dep < rowFront(L, k)) --k;   // the division rounded across a row front
+        // the row from an estimate within one of it, then the row fronts themselves decide: the reference divides
+        // ((dep - row0) / rowDepth), a product with the reciprocal (a few ulps off the quotient; hoisted out of the
+        // callers' loops) gives the same row after the same correction
+        int k = cli(int(floorf((dep - L.row0) * (1.0f / L.rowDepth))), 0, L.rows - 1);
+        if (dep < rowFront(L, k)) --k;   // the estimate rounded across a row front
         else if (k + 1 < L.rows && dep >= rowFront(L, k + 1)) ++k;
         return 2 + 2 * k + (dep >= rowFront(L, k) + L.clearway ? 1 : 0);
     }

@@ -7,7 +7,9 @@ S, M and L at uniform points at the venues' heights, on and next to every cell a
 searches' ties; these also through a table whose lateral buckets all start at the band's last cell, so that every
 search walks back), on and next to the table's bucket edges, and at non-finite points: cell key, floor height, kind,
 signed distance and gradient, the nearest wall at four reaches, the incline slope, the gangway and the step-up scan must
-agree bit for bit. The program is compiled on the host and linked against libswarm_hip.so (nothing runs on a GPU).
+agree bit for bit; and the band lookup (its row estimated by a product with the reciprocal row depth) must give the
+band that the reference's division gives, at 64 floats either side of every row front and clearway edge and at a
+million random depths per venue. The program is compiled on the host and linked against libswarm_hip.so (nothing runs on a GPU).
 """
 import os
 import shutil

@@ -3,7 +3,8 @@
 // the windowed wall and distance scans -- must return the generator path's bits at every point. Points: uniform over
 // the domain at several heights, on and next to every cell and band boundary (ties of the nearest-cell searches), and
 // non-finite ones; the boundary points also through a table whose buckets all start at their band's last cell (the
-// search's walk back). Prints "ok <points>" or the first difference. Built and run by test_venue_table.py (no GPU needed).
+// search's walk back); and the band lookup's row estimate (a product with the reciprocal) against the reference's
+// division at every row front. Prints "ok <points>" or the first difference. Built and run by test_venue_table.py (no GPU needed).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -49,6 +50,50 @@ static bool checkPoint(const char* nm, const seated::Layout& L, const seated::Ce
     return true;
 }
 
+// bandOf as the reference evaluates it (src/venue.cpp: the row from the correctly rounded quotient, then corrected)
+static int bandOfDivided(const seated::Layout& L, float dep) {
+    const int nb = seated::bandCount(L);
+    if (dep < L.standFront) return 0;
+    if (dep < L.row0) return 1;
+    if (dep < L.concFront) {
+        int k = seated::cli(int(std::floor((dep - L.row0) / L.rowDepth)), 0, L.rows - 1);
+        if (dep < seated::rowFront(L, k)) --k;
+        else if (k + 1 < L.rows && dep >= seated::rowFront(L, k + 1)) ++k;
+        return 2 + 2 * k + (dep >= seated::rowFront(L, k) + L.clearway ? 1 : 0);
+    }
+    if (dep < L.concBack) return nb - 4;
+    if (dep < L.outerFace) return nb - 3;
+    if (dep < L.landingEnd) return nb - 2;
+    return nb - 1;
+}
+
+// bandOf against the divided form at every row front and clearway edge, 64 floats either side, and at random depths
+static bool checkBands(const char* nm, const seated::Layout& L, std::mt19937& rng) {
+    std::uniform_real_distribution<float> u(0.0f, 1.0f);
+    auto one = [&](float dep) {
+        ++g_checked;
+        if (seated::bandOf(L, dep) != bandOfDivided(L, dep)) {
+            std::printf("DIFF venue %s: bandOf at depth %a: %d, divided %d\n", nm, dep, seated::bandOf(L, dep), bandOfDivided(L, dep));
+            return false;
+        }
+        return true;
+    };
+    for (int k = 0; k <= L.rows; ++k) {
+        const float edges[2] = { seated::rowFront(L, k), seated::rowFront(L, k) + L.clearway };
+        for (float e : edges) {
+            float lo = e, hi = e;
+            for (int i = 0; i < 64; ++i) {
+                if (!one(lo) || !one(hi)) return false;
+                lo = std::nextafter(lo, -1.0f);
+                hi = std::nextafter(hi, 2e9f);
+            }
+        }
+    }
+    for (int i = 0; i < 1000000; ++i)
+        if (!one(u(rng) * L.domainHalf)) return false;
+    return true;
+}
+
 int main(int argc, char** argv) {
     const int perVenue = argc > 1 ? std::atoi(argv[1]) : 200000;
     for (const char* nm : { "S", "M", "L" }) {
@@ -62,6 +107,7 @@ int main(int argc, char** argv) {
             for (int b = 0; b < seated::CELL_BUCKETS; ++b) worst.bucket[j * seated::CELL_BUCKETS + size_t(b)] = worst.start[j + 1] - 1;
         const seated::CellTable tbWorst = worst.view();
         std::mt19937 rng(12345u);
+        if (!checkBands(nm, L, rng)) return 1;
         std::uniform_real_distribution<float> u(-1.0f, 1.0f), u01(0.0f, 1.0f);
         const float H = L.domainHalf;
         const float zs[5] = { 0.0f, L.firstRow, L.concourseZ, L.wallTop, L.barrierTop };
