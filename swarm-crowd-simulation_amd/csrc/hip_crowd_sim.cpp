@@ -1032,7 +1032,7 @@ void HipCrowdSim::allocAdjoint(int every, bool dual) {
 
 size_t HipCrowdSim::tapeBytesPerStep() const {   // allocAdjoint's tape, per step
     const size_t n = size_t(n_), P = size_t(mp_.projIters), nbs = size_t(nb_) + 1;
-    return n * (15 * 4 + 2 * 4 + 16 + 2 * size_t(NBR_CAP) * 4 + 4 + sizeof(swarm_predict_rec) + sizeof(swarm_final_rec) + P * 8 + 3 * 8 +
+    return n * (15 * 4 + 2 * 4 + 16 + size_t(NBR_CAP) * 4 + 4 + sizeof(swarm_predict_rec) + sizeof(swarm_final_rec) + P * 8 + 3 * 8 +
                 (P + 1) * 8) + nbs * 4;
 }
 
